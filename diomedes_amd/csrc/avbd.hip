@@ -1,0 +1,381 @@
+// avbd.hip — the AVBD solver kernels for gfx950.
+//
+// Counterparts of the reference's rust-gpu entry points
+// (engine/gpu_physics/avbd/src/lib.rs): avbd_init :202-254, avbd_primal
+// :258-551, avbd_dual :565-663, avbd_recover :667-699, scheduled as in
+// engine/src/render/gpu_physics.rs:541-550 (init; I x (primal, dual) with
+// parity k % 2; recover with parity (I - 1) % 2).
+//
+// Jacobi mode: one lane per body for the primal (its 6x6 block is assembled in
+// 27 registers and solved by an in-register LDL^T), one lane per contact for
+// the dual.  Each Hessian / gradient entry is accumulated in the reference's
+// order (CSR contacts in ascending contact index, rows n, t1, t2; then joints
+// in ascending constraint index), so results are bit-identical to a
+// sequential FP32 evaluation with no fused multiply-adds.  Joints are found
+// through a per-body CSR instead of the reference's scan of every constraint
+// per body (avbd/src/lib.rs:422-539), which visits the same joints in the
+// same order.
+#include <hip/hip_runtime.h>
+
+#include "dio_device.hpp"
+
+namespace dio {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// The 6x6 block system: lin / ang lower triangles (00,10,20,11,21,22), the
+// full ang-by-lin cross block, and the two gradient halves.
+struct Block6 {
+    float l[6], a[6], c[9], rl[3], ra[3];
+
+    __device__ __forceinline__ void add_lin(f3 j, float k) {
+        l[0] = l[0] + (j.x * j.x) * k;
+        l[1] = l[1] + (j.y * j.x) * k;
+        l[2] = l[2] + (j.z * j.x) * k;
+        l[3] = l[3] + (j.y * j.y) * k;
+        l[4] = l[4] + (j.z * j.y) * k;
+        l[5] = l[5] + (j.z * j.z) * k;
+    }
+    __device__ __forceinline__ void add_ang(f3 j, float k) {
+        a[0] = a[0] + (j.x * j.x) * k;
+        a[1] = a[1] + (j.y * j.x) * k;
+        a[2] = a[2] + (j.z * j.x) * k;
+        a[3] = a[3] + (j.y * j.y) * k;
+        a[4] = a[4] + (j.z * j.y) * k;
+        a[5] = a[5] + (j.z * j.z) * k;
+    }
+    // c[3 r + col] += ja[r] * jl[col] * k
+    __device__ __forceinline__ void add_cross(f3 ja, f3 jl, float k) {
+        c[0] = c[0] + (ja.x * jl.x) * k;
+        c[1] = c[1] + (ja.x * jl.y) * k;
+        c[2] = c[2] + (ja.x * jl.z) * k;
+        c[3] = c[3] + (ja.y * jl.x) * k;
+        c[4] = c[4] + (ja.y * jl.y) * k;
+        c[5] = c[5] + (ja.y * jl.z) * k;
+        c[6] = c[6] + (ja.z * jl.x) * k;
+        c[7] = c[7] + (ja.z * jl.y) * k;
+        c[8] = c[8] + (ja.z * jl.z) * k;
+    }
+    __device__ __forceinline__ void add_rhs(f3 jl, f3 ja, float f) {
+        rl[0] = rl[0] + jl.x * f;
+        rl[1] = rl[1] + jl.y * f;
+        rl[2] = rl[2] + jl.z * f;
+        ra[0] = ra[0] + ja.x * f;
+        ra[1] = ra[1] + ja.y * f;
+        ra[2] = ra[2] + ja.z * f;
+    }
+    // add_constraint_row (avbd/src/lib.rs:182-198)
+    __device__ __forceinline__ void row(f3 jl, f3 ja, float force, float k) {
+        add_lin(jl, k);
+        add_ang(ja, k);
+        add_cross(ja, jl, k);
+        add_rhs(jl, ja, force);
+    }
+};
+
+// solve6 (avbd/src/lib.rs:107-180): LDL^T of the block, reading the lower
+// triangle; solves H x = -g with bl = -rhs_lin, ba = -rhs_ang.
+__device__ __forceinline__ void solve6(const Block6& B, float x[6]) {
+    const float a11 = B.l[0], a21 = B.l[1], a31 = B.l[2], a22 = B.l[3], a32 = B.l[4], a33 = B.l[5];
+    const float a41 = B.c[0], a51 = B.c[3], a61 = B.c[6];
+    const float a42 = B.c[1], a52 = B.c[4], a62 = B.c[7];
+    const float a43 = B.c[2], a53 = B.c[5], a63 = B.c[8];
+    const float a44 = B.a[0], a54 = B.a[1], a64 = B.a[2], a55 = B.a[3], a65 = B.a[4], a66 = B.a[5];
+    const float l21 = a21 / a11, l31 = a31 / a11, l41 = a41 / a11, l51 = a51 / a11, l61 = a61 / a11;
+    const float d1 = a11;
+    const float d2 = a22 - l21 * l21 * d1;
+    const float l32 = (a32 - l21 * l31 * d1) / d2;
+    const float l42 = (a42 - l21 * l41 * d1) / d2;
+    const float l52 = (a52 - l21 * l51 * d1) / d2;
+    const float l62 = (a62 - l21 * l61 * d1) / d2;
+    const float d3 = a33 - (l31 * l31 * d1 + l32 * l32 * d2);
+    const float l43 = (a43 - l31 * l41 * d1 - l32 * l42 * d2) / d3;
+    const float l53 = (a53 - l31 * l51 * d1 - l32 * l52 * d2) / d3;
+    const float l63 = (a63 - l31 * l61 * d1 - l32 * l62 * d2) / d3;
+    const float d4 = a44 - (l41 * l41 * d1 + l42 * l42 * d2 + l43 * l43 * d3);
+    const float l54 = (a54 - l41 * l51 * d1 - l42 * l52 * d2 - l43 * l53 * d3) / d4;
+    const float l64 = (a64 - l41 * l61 * d1 - l42 * l62 * d2 - l43 * l63 * d3) / d4;
+    const float d5 = a55 - (l51 * l51 * d1 + l52 * l52 * d2 + l53 * l53 * d3 + l54 * l54 * d4);
+    const float l65 = (a65 - l51 * l61 * d1 - l52 * l62 * d2 - l53 * l63 * d3 - l54 * l64 * d4) / d5;
+    const float d6 = a66 - (l61 * l61 * d1 + l62 * l62 * d2 + l63 * l63 * d3 + l64 * l64 * d4 + l65 * l65 * d5);
+    const float b1 = -B.rl[0], b2 = -B.rl[1], b3 = -B.rl[2], b4 = -B.ra[0], b5 = -B.ra[1], b6 = -B.ra[2];
+    const float y1 = b1;
+    const float y2 = b2 - l21 * y1;
+    const float y3 = b3 - l31 * y1 - l32 * y2;
+    const float y4 = b4 - l41 * y1 - l42 * y2 - l43 * y3;
+    const float y5 = b5 - l51 * y1 - l52 * y2 - l53 * y3 - l54 * y4;
+    const float y6 = b6 - l61 * y1 - l62 * y2 - l63 * y3 - l64 * y4 - l65 * y5;
+    const float z1 = y1 / d1, z2 = y2 / d2, z3 = y3 / d3, z4 = y4 / d4, z5 = y5 / d5, z6 = y6 / d6;
+    const float x6 = z6;
+    const float x5 = z5 - l65 * x6;
+    const float x4 = z4 - l54 * x5 - l64 * x6;
+    const float x3 = z3 - l43 * x4 - l53 * x5 - l63 * x6;
+    const float x2 = z2 - l32 * x3 - l42 * x4 - l52 * x5 - l62 * x6;
+    const float x1 = z1 - l21 * x2 - l31 * x3 - l41 * x4 - l51 * x5 - l61 * x6;
+    x[0] = x1; x[1] = x2; x[2] = x3; x[3] = x4; x[4] = x5; x[5] = x6;
+}
+
+// Clamped contact force (avbd/src/lib.rs:388-398 and :641-651).
+struct Force3 {
+    float f0, f1, f2, bounds, tang;
+};
+__device__ __forceinline__ Force3 contact_force(float4 pen, float4 lam, float friction, float c0, float c1,
+                                                float c2) {
+    Force3 r;
+    r.f0 = rmin(pen.x * c0 + lam.x, 0.0f);
+    const float f1r = pen.y * c1 + lam.y;
+    const float f2r = pen.z * c2 + lam.z;
+    r.bounds = fabsf(r.f0) * friction;
+    r.tang = sqrtf(f1r * f1r + f2r * f2r);
+    r.f1 = f1r;
+    r.f2 = f2r;
+    if (r.tang > r.bounds && r.tang > 0.0f) {
+        const float s = r.bounds / r.tang;
+        r.f1 = f1r * s;
+        r.f2 = f2r * s;
+    }
+    return r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_avbd_init(SolveView v, AvbdParams p) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= p.n) return;
+    const float dt = p.dt;
+    const f3 g{p.gx, p.gy, p.gz};
+    const float inv_mass = v.inv_mm[i].w;
+    const float4 pos4 = v.cur_pos[i];
+    const float4 rot4 = v.cur_rot[i];
+    const f3 pos = xyz(pos4);
+    const q4 rot = as_q(rot4);
+    const f3 vel = xyz(v.vel[i]);
+    const f3 angv = xyz(v.ang[i]);
+    v.init_lin[i] = f4(pos, 0.0f);
+    v.init_ang[i] = rot4;
+    f3 ip = add(pos, scale(vel, dt));
+    if (inv_mass > 0.0f) ip = add(ip, scale(g, dt * dt));
+    v.iner_lin[i] = f4(ip, 0.0f);
+    const q4 irot = quat_step(rot, scale(angv, dt));
+    v.iner_ang[i] = f4q(irot);
+    // adaptive gravity warm start (avbd/src/lib.rs:235-251)
+    f3 wp = pos;
+    q4 wr = rot;
+    if (inv_mass > 0.0f) {
+        const f3 accel = divs(sub(vel, xyz(v.prev_vel[i])), dt);
+        const float gm = len3(g);
+        const float weight = gm > 0.0f ? rclamp(dot3(accel, divs(g, gm)) / gm, 0.0f, 1.0f) : 0.0f;
+        wp = add(add(pos, scale(vel, dt)), scale(g, weight * dt * dt));
+        wr = irot;
+    }
+    // both parity buffers start from S_k (gpu_physics.rs:477-482); buffer 0
+    // then takes the warm start
+    v.pos[1][i] = f4(pos, 0.0f);
+    v.rot[1][i] = rot4;
+    v.pos[0][i] = f4(wp, 0.0f);
+    v.rot[0][i] = f4q(wr);
+}
+
+__global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams p, uint32_t parity) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= p.n) return;
+    const float4 imm = v.inv_mm[i];
+    if (imm.w == 0.0f) return;
+    const float4* __restrict__ rp = v.pos[parity];
+    const float4* __restrict__ rr = v.rot[parity];
+    const f3 snap_pos = xyz(rp[i]);
+    const q4 snap_rot = as_q(rr[i]);
+    const float dt = p.dt;
+    const float mass = 1.0f / imm.w;
+    const f3 moment{1.0f / imm.x, 1.0f / imm.y, 1.0f / imm.z};
+    const float dt2 = dt * dt;
+    const float m_dt2 = mass / dt2;
+    const f3 i_dt2 = divs(moment, dt2);
+
+    Block6 B;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { B.l[k] = 0.0f; B.a[k] = 0.0f; }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) B.c[k] = 0.0f;
+    B.l[0] = m_dt2; B.l[3] = m_dt2; B.l[5] = m_dt2;
+    B.a[0] = i_dt2.x; B.a[3] = i_dt2.y; B.a[5] = i_dt2.z;
+    const f3 rl = scale(sub(snap_pos, xyz(v.iner_lin[i])), 1.0f);
+    B.rl[0] = m_dt2 * rl.x; B.rl[1] = m_dt2 * rl.y; B.rl[2] = m_dt2 * rl.z;
+    const f3 ad = ang_diff(snap_rot, as_q(v.iner_ang[i]));
+    B.ra[0] = i_dt2.x * ad.x; B.ra[1] = i_dt2.y * ad.y; B.ra[2] = i_dt2.z * ad.z;
+
+    const f3 dsl = sub(snap_pos, xyz(v.init_lin[i]));
+    const f3 dsa = ang_diff(snap_rot, as_q(v.init_ang[i]));
+    const float kk = 1.0f - p.alpha;
+    const f3 zero{0.0f, 0.0f, 0.0f};
+
+    const uint32_t jb = v.offsets[i], je = v.offsets[i + 1];
+    for (uint32_t j = jb; j < je; ++j) {
+        const uint32_t ci = v.indices[j];
+        const uint2 ab = v.c_ab[ci];
+        const bool is_a = ab.x == i;
+        const uint32_t other = is_a ? ab.y : ab.x;
+        const float4 n4 = v.c_n[ci];
+        const f3 n0 = xyz(n4);
+        const f3 t1 = xyz(v.c_t1[ci]);
+        const f3 t2 = xyz(v.c_t2[ci]);
+        const f3 ra = xyz(v.c_ra[ci]);
+        const f3 rb = xyz(v.c_rb[ci]);
+        const f3 r0 = is_a ? n0 : neg(n0);
+        const f3 r1 = is_a ? t1 : neg(t1);
+        const f3 r2 = is_a ? t2 : neg(t2);
+        const f3 r_world = qrot(snap_rot, is_a ? ra : rb);
+        f3 dol = zero, doa = zero, r_other = zero;
+        if (other != kContainer) {
+            const q4 orr = as_q(rr[other]);
+            dol = sub(xyz(rp[other]), xyz(v.init_lin[other]));
+            doa = ang_diff(orr, as_q(v.init_ang[other]));
+            r_other = qrot(orr, is_a ? rb : ra);
+        }
+        const f3 j0 = cross3(r_world, r0), j1 = cross3(r_world, r1), j2 = cross3(r_world, r2);
+        const float4 c04 = v.c_c0[ci];
+        float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
+        c0 = c0 + (dot3(r0, dsl) + dot3(j0, dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
+        c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
+        c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
+        const float4 pen = v.c_pen[ci];
+        const Force3 F = contact_force(pen, v.c_lam[ci], n4.w, c0, c1, c2);
+        B.add_lin(r0, pen.x); B.add_lin(r1, pen.y); B.add_lin(r2, pen.z);
+        B.add_ang(j0, pen.x); B.add_ang(j1, pen.y); B.add_ang(j2, pen.z);
+        B.add_cross(j0, r0, pen.x); B.add_cross(j1, r1, pen.y); B.add_cross(j2, r2, pen.z);
+        B.rl[0] = B.rl[0] + r0.x * F.f0; B.rl[1] = B.rl[1] + r0.y * F.f0; B.rl[2] = B.rl[2] + r0.z * F.f0;
+        B.rl[0] = B.rl[0] + r1.x * F.f1; B.rl[1] = B.rl[1] + r1.y * F.f1; B.rl[2] = B.rl[2] + r1.z * F.f1;
+        B.rl[0] = B.rl[0] + r2.x * F.f2; B.rl[1] = B.rl[1] + r2.y * F.f2; B.rl[2] = B.rl[2] + r2.z * F.f2;
+        B.ra[0] = B.ra[0] + j0.x * F.f0; B.ra[1] = B.ra[1] + j0.y * F.f0; B.ra[2] = B.ra[2] + j0.z * F.f0;
+        B.ra[0] = B.ra[0] + j1.x * F.f1; B.ra[1] = B.ra[1] + j1.y * F.f1; B.ra[2] = B.ra[2] + j1.z * F.f1;
+        B.ra[0] = B.ra[0] + j2.x * F.f2; B.ra[1] = B.ra[1] + j2.y * F.f2; B.ra[2] = B.ra[2] + j2.z * F.f2;
+    }
+
+    // joints (avbd/src/lib.rs:420-539)
+    if (v.j_off) {
+        const uint32_t kb = v.j_off[i], ke = v.j_off[i + 1];
+        for (uint32_t q = kb; q < ke; ++q) {
+            const uint32_t cidx = v.j_idx[q];
+            const uint4 abk = v.j_abk[cidx];
+            const float4 aa4 = v.j_anc_a[cidx];
+            const float4 ab4 = v.j_anc_b[cidx];
+            const bool is_a = abk.x == i;
+            const f3 aa = xyz(aa4), abn = xyz(ab4);
+            const f3 pa = is_a ? snap_pos : xyz(rp[abk.x]);
+            const f3 pb = !is_a ? snap_pos : xyz(rp[abk.y]);
+            const q4 qa = is_a ? snap_rot : as_q(rr[abk.x]);
+            const q4 qb = !is_a ? snap_rot : as_q(rr[abk.y]);
+            const f3 p_a = add(pa, qrot(qa, aa));
+            const f3 p_b = add(pb, qrot(qb, abn));
+            const f3 self_anchor = is_a ? aa : abn;
+            const float sign = is_a ? 1.0f : -1.0f;
+            const f3 delta = sub(p_a, p_b);
+            const f3 r_self = qrot(snap_rot, self_anchor);
+            const float stiff = ab4.w;
+            if (abk.z == kBallSocket) {
+                const f3 jx = scale(f3{1.0f, 0.0f, 0.0f}, sign);
+                const f3 jy = scale(f3{0.0f, 1.0f, 0.0f}, sign);
+                const f3 jz = scale(f3{0.0f, 0.0f, 1.0f}, sign);
+                B.row(jx, cross3(r_self, jx), delta.x * stiff, stiff);
+                B.row(jy, cross3(r_self, jy), delta.y * stiff, stiff);
+                B.row(jz, cross3(r_self, jz), delta.z * stiff, stiff);
+            } else {
+                const float len = len3(delta);
+                const bool active = len > 1.0e-6f && (abk.z != kRope || len > aa4.w);
+                if (active) {
+                    const f3 nrm = divs(delta, len);
+                    const f3 jl = scale(nrm, sign);
+                    B.row(jl, cross3(r_self, jl), stiff * (len - aa4.w), stiff);
+                }
+            }
+        }
+    }
+
+    float x[6];
+    solve6(B, x);
+    const f3 new_pos = add(snap_pos, f3{x[0], x[1], x[2]});
+    const q4 new_rot = quat_step(snap_rot, f3{x[3], x[4], x[5]});
+    v.pos[1 - parity][i] = f4(new_pos, 0.0f);
+    v.rot[1 - parity][i] = f4q(new_rot);
+}
+
+__global__ void __launch_bounds__(kBlock) k_avbd_dual(SolveView v, AvbdParams p, uint32_t parity) {
+    const uint32_t m = v.m_dev ? *v.m_dev : v.m_host;
+    const float4* __restrict__ wp = v.pos[1 - parity];
+    const float4* __restrict__ wr = v.rot[1 - parity];
+    const float kk = 1.0f - p.alpha;
+    const f3 zero{0.0f, 0.0f, 0.0f};
+    for (uint32_t ci = blockIdx.x * kBlock + threadIdx.x; ci < m; ci += gridDim.x * kBlock) {
+        const uint2 ab = v.c_ab[ci];
+        const q4 rot_a = as_q(wr[ab.x]);
+        const f3 dal = sub(xyz(wp[ab.x]), xyz(v.init_lin[ab.x]));
+        const f3 daa = ang_diff(rot_a, as_q(v.init_ang[ab.x]));
+        f3 dbl = zero, dba = zero, rbw = zero;
+        if (ab.y != kContainer) {
+            const q4 rot_b = as_q(wr[ab.y]);
+            dbl = sub(xyz(wp[ab.y]), xyz(v.init_lin[ab.y]));
+            dba = ang_diff(rot_b, as_q(v.init_ang[ab.y]));
+            rbw = qrot(rot_b, xyz(v.c_rb[ci]));
+        }
+        const f3 raw = qrot(rot_a, xyz(v.c_ra[ci]));
+        const float4 n4 = v.c_n[ci];
+        const f3 r0 = xyz(n4), r1 = xyz(v.c_t1[ci]), r2 = xyz(v.c_t2[ci]);
+        const float4 c04 = v.c_c0[ci];
+        float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
+        c0 = c0 + (dot3(r0, dal) + dot3(cross3(raw, r0), daa) - dot3(r0, dbl) - dot3(cross3(rbw, r0), dba));
+        c1 = c1 + (dot3(r1, dal) + dot3(cross3(raw, r1), daa) - dot3(r1, dbl) - dot3(cross3(rbw, r1), dba));
+        c2 = c2 + (dot3(r2, dal) + dot3(cross3(raw, r2), daa) - dot3(r2, dbl) - dot3(cross3(rbw, r2), dba));
+        float4 pen = v.c_pen[ci];
+        const Force3 F = contact_force(pen, v.c_lam[ci], n4.w, c0, c1, c2);
+        v.c_lam[ci] = make_float4(F.f0, F.f1, F.f2, 0.0f);
+        if (F.f0 < 0.0f) pen.x = rmin(pen.x + p.beta_lin * fabsf(c0), p.penalty_max);
+        if (F.tang <= F.bounds) {
+            pen.y = rmin(pen.y + p.beta_lin * fabsf(c1), p.penalty_max);
+            pen.z = rmin(pen.z + p.beta_lin * fabsf(c2), p.penalty_max);
+        }
+        v.c_pen[ci] = pen;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_avbd_recover(SolveView v, AvbdParams p, uint32_t parity) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= p.n) return;
+    if (v.inv_mm[i].w == 0.0f) return;
+    const float4 pos4 = v.pos[1 - parity][i];
+    const float4 rot4 = v.rot[1 - parity][i];
+    const float dt = p.dt;
+    const float4 vel = v.vel[i];
+    v.prev_vel[i] = vel;
+    const f3 nv = divs(sub(xyz(pos4), xyz(v.init_lin[i])), dt);
+    v.vel[i] = f4(nv, 0.0f);
+    const q4 rel = qmul(as_q(rot4), qconj(as_q(v.init_ang[i])));
+    v.ang[i] = make_float4(2.0f * rel.x / dt, 2.0f * rel.y / dt, 2.0f * rel.z / dt, 0.0f);
+    if (v.out_pos) {
+        v.out_pos[i] = pos4;
+        v.out_rot[i] = rot4;
+    }
+}
+
+inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t s) {
+    if (p.n == 0) return;
+    hipLaunchKernelGGL(k_avbd_init, dim3(blocks_for(p.n)), dim3(kBlock), 0, s, v, p);
+}
+void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t s) {
+    if (p.n == 0) return;
+    hipLaunchKernelGGL(k_avbd_primal, dim3(blocks_for(p.n)), dim3(kBlock), 0, s, v, p, parity);
+}
+void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t m_grid, hipStream_t s) {
+    uint32_t blocks = blocks_for(m_grid == 0 ? 1 : m_grid);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_avbd_dual, dim3(blocks), dim3(kBlock), 0, s, v, p, parity);
+}
+void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t s) {
+    if (p.n == 0) return;
+    hipLaunchKernelGGL(k_avbd_recover, dim3(blocks_for(p.n)), dim3(kBlock), 0, s, v, p, parity);
+}
+
+}  // namespace dio

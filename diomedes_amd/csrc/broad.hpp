@@ -1,0 +1,50 @@
+// broad.hpp — LBVH broad phase: buffers and launcher.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/diomedes_hip.h"
+#include "shapes.hpp"
+
+namespace dio {
+
+// Body poses and shapes (SoA) as the collision kernels read them.
+struct BodyView {
+    const float4* pos;
+    const float4* rot;
+    const uint32_t* tag;
+    const float4* corners;  // 4 per body (GpuShape corners)
+    const float4* hull;     // hull vertex pool (may be null)
+};
+
+// Device buffers of one broad-phase pipeline (capacity n_cap bodies).
+struct BroadBuffers {
+    float4* aabb_min;  // n_cap
+    float4* aabb_max;
+    float4* bounds;    // 2
+    uint32_t *keys0, *vals0, *keys1, *vals1;  // n_cap each
+    uint32_t* sort_scratch;                   // sort_scratch_words(n_cap)
+    uint32_t *left, *right, *parent_int;      // n_cap - 1 (at least 1)
+    uint32_t* parent_leaf;                    // n_cap
+    uint32_t* flags;                          // n_cap - 1
+    float4 *node_min, *node_max;              // n_cap - 1
+    uint32_t* counts;                         // n_cap
+    uint32_t* offsets;                        // n_cap + 1
+    uint32_t* scan_partials;                  // scan_partials_size(n_cap)
+    uint2* pairs;                             // p_max
+    uint32_t* p_dev;                          // pair count min(total, p_max)
+    uint32_t* overflow;                       // largest total that exceeded p_max
+    uint32_t* err;                            // traversal stack overflow flag
+    static size_t sort_scratch_words(uint32_t n_cap);
+};
+
+void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, float4* rot, uint32_t* tag,
+                              float4* corners, hipStream_t s);
+
+// AABBs, LBVH and the sorted pair list of n bodies; the pair count stays on
+// the device (b.p_dev).
+void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s);
+
+}  // namespace dio

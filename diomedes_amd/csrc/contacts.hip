@@ -1,0 +1,310 @@
+// contacts.hip — the contact build of one step, on the device.
+//
+// The reference does this on the CPU every frame: raw_contact for each
+// decoded narrow contact (physics/gpu.rs:239-264), container_contacts for
+// each dynamic body (:352-404), then finalize_contacts (:266-311): a HashMap
+// warm start in which the LAST old contact of a key (a, b) is kept and the
+// FIRST new contact of that key takes it, the alpha*gamma / gamma scaling,
+// and the CSR in contact order.
+//
+// Here both contact lists are ordered runs of keys — narrow contacts in
+// (sorted pair, slot) order and container contacts by body — so "first new
+// contact of a key" is slot 0 of a pair (or the first wall contact of a
+// body) and "last old contact of a key" is found by binary search of the old
+// pair list (or from the old per-body container offsets).  The CSR is a
+// histogram, a scan, an atomic fill and a per-body sort of each segment, which
+// reproduces the reference's ascending contact order.
+#include <hip/hip_runtime.h>
+
+#include "contacts.hpp"
+#include "prim.hpp"
+
+namespace dio {
+
+namespace {
+
+constexpr int kBlock = 256;
+inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+__device__ __forceinline__ void put_contact(const ContactArrays& c, uint32_t i, uint32_t a, uint32_t b, f3 n, f3 t1,
+                                            f3 t2, f3 ra, f3 rb, float c0x, float friction, float4 lam, float4 pen) {
+    c.ab[i] = make_uint2(a, b);
+    c.n[i] = f4(n, friction);
+    c.t1[i] = f4(t1, 0.0f);
+    c.t2[i] = f4(t2, 0.0f);
+    c.ra[i] = f4(ra, 0.0f);
+    c.rb[i] = f4(rb, 0.0f);
+    c.c0[i] = make_float4(c0x, 0.0f, 0.0f, 0.0f);
+    c.lam[i] = lam;
+    c.pen[i] = pen;
+}
+
+// finalize_contacts scaling (physics/gpu.rs:278-282)
+__device__ __forceinline__ void warm_scale(const BuildOptions& o, float4& lam, float4& pen) {
+    const float ag = o.alpha * o.gamma;
+    lam = make_float4(lam.x * ag, lam.y * ag, lam.z * ag, 0.0f);
+    pen = make_float4(rmin(rmax(pen.x * o.gamma, o.penalty_min), o.penalty_max),
+                      rmin(rmax(pen.y * o.gamma, o.penalty_min), o.penalty_max),
+                      rmin(rmax(pen.z * o.gamma, o.penalty_min), o.penalty_max), 0.0f);
+}
+
+// index of the last old contact with key (a, b) among the narrow prefix, or -1
+__device__ __forceinline__ int64_t find_old_pair(const NarrowSource& old_src, uint32_t a, uint32_t b) {
+    const uint32_t np = *old_src.p_dev;
+    uint32_t lo = 0, hi = np;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint2 q = old_src.pairs[mid];
+        if (q.x < a || (q.x == a && q.y < b)) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < np) {
+        const uint2 q = old_src.pairs[lo];
+        if (q.x == a && q.y == b) {
+            const uint32_t s0 = old_src.off[lo], s1 = old_src.off[lo + 1];
+            if (s1 > s0) return (int64_t)s1 - 1;
+        }
+    }
+    return -1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_contacts_narrow(WorldView w, BuildOptions o, NarrowSource src,
+                                                            ContactSet dst, NarrowSource old_src, ContactSet old,
+                                                            uint32_t* overflow) {
+    const uint32_t np = *src.p_dev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t t = src.off[np];
+        *dst.m_narrow = t < dst.m_cap ? t : dst.m_cap;
+        if (t > dst.m_cap) atomicMax(overflow, t);
+    }
+    const uint32_t old_m = *old.m_dev;
+    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < np; p += gridDim.x * kBlock) {
+        const uint32_t base = src.off[p], cnt = src.off[p + 1] - base;
+        if (cnt == 0) continue;
+        const uint2 pr = src.pairs[p];
+        const f3 pa_pos = xyz(w.pos[pr.x]), pb_pos = xyz(w.pos[pr.y]);
+        const q4 qa = as_q(w.rot[pr.x]), qb = as_q(w.rot[pr.y]);
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t ci = base + k;
+            if (ci >= dst.m_cap) break;
+            const float4 n4 = src.n[p * 4 + k];
+            const f3 normal = xyz(n4);
+            f3 t1, t2;
+            tangents(normal, t1, t2);
+            const f3 ra = qrot_inv(qa, sub(xyz(src.pa[p * 4 + k]), pa_pos));
+            const f3 rb = qrot_inv(qb, sub(xyz(src.pb[p * 4 + k]), pb_pos));
+            float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float4 pen = make_float4(o.penalty_min, o.penalty_min, o.penalty_min, 0.0f);
+            if (k == 0) {
+                const int64_t q = find_old_pair(old_src, pr.x, pr.y);
+                if (q >= 0 && (uint32_t)q < old_m) {
+                    lam = old.c.lam[q];
+                    pen = old.c.pen[q];
+                }
+            }
+            warm_scale(o, lam, pen);
+            put_contact(dst.c, ci, pr.x, pr.y, neg(normal), t1, t2, ra, rb, o.margin - n4.w, o.friction, lam, pen);
+        }
+    }
+}
+
+// container_contacts (physics/gpu.rs:352-404) for one body; emit(point, depth,
+// normal, t1, t2) for each contact in wall / point order.
+template <class Emit>
+__device__ __forceinline__ void wall_contacts(const WorldView& w, uint32_t i, const ContainerParams& ct,
+                                              const BuildOptions& o, Emit&& emit) {
+    ShapeRef s;
+    s.pos = xyz(w.pos[i]);
+    s.rot = as_q(w.rot[i]);
+    s.tag = w.tag[i];
+    s.c[0] = w.corners[4 * i];
+    s.c[1] = w.corners[4 * i + 1];
+    s.c[2] = w.corners[4 * i + 2];
+    s.c[3] = w.corners[4 * i + 3];
+    const f3 vel = xyz(w.vel[i]);
+    const q4 crot{ct.qx, ct.qy, ct.qz, ct.qw};
+    const f3 center{ct.cx, ct.cy, ct.cz};
+    const f3 axes[3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+    for (int wall = 0; wall < 6; ++wall) {
+        const float sign = (wall & 1) ? -1.0f : 1.0f;
+        const f3 outward = qrot(crot, scale(axes[wall >> 1], sign));
+        const float plane = dot3(center, outward) + ct.half;
+        f3 pts[4];
+        uint32_t count;
+        if (s.tag == kCube || s.tag == kBox) {
+            // cube_support_face (physics/gpu.rs:314-350)
+            f3 ax[3];
+            box_axes(s.rot, ax);
+            uint32_t ai = 0;
+            float best = fabsf(dot3(ax[0], outward));
+            for (uint32_t k = 1; k < 3; ++k) {
+                const float al = fabsf(dot3(ax[k], outward));
+                if (al > best) { ai = k; best = al; }
+            }
+            const f3 fa = ax[ai];
+            const f3 fnrm = scale(fa, dot3(fa, outward) >= 0.0f ? 1.0f : -1.0f);
+            const uint32_t iu = ai == 0 ? 1u : 0u;
+            const uint32_t iv = ai == 2 ? 1u : 2u;
+            const float hn = box_half(s, ai), hu = box_half(s, iu), hv = box_half(s, iv);
+            const f3 c = add(s.pos, scale(fnrm, hn));
+            const f3 u = ax[iu], v = ax[iv];
+            pts[0] = add(add(c, scale(u, hu)), scale(v, hv));
+            pts[1] = add(sub(c, scale(u, hu)), scale(v, hv));
+            pts[2] = sub(sub(c, scale(u, hu)), scale(v, hv));
+            pts[3] = sub(add(c, scale(u, hu)), scale(v, hv));
+            count = 4;
+        } else {
+            const f3 local = qrot_inv(s.rot, outward);
+            pts[0] = add(qrot(s.rot, support_host(s, local, HullPool{w.hull})), s.pos);
+            count = 1;
+        }
+        const f3 normal = neg(outward);
+        f3 t1, t2;
+        tangents(normal, t1, t2);
+        for (uint32_t k = 0; k < count; ++k) {
+            const float cur = dot3(pts[k], outward) - plane;
+            const float pred = dot3(add(pts[k], scale(vel, o.dt)), outward) - plane;
+            const float depth = rmax(cur, pred);
+            if (depth > 0.0f) emit(pts[k], depth, normal, t1, t2, s);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_container_count(WorldView w, ContainerParams ct, BuildOptions o,
+                                                            uint32_t* cnt) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= w.n) return;
+    uint32_t c = 0;
+    if (ct.enabled && w.inv_mm[i].w != 0.0f)
+        wall_contacts(w, i, ct, o, [&](f3, float, f3, f3, f3, const ShapeRef&) { ++c; });
+    cnt[i] = c;
+}
+
+__global__ void __launch_bounds__(kBlock) k_container_write(WorldView w, ContainerParams ct, BuildOptions o,
+                                                            ContactSet dst, ContactSet old, uint32_t* overflow) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= w.n) return;
+    const uint32_t mn = *dst.m_narrow;
+    if (i == 0) {
+        const uint32_t t = mn + dst.cont_off[w.n];
+        *dst.m_dev = t < dst.m_cap ? t : dst.m_cap;
+        if (t > dst.m_cap) atomicMax(overflow, t);
+    }
+    if (dst.cont_off[i + 1] == dst.cont_off[i]) return;
+    // warm value of key (i, CONTAINER): the last old container contact of body i
+    float4 wl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 wpn = make_float4(o.penalty_min, o.penalty_min, o.penalty_min, 0.0f);
+    const uint32_t o0 = old.cont_off[i], o1 = old.cont_off[i + 1];
+    if (o1 > o0) {
+        const uint32_t q = *old.m_narrow + o1 - 1;
+        if (q < *old.m_dev) {
+            wl = old.c.lam[q];
+            wpn = old.c.pen[q];
+        }
+    }
+    uint32_t ci = mn + dst.cont_off[i];
+    bool first = true;
+    wall_contacts(w, i, ct, o, [&](f3 p, float depth, f3 normal, f3 t1, f3 t2, const ShapeRef& s) {
+        if (ci < dst.m_cap) {
+            float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float4 pen = make_float4(o.penalty_min, o.penalty_min, o.penalty_min, 0.0f);
+            if (first) { lam = wl; pen = wpn; }
+            warm_scale(o, lam, pen);
+            const f3 ra = qrot_inv(s.rot, sub(p, s.pos));
+            put_contact(dst.c, ci, i, kContainer, normal, t1, t2, ra, f3{0.0f, 0.0f, 0.0f}, o.margin - depth,
+                        o.friction, lam, pen);
+        }
+        first = false;
+        ++ci;
+    });
+}
+
+__global__ void __launch_bounds__(kBlock) k_csr_count(ContactSet cs, uint32_t* cnt) {
+    const uint32_t m = *cs.m_dev;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < m; c += gridDim.x * kBlock) {
+        const uint2 ab = cs.c.ab[c];
+        atomicAdd(&cnt[ab.x], 1u);
+        if (ab.y != kContainer) atomicAdd(&cnt[ab.y], 1u);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_csr_fill(ContactSet cs, const uint32_t* off, uint32_t* cur, uint32_t* idx,
+                                                     uint32_t idx_cap) {
+    const uint32_t m = *cs.m_dev;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < m; c += gridDim.x * kBlock) {
+        const uint2 ab = cs.c.ab[c];
+        const uint32_t sa = off[ab.x] + atomicAdd(&cur[ab.x], 1u);
+        if (sa < idx_cap) idx[sa] = c;
+        if (ab.y != kContainer) {
+            const uint32_t sb = off[ab.y] + atomicAdd(&cur[ab.y], 1u);
+            if (sb < idx_cap) idx[sb] = c;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_csr_sort(uint32_t n, const uint32_t* off, uint32_t* idx, uint32_t idx_cap,
+                                                     uint32_t* overflow) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0 && off[n] > idx_cap) atomicMax(overflow, off[n]);
+    if (i >= n) return;
+    const uint32_t b = off[i];
+    uint32_t e = off[i + 1];
+    if (e > idx_cap) e = idx_cap;
+    for (uint32_t x = b + 1; x < e; ++x) {
+        const uint32_t v = idx[x];
+        uint32_t y = x;
+        while (y > b && idx[y - 1] > v) {
+            idx[y] = idx[y - 1];
+            --y;
+        }
+        idx[y] = v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_unpack_contacts(const dio_avbd_contact* rec, uint32_t m, ContactArrays c) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const dio_avbd_contact& r = rec[i];
+    c.ab[i] = make_uint2(r.a, r.b);
+    c.n[i] = make_float4(r.normal[0], r.normal[1], r.normal[2], r.friction);
+    c.t1[i] = make_float4(r.tangent1[0], r.tangent1[1], r.tangent1[2], 0.0f);
+    c.t2[i] = make_float4(r.tangent2[0], r.tangent2[1], r.tangent2[2], 0.0f);
+    c.ra[i] = make_float4(r.r_a[0], r.r_a[1], r.r_a[2], 0.0f);
+    c.rb[i] = make_float4(r.r_b[0], r.r_b[1], r.r_b[2], 0.0f);
+    c.c0[i] = make_float4(r.c0[0], r.c0[1], r.c0[2], 0.0f);
+    c.pen[i] = make_float4(r.penalty[0], r.penalty[1], r.penalty[2], 0.0f);
+    c.lam[i] = make_float4(r.lambda[0], r.lambda[1], r.lambda[2], 0.0f);
+}
+
+}  // namespace
+
+void run_contact_build(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const NarrowSource& src,
+                       const ContactSet& dst, const NarrowSource& old_src, const ContactSet& old,
+                       uint32_t* scan_partials, uint32_t* overflow, hipStream_t s) {
+    uint32_t pb = blocks_for(dst.m_cap > 0 ? dst.m_cap / 4 + 1 : 1);
+    if (pb > 2048) pb = 2048;
+    hipLaunchKernelGGL(k_contacts_narrow, dim3(pb), dim3(kBlock), 0, s, w, o, src, dst, old_src, old, overflow);
+    const uint32_t nb = blocks_for(w.n > 0 ? w.n : 1);
+    hipLaunchKernelGGL(k_container_count, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst.cont_cnt);
+    launch_scan_exclusive(dst.cont_cnt, dst.cont_off, w.n, nullptr, scan_partials, s);
+    hipLaunchKernelGGL(k_container_write, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst, old, overflow);
+}
+
+void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s) {
+    launch_fill_u32(b.cnt, 0u, n, s);
+    launch_fill_u32(b.cur, 0u, n, s);
+    uint32_t mb = blocks_for(cs.m_cap > 0 ? cs.m_cap : 1);
+    if (mb > 2048) mb = 2048;
+    hipLaunchKernelGGL(k_csr_count, dim3(mb), dim3(kBlock), 0, s, cs, b.cnt);
+    launch_scan_exclusive(b.cnt, b.off, n, nullptr, b.scan_partials, s);
+    hipLaunchKernelGGL(k_csr_fill, dim3(mb), dim3(kBlock), 0, s, cs, (const uint32_t*)b.off, b.cur, b.idx, b.idx_cap);
+    hipLaunchKernelGGL(k_csr_sort, dim3(blocks_for(n > 0 ? n : 1)), dim3(kBlock), 0, s, n, (const uint32_t*)b.off,
+                       b.idx, b.idx_cap, overflow);
+}
+
+void launch_unpack_contacts(const dio_avbd_contact* rec, uint32_t m, const ContactArrays& c, hipStream_t s) {
+    if (m == 0) return;
+    hipLaunchKernelGGL(k_unpack_contacts, dim3(blocks_for(m)), dim3(kBlock), 0, s, rec, m, c);
+}
+
+}  // namespace dio

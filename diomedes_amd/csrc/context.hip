@@ -1,0 +1,1214 @@
+// context.hip — host side of the C ABI (include/diomedes_hip.h).
+//
+// A dio_ctx owns one HIP stream and fixed-capacity device buffers (the
+// reference's NarrowPhaseCompute / AvbdCompute objects own fixed Vulkan
+// buffers the same way, render/compute.rs:176-397, render/gpu_physics.rs:
+// 224-433).  Three groups of buffers:
+//   - the granular broad / narrow pipeline (dio_broad_*, dio_narrow_*),
+//   - the granular AVBD solve (dio_avbd_*),
+//   - the fused world: SoA bodies, joints, two pending narrow outputs and two
+//     contact sets used in alternation, the CSR and the solver's pose
+//     buffers.  dio_step runs the demo step protocol on the device; with
+//     graphs enabled each step is two graph launches (contact build + broad +
+//     narrow, then the AVBD iterations) with no host synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/diomedes_hip.h"
+#include "broad.hpp"
+#include "contacts.hpp"
+#include "dio_device.hpp"
+#include "narrow.hpp"
+#include "prim.hpp"
+
+using namespace dio;
+
+namespace {
+
+struct HipError {
+    hipError_t code;
+    const char* what;
+};
+
+#define DIO_HIP_CHECK(expr)                                  \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) throw HipError{e_, #expr};     \
+    } while (0)
+
+struct StatusError {
+    dio_status status;
+    std::string msg;
+    uint64_t required;
+};
+
+// Device allocations owned by a context; freed together.
+struct DevArena {
+    std::vector<void*> ptrs;
+    template <class T> T* alloc(size_t count) {
+        void* p = nullptr;
+        DIO_HIP_CHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    void release() {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+struct PinnedArena {
+    std::vector<void*> ptrs;
+    template <class T> T* alloc(size_t count) {
+        void* p = nullptr;
+        DIO_HIP_CHECK(hipHostMalloc(&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault));
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    void release() {
+        for (void* p : ptrs) (void)hipHostFree(p);
+        ptrs.clear();
+    }
+};
+
+BroadBuffers alloc_broad(DevArena& A, uint32_t n_cap, uint32_t p_max) {
+    BroadBuffers b{};
+    const uint32_t ni = n_cap > 1 ? n_cap - 1 : 1;
+    b.aabb_min = A.alloc<float4>(n_cap);
+    b.aabb_max = A.alloc<float4>(n_cap);
+    b.bounds = A.alloc<float4>(2);
+    b.keys0 = A.alloc<uint32_t>(n_cap);
+    b.vals0 = A.alloc<uint32_t>(n_cap);
+    b.keys1 = A.alloc<uint32_t>(n_cap);
+    b.vals1 = A.alloc<uint32_t>(n_cap);
+    b.sort_scratch = A.alloc<uint32_t>(BroadBuffers::sort_scratch_words(n_cap));
+    b.left = A.alloc<uint32_t>(ni);
+    b.right = A.alloc<uint32_t>(ni);
+    b.parent_int = A.alloc<uint32_t>(ni);
+    b.parent_leaf = A.alloc<uint32_t>(n_cap);
+    b.flags = A.alloc<uint32_t>(ni);
+    b.node_min = A.alloc<float4>(ni);
+    b.node_max = A.alloc<float4>(ni);
+    b.counts = A.alloc<uint32_t>(n_cap);
+    b.offsets = A.alloc<uint32_t>(n_cap + 1);
+    b.scan_partials = A.alloc<uint32_t>(scan_partials_size(n_cap + 1));
+    b.pairs = A.alloc<uint2>(p_max);
+    b.p_dev = A.alloc<uint32_t>(1);
+    b.overflow = A.alloc<uint32_t>(1);
+    b.err = A.alloc<uint32_t>(1);
+    return b;
+}
+
+// narrow slots for a pair list of capacity p_max (pairs / p_dev set by caller)
+NarrowOut alloc_narrow(DevArena& A, uint32_t p_max) {
+    NarrowOut o{};
+    o.n = A.alloc<float4>((size_t)p_max * 4);
+    o.pa = A.alloc<float4>((size_t)p_max * 4);
+    o.pb = A.alloc<float4>((size_t)p_max * 4);
+    o.cnt = A.alloc<uint32_t>(p_max);
+    o.off = A.alloc<uint32_t>((size_t)p_max + 1);
+    o.scan_partials = A.alloc<uint32_t>(scan_partials_size(p_max + 1));
+    o.gjk_list = A.alloc<uint32_t>(p_max);
+    o.gjk_count = A.alloc<uint32_t>(1);
+    o.p_max = p_max;
+    return o;
+}
+
+ContactArrays alloc_contacts(DevArena& A, uint32_t m_cap) {
+    ContactArrays c{};
+    c.ab = A.alloc<uint2>(m_cap);
+    c.n = A.alloc<float4>(m_cap);
+    c.t1 = A.alloc<float4>(m_cap);
+    c.t2 = A.alloc<float4>(m_cap);
+    c.ra = A.alloc<float4>(m_cap);
+    c.rb = A.alloc<float4>(m_cap);
+    c.c0 = A.alloc<float4>(m_cap);
+    c.pen = A.alloc<float4>(m_cap);
+    c.lam = A.alloc<float4>(m_cap);
+    return c;
+}
+
+// joint CSR in ascending constraint order (a self-joint is listed once)
+void joint_csr(const dio_avbd_constraint* cons, uint32_t c, uint32_t n, std::vector<uint32_t>& off,
+               std::vector<uint32_t>& idx) {
+    off.assign(n + 1, 0);
+    for (uint32_t k = 0; k < c; ++k) {
+        off[cons[k].a + 1] += 1;
+        if (cons[k].b != cons[k].a) off[cons[k].b + 1] += 1;
+    }
+    for (uint32_t i = 0; i < n; ++i) off[i + 1] += off[i];
+    idx.assign(off[n], 0);
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (uint32_t k = 0; k < c; ++k) {
+        idx[cur[cons[k].a]++] = k;
+        if (cons[k].b != cons[k].a) idx[cur[cons[k].b]++] = k;
+    }
+}
+
+struct JointDev {
+    uint4* abk = nullptr;
+    float4* anc_a = nullptr;
+    float4* anc_b = nullptr;
+    uint32_t* off = nullptr;
+    uint32_t* idx = nullptr;
+    uint32_t c = 0;
+};
+
+}  // namespace
+
+struct dio_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    uint64_t required = 0;
+    dio_limits lim{};
+    uint32_t n_cap = 0, p_max = 0, m_cap = 0, c_cap = 0, h_cap = 0;
+    DevArena dev;
+    PinnedArena pin;
+    // reusable scan scratch (sized for the largest scanned array)
+    uint32_t* scan_partials = nullptr;
+
+    // ---- granular broad / narrow (NarrowPhaseCompute)
+    bool g_ready = false;
+    dio_gpu_body* g_rec = nullptr;          // device staging
+    dio_gpu_body* g_rec_host = nullptr;     // pinned
+    float4 *g_pos = nullptr, *g_rot = nullptr, *g_corners = nullptr, *g_hull = nullptr;
+    uint32_t* g_tag = nullptr;
+    BroadBuffers gb{};
+    NarrowOut gn{};
+    float4* g_simplex = nullptr;
+    uint2* g_pairs_up = nullptr;            // caller-given pairs (dio_narrow_submit_pairs)
+    uint2* g_pairs_host = nullptr;
+    uint32_t* g_np_dev = nullptr;           // pair count for the narrow dispatch
+    dio_gpu_contact_out* g_out = nullptr;   // packed records
+    uint32_t g_n = 0, g_last_pairs = 0;
+    bool g_broad_submitted = false, g_narrow_submitted = false;
+
+    // ---- granular AVBD (AvbdCompute)
+    bool a_ready = false;
+    dio_avbd_body_state* a_state_host = nullptr;
+    float* a_pose_host = nullptr;  // n x 8 (pos xyz0, rot xyzw)
+    dio_avbd_contact* a_con_host = nullptr;
+    dio_avbd_constraint* a_cons_host = nullptr;
+    uint32_t* a_csr_host = nullptr;  // offsets (n_cap + 1) then indices (2 m_cap)
+    dio_avbd_body_state* a_state = nullptr;
+    float* a_pose = nullptr;
+    dio_avbd_contact* a_con_rec = nullptr;
+    float4 *a_cur_pos = nullptr, *a_cur_rot = nullptr;
+    float4 *a_vel = nullptr, *a_ang = nullptr, *a_prev = nullptr, *a_inv_mm = nullptr;
+    float4 *a_pos[2] = {nullptr, nullptr}, *a_rot[2] = {nullptr, nullptr};
+    float4 *a_init_lin = nullptr, *a_init_ang = nullptr, *a_iner_lin = nullptr, *a_iner_ang = nullptr;
+    ContactArrays a_c{};
+    uint32_t *a_off = nullptr, *a_idx = nullptr;
+    JointDev a_j;
+    dio_avbd_constraint* a_cons_stage = nullptr;
+    uint32_t a_n = 0, a_m = 0, a_iters = 0;
+    bool a_submitted = false;
+
+    // ---- fused world
+    bool w_ready = false;
+    uint32_t w_n = 0;
+    std::vector<float> w_friction;
+    float4 *w_pos = nullptr, *w_rot = nullptr, *w_vel = nullptr, *w_ang = nullptr, *w_prev = nullptr,
+           *w_inv_mm = nullptr, *w_corners = nullptr, *w_hull = nullptr;
+    uint32_t* w_tag = nullptr;
+    uint32_t w_hull_count = 0;
+    uint32_t last_lag = 1;
+    uint32_t w_n_dyn = 0;
+    dio_gpu_body* w_stage_b = nullptr;
+    dio_avbd_body_state* w_stage_s = nullptr;
+    dio_avbd_constraint* w_stage_c = nullptr;
+    JointDev w_j;
+    ContainerParams w_ct{};
+    BroadBuffers wb{};
+    struct Pending {
+        uint2* pairs;
+        uint32_t* p_dev;
+        NarrowOut out;
+        float4* simplex;
+    } pend[2]{};
+    ContactSet cs[2]{};
+    CsrBuffers csr{};
+    float4 *s_pos[2] = {nullptr, nullptr}, *s_rot[2] = {nullptr, nullptr};
+    float4 *s_init_lin = nullptr, *s_init_ang = nullptr, *s_iner_lin = nullptr, *s_iner_ang = nullptr;
+    uint32_t* w_overflow = nullptr;
+    unsigned long long* w_acc = nullptr;  // sums of pairs, contacts, CSR nnz over a dio_step call
+    std::vector<hipEvent_t> prof_events;
+    uint64_t w_step = 0;
+    dio_step_stats stats{};
+    // graph cache: [segment][parity], segment 0 = contacts + broad + narrow, 1 = AVBD
+    hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    dio_avbd_options g_opts{};
+    dio_step_opts g_sopts{};
+    bool graphs_valid = false;
+    std::vector<hipEvent_t> events;
+
+    void fail(const std::string& m) { err = m; }
+    void drop_graphs() {
+        for (auto& seg : graphs)
+            for (auto& g : seg)
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
+        graphs_valid = false;
+    }
+    ~dio_ctx() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        for (auto e : events) (void)hipEventDestroy(e);
+        for (auto e : prof_events) (void)hipEventDestroy(e);
+        dev.release();
+        pin.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <class F> dio_status guarded(dio_ctx* ctx, F&& f) {
+    if (!ctx) return DIO_INVALID;
+    try {
+        return f();
+    } catch (const HipError& e) {
+        ctx->err = std::string(e.what) + ": " + hipGetErrorString(e.code);
+        return DIO_HIP;
+    } catch (const StatusError& e) {
+        ctx->err = e.msg;
+        if (e.status == DIO_CAPACITY) ctx->required = e.required;
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        ctx->err = "host allocation failed";
+        return DIO_HIP;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return DIO_INVALID;
+    }
+}
+
+[[noreturn]] void invalid(const std::string& m) { throw StatusError{DIO_INVALID, m, 0}; }
+[[noreturn]] void capacity(const std::string& m, uint64_t need) { throw StatusError{DIO_CAPACITY, m, need}; }
+
+void ensure_granular_broad(dio_ctx* c) {
+    if (c->g_ready) return;
+    DevArena& A = c->dev;
+    c->g_rec = A.alloc<dio_gpu_body>(c->n_cap);
+    c->g_rec_host = c->pin.alloc<dio_gpu_body>(c->n_cap);
+    c->g_pos = A.alloc<float4>(c->n_cap);
+    c->g_rot = A.alloc<float4>(c->n_cap);
+    c->g_tag = A.alloc<uint32_t>(c->n_cap);
+    c->g_corners = A.alloc<float4>((size_t)c->n_cap * 4);
+    c->g_hull = A.alloc<float4>(1);
+    c->gb = alloc_broad(A, c->n_cap, c->p_max);
+    c->gn = alloc_narrow(A, c->p_max);
+    c->g_simplex = A.alloc<float4>((size_t)c->p_max * 8);
+    c->g_pairs_up = A.alloc<uint2>(c->p_max);
+    c->g_pairs_host = c->pin.alloc<uint2>(c->p_max);
+    c->g_np_dev = A.alloc<uint32_t>(1);
+    c->g_out = A.alloc<dio_gpu_contact_out>((size_t)c->p_max * 4);
+    c->g_ready = true;
+}
+
+void ensure_granular_avbd(dio_ctx* c) {
+    if (c->a_ready) return;
+    DevArena& A = c->dev;
+    PinnedArena& P = c->pin;
+    const uint32_t n = c->n_cap, m = c->m_cap;
+    c->a_state_host = P.alloc<dio_avbd_body_state>(n);
+    c->a_pose_host = P.alloc<float>((size_t)n * 8);
+    c->a_con_host = P.alloc<dio_avbd_contact>(m);
+    c->a_cons_host = P.alloc<dio_avbd_constraint>(c->c_cap);
+    c->a_csr_host = P.alloc<uint32_t>((size_t)n + 1 + 2 * (size_t)m);
+    c->a_state = A.alloc<dio_avbd_body_state>(n);
+    c->a_pose = A.alloc<float>((size_t)n * 8);
+    c->a_con_rec = A.alloc<dio_avbd_contact>(m);
+    c->a_cur_pos = A.alloc<float4>(n);
+    c->a_cur_rot = A.alloc<float4>(n);
+    c->a_vel = A.alloc<float4>(n);
+    c->a_ang = A.alloc<float4>(n);
+    c->a_prev = A.alloc<float4>(n);
+    c->a_inv_mm = A.alloc<float4>(n);
+    for (int k = 0; k < 2; ++k) {
+        c->a_pos[k] = A.alloc<float4>(n);
+        c->a_rot[k] = A.alloc<float4>(n);
+    }
+    c->a_init_lin = A.alloc<float4>(n);
+    c->a_init_ang = A.alloc<float4>(n);
+    c->a_iner_lin = A.alloc<float4>(n);
+    c->a_iner_ang = A.alloc<float4>(n);
+    c->a_c = alloc_contacts(A, m);
+    c->a_off = A.alloc<uint32_t>((size_t)n + 1);
+    c->a_idx = A.alloc<uint32_t>(2 * (size_t)m);
+    c->a_j.abk = A.alloc<uint4>(c->c_cap);
+    c->a_j.anc_a = A.alloc<float4>(c->c_cap);
+    c->a_j.anc_b = A.alloc<float4>(c->c_cap);
+    c->a_j.off = A.alloc<uint32_t>((size_t)n + 1);
+    c->a_j.idx = A.alloc<uint32_t>(2 * (size_t)c->c_cap);
+    c->a_cons_stage = A.alloc<dio_avbd_constraint>(c->c_cap);
+    c->a_ready = true;
+}
+
+// unpack GpuAvbdBodyState + poses (granular upload)
+__global__ void k_unpack_state(const dio_avbd_body_state* st, const float* pose, uint32_t n, float4* vel, float4* ang,
+                               float4* prev, float4* inv_mm, float4* pos, float4* rot) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const dio_avbd_body_state s = st[i];
+    vel[i] = make_float4(s.vel[0], s.vel[1], s.vel[2], 0.0f);
+    ang[i] = make_float4(s.ang[0], s.ang[1], s.ang[2], 0.0f);
+    prev[i] = make_float4(s.prev_vel[0], s.prev_vel[1], s.prev_vel[2], 0.0f);
+    inv_mm[i] = make_float4(s.inv_moment[0], s.inv_moment[1], s.inv_moment[2], s.inv_mass);
+    if (pose) {
+        pos[i] = make_float4(pose[8 * i], pose[8 * i + 1], pose[8 * i + 2], 0.0f);
+        rot[i] = make_float4(pose[8 * i + 4], pose[8 * i + 5], pose[8 * i + 6], pose[8 * i + 7]);
+    }
+}
+
+__global__ void k_unpack_joints(const dio_avbd_constraint* rec, uint32_t c, uint4* abk, float4* a, float4* b) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= c) return;
+    const dio_avbd_constraint r = rec[i];
+    abk[i] = make_uint4(r.a, r.b, r.kind, 0u);
+    a[i] = make_float4(r.anchor_a[0], r.anchor_a[1], r.anchor_a[2], r.anchor_a[3]);
+    b[i] = make_float4(r.anchor_b[0], r.anchor_b[1], r.anchor_b[2], r.anchor_b[3]);
+}
+
+inline uint32_t blocks256(uint32_t n) { return (n + 255) / 256; }
+
+// upload constraints (records staged in pinned host memory) and their CSR
+void upload_joints(dio_ctx* c, JointDev& J, const dio_avbd_constraint* cons, uint32_t nc, uint32_t n,
+                   dio_avbd_constraint* stage_dev) {
+    for (uint32_t k = 0; k < nc; ++k)
+        if (cons[k].a >= n || cons[k].b >= n) invalid("constraint body index out of range");
+    std::vector<uint32_t> off, idx;
+    joint_csr(cons, nc, n, off, idx);
+    J.c = nc;
+    DIO_HIP_CHECK(hipMemcpyAsync(J.off, off.data(), off.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (!idx.empty())
+        DIO_HIP_CHECK(hipMemcpyAsync(J.idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (nc > 0) {
+        DIO_HIP_CHECK(hipMemcpyAsync(stage_dev, cons, nc * sizeof(dio_avbd_constraint), hipMemcpyHostToDevice,
+                                     c->stream));
+        hipLaunchKernelGGL(k_unpack_joints, dim3(blocks256(nc)), dim3(256), 0, c->stream, stage_dev, nc, J.abk,
+                           J.anc_a, J.anc_b);
+    }
+    // off / idx are host vectors: make the copies complete before they go
+    DIO_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+AvbdParams params_of(float dt, const float g[3], float alpha, float beta, float pmax, uint32_t n) {
+    AvbdParams p;
+    p.dt = dt;
+    p.alpha = alpha;
+    p.beta_lin = beta;
+    p.penalty_max = pmax;
+    p.gx = g[0];
+    p.gy = g[1];
+    p.gz = g[2];
+    p.n = n;
+    return p;
+}
+
+// Event pairs around each AVBD kernel (profile mode): kind 0 init, 1 primal,
+// 2 dual, 3 recover.
+struct KernelTimer {
+    std::vector<hipEvent_t>* pool = nullptr;
+    size_t next = 0;
+    std::vector<std::pair<int, size_t>> marks;  // kind, index of the start event
+    hipEvent_t take() {
+        if (next >= pool->size()) {
+            hipEvent_t e;
+            DIO_HIP_CHECK(hipEventCreate(&e));
+            pool->push_back(e);
+        }
+        return (*pool)[next++];
+    }
+    template <class F> void time(int kind, hipStream_t s, F&& launch) {
+        const size_t i = next;
+        DIO_HIP_CHECK(hipEventRecord(take(), s));
+        launch();
+        DIO_HIP_CHECK(hipEventRecord(take(), s));
+        marks.push_back({kind, i});
+    }
+};
+
+void schedule_avbd(const SolveView& v, const AvbdParams& p, uint32_t iters, uint32_t m_grid, hipStream_t s,
+                   KernelTimer* kt = nullptr) {
+    auto run = [&](int kind, auto&& launch) {
+        if (kt) kt->time(kind, s, launch);
+        else launch();
+    };
+    run(0, [&] { launch_avbd_init(v, p, s); });
+    for (uint32_t k = 0; k < iters; ++k) {
+        run(1, [&] { launch_avbd_primal(v, p, k % 2, s); });
+        run(2, [&] { launch_avbd_dual(v, p, k % 2, m_grid, s); });
+    }
+    run(3, [&] { launch_avbd_recover(v, p, (iters - 1) % 2, s); });
+}
+
+__global__ void k_stats_accum(const uint32_t* pairs, const uint32_t* m, const uint32_t* nnz, unsigned long long* acc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        acc[0] += *pairs;
+        acc[1] += *m;
+        acc[2] += *nnz;
+    }
+}
+__global__ void k_stats_zero(unsigned long long* acc) {
+    if (threadIdx.x < 4) acc[threadIdx.x] = 0ull;
+}
+
+}  // namespace
+
+// =========================================================================
+extern "C" {
+
+uint32_t dio_abi_version(void) { return DIO_ABI_VERSION; }
+
+void dio_default_options(dio_avbd_options* o) {
+    if (!o) return;
+    o->dt = 1.0f / 60.0f;
+    o->gravity[0] = 0.0f;
+    o->gravity[1] = -10.0f;
+    o->gravity[2] = 0.0f;
+    o->iterations = 10;
+    o->alpha = 0.99f;
+    o->beta_lin = 10000.0f;
+    o->margin = 0.01f;
+    o->penalty_min = 1.0f;
+    o->penalty_max = 1e10f;
+    o->gamma = 0.999f;
+    o->friction = 0.8f;
+}
+
+dio_status dio_create(int device, const dio_limits* limits, dio_ctx** out) {
+    if (!out || !limits || limits->max_bodies == 0) return DIO_INVALID;
+    *out = nullptr;
+    dio_ctx* c = nullptr;
+    try {
+        c = new dio_ctx();
+    } catch (...) {
+        return DIO_HIP;
+    }
+    dio_status st = guarded(c, [&]() -> dio_status {
+        c->device = device;
+        DIO_HIP_CHECK(hipSetDevice(device));
+        DIO_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->lim = *limits;
+        c->n_cap = limits->max_bodies;
+        c->p_max = limits->max_pairs ? limits->max_pairs : std::max<uint32_t>(1024u, 16u * c->n_cap);
+        c->m_cap = limits->max_contacts ? limits->max_contacts : 4u * c->p_max + 24u * c->n_cap;
+        c->c_cap = limits->max_constraints ? limits->max_constraints : std::max<uint32_t>(16u, c->n_cap);
+        c->h_cap = limits->max_hull_vertices ? limits->max_hull_vertices : 4096u;
+        const uint32_t big = std::max({c->n_cap + 1, c->p_max + 1, c->m_cap + 1});
+        c->scan_partials = c->dev.alloc<uint32_t>(scan_partials_size(big));
+        return DIO_OK;
+    });
+    if (st != DIO_OK) {
+        delete c;
+        return st;
+    }
+    *out = c;
+    return DIO_OK;
+}
+
+void dio_destroy(dio_ctx* ctx) { delete ctx; }
+
+const char* dio_last_error(const dio_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+uint64_t dio_required_capacity(const dio_ctx* ctx) { return ctx ? ctx->required : 0; }
+
+dio_status dio_synchronize(dio_ctx* ctx) {
+    return guarded(ctx, [&]() {
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return DIO_OK;
+    });
+}
+
+// ------------------------------------------------------------ broad phase
+dio_status dio_broad_submit(dio_ctx* ctx, const dio_gpu_body* bodies, uint32_t n) {
+    return guarded(ctx, [&]() {
+        if (n > 0 && !bodies) invalid("bodies is null");
+        if (n > ctx->n_cap) capacity("too many bodies for the context", n);
+        ensure_granular_broad(ctx);
+        DIO_HIP_CHECK(hipSetDevice(ctx->device));
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // staging buffer reuse
+        ctx->g_n = n;
+        if (n > 0) {
+            std::memcpy(ctx->g_rec_host, bodies, (size_t)n * sizeof(dio_gpu_body));
+            DIO_HIP_CHECK(hipMemcpyAsync(ctx->g_rec, ctx->g_rec_host, (size_t)n * sizeof(dio_gpu_body),
+                                         hipMemcpyHostToDevice, ctx->stream));
+            launch_unpack_gpu_bodies(ctx->g_rec, n, ctx->g_pos, ctx->g_rot, ctx->g_tag, ctx->g_corners, ctx->stream);
+        }
+        launch_fill_u32(ctx->gb.overflow, 0u, 1, ctx->stream);
+        launch_fill_u32(ctx->gb.err, 0u, 1, ctx->stream);
+        BodyView bv{ctx->g_pos, ctx->g_rot, ctx->g_tag, ctx->g_corners, ctx->g_hull};
+        run_broad(bv, n, ctx->gb, ctx->p_max, ctx->stream);
+        DIO_HIP_CHECK(hipGetLastError());
+        ctx->g_broad_submitted = true;
+        return DIO_OK;
+    });
+}
+
+dio_status dio_broad_count(dio_ctx* ctx, uint32_t* count) {
+    return guarded(ctx, [&]() {
+        if (!count) invalid("count is null");
+        if (!ctx->g_broad_submitted) throw StatusError{DIO_NOT_READY, "no broad phase submitted", 0};
+        uint32_t v[3] = {0, 0, 0};
+        DIO_HIP_CHECK(hipMemcpyAsync(&v[0], ctx->gb.offsets + ctx->g_n, 4, hipMemcpyDeviceToHost, ctx->stream));
+        DIO_HIP_CHECK(hipMemcpyAsync(&v[1], ctx->gb.overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
+        DIO_HIP_CHECK(hipMemcpyAsync(&v[2], ctx->gb.err, 4, hipMemcpyDeviceToHost, ctx->stream));
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (v[2]) throw StatusError{DIO_HIP, "LBVH build / traversal error (malformed tree or stack overflow)", 0};
+        *count = v[0];
+        if (v[0] > ctx->p_max) capacity("candidate pairs exceed max_pairs", v[0]);
+        return DIO_OK;
+    });
+}
+
+dio_status dio_broad_read_pairs(dio_ctx* ctx, dio_gpu_pair* pairs, uint32_t cap, uint32_t* n) {
+    return guarded(ctx, [&]() {
+        if (!n) invalid("n is null");
+        uint32_t count = 0;
+        dio_status st = dio_broad_count(ctx, &count);
+        if (st != DIO_OK) return st;
+        *n = count;
+        if (count > cap) capacity("output array too small", count);
+        if (count > 0) {
+            if (!pairs) invalid("pairs is null");
+            DIO_HIP_CHECK(hipMemcpy(pairs, ctx->gb.pairs, (size_t)count * 8, hipMemcpyDeviceToHost));
+        }
+        return DIO_OK;
+    });
+}
+
+dio_status dio_broad_read_aabbs(dio_ctx* ctx, float* aabbs, uint32_t cap_bodies) {
+    return guarded(ctx, [&]() {
+        if (!ctx->g_broad_submitted) throw StatusError{DIO_NOT_READY, "no broad phase submitted", 0};
+        if (cap_bodies < ctx->g_n) capacity("output array too small", ctx->g_n);
+        if (ctx->g_n == 0) return DIO_OK;
+        if (!aabbs) invalid("aabbs is null");
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        std::vector<float4> mn(ctx->g_n), mx(ctx->g_n);
+        DIO_HIP_CHECK(hipMemcpy(mn.data(), ctx->gb.aabb_min, ctx->g_n * 16, hipMemcpyDeviceToHost));
+        DIO_HIP_CHECK(hipMemcpy(mx.data(), ctx->gb.aabb_max, ctx->g_n * 16, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < ctx->g_n; ++i) {
+            std::memcpy(aabbs + 8 * i, &mn[i], 16);
+            std::memcpy(aabbs + 8 * i + 4, &mx[i], 16);
+        }
+        return DIO_OK;
+    });
+}
+
+// ------------------------------------------------------------ narrow phase
+static void narrow_dispatch(dio_ctx* ctx, const uint2* pairs, uint32_t count) {
+    NarrowOut o = ctx->gn;
+    o.pairs = pairs;
+    o.p_dev = ctx->g_np_dev;
+    DIO_HIP_CHECK(hipMemcpyAsync(ctx->g_np_dev, &ctx->g_last_pairs, 4, hipMemcpyHostToDevice, ctx->stream));
+    BodyView bv{ctx->g_pos, ctx->g_rot, ctx->g_tag, ctx->g_corners, ctx->g_hull};
+    run_narrow(bv, o, ctx->g_simplex, ctx->stream);
+    launch_pack_contact_out(o, count, ctx->g_out, ctx->stream);
+    DIO_HIP_CHECK(hipGetLastError());
+    // g_last_pairs is a host word read by the async copy above
+    DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->g_narrow_submitted = true;
+}
+
+dio_status dio_narrow_submit(dio_ctx* ctx, uint32_t pair_count) {
+    return guarded(ctx, [&]() {
+        if (!ctx->g_broad_submitted) throw StatusError{DIO_NOT_READY, "no broad phase submitted", 0};
+        if (pair_count > ctx->p_max) capacity("too many pairs for the compute buffers", pair_count);
+        ctx->g_last_pairs = pair_count;
+        narrow_dispatch(ctx, ctx->gb.pairs, pair_count);
+        return DIO_OK;
+    });
+}
+
+dio_status dio_narrow_submit_pairs(dio_ctx* ctx, const dio_gpu_body* bodies, uint32_t n, const dio_gpu_pair* pairs,
+                                   uint32_t pair_count) {
+    return guarded(ctx, [&]() {
+        if (n > ctx->n_cap) capacity("too many bodies for the compute buffers", n);
+        if (pair_count > ctx->p_max) capacity("too many pairs for the compute buffers", pair_count);
+        if ((n > 0 && !bodies) || (pair_count > 0 && !pairs)) invalid("null input");
+        for (uint32_t k = 0; k < pair_count; ++k)
+            if (pairs[k].a >= n || pairs[k].b >= n) invalid("pair index out of range");
+        ensure_granular_broad(ctx);
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->g_n = n;
+        if (n > 0) {
+            std::memcpy(ctx->g_rec_host, bodies, (size_t)n * sizeof(dio_gpu_body));
+            DIO_HIP_CHECK(hipMemcpyAsync(ctx->g_rec, ctx->g_rec_host, (size_t)n * sizeof(dio_gpu_body),
+                                         hipMemcpyHostToDevice, ctx->stream));
+            launch_unpack_gpu_bodies(ctx->g_rec, n, ctx->g_pos, ctx->g_rot, ctx->g_tag, ctx->g_corners, ctx->stream);
+        }
+        if (pair_count > 0) {
+            std::memcpy(ctx->g_pairs_host, pairs, (size_t)pair_count * 8);
+            DIO_HIP_CHECK(hipMemcpyAsync(ctx->g_pairs_up, ctx->g_pairs_host, (size_t)pair_count * 8,
+                                         hipMemcpyHostToDevice, ctx->stream));
+        }
+        ctx->g_last_pairs = pair_count;
+        ctx->g_broad_submitted = false;
+        narrow_dispatch(ctx, ctx->g_pairs_up, pair_count);
+        return DIO_OK;
+    });
+}
+
+dio_status dio_narrow_read(dio_ctx* ctx, dio_gpu_contact_out* slots, uint32_t cap_slots, uint32_t* n_slots) {
+    return guarded(ctx, [&]() {
+        if (!n_slots) invalid("n_slots is null");
+        if (!ctx->g_narrow_submitted) {
+            *n_slots = 0;
+            return DIO_OK;
+        }
+        const uint32_t ns = ctx->g_last_pairs * 4;
+        *n_slots = ns;
+        if (ns > cap_slots) capacity("output array too small", ns);
+        if (ns > 0) {
+            if (!slots) invalid("slots is null");
+            DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            DIO_HIP_CHECK(hipMemcpy(slots, ctx->g_out, (size_t)ns * sizeof(dio_gpu_contact_out),
+                                    hipMemcpyDeviceToHost));
+        }
+        return DIO_OK;
+    });
+}
+
+// ------------------------------------------------------------ AVBD solve
+dio_status dio_avbd_submit(dio_ctx* ctx, const dio_avbd_body_state* state, const float* positions,
+                           const float* orientations, uint32_t n, const dio_avbd_contact* contacts, uint32_t m,
+                           const dio_avbd_constraint* constraints, uint32_t c, const uint32_t* offsets,
+                           const uint32_t* indices, uint32_t nidx, const dio_run_options* options) {
+    return guarded(ctx, [&]() {
+        if (!options) invalid("options is null");
+        if (n > ctx->n_cap) capacity("too many bodies for the AVBD buffers", n);
+        if (m > ctx->m_cap) capacity("too many contacts for the AVBD buffers", m);
+        if (nidx > 2 * ctx->m_cap) capacity("too many CSR indices", nidx);
+        if (c > ctx->c_cap) capacity("too many constraints", c);
+        if (options->iterations == 0) invalid("iterations must be at least 1");
+        ensure_granular_avbd(ctx);
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->a_n = n;
+        ctx->a_m = m;
+        ctx->a_iters = options->iterations;
+        ctx->a_submitted = true;
+        if (n == 0) return DIO_OK;  // gpu_physics.rs:462-464
+        if (!state || !positions || !orientations || !offsets || (m > 0 && !contacts) || (nidx > 0 && !indices) ||
+            (c > 0 && !constraints))
+            invalid("null input");
+        if (offsets[n] != nidx) invalid("offsets[n] must equal nidx");
+        for (uint32_t k = 0; k < m; ++k)
+            if (contacts[k].a >= n || (contacts[k].b != DIO_CONTAINER && contacts[k].b >= n))
+                invalid("contact body index out of range");
+        for (uint32_t k = 0; k < nidx; ++k)
+            if (indices[k] >= m) invalid("CSR index out of range");
+        for (uint32_t i = 0; i < n; ++i)
+            if (offsets[i] > offsets[i + 1]) invalid("CSR offsets must be non-decreasing");
+        std::memcpy(ctx->a_state_host, state, (size_t)n * sizeof(dio_avbd_body_state));
+        for (uint32_t i = 0; i < n; ++i) {
+            float* p = ctx->a_pose_host + 8 * (size_t)i;
+            p[0] = positions[3 * i]; p[1] = positions[3 * i + 1]; p[2] = positions[3 * i + 2]; p[3] = 0.0f;
+            p[4] = orientations[4 * i]; p[5] = orientations[4 * i + 1];
+            p[6] = orientations[4 * i + 2]; p[7] = orientations[4 * i + 3];
+        }
+        if (m > 0) std::memcpy(ctx->a_con_host, contacts, (size_t)m * sizeof(dio_avbd_contact));
+        std::memcpy(ctx->a_csr_host, offsets, ((size_t)n + 1) * 4);
+        if (nidx > 0) std::memcpy(ctx->a_csr_host + n + 1, indices, (size_t)nidx * 4);
+        hipStream_t s = ctx->stream;
+        DIO_HIP_CHECK(hipMemcpyAsync(ctx->a_state, ctx->a_state_host, (size_t)n * sizeof(dio_avbd_body_state),
+                                     hipMemcpyHostToDevice, s));
+        DIO_HIP_CHECK(hipMemcpyAsync(ctx->a_pose, ctx->a_pose_host, (size_t)n * 32, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_unpack_state, dim3(blocks256(n)), dim3(256), 0, s, ctx->a_state, ctx->a_pose, n,
+                           ctx->a_vel, ctx->a_ang, ctx->a_prev, ctx->a_inv_mm, ctx->a_cur_pos, ctx->a_cur_rot);
+        if (m > 0) {
+            DIO_HIP_CHECK(hipMemcpyAsync(ctx->a_con_rec, ctx->a_con_host, (size_t)m * sizeof(dio_avbd_contact),
+                                         hipMemcpyHostToDevice, s));
+            launch_unpack_contacts(ctx->a_con_rec, m, ctx->a_c, s);
+        }
+        DIO_HIP_CHECK(hipMemcpyAsync(ctx->a_off, ctx->a_csr_host, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, s));
+        if (nidx > 0)
+            DIO_HIP_CHECK(hipMemcpyAsync(ctx->a_idx, ctx->a_csr_host + n + 1, (size_t)nidx * 4, hipMemcpyHostToDevice,
+                                         s));
+        if (c > 0) std::memcpy(ctx->a_cons_host, constraints, (size_t)c * sizeof(dio_avbd_constraint));
+        upload_joints(ctx, ctx->a_j, ctx->a_cons_host, c, n, ctx->a_cons_stage);
+        SolveView v{};
+        v.cur_pos = ctx->a_cur_pos;
+        v.cur_rot = ctx->a_cur_rot;
+        v.pos[0] = ctx->a_pos[0]; v.pos[1] = ctx->a_pos[1];
+        v.rot[0] = ctx->a_rot[0]; v.rot[1] = ctx->a_rot[1];
+        v.init_lin = ctx->a_init_lin; v.init_ang = ctx->a_init_ang;
+        v.iner_lin = ctx->a_iner_lin; v.iner_ang = ctx->a_iner_ang;
+        v.vel = ctx->a_vel; v.ang = ctx->a_ang; v.prev_vel = ctx->a_prev; v.inv_mm = ctx->a_inv_mm;
+        v.c_ab = ctx->a_c.ab; v.c_n = ctx->a_c.n; v.c_t1 = ctx->a_c.t1; v.c_t2 = ctx->a_c.t2;
+        v.c_ra = ctx->a_c.ra; v.c_rb = ctx->a_c.rb; v.c_c0 = ctx->a_c.c0; v.c_pen = ctx->a_c.pen;
+        v.c_lam = ctx->a_c.lam;
+        v.m_dev = nullptr;
+        v.m_host = m;
+        v.offsets = ctx->a_off;
+        v.indices = ctx->a_idx;
+        v.j_off = ctx->a_j.off; v.j_idx = ctx->a_j.idx; v.j_abk = ctx->a_j.abk;
+        v.j_anc_a = ctx->a_j.anc_a; v.j_anc_b = ctx->a_j.anc_b;
+        v.out_pos = nullptr;
+        v.out_rot = nullptr;
+        const AvbdParams p = params_of(options->dt, options->gravity, options->alpha, options->beta_lin,
+                                       options->penalty_max, n);
+        schedule_avbd(v, p, options->iterations, m, s);
+        DIO_HIP_CHECK(hipGetLastError());
+        return DIO_OK;
+    });
+}
+
+dio_status dio_avbd_read(dio_ctx* ctx, float* positions, float* orientations, float* velocities,
+                         float* angular_velocities, float* prev_velocities, float* lambda, float* penalty) {
+    return guarded(ctx, [&]() {
+        if (!ctx->a_submitted) throw StatusError{DIO_NOT_READY, "no AVBD solve submitted", 0};
+        const uint32_t n = ctx->a_n, m = ctx->a_m;
+        if (n == 0) return DIO_OK;
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        const uint32_t target = 1 - (ctx->a_iters - 1) % 2;  // gpu_physics.rs:584-585
+        std::vector<float4> buf(std::max(n, m));
+        auto get3 = [&](const float4* src, uint32_t count, float* dst) {
+            if (!dst || count == 0) return;
+            DIO_HIP_CHECK(hipMemcpy(buf.data(), src, (size_t)count * 16, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < count; ++i) {
+                dst[3 * i] = buf[i].x; dst[3 * i + 1] = buf[i].y; dst[3 * i + 2] = buf[i].z;
+            }
+        };
+        get3(ctx->a_pos[target], n, positions);
+        if (orientations) {
+            DIO_HIP_CHECK(hipMemcpy(orientations, ctx->a_rot[target], (size_t)n * 16, hipMemcpyDeviceToHost));
+        }
+        get3(ctx->a_vel, n, velocities);
+        get3(ctx->a_ang, n, angular_velocities);
+        get3(ctx->a_prev, n, prev_velocities);
+        get3(ctx->a_c.lam, m, lambda);
+        get3(ctx->a_c.pen, m, penalty);
+        return DIO_OK;
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ fused world
+namespace {
+
+void ensure_world(dio_ctx* c) {
+    if (c->w_ready) return;
+    DevArena& A = c->dev;
+    const uint32_t n = c->n_cap, m = c->m_cap, p = c->p_max;
+    c->w_pos = A.alloc<float4>(n);
+    c->w_rot = A.alloc<float4>(n);
+    c->w_vel = A.alloc<float4>(n);
+    c->w_ang = A.alloc<float4>(n);
+    c->w_prev = A.alloc<float4>(n);
+    c->w_inv_mm = A.alloc<float4>(n);
+    c->w_tag = A.alloc<uint32_t>(n);
+    c->w_corners = A.alloc<float4>((size_t)n * 4);
+    c->w_hull = A.alloc<float4>(c->h_cap);
+    c->w_j.abk = A.alloc<uint4>(c->c_cap);
+    c->w_j.anc_a = A.alloc<float4>(c->c_cap);
+    c->w_j.anc_b = A.alloc<float4>(c->c_cap);
+    c->w_j.off = A.alloc<uint32_t>((size_t)n + 1);
+    c->w_j.idx = A.alloc<uint32_t>(2 * (size_t)c->c_cap);
+    c->wb = alloc_broad(A, n, p);
+    for (int k = 0; k < 2; ++k) {
+        auto& pd = c->pend[k];
+        pd.pairs = A.alloc<uint2>(p);
+        pd.p_dev = A.alloc<uint32_t>(1);
+        pd.out = alloc_narrow(A, p);
+        pd.out.pairs = pd.pairs;
+        pd.out.p_dev = pd.p_dev;
+        pd.simplex = A.alloc<float4>((size_t)p * 8);
+        ContactSet& cs = c->cs[k];
+        cs.c = alloc_contacts(A, m);
+        cs.m_dev = A.alloc<uint32_t>(1);
+        cs.m_narrow = A.alloc<uint32_t>(1);
+        cs.cont_cnt = A.alloc<uint32_t>(n);
+        cs.cont_off = A.alloc<uint32_t>((size_t)n + 1);
+        cs.m_cap = m;
+    }
+    c->csr.cnt = A.alloc<uint32_t>(n);
+    c->csr.cur = A.alloc<uint32_t>(n);
+    c->csr.off = A.alloc<uint32_t>((size_t)n + 1);
+    c->csr.idx = A.alloc<uint32_t>(2 * (size_t)m);
+    c->csr.scan_partials = A.alloc<uint32_t>(scan_partials_size(n + 1));
+    c->csr.idx_cap = 2 * m;
+    for (int k = 0; k < 2; ++k) {
+        c->s_pos[k] = A.alloc<float4>(n);
+        c->s_rot[k] = A.alloc<float4>(n);
+    }
+    c->s_init_lin = A.alloc<float4>(n);
+    c->s_init_ang = A.alloc<float4>(n);
+    c->s_iner_lin = A.alloc<float4>(n);
+    c->s_iner_ang = A.alloc<float4>(n);
+    c->w_overflow = A.alloc<uint32_t>(1);
+    c->w_acc = A.alloc<unsigned long long>(4);
+    c->w_stage_b = A.alloc<dio_gpu_body>(n);
+    c->w_stage_s = A.alloc<dio_avbd_body_state>(n);
+    c->w_stage_c = A.alloc<dio_avbd_constraint>(c->c_cap);
+    c->w_ready = true;
+}
+
+NarrowSource source_of(const dio_ctx::Pending& p) {
+    return NarrowSource{p.pairs, p.out.off, p.p_dev, p.out.n, p.out.pa, p.out.pb};
+}
+
+// Enqueue segment `seg` of one step with parity s (0: contacts + broad +
+// narrow, 1: AVBD).
+void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, const dio_step_opts& so,
+                     KernelTimer* kt = nullptr) {
+    hipStream_t st = c->stream;
+    const uint32_t n = c->w_n;
+    WorldView wv{c->w_pos, c->w_rot, c->w_vel, c->w_inv_mm, c->w_tag, c->w_corners, c->w_hull, n};
+    BodyView bv{c->w_pos, c->w_rot, c->w_tag, c->w_corners, c->w_hull};
+    BuildOptions bo{o.dt, o.alpha, o.margin, o.penalty_min, o.penalty_max, o.gamma, o.friction};
+    const int lag = so.contact_lag ? 1 : 0;
+    if (seg == 0) {
+        // lag 1: contacts come from pend[s] (previous step's narrow phase), the
+        // broad / narrow output of this step goes to pend[1 - s].  lag 0: this
+        // step's narrow output goes to pend[s] and is used at once.
+        const int src = s;
+        const int dst_pend = lag ? 1 - s : s;
+        auto broad_narrow = [&]() {
+            BroadBuffers b = c->wb;
+            b.pairs = c->pend[dst_pend].pairs;
+            b.p_dev = c->pend[dst_pend].p_dev;
+            b.overflow = c->w_overflow;
+            run_broad(bv, n, b, c->p_max, st);
+            run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, st);
+        };
+        if (!lag) broad_narrow();
+        run_contact_build(wv, c->w_ct, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]),
+                          c->cs[1 - s], c->scan_partials, c->w_overflow, st);
+        run_csr(c->cs[s], n, c->csr, c->w_overflow, st);
+        if (lag) broad_narrow();
+    } else {
+        SolveView v{};
+        v.cur_pos = c->w_pos;
+        v.cur_rot = c->w_rot;
+        v.pos[0] = c->s_pos[0]; v.pos[1] = c->s_pos[1];
+        v.rot[0] = c->s_rot[0]; v.rot[1] = c->s_rot[1];
+        v.init_lin = c->s_init_lin; v.init_ang = c->s_init_ang;
+        v.iner_lin = c->s_iner_lin; v.iner_ang = c->s_iner_ang;
+        v.vel = c->w_vel; v.ang = c->w_ang; v.prev_vel = c->w_prev; v.inv_mm = c->w_inv_mm;
+        const ContactSet& cs = c->cs[s];
+        v.c_ab = cs.c.ab; v.c_n = cs.c.n; v.c_t1 = cs.c.t1; v.c_t2 = cs.c.t2; v.c_ra = cs.c.ra; v.c_rb = cs.c.rb;
+        v.c_c0 = cs.c.c0; v.c_pen = cs.c.pen; v.c_lam = cs.c.lam;
+        v.m_dev = cs.m_dev;
+        v.m_host = 0;
+        v.offsets = c->csr.off;
+        v.indices = c->csr.idx;
+        v.j_off = c->w_j.off; v.j_idx = c->w_j.idx; v.j_abk = c->w_j.abk;
+        v.j_anc_a = c->w_j.anc_a; v.j_anc_b = c->w_j.anc_b;
+        v.out_pos = c->w_pos;
+        v.out_rot = c->w_rot;
+        const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
+        schedule_avbd(v, p, o.iterations, c->m_cap, st, kt);
+        const int bn_pend = so.contact_lag ? 1 - s : s;
+        hipLaunchKernelGGL(k_stats_accum, dim3(1), dim3(64), 0, st, (const uint32_t*)c->pend[bn_pend].p_dev,
+                           (const uint32_t*)cs.m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
+    }
+}
+
+bool same_opts(const dio_avbd_options& a, const dio_avbd_options& b, const dio_step_opts& x,
+               const dio_step_opts& y) {
+    return std::memcmp(&a, &b, sizeof(a)) == 0 && std::memcmp(&x, &y, sizeof(x)) == 0;
+}
+
+bool graphs_disabled() {
+    const char* e = std::getenv("DIO_NO_GRAPH");
+    return e && e[0] == '1';
+}
+
+void build_graphs(dio_ctx* c, const dio_avbd_options& o, const dio_step_opts& so) {
+    c->drop_graphs();
+    for (int seg = 0; seg < 2; ++seg)
+        for (int s = 0; s < 2; ++s) {
+            hipGraph_t g = nullptr;
+            DIO_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_segment(c, seg, s, o, so);
+            DIO_HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+            hipGraphExec_t ge = nullptr;
+            hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            DIO_HIP_CHECK(e);
+            c->graphs[seg][s] = ge;
+        }
+    c->g_opts = o;
+    c->g_sopts = so;
+    c->graphs_valid = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+dio_status dio_world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd_body_state* state, uint32_t n,
+                            const dio_avbd_constraint* constraints, uint32_t c, const dio_container* container) {
+    return guarded(ctx, [&]() {
+        if (n > ctx->n_cap) capacity("too many bodies for the context", n);
+        if (c > ctx->c_cap) capacity("too many constraints for the context", c);
+        if ((n > 0 && (!bodies || !state)) || (c > 0 && !constraints)) invalid("null input");
+        ensure_world(ctx);
+        hipStream_t s = ctx->stream;
+        DIO_HIP_CHECK(hipStreamSynchronize(s));
+        ctx->drop_graphs();
+        ctx->w_n = n;
+        ctx->w_step = 0;
+        ctx->w_friction.resize(n);
+        ctx->w_n_dyn = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            ctx->w_friction[i] = state[i].friction;
+            if (state[i].inv_mass != 0.0f) ctx->w_n_dyn += 1;
+        }
+        if (n > 0) {
+            dio_gpu_body* stage_b = ctx->w_stage_b;
+            dio_avbd_body_state* stage_s = ctx->w_stage_s;
+            DIO_HIP_CHECK(hipMemcpy(stage_b, bodies, (size_t)n * sizeof(dio_gpu_body), hipMemcpyHostToDevice));
+            DIO_HIP_CHECK(hipMemcpy(stage_s, state, (size_t)n * sizeof(dio_avbd_body_state), hipMemcpyHostToDevice));
+            launch_unpack_gpu_bodies(stage_b, n, ctx->w_pos, ctx->w_rot, ctx->w_tag, ctx->w_corners, s);
+            hipLaunchKernelGGL(k_unpack_state, dim3(blocks256(n)), dim3(256), 0, s, stage_s, (const float*)nullptr, n,
+                               ctx->w_vel, ctx->w_ang, ctx->w_prev, ctx->w_inv_mm, (float4*)nullptr,
+                               (float4*)nullptr);
+            DIO_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        upload_joints(ctx, ctx->w_j, constraints, c, n, ctx->w_stage_c);
+        ContainerParams ct{};
+        if (container && container->enabled) {
+            ct.cx = container->center[0]; ct.cy = container->center[1]; ct.cz = container->center[2];
+            ct.qx = container->rotation[0]; ct.qy = container->rotation[1];
+            ct.qz = container->rotation[2]; ct.qw = container->rotation[3];
+            ct.half = container->half_extent;
+            ct.enabled = 1;
+        }
+        ctx->w_ct = ct;
+        // empty lagged narrow output and empty previous contact sets
+        for (int k = 0; k < 2; ++k) {
+            launch_fill_u32(ctx->pend[k].p_dev, 0u, 1, s);
+            launch_fill_u32(ctx->pend[k].out.off, 0u, 1, s);
+            launch_fill_u32(ctx->cs[k].m_dev, 0u, 1, s);
+            launch_fill_u32(ctx->cs[k].m_narrow, 0u, 1, s);
+            launch_fill_u32(ctx->cs[k].cont_off, 0u, n + 1, s);
+        }
+        launch_fill_u32(ctx->w_overflow, 0u, 1, s);
+        launch_fill_u32(ctx->wb.err, 0u, 1, s);
+        DIO_HIP_CHECK(hipStreamSynchronize(s));
+        std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+        return DIO_OK;
+    });
+}
+
+dio_status dio_world_upload_hulls(dio_ctx* ctx, const float* vertices, uint32_t count) {
+    return guarded(ctx, [&]() {
+        if (count > ctx->h_cap) capacity("too many hull vertices", count);
+        if (count > 0 && !vertices) invalid("vertices is null");
+        ensure_world(ctx);
+        ensure_granular_broad(ctx);
+        std::vector<float4> v(count);
+        for (uint32_t i = 0; i < count; ++i)
+            v[i] = make_float4(vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2], 0.0f);
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (count > 0) DIO_HIP_CHECK(hipMemcpy(ctx->w_hull, v.data(), (size_t)count * 16, hipMemcpyHostToDevice));
+        ctx->w_hull_count = count;
+        ctx->g_hull = ctx->w_hull;  // the granular pipeline shares the pool
+        ctx->drop_graphs();
+        return DIO_OK;
+    });
+}
+
+dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_step_opts* opts, uint32_t nsteps) {
+    return guarded(ctx, [&]() {
+        if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
+        if (!options) invalid("options is null");
+        if (options->iterations == 0) invalid("iterations must be at least 1");
+        dio_step_opts so{1, 0, 0};
+        if (opts) so = *opts;
+        if (so.mode != 0) invalid("only the Jacobi mode (0) is available");
+        ctx->last_lag = so.contact_lag ? 1 : 0;
+        hipStream_t s = ctx->stream;
+        DIO_HIP_CHECK(hipSetDevice(ctx->device));
+        const bool use_graph = !graphs_disabled() && !so.profile;
+        KernelTimer kt;
+        kt.pool = &ctx->prof_events;
+        if (use_graph && ctx->w_n > 0 && (!ctx->graphs_valid || !same_opts(ctx->g_opts, *options, ctx->g_sopts, so)))
+            build_graphs(ctx, *options, so);
+        if (ctx->events.size() < 2 * (size_t)nsteps + 2) {
+            size_t have = ctx->events.size();
+            ctx->events.resize(2 * (size_t)nsteps + 2);
+            for (size_t k = have; k < ctx->events.size(); ++k) DIO_HIP_CHECK(hipEventCreate(&ctx->events[k]));
+        }
+        hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, s, ctx->w_acc);
+        DIO_HIP_CHECK(hipEventRecord(ctx->events[0], s));
+        for (uint32_t k = 0; k < nsteps && ctx->w_n > 0; ++k) {
+            const int par = (int)(ctx->w_step % 2);
+            if (use_graph) {
+                DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[0][par], s));
+                DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
+                DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[1][par], s));
+                DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
+            } else {
+                enqueue_segment(ctx, 0, par, *options, so);
+                DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
+                enqueue_segment(ctx, 1, par, *options, so, so.profile ? &kt : nullptr);
+                DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
+            }
+            ctx->w_step += 1;
+        }
+        DIO_HIP_CHECK(hipEventRecord(ctx->events[1], s));
+        DIO_HIP_CHECK(hipGetLastError());
+        DIO_HIP_CHECK(hipStreamSynchronize(s));
+        // stats of the last step
+        dio_step_stats& st = ctx->stats;
+        std::memset(&st, 0, sizeof(st));
+        st.steps = nsteps;
+        st.dynamic_bodies = ctx->w_n_dyn;
+        if (ctx->w_n > 0 && nsteps > 0) {
+            const int par = (int)((ctx->w_step - 1) % 2);
+            const int lag = so.contact_lag ? 1 : 0;
+            const int bn_pend = lag ? 1 - par : par;
+            uint32_t w[5] = {0, 0, 0, 0, 0};
+            DIO_HIP_CHECK(hipMemcpy(&w[0], ctx->pend[bn_pend].p_dev, 4, hipMemcpyDeviceToHost));
+            DIO_HIP_CHECK(hipMemcpy(&w[1], ctx->cs[par].m_dev, 4, hipMemcpyDeviceToHost));
+            DIO_HIP_CHECK(hipMemcpy(&w[2], ctx->csr.off + ctx->w_n, 4, hipMemcpyDeviceToHost));
+            DIO_HIP_CHECK(hipMemcpy(&w[3], ctx->cs[par].m_narrow, 4, hipMemcpyDeviceToHost));
+            DIO_HIP_CHECK(hipMemcpy(&w[4], ctx->w_overflow, 4, hipMemcpyDeviceToHost));
+            uint32_t terr = 0;
+            DIO_HIP_CHECK(hipMemcpy(&terr, ctx->wb.err, 4, hipMemcpyDeviceToHost));
+            st.pairs = w[0];
+            st.contacts = w[1];
+            st.csr_nnz = w[2];
+            st.narrow_contacts = w[3];
+            float total = 0.0f;
+            DIO_HIP_CHECK(hipEventElapsedTime(&total, ctx->events[0], ctx->events[1]));
+            st.ms_total = total;
+            float avbd = 0.0f;
+            for (uint32_t k = 0; k < nsteps; ++k) {
+                float t = 0.0f;
+                DIO_HIP_CHECK(hipEventElapsedTime(&t, ctx->events[2 + 2 * k], ctx->events[3 + 2 * k]));
+                avbd += t;
+            }
+            st.ms_avbd = avbd;
+            unsigned long long acc[4] = {0, 0, 0, 0};
+            DIO_HIP_CHECK(hipMemcpy(acc, ctx->w_acc, sizeof(acc), hipMemcpyDeviceToHost));
+            st.sum_pairs = acc[0];
+            st.sum_contacts = acc[1];
+            st.sum_csr_nnz = acc[2];
+            float* ms[4] = {&st.ms_init, &st.ms_primal, &st.ms_dual, &st.ms_recover};
+            uint32_t* cnt[4] = {&st.n_init, &st.n_primal, &st.n_dual, &st.n_recover};
+            for (const auto& mk : kt.marks) {
+                float t = 0.0f;
+                DIO_HIP_CHECK(hipEventElapsedTime(&t, ctx->prof_events[mk.second], ctx->prof_events[mk.second + 1]));
+                *ms[mk.first] += t;
+                *cnt[mk.first] += 1;
+            }
+            if (terr) throw StatusError{DIO_HIP, "LBVH build / traversal error (malformed tree or stack overflow)", 0};
+            if (w[4]) capacity("a step exceeded max_pairs / max_contacts; results are truncated", w[4]);
+        }
+        return DIO_OK;
+    });
+}
+
+dio_status dio_step_get_stats(dio_ctx* ctx, dio_step_stats* out) {
+    return guarded(ctx, [&]() {
+        if (!out) invalid("out is null");
+        *out = ctx->stats;
+        return DIO_OK;
+    });
+}
+
+dio_status dio_world_download(dio_ctx* ctx, float* positions, float* orientations, float* velocities,
+                              float* angular_velocities, float* prev_velocities) {
+    return guarded(ctx, [&]() {
+        if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
+        const uint32_t n = ctx->w_n;
+        if (n == 0) return DIO_OK;
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        std::vector<float4> buf(n);
+        auto get3 = [&](const float4* src, float* dst) {
+            if (!dst) return;
+            DIO_HIP_CHECK(hipMemcpy(buf.data(), src, (size_t)n * 16, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; ++i) {
+                dst[3 * i] = buf[i].x; dst[3 * i + 1] = buf[i].y; dst[3 * i + 2] = buf[i].z;
+            }
+        };
+        get3(ctx->w_pos, positions);
+        if (orientations) DIO_HIP_CHECK(hipMemcpy(orientations, ctx->w_rot, (size_t)n * 16, hipMemcpyDeviceToHost));
+        get3(ctx->w_vel, velocities);
+        get3(ctx->w_ang, angular_velocities);
+        get3(ctx->w_prev, prev_velocities);
+        return DIO_OK;
+    });
+}
+
+dio_status dio_world_contacts(dio_ctx* ctx, dio_avbd_contact* out, uint32_t cap, uint32_t* m_out) {
+    return guarded(ctx, [&]() {
+        if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
+        if (!m_out) invalid("m is null");
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->w_step == 0) {
+            *m_out = 0;
+            return DIO_OK;
+        }
+        const ContactSet& cs = ctx->cs[(ctx->w_step - 1) % 2];
+        uint32_t m = 0;
+        DIO_HIP_CHECK(hipMemcpy(&m, cs.m_dev, 4, hipMemcpyDeviceToHost));
+        *m_out = m;
+        if (m > cap) capacity("output array too small", m);
+        if (m == 0) return DIO_OK;
+        if (!out) invalid("out is null");
+        std::vector<uint2> ab(m);
+        std::vector<float4> f[8];
+        const float4* src[8] = {cs.c.n, cs.c.t1, cs.c.t2, cs.c.ra, cs.c.rb, cs.c.c0, cs.c.pen, cs.c.lam};
+        DIO_HIP_CHECK(hipMemcpy(ab.data(), cs.c.ab, (size_t)m * 8, hipMemcpyDeviceToHost));
+        for (int k = 0; k < 8; ++k) {
+            f[k].resize(m);
+            DIO_HIP_CHECK(hipMemcpy(f[k].data(), src[k], (size_t)m * 16, hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < m; ++i) {
+            dio_avbd_contact& r = out[i];
+            std::memset(&r, 0, sizeof(r));
+            r.a = ab[i].x;
+            r.b = ab[i].y;
+            r.friction = f[0][i].w;
+            float* dst[8] = {r.normal, r.tangent1, r.tangent2, r.r_a, r.r_b, r.c0, r.penalty, r.lambda};
+            for (int k = 0; k < 8; ++k) {
+                dst[k][0] = f[k][i].x;
+                dst[k][1] = f[k][i].y;
+                dst[k][2] = f[k][i].z;
+                dst[k][3] = 0.0f;
+            }
+        }
+        return DIO_OK;
+    });
+}
+
+dio_status dio_world_pairs(dio_ctx* ctx, dio_gpu_pair* out, uint32_t cap, uint32_t* n_out) {
+    return guarded(ctx, [&]() {
+        if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
+        if (!n_out) invalid("n is null");
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->w_step == 0) {
+            *n_out = 0;
+            return DIO_OK;
+        }
+        const int par = (int)((ctx->w_step - 1) % 2);
+        const dio_ctx::Pending& p = ctx->pend[ctx->last_lag ? 1 - par : par];
+        uint32_t np = 0;
+        DIO_HIP_CHECK(hipMemcpy(&np, p.p_dev, 4, hipMemcpyDeviceToHost));
+        *n_out = np;
+        if (np > cap) capacity("output array too small", np);
+        if (np > 0) {
+            if (!out) invalid("out is null");
+            DIO_HIP_CHECK(hipMemcpy(out, p.pairs, (size_t)np * 8, hipMemcpyDeviceToHost));
+        }
+        return DIO_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// dio_device.hpp — device data layout shared by the step's kernels.
+//
+// Bodies, contacts and joints live in HBM as structure-of-arrays of float4
+// (one 16-byte load per field per lane, coalesced for sweeps and a single
+// cache line per field for gathers).  Counts that the GPU produces (pairs,
+// contacts) stay in device memory and kernels read them there, so a whole step
+// is a fixed sequence of launches that can be captured in a HIP graph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dio_math.hpp"
+
+namespace dio {
+
+// AvbdPush (gpu_physics.rs:61-71) less the per-launch parity
+struct AvbdParams {
+    float dt, alpha, beta_lin, penalty_max;
+    float gx, gy, gz;
+    uint32_t n;  // body count
+};
+
+// Everything the four AVBD kernels touch (gpu_physics.rs bindings 0-10).
+struct SolveView {
+    // current pose S_k (read by init) and the two Jacobi parity buffers
+    const float4* cur_pos;
+    const float4* cur_rot;
+    float4* pos[2];
+    float4* rot[2];
+    float4* init_lin;
+    float4* init_ang;
+    float4* iner_lin;
+    float4* iner_ang;
+    // GpuAvbdBodyState: vel, ang, prev_vel; inv_moment.xyz with inv_mass in .w
+    float4* vel;
+    float4* ang;
+    float4* prev_vel;
+    const float4* inv_mm;
+    // contacts (GpuAvbdContact as SoA; friction rides in normal.w)
+    const uint2* c_ab;
+    const float4* c_n;
+    const float4* c_t1;
+    const float4* c_t2;
+    const float4* c_ra;
+    const float4* c_rb;
+    const float4* c_c0;
+    float4* c_pen;
+    float4* c_lam;
+    const uint32_t* m_dev;  // contact count in device memory, or null: m_host
+    uint32_t m_host;
+    // per-body contact CSR (physics/gpu.rs:286-310)
+    const uint32_t* offsets;
+    const uint32_t* indices;
+    // per-body joint CSR in ascending constraint order; joint records SoA
+    const uint32_t* j_off;
+    const uint32_t* j_idx;
+    const uint4* j_abk;      // a, b, kind, 0
+    const float4* j_anc_a;   // anchor_a.xyz, rest / max length
+    const float4* j_anc_b;   // anchor_b.xyz, stiffness
+    // fused path: recover writes the final pose back into the world pose
+    float4* out_pos;
+    float4* out_rot;
+};
+
+// Launchers (avbd.hip).  All run on `stream`.
+void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t stream);
+void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream);
+void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t m_grid, hipStream_t stream);
+void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream);
+
+}  // namespace dio

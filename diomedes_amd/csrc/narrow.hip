@@ -1,0 +1,496 @@
+// narrow.hip — narrow phase for gfx950.
+//
+// Counterpart of narrowphase_main (narrowphase/src/lib.rs:89-158).  Pairs are
+// dispatched by class:
+//  - box / box (CUBE or BOX): the OBB SAT manifold of cuboid.rs, up to 4
+//    contacts, one lane per pair, in registers;
+//  - sphere / sphere: the analytic contact of lib.rs:125-144;
+//  - everything else: GJK (gjk.rs) runs on one lane per pair; intersecting
+//    pairs are appended (with their final simplex) to a list that the EPA
+//    kernel consumes with one 64-lane wavefront per pair.  The EPA polytope
+//    (256 vertices, 512 faces; ~26 KB, too large for one lane's registers)
+//    lives in LDS, and the face loops of epa.rs — closest face, visibility,
+//    horizon search, compaction, stitching — are split across the lanes with
+//    ballot / shuffle prefix sums that keep the reference's sequential
+//    orders (first minimal face, horizon edges in face-then-edge order,
+//    kept faces in index order).
+#include <hip/hip_runtime.h>
+
+#include "narrow.hpp"
+#include "prim.hpp"
+
+namespace dio {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kMaxV = 256;
+constexpr int kMaxF = 512;
+constexpr int kFacesPerLane = kMaxF / kWave;  // 8
+
+inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+__device__ __forceinline__ ShapeRef load_shape(const BodyView& bv, uint32_t i) {
+    ShapeRef s;
+    s.pos = xyz(bv.pos[i]);
+    s.rot = as_q(bv.rot[i]);
+    s.tag = bv.tag[i];
+    s.c[0] = bv.corners[4 * i];
+    s.c[1] = bv.corners[4 * i + 1];
+    s.c[2] = bv.corners[4 * i + 2];
+    s.c[3] = bv.corners[4 * i + 3];
+    return s;
+}
+
+// ------------------------------------------------------------------ GJK
+// gjk.rs:67-73
+__device__ __forceinline__ void support_witness(const ShapeRef& a, const ShapeRef& b, f3 d, const HullPool& hp,
+                                                f3& point, f3& witness) {
+    const f3 ap = add(qrot(a.rot, support_local(a, qrot_inv(a.rot, d), hp)), a.pos);
+    const f3 bp = add(qrot(b.rot, support_local(b, qrot_inv(b.rot, neg(d)), hp)), b.pos);
+    point = sub(ap, bp);
+    witness = ap;
+}
+
+struct Simplex {
+    f3 p[4], w[4];
+    uint32_t len;
+    __device__ void push_front(f3 pt, f3 wt) {
+        for (uint32_t i = len; i > 0; --i)
+            if (i < 4) { p[i] = p[i - 1]; w[i] = w[i - 1]; }
+        p[0] = pt;
+        w[0] = wt;
+        len = len + 1 < 4 ? len + 1 : 4;
+    }
+    __device__ void remove(uint32_t index) {
+        for (uint32_t i = index; i + 1 < len; ++i) { p[i] = p[i + 1]; w[i] = w[i + 1]; }
+        len -= 1;
+    }
+    __device__ void swap12() {
+        f3 t = p[1]; p[1] = p[2]; p[2] = t;
+        t = w[1]; w[1] = w[2]; w[2] = t;
+    }
+};
+
+__device__ __forceinline__ bool same_dir(f3 a, f3 b) { return dot3(a, b) > 0.0f; }
+__device__ __forceinline__ f3 triple(f3 a, f3 b, f3 c) { return cross3(cross3(a, b), c); }
+// gjk.rs:214-223
+__device__ __forceinline__ f3 any_perpendicular(f3 v) {
+    f3 axis;
+    if (fabsf(v.x) <= fabsf(v.y) && fabsf(v.x) <= fabsf(v.z)) axis = f3{1.0f, 0.0f, 0.0f};
+    else if (fabsf(v.y) <= fabsf(v.z)) axis = f3{0.0f, 1.0f, 0.0f};
+    else axis = f3{0.0f, 0.0f, 1.0f};
+    return normalize_or_zero3(cross3(axis, v));
+}
+
+// gjk.rs:112-201
+__device__ bool contains_origin(Simplex& s, f3& dir) {
+    if (s.len == 4) {
+        const f3 a = s.p[0], b = s.p[1], c = s.p[2], d = s.p[3];
+        const f3 ao = neg(a), ab = sub(b, a), ac = sub(c, a), ad = sub(d, a);
+        const f3 abc = cross3(ab, ac), acd = cross3(ac, ad), adb = cross3(ad, ab);
+        if (same_dir(abc, ao)) s.remove(3);
+        else if (same_dir(acd, ao)) s.remove(1);
+        else if (same_dir(adb, ao)) s.remove(2);
+        else return true;
+    }
+    if (s.len == 1) {
+        dir = neg(s.p[0]);
+        return false;
+    }
+    if (s.len == 2) {
+        const f3 a = s.p[0], b = s.p[1];
+        const f3 ab = sub(b, a), ao = neg(a);
+        if (same_dir(ab, ao)) {
+            f3 d = triple(ab, ao, ab);
+            if (dot3(d, d) < 1e-12f) d = any_perpendicular(ab);
+            dir = d;
+        } else {
+            s.remove(1);
+            dir = ao;
+        }
+        return false;
+    }
+    if (s.len == 3) {
+        const f3 a = s.p[0], b = s.p[1], c = s.p[2];
+        const f3 ab = sub(b, a), ac = sub(c, a), ao = neg(a);
+        const f3 abc = cross3(ab, ac);
+        if (same_dir(cross3(abc, ac), ao)) {
+            if (same_dir(ac, ao)) { s.remove(1); dir = triple(ac, ao, ac); }
+            else if (same_dir(ab, ao)) { s.remove(2); dir = triple(ab, ao, ab); }
+            else { s.len = 1; dir = ao; }
+        } else if (same_dir(cross3(ab, abc), ao)) {
+            if (same_dir(ab, ao)) { s.remove(2); dir = triple(ab, ao, ab); }
+            else { s.len = 1; dir = ao; }
+        } else {
+            if (dot3(abc, ao) > 0.0f) dir = abc;
+            else { s.swap12(); dir = neg(abc); }
+        }
+        return false;
+    }
+    return false;
+}
+
+// gjk.rs:77-109
+__device__ bool gjk(const ShapeRef& a, const ShapeRef& b, const HullPool& hp, Simplex& s) {
+    f3 dir = sub(a.pos, b.pos);
+    if (eq3(dir, f3{0.0f, 0.0f, 0.0f})) dir = f3{1.0f, 0.0f, 0.0f};
+    s.len = 0;
+    f3 pt, wt;
+    support_witness(a, b, dir, hp, pt, wt);
+    s.push_front(pt, wt);
+    dir = neg(s.p[0]);
+    for (int it = 0; it < 64; ++it) {
+        support_witness(a, b, dir, hp, pt, wt);
+        if (dot3(pt, dir) < 0.0f) return false;
+        s.push_front(pt, wt);
+        if (contains_origin(s, dir)) return true;
+        if (dot3(dir, dir) < 1e-12f) dir = any_perpendicular(s.p[0]);
+    }
+    return false;
+}
+
+__device__ __forceinline__ void write_slot(const NarrowOut& o, uint32_t slot, f3 n, float depth, f3 pa, f3 pb) {
+    o.n[slot] = f4(n, depth);
+    o.pa[slot] = f4(pa, 0.0f);
+    o.pb[slot] = f4(pb, 0.0f);
+}
+
+// One lane per pair: SAT, sphere, or GJK (intersecting pairs go to EPA).
+__global__ void __launch_bounds__(kBlock) k_narrow(BodyView bv, NarrowOut o, float4* simplex_store) {
+    const uint32_t np = *o.p_dev;
+    const HullPool hp{bv.hull};
+    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < np; p += gridDim.x * kBlock) {
+        const uint2 pr = o.pairs[p];
+        const ShapeRef a = load_shape(bv, pr.x);
+        const ShapeRef b = load_shape(bv, pr.y);
+        uint32_t count = 0;
+        const bool boxa = a.tag == kCube || a.tag == kBox, boxb = b.tag == kCube || b.tag == kBox;
+        if (boxa && boxb) {
+            Manifold m;
+            if (box_manifold(a, b, m)) {
+                for (uint32_t k = 0; k < m.count; ++k) write_slot(o, p * 4 + k, m.normal, m.depth, m.pa[k], m.pb[k]);
+                count = m.count;
+            }
+        } else if (a.tag == kSphere && b.tag == kSphere) {
+            const f3 delta = sub(b.pos, a.pos);
+            const float distance = len3(delta);
+            const float radius = a.c[0].x + b.c[0].x;
+            if (!(distance >= radius || distance <= 1.1920929e-7f)) {
+                const f3 n = divs(delta, distance);
+                write_slot(o, p * 4, n, radius - distance, add(a.pos, scale(n, a.c[0].x)),
+                           sub(b.pos, scale(n, b.c[0].x)));
+                count = 1;
+            }
+        } else {
+            Simplex s;
+            if (gjk(a, b, hp, s)) {
+                const uint32_t q = atomicAdd(o.gjk_count, 1u);
+                o.gjk_list[q] = p;
+                for (int k = 0; k < 4; ++k) {
+                    simplex_store[8 * (size_t)p + k] = f4(s.p[k], 0.0f);
+                    simplex_store[8 * (size_t)p + 4 + k] = f4(s.w[k], 0.0f);
+                }
+            }
+        }
+        o.cnt[p] = count;
+    }
+}
+
+// ------------------------------------------------------------------ EPA
+struct EpaLds {
+    float vx[kMaxV], vy[kMaxV], vz[kMaxV];
+    float wx[kMaxV], wy[kMaxV], wz[kMaxV];
+    uint32_t fa[kMaxF], fb[kMaxF], fc[kMaxF];
+    float fnx[kMaxF], fny[kMaxF], fnz[kMaxF], fd[kMaxF];
+    uint32_t vis[kMaxF];
+    uint32_t ha[kMaxF], hb[kMaxF];
+    uint32_t count;  // broadcast word
+};
+
+__device__ __forceinline__ f3 vtx(const EpaLds& L, uint32_t i) { return f3{L.vx[i], L.vy[i], L.vz[i]}; }
+__device__ __forceinline__ f3 wit(const EpaLds& L, uint32_t i) { return f3{L.wx[i], L.wy[i], L.wz[i]}; }
+
+struct FaceR {
+    uint32_t a, b, c;
+    f3 n;
+    float d;
+    bool ok;
+};
+
+// Polytope::add_face (epa.rs:58-85) without the store: the face and whether
+// it is non-degenerate
+__device__ __forceinline__ FaceR make_face(const EpaLds& L, uint32_t a, uint32_t b, uint32_t c) {
+    FaceR f;
+    const f3 va = vtx(L, a);
+    f3 n = cross3(sub(vtx(L, b), va), sub(vtx(L, c), va));
+    f.ok = dot3(n, n) >= 1e-12f;
+    f.a = a; f.b = b; f.c = c;
+    if (!f.ok) return f;
+    n = normalize3(n);
+    f.n = n;
+    f.d = dot3(n, va);
+    if (f.d < 0.0f) {
+        f.a = c;
+        f.c = a;
+        f.n = neg(n);
+        f.d = -f.d;
+    }
+    return f;
+}
+__device__ __forceinline__ void store_face(EpaLds& L, uint32_t slot, const FaceR& f) {
+    L.fa[slot] = f.a; L.fb[slot] = f.b; L.fc[slot] = f.c;
+    L.fnx[slot] = f.n.x; L.fny[slot] = f.n.y; L.fnz[slot] = f.n.z; L.fd[slot] = f.d;
+}
+
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t& total) {
+    const int lane = threadIdx.x & (kWave - 1);
+    uint32_t inc = x;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, kWave);
+        if (lane >= off) inc += y;
+    }
+    total = __shfl(inc, kWave - 1, kWave);
+    return inc - x;
+}
+
+__device__ __forceinline__ bool edge_in(uint32_t ea, uint32_t eb, uint32_t fa, uint32_t fb, uint32_t fc) {
+    return (fa == ea && fb == eb) || (fb == ea && fc == eb) || (fc == ea && fa == eb) || (fa == eb && fb == ea) ||
+           (fb == eb && fc == ea) || (fc == eb && fa == ea);
+}
+
+// epa.rs:193-240
+__device__ void face_contact(const EpaLds& L, uint32_t fi, f3& normal, float& depth, f3& pa, f3& pb) {
+    const uint32_t ia = L.fa[fi], ib = L.fb[fi], ic = L.fc[fi];
+    const f3 fn{L.fnx[fi], L.fny[fi], L.fnz[fi]};
+    const float fdist = L.fd[fi];
+    const f3 va = vtx(L, ia), vb = vtx(L, ib), vc = vtx(L, ic);
+    const f3 closest = scale(fn, fdist);
+    const f3 v0 = sub(vb, va), v1 = sub(vc, va), v2 = sub(closest, va);
+    const float d00 = dot3(v0, v0), d01 = dot3(v0, v1), d11 = dot3(v1, v1), d20 = dot3(v2, v0), d21 = dot3(v2, v1);
+    const float denom = d00 * d11 - d01 * d01;
+    float w1, w2, w3;
+    if (fabsf(denom) > 1e-12f) {
+        w2 = (d11 * d20 - d01 * d21) / denom;
+        w3 = (d00 * d21 - d01 * d20) / denom;
+        w1 = 1.0f - w2 - w3;
+    } else {
+        w1 = 1.0f; w2 = 0.0f; w3 = 0.0f;
+    }
+    w1 = rclamp(w1, 0.0f, 1.0f);
+    w2 = rclamp(w2, 0.0f, 1.0f);
+    w3 = rclamp(w3, 0.0f, 1.0f);
+    const float sum = w1 + w2 + w3;
+    if (sum > 1e-12f) { w1 = w1 / sum; w2 = w2 / sum; w3 = w3 / sum; }
+    else { w1 = 1.0f; w2 = 0.0f; w3 = 0.0f; }
+    const f3 pw = add(add(scale(wit(L, ia), w1), scale(wit(L, ib), w2)), scale(wit(L, ic), w3));
+    const f3 mk = add(add(scale(va, w1), scale(vb, w2)), scale(vc, w3));
+    normal = fn;
+    depth = fdist;
+    pa = pw;
+    pb = sub(pw, mk);
+}
+
+// One 64-lane workgroup per GJK-intersecting pair (epa.rs:89-189).
+__global__ void __launch_bounds__(kWave) k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
+    __shared__ EpaLds L;
+    const int lane = threadIdx.x;
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
+    const HullPool hp{bv.hull};
+    const uint32_t jobs = *o.gjk_count;
+    for (uint32_t job = blockIdx.x; job < jobs; job += gridDim.x) {
+        const uint32_t p = o.gjk_list[job];
+        const uint2 pr = o.pairs[p];
+        const ShapeRef a = load_shape(bv, pr.x);
+        const ShapeRef b = load_shape(bv, pr.y);
+        if (lane < 4) {
+            const float4 sp = simplex_store[8 * (size_t)p + lane];
+            const float4 sw = simplex_store[8 * (size_t)p + 4 + lane];
+            L.vx[lane] = sp.x; L.vy[lane] = sp.y; L.vz[lane] = sp.z;
+            L.wx[lane] = sw.x; L.wy[lane] = sw.y; L.wz[lane] = sw.z;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            // Polytope::new zeroes the faces; then the tetrahedron's four faces
+            L.fa[0] = 0; L.fb[0] = 0; L.fc[0] = 0;
+            L.fnx[0] = 0.0f; L.fny[0] = 0.0f; L.fnz[0] = 0.0f; L.fd[0] = 0.0f;
+            uint32_t fc = 0;
+            const uint32_t tri[4][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
+            for (int t = 0; t < 4; ++t) {
+                const FaceR f = make_face(L, tri[t][0], tri[t][1], tri[t][2]);
+                if (f.ok) store_face(L, fc++, f);
+            }
+            L.count = fc;
+        }
+        __syncthreads();
+        uint32_t face_count = L.count;
+        uint32_t vertex_count = 4;
+        int result_face = -1;
+        for (int it = 0; it < 64; ++it) {
+            // closest face: first minimal distance in index order (strict <)
+            float md = 0.0f;
+            uint32_t mi = 0;
+            bool have = false;
+            for (int k = 0; k < kFacesPerLane; ++k) {
+                const uint32_t f = (uint32_t)(k * kWave + lane);
+                if (f < face_count) {
+                    const float d = L.fd[f];
+                    if (!have) {
+                        if (d == d) { md = d; mi = f; have = true; }
+                    } else if (d < md) {
+                        md = d; mi = f;
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const float od = __shfl_xor(md, off, kWave);
+                const uint32_t oi = __shfl_xor(mi, off, kWave);
+                const bool oh = __shfl_xor((int)have, off, kWave) != 0;
+                if (oh && (!have || od < md || (od == md && oi < mi))) { md = od; mi = oi; have = true; }
+            }
+            // the sequential scan starts from face 0: a NaN there is never replaced
+            if (face_count == 0 || !(L.fd[0] == L.fd[0])) mi = 0;
+            const f3 mn{L.fnx[mi], L.fny[mi], L.fnz[mi]};
+            const float mdist = L.fd[mi];
+            f3 sp, sw;
+            support_witness(a, b, mn, hp, sp, sw);
+            const float sd = dot3(mn, sp);
+            if (sd - mdist < 1e-4f) { result_face = (int)mi; break; }
+            const uint32_t ni = vertex_count;
+            if (ni >= (uint32_t)kMaxV) { result_face = (int)mi; break; }
+            if (lane == 0) {
+                L.vx[ni] = sp.x; L.vy[ni] = sp.y; L.vz[ni] = sp.z;
+                L.wx[ni] = sw.x; L.wy[ni] = sw.y; L.wz[ni] = sw.z;
+            }
+            vertex_count += 1;
+            // visibility of each face from the new vertex
+            for (int k = 0; k < kFacesPerLane; ++k) {
+                const uint32_t f = (uint32_t)(k * kWave + lane);
+                if (f < face_count) {
+                    const f3 fn{L.fnx[f], L.fny[f], L.fnz[f]};
+                    L.vis[f] = dot3(fn, sub(sp, vtx(L, L.fa[f]))) > 0.0f ? 1u : 0u;
+                }
+            }
+            __syncthreads();
+            // horizon: edges of visible faces that no other visible face has,
+            // in (face, edge) order, at most kMaxF of them
+            uint32_t hbase = 0;
+            for (int k = 0; k < kFacesPerLane; ++k) {
+                const uint32_t f = (uint32_t)(k * kWave + lane);
+                uint32_t e[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+                uint32_t hmask = 0;
+                if (f < face_count && L.vis[f]) {
+                    const uint32_t fa = L.fa[f], fb = L.fb[f], fcc = L.fc[f];
+                    e[0][0] = fa; e[0][1] = fb; e[1][0] = fb; e[1][1] = fcc; e[2][0] = fcc; e[2][1] = fa;
+                    for (int q = 0; q < 3; ++q) {
+                        bool shared = false;
+                        for (uint32_t g = 0; g < face_count && !shared; ++g)
+                            if (g != f && L.vis[g] && edge_in(e[q][0], e[q][1], L.fa[g], L.fb[g], L.fc[g]))
+                                shared = true;
+                        if (!shared) hmask |= 1u << q;
+                    }
+                }
+                uint32_t tot;
+                const uint32_t pre = wave_prefix((uint32_t)__popc(hmask), tot);
+                uint32_t pos = hbase + pre;
+                for (int q = 0; q < 3; ++q) {
+                    if (hmask & (1u << q)) {
+                        if (pos < (uint32_t)kMaxF) { L.ha[pos] = e[q][0]; L.hb[pos] = e[q][1]; }
+                        ++pos;
+                    }
+                }
+                hbase += tot;
+            }
+            const uint32_t hcount = hbase < (uint32_t)kMaxF ? hbase : (uint32_t)kMaxF;
+            // compact the kept (non-visible) faces in index order
+            FaceR keep[kFacesPerLane];
+            uint32_t kflag = 0;
+            for (int k = 0; k < kFacesPerLane; ++k) {
+                const uint32_t f = (uint32_t)(k * kWave + lane);
+                if (f < face_count && !L.vis[f]) {
+                    kflag |= 1u << k;
+                    keep[k].a = L.fa[f]; keep[k].b = L.fb[f]; keep[k].c = L.fc[f];
+                    keep[k].n = f3{L.fnx[f], L.fny[f], L.fnz[f]};
+                    keep[k].d = L.fd[f];
+                }
+            }
+            __syncthreads();
+            uint32_t wbase = 0;
+            for (int k = 0; k < kFacesPerLane; ++k) {
+                const bool kf = (kflag >> k) & 1u;
+                const uint64_t bal = __ballot(kf);
+                if (kf) store_face(L, wbase + (uint32_t)__popcll(bal & lt), keep[k]);
+                wbase += (uint32_t)__popcll(bal);
+            }
+            face_count = wbase;
+            __syncthreads();
+            // stitch the horizon to the new vertex (epa.rs:182-185)
+            for (uint32_t hb0 = 0; hb0 < hcount; hb0 += kWave) {
+                const uint32_t h = hb0 + lane;
+                FaceR f;
+                f.ok = false;
+                if (h < hcount) f = make_face(L, ni, L.ha[h], L.hb[h]);
+                const uint64_t bal = __ballot(f.ok);
+                const uint32_t slot = face_count + (uint32_t)__popcll(bal & lt);
+                if (f.ok && slot < (uint32_t)kMaxF) store_face(L, slot, f);
+                const uint32_t added = face_count + (uint32_t)__popcll(bal);
+                face_count = added < (uint32_t)kMaxF ? added : (uint32_t)kMaxF;
+            }
+            __syncthreads();
+        }
+        if (result_face >= 0 && lane == 0) {
+            f3 n, pa, pb;
+            float depth;
+            face_contact(L, (uint32_t)result_face, n, depth, pa, pb);
+            write_slot(o, p * 4, n, depth, pa, pb);
+            o.cnt[p] = 1;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pack_contact_out(NarrowOut o, uint32_t np, dio_gpu_contact_out* rec) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= np * 4) return;
+    const uint32_t p = s / 4, k = s % 4;
+    dio_gpu_contact_out r;
+    r.pad = 0;
+    r.pad2[0] = r.pad2[1] = r.pad2[2] = 0.0f;
+    if (k < o.cnt[p]) {
+        const uint2 pr = o.pairs[p];
+        const float4 n = o.n[s], pa = o.pa[s], pb = o.pb[s];
+        r.valid = 1; r.a = pr.x; r.b = pr.y;
+        r.normal[0] = n.x; r.normal[1] = n.y; r.normal[2] = n.z; r.normal[3] = 0.0f;
+        r.depth = n.w;
+        r.point_a[0] = pa.x; r.point_a[1] = pa.y; r.point_a[2] = pa.z; r.point_a[3] = 0.0f;
+        r.point_b[0] = pb.x; r.point_b[1] = pb.y; r.point_b[2] = pb.z; r.point_b[3] = 0.0f;
+    } else {
+        r.valid = 0; r.a = 0; r.b = 0;
+        for (int q = 0; q < 4; ++q) { r.normal[q] = 0.0f; r.point_a[q] = 0.0f; r.point_b[q] = 0.0f; }
+        r.depth = 0.0f;
+    }
+    rec[s] = r;
+}
+
+}  // namespace
+
+void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, hipStream_t s) {
+    launch_fill_u32(o.gjk_count, 0u, 1, s);
+    uint32_t blocks = blocks_for(o.p_max > 0 ? o.p_max : 1);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_narrow, dim3(blocks), dim3(kBlock), 0, s, bv, o, simplex_store);
+    uint32_t eblocks = o.p_max > 0 ? o.p_max : 1;
+    if (eblocks > 2048) eblocks = 2048;
+    hipLaunchKernelGGL(k_epa, dim3(eblocks), dim3(kWave), 0, s, bv, o, (const float4*)simplex_store);
+    launch_scan_exclusive(o.cnt, o.off, o.p_max, o.p_dev, o.scan_partials, s);
+}
+
+void launch_pack_contact_out(const NarrowOut& o, uint32_t np, dio_gpu_contact_out* rec, hipStream_t s) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_pack_contact_out, dim3(blocks_for(np * 4)), dim3(kBlock), 0, s, o, np, rec);
+}
+
+}  // namespace dio

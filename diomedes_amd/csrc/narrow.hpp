@@ -1,0 +1,36 @@
+// narrow.hpp — narrow phase buffers and launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "broad.hpp"
+
+namespace dio {
+
+// Narrow-phase output for a pair list: 4 slots per pair (slot pair*4 + k is
+// valid for k < cnt[pair]), the layout of the reference's contact buffer
+// (narrowphase/src/lib.rs:63-69) as SoA.
+struct NarrowOut {
+    float4* n;        // normal.xyz, depth in .w
+    float4* pa;       // point on a
+    float4* pb;       // point on b
+    uint32_t* cnt;    // contacts per pair
+    uint32_t* off;    // exclusive scan of cnt (p_max + 1): contact index of each pair
+    uint32_t* scan_partials;
+    uint32_t* gjk_list;   // pairs routed to GJK + EPA
+    uint32_t* gjk_count;
+    const uint2* pairs;   // the pair list these slots belong to
+    const uint32_t* p_dev;
+    uint32_t p_max;
+};
+
+// Narrow phase over pairs [0, *p_dev) and the scan of the per-pair counts;
+// the contact total is left in out.off[*p_dev].
+// simplex_store holds 8 float4 per pair (the GJK simplex handed to EPA).
+void run_narrow(const BodyView& bv, const NarrowOut& out, float4* simplex_store, hipStream_t s);
+
+// Pack slots into the reference's GpuContactOut records (compute.rs:123-133).
+void launch_pack_contact_out(const NarrowOut& out, uint32_t pair_count, dio_gpu_contact_out* rec, hipStream_t s);
+
+}  // namespace dio

@@ -1,0 +1,309 @@
+// shapes.hpp — shape support functions and the box-box SAT manifold.
+//
+// Counterparts of engine/gpu_physics/narrowphase/src/{support,cuboid}.rs and
+// engine/gpu_physics/broadphase/src/lib.rs:79-151, evaluated in the same
+// float operation order.  BOX (per-axis half extents) and HULL (vertex pool)
+// extend the reference's three shapes; a CUBE keeps the reference's scalar
+// arithmetic exactly (SAT and walls read only its first half extent).
+#pragma once
+
+#include "dio_math.hpp"
+
+namespace dio {
+
+// A body as the collision code sees it: pose + shape.
+struct ShapeRef {
+    f3 pos;
+    q4 rot;
+    uint32_t tag;
+    float4 c[4];  // GpuShape corners
+};
+
+struct HullPool {
+    const float4* v;  // xyz per vertex
+};
+
+// narrowphase/src/support.rs:21-61: tetrahedra and hulls keep the first
+// maximal vertex
+DIO_HD f3 support_local(const ShapeRef& s, f3 d, const HullPool& hp) {
+    switch (s.tag) {
+        case kSphere:
+            return scale(normalize_or_zero3(d), s.c[0].x);
+        case kCube:
+        case kBox:
+            return f3{d.x >= 0.0f ? s.c[0].x : -s.c[0].x, d.y >= 0.0f ? s.c[0].y : -s.c[0].y,
+                      d.z >= 0.0f ? s.c[0].z : -s.c[0].z};
+        case kTetrahedron: {
+            f3 best = xyz(s.c[0]);
+            float bd = dot3(best, d);
+            for (int i = 1; i < 4; ++i) {
+                const f3 cc = xyz(s.c[i]);
+                const float dd = dot3(cc, d);
+                if (dd > bd) { bd = dd; best = cc; }
+            }
+            return best;
+        }
+        case kHull: {
+            const uint32_t first = (uint32_t)s.c[0].x, count = (uint32_t)s.c[0].y;
+            f3 best = xyz(hp.v[first]);
+            float bd = dot3(best, d);
+            for (uint32_t i = 1; i < count; ++i) {
+                const f3 cc = xyz(hp.v[first + i]);
+                const float dd = dot3(cc, d);
+                if (dd > bd) { bd = dd; best = cc; }
+            }
+            return best;
+        }
+        default:
+            return f3{0.0f, 0.0f, 0.0f};
+    }
+}
+
+// physics/shape.rs:20-48 (host Shape::support used by the container walls):
+// the tetrahedron keeps the LAST maximal corner under f32::total_cmp.
+DIO_HD int32_t total_key(float v) {
+    int32_t i = __builtin_bit_cast(int32_t, v);
+    return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1);
+}
+DIO_HD f3 support_host(const ShapeRef& s, f3 d, const HullPool& hp) {
+    if (s.tag == kCube) {
+        const float h = s.c[0].x;
+        return f3{d.x >= 0.0f ? h : -h, d.y >= 0.0f ? h : -h, d.z >= 0.0f ? h : -h};
+    }
+    if (s.tag == kTetrahedron) {
+        int best = 0;
+        int32_t bk = total_key(dot3(xyz(s.c[0]), d));
+        for (int i = 1; i < 4; ++i) {
+            const int32_t k = total_key(dot3(xyz(s.c[i]), d));
+            if (k >= bk) { bk = k; best = i; }
+        }
+        return xyz(s.c[best]);
+    }
+    return support_local(s, d, hp);
+}
+
+// broadphase/src/lib.rs:116-119
+DIO_HD f3 world_support(const ShapeRef& s, f3 d, const HullPool& hp) {
+    const f3 local = qrot_inv(s.rot, d);
+    return add(qrot(s.rot, support_local(s, local, hp)), s.pos);
+}
+
+// broadphase/src/lib.rs:123-151
+DIO_HD void shape_aabb(const ShapeRef& s, const HullPool& hp, f3& mn, f3& mx) {
+    mn = s.pos;
+    mx = s.pos;
+    const f3 sx = world_support(s, f3{1.0f, 0.0f, 0.0f}, hp);
+    const f3 snx = world_support(s, neg(f3{1.0f, 0.0f, 0.0f}), hp);
+    const f3 sy = world_support(s, f3{0.0f, 1.0f, 0.0f}, hp);
+    const f3 sny = world_support(s, neg(f3{0.0f, 1.0f, 0.0f}), hp);
+    const f3 sz = world_support(s, f3{0.0f, 0.0f, 1.0f}, hp);
+    const f3 snz = world_support(s, neg(f3{0.0f, 0.0f, 1.0f}), hp);
+    if (sx.x > mx.x) mx.x = sx.x;
+    if (snx.x < mn.x) mn.x = snx.x;
+    if (sy.y > mx.y) mx.y = sy.y;
+    if (sny.y < mn.y) mn.y = sny.y;
+    if (sz.z > mx.z) mx.z = sz.z;
+    if (snz.z < mn.z) mn.z = snz.z;
+}
+
+// broadphase/src/lib.rs:195-201 (closed intervals)
+DIO_HD bool aabb_overlap(float4 mna, float4 mxa, float4 mnb, float4 mxb) {
+    return mna.x <= mxb.x && mnb.x <= mxa.x && mna.y <= mxb.y && mnb.y <= mxa.y && mna.z <= mxb.z &&
+           mnb.z <= mxa.z;
+}
+
+// ------------------------------------------------------------ box SAT
+DIO_HD void box_axes(q4 r, f3 ax[3]) {
+    ax[0] = qrot(r, f3{1.0f, 0.0f, 0.0f});
+    ax[1] = qrot(r, f3{0.0f, 1.0f, 0.0f});
+    ax[2] = qrot(r, f3{0.0f, 0.0f, 1.0f});
+}
+DIO_HD float box_half(const ShapeRef& s, uint32_t axis) {
+    return s.tag == kCube ? s.c[0].x : (axis == 0 ? s.c[0].x : (axis == 1 ? s.c[0].y : s.c[0].z));
+}
+// cuboid.rs:37-40
+DIO_HD float box_radius(const ShapeRef& s, const f3 ax[3], f3 axis) {
+    if (s.tag == kCube)
+        return s.c[0].x * (fabsf(dot3(axis, ax[0])) + fabsf(dot3(axis, ax[1])) + fabsf(dot3(axis, ax[2])));
+    return s.c[0].x * fabsf(dot3(axis, ax[0])) + s.c[0].y * fabsf(dot3(axis, ax[1])) +
+           s.c[0].z * fabsf(dot3(axis, ax[2]));
+}
+
+struct SatBest {
+    float overlap;
+    f3 normal;
+    uint32_t kind, index;
+};
+
+// cuboid.rs:19-52
+DIO_HD bool sat_axis(f3 raw, uint32_t kind, uint32_t index, const ShapeRef& a, const f3 aa[3], const ShapeRef& b,
+                     const f3 ba[3], f3 delta, SatBest& best) {
+    if (dot3(raw, raw) <= 1.0e-8f) return true;
+    const f3 axis = normalize3(raw);
+    const float ra = box_radius(a, aa, axis);
+    const float rb = box_radius(b, ba, axis);
+    const float overlap = ra + rb - fabsf(dot3(delta, axis));
+    if (overlap < 0.0f) return false;
+    if (overlap < best.overlap) {
+        best.overlap = overlap;
+        best.normal = dot3(delta, axis) >= 0.0f ? axis : neg(axis);
+        best.kind = kind;
+        best.index = index;
+    }
+    return true;
+}
+
+// cuboid.rs:54-167
+DIO_HD bool box_sat(const ShapeRef& a, const ShapeRef& b, SatBest& best) {
+    f3 aa[3], ba[3];
+    box_axes(a.rot, aa);
+    box_axes(b.rot, ba);
+    const f3 delta = sub(b.pos, a.pos);
+    best.overlap = 3.40282347e+38f;
+    best.normal = f3{1.0f, 0.0f, 0.0f};
+    best.kind = 0;
+    best.index = 0;
+    for (uint32_t i = 0; i < 3; ++i)
+        if (!sat_axis(aa[i], 0, i, a, aa, b, ba, delta, best)) return false;
+    for (uint32_t i = 0; i < 3; ++i)
+        if (!sat_axis(ba[i], 1, i, a, aa, b, ba, delta, best)) return false;
+    for (uint32_t i = 0; i < 3; ++i)
+        for (uint32_t j = 0; j < 3; ++j)
+            if (!sat_axis(cross3(aa[i], ba[j]), 2, i * 3 + j, a, aa, b, ba, delta, best)) return false;
+    return true;
+}
+
+// cuboid.rs:169-196 (vertices in the reference's order)
+DIO_HD void box_face(const ShapeRef& s, uint32_t ai, f3 outward, f3 verts[4], f3& center, f3& u, f3& v, float& eu,
+                     float& ev) {
+    f3 ax[3];
+    box_axes(s.rot, ax);
+    const f3 axis = ax[ai];
+    const float hn = box_half(s, ai);
+    const f3 fnrm = scale(axis, dot3(axis, outward) >= 0.0f ? 1.0f : -1.0f);
+    center = add(s.pos, scale(fnrm, hn));
+    const uint32_t iu = ai == 0 ? 1u : 0u;
+    const uint32_t iv = ai == 2 ? 1u : 2u;
+    u = ax[iu];
+    v = ax[iv];
+    eu = box_half(s, iu);
+    ev = box_half(s, iv);
+    verts[0] = add(add(center, scale(u, eu)), scale(v, ev));
+    verts[1] = add(sub(center, scale(u, eu)), scale(v, ev));
+    verts[2] = sub(sub(center, scale(u, eu)), scale(v, ev));
+    verts[3] = sub(add(center, scale(u, eu)), scale(v, ev));
+}
+
+// cuboid.rs:198-214
+DIO_HD uint32_t box_incident_axis(const ShapeRef& s, f3 n) {
+    f3 ax[3];
+    box_axes(s.rot, ax);
+    uint32_t best = 0;
+    float bd = fabsf(dot3(ax[0], n));
+    for (uint32_t i = 1; i < 3; ++i) {
+        const float d = fabsf(dot3(ax[i], n));
+        if (d > bd) { bd = d; best = i; }
+    }
+    return best;
+}
+
+// cuboid.rs:216-250
+DIO_HD uint32_t clip_plane(const f3* in, uint32_t in_count, f3 n, float offset, f3* out) {
+    if (in_count == 0) return 0;
+    uint32_t count = 0;
+    f3 prev = in[in_count - 1];
+    float pd = dot3(n, prev) - offset;
+    for (uint32_t i = 0; i < in_count; ++i) {
+        const f3 cur = in[i];
+        const float cd = dot3(n, cur) - offset;
+        const bool pin = pd <= 1.0e-5f;
+        const bool cin = cd <= 1.0e-5f;
+        if (pin != cin) {
+            const float den = pd - cd;
+            if (fabsf(den) > 1.0e-8f && count < 8) {
+                const float t = rclamp(pd / den, 0.0f, 1.0f);
+                out[count++] = add(prev, scale(sub(cur, prev), t));
+            }
+        }
+        if (cin && count < 8) out[count++] = cur;
+        prev = cur;
+        pd = cd;
+    }
+    return count;
+}
+
+struct Manifold {
+    f3 normal;
+    float depth;
+    f3 pa[4], pb[4];
+    uint32_t count;
+};
+
+// cuboid.rs:252-272
+DIO_HD void centered_contact(const ShapeRef& a, const ShapeRef& b, f3 n, float depth, Manifold& m) {
+    f3 aa[3], ba[3];
+    box_axes(a.rot, aa);
+    box_axes(b.rot, ba);
+    const float ra = box_radius(a, aa, n);
+    const float rb = box_radius(b, ba, n);
+    m.pa[0] = add(a.pos, scale(n, ra));
+    m.pb[0] = sub(b.pos, scale(n, rb));
+    m.normal = n;
+    m.depth = depth;
+    m.count = 1;
+}
+
+// cuboid.rs:276-377
+DIO_HD bool box_manifold(const ShapeRef& a, const ShapeRef& b, Manifold& m) {
+    SatBest best;
+    if (!box_sat(a, b, best)) return false;
+    const f3 normal = best.normal;
+    const float depth = best.overlap;
+    if (best.kind == 2) {
+        centered_contact(a, b, normal, depth, m);
+        return true;
+    }
+    const bool ref_is_a = best.kind == 0;
+    const f3 rn = ref_is_a ? normal : neg(normal);
+    const ShapeRef& rb = ref_is_a ? a : b;
+    const ShapeRef& ib = ref_is_a ? b : a;
+    const uint32_t inc = box_incident_axis(ib, rn);
+    f3 rface[4], center, u, v;
+    float eu, ev;
+    box_face(rb, best.index, rn, rface, center, u, v, eu, ev);
+    f3 pa[8], pb[8], iface[4], c2, u2, v2;
+    float eu2, ev2;
+    box_face(ib, inc, neg(rn), iface, c2, u2, v2, eu2, ev2);
+    for (int i = 0; i < 4; ++i) pa[i] = iface[i];
+    uint32_t count = 4;
+    count = clip_plane(pa, count, u, dot3(u, center) + eu, pb);
+    count = clip_plane(pb, count, neg(u), dot3(neg(u), center) + eu, pa);
+    count = clip_plane(pa, count, v, dot3(v, center) + ev, pb);
+    count = clip_plane(pb, count, neg(v), dot3(neg(v), center) + ev, pa);
+    if (count == 0) {
+        centered_contact(a, b, normal, depth, m);
+        return true;
+    }
+    const float rplane = dot3(rn, center);
+    uint32_t cc = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        if (cc >= 4) break;
+        const f3 ip = pa[i];
+        const float pd = rplane - dot3(rn, ip);
+        if (pd < -1.0e-5f) continue;
+        const f3 rp = add(ip, scale(rn, rmax(pd, 0.0f)));
+        m.pa[cc] = ref_is_a ? rp : ip;
+        m.pb[cc] = ref_is_a ? ip : rp;
+        ++cc;
+    }
+    if (cc == 0) {
+        centered_contact(a, b, normal, depth, m);
+        return true;
+    }
+    m.normal = normal;
+    m.depth = depth;
+    m.count = cc;
+    return true;
+}
+
+}  // namespace dio

@@ -1,0 +1,279 @@
+/*
+ * diomedes_hip.h — C ABI of the MI355X-native diomedes physics step.
+ *
+ * The step is the reference engine's broad phase, narrow phase, contact
+ * build (raw contacts, container walls, warm start, CSR) and AVBD solve
+ * (init, I x (primal, dual), recover).  Two families of entry points:
+ *
+ *  - granular calls that replace the reference's Vulkan compute objects
+ *    one for one (NarrowPhaseCompute and AvbdCompute behind the six
+ *    Renderer physics methods, engine/src/render/renderer.rs:75-175);
+ *  - a fused, device-resident world (dio_world_upload / dio_step /
+ *    dio_world_download) that runs the demo step protocol
+ *    (demos/avbd_box/src/main.rs:189-272) entirely on the GPU, replacing
+ *    AvbdSolver::prepare_contacts and the host decode/serialise loops.
+ *
+ * Records are byte-identical to the reference's #[repr(C)] structs so a Rust
+ * binding can pass its existing slices unchanged.  Every function returns a
+ * dio_status; no function aborts or throws across the ABI.  A dio_ctx is not
+ * thread-safe (it mirrors the reference's `&mut self` objects); separate
+ * contexts may be used from separate threads, each owns one HIP stream.
+ * Submit calls stage host inputs before returning, so callers may free them
+ * immediately; read calls block until the GPU work they read is complete.
+ */
+#ifndef DIOMEDES_HIP_H
+#define DIOMEDES_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DIO_ABI_VERSION 1u
+
+/* `b` of a contact against the container box (physics/gpu.rs:15, avbd/src/lib.rs:75) */
+#define DIO_CONTAINER 0xFFFFFFFFu
+
+typedef enum dio_status {
+    DIO_OK = 0,
+    DIO_CAPACITY = 1,  /* a buffer limit was exceeded; dio_required_capacity names the size */
+    DIO_INVALID = 2,   /* bad argument (null pointer, out-of-range index, zero iterations, ...) */
+    DIO_HIP = 3,       /* a HIP runtime call failed; dio_last_error has the message */
+    DIO_NOT_READY = 4  /* read before the matching submit */
+} dio_status;
+
+/* shape tags (compute.rs:37-39; BOX and HULL extend the reference) */
+enum {
+    DIO_SHAPE_SPHERE = 0,      /* corners[0][0] = radius */
+    DIO_SHAPE_CUBE = 1,        /* corners[0][0..3] = [h, h, h]; SAT and walls read only [0][0] */
+    DIO_SHAPE_TETRAHEDRON = 2, /* corners[0..4] = the four local corners */
+    DIO_SHAPE_BOX = 3,         /* corners[0][0..3] = per-axis half extents (extension) */
+    DIO_SHAPE_HULL = 4         /* corners[0][0] = first vertex, [0][1] = vertex count, as
+                                  floats holding exact integers, into the hull vertex pool
+                                  given to dio_world_upload_hulls (extension) */
+};
+
+/* constraint kinds (avbd/src/lib.rs:76-79) */
+enum {
+    DIO_CONSTRAINT_SPRING = 0,
+    DIO_CONSTRAINT_ROD = 1,
+    DIO_CONSTRAINT_BALL_SOCKET = 2,
+    DIO_CONSTRAINT_ROPE = 3
+};
+
+/* GpuShape, 80 bytes (compute.rs:28-34) */
+typedef struct dio_gpu_shape {
+    uint32_t tag;
+    uint32_t pad[3];
+    float corners[4][4];
+} dio_gpu_shape;
+
+/* GpuBody, 112 bytes (compute.rs:78-82) */
+typedef struct dio_gpu_body {
+    float position[4];
+    float orientation[4]; /* quaternion x, y, z, w */
+    dio_gpu_shape shape;
+} dio_gpu_body;
+
+/* GpuPair, 8 bytes (compute.rs:97-100) */
+typedef struct dio_gpu_pair {
+    uint32_t a;
+    uint32_t b;
+} dio_gpu_pair;
+
+/* GpuContactOut, 80 bytes (compute.rs:123-133) */
+typedef struct dio_gpu_contact_out {
+    uint32_t valid;
+    uint32_t a;
+    uint32_t b;
+    uint32_t pad;
+    float normal[4];
+    float depth;
+    float pad2[3];
+    float point_a[4];
+    float point_b[4];
+} dio_gpu_contact_out;
+
+/* GpuAvbdBodyState, 80 bytes (gpu_physics.rs:18-26) */
+typedef struct dio_avbd_body_state {
+    float vel[4];
+    float ang[4];
+    float prev_vel[4];
+    float inv_moment[4];
+    float inv_mass;
+    float friction;
+    float pad[2];
+} dio_avbd_body_state;
+
+/* GpuAvbdContact, 144 bytes (gpu_physics.rs:31-44) */
+typedef struct dio_avbd_contact {
+    uint32_t a;
+    uint32_t b;
+    float friction;
+    float pad;
+    float normal[4];
+    float tangent1[4];
+    float tangent2[4];
+    float r_a[4];
+    float r_b[4];
+    float c0[4];
+    float penalty[4];
+    float lambda[4];
+} dio_avbd_contact;
+
+/* GpuAvbdConstraint, 48 bytes (gpu_physics.rs:49-56): anchor_a[3] is the
+   rest / max length (0 for sockets), anchor_b[3] the stiffness */
+typedef struct dio_avbd_constraint {
+    uint32_t a;
+    uint32_t b;
+    uint32_t kind;
+    uint32_t pad;
+    float anchor_a[4];
+    float anchor_b[4];
+} dio_avbd_constraint;
+
+/* AvbdRunOptions (gpu_physics.rs:76-83) */
+typedef struct dio_run_options {
+    float dt;
+    float gravity[3];
+    float alpha;
+    float beta_lin;
+    float penalty_max;
+    uint32_t iterations;
+} dio_run_options;
+
+/* AvbdOptions (physics/gpu.rs:127-138); defaults in dio_default_options */
+typedef struct dio_avbd_options {
+    float dt;
+    float gravity[3];
+    uint32_t iterations;
+    float alpha;
+    float beta_lin;
+    float margin;
+    float penalty_min;
+    float penalty_max;
+    float gamma;
+    float friction;
+} dio_avbd_options;
+
+/* AvbdContainer (physics/gpu.rs:117-123) plus a presence flag (the
+   reference's Option<AvbdContainer>) */
+typedef struct dio_container {
+    float center[3];
+    float rotation[4]; /* x, y, z, w */
+    float half_extent;
+    uint32_t enabled;
+} dio_container;
+
+/* Buffer capacities; zero fields take defaults derived from max_bodies. */
+typedef struct dio_limits {
+    uint32_t max_bodies;
+    uint32_t max_pairs;
+    uint32_t max_contacts;
+    uint32_t max_constraints;
+    uint32_t max_hull_vertices;
+} dio_limits;
+
+/* Step protocol options for the fused path. */
+typedef struct dio_step_opts {
+    uint32_t contact_lag; /* 1: the reference demos' protocol (contacts found on the
+                             previous state, lever arms on the current one); 0: synchronous */
+    uint32_t mode;        /* 0: Jacobi sweep with ping-pong pose buffers (reference) */
+    uint32_t profile;     /* 1: launch eagerly and time every AVBD kernel with HIP events */
+} dio_step_opts;
+
+/* Counters and times of the last dio_step call. */
+typedef struct dio_step_stats {
+    uint32_t steps;
+    uint32_t pairs;            /* candidate pairs of the last step's broad phase */
+    uint32_t contacts;         /* contacts solved in the last step (narrow + container) */
+    uint32_t csr_nnz;          /* CSR entries of the last step */
+    uint32_t narrow_contacts;  /* narrow-phase part of `contacts` */
+    uint32_t dynamic_bodies;
+    uint64_t sum_pairs;        /* sums over the steps of the call */
+    uint64_t sum_contacts;
+    uint64_t sum_csr_nnz;
+    float ms_total;            /* GPU time of the whole call (events around all steps) */
+    float ms_avbd;             /* summed AVBD time: init + I x (primal + dual) + recover */
+    /* profile mode only: summed kernel times and launch counts */
+    float ms_init, ms_primal, ms_dual, ms_recover;
+    uint32_t n_init, n_primal, n_dual, n_recover;
+} dio_step_stats;
+
+typedef struct dio_ctx dio_ctx;
+
+/* ---- library / context ------------------------------------------------ */
+uint32_t dio_abi_version(void);
+void dio_default_options(dio_avbd_options* out); /* AvbdOptions::default(), physics/gpu.rs:140-155 */
+dio_status dio_create(int device, const dio_limits* limits, dio_ctx** out);
+void dio_destroy(dio_ctx* ctx);
+const char* dio_last_error(const dio_ctx* ctx);
+uint64_t dio_required_capacity(const dio_ctx* ctx); /* size asked for by the last DIO_CAPACITY */
+dio_status dio_synchronize(dio_ctx* ctx);
+
+/* ---- broad + narrow phase (NarrowPhaseCompute, render/compute.rs:490-750) ---- */
+/* Replaces submit_broad (compute.rs:490-567): upload bodies, compute AABBs and the
+   candidate pairs.  The pair list is the set {(i, j): i < j, closed AABB overlap}
+   sorted lexicographically, without the reference's 16384-pair truncation. */
+dio_status dio_broad_submit(dio_ctx* ctx, const dio_gpu_body* bodies, uint32_t n);
+/* Replaces read_pair_count (compute.rs:570-579). */
+dio_status dio_broad_count(dio_ctx* ctx, uint32_t* count);
+/* Replaces read_pairs (compute.rs:583-598); DIO_CAPACITY when cap is too small. */
+dio_status dio_broad_read_pairs(dio_ctx* ctx, dio_gpu_pair* pairs, uint32_t cap, uint32_t* n);
+/* Replaces read_aabbs-style parity reads: 2 float4 (min, max) per body. */
+dio_status dio_broad_read_aabbs(dio_ctx* ctx, float* aabbs, uint32_t cap_bodies);
+/* Replaces submit_narrow (compute.rs:601-671): narrow phase over the first
+   pair_count pairs of the last broad phase. */
+dio_status dio_narrow_submit(dio_ctx* ctx, uint32_t pair_count);
+/* Replaces submit (compute.rs:404-477): narrow phase over caller-given pairs. */
+dio_status dio_narrow_submit_pairs(dio_ctx* ctx, const dio_gpu_body* bodies, uint32_t n,
+                                   const dio_gpu_pair* pairs, uint32_t pair_count);
+/* Replaces read (compute.rs:675-693): 4 slots per pair, slot pair*4+k. */
+dio_status dio_narrow_read(dio_ctx* ctx, dio_gpu_contact_out* slots, uint32_t cap_slots,
+                           uint32_t* n_slots);
+
+/* ---- AVBD solve (AvbdCompute, render/gpu_physics.rs:439-622) ---------- */
+/* Replaces AvbdCompute::submit.  positions: n x 3 floats, orientations: n x 4
+   (x, y, z, w).  offsets: n + 1 CSR offsets, indices: nidx contact indices. */
+dio_status dio_avbd_submit(dio_ctx* ctx, const dio_avbd_body_state* state,
+                           const float* positions, const float* orientations, uint32_t n,
+                           const dio_avbd_contact* contacts, uint32_t m,
+                           const dio_avbd_constraint* constraints, uint32_t c,
+                           const uint32_t* offsets, const uint32_t* indices, uint32_t nidx,
+                           const dio_run_options* options);
+/* Replaces AvbdCompute::read (AvbdGpuResult, gpu_physics.rs:86-94).  Any
+   output pointer may be null.  positions n x 3, orientations n x 4, velocity
+   arrays n x 3, lambda / penalty m x 3. */
+dio_status dio_avbd_read(dio_ctx* ctx, float* positions, float* orientations,
+                         float* velocities, float* angular_velocities, float* prev_velocities,
+                         float* lambda, float* penalty);
+
+/* ---- fused device-resident world (AvbdSolver + Renderer step) --------- */
+/* Upload the world: bodies (pose + shape), their dynamics state, joints and
+   the optional container.  Resets contacts and the lagged narrow output. */
+dio_status dio_world_upload(dio_ctx* ctx, const dio_gpu_body* bodies,
+                            const dio_avbd_body_state* state, uint32_t n,
+                            const dio_avbd_constraint* constraints, uint32_t c,
+                            const dio_container* container);
+/* Vertex pool for DIO_SHAPE_HULL bodies (x, y, z per vertex). */
+dio_status dio_world_upload_hulls(dio_ctx* ctx, const float* vertices, uint32_t count);
+/* Run nsteps physics steps on the device without host round trips. */
+dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_step_opts* opts,
+                    uint32_t nsteps);
+dio_status dio_step_get_stats(dio_ctx* ctx, dio_step_stats* out);
+/* Download body state (any pointer may be null): n x 3 / n x 4 floats. */
+dio_status dio_world_download(dio_ctx* ctx, float* positions, float* orientations,
+                              float* velocities, float* angular_velocities,
+                              float* prev_velocities);
+/* Contacts of the last step with their solved lambda / penalty (the reference's
+   AvbdSolver::contacts() after sync_bodies_from_gpu). */
+dio_status dio_world_contacts(dio_ctx* ctx, dio_avbd_contact* out, uint32_t cap, uint32_t* m);
+/* Candidate pairs of the last step's broad phase (sorted, a < b). */
+dio_status dio_world_pairs(dio_ctx* ctx, dio_gpu_pair* out, uint32_t cap, uint32_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIOMEDES_HIP_H */

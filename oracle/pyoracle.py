@@ -1,0 +1,295 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and the CPU baseline.  The
+product library never imports this module.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_float, c_int, c_uint32, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+from diomedes_amd import records as R
+
+LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+_lib = None
+
+_D = POINTER(c_double)
+_U = POINTER(c_uint32)
+_F = POINTER(c_float)
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: run `make -C oracle`")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    sig = {
+        "oracle_threads": (c_int, [c_int]),
+        "oracle_world_create": (c_void_p, [c_int, c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, _F,
+                                           c_uint32]),
+        "oracle_world_destroy": (None, [c_void_p]),
+        "oracle_world_step": (None, [c_void_p, POINTER(R.Options), c_int, c_uint32]),
+        "oracle_world_get_state": (None, [c_void_p, _D, _D, _D, _D, _D]),
+        "oracle_world_contact_count": (c_uint32, [c_void_p]),
+        "oracle_world_get_contacts": (None, [c_void_p, _U, _D]),
+        "oracle_world_pair_count": (c_uint32, [c_void_p]),
+        "oracle_world_get_pairs": (None, [c_void_p, _U]),
+        "oracle_world_csr_nnz": (c_uint32, [c_void_p]),
+        "oracle_world_get_csr": (None, [c_void_p, _U, _U]),
+        "oracle_world_times": (None, [c_void_p, _D]),
+        "oracle_world_reset_times": (None, [c_void_p]),
+        "oracle_aabbs": (None, [c_int, c_void_p, c_uint32, _F, c_uint32, _D]),
+        "oracle_broad_pairs": (c_uint32, [c_int, c_void_p, c_uint32, _F, c_uint32, _U, c_uint32]),
+        "oracle_narrow_pair": (c_uint32, [c_int, c_void_p, c_void_p, _F, c_uint32, _D]),
+        "oracle_gjk": (c_int, [c_int, c_void_p, c_void_p, _F, c_uint32]),
+        "oracle_solve6": (None, [c_int, _D, _D, _D, _D, _D]),
+        "oracle_container_contacts": (c_uint32, [c_int, c_void_p, c_void_p, POINTER(R.Container), POINTER(R.Options),
+                                                 c_uint32, _F, c_uint32, _U, _D]),
+        "oracle_raw_contact": (None, [c_int, c_void_p, c_void_p, c_uint32, c_uint32, _D, c_double, _D, _D,
+                                      POINTER(R.Options), _U, _D]),
+        "oracle_finalize_contacts": (None, [c_int, c_void_p, c_uint32, c_void_p, c_uint32, c_uint32,
+                                            POINTER(R.Options), _U, _D, _U, _U]),
+        "oracle_avbd_run": (None, [c_int, c_void_p, _F, _F, c_uint32, c_void_p, c_uint32, c_void_p, c_uint32, _U, _U,
+                                   c_uint32, POINTER(R.RunOptions), _D, _D, _D, _D, _D, _D, _D]),
+        "oracle_qrot": (None, [c_int, _D, _D, _D]),
+        "oracle_qmul": (None, [c_int, _D, _D, _D]),
+        "oracle_support": (None, [c_int, c_int, c_void_p, _D, _D]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a, t=None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(c_void_p if t is None else POINTER(t))
+
+
+def threads(n: int = 0) -> int:
+    """Set (n > 0) and return the OpenMP thread count of the oracle."""
+    return load().oracle_threads(n)
+
+
+# contacts come back as 25 doubles per contact
+CONTACT_FIELDS = {"friction": (0, 1), "normal": (1, 4), "tangent1": (4, 7), "tangent2": (7, 10), "r_a": (10, 13),
+                  "r_b": (13, 16), "c0": (16, 19), "penalty": (19, 22), "lambda": (22, 25)}
+
+
+def split_contacts(ab: np.ndarray, data: np.ndarray) -> dict:
+    out = {"a": ab[:, 0].copy(), "b": ab[:, 1].copy()}
+    for k, (s, e) in CONTACT_FIELDS.items():
+        out[k] = data[:, s:e] if e - s > 1 else data[:, s]
+    return out
+
+
+class OracleWorld:
+    """The step protocol on the CPU, in float32 (precision 32) or float64."""
+
+    def __init__(self, scene, precision: int = 32):
+        lib = load()
+        self.scene = scene
+        self.precision = precision
+        bodies = np.ascontiguousarray(scene.bodies, R.GPU_BODY)
+        state = np.ascontiguousarray(scene.state, R.BODY_STATE)
+        cons = np.ascontiguousarray(scene.constraints, R.AVBD_CONSTRAINT)
+        hull = np.ascontiguousarray(scene.hull, np.float32).reshape(-1, 3)
+        ct = scene.container.to_c() if scene.container is not None else None
+        self.n = len(bodies)
+        self._h = lib.oracle_world_create(precision, _p(bodies), _p(state), len(bodies), _p(cons), len(cons),
+                                          ctypes.byref(ct) if ct is not None else None, _p(hull, c_float), len(hull))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().oracle_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, options=None, nsteps: int = 1, lag: int = 1) -> None:
+        o = (options or self.scene.options).to_c()
+        load().oracle_world_step(self._h, ctypes.byref(o), lag, nsteps)
+
+    def state(self) -> dict:
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("positions", "velocities", "angular_velocities", "prev_velocities")}
+        out["orientations"] = np.zeros((n, 4))
+        load().oracle_world_get_state(self._h, _p(out["positions"], c_double), _p(out["orientations"], c_double),
+                                      _p(out["velocities"], c_double), _p(out["angular_velocities"], c_double),
+                                      _p(out["prev_velocities"], c_double))
+        return out
+
+    def contacts(self) -> dict:
+        lib = load()
+        m = lib.oracle_world_contact_count(self._h)
+        ab = np.zeros((m, 2), np.uint32)
+        data = np.zeros((m, 25))
+        lib.oracle_world_get_contacts(self._h, _p(ab, c_uint32), _p(data, c_double))
+        return split_contacts(ab, data)
+
+    def pairs(self) -> np.ndarray:
+        lib = load()
+        p = lib.oracle_world_pair_count(self._h)
+        out = np.zeros((p, 2), np.uint32)
+        lib.oracle_world_get_pairs(self._h, _p(out, c_uint32))
+        return out
+
+    def csr(self) -> tuple:
+        lib = load()
+        nnz = lib.oracle_world_csr_nnz(self._h)
+        off = np.zeros(self.n + 1, np.uint32)
+        idx = np.zeros(nnz, np.uint32)
+        lib.oracle_world_get_csr(self._h, _p(off, c_uint32), _p(idx, c_uint32))
+        return off, idx
+
+    def times(self) -> dict:
+        t = np.zeros(9)
+        load().oracle_world_times(self._h, _p(t, c_double))
+        keys = ("broad", "narrow", "contacts", "avbd", "total", "init", "primal", "dual", "recover")
+        return dict(zip(keys, t.tolist()))
+
+    def reset_times(self) -> None:
+        load().oracle_world_reset_times(self._h)
+
+
+# ---------------------------------------------------------------- unit level
+def aabbs(bodies: np.ndarray, hull=None, precision: int = 32) -> np.ndarray:
+    bodies = np.ascontiguousarray(bodies, R.GPU_BODY)
+    hull = np.zeros((0, 3), np.float32) if hull is None else np.ascontiguousarray(hull, np.float32)
+    out = np.zeros((len(bodies), 6))
+    load().oracle_aabbs(precision, _p(bodies), len(bodies), _p(hull, c_float), len(hull), _p(out, c_double))
+    return out
+
+
+def broad_pairs(bodies: np.ndarray, hull=None, precision: int = 32) -> np.ndarray:
+    bodies = np.ascontiguousarray(bodies, R.GPU_BODY)
+    hull = np.zeros((0, 3), np.float32) if hull is None else np.ascontiguousarray(hull, np.float32)
+    lib = load()
+    n = lib.oracle_broad_pairs(precision, _p(bodies), len(bodies), _p(hull, c_float), len(hull), None, 0)
+    out = np.zeros((n, 2), np.uint32)
+    lib.oracle_broad_pairs(precision, _p(bodies), len(bodies), _p(hull, c_float), len(hull), _p(out, c_uint32), n)
+    return out
+
+
+def narrow_pair(a, b, hull=None, precision: int = 32) -> list:
+    """Up to 4 contacts (normal, depth, point_a, point_b) of one pair."""
+    a = np.ascontiguousarray(np.atleast_1d(a), R.GPU_BODY)
+    b = np.ascontiguousarray(np.atleast_1d(b), R.GPU_BODY)
+    hull = np.zeros((0, 3), np.float32) if hull is None else np.ascontiguousarray(hull, np.float32)
+    out = np.zeros(40)
+    c = load().oracle_narrow_pair(precision, _p(a), _p(b), _p(hull, c_float), len(hull), _p(out, c_double))
+    return [
+        {"normal": out[10 * k:10 * k + 3], "depth": out[10 * k + 3], "point_a": out[10 * k + 4:10 * k + 7],
+         "point_b": out[10 * k + 7:10 * k + 10]}
+        for k in range(c)
+    ]
+
+
+def gjk(a, b, hull=None, precision: int = 32) -> bool:
+    a = np.ascontiguousarray(np.atleast_1d(a), R.GPU_BODY)
+    b = np.ascontiguousarray(np.atleast_1d(b), R.GPU_BODY)
+    hull = np.zeros((0, 3), np.float32) if hull is None else np.ascontiguousarray(hull, np.float32)
+    return bool(load().oracle_gjk(precision, _p(a), _p(b), _p(hull, c_float), len(hull)))
+
+
+def solve6(lin6, ang6, cross9, b6, precision: int = 64) -> np.ndarray:
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (lin6, ang6, cross9, b6)]
+    x = np.zeros(6)
+    load().oracle_solve6(precision, *[_p(a, c_double) for a in arrs], _p(x, c_double))
+    return x
+
+
+def container_contacts(body, state, container, options, index: int = 0, hull=None, precision: int = 32) -> dict:
+    body = np.ascontiguousarray(np.atleast_1d(body), R.GPU_BODY)
+    state = np.ascontiguousarray(np.atleast_1d(state), R.BODY_STATE)
+    hull = np.zeros((0, 3), np.float32) if hull is None else np.ascontiguousarray(hull, np.float32)
+    ab = np.zeros((24, 2), np.uint32)
+    data = np.zeros((24, 25))
+    ct, o = container.to_c(), options.to_c()
+    m = load().oracle_container_contacts(precision, _p(body), _p(state), ctypes.byref(ct), ctypes.byref(o), index,
+                                         _p(hull, c_float), len(hull), _p(ab, c_uint32), _p(data, c_double))
+    return split_contacts(ab[:m], data[:m])
+
+
+def raw_contact(body_a, body_b, a: int, b: int, normal, depth: float, pa, pb, options, precision: int = 32) -> dict:
+    ba = np.ascontiguousarray(np.atleast_1d(body_a), R.GPU_BODY)
+    bb = np.ascontiguousarray(np.atleast_1d(body_b), R.GPU_BODY)
+    n, pa, pb = (np.ascontiguousarray(x, np.float64) for x in (normal, pa, pb))
+    ab = np.zeros((1, 2), np.uint32)
+    data = np.zeros((1, 25))
+    o = options.to_c()
+    load().oracle_raw_contact(precision, _p(ba), _p(bb), a, b, _p(n, c_double), float(depth), _p(pa, c_double),
+                              _p(pb, c_double), ctypes.byref(o), _p(ab, c_uint32), _p(data, c_double))
+    return split_contacts(ab, data)
+
+
+def finalize_contacts(old: np.ndarray, new: np.ndarray, n: int, options, precision: int = 32):
+    old = np.ascontiguousarray(old, R.AVBD_CONTACT)
+    new = np.ascontiguousarray(new, R.AVBD_CONTACT)
+    m = len(new)
+    ab = np.zeros((m, 2), np.uint32)
+    data = np.zeros((m, 25))
+    nnz = int(np.sum(new["b"] != R.CONTAINER) + m)
+    off = np.zeros(n + 1, np.uint32)
+    idx = np.zeros(max(nnz, 1), np.uint32)
+    o = options.to_c()
+    load().oracle_finalize_contacts(precision, _p(old), len(old), _p(new), m, n, ctypes.byref(o), _p(ab, c_uint32),
+                                    _p(data, c_double), _p(off, c_uint32), _p(idx, c_uint32))
+    return split_contacts(ab, data), off, idx[:nnz]
+
+
+def avbd_run(state, positions, orientations, contacts, constraints, offsets, indices, run_options,
+             precision: int = 32) -> dict:
+    st = np.ascontiguousarray(state, R.BODY_STATE)
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    rot = np.ascontiguousarray(orientations, np.float32).reshape(-1, 4)
+    con = np.ascontiguousarray(contacts, R.AVBD_CONTACT)
+    cns = np.ascontiguousarray(constraints, R.AVBD_CONSTRAINT)
+    off = np.ascontiguousarray(offsets, np.uint32)
+    idx = np.ascontiguousarray(indices, np.uint32)
+    n, m = len(st), len(con)
+    out = {"positions": np.zeros((n, 3)), "orientations": np.zeros((n, 4)), "velocities": np.zeros((n, 3)),
+           "angular_velocities": np.zeros((n, 3)), "prev_velocities": np.zeros((n, 3)), "lambda": np.zeros((m, 3)),
+           "penalty": np.zeros((m, 3))}
+    d = lambda k: _p(out[k], c_double)  # noqa: E731
+    load().oracle_avbd_run(precision, _p(st), _p(pos, c_float), _p(rot, c_float), n, _p(con), m, _p(cns), len(cns),
+                           _p(off, c_uint32), _p(idx, c_uint32), len(idx), ctypes.byref(run_options),
+                           d("positions"), d("orientations"), d("velocities"), d("angular_velocities"),
+                           d("prev_velocities"), d("lambda"), d("penalty"))
+    return out
+
+
+def qrot(q, v, precision: int = 32) -> np.ndarray:
+    q, v = np.ascontiguousarray(q, np.float64), np.ascontiguousarray(v, np.float64)
+    out = np.zeros(3)
+    load().oracle_qrot(precision, _p(q, c_double), _p(v, c_double), _p(out, c_double))
+    return out
+
+
+def qmul(a, b, precision: int = 32) -> np.ndarray:
+    a, b = np.ascontiguousarray(a, np.float64), np.ascontiguousarray(b, np.float64)
+    out = np.zeros(4)
+    load().oracle_qmul(precision, _p(a, c_double), _p(b, c_double), _p(out, c_double))
+    return out
+
+
+def support(body, direction, host: bool = False, precision: int = 32) -> np.ndarray:
+    body = np.ascontiguousarray(np.atleast_1d(body), R.GPU_BODY)
+    d = np.ascontiguousarray(direction, np.float64)
+    out = np.zeros(3)
+    load().oracle_support(precision, 1 if host else 0, _p(body), _p(d, c_double), _p(out, c_double))
+    return out

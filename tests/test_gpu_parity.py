@@ -1,0 +1,212 @@
+"""GPU parity: the HIP kernels against the FP32 oracle, through the C ABI.
+
+Bit-exact is the bar for everything on the FP32 path: pair lists (integer),
+AABBs, narrow-phase contacts, AVBD poses / velocities / multipliers, and whole
+multi-step trajectories of the fused device step.  Both sides evaluate the
+same float operations in the same order with no fused multiply-adds.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from diomedes_amd import PhysicsContext, scenes
+from diomedes_amd import records as R
+from diomedes_amd.physics import AvbdOptions
+
+from .helpers import assert_bits, contacts_to_records
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+# ------------------------------------------------------------ broad phase
+@pytest.mark.parametrize("nx,ny,nz,seed", [(4, 2, 4, 1), (8, 2, 8, 2), (32, 8, 32, 0)])
+def test_broad_pairs_match_dense_sweep(oracle, nx, ny, nz, seed):
+    _print("broad", nx, ny, nz, seed)
+    scene = scenes.cube_field(seed, nx=nx, ny=ny, nz=nz, spacing=1.2)
+    with PhysicsContext(scene.n, max_pairs=64 * scene.n) as ctx:
+        ctx.broad_phase_submit(scene.bodies)
+        pairs = ctx.broad_phase_pairs()
+        aabbs = ctx.broad_phase_aabbs(scene.n)
+    ref = oracle.broad_pairs(scene.bodies)
+    ref_aabb = oracle.aabbs(scene.bodies)
+    assert_bits(aabbs[:, [0, 1, 2, 4, 5, 6]], ref_aabb, "aabbs")
+    got = np.stack([pairs["a"], pairs["b"]], axis=1)
+    assert got.shape == ref.shape, (got.shape, ref.shape)
+    assert np.array_equal(got, ref)
+
+
+def test_broad_degenerate_inputs(oracle):
+    # one body, two coincident bodies, many identical bodies (equal Morton codes)
+    for n in (1, 2, 50):
+        scene = scenes.cube_field(3, nx=n, ny=1, nz=1, spacing=0.0)
+        with PhysicsContext(max(n, 1), max_pairs=n * n + 16) as ctx:
+            ctx.broad_phase_submit(scene.bodies)
+            pairs = ctx.broad_phase_pairs()
+        ref = oracle.broad_pairs(scene.bodies)
+        assert np.array_equal(np.stack([pairs["a"], pairs["b"]], 1).reshape(-1, 2), ref.reshape(-1, 2))
+        assert len(ref) == n * (n - 1) // 2
+
+
+def test_broad_capacity_reports_required_size(oracle):
+    from diomedes_amd._lib import DioError
+
+    scene = scenes.cube_field(3, nx=20, ny=1, nz=1, spacing=0.0)  # 190 pairs
+    with PhysicsContext(scene.n, max_pairs=64) as ctx:
+        ctx.broad_phase_submit(scene.bodies)
+        with pytest.raises(DioError) as e:
+            ctx.broad_phase_count()
+        assert e.value.status == "CAPACITY" and e.value.required == 190
+
+
+# ------------------------------------------------------------ narrow phase
+def _settled_field(oracle, nx=6, ny=3, nz=6, steps=40, seed=4):
+    scene = scenes.cube_field(seed, nx=nx, ny=ny, nz=nz)
+    w = oracle.OracleWorld(scene, 32)
+    w.step(nsteps=steps)
+    st = w.state()
+    bodies = scene.bodies.copy()
+    bodies["position"][:, :3] = st["positions"].astype(np.float32)
+    bodies["orientation"] = st["orientations"].astype(np.float32)
+    return scene, w, bodies
+
+
+def test_narrow_box_manifolds_bitexact(oracle):
+    scene, _, bodies = _settled_field(oracle)
+    pairs = oracle.broad_pairs(bodies)
+    _print("narrow pairs", len(pairs))
+    assert len(pairs) > 20
+    p = np.zeros(len(pairs), R.GPU_PAIR)
+    p["a"], p["b"] = pairs[:, 0], pairs[:, 1]
+    with PhysicsContext(scene.n) as ctx:
+        raw = ctx.narrow_phase_run(bodies, p)
+    assert len(raw) == 4 * len(pairs)
+    total = 0
+    for i, (a, b) in enumerate(pairs):
+        ref = oracle.narrow_pair(bodies[a], bodies[b])
+        slots = raw[4 * i:4 * i + 4]
+        assert int(np.sum(slots["valid"])) == len(ref), (i, a, b)
+        for k, c in enumerate(ref):
+            s = slots[k]
+            assert s["valid"] == 1 and s["a"] == a and s["b"] == b
+            assert_bits(s["normal"][:3], c["normal"], f"normal {i}/{k}")
+            assert_bits(s["depth"], c["depth"], f"depth {i}/{k}")
+            assert_bits(s["point_a"][:3], c["point_a"], f"pa {i}/{k}")
+            assert_bits(s["point_b"][:3], c["point_b"], f"pb {i}/{k}")
+            total += 1
+        assert np.all(slots["valid"][len(ref):] == 0)
+    assert total > 0
+
+
+# ------------------------------------------------------------ AVBD solve
+def test_avbd_granular_solve_bitexact(oracle):
+    scene, w, _ = _settled_field(oracle, steps=30)
+    # the oracle world's current contacts / CSR / state feed one granular solve
+    c = w.contacts()
+    off, idx = w.csr()
+    st = w.state()
+    state = scene.state.copy()
+    state["vel"][:, :3] = st["velocities"]
+    state["ang"][:, :3] = st["angular_velocities"]
+    state["prev_vel"][:, :3] = st["prev_velocities"]
+    pos = st["positions"].astype(np.float32)
+    rot = st["orientations"].astype(np.float32)
+    con = contacts_to_records(c)
+    opts = scene.options
+    _print("granular avbd contacts", len(con))
+    ref = oracle.avbd_run(state, pos, rot, con, scene.constraints, off, idx, opts.run_options())
+    with PhysicsContext(scene.n) as ctx:
+        ctx.gpu_physics_submit(state, pos, rot, con, scene.constraints, off, idx, opts)
+        got = ctx.gpu_physics_read()
+    for k in ("positions", "orientations", "velocities", "angular_velocities", "prev_velocities", "lambda", "penalty"):
+        assert_bits(got[k], ref[k], k)
+
+
+def test_avbd_granular_joints_bitexact(oracle):
+    """Springs, rods, a ball socket and a rope (the constraints demo's set,
+    demos/constraints/src/main.rs:110-146) through one solve."""
+    from diomedes_amd.physics import AvbdBody, BallSocket, Rod, Rope, Scene, Spring
+
+    bodies = [
+        AvbdBody.cube((-7.0, 1.6, 0.0), 0.4, 0.0), AvbdBody.cube((-7.0, 0.0, 0.0), 0.3, 1.0),
+        AvbdBody.cube((-2.5, 1.6, 0.0), 0.4, 0.0), AvbdBody.cube((-1.7, 0.2, 0.0), 0.3, 1.0),
+        AvbdBody.cube((-2.2, -1.0, 0.0), 0.3, 1.0), AvbdBody.cube((2.2, 1.7, 0.0), 0.4, 0.0),
+        AvbdBody.cube((2.2, 2.05, 0.0), 0.3, 1.0), AvbdBody.cube((6.5, 1.7, 0.0), 0.4, 0.0),
+        AvbdBody.cube((6.5, -0.2, 0.0), 0.3, 1.0),
+    ]
+    for i, v in ((1, (9.2, 4.4, 5.0)), (3, (-5.4, 4.0, 5.0)), (4, (5.7, 4.2, 4.0)), (6, (5.0, 4.0, 4.8)),
+                 (8, (-5.8, 5.5, 4.0))):
+        bodies[i].velocity = v
+    cons = [Spring(0, 1, (0, 0, 0), (0, 0, 0), 12.0, 1.6), Rod(2, 3, (0, 0, 0), (0, 0, 0), 1.6),
+            Rod(3, 4, (0, 0, 0), (0, 0, 0), 1.4), BallSocket(5, 6, (0, 0, 0), (0.0, -0.3, 0.0)),
+            Rope(7, 8, (0, 0, 0), (0, 0, 0), 1.5)]
+    opts = AvbdOptions(gravity=(0.0, -3.5, 0.0), iterations=16)
+    scene = Scene.from_bodies(bodies, cons, None, opts)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        for k in range(4):
+            w.step(opts, nsteps=30)
+            ctx.step(opts, 30)
+            got, ref = ctx.download(), w.state()
+            for key in ("positions", "orientations", "velocities", "angular_velocities", "prev_velocities"):
+                assert_bits(got[key], ref[key], f"step {30 * (k + 1)} {key}")
+
+
+# ------------------------------------------------------------ fused step
+def _compare_world(ctx, w, step, check_contacts=True):
+    got, ref = ctx.download(), w.state()
+    for key in ("positions", "orientations", "velocities", "angular_velocities", "prev_velocities"):
+        assert_bits(got[key], ref[key], f"step {step} {key}")
+    if check_contacts:
+        gc = ctx.contacts()
+        rc = w.contacts()
+        assert len(gc) == len(rc["a"]), f"step {step}: {len(gc)} contacts vs {len(rc['a'])}"
+        assert np.array_equal(gc["a"], rc["a"]) and np.array_equal(gc["b"], rc["b"])
+        assert_bits(gc["lambda_"][:, :3], rc["lambda"], f"step {step} lambda")
+        assert_bits(gc["penalty"][:, :3], rc["penalty"], f"step {step} penalty")
+        assert_bits(gc["r_a"][:, :3], rc["r_a"], f"step {step} r_a")
+
+
+def test_box_stack_300_steps_bitexact(oracle):
+    scene = scenes.box_stack(0)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        for k in range(10):
+            ctx.step(scene.options, 30)
+            w.step(nsteps=30)
+            _compare_world(ctx, w, 30 * (k + 1))
+        pairs = ctx.pairs()
+        ref_pairs = w.pairs()
+        assert np.array_equal(np.stack([pairs["a"], pairs["b"]], 1).reshape(-1, 2), ref_pairs.reshape(-1, 2))
+
+
+@pytest.mark.parametrize("lag", [1, 0])
+def test_cube_field_small_bitexact(oracle, lag):
+    scene = scenes.cube_field(7, nx=8, ny=2, nz=8)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        for k in range(6):
+            ctx.step(scene.options, 20, lag=lag)
+            w.step(nsteps=20, lag=lag)
+            _compare_world(ctx, w, 20 * (k + 1))
+
+
+def test_cube_field_8192_first_steps_bitexact(oracle):
+    scene = scenes.cube_field(0)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        for k in range(3):
+            ctx.step(scene.options, 1)
+            w.step(nsteps=1)
+            _compare_world(ctx, w, k + 1)
+            s = ctx.stats()
+            _print("8192 step", k, "pairs", s.pairs, "contacts", s.contacts)
