@@ -1028,25 +1028,15 @@ inline double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// gpu_physics.rs:439-622 + avbd/src/lib.rs:202-699.  Inputs: per-body state,
-// poses, contacts (λ / penalty updated in place), joints, CSR.  Outputs: the
-// pose of the final parity buffer and the recovered velocities.
-template <class T>
-void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::vector<Constraint<T>>& cons,
-                const std::vector<uint32_t>& offsets, const std::vector<uint32_t>& indices, const RunOptions<T>& pc,
-                std::vector<V3<T>>& out_pos, std::vector<Q<T>>& out_rot, SolveTimes* times = nullptr) {
+// avbd_init (avbd/src/lib.rs:202-254): initial and inertial poses, then the
+// adaptive gravity warm start into parity buffer 0
+template <class T> void avbd_init_only(SolveState<T>& S, const RunOptions<T>& pc) {
     const int64_t n = (int64_t)S.inv_mass.size();
-    const int64_t m = (int64_t)contacts.size();
-    out_pos.assign(n, V3<T>{});
-    out_rot.assign(n, Q<T>{});
-    if (n == 0) return;
     S.initial_lin.assign(n, V3<T>{});
     S.inertial_lin.assign(n, V3<T>{});
     S.initial_ang.assign(n, Q<T>{});
     S.inertial_ang.assign(n, Q<T>{});
     const T dt = pc.dt;
-    double t0 = now_s();
-    // avbd_init (avbd/src/lib.rs:202-254)
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         V3<T> pos = S.pos[0][i];
@@ -1070,6 +1060,23 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
         S.pos[0][i] = wp;
         S.rot[0][i] = wr;
     }
+}
+
+// gpu_physics.rs:439-622 + avbd/src/lib.rs:202-699.  Inputs: per-body state,
+// poses, contacts (λ / penalty updated in place), joints, CSR.  Outputs: the
+// pose of the final parity buffer and the recovered velocities.
+template <class T>
+void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::vector<Constraint<T>>& cons,
+                const std::vector<uint32_t>& offsets, const std::vector<uint32_t>& indices, const RunOptions<T>& pc,
+                std::vector<V3<T>>& out_pos, std::vector<Q<T>>& out_rot, SolveTimes* times = nullptr) {
+    const int64_t n = (int64_t)S.inv_mass.size();
+    const int64_t m = (int64_t)contacts.size();
+    out_pos.assign(n, V3<T>{});
+    out_rot.assign(n, Q<T>{});
+    if (n == 0) return;
+    const T dt = pc.dt;
+    double t0 = now_s();
+    avbd_init_only(S, pc);
     double t1 = now_s();
     if (times) times->init += t1 - t0;
     // per-body joint lists in ascending constraint order (equivalent to the

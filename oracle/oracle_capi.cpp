@@ -428,6 +428,44 @@ OAPI void oracle_avbd_run(int precision, const dio_avbd_body_state* st, const fl
                           o_prev, o_lam, o_pen);
 }
 
+// avbd_init alone (avbd/src/lib.rs:202-254): the warm-started pose (pos0 /
+// rot0) and the inertial pose of every body, for the unit test of the
+// adaptive gravity warm start.  Outputs n x 3 / n x 4 doubles.
+template <class T>
+static void avbd_init_t(const dio_avbd_body_state* st, const float* pos, const float* rot, uint32_t n,
+                        const dio_run_options* ro, double* warm_pos, double* warm_rot, double* iner_pos,
+                        double* iner_rot) {
+    SolveState<T> S;
+    S.vel.resize(n); S.ang.resize(n); S.prev_vel.resize(n); S.inv_moment.resize(n); S.inv_mass.resize(n);
+    S.pos[0].resize(n); S.rot[0].resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        S.vel[i] = {T(st[i].vel[0]), T(st[i].vel[1]), T(st[i].vel[2])};
+        S.ang[i] = {T(st[i].ang[0]), T(st[i].ang[1]), T(st[i].ang[2])};
+        S.prev_vel[i] = {T(st[i].prev_vel[0]), T(st[i].prev_vel[1]), T(st[i].prev_vel[2])};
+        S.inv_moment[i] = {T(st[i].inv_moment[0]), T(st[i].inv_moment[1]), T(st[i].inv_moment[2])};
+        S.inv_mass[i] = T(st[i].inv_mass);
+        S.pos[0][i] = {T(pos[3 * i]), T(pos[3 * i + 1]), T(pos[3 * i + 2])};
+        S.rot[0][i] = {T(rot[4 * i]), T(rot[4 * i + 1]), T(rot[4 * i + 2]), T(rot[4 * i + 3])};
+    }
+    S.pos[1] = S.pos[0];
+    S.rot[1] = S.rot[0];
+    RunOptions<T> pc{T(ro->dt), {T(ro->gravity[0]), T(ro->gravity[1]), T(ro->gravity[2])}, T(ro->alpha),
+                     T(ro->beta_lin), T(ro->penalty_max), 0};
+    avbd_init_only(S, pc);
+    for (uint32_t i = 0; i < n; ++i) {
+        put3(warm_pos + 3 * i, S.pos[0][i]);
+        put4(warm_rot + 4 * i, S.rot[0][i]);
+        put3(iner_pos + 3 * i, S.inertial_lin[i]);
+        put4(iner_rot + 4 * i, S.inertial_ang[i]);
+    }
+}
+OAPI void oracle_avbd_init(int precision, const dio_avbd_body_state* st, const float* pos, const float* rot,
+                           uint32_t n, const dio_run_options* ro, double* warm_pos, double* warm_rot,
+                           double* iner_pos, double* iner_rot) {
+    if (precision == 64) avbd_init_t<double>(st, pos, rot, n, ro, warm_pos, warm_rot, iner_pos, iner_rot);
+    else avbd_init_t<float>(st, pos, rot, n, ro, warm_pos, warm_rot, iner_pos, iner_rot);
+}
+
 // glam primitives for the known-answer tests: Quat * Vec3 and Quat * Quat
 OAPI void oracle_qrot(int precision, const double* q, const double* v, double* out) {
     if (precision == 64) put3(out, qrot(Q<double>{q[0], q[1], q[2], q[3]}, V3<double>{v[0], v[1], v[2]}));

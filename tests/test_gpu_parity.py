@@ -210,3 +210,26 @@ def test_cube_field_8192_first_steps_bitexact(oracle):
             _compare_world(ctx, w, k + 1)
             s = ctx.stats()
             _print("8192 step", k, "pairs", s.pairs, "contacts", s.contacts)
+
+
+# ------------------------------------------------------------ golden fixtures
+def test_box_stack_matches_golden_fixture():
+    """The committed FP32 trajectory (tests/golden/box_stack_s0.npz), bit-exact."""
+    from pathlib import Path
+
+    g = np.load(Path(__file__).resolve().parent / "golden" / "box_stack_s0.npz")
+    scene = scenes.box_stack(0)
+    every = int(g["every"])
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        for s in range(int(g["steps"]) // every):
+            ctx.step(scene.options, every)
+            got = ctx.download()
+            for k in ("positions", "orientations", "velocities", "angular_velocities"):
+                assert_bits(got[k], g[f"fp32_{k}"][s], f"step {every * (s + 1)} {k}")
+        gc = ctx.contacts()
+        assert np.array_equal(np.stack([gc["a"], gc["b"]], 1), g["fp32_contact_ab"])
+        assert_bits(gc["lambda_"][:, :3], g["fp32_contact_lambda"], "lambda")
+        # error against the FP64 oracle, stated
+        p64 = g["fp64_positions"][-1]
+        _print("box stack: max |x_gpu - x_fp64| after 300 steps =", float(np.max(np.abs(got["positions"] - p64))))
