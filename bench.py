@@ -95,6 +95,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-steps", type=int, default=20, help="steps of the per-kernel timing pass")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -105,6 +107,17 @@ def main() -> int:
         return 2
 
     from diomedes_amd import PhysicsContext, scenes
+
+    if args.calibrate_copy_mib:
+        import torch
+
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     scene = scenes.cube_field(args.seed)
     opts = scene.options

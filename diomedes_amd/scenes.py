@@ -17,6 +17,20 @@ AvbdContainer whose other walls are out of reach):
   (mass (2h)^3), grid spacing 2.0 with U(-0.1, 0.1) jitter, lowest layer at
   y = 2.0, uniform random orientations; g = -9.81, dt = 1/60, 4 iterations,
   friction 0.6.
+- ``mixed_field``: n bodies, a third each of subdivision-1 icosphere hulls
+  (42 vertices, radius U(0.3, 0.6)), regular tetrahedra (edge U(0.5, 1.0))
+  and boxes (half extents U(0.2, 0.5)), density 1, velocities N(0, 2) per
+  axis, on a grid of spacing 2.0 inside a 60 m container (centre (0, 30, 0),
+  half 30); plus ropes of rope_links small spheres (r 0.05, mass 0.1, the
+  links are bodies) joined by Rope(max 0.2) constraints, and Hookean springs
+  (k 1e4, rest = initial distance) between random grid neighbours;
+  dt = 1/120, 8 iterations.
+- ``batched_scenes``: a scene table of independent drops; scene sizes
+  round(exp(U(ln 8, ln 512))); each scene's cubes / spheres fall on its own
+  floor (see BatchedScenes).
+- ``wall_throw``: a 64 x 32 wall of bricks (0.5 x 0.25 x 0.25, density 1) on
+  the floor, hit by a sphere (r 0.5, mass 20) thrown with v0 ~ (15 +- 2,
+  2 +- 0.5, 0); dt = 1/60, 6 iterations.
 """
 
 from __future__ import annotations
@@ -138,3 +152,165 @@ def cube_field(seed: int = 0, nx: int = 32, ny: int = 8, nz: int = 32, spacing: 
     opts = AvbdOptions(dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), iterations=4, friction=0.6)
     return Scene(bodies, state, container=floor_container(extent), options=opts,
                  name=f"cube_field{nx}x{nz}x{ny}_s{seed}")
+
+
+# ------------------------------------------------------------- mixed convex
+def icosphere(subdivisions: int = 1) -> np.ndarray:
+    """Unit icosphere vertices: the icosahedron, each edge split at its
+    midpoint `subdivisions` times, projected to the sphere (42 for 1)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(x, np.float64) / np.linalg.norm(x) for x in v]
+    for _ in range(subdivisions):
+        cache, nf = {}, []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    return np.array(verts)
+
+
+def regular_tetrahedron(edge: float) -> np.ndarray:
+    c = np.array([(1, 1, 1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1)], np.float64)
+    return c * (edge / (2.0 * 2 ** 0.5))
+
+
+def mixed_field(seed: int = 0, n: int = 20000, ropes: int = 128, rope_links: int = 64, springs: int = 512,
+                extent: float = 30.0) -> Scene:
+    """Draw order: shape parameters (n), orientations (3n), velocities (3n
+    normals), spring endpoints (2 x springs)."""
+    rng = SplitMix64(seed)
+    kinds = np.arange(n) % 3  # 0 hull, 1 tetrahedron, 2 box
+    size = rng.uniform(0.0, 1.0, n)
+    q = shoemake(rng, n)
+    vel = rng.normal(0.0, 2.0, 3 * n).reshape(n, 3)
+    side = int(np.ceil(n ** (1.0 / 3.0)))
+    idx = np.arange(n)
+    spacing = min(2.0, 1.8 * extent / side)
+    base = np.stack([idx % side, (idx // side) % side, idx // (side * side)], axis=1).astype(np.float64)
+    pos = (base - (side - 1) / 2.0) * spacing
+    pos[:, 1] += extent
+    ico = icosphere(1).astype(np.float32)
+    n_rope_bodies = ropes * rope_links
+    total = n + n_rope_bodies
+    bodies = np.zeros(total, R.GPU_BODY)
+    state = np.zeros(total, R.BODY_STATE)
+    hull_pool = []
+    for i in range(n):
+        k = kinds[i]
+        if k == 0:
+            r = np.float32(0.3 + 0.3 * size[i])
+            first = len(hull_pool) * 0  # one shared unit pool scaled per body is not possible: store per body
+            verts = (ico * r).astype(np.float32)
+            bodies[i]["tag"] = R.HULL
+            bodies[i]["corners"][0, 0] = np.float32(len(hull_pool) * 42)
+            bodies[i]["corners"][0, 1] = np.float32(42)
+            hull_pool.append(verts)
+            mass = np.float32(4.0 / 3.0 * np.pi) * r * r * r
+            inv_i = np.float32(1.0) / (np.float32(0.4) * mass * r * r)
+            state[i]["inv_moment"][:3] = inv_i
+            del first
+        elif k == 1:
+            e = 0.5 + 0.5 * size[i]
+            c = regular_tetrahedron(e).astype(np.float32)
+            bodies[i]["tag"] = R.TETRAHEDRON
+            bodies[i]["corners"][:, :3] = c
+            mass = np.float32(e ** 3 / (6.0 * 2 ** 0.5))
+            radius = np.float32(np.max(np.sqrt(np.sum(c * c, axis=1))))
+            state[i]["inv_moment"][:3] = np.float32(1.0) / (np.float32(0.4) * mass * radius * radius)
+        else:
+            h = np.float32([0.2 + 0.3 * size[i], 0.2 + 0.3 * ((size[i] * 7.31) % 1.0), 0.2 + 0.3 * ((size[i] * 3.17) % 1.0)])
+            bodies[i]["tag"] = R.BOX
+            bodies[i]["corners"][0, :3] = h
+            mass = np.float32(8.0) * h[0] * h[1] * h[2]
+            for ax, (j, k2) in enumerate(((1, 2), (0, 2), (0, 1))):
+                state[i]["inv_moment"][ax] = np.float32(1.0) / (mass / np.float32(3.0) * (h[j] * h[j] + h[k2] * h[k2]))
+        state[i]["inv_mass"] = np.float32(1.0) / np.float32(mass)
+    bodies["position"][:n, :3] = pos.astype(np.float32)
+    bodies["orientation"][:n] = q
+    state["vel"][:n, :3] = vel.astype(np.float32)
+    state["friction"] = np.float32(0.8)
+    # ropes: chains of small spheres hanging along -y from the top of the volume
+    cons = []
+    rb = n
+    for r_i in range(ropes):
+        x0 = -extent * 0.9 + (2 * extent * 0.9) * (r_i % 16) / 15.0
+        z0 = -extent * 0.9 + (2 * extent * 0.9) * (r_i // 16) / max(1, (ropes - 1) // 16)
+        for l in range(rope_links):
+            b = rb + r_i * rope_links + l
+            bodies[b]["tag"] = R.SPHERE
+            bodies[b]["corners"][0, 0] = np.float32(0.05)
+            bodies[b]["position"][:3] = (x0, 2 * extent - 1.0 - 0.2 * l, z0)
+            bodies[b]["orientation"][3] = 1.0
+            m = np.float32(0.1)
+            state[b]["inv_mass"] = np.float32(1.0) / m
+            state[b]["inv_moment"][:3] = np.float32(1.0) / (np.float32(0.4) * m * np.float32(0.05) * np.float32(0.05))
+            if l > 0:
+                cons.append((b - 1, b, R.ROPE, 0.2, 1000.0))
+    # springs between grid neighbours (index + 1 along x)
+    a_idx = rng.uniform(0, 1, springs)
+    for s_i in range(springs):
+        a = int(a_idx[s_i] * (n - 1))
+        if (a + 1) % side == 0:
+            a -= 1
+        b = a + 1
+        rest = float(np.linalg.norm(pos[b] - pos[a]))
+        cons.append((a, b, R.SPRING, rest, 1.0e4))
+    cons_arr = np.zeros(len(cons), R.AVBD_CONSTRAINT)
+    for i, (a, b, kind, length, k) in enumerate(cons):
+        cons_arr[i]["a"], cons_arr[i]["b"], cons_arr[i]["kind"] = a, b, kind
+        cons_arr[i]["anchor_a"][3] = length
+        cons_arr[i]["anchor_b"][3] = k
+    hull = np.concatenate(hull_pool).astype(np.float32) if hull_pool else np.zeros((0, 3), np.float32)
+    container = AvbdContainer(center=(0.0, extent, 0.0), rotation=(0.0, 0.0, 0.0, 1.0), half_extent=extent)
+    opts = AvbdOptions(dt=1.0 / 120.0, gravity=(0.0, -9.81, 0.0), iterations=8)
+    return Scene(bodies, state, cons_arr, container, hull, opts, name=f"mixed_field{n}_s{seed}")
+
+
+# ------------------------------------------------------------- wall throw
+def wall_throw(seed: int = 0, columns: int = 64, rows: int = 32) -> Scene:
+    """Bricks 0.5 (z) x 0.25 (y) x 0.25 (x) stacked in a wall in the y-z
+    plane at x = 0; the sphere starts at (-4, 1.5, 0).  Draw order: v0.x,
+    v0.y."""
+    rng = SplitMix64(seed)
+    vx = rng.uniform(13.0, 17.0, 1)[0]
+    vy = rng.uniform(1.5, 2.5, 1)[0]
+    h = np.float32([0.125, 0.125, 0.25])
+    n = columns * rows
+    bodies = np.zeros(n + 1, R.GPU_BODY)
+    state = np.zeros(n + 1, R.BODY_STATE)
+    r_, c_ = np.meshgrid(np.arange(rows), np.arange(columns), indexing="ij")
+    bodies["position"][:n, 0] = 0.0
+    bodies["position"][:n, 1] = (h[1] + 2 * h[1] * r_.ravel()).astype(np.float32)
+    bodies["position"][:n, 2] = ((c_.ravel() - (columns - 1) / 2.0) * 2 * h[2]).astype(np.float32)
+    bodies["orientation"][:, 3] = 1.0
+    bodies["tag"][:n] = R.BOX
+    bodies["corners"][:n, 0, :3] = h
+    mass = np.float32(8.0) * h[0] * h[1] * h[2]
+    state["inv_mass"][:n] = np.float32(1.0) / mass
+    for ax, (j, k) in enumerate(((1, 2), (0, 2), (0, 1))):
+        state["inv_moment"][:n, ax] = np.float32(1.0) / (mass / np.float32(3.0) * (h[j] * h[j] + h[k] * h[k]))
+    # projectile
+    r, m = np.float32(0.5), np.float32(20.0)
+    bodies[n]["tag"] = R.SPHERE
+    bodies[n]["corners"][0, 0] = r
+    bodies[n]["position"][:3] = (-4.0, 1.5, 0.0)
+    state[n]["inv_mass"] = np.float32(1.0) / m
+    state[n]["inv_moment"][:3] = np.float32(1.0) / (np.float32(0.4) * m * r * r)
+    state[n]["vel"][:3] = (vx, vy, 0.0)
+    state["friction"] = np.float32(0.8)
+    opts = AvbdOptions(dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), iterations=6)
+    return Scene(bodies, state, container=floor_container(40.0), options=opts, name=f"wall_throw{n}_s{seed}")
