@@ -44,6 +44,9 @@ SIGNATURES = {
     "dio_world_upload_hulls": (c_int, [_VP, _FP, c_uint32]),
     "dio_step": (c_int, [_VP, POINTER(R.Options), POINTER(R.StepOpts), c_uint32]),
     "dio_step_get_stats": (c_int, [_VP, POINTER(R.StepStats)]),
+    "dio_set_tuning": (c_int, [_VP, POINTER(R.Tuning)]),
+    "dio_get_tuning": (c_int, [_VP, POINTER(R.Tuning)]),
+    "dio_autotune": (c_int, [_VP, POINTER(R.Options), c_uint32, POINTER(R.Tuning), _FP]),
     "dio_world_download": (c_int, [_VP, _FP, _FP, _FP, _FP, _FP]),
     "dio_world_contacts": (c_int, [_VP, _VP, c_uint32, _U32P]),
     "dio_world_pairs": (c_int, [_VP, _VP, c_uint32, _U32P]),

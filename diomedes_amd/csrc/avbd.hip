@@ -299,6 +299,228 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
     v.rot[1 - parity][i] = f4q(new_rot);
 }
 
+// ---------------------------------------------------------------------------
+// Grouped primal: G lanes per body.  A body's contacts and joints ("items")
+// are processed in chunks of G; lane l of the group builds the up-to-3
+// constraint rows of item chunk*G + l (each row: j_lin, j_ang, force,
+// stiffness) and stages them in LDS.  Each lane owns a fixed subset of the 27
+// block entries and adds, for every staged row in item order, that row's
+// term (a[p] * b[q]) * k (or j[p] * f for the gradient) -- the same sequence
+// of roundings per entry as the one-lane-per-body loop, so the result is
+// bit-identical.  The 27 entries are then gathered through LDS and every lane
+// of the group runs the 6x6 LDL^T; lane 0 stores the pose.
+namespace {
+// entry e: term = R[kP[e]] * R[kQ[e]] (* R[7] = k for the Hessian entries);
+// R = (j_lin.xyz, j_ang.xyz, force, stiffness)
+__constant__ int8_t kP[27] = {0, 1, 2, 1, 2, 2, 3, 4, 5, 4, 5, 5, 3, 3, 3, 4, 4, 4, 5, 5, 5, 0, 1, 2, 3, 4, 5};
+__constant__ int8_t kQ[27] = {0, 0, 0, 1, 1, 2, 3, 3, 3, 4, 4, 5, 0, 1, 2, 0, 1, 2, 0, 1, 2, 6, 6, 6, 6, 6, 6};
+constexpr int kRowFloats = 8;
+constexpr int kItemFloats = 3 * kRowFloats + 1;  // 3 rows + row count
+}  // namespace
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t parity) {
+    constexpr int kBodies = kBlock / G;
+    constexpr int kOwned = (27 + G - 1) / G;
+    __shared__ float stage[kBodies][G][kItemFloats];
+    __shared__ float gathered[kBodies][28];
+    const int slot = threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    const uint32_t i = blockIdx.x * kBodies + slot;
+    const float4* __restrict__ rp = v.pos[parity];
+    const float4* __restrict__ rr = v.rot[parity];
+    const float dt = p.dt;
+
+    bool active = i < p.n;
+    float4 imm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (active) {
+        imm = v.inv_mm[i];
+        active = imm.w != 0.0f;
+    }
+    f3 snap_pos{0.0f, 0.0f, 0.0f}, dsl{0.0f, 0.0f, 0.0f}, dsa{0.0f, 0.0f, 0.0f};
+    q4 snap_rot{0.0f, 0.0f, 0.0f, 1.0f};
+    uint32_t jb = 0, ncontacts = 0, kb = 0, njoints = 0;
+    float acc[kOwned];
+    if (active) {
+        snap_pos = xyz(rp[i]);
+        snap_rot = as_q(rr[i]);
+        dsl = sub(snap_pos, xyz(v.init_lin[i]));
+        dsa = ang_diff(snap_rot, as_q(v.init_ang[i]));
+        jb = v.offsets[i];
+        ncontacts = v.offsets[i + 1] - jb;
+        if (v.j_off) {
+            kb = v.j_off[i];
+            njoints = v.j_off[i + 1] - kb;
+        }
+        // initial block: diag(M / dt^2, I / dt^2) and the inertial gradient
+        const float mass = 1.0f / imm.w;
+        const f3 moment{1.0f / imm.x, 1.0f / imm.y, 1.0f / imm.z};
+        const float dt2 = dt * dt;
+        const float m_dt2 = mass / dt2;
+        const f3 i_dt2 = divs(moment, dt2);
+        const f3 dl = sub(snap_pos, xyz(v.iner_lin[i]));
+        const f3 ad = ang_diff(snap_rot, as_q(v.iner_ang[i]));
+#pragma unroll
+        for (int t = 0; t < kOwned; ++t) {
+            const int e = lane + G * t;
+            float x = 0.0f;
+            if (e == 0 || e == 3 || e == 5) x = m_dt2;
+            else if (e == 6) x = i_dt2.x;
+            else if (e == 9) x = i_dt2.y;
+            else if (e == 11) x = i_dt2.z;
+            else if (e == 21) x = m_dt2 * dl.x;
+            else if (e == 22) x = m_dt2 * dl.y;
+            else if (e == 23) x = m_dt2 * dl.z;
+            else if (e == 24) x = i_dt2.x * ad.x;
+            else if (e == 25) x = i_dt2.y * ad.y;
+            else if (e == 26) x = i_dt2.z * ad.z;
+            acc[t] = x;
+        }
+    }
+    const uint32_t nitems = ncontacts + njoints;
+    const float kk = 1.0f - p.alpha;
+    const f3 zero{0.0f, 0.0f, 0.0f};
+
+    for (uint32_t base = 0;; base += G) {
+        // ---- build the rows of item base + lane
+        float* st = stage[slot][lane];
+        int nrows = 0;
+        const uint32_t item = base + (uint32_t)lane;
+        if (active && item < nitems) {
+            if (item < ncontacts) {
+                const uint32_t ci = v.indices[jb + item];
+                const uint2 ab = v.c_ab[ci];
+                const bool is_a = ab.x == i;
+                const uint32_t other = is_a ? ab.y : ab.x;
+                const float4 n4 = v.c_n[ci];
+                const f3 n0 = xyz(n4);
+                const f3 t1 = xyz(v.c_t1[ci]);
+                const f3 t2 = xyz(v.c_t2[ci]);
+                const f3 ra = xyz(v.c_ra[ci]);
+                const f3 rb = xyz(v.c_rb[ci]);
+                const f3 r0 = is_a ? n0 : neg(n0);
+                const f3 r1 = is_a ? t1 : neg(t1);
+                const f3 r2 = is_a ? t2 : neg(t2);
+                const f3 r_world = qrot(snap_rot, is_a ? ra : rb);
+                f3 dol = zero, doa = zero, r_other = zero;
+                if (other != kContainer) {
+                    const q4 orr = as_q(rr[other]);
+                    dol = sub(xyz(rp[other]), xyz(v.init_lin[other]));
+                    doa = ang_diff(orr, as_q(v.init_ang[other]));
+                    r_other = qrot(orr, is_a ? rb : ra);
+                }
+                const f3 j0 = cross3(r_world, r0), j1 = cross3(r_world, r1), j2 = cross3(r_world, r2);
+                const float4 c04 = v.c_c0[ci];
+                float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
+                c0 = c0 + (dot3(r0, dsl) + dot3(j0, dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
+                c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
+                c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
+                const float4 pen = v.c_pen[ci];
+                const Force3 F = contact_force(pen, v.c_lam[ci], n4.w, c0, c1, c2);
+                const f3 jl[3] = {r0, r1, r2};
+                const f3 ja[3] = {j0, j1, j2};
+                const float fr[3] = {F.f0, F.f1, F.f2};
+                const float kr[3] = {pen.x, pen.y, pen.z};
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    st[8 * r + 0] = jl[r].x; st[8 * r + 1] = jl[r].y; st[8 * r + 2] = jl[r].z;
+                    st[8 * r + 3] = ja[r].x; st[8 * r + 4] = ja[r].y; st[8 * r + 5] = ja[r].z;
+                    st[8 * r + 6] = fr[r]; st[8 * r + 7] = kr[r];
+                }
+                nrows = 3;
+            } else {
+                const uint32_t cidx = v.j_idx[kb + (item - ncontacts)];
+                const uint4 abk = v.j_abk[cidx];
+                const float4 aa4 = v.j_anc_a[cidx];
+                const float4 ab4 = v.j_anc_b[cidx];
+                const bool is_a = abk.x == i;
+                const f3 aa = xyz(aa4), abn = xyz(ab4);
+                const f3 pa = is_a ? snap_pos : xyz(rp[abk.x]);
+                const f3 pb = !is_a ? snap_pos : xyz(rp[abk.y]);
+                const q4 qa = is_a ? snap_rot : as_q(rr[abk.x]);
+                const q4 qb = !is_a ? snap_rot : as_q(rr[abk.y]);
+                const f3 delta = sub(add(pa, qrot(qa, aa)), add(pb, qrot(qb, abn)));
+                const f3 r_self = qrot(snap_rot, is_a ? aa : abn);
+                const float sign = is_a ? 1.0f : -1.0f;
+                const float stiff = ab4.w;
+                if (abk.z == kBallSocket) {
+                    const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
+                                      scale(f3{0.0f, 0.0f, 1.0f}, sign)};
+                    const float fs[3] = {delta.x * stiff, delta.y * stiff, delta.z * stiff};
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        const f3 ja = cross3(r_self, js[r]);
+                        st[8 * r + 0] = js[r].x; st[8 * r + 1] = js[r].y; st[8 * r + 2] = js[r].z;
+                        st[8 * r + 3] = ja.x; st[8 * r + 4] = ja.y; st[8 * r + 5] = ja.z;
+                        st[8 * r + 6] = fs[r]; st[8 * r + 7] = stiff;
+                    }
+                    nrows = 3;
+                } else {
+                    const float len = len3(delta);
+                    if (len > 1.0e-6f && (abk.z != kRope || len > aa4.w)) {
+                        const f3 jl = scale(divs(delta, len), sign);
+                        const f3 ja = cross3(r_self, jl);
+                        st[0] = jl.x; st[1] = jl.y; st[2] = jl.z;
+                        st[3] = ja.x; st[4] = ja.y; st[5] = ja.z;
+                        st[6] = stiff * (len - aa4.w); st[7] = stiff;
+                        nrows = 1;
+                    }
+                }
+            }
+        }
+        st[24] = __int_as_float(nrows);
+        __syncthreads();
+        // ---- accumulate the owned entries over the staged rows, in item order
+        if (active) {
+            const int nin = (int)min((uint32_t)G, nitems > base ? nitems - base : 0u);
+            for (int c = 0; c < nin; ++c) {
+                const float* it = stage[slot][c];
+                const int nr = __float_as_int(it[24]);
+                for (int r = 0; r < nr; ++r) {
+                    const float* R = it + 8 * r;
+#pragma unroll
+                    for (int t = 0; t < kOwned; ++t) {
+                        const int e = lane + G * t;
+                        if (e < 27) {
+                            float term = R[kP[e]] * R[kQ[e]];
+                            if (e < 21) term = term * R[7];
+                            acc[t] = acc[t] + term;
+                        }
+                    }
+                }
+            }
+        }
+        const bool more = active && base + G < nitems;
+        if (!__syncthreads_or(more)) break;
+    }
+    // ---- gather the 27 entries and solve
+    if (active) {
+#pragma unroll
+        for (int t = 0; t < kOwned; ++t) {
+            const int e = lane + G * t;
+            if (e < 27) gathered[slot][e] = acc[t];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const float* gs = gathered[slot];
+    Block6 B;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { B.l[k] = gs[k]; B.a[k] = gs[6 + k]; }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) B.c[k] = gs[12 + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { B.rl[k] = gs[21 + k]; B.ra[k] = gs[24 + k]; }
+    if (lane == 0) {
+        float x[6];
+        solve6(B, x);
+        const f3 new_pos = add(snap_pos, f3{x[0], x[1], x[2]});
+        const q4 new_rot = quat_step(snap_rot, f3{x[3], x[4], x[5]});
+        v.pos[1 - parity][i] = f4(new_pos, 0.0f);
+        v.rot[1 - parity][i] = f4q(new_rot);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_avbd_dual(SolveView v, AvbdParams p, uint32_t parity) {
     const uint32_t m = v.m_dev ? *v.m_dev : v.m_host;
     const float4* __restrict__ wp = v.pos[1 - parity];
@@ -364,9 +586,24 @@ void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t s) {
     if (p.n == 0) return;
     hipLaunchKernelGGL(k_avbd_init, dim3(blocks_for(p.n)), dim3(kBlock), 0, s, v, p);
 }
-void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t s) {
+void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t s, int lanes) {
     if (p.n == 0) return;
-    hipLaunchKernelGGL(k_avbd_primal, dim3(blocks_for(p.n)), dim3(kBlock), 0, s, v, p, parity);
+    switch (lanes) {
+        case 4:
+            hipLaunchKernelGGL(k_avbd_primal_grouped<4>, dim3((p.n + kBlock / 4 - 1) / (kBlock / 4)), dim3(kBlock), 0, s,
+                               v, p, parity);
+            break;
+        case 8:
+            hipLaunchKernelGGL(k_avbd_primal_grouped<8>, dim3((p.n + kBlock / 8 - 1) / (kBlock / 8)), dim3(kBlock), 0, s,
+                               v, p, parity);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_avbd_primal_grouped<16>, dim3((p.n + kBlock / 16 - 1) / (kBlock / 16)), dim3(kBlock),
+                               0, s, v, p, parity);
+            break;
+        default:
+            hipLaunchKernelGGL(k_avbd_primal, dim3(blocks_for(p.n)), dim3(kBlock), 0, s, v, p, parity);
+    }
 }
 void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t m_grid, hipStream_t s) {
     uint32_t blocks = blocks_for(m_grid == 0 ? 1 : m_grid);

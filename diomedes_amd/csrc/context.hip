@@ -209,6 +209,7 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
+    dio_tuning tuning{8, {0, 0, 0, 0, 0, 0, 0}};
     uint32_t a_n = 0, a_m = 0, a_iters = 0;
     bool a_submitted = false;
 
@@ -440,14 +441,14 @@ struct KernelTimer {
 };
 
 void schedule_avbd(const SolveView& v, const AvbdParams& p, uint32_t iters, uint32_t m_grid, hipStream_t s,
-                   KernelTimer* kt = nullptr) {
+                   KernelTimer* kt = nullptr, int lanes = 8) {
     auto run = [&](int kind, auto&& launch) {
         if (kt) kt->time(kind, s, launch);
         else launch();
     };
     run(0, [&] { launch_avbd_init(v, p, s); });
     for (uint32_t k = 0; k < iters; ++k) {
-        run(1, [&] { launch_avbd_primal(v, p, k % 2, s); });
+        run(1, [&] { launch_avbd_primal(v, p, k % 2, s, lanes); });
         run(2, [&] { launch_avbd_dual(v, p, k % 2, m_grid, s); });
     }
     run(3, [&] { launch_avbd_recover(v, p, (iters - 1) % 2, s); });
@@ -758,7 +759,7 @@ dio_status dio_avbd_submit(dio_ctx* ctx, const dio_avbd_body_state* state, const
         v.out_rot = nullptr;
         const AvbdParams p = params_of(options->dt, options->gravity, options->alpha, options->beta_lin,
                                        options->penalty_max, n);
-        schedule_avbd(v, p, options->iterations, m, s);
+        schedule_avbd(v, p, options->iterations, m, s, nullptr, (int)ctx->tuning.primal_lanes);
         DIO_HIP_CHECK(hipGetLastError());
         return DIO_OK;
     });
@@ -859,6 +860,8 @@ NarrowSource source_of(const dio_ctx::Pending& p) {
     return NarrowSource{p.pairs, p.out.off, p.p_dev, p.out.n, p.out.pa, p.out.pb};
 }
 
+SolveView world_solve_view(dio_ctx* c, int s);
+
 // Enqueue segment `seg` of one step with parity s (0: contacts + broad +
 // narrow, 1: AVBD).
 void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, const dio_step_opts& so,
@@ -889,6 +892,17 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         run_csr(c->cs[s], n, c->csr, c->w_overflow, st);
         if (lag) broad_narrow();
     } else {
+        SolveView v = world_solve_view(c, s);
+        const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
+        schedule_avbd(v, p, o.iterations, c->m_cap, st, kt, (int)c->tuning.primal_lanes);
+        const int bn_pend = so.contact_lag ? 1 - s : s;
+        hipLaunchKernelGGL(k_stats_accum, dim3(1), dim3(64), 0, st, (const uint32_t*)c->pend[bn_pend].p_dev,
+                           (const uint32_t*)c->cs[s].m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
+    }
+}
+
+SolveView world_solve_view(dio_ctx* c, int s) {
+    {
         SolveView v{};
         v.cur_pos = c->w_pos;
         v.cur_rot = c->w_rot;
@@ -908,11 +922,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         v.j_anc_a = c->w_j.anc_a; v.j_anc_b = c->w_j.anc_b;
         v.out_pos = c->w_pos;
         v.out_rot = c->w_rot;
-        const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
-        schedule_avbd(v, p, o.iterations, c->m_cap, st, kt);
-        const int bn_pend = so.contact_lag ? 1 - s : s;
-        hipLaunchKernelGGL(k_stats_accum, dim3(1), dim3(64), 0, st, (const uint32_t*)c->pend[bn_pend].p_dev,
-                           (const uint32_t*)cs.m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
+        return v;
     }
 }
 
@@ -1110,6 +1120,70 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
             if (terr) throw StatusError{DIO_HIP, "LBVH build / traversal error (malformed tree or stack overflow)", 0};
             if (w[4]) capacity("a step exceeded max_pairs / max_contacts; results are truncated", w[4]);
         }
+        return DIO_OK;
+    });
+}
+
+dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
+    return guarded(ctx, [&]() {
+        if (!t) invalid("tuning is null");
+        dio_tuning v = *t;
+        if (v.primal_lanes == 0) v.primal_lanes = 8;
+        if (v.primal_lanes != 1 && v.primal_lanes != 4 && v.primal_lanes != 8 && v.primal_lanes != 16)
+            invalid("primal_lanes must be 1, 4, 8 or 16");
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->tuning = v;
+        ctx->drop_graphs();
+        return DIO_OK;
+    });
+}
+
+dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out) {
+    return guarded(ctx, [&]() {
+        if (!out) invalid("out is null");
+        *out = ctx->tuning;
+        return DIO_OK;
+    });
+}
+
+dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t reps, dio_tuning* chosen,
+                        float* us_per_launch) {
+    return guarded(ctx, [&]() {
+        if (!options) invalid("options is null");
+        if (!ctx->w_ready || ctx->w_step == 0) throw StatusError{DIO_NOT_READY, "run at least one dio_step first", 0};
+        if (reps == 0) reps = 20;
+        hipStream_t s = ctx->stream;
+        DIO_HIP_CHECK(hipStreamSynchronize(s));
+        const int par = (int)((ctx->w_step - 1) % 2);
+        const SolveView v = world_solve_view(ctx, par);
+        const AvbdParams p = params_of(options->dt, options->gravity, options->alpha, options->beta_lin,
+                                       options->penalty_max, ctx->w_n);
+        const int shapes[4] = {1, 4, 8, 16};
+        float best = 3.4e38f;
+        int best_lanes = 8;
+        hipEvent_t e0, e1;
+        DIO_HIP_CHECK(hipEventCreate(&e0));
+        DIO_HIP_CHECK(hipEventCreate(&e1));
+        for (int k = 0; k < 4; ++k) {
+            launch_avbd_primal(v, p, 0, s, shapes[k]);  // warm the code path
+            DIO_HIP_CHECK(hipEventRecord(e0, s));
+            for (uint32_t r = 0; r < reps; ++r) launch_avbd_primal(v, p, 0, s, shapes[k]);
+            DIO_HIP_CHECK(hipEventRecord(e1, s));
+            DIO_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            DIO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            const float us = 1000.0f * ms / (float)reps;
+            if (us_per_launch) us_per_launch[k] = us;
+            if (us < best) {
+                best = us;
+                best_lanes = shapes[k];
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->tuning.primal_lanes = (uint32_t)best_lanes;
+        ctx->drop_graphs();
+        if (chosen) *chosen = ctx->tuning;
         return DIO_OK;
     });
 }

@@ -65,7 +65,8 @@ struct SolveView {
 
 // Launchers (avbd.hip).  All run on `stream`.
 void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t stream);
-void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream);
+// lanes: lanes per body (1 = one lane per body; 4, 8, 16 = grouped kernel)
+void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream, int lanes = 8);
 void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t m_grid, hipStream_t stream);
 void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream);
 

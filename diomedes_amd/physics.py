@@ -465,6 +465,25 @@ class PhysicsContext:
         o = options.to_c()
         self._check(self._lib.dio_step(self._h, ctypes.byref(o), ctypes.byref(so), nsteps))
 
+    def set_tuning(self, primal_lanes: int = 8) -> None:
+        t = R.Tuning()
+        t.primal_lanes = primal_lanes
+        self._check(self._lib.dio_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self) -> dict:
+        t = R.Tuning()
+        self._check(self._lib.dio_get_tuning(self._h, ctypes.byref(t)))
+        return {"primal_lanes": t.primal_lanes}
+
+    def autotune(self, options: AvbdOptions, reps: int = 20) -> dict:
+        """Time every primal launch shape on the current contacts, keep the
+        fastest; returns the choice and the per-shape microseconds."""
+        t = R.Tuning()
+        us = (ctypes.c_float * 4)()
+        o = options.to_c()
+        self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
+        return {"primal_lanes": t.primal_lanes, "us_per_launch": dict(zip((1, 4, 8, 16), list(us)))}
+
     def stats(self) -> R.StepStats:
         s = R.StepStats()
         self._check(self._lib.dio_step_get_stats(self._h, ctypes.byref(s)))

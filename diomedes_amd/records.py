@@ -170,6 +170,10 @@ class StepStats(ctypes.Structure):
 assert ctypes.sizeof(StepStats) == 88
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("primal_lanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7)]
+
+
 def f32(x) -> float:
     """Round a Python float to the nearest float32 (what the ABI receives)."""
     return float(np.float32(x))

@@ -201,6 +201,12 @@ typedef struct dio_step_stats {
     uint32_t n_init, n_primal, n_dual, n_recover;
 } dio_step_stats;
 
+/* Kernel launch shapes (the autotuner's search space). */
+typedef struct dio_tuning {
+    uint32_t primal_lanes;  /* lanes per body in the primal: 1, 4, 8 or 16 (0 = default 8) */
+    uint32_t reserved[7];
+} dio_tuning;
+
 typedef struct dio_ctx dio_ctx;
 
 /* ---- library / context ------------------------------------------------ */
@@ -262,6 +268,16 @@ dio_status dio_world_upload_hulls(dio_ctx* ctx, const float* vertices, uint32_t 
 dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_step_opts* opts,
                     uint32_t nsteps);
 dio_status dio_step_get_stats(dio_ctx* ctx, dio_step_stats* out);
+/* Launch shapes; changing them drops the captured step graphs. */
+dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* tuning);
+dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out);
+/* Autotuner: after at least one dio_step, time every primal shape on the
+   current contact set (reps launches each, HIP events), keep the fastest, and
+   report the per-shape average microseconds in us_per_launch[0..3] (lanes 1,
+   4, 8, 16).  The primal with a fixed parity is idempotent, so the world state
+   is unchanged. */
+dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t reps, dio_tuning* chosen,
+                        float* us_per_launch);
 /* Download body state (any pointer may be null): n x 3 / n x 4 floats. */
 dio_status dio_world_download(dio_ctx* ctx, float* positions, float* orientations,
                               float* velocities, float* angular_velocities,

@@ -59,6 +59,7 @@ def _c_sizeof(type_name: str) -> int:
     ("dio_run_options", ctypes.sizeof(R.RunOptions)), ("dio_avbd_options", ctypes.sizeof(R.Options)),
     ("dio_container", ctypes.sizeof(R.Container)), ("dio_limits", ctypes.sizeof(R.Limits)),
     ("dio_step_opts", ctypes.sizeof(R.StepOpts)), ("dio_step_stats", ctypes.sizeof(R.StepStats)),
+    ("dio_tuning", ctypes.sizeof(R.Tuning)),
 ])
 def test_record_sizes_match_the_header(ctype, size):
     assert _c_sizeof(ctype) == size

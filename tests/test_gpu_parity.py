@@ -104,7 +104,8 @@ def test_narrow_box_manifolds_bitexact(oracle):
 
 
 # ------------------------------------------------------------ AVBD solve
-def test_avbd_granular_solve_bitexact(oracle):
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
+def test_avbd_granular_solve_bitexact(oracle, lanes):
     scene, w, _ = _settled_field(oracle, steps=30)
     # the oracle world's current contacts / CSR / state feed one granular solve
     c = w.contacts()
@@ -121,13 +122,15 @@ def test_avbd_granular_solve_bitexact(oracle):
     _print("granular avbd contacts", len(con))
     ref = oracle.avbd_run(state, pos, rot, con, scene.constraints, off, idx, opts.run_options())
     with PhysicsContext(scene.n) as ctx:
+        ctx.set_tuning(primal_lanes=lanes)
         ctx.gpu_physics_submit(state, pos, rot, con, scene.constraints, off, idx, opts)
         got = ctx.gpu_physics_read()
     for k in ("positions", "orientations", "velocities", "angular_velocities", "prev_velocities", "lambda", "penalty"):
         assert_bits(got[k], ref[k], k)
 
 
-def test_avbd_granular_joints_bitexact(oracle):
+@pytest.mark.parametrize("lanes", [1, 8])
+def test_avbd_granular_joints_bitexact(oracle, lanes):
     """Springs, rods, a ball socket and a rope (the constraints demo's set,
     demos/constraints/src/main.rs:110-146) through one solve."""
     from diomedes_amd.physics import AvbdBody, BallSocket, Rod, Rope, Scene, Spring
@@ -149,6 +152,7 @@ def test_avbd_granular_joints_bitexact(oracle):
     scene = Scene.from_bodies(bodies, cons, None, opts)
     w = oracle.OracleWorld(scene, 32)
     with PhysicsContext(scene.n) as ctx:
+        ctx.set_tuning(primal_lanes=lanes)
         ctx.upload(scene)
         for k in range(4):
             w.step(opts, nsteps=30)
@@ -187,16 +191,34 @@ def test_box_stack_300_steps_bitexact(oracle):
         assert np.array_equal(np.stack([pairs["a"], pairs["b"]], 1).reshape(-1, 2), ref_pairs.reshape(-1, 2))
 
 
-@pytest.mark.parametrize("lag", [1, 0])
-def test_cube_field_small_bitexact(oracle, lag):
+@pytest.mark.parametrize("lag,lanes", [(1, 8), (0, 8), (1, 4), (1, 16), (1, 1)])
+def test_cube_field_small_bitexact(oracle, lag, lanes):
     scene = scenes.cube_field(7, nx=8, ny=2, nz=8)
     w = oracle.OracleWorld(scene, 32)
     with PhysicsContext(scene.n) as ctx:
+        ctx.set_tuning(primal_lanes=lanes)
         ctx.upload(scene)
         for k in range(6):
             ctx.step(scene.options, 20, lag=lag)
             w.step(nsteps=20, lag=lag)
             _compare_world(ctx, w, 20 * (k + 1))
+
+
+def test_autotune_keeps_state_and_parity(oracle):
+    """The autotuner times the primal shapes without changing the world."""
+    scene = scenes.cube_field(11, nx=8, ny=2, nz=8)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        ctx.step(scene.options, 40)
+        w.step(nsteps=40)
+        res = ctx.autotune(scene.options, reps=5)
+        _print("autotune", res)
+        assert res["primal_lanes"] in (1, 4, 8, 16)
+        _compare_world(ctx, w, 40)
+        ctx.step(scene.options, 20)
+        w.step(nsteps=20)
+        _compare_world(ctx, w, 60)
 
 
 def test_cube_field_8192_first_steps_bitexact(oracle):
@@ -233,3 +255,86 @@ def test_box_stack_matches_golden_fixture():
         # error against the FP64 oracle, stated
         p64 = g["fp64_positions"][-1]
         _print("box stack: max |x_gpu - x_fp64| after 300 steps =", float(np.max(np.abs(got["positions"] - p64))))
+
+
+# ------------------------------------------------------------ convex shapes
+def _convex_pair_bodies(seed: int, count: int):
+    """Random overlapping pairs of tetrahedra, hulls, spheres and boxes."""
+    from diomedes_amd.physics import AvbdBody, Shape, gpu_bodies
+
+    rng = np.random.default_rng(seed)
+    ico = scenes.icosphere(1).astype(np.float32)
+    hull = []
+    bodies = []
+
+    def shape(kind):
+        if kind == 0:
+            return Shape.tetrahedron(scenes.regular_tetrahedron(rng.uniform(0.5, 1.0)) @ _rot(rng).T)
+        if kind == 1:
+            r = rng.uniform(0.3, 0.6)
+            hull.append(ico * np.float32(r))
+            return Shape.hull(42 * (len(hull) - 1), 42)
+        if kind == 2:
+            return Shape.sphere(rng.uniform(0.2, 0.6))
+        return Shape.box(rng.uniform(0.2, 0.5, 3))
+
+    for k in range(count):
+        ka, kb = rng.integers(0, 4, 2)
+        if ka == 2 and kb == 2 or {ka, kb} == {3}:
+            ka = 0
+        q = rng.normal(size=(2, 4))
+        q /= np.linalg.norm(q, axis=1, keepdims=True)
+        pa = rng.normal(size=3)
+        bodies.append(AvbdBody(shape(ka), tuple(pa), tuple(q[0])))
+        bodies.append(AvbdBody(shape(kb), tuple(pa + rng.normal(size=3) * 0.4), tuple(q[1])))
+    hull = np.concatenate(hull) if hull else np.zeros((0, 3), np.float32)
+    return gpu_bodies(bodies), hull
+
+
+def _rot(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_narrow_convex_gjk_epa_bitexact(oracle, seed):
+    bodies, hull = _convex_pair_bodies(seed, 300)
+    n = len(bodies)
+    pairs = np.zeros(n // 2, R.GPU_PAIR)
+    pairs["a"] = np.arange(0, n, 2)
+    pairs["b"] = np.arange(1, n, 2)
+    with PhysicsContext(n, max_hull_vertices=max(len(hull), 1)) as ctx:
+        if len(hull):
+            from diomedes_amd.physics import Scene
+            from diomedes_amd import records as RR
+
+            st = np.zeros(n, RR.BODY_STATE)
+            ctx.upload(Scene(bodies, st, hull=hull))  # the hull pool is shared with the granular pipeline
+        raw = ctx.narrow_phase_run(bodies, pairs)
+    hits = 0
+    for i in range(len(pairs)):
+        a, b = int(pairs[i]["a"]), int(pairs[i]["b"])
+        ref = oracle.narrow_pair(bodies[a], bodies[b], hull=hull)
+        slots = raw[4 * i:4 * i + 4]
+        assert int(np.sum(slots["valid"])) == len(ref), (i, bodies[a]["tag"], bodies[b]["tag"])
+        for k, c in enumerate(ref):
+            assert_bits(slots[k]["normal"][:3], c["normal"], f"normal {i}")
+            assert_bits(slots[k]["depth"], c["depth"], f"depth {i}")
+            assert_bits(slots[k]["point_a"][:3], c["point_a"], f"pa {i}")
+            assert_bits(slots[k]["point_b"][:3], c["point_b"], f"pb {i}")
+            hits += 1
+    _print("convex contacts", hits)
+    assert hits > 50
+
+
+def test_mixed_field_small_bitexact(oracle):
+    """Hulls, tetrahedra, boxes, ropes and springs through the fused step."""
+    scene = scenes.mixed_field(5, n=300, ropes=4, rope_links=8, springs=16, extent=8.0)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n, max_hull_vertices=len(scene.hull), max_constraints=len(scene.constraints)) as ctx:
+        ctx.upload(scene)
+        for k in range(4):
+            ctx.step(scene.options, 15)
+            w.step(nsteps=15)
+            _compare_world(ctx, w, 15 * (k + 1))
+            _print("mixed step", 15 * (k + 1), "contacts", ctx.stats().contacts)
