@@ -110,170 +110,200 @@ __global__ void __launch_bounds__(kBlock) k_morton(const float4* mn, const float
     vals[i] = i;
 }
 
-// Karras: common-prefix length of sorted keys i and j, ties broken by index
-__device__ __forceinline__ int delta(const uint32_t* codes, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    const uint32_t ki = codes[i], kj = codes[j];
-    if (ki == kj) return 32 + __clz((uint32_t)(i ^ j));
-    return __clz(ki ^ kj);
-}
+// ---------------------------------------------------------------------------
+// Implicit LBVH: a complete binary tree in heap layout over the bodies in
+// Morton order (perm[p] = body at sorted position p).  With N2 = the next
+// power of two >= n, node 0 is the root, node i has children 2i+1, 2i+2, and
+// sorted position p is the leaf N2 - 1 + p; positions p >= n are empty leaves
+// (inverted boxes that overlap nothing).  Boxes are refit bottom-up by
+// LDS reductions: chunks of C leaves per workgroup, then one workgroup for the
+// top levels -- no atomics, no inter-workgroup fences.  Leaf boxes are copies
+// of the bodies' AABBs, so the leaf test is the reference's exact closed-
+// interval test and the pair set equals the dense sweep's for any tree shape.
 
-__global__ void __launch_bounds__(kBlock) k_lbvh_build(const uint32_t* codes, int n, uint32_t* left,
-                                                       uint32_t* right, uint32_t* parent_int,
-                                                       uint32_t* parent_leaf) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n - 1) return;
-    const int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
-    const int dmin = delta(codes, n, i, i - d);
-    int lmax = 2;
-    while (delta(codes, n, i, i + lmax * d) > dmin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
-    const int j = i + l * d;
-    const int dnode = delta(codes, n, i, j);
-    int s = 0;
-    int t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (delta(codes, n, i, i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    const int gamma = i + s * d + (d < 0 ? d : 0);
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    uint32_t lc, rc;
-    if (lo == gamma) {
-        lc = (uint32_t)gamma | kLeafBit;
-        parent_leaf[gamma] = (uint32_t)i;
-    } else {
-        lc = (uint32_t)gamma;
-        parent_int[gamma] = (uint32_t)i;
-    }
-    if (hi == gamma + 1) {
-        rc = (uint32_t)(gamma + 1) | kLeafBit;
-        parent_leaf[gamma + 1] = (uint32_t)i;
-    } else {
-        rc = (uint32_t)(gamma + 1);
-        parent_int[gamma + 1] = (uint32_t)i;
-    }
-    left[i] = lc;
-    right[i] = rc;
-    if (i == 0) parent_int[0] = kNone;
-}
+__device__ __forceinline__ int ilog2(uint32_t x) { return 31 - __clz(x); }
 
-__device__ __forceinline__ float ld_coherent(const float* p) {
-    return __builtin_bit_cast(float, __hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ __forceinline__ void child_box(uint32_t c, const uint32_t* vals, const float4* mn, const float4* mx,
-                                          const float4* nmn, const float4* nmx, float4& a, float4& b) {
-    if (c & kLeafBit) {
-        const uint32_t body = vals[c & ~kLeafBit];
-        a = mn[body];
-        b = mx[body];
-    } else {
-        const float* pa = reinterpret_cast<const float*>(nmn + c);
-        const float* pb = reinterpret_cast<const float*>(nmx + c);
-        a = make_float4(ld_coherent(pa), ld_coherent(pa + 1), ld_coherent(pa + 2), 0.0f);
-        b = make_float4(ld_coherent(pb), ld_coherent(pb + 1), ld_coherent(pb + 2), 0.0f);
-    }
-}
-
-// bottom-up refit: the second child to arrive at a node computes its box
-__global__ void __launch_bounds__(kBlock) k_lbvh_refit(int n, const uint32_t* vals, const float4* mn,
-                                                       const float4* mx, const uint32_t* left, const uint32_t* right,
-                                                       const uint32_t* parent_int, const uint32_t* parent_leaf,
-                                                       uint32_t* flags, float4* nmn, float4* nmx, uint32_t* err) {
-    const int leaf = blockIdx.x * kBlock + threadIdx.x;
-    if (leaf >= n) return;
-    uint32_t node = parent_leaf[leaf];
-    // a tree of n leaves is at most n - 1 levels deep; any longer walk or
-    // out-of-range index means the build went wrong: flag it and stop
-    for (int level = 0; node != kNone; ++level) {
-        if (node >= (uint32_t)(n - 1) || level >= n) {
-            atomicOr(err, 2u);
-            return;
+// leaves and the lower levels of one chunk of C <= 1024 sorted positions
+constexpr uint32_t kMaxChunk = 1024;
+__global__ void __launch_bounds__(256) k_bvh_lower(const uint32_t* perm, uint32_t n, uint32_t n2, uint32_t chunk,
+                                                   const float4* mn, const float4* mx, float4* nmn, float4* nmx) {
+    __shared__ float4 lo[kMaxChunk], hi[kMaxChunk];
+    const uint32_t base = blockIdx.x * chunk;
+    const uint32_t leaf0 = n2 - 1;
+    for (uint32_t t = threadIdx.x; t < chunk; t += 256) {
+        const uint32_t p = base + t;
+        float4 a = make_float4(3.4e38f, 3.4e38f, 3.4e38f, 0.0f), b = make_float4(-3.4e38f, -3.4e38f, -3.4e38f, 0.0f);
+        if (p < n) {
+            const uint32_t body = perm[p];
+            a = mn[body];
+            b = mx[body];
         }
-        __threadfence();
-        const uint32_t old = atomicAdd(&flags[node], 1u);
-        if (old == 0) return;
-        __threadfence();
-        float4 a0, a1, b0, b1;
-        child_box(left[node], vals, mn, mx, nmn, nmx, a0, a1);
-        child_box(right[node], vals, mn, mx, nmn, nmx, b0, b1);
-        nmn[node] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.0f);
-        nmx[node] = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.0f);
-        node = parent_int[node];
+        lo[t] = a;
+        hi[t] = b;
+        nmn[leaf0 + p] = a;
+        nmx[leaf0 + p] = b;
+    }
+    __syncthreads();
+    // level above the leaves: `width` nodes of this chunk starting at heap index
+    // level_first + blockIdx.x * width
+    uint32_t width = chunk >> 1;
+    uint32_t level_first = (leaf0 + 1) / 2 - 1;
+    while (width >= 1) {
+        float4 r0[kMaxChunk / 512], r1[kMaxChunk / 512];
+        uint32_t k = 0;
+        for (uint32_t t = threadIdx.x; t < width; t += 256, ++k) {
+            const float4 a0 = lo[2 * t], a1 = hi[2 * t], b0 = lo[2 * t + 1], b1 = hi[2 * t + 1];
+            r0[k] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.0f);
+            r1[k] = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.0f);
+            nmn[level_first + blockIdx.x * width + t] = r0[k];
+            nmx[level_first + blockIdx.x * width + t] = r1[k];
+        }
+        __syncthreads();
+        k = 0;
+        for (uint32_t t = threadIdx.x; t < width; t += 256, ++k) {
+            lo[t] = r0[k];
+            hi[t] = r1[k];
+        }
+        __syncthreads();
+        width >>= 1;
+        level_first = (level_first + 1) / 2 - 1;
     }
 }
 
-// Traverse the tree for body `a`; call emit(b) for each overlapping body b > a.
-// Returns false if the stack would overflow (the tree is never that deep for
-// 30-bit codes with index tie-breaks; the check keeps every access in bounds).
+// the top levels: reduce the n2 / chunk chunk roots to the root (one workgroup)
+__global__ void __launch_bounds__(1024) k_bvh_top(uint32_t n2, uint32_t chunk, float4* nmn, float4* nmx) {
+    __shared__ float4 lo[1024], hi[1024];
+    uint32_t width = n2 / chunk;  // number of chunk roots
+    if (width <= 1) return;
+    uint32_t first = width - 1;  // heap index of the first chunk root
+    for (uint32_t t = threadIdx.x; t < width; t += 1024) {
+        lo[t] = nmn[first + t];
+        hi[t] = nmx[first + t];
+    }
+    __syncthreads();
+    while (width > 1) {
+        width >>= 1;
+        first = (first + 1) / 2 - 1;
+        float4 u0, u1;
+        const bool act = threadIdx.x < width;
+        if (act) {
+            const float4 a0 = lo[2 * threadIdx.x], a1 = hi[2 * threadIdx.x];
+            const float4 b0 = lo[2 * threadIdx.x + 1], b1 = hi[2 * threadIdx.x + 1];
+            u0 = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.0f);
+            u1 = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.0f);
+            nmn[first + threadIdx.x] = u0;
+            nmx[first + threadIdx.x] = u1;
+        }
+        __syncthreads();
+        if (act) {
+            lo[threadIdx.x] = u0;
+            hi[threadIdx.x] = u1;
+        }
+        __syncthreads();
+    }
+}
+
+constexpr uint32_t kTopCached = 511;  // nodes of the top 9 levels kept in LDS
+constexpr int kTravBlock = 128;
+
+// Traverse the subtree of one start node for body a; emit(b) for each
+// overlapping body b > a.  False if the stack would overflow (never for the
+// <= 22-level trees this library builds; the check keeps accesses in bounds).
 template <class Emit>
-__device__ __forceinline__ bool traverse(uint32_t a, int n, const uint32_t* vals, const float4* mn, const float4* mx,
-                                         const uint32_t* left, const uint32_t* right, const float4* nmn,
-                                         const float4* nmx, Emit&& emit) {
-    if (n < 2) return true;
+__device__ __forceinline__ bool traverse_sub(uint32_t a, uint32_t start, uint32_t n, uint32_t n2, const uint32_t* perm,
+                                             const float4* mn, const float4* mx, const float4* nmn,
+                                             const float4* nmx, const float4* clo, const float4* chi,
+                                             uint32_t ncached, Emit&& emit) {
     const float4 qa = mn[a], qb = mx[a];
+    const uint32_t leaf0 = n2 - 1;
     uint32_t stack[kStack];
     int sp = 0;
-    stack[sp++] = 0;
+    auto test = [&](uint32_t node) -> bool {
+        const float4 lo = node < ncached ? clo[node] : nmn[node];
+        const float4 hi = node < ncached ? chi[node] : nmx[node];
+        return aabb_overlap(qa, qb, lo, hi);
+    };
+    auto visit = [&](uint32_t node) -> bool {
+        if (node >= leaf0) {
+            const uint32_t p = node - leaf0;
+            if (p < n) {
+                const uint32_t b = perm[p];
+                if (b > a && aabb_overlap(qa, qb, mn[b], mx[b])) emit(b);
+            }
+            return true;
+        }
+        if (!test(node)) return true;
+        if (sp >= kStack) return false;
+        stack[sp++] = node;
+        return true;
+    };
+    if (!visit(start)) return false;
     while (sp > 0) {
         const uint32_t node = stack[--sp];
-        const uint32_t ch[2] = {left[node], right[node]};
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const uint32_t x = ch[c];
-            if (x & kLeafBit) {
-                if ((x & ~kLeafBit) >= (uint32_t)n) return false;
-                const uint32_t b = vals[x & ~kLeafBit];
-                if (b >= (uint32_t)n) return false;
-                if (b > a && aabb_overlap(qa, qb, mn[b], mx[b])) emit(b);
-            } else {
-                if (x >= (uint32_t)(n - 1)) return false;
-                if (!aabb_overlap(qa, qb, nmn[x], nmx[x])) continue;
-                if (sp >= kStack) return false;
-                stack[sp++] = x;
-            }
-        }
+        if (!visit(2 * node + 1)) return false;
+        if (!visit(2 * node + 2)) return false;
     }
     return true;
 }
 
-__global__ void __launch_bounds__(kBlock) k_pairs_count(int n, const uint32_t* vals, const float4* mn,
-                                                        const float4* mx, const uint32_t* left, const uint32_t* right,
-                                                        const float4* nmn, const float4* nmx, uint32_t* counts,
-                                                        uint32_t* err) {
-    const int a = blockIdx.x * kBlock + threadIdx.x;
-    if (a >= n) return;
-    uint32_t c = 0;
-    if (!traverse((uint32_t)a, n, vals, mn, mx, left, right, nmn, nmx, [&](uint32_t) { ++c; })) atomicOr(err, 1u);
-    counts[a] = c;
-}
-
-__global__ void __launch_bounds__(kBlock) k_pairs_write(int n, const uint32_t* vals, const float4* mn,
-                                                        const float4* mx, const uint32_t* left, const uint32_t* right,
-                                                        const float4* nmn, const float4* nmx, const uint32_t* offsets,
-                                                        uint32_t p_max, uint2* pairs) {
-    const int a = blockIdx.x * kBlock + threadIdx.x;
-    if (a >= n) return;
-    const uint32_t base = offsets[a];
-    uint32_t k = 0;
-    traverse((uint32_t)a, n, vals, mn, mx, left, right, nmn, nmx, [&](uint32_t b) {
-        if (base + k < p_max) pairs[base + k] = make_uint2((uint32_t)a, b);
-        ++k;
-    });
-    // sort this body's pairs by b (insertion sort; lists are short)
-    uint32_t end = base + k;
-    if (end > p_max) end = p_max;
-    for (uint32_t x = base + 1; x < end; ++x) {
-        const uint2 v = pairs[x];
-        uint32_t y = x;
-        while (y > base && pairs[y - 1].y > v.y) {
-            pairs[y] = pairs[y - 1];
-            --y;
+// S lanes per body: lane s of the group walks the subtree of node S-1+s (the
+// S nodes of level log2 S).  Queries run in Morton order (group q takes body
+// perm[q]) so neighbouring lanes walk similar paths.  mode 0 counts into
+// counts[a*S + s]; mode 1 writes at offsets[a*S + s] and then lane 0 sorts the
+// body's segment by b.
+template <int S, int MODE>
+__global__ void __launch_bounds__(kTravBlock) k_pairs(uint32_t n, uint32_t n2, const uint32_t* perm, const float4* mn,
+                                                      const float4* mx, const float4* nmn, const float4* nmx,
+                                                      uint32_t* counts, const uint32_t* offsets, uint32_t p_max,
+                                                      uint2* pairs, uint32_t* err) {
+    __shared__ float4 clo[kTopCached], chi[kTopCached];
+    const uint32_t nnodes = 2 * n2 - 1;
+    const uint32_t ncached = nnodes < kTopCached ? nnodes : kTopCached;
+    for (uint32_t t = threadIdx.x; t < ncached; t += kTravBlock) {
+        clo[t] = nmn[t];
+        chi[t] = nmx[t];
+    }
+    __syncthreads();
+    const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
+    const uint32_t q = gid / S, sub = gid % S;
+    const bool valid = q < n;
+    const uint32_t a = valid ? perm[q] : 0u;
+    // the subtree this lane walks; a tree smaller than S nodes is walked by lane 0
+    uint32_t start = 0;
+    bool walks = valid && (n > 1);
+    if (S > 1) {
+        if (n2 >= (uint32_t)S) start = (uint32_t)S - 1 + sub;
+        else walks = walks && sub == 0;
+    }
+    if (MODE == 0) {
+        uint32_t c = 0;
+        if (walks && !traverse_sub(a, start, n, n2, perm, mn, mx, nmn, nmx, clo, chi, ncached,
+                                   [&](uint32_t) { ++c; }))
+            atomicOr(err, 1u);
+        if (valid) counts[a * S + sub] = c;
+    } else {
+        uint32_t k = 0;
+        const uint32_t base = valid ? offsets[a * S + sub] : 0u;
+        if (walks)
+            traverse_sub(a, start, n, n2, perm, mn, mx, nmn, nmx, clo, chi, ncached, [&](uint32_t b) {
+                if (base + k < p_max) pairs[base + k] = make_uint2(a, b);
+                ++k;
+            });
+        __syncthreads();
+        if (valid && sub == 0) {
+            const uint32_t b0 = offsets[a * S];
+            uint32_t e = offsets[a * S + S];
+            if (e > p_max) e = p_max;
+            for (uint32_t x = b0 + 1; x < e; ++x) {
+                const uint2 v = pairs[x];
+                uint32_t y = x;
+                while (y > b0 && pairs[y - 1].y > v.y) {
+                    pairs[y] = pairs[y - 1];
+                    --y;
+                }
+                pairs[y] = v;
+            }
         }
-        pairs[y] = v;
     }
 }
 
@@ -307,33 +337,62 @@ void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, 
 
 size_t BroadBuffers::sort_scratch_words(uint32_t n_cap) { return radix_scratch_size(n_cap); }
 
-void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
+uint32_t BroadBuffers::pow2_ceil(uint32_t n) {
+    uint32_t p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+namespace {
+template <int S>
+void launch_pairs(int mode, uint32_t n, uint32_t n2, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
+    const uint32_t blocks = (n * S + kTravBlock - 1) / kTravBlock;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_pairs<S, 0>), dim3(blocks), dim3(kTravBlock), 0, s, n, n2, (const uint32_t*)b.perm,
+                           (const float4*)b.aabb_min, (const float4*)b.aabb_max, (const float4*)b.node_min,
+                           (const float4*)b.node_max, b.counts, (const uint32_t*)b.offsets, p_max, b.pairs, b.err);
+    else
+        hipLaunchKernelGGL((k_pairs<S, 1>), dim3(blocks), dim3(kTravBlock), 0, s, n, n2, (const uint32_t*)b.perm,
+                           (const float4*)b.aabb_min, (const float4*)b.aabb_max, (const float4*)b.node_min,
+                           (const float4*)b.node_max, b.counts, (const uint32_t*)b.offsets, p_max, b.pairs, b.err);
+}
+void pairs_pass(int lanes, int mode, uint32_t n, uint32_t n2, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
+    switch (lanes) {
+        case 1: launch_pairs<1>(mode, n, n2, b, p_max, s); break;
+        case 2: launch_pairs<2>(mode, n, n2, b, p_max, s); break;
+        case 8: launch_pairs<8>(mode, n, n2, b, p_max, s); break;
+        default: launch_pairs<4>(mode, n, n2, b, p_max, s); break;
+    }
+}
+}  // namespace
+
+void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s, bool resort,
+               int lanes) {
     if (n == 0) {
         launch_fill_u32(b.offsets, 0u, 1, s);
         hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets, p_max, b.p_dev, b.overflow);
         return;
     }
+    if (lanes != 1 && lanes != 2 && lanes != 8) lanes = 4;
     hipLaunchKernelGGL(k_aabb, dim3(blocks_for(n)), dim3(kBlock), 0, s, bv, n, b.aabb_min, b.aabb_max);
-    hipLaunchKernelGGL(k_bounds, dim3(1), dim3(1024), 0, s, b.aabb_min, b.aabb_max, n, b.bounds);
-    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.aabb_min, b.aabb_max, n, b.bounds,
-                       b.keys0, b.vals0);
-    const int which = radix_sort_pairs(b.keys0, b.vals0, b.keys1, b.vals1, n, 30, b.sort_scratch, s);
-    const uint32_t* codes = which ? b.keys1 : b.keys0;
-    const uint32_t* vals = which ? b.vals1 : b.vals0;
-    const int ni = (int)n;
-    if (n >= 2) {
-        hipLaunchKernelGGL(k_lbvh_build, dim3(blocks_for(n - 1)), dim3(kBlock), 0, s, codes, ni, b.left, b.right,
-                           b.parent_int, b.parent_leaf);
-        launch_fill_u32(b.flags, 0u, n - 1, s);
-        hipLaunchKernelGGL(k_lbvh_refit, dim3(blocks_for(n)), dim3(kBlock), 0, s, ni, vals, b.aabb_min, b.aabb_max,
-                           b.left, b.right, b.parent_int, b.parent_leaf, b.flags, b.node_min, b.node_max, b.err);
+    if (resort) {
+        hipLaunchKernelGGL(k_bounds, dim3(1), dim3(1024), 0, s, b.aabb_min, b.aabb_max, n, b.bounds);
+        hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.aabb_min, b.aabb_max, n, b.bounds,
+                           b.keys0, b.perm);
+        // 30-bit keys: four 8-bit passes, so the sorted values land back in perm
+        radix_sort_pairs(b.keys0, b.perm, b.keys1, b.vals1, n, 30, b.sort_scratch, s);
     }
-    hipLaunchKernelGGL(k_pairs_count, dim3(blocks_for(n)), dim3(kBlock), 0, s, ni, vals, b.aabb_min, b.aabb_max,
-                       b.left, b.right, b.node_min, b.node_max, b.counts, b.err);
-    launch_scan_exclusive(b.counts, b.offsets, n, nullptr, b.scan_partials, s);
-    hipLaunchKernelGGL(k_pairs_write, dim3(blocks_for(n)), dim3(kBlock), 0, s, ni, vals, b.aabb_min, b.aabb_max,
-                       b.left, b.right, b.node_min, b.node_max, b.offsets, p_max, b.pairs);
-    hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets + n, p_max, b.p_dev, b.overflow);
+    const uint32_t n2 = BroadBuffers::pow2_ceil(n);
+    uint32_t chunk = n2 < 512u ? n2 : 512u;
+    while (n2 / chunk > 1024u) chunk <<= 1;
+    hipLaunchKernelGGL(k_bvh_lower, dim3(n2 / chunk), dim3(256), 0, s, (const uint32_t*)b.perm, n, n2, chunk,
+                       (const float4*)b.aabb_min, (const float4*)b.aabb_max, b.node_min, b.node_max);
+    hipLaunchKernelGGL(k_bvh_top, dim3(1), dim3(1024), 0, s, n2, chunk, b.node_min, b.node_max);
+    const uint32_t nc = n * (uint32_t)lanes;
+    pairs_pass(lanes, 0, n, n2, b, p_max, s);
+    launch_scan_exclusive(b.counts, b.offsets, nc, nullptr, b.scan_partials, s);
+    pairs_pass(lanes, 1, n, n2, b, p_max, s);
+    hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets + nc, p_max, b.p_dev, b.overflow);
 }
 
 }  // namespace dio

@@ -24,27 +24,33 @@ struct BroadBuffers {
     float4* aabb_min;  // n_cap
     float4* aabb_max;
     float4* bounds;    // 2
-    uint32_t *keys0, *vals0, *keys1, *vals1;  // n_cap each
+    uint32_t *keys0, *keys1, *vals1;          // n_cap each (radix sort ping-pong)
+    uint32_t* perm;                           // n_cap: body at each Morton-sorted position
     uint32_t* sort_scratch;                   // sort_scratch_words(n_cap)
-    uint32_t *left, *right, *parent_int;      // n_cap - 1 (at least 1)
-    uint32_t* parent_leaf;                    // n_cap
-    uint32_t* flags;                          // n_cap - 1
-    float4 *node_min, *node_max;              // n_cap - 1
-    uint32_t* counts;                         // n_cap
-    uint32_t* offsets;                        // n_cap + 1
-    uint32_t* scan_partials;                  // scan_partials_size(n_cap)
+    float4 *node_min, *node_max;              // 2 * pow2_ceil(n_cap) - 1 heap nodes
+    uint32_t* counts;                         // kMaxTraversalLanes * n_cap
+    uint32_t* offsets;                        // kMaxTraversalLanes * n_cap + 1
+    uint32_t* scan_partials;                  // scan_partials_size(kMaxTraversalLanes * n_cap + 1)
     uint2* pairs;                             // p_max
     uint32_t* p_dev;                          // pair count min(total, p_max)
     uint32_t* overflow;                       // largest total that exceeded p_max
     uint32_t* err;                            // traversal stack overflow flag
     static size_t sort_scratch_words(uint32_t n_cap);
+    static uint32_t pow2_ceil(uint32_t n);
 };
+
+constexpr uint32_t kMaxTraversalLanes = 8;
+constexpr uint32_t kMaxBroadBodies = 1u << 20;  // two-level refit: 1024 chunks of <= 1024 leaves
 
 void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, float4* rot, uint32_t* tag,
                               float4* corners, hipStream_t s);
 
 // AABBs, LBVH and the sorted pair list of n bodies; the pair count stays on
-// the device (b.p_dev).
-void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s);
+// the device (b.p_dev).  resort: recompute Morton codes and the sorted order
+// (otherwise the tree is refit over the previous order, which keeps the pair
+// set exact and only changes the traversal cost).  lanes: traversal lanes per
+// body (1, 2, 4 or 8).
+void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s, bool resort,
+               int lanes);
 
 }  // namespace dio

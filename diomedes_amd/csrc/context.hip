@@ -80,25 +80,21 @@ struct PinnedArena {
 
 BroadBuffers alloc_broad(DevArena& A, uint32_t n_cap, uint32_t p_max) {
     BroadBuffers b{};
-    const uint32_t ni = n_cap > 1 ? n_cap - 1 : 1;
+    const size_t nodes = 2 * (size_t)BroadBuffers::pow2_ceil(n_cap) - 1;
+    const size_t nc = (size_t)kMaxTraversalLanes * n_cap;
     b.aabb_min = A.alloc<float4>(n_cap);
     b.aabb_max = A.alloc<float4>(n_cap);
     b.bounds = A.alloc<float4>(2);
     b.keys0 = A.alloc<uint32_t>(n_cap);
-    b.vals0 = A.alloc<uint32_t>(n_cap);
     b.keys1 = A.alloc<uint32_t>(n_cap);
     b.vals1 = A.alloc<uint32_t>(n_cap);
+    b.perm = A.alloc<uint32_t>(n_cap);
     b.sort_scratch = A.alloc<uint32_t>(BroadBuffers::sort_scratch_words(n_cap));
-    b.left = A.alloc<uint32_t>(ni);
-    b.right = A.alloc<uint32_t>(ni);
-    b.parent_int = A.alloc<uint32_t>(ni);
-    b.parent_leaf = A.alloc<uint32_t>(n_cap);
-    b.flags = A.alloc<uint32_t>(ni);
-    b.node_min = A.alloc<float4>(ni);
-    b.node_max = A.alloc<float4>(ni);
-    b.counts = A.alloc<uint32_t>(n_cap);
-    b.offsets = A.alloc<uint32_t>(n_cap + 1);
-    b.scan_partials = A.alloc<uint32_t>(scan_partials_size(n_cap + 1));
+    b.node_min = A.alloc<float4>(nodes);
+    b.node_max = A.alloc<float4>(nodes);
+    b.counts = A.alloc<uint32_t>(nc);
+    b.offsets = A.alloc<uint32_t>(nc + 1);
+    b.scan_partials = A.alloc<uint32_t>(scan_partials_size((uint32_t)nc + 1));
     b.pairs = A.alloc<uint2>(p_max);
     b.p_dev = A.alloc<uint32_t>(1);
     b.overflow = A.alloc<uint32_t>(1);
@@ -209,7 +205,10 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{8, {0, 0, 0, 0, 0, 0, 0}};
+    dio_tuning tuning{8, 4, 4, {0, 0, 0, 0, 0}};
+    uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
+    uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
+    bool w_sort_pending = true;
     uint32_t a_n = 0, a_m = 0, a_iters = 0;
     bool a_submitted = false;
 
@@ -244,8 +243,9 @@ struct dio_ctx {
     std::vector<hipEvent_t> prof_events;
     uint64_t w_step = 0;
     dio_step_stats stats{};
-    // graph cache: [segment][parity], segment 0 = contacts + broad + narrow, 1 = AVBD
-    hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // graph cache [variant][parity]: 0 = contacts + broad + narrow, 1 = AVBD,
+    // 2 = contacts + broad (with a Morton re-sort) + narrow
+    hipGraphExec_t graphs[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     dio_avbd_options g_opts{};
     dio_step_opts g_sopts{};
     bool graphs_valid = false;
@@ -502,12 +502,13 @@ dio_status dio_create(int device, const dio_limits* limits, dio_ctx** out) {
         DIO_HIP_CHECK(hipSetDevice(device));
         DIO_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->lim = *limits;
+        if (limits->max_bodies > kMaxBroadBodies) capacity("max_bodies exceeds the broad phase limit", kMaxBroadBodies);
         c->n_cap = limits->max_bodies;
         c->p_max = limits->max_pairs ? limits->max_pairs : std::max<uint32_t>(1024u, 16u * c->n_cap);
         c->m_cap = limits->max_contacts ? limits->max_contacts : 4u * c->p_max + 24u * c->n_cap;
         c->c_cap = limits->max_constraints ? limits->max_constraints : std::max<uint32_t>(16u, c->n_cap);
         c->h_cap = limits->max_hull_vertices ? limits->max_hull_vertices : 4096u;
-        const uint32_t big = std::max({c->n_cap + 1, c->p_max + 1, c->m_cap + 1});
+        const uint32_t big = std::max({c->n_cap * kMaxTraversalLanes + 1, c->p_max + 1, c->m_cap + 1});
         c->scan_partials = c->dev.alloc<uint32_t>(scan_partials_size(big));
         return DIO_OK;
     });
@@ -550,7 +551,8 @@ dio_status dio_broad_submit(dio_ctx* ctx, const dio_gpu_body* bodies, uint32_t n
         launch_fill_u32(ctx->gb.overflow, 0u, 1, ctx->stream);
         launch_fill_u32(ctx->gb.err, 0u, 1, ctx->stream);
         BodyView bv{ctx->g_pos, ctx->g_rot, ctx->g_tag, ctx->g_corners, ctx->g_hull};
-        run_broad(bv, n, ctx->gb, ctx->p_max, ctx->stream);
+        ctx->g_lanes = ctx->tuning.broad_lanes;
+        run_broad(bv, n, ctx->gb, ctx->p_max, ctx->stream, true, (int)ctx->g_lanes);
         DIO_HIP_CHECK(hipGetLastError());
         ctx->g_broad_submitted = true;
         return DIO_OK;
@@ -562,7 +564,8 @@ dio_status dio_broad_count(dio_ctx* ctx, uint32_t* count) {
         if (!count) invalid("count is null");
         if (!ctx->g_broad_submitted) throw StatusError{DIO_NOT_READY, "no broad phase submitted", 0};
         uint32_t v[3] = {0, 0, 0};
-        DIO_HIP_CHECK(hipMemcpyAsync(&v[0], ctx->gb.offsets + ctx->g_n, 4, hipMemcpyDeviceToHost, ctx->stream));
+        DIO_HIP_CHECK(hipMemcpyAsync(&v[0], ctx->gb.offsets + (size_t)ctx->g_n * ctx->g_lanes, 4, hipMemcpyDeviceToHost,
+                                     ctx->stream));
         DIO_HIP_CHECK(hipMemcpyAsync(&v[1], ctx->gb.overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
         DIO_HIP_CHECK(hipMemcpyAsync(&v[2], ctx->gb.err, 4, hipMemcpyDeviceToHost, ctx->stream));
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -865,7 +868,7 @@ SolveView world_solve_view(dio_ctx* c, int s);
 // Enqueue segment `seg` of one step with parity s (0: contacts + broad +
 // narrow, 1: AVBD).
 void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, const dio_step_opts& so,
-                     KernelTimer* kt = nullptr) {
+                     KernelTimer* kt = nullptr, bool resort = true) {
     hipStream_t st = c->stream;
     const uint32_t n = c->w_n;
     WorldView wv{c->w_pos, c->w_rot, c->w_vel, c->w_inv_mm, c->w_tag, c->w_corners, c->w_hull, n};
@@ -883,7 +886,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
             b.pairs = c->pend[dst_pend].pairs;
             b.p_dev = c->pend[dst_pend].p_dev;
             b.overflow = c->w_overflow;
-            run_broad(bv, n, b, c->p_max, st);
+            run_broad(bv, n, b, c->p_max, st, resort, (int)c->tuning.broad_lanes);
             run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, st);
         };
         if (!lag) broad_narrow();
@@ -938,17 +941,18 @@ bool graphs_disabled() {
 
 void build_graphs(dio_ctx* c, const dio_avbd_options& o, const dio_step_opts& so) {
     c->drop_graphs();
-    for (int seg = 0; seg < 2; ++seg)
+    for (int var = 0; var < 3; ++var)
         for (int s = 0; s < 2; ++s) {
+            const int seg = var == 1 ? 1 : 0;
             hipGraph_t g = nullptr;
             DIO_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_segment(c, seg, s, o, so);
+            enqueue_segment(c, seg, s, o, so, nullptr, var == 2);
             DIO_HIP_CHECK(hipStreamEndCapture(c->stream, &g));
             hipGraphExec_t ge = nullptr;
             hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             DIO_HIP_CHECK(e);
-            c->graphs[seg][s] = ge;
+            c->graphs[var][s] = ge;
         }
     c->g_opts = o;
     c->g_sopts = so;
@@ -971,6 +975,7 @@ dio_status dio_world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_
         ctx->drop_graphs();
         ctx->w_n = n;
         ctx->w_step = 0;
+        ctx->w_sort_pending = true;
         ctx->w_friction.resize(n);
         ctx->w_n_dyn = 0;
         for (uint32_t i = 0; i < n; ++i) {
@@ -1057,13 +1062,18 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         DIO_HIP_CHECK(hipEventRecord(ctx->events[0], s));
         for (uint32_t k = 0; k < nsteps && ctx->w_n > 0; ++k) {
             const int par = (int)(ctx->w_step % 2);
+            const bool resort = ctx->w_sort_pending || ctx->w_step - ctx->w_sorted_at >= ctx->tuning.resort_interval;
+            if (resort) {
+                ctx->w_sorted_at = ctx->w_step;
+                ctx->w_sort_pending = false;
+            }
             if (use_graph) {
-                DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[0][par], s));
+                DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[resort ? 2 : 0][par], s));
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
                 DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[1][par], s));
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
             } else {
-                enqueue_segment(ctx, 0, par, *options, so);
+                enqueue_segment(ctx, 0, par, *options, so, nullptr, resort);
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
                 enqueue_segment(ctx, 1, par, *options, so, so.profile ? &kt : nullptr);
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
@@ -1129,8 +1139,12 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         if (!t) invalid("tuning is null");
         dio_tuning v = *t;
         if (v.primal_lanes == 0) v.primal_lanes = 8;
+        if (v.broad_lanes == 0) v.broad_lanes = 4;
+        if (v.resort_interval == 0) v.resort_interval = 4;
         if (v.primal_lanes != 1 && v.primal_lanes != 4 && v.primal_lanes != 8 && v.primal_lanes != 16)
             invalid("primal_lanes must be 1, 4, 8 or 16");
+        if (v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 && v.broad_lanes != 8)
+            invalid("broad_lanes must be 1, 2, 4 or 8");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
@@ -1179,9 +1193,35 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
                 best_lanes = shapes[k];
             }
         }
+        // broad phase: traversal lanes, timed on the world's poses in the
+        // granular broad buffers (one re-sort first, then refits as between
+        // re-sorts of the fused step)
+        ensure_granular_broad(ctx);
+        const BodyView bv{ctx->w_pos, ctx->w_rot, ctx->w_tag, ctx->w_corners, ctx->w_hull};
+        const int blanes[4] = {1, 2, 4, 8};
+        float bbest = 3.4e38f;
+        int best_blanes = 4;
+        run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, true, 4);
+        for (int k = 0; k < 4; ++k) {
+            run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, false, blanes[k]);
+            DIO_HIP_CHECK(hipEventRecord(e0, s));
+            for (uint32_t r = 0; r < reps; ++r) run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, false, blanes[k]);
+            DIO_HIP_CHECK(hipEventRecord(e1, s));
+            DIO_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            DIO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            const float us = 1000.0f * ms / (float)reps;
+            if (us_per_launch) us_per_launch[4 + k] = us;
+            if (us < bbest) {
+                bbest = us;
+                best_blanes = blanes[k];
+            }
+        }
+        ctx->g_broad_submitted = false;  // the granular broad buffers were used as scratch
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         ctx->tuning.primal_lanes = (uint32_t)best_lanes;
+        ctx->tuning.broad_lanes = (uint32_t)best_blanes;
         ctx->drop_graphs();
         if (chosen) *chosen = ctx->tuning;
         return DIO_OK;

@@ -465,24 +465,34 @@ class PhysicsContext:
         o = options.to_c()
         self._check(self._lib.dio_step(self._h, ctypes.byref(o), ctypes.byref(so), nsteps))
 
-    def set_tuning(self, primal_lanes: int = 8) -> None:
+    def set_tuning(self, primal_lanes: int = 0, broad_lanes: int = 0, resort_interval: int = 0) -> None:
+        """Launch shapes (0 keeps the current value)."""
+        cur = self.tuning()
         t = R.Tuning()
-        t.primal_lanes = primal_lanes
+        t.primal_lanes = primal_lanes or cur["primal_lanes"]
+        t.broad_lanes = broad_lanes or cur["broad_lanes"]
+        t.resort_interval = resort_interval or cur["resort_interval"]
         self._check(self._lib.dio_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
         t = R.Tuning()
         self._check(self._lib.dio_get_tuning(self._h, ctypes.byref(t)))
-        return {"primal_lanes": t.primal_lanes}
+        return {"primal_lanes": t.primal_lanes, "broad_lanes": t.broad_lanes, "resort_interval": t.resort_interval}
 
     def autotune(self, options: AvbdOptions, reps: int = 20) -> dict:
-        """Time every primal launch shape on the current contacts, keep the
-        fastest; returns the choice and the per-shape microseconds."""
+        """Time every primal and broad-phase launch shape on the current
+        world, keep the fastest of each; returns the choice and the per-shape
+        microseconds."""
         t = R.Tuning()
-        us = (ctypes.c_float * 4)()
+        us = (ctypes.c_float * 8)()
         o = options.to_c()
         self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
-        return {"primal_lanes": t.primal_lanes, "us_per_launch": dict(zip((1, 4, 8, 16), list(us)))}
+        return {
+            "primal_lanes": t.primal_lanes,
+            "broad_lanes": t.broad_lanes,
+            "us_per_launch": dict(zip((1, 4, 8, 16), list(us)[:4])),
+            "broad_us_per_pass": dict(zip((1, 2, 4, 8), list(us)[4:])),
+        }
 
     def stats(self) -> R.StepStats:
         s = R.StepStats()

@@ -171,7 +171,8 @@ assert ctypes.sizeof(StepStats) == 88
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("primal_lanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7)]
+    _fields_ = [("primal_lanes", ctypes.c_uint32), ("broad_lanes", ctypes.c_uint32),
+                ("resort_interval", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
 
 
 def f32(x) -> float:

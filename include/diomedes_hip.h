@@ -203,8 +203,11 @@ typedef struct dio_step_stats {
 
 /* Kernel launch shapes (the autotuner's search space). */
 typedef struct dio_tuning {
-    uint32_t primal_lanes;  /* lanes per body in the primal: 1, 4, 8 or 16 (0 = default 8) */
-    uint32_t reserved[7];
+    uint32_t primal_lanes;     /* lanes per body in the primal: 1, 4, 8 or 16 (0 = default 8) */
+    uint32_t broad_lanes;      /* lanes per body in the LBVH traversal: 1, 2, 4 or 8 (0 = default 4) */
+    uint32_t resort_interval;  /* fused step: Morton re-sort every this many steps; the tree is refit
+                                  over the previous order in between (0 = default 4) */
+    uint32_t reserved[5];
 } dio_tuning;
 
 typedef struct dio_ctx dio_ctx;
@@ -272,10 +275,13 @@ dio_status dio_step_get_stats(dio_ctx* ctx, dio_step_stats* out);
 dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* tuning);
 dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out);
 /* Autotuner: after at least one dio_step, time every primal shape on the
-   current contact set (reps launches each, HIP events), keep the fastest, and
-   report the per-shape average microseconds in us_per_launch[0..3] (lanes 1,
-   4, 8, 16).  The primal with a fixed parity is idempotent, so the world state
-   is unchanged. */
+   current contact set and every broad-phase traversal shape on the current
+   poses (reps launches each, HIP events), keep the fastest of each, and report
+   the average microseconds per launch in us_per_launch[0..7]: primal lanes 1,
+   4, 8, 16, then broad lanes 1, 2, 4, 8 (a refit + traversal pass).  The
+   primal with a fixed parity is idempotent and the broad timing runs in the
+   granular broad buffers, so the world state is unchanged; a granular broad
+   result submitted before the call must be submitted again. */
 dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t reps, dio_tuning* chosen,
                         float* us_per_launch);
 /* Download body state (any pointer may be null): n x 3 / n x 4 floats. */

@@ -25,11 +25,13 @@ def _print(*a):
 
 
 # ------------------------------------------------------------ broad phase
-@pytest.mark.parametrize("nx,ny,nz,seed", [(4, 2, 4, 1), (8, 2, 8, 2), (32, 8, 32, 0)])
-def test_broad_pairs_match_dense_sweep(oracle, nx, ny, nz, seed):
-    _print("broad", nx, ny, nz, seed)
+@pytest.mark.parametrize("blanes", [1, 2, 4, 8])
+@pytest.mark.parametrize("nx,ny,nz,seed", [(4, 2, 4, 1), (8, 2, 8, 2), (32, 8, 32, 0), (1, 1, 3, 5)])
+def test_broad_pairs_match_dense_sweep(oracle, nx, ny, nz, seed, blanes):
+    _print("broad", nx, ny, nz, seed, blanes)
     scene = scenes.cube_field(seed, nx=nx, ny=ny, nz=nz, spacing=1.2)
     with PhysicsContext(scene.n, max_pairs=64 * scene.n) as ctx:
+        ctx.set_tuning(broad_lanes=blanes)
         ctx.broad_phase_submit(scene.bodies)
         pairs = ctx.broad_phase_pairs()
         aabbs = ctx.broad_phase_aabbs(scene.n)
@@ -191,12 +193,17 @@ def test_box_stack_300_steps_bitexact(oracle):
         assert np.array_equal(np.stack([pairs["a"], pairs["b"]], 1).reshape(-1, 2), ref_pairs.reshape(-1, 2))
 
 
-@pytest.mark.parametrize("lag,lanes", [(1, 8), (0, 8), (1, 4), (1, 16), (1, 1)])
-def test_cube_field_small_bitexact(oracle, lag, lanes):
+@pytest.mark.parametrize(
+    "lag,lanes,blanes,resort",
+    [(1, 8, 4, 4), (0, 8, 4, 4), (1, 4, 2, 1), (1, 16, 8, 1000), (1, 1, 1, 4), (0, 4, 8, 1000)],
+)
+def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort):
+    """resort: Morton re-sort interval of the fused step; 1000 refits the first
+    step's order for the whole run, which must not change the pair set."""
     scene = scenes.cube_field(7, nx=8, ny=2, nz=8)
     w = oracle.OracleWorld(scene, 32)
     with PhysicsContext(scene.n) as ctx:
-        ctx.set_tuning(primal_lanes=lanes)
+        ctx.set_tuning(primal_lanes=lanes, broad_lanes=blanes, resort_interval=resort)
         ctx.upload(scene)
         for k in range(6):
             ctx.step(scene.options, 20, lag=lag)
@@ -215,6 +222,7 @@ def test_autotune_keeps_state_and_parity(oracle):
         res = ctx.autotune(scene.options, reps=5)
         _print("autotune", res)
         assert res["primal_lanes"] in (1, 4, 8, 16)
+        assert res["broad_lanes"] in (1, 2, 4, 8)
         _compare_world(ctx, w, 40)
         ctx.step(scene.options, 20)
         w.step(nsteps=20)
@@ -338,3 +346,19 @@ def test_mixed_field_small_bitexact(oracle):
             w.step(nsteps=15)
             _compare_world(ctx, w, 15 * (k + 1))
             _print("mixed step", 15 * (k + 1), "contacts", ctx.stats().contacts)
+
+
+@pytest.mark.parametrize("resort", [1, 8])
+def test_wall_throw_small_bitexact(oracle, resort):
+    """A fast projectile through a brick wall: the tree is refit over a stale
+    Morton order between re-sorts while bodies scatter."""
+    scene = scenes.wall_throw(2, columns=16, rows=8)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.set_tuning(resort_interval=resort)
+        ctx.upload(scene)
+        for k in range(4):
+            ctx.step(scene.options, 15)
+            w.step(nsteps=15)
+            _compare_world(ctx, w, 15 * (k + 1))
+            _print("wall step", 15 * (k + 1), "pairs", ctx.stats().pairs, "contacts", ctx.stats().contacts)
