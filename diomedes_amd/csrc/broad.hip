@@ -90,8 +90,9 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
     return v;
 }
 
-__device__ __forceinline__ uint32_t quantize(float c, float lo, float hi) {
-    const float e = hi - lo;
+// one scale for all three axes (cubic Morton cells): a flat or elongated scene
+// keeps spatially compact runs of codes, hence compact tree nodes
+__device__ __forceinline__ uint32_t quantize(float c, float lo, float e) {
     float t = e > 0.0f ? (c - lo) / e * 1024.0f : 0.0f;
     t = fminf(fmaxf(t, 0.0f), 1023.0f);
     return (uint32_t)t;
@@ -103,9 +104,10 @@ __global__ void __launch_bounds__(kBlock) k_morton(const float4* mn, const float
     if (i >= n) return;
     const float4 lo = bounds[0], hi = bounds[1];
     const float4 a = mn[i], b = mx[i];
-    const uint32_t qx = quantize((a.x + b.x) * 0.5f, lo.x, hi.x);
-    const uint32_t qy = quantize((a.y + b.y) * 0.5f, lo.y, hi.y);
-    const uint32_t qz = quantize((a.z + b.z) * 0.5f, lo.z, hi.z);
+    const float e = fmaxf(fmaxf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+    const uint32_t qx = quantize((a.x + b.x) * 0.5f, lo.x, e);
+    const uint32_t qy = quantize((a.y + b.y) * 0.5f, lo.y, e);
+    const uint32_t qz = quantize((a.z + b.z) * 0.5f, lo.z, e);
     keys[i] = (expand_bits(qx) << 2) | (expand_bits(qy) << 1) | expand_bits(qz);
     vals[i] = i;
 }
@@ -248,61 +250,127 @@ __device__ __forceinline__ bool traverse_sub(uint32_t a, uint32_t start, uint32_
 
 // S lanes per body: lane s of the group walks the subtree of node S-1+s (the
 // S nodes of level log2 S).  Queries run in Morton order (group q takes body
-// perm[q]) so neighbouring lanes walk similar paths.  mode 0 counts into
-// counts[a*S + s]; mode 1 writes at offsets[a*S + s] and then lane 0 sorts the
-// body's segment by b.
-template <int S, int MODE>
-__global__ void __launch_bounds__(kTravBlock) k_pairs(uint32_t n, uint32_t n2, const uint32_t* perm, const float4* mn,
-                                                      const float4* mx, const float4* nmn, const float4* nmx,
-                                                      uint32_t* counts, const uint32_t* offsets, uint32_t p_max,
-                                                      uint2* pairs, uint32_t* err) {
-    __shared__ float4 clo[kTopCached], chi[kTopCached];
+// perm[q]) so neighbouring lanes walk similar paths.
+//
+// Pass 1 (k_pairs_count) counts into counts[a*S + s] and keeps the first
+// kCache / S bodies it finds in cache[a*kCache + s*(kCache/S) ...].  Pass 2
+// (k_pairs_emit), after the scan, places each body's pairs by rank (b is
+// distinct within a body's list, so the rank among the list is its sorted
+// position): from the cache in LDS when every lane of the body fit, otherwise
+// by walking again and sorting the body's segment in place.
+template <int S>
+__device__ __forceinline__ bool lane_start(uint32_t n, uint32_t n2, uint32_t sub, uint32_t& start) {
+    // the subtree this lane walks; a tree smaller than S nodes is walked by lane 0
+    start = 0;
+    if (n <= 1) return false;
+    if (S > 1) {
+        if (n2 >= (uint32_t)S) start = (uint32_t)S - 1 + sub;
+        else return sub == 0;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void load_top(const float4* nmn, const float4* nmx, uint32_t n2, float4* clo, float4* chi,
+                                         uint32_t& ncached) {
     const uint32_t nnodes = 2 * n2 - 1;
-    const uint32_t ncached = nnodes < kTopCached ? nnodes : kTopCached;
+    ncached = nnodes < kTopCached ? nnodes : kTopCached;
     for (uint32_t t = threadIdx.x; t < ncached; t += kTravBlock) {
         clo[t] = nmn[t];
         chi[t] = nmx[t];
     }
     __syncthreads();
+}
+
+template <int S>
+__global__ void __launch_bounds__(kTravBlock) k_pairs_count(uint32_t n, uint32_t n2, const uint32_t* perm,
+                                                            const float4* mn, const float4* mx, const float4* nmn,
+                                                            const float4* nmx, uint32_t* counts, uint32_t* cache,
+                                                            uint32_t* err) {
+    constexpr uint32_t C = kPairCache / S;
+    __shared__ float4 clo[kTopCached], chi[kTopCached];
+    uint32_t ncached;
+    load_top(nmn, nmx, n2, clo, chi, ncached);
+    const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
+    const uint32_t q = gid / S, sub = gid % S;
+    if (q >= n) return;
+    const uint32_t a = perm[q];
+    uint32_t start;
+    uint32_t c = 0;
+    uint32_t* mine = cache + (size_t)a * kPairCache + sub * C;
+    if (lane_start<S>(n, n2, sub, start) &&
+        !traverse_sub(a, start, n, n2, perm, mn, mx, nmn, nmx, clo, chi, ncached, [&](uint32_t b) {
+            if (c < C) mine[c] = b;
+            ++c;
+        }))
+        atomicOr(err, 1u);
+    counts[a * S + sub] = c;
+}
+
+template <int S>
+__global__ void __launch_bounds__(kTravBlock) k_pairs_emit(uint32_t n, uint32_t n2, const uint32_t* perm,
+                                                           const float4* mn, const float4* mx, const float4* nmn,
+                                                           const float4* nmx, const uint32_t* offsets,
+                                                           const uint32_t* cache, uint32_t p_max, uint2* pairs) {
+    constexpr uint32_t C = kPairCache / S;
+    constexpr int kBodies = kTravBlock / S;
+    __shared__ float4 clo[kTopCached], chi[kTopCached];
+    __shared__ uint32_t list[kBodies][kPairCache];
+    __shared__ uint32_t spill[kBodies];
+    const uint32_t slot = threadIdx.x / S;
+    if (threadIdx.x < (uint32_t)kBodies) spill[threadIdx.x] = 0u;
+    uint32_t ncached;
+    load_top(nmn, nmx, n2, clo, chi, ncached);  // includes a barrier
     const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
     const uint32_t q = gid / S, sub = gid % S;
     const bool valid = q < n;
     const uint32_t a = valid ? perm[q] : 0u;
-    // the subtree this lane walks; a tree smaller than S nodes is walked by lane 0
-    uint32_t start = 0;
-    bool walks = valid && (n > 1);
-    if (S > 1) {
-        if (n2 >= (uint32_t)S) start = (uint32_t)S - 1 + sub;
-        else walks = walks && sub == 0;
+    uint32_t seg = 0, base = 0, cnt = 0;
+    if (valid) {
+        seg = offsets[a * S];
+        base = offsets[a * S + sub];
+        cnt = offsets[a * S + sub + 1] - base;
+        if (cnt > C) spill[slot] = 1u;
     }
-    if (MODE == 0) {
-        uint32_t c = 0;
-        if (walks && !traverse_sub(a, start, n, n2, perm, mn, mx, nmn, nmx, clo, chi, ncached,
-                                   [&](uint32_t) { ++c; }))
-            atomicOr(err, 1u);
-        if (valid) counts[a * S + sub] = c;
-    } else {
-        uint32_t k = 0;
-        const uint32_t base = valid ? offsets[a * S + sub] : 0u;
-        if (walks)
-            traverse_sub(a, start, n, n2, perm, mn, mx, nmn, nmx, clo, chi, ncached, [&](uint32_t b) {
-                if (base + k < p_max) pairs[base + k] = make_uint2(a, b);
-                ++k;
-            });
-        __syncthreads();
-        if (valid && sub == 0) {
-            const uint32_t b0 = offsets[a * S];
-            uint32_t e = offsets[a * S + S];
-            if (e > p_max) e = p_max;
-            for (uint32_t x = b0 + 1; x < e; ++x) {
-                const uint2 v = pairs[x];
-                uint32_t y = x;
-                while (y > b0 && pairs[y - 1].y > v.y) {
-                    pairs[y] = pairs[y - 1];
-                    --y;
-                }
-                pairs[y] = v;
+    __syncthreads();
+    const bool fast = valid && spill[slot] == 0u;
+    const uint32_t* mine = cache + (size_t)a * kPairCache + sub * C;
+    if (fast) {
+        for (uint32_t j = 0; j < cnt; ++j) list[slot][base - seg + j] = mine[j];
+    } else if (valid) {
+        uint32_t start;
+        if (cnt > C) {
+            uint32_t k = 0;
+            if (lane_start<S>(n, n2, sub, start))
+                traverse_sub(a, start, n, n2, perm, mn, mx, nmn, nmx, clo, chi, ncached, [&](uint32_t b) {
+                    if (base + k < p_max) pairs[base + k] = make_uint2(a, b);
+                    ++k;
+                });
+        } else {
+            for (uint32_t j = 0; j < cnt; ++j)
+                if (base + j < p_max) pairs[base + j] = make_uint2(a, mine[j]);
+        }
+    }
+    __syncthreads();
+    if (fast) {
+        const uint32_t total = offsets[a * S + S] - seg;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const uint32_t b = list[slot][base - seg + j];
+            uint32_t rank = 0;
+            for (uint32_t x = 0; x < total; ++x) rank += list[slot][x] < b ? 1u : 0u;
+            if (seg + rank < p_max) pairs[seg + rank] = make_uint2(a, b);
+        }
+    } else if (valid && sub == 0) {
+        const uint32_t b0 = seg;
+        uint32_t e = offsets[a * S + S];
+        if (e > p_max) e = p_max;
+        for (uint32_t x = b0 + 1; x < e; ++x) {
+            const uint2 v = pairs[x];
+            uint32_t y = x;
+            while (y > b0 && pairs[y - 1].y > v.y) {
+                pairs[y] = pairs[y - 1];
+                --y;
             }
+            pairs[y] = v;
         }
     }
 }
@@ -348,19 +416,21 @@ template <int S>
 void launch_pairs(int mode, uint32_t n, uint32_t n2, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
     const uint32_t blocks = (n * S + kTravBlock - 1) / kTravBlock;
     if (mode == 0)
-        hipLaunchKernelGGL((k_pairs<S, 0>), dim3(blocks), dim3(kTravBlock), 0, s, n, n2, (const uint32_t*)b.perm,
+        hipLaunchKernelGGL((k_pairs_count<S>), dim3(blocks), dim3(kTravBlock), 0, s, n, n2, (const uint32_t*)b.perm,
                            (const float4*)b.aabb_min, (const float4*)b.aabb_max, (const float4*)b.node_min,
-                           (const float4*)b.node_max, b.counts, (const uint32_t*)b.offsets, p_max, b.pairs, b.err);
+                           (const float4*)b.node_max, b.counts, b.pair_cache, b.err);
     else
-        hipLaunchKernelGGL((k_pairs<S, 1>), dim3(blocks), dim3(kTravBlock), 0, s, n, n2, (const uint32_t*)b.perm,
+        hipLaunchKernelGGL((k_pairs_emit<S>), dim3(blocks), dim3(kTravBlock), 0, s, n, n2, (const uint32_t*)b.perm,
                            (const float4*)b.aabb_min, (const float4*)b.aabb_max, (const float4*)b.node_min,
-                           (const float4*)b.node_max, b.counts, (const uint32_t*)b.offsets, p_max, b.pairs, b.err);
+                           (const float4*)b.node_max, (const uint32_t*)b.offsets, (const uint32_t*)b.pair_cache, p_max,
+                           b.pairs);
 }
 void pairs_pass(int lanes, int mode, uint32_t n, uint32_t n2, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
     switch (lanes) {
         case 1: launch_pairs<1>(mode, n, n2, b, p_max, s); break;
         case 2: launch_pairs<2>(mode, n, n2, b, p_max, s); break;
         case 8: launch_pairs<8>(mode, n, n2, b, p_max, s); break;
+        case 16: launch_pairs<16>(mode, n, n2, b, p_max, s); break;
         default: launch_pairs<4>(mode, n, n2, b, p_max, s); break;
     }
 }
@@ -373,7 +443,7 @@ void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p
         hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets, p_max, b.p_dev, b.overflow);
         return;
     }
-    if (lanes != 1 && lanes != 2 && lanes != 8) lanes = 4;
+    if (lanes != 1 && lanes != 2 && lanes != 8 && lanes != 16) lanes = 4;
     hipLaunchKernelGGL(k_aabb, dim3(blocks_for(n)), dim3(kBlock), 0, s, bv, n, b.aabb_min, b.aabb_max);
     if (resort) {
         hipLaunchKernelGGL(k_bounds, dim3(1), dim3(1024), 0, s, b.aabb_min, b.aabb_max, n, b.bounds);

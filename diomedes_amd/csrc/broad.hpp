@@ -31,6 +31,7 @@ struct BroadBuffers {
     uint32_t* counts;                         // kMaxTraversalLanes * n_cap
     uint32_t* offsets;                        // kMaxTraversalLanes * n_cap + 1
     uint32_t* scan_partials;                  // scan_partials_size(kMaxTraversalLanes * n_cap + 1)
+    uint32_t* pair_cache;                     // kPairCache * n_cap: partners found by the counting pass
     uint2* pairs;                             // p_max
     uint32_t* p_dev;                          // pair count min(total, p_max)
     uint32_t* overflow;                       // largest total that exceeded p_max
@@ -39,7 +40,8 @@ struct BroadBuffers {
     static uint32_t pow2_ceil(uint32_t n);
 };
 
-constexpr uint32_t kMaxTraversalLanes = 8;
+constexpr uint32_t kMaxTraversalLanes = 16;
+constexpr uint32_t kPairCache = 32;  // partners per body kept between the two traversal passes
 constexpr uint32_t kMaxBroadBodies = 1u << 20;  // two-level refit: 1024 chunks of <= 1024 leaves
 
 void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, float4* rot, uint32_t* tag,
@@ -49,7 +51,7 @@ void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, 
 // the device (b.p_dev).  resort: recompute Morton codes and the sorted order
 // (otherwise the tree is refit over the previous order, which keeps the pair
 // set exact and only changes the traversal cost).  lanes: traversal lanes per
-// body (1, 2, 4 or 8).
+// body (1, 2, 4, 8 or 16).
 void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s, bool resort,
                int lanes);
 

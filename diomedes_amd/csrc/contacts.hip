@@ -228,37 +228,56 @@ __global__ void __launch_bounds__(kBlock) k_csr_count(ContactSet cs, uint32_t* c
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_csr_fill(ContactSet cs, const uint32_t* off, uint32_t* cur, uint32_t* idx,
+__global__ void __launch_bounds__(kBlock) k_csr_fill(ContactSet cs, const uint32_t* off, uint32_t* cur, uint32_t* tmp,
                                                      uint32_t idx_cap) {
     const uint32_t m = *cs.m_dev;
     for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < m; c += gridDim.x * kBlock) {
         const uint2 ab = cs.c.ab[c];
         const uint32_t sa = off[ab.x] + atomicAdd(&cur[ab.x], 1u);
-        if (sa < idx_cap) idx[sa] = c;
+        if (sa < idx_cap) tmp[sa] = c;
         if (ab.y != kContainer) {
             const uint32_t sb = off[ab.y] + atomicAdd(&cur[ab.y], 1u);
-            if (sb < idx_cap) idx[sb] = c;
+            if (sb < idx_cap) tmp[sb] = c;
         }
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_csr_sort(uint32_t n, const uint32_t* off, uint32_t* idx, uint32_t idx_cap,
+__device__ __forceinline__ uint32_t peer_code(uint2 ab, uint32_t body) {
+    return ab.x == body ? ab.y : (ab.x | kPeerIsB);
+}
+
+// Each body's entries arrive in atomic order; an entry's sorted position is
+// its rank among the body's entries (contact indices are distinct within a
+// body), computed by kCsrLanes lanes per body.  Also writes the peer code of
+// every entry (the other body, flagged when this body is the contact's b).
+constexpr int kCsrLanes = 8;
+__global__ void __launch_bounds__(kBlock) k_csr_rank(uint32_t n, const uint32_t* off, const uint32_t* tmp,
+                                                     const uint2* ab, uint32_t* idx, uint32_t* peer, uint32_t idx_cap,
                                                      uint32_t* overflow) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i == 0 && off[n] > idx_cap) atomicMax(overflow, off[n]);
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g == 0 && off[n] > idx_cap) atomicMax(overflow, off[n]);
+    const uint32_t i = g / kCsrLanes, lane = g % kCsrLanes;
     if (i >= n) return;
     const uint32_t b = off[i];
     uint32_t e = off[i + 1];
     if (e > idx_cap) e = idx_cap;
-    for (uint32_t x = b + 1; x < e; ++x) {
-        const uint32_t v = idx[x];
-        uint32_t y = x;
-        while (y > b && idx[y - 1] > v) {
-            idx[y] = idx[y - 1];
-            --y;
-        }
-        idx[y] = v;
+    for (uint32_t x = b + lane; x < e; x += kCsrLanes) {
+        const uint32_t c = tmp[x];
+        uint32_t r = 0;
+        for (uint32_t y = b; y < e; ++y) r += tmp[y] < c ? 1u : 0u;
+        idx[b + r] = c;
+        peer[b + r] = peer_code(ab[c], i);
     }
+}
+
+// peer codes of a CSR given in sorted form (the granular solve)
+__global__ void __launch_bounds__(kBlock) k_csr_peers(uint32_t n, const uint32_t* off, const uint32_t* idx,
+                                                      const uint2* ab, uint32_t* peer) {
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = g / kCsrLanes, lane = g % kCsrLanes;
+    if (i >= n) return;
+    const uint32_t e = off[i + 1];
+    for (uint32_t x = off[i] + lane; x < e; x += kCsrLanes) peer[x] = peer_code(ab[idx[x]], i);
 }
 
 __global__ void __launch_bounds__(kBlock) k_unpack_contacts(const dio_avbd_contact* rec, uint32_t m, ContactArrays c) {
@@ -297,9 +316,17 @@ void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* ov
     if (mb > 2048) mb = 2048;
     hipLaunchKernelGGL(k_csr_count, dim3(mb), dim3(kBlock), 0, s, cs, b.cnt);
     launch_scan_exclusive(b.cnt, b.off, n, nullptr, b.scan_partials, s);
-    hipLaunchKernelGGL(k_csr_fill, dim3(mb), dim3(kBlock), 0, s, cs, (const uint32_t*)b.off, b.cur, b.idx, b.idx_cap);
-    hipLaunchKernelGGL(k_csr_sort, dim3(blocks_for(n > 0 ? n : 1)), dim3(kBlock), 0, s, n, (const uint32_t*)b.off,
-                       b.idx, b.idx_cap, overflow);
+    hipLaunchKernelGGL(k_csr_fill, dim3(mb), dim3(kBlock), 0, s, cs, (const uint32_t*)b.off, b.cur, b.tmp, b.idx_cap);
+    const uint32_t rb = blocks_for((n > 0 ? n : 1) * (uint32_t)kCsrLanes);
+    hipLaunchKernelGGL(k_csr_rank, dim3(rb), dim3(kBlock), 0, s, n, (const uint32_t*)b.off, (const uint32_t*)b.tmp,
+                       (const uint2*)cs.c.ab, b.idx, b.peer, b.idx_cap, overflow);
+}
+
+void launch_csr_peers(uint32_t n, const uint32_t* off, const uint32_t* idx, const uint2* ab, uint32_t* peer,
+                      hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_csr_peers, dim3(blocks_for(n * (uint32_t)kCsrLanes)), dim3(kBlock), 0, s, n, off, idx, ab,
+                       peer);
 }
 
 void launch_unpack_contacts(const dio_avbd_contact* rec, uint32_t m, const ContactArrays& c, hipStream_t s) {

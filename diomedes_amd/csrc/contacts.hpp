@@ -64,10 +64,16 @@ struct CsrBuffers {
     uint32_t* cnt;      // n_cap
     uint32_t* cur;      // n_cap
     uint32_t* off;      // n_cap + 1
-    uint32_t* idx;      // 2 * m_cap
+    uint32_t* idx;      // 2 * m_cap, ascending contact index within each body
+    uint32_t* peer;     // 2 * m_cap, peer code of each entry (kPeerIsB flag | other body, or kContainer)
+    uint32_t* tmp;      // 2 * m_cap, entries in atomic order before ranking
     uint32_t* scan_partials;
     uint32_t idx_cap;
 };
+
+// Peer codes of a sorted CSR (the granular solve's host-built CSR).
+void launch_csr_peers(uint32_t n, const uint32_t* off, const uint32_t* idx, const uint2* ab, uint32_t* peer,
+                      hipStream_t s);
 
 // Build `dst` from the narrow output `src` and the container, warm-started
 // from `old` (built from `old_src`).  `overflow` records a capacity excess.

@@ -13,6 +13,10 @@
 //     narrow, then the AVBD iterations) with no host synchronisation.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -95,6 +99,7 @@ BroadBuffers alloc_broad(DevArena& A, uint32_t n_cap, uint32_t p_max) {
     b.counts = A.alloc<uint32_t>(nc);
     b.offsets = A.alloc<uint32_t>(nc + 1);
     b.scan_partials = A.alloc<uint32_t>(scan_partials_size((uint32_t)nc + 1));
+    b.pair_cache = A.alloc<uint32_t>((size_t)kPairCache * n_cap);
     b.pairs = A.alloc<uint2>(p_max);
     b.p_dev = A.alloc<uint32_t>(1);
     b.overflow = A.alloc<uint32_t>(1);
@@ -202,10 +207,10 @@ struct dio_ctx {
     float4 *a_pos[2] = {nullptr, nullptr}, *a_rot[2] = {nullptr, nullptr};
     float4 *a_init_lin = nullptr, *a_init_ang = nullptr, *a_iner_lin = nullptr, *a_iner_ang = nullptr;
     ContactArrays a_c{};
-    uint32_t *a_off = nullptr, *a_idx = nullptr;
+    uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{8, 4, 4, {0, 0, 0, 0, 0}};
+    dio_tuning tuning{32, 1, 4, {0, 0, 0, 0, 0}};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -347,6 +352,7 @@ void ensure_granular_avbd(dio_ctx* c) {
     c->a_c = alloc_contacts(A, m);
     c->a_off = A.alloc<uint32_t>((size_t)n + 1);
     c->a_idx = A.alloc<uint32_t>(2 * (size_t)m);
+    c->a_peer = A.alloc<uint32_t>(2 * (size_t)m);
     c->a_j.abk = A.alloc<uint4>(c->c_cap);
     c->a_j.anc_a = A.alloc<float4>(c->c_cap);
     c->a_j.anc_b = A.alloc<float4>(c->c_cap);
@@ -431,11 +437,13 @@ struct KernelTimer {
         }
         return (*pool)[next++];
     }
-    template <class F> void time(int kind, hipStream_t s, F&& launch) {
+    // the launch records its own start / stop events (no marker packets)
+    template <class F> void time(int kind, F&& launch) {
         const size_t i = next;
-        DIO_HIP_CHECK(hipEventRecord(take(), s));
-        launch();
-        DIO_HIP_CHECK(hipEventRecord(take(), s));
+        LaunchEvents ev;
+        ev.start = take();
+        ev.stop = take();
+        launch(ev);
         marks.push_back({kind, i});
     }
 };
@@ -443,15 +451,15 @@ struct KernelTimer {
 void schedule_avbd(const SolveView& v, const AvbdParams& p, uint32_t iters, uint32_t m_grid, hipStream_t s,
                    KernelTimer* kt = nullptr, int lanes = 8) {
     auto run = [&](int kind, auto&& launch) {
-        if (kt) kt->time(kind, s, launch);
-        else launch();
+        if (kt) kt->time(kind, launch);
+        else launch(LaunchEvents{});
     };
-    run(0, [&] { launch_avbd_init(v, p, s); });
+    run(0, [&](const LaunchEvents& ev) { launch_avbd_init(v, p, s, ev); });
     for (uint32_t k = 0; k < iters; ++k) {
-        run(1, [&] { launch_avbd_primal(v, p, k % 2, s, lanes); });
-        run(2, [&] { launch_avbd_dual(v, p, k % 2, m_grid, s); });
+        run(1, [&](const LaunchEvents& ev) { launch_avbd_primal(v, p, k % 2, s, lanes, ev); });
+        run(2, [&](const LaunchEvents& ev) { launch_avbd_dual(v, p, k % 2, m_grid, s, ev); });
     }
-    run(3, [&] { launch_avbd_recover(v, p, (iters - 1) % 2, s); });
+    run(3, [&](const LaunchEvents& ev) { launch_avbd_recover(v, p, (iters - 1) % 2, s, ev); });
 }
 
 __global__ void k_stats_accum(const uint32_t* pairs, const uint32_t* m, const uint32_t* nnz, unsigned long long* acc) {
@@ -756,6 +764,8 @@ dio_status dio_avbd_submit(dio_ctx* ctx, const dio_avbd_body_state* state, const
         v.m_host = m;
         v.offsets = ctx->a_off;
         v.indices = ctx->a_idx;
+        v.peers = ctx->a_peer;
+        launch_csr_peers(n, ctx->a_off, ctx->a_idx, ctx->a_c.ab, ctx->a_peer, s);
         v.j_off = ctx->a_j.off; v.j_idx = ctx->a_j.idx; v.j_abk = ctx->a_j.abk;
         v.j_anc_a = ctx->a_j.anc_a; v.j_anc_b = ctx->a_j.anc_b;
         v.out_pos = nullptr;
@@ -841,6 +851,8 @@ void ensure_world(dio_ctx* c) {
     c->csr.cur = A.alloc<uint32_t>(n);
     c->csr.off = A.alloc<uint32_t>((size_t)n + 1);
     c->csr.idx = A.alloc<uint32_t>(2 * (size_t)m);
+    c->csr.peer = A.alloc<uint32_t>(2 * (size_t)m);
+    c->csr.tmp = A.alloc<uint32_t>(2 * (size_t)m);
     c->csr.scan_partials = A.alloc<uint32_t>(scan_partials_size(n + 1));
     c->csr.idx_cap = 2 * m;
     for (int k = 0; k < 2; ++k) {
@@ -921,6 +933,7 @@ SolveView world_solve_view(dio_ctx* c, int s) {
         v.m_host = 0;
         v.offsets = c->csr.off;
         v.indices = c->csr.idx;
+        v.peers = c->csr.peer;
         v.j_off = c->w_j.off; v.j_idx = c->w_j.idx; v.j_abk = c->w_j.abk;
         v.j_anc_a = c->w_j.anc_a; v.j_anc_b = c->w_j.anc_b;
         v.out_pos = c->w_pos;
@@ -1060,6 +1073,9 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         }
         hipLaunchKernelGGL(k_stats_zero, dim3(1), dim3(64), 0, s, ctx->w_acc);
         DIO_HIP_CHECK(hipEventRecord(ctx->events[0], s));
+        const bool host_timing = std::getenv("DIO_HOST_TIMING") != nullptr;
+        const auto h0 = std::chrono::steady_clock::now();
+        double h_launch = 0.0;
         for (uint32_t k = 0; k < nsteps && ctx->w_n > 0; ++k) {
             const int par = (int)(ctx->w_step % 2);
             const bool resort = ctx->w_sort_pending || ctx->w_step - ctx->w_sorted_at >= ctx->tuning.resort_interval;
@@ -1068,10 +1084,12 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
                 ctx->w_sort_pending = false;
             }
             if (use_graph) {
+                const auto a = std::chrono::steady_clock::now();
                 DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[resort ? 2 : 0][par], s));
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
                 DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[1][par], s));
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
+                h_launch += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
             } else {
                 enqueue_segment(ctx, 0, par, *options, so, nullptr, resort);
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
@@ -1082,7 +1100,14 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         }
         DIO_HIP_CHECK(hipEventRecord(ctx->events[1], s));
         DIO_HIP_CHECK(hipGetLastError());
+        const auto h1 = std::chrono::steady_clock::now();
         DIO_HIP_CHECK(hipStreamSynchronize(s));
+        if (host_timing) {
+            const auto h2 = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "dio_step host timing: %u steps, submit loop %.3f ms (graph launches %.3f ms), "
+                         "wait %.3f ms\n", nsteps, std::chrono::duration<double, std::milli>(h1 - h0).count(),
+                         h_launch, std::chrono::duration<double, std::milli>(h2 - h1).count());
+        }
         // stats of the last step
         dio_step_stats& st = ctx->stats;
         std::memset(&st, 0, sizeof(st));
@@ -1138,13 +1163,14 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
     return guarded(ctx, [&]() {
         if (!t) invalid("tuning is null");
         dio_tuning v = *t;
-        if (v.primal_lanes == 0) v.primal_lanes = 8;
-        if (v.broad_lanes == 0) v.broad_lanes = 4;
+        if (v.primal_lanes == 0) v.primal_lanes = 32;
+        if (v.broad_lanes == 0) v.broad_lanes = 1;
         if (v.resort_interval == 0) v.resort_interval = 4;
-        if (v.primal_lanes != 1 && v.primal_lanes != 4 && v.primal_lanes != 8 && v.primal_lanes != 16)
-            invalid("primal_lanes must be 1, 4, 8 or 16");
-        if (v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 && v.broad_lanes != 8)
-            invalid("broad_lanes must be 1, 2, 4 or 8");
+        if (v.primal_lanes != 1 && v.primal_lanes != 4 && v.primal_lanes != 8 && v.primal_lanes != 16 &&
+            v.primal_lanes != 32)
+            invalid("primal_lanes must be 1, 4, 8, 16 or 32");
+        if (v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 && v.broad_lanes != 8 && v.broad_lanes != 16)
+            invalid("broad_lanes must be 1, 2, 4, 8 or 16");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
@@ -1172,13 +1198,13 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
         const SolveView v = world_solve_view(ctx, par);
         const AvbdParams p = params_of(options->dt, options->gravity, options->alpha, options->beta_lin,
                                        options->penalty_max, ctx->w_n);
-        const int shapes[4] = {1, 4, 8, 16};
+        const int shapes[DIO_TUNE_PRIMAL_SHAPES] = {1, 4, 8, 16, 32};
         float best = 3.4e38f;
         int best_lanes = 8;
         hipEvent_t e0, e1;
         DIO_HIP_CHECK(hipEventCreate(&e0));
         DIO_HIP_CHECK(hipEventCreate(&e1));
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < DIO_TUNE_PRIMAL_SHAPES; ++k) {
             launch_avbd_primal(v, p, 0, s, shapes[k]);  // warm the code path
             DIO_HIP_CHECK(hipEventRecord(e0, s));
             for (uint32_t r = 0; r < reps; ++r) launch_avbd_primal(v, p, 0, s, shapes[k]);
@@ -1198,11 +1224,11 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
         // re-sorts of the fused step)
         ensure_granular_broad(ctx);
         const BodyView bv{ctx->w_pos, ctx->w_rot, ctx->w_tag, ctx->w_corners, ctx->w_hull};
-        const int blanes[4] = {1, 2, 4, 8};
+        const int blanes[DIO_TUNE_BROAD_SHAPES] = {1, 2, 4, 8, 16};
         float bbest = 3.4e38f;
         int best_blanes = 4;
         run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, true, 4);
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < DIO_TUNE_BROAD_SHAPES; ++k) {
             run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, false, blanes[k]);
             DIO_HIP_CHECK(hipEventRecord(e0, s));
             for (uint32_t r = 0; r < reps; ++r) run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, false, blanes[k]);
@@ -1211,7 +1237,7 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
             float ms = 0.0f;
             DIO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             const float us = 1000.0f * ms / (float)reps;
-            if (us_per_launch) us_per_launch[4 + k] = us;
+            if (us_per_launch) us_per_launch[DIO_TUNE_PRIMAL_SHAPES + k] = us;
             if (us < bbest) {
                 bbest = us;
                 best_blanes = blanes[k];

@@ -52,6 +52,7 @@ struct SolveView {
     // per-body contact CSR (physics/gpu.rs:286-310)
     const uint32_t* offsets;
     const uint32_t* indices;
+    const uint32_t* peers;   // peer code per CSR entry (kPeerIsB | other, or kContainer)
     // per-body joint CSR in ascending constraint order; joint records SoA
     const uint32_t* j_off;
     const uint32_t* j_idx;
@@ -63,11 +64,21 @@ struct SolveView {
     float4* out_rot;
 };
 
+// Optional timing events of one launch: recorded by the dispatch itself at the
+// kernel's start and end (null: a plain launch).
+struct LaunchEvents {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+
 // Launchers (avbd.hip).  All run on `stream`.
-void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t stream);
+void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t stream, const LaunchEvents& ev = {});
 // lanes: lanes per body (1 = one lane per body; 4, 8, 16 = grouped kernel)
-void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream, int lanes = 8);
-void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t m_grid, hipStream_t stream);
-void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream);
+void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream, int lanes = 8,
+                        const LaunchEvents& ev = {});
+void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t m_grid, hipStream_t stream,
+                      const LaunchEvents& ev = {});
+void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream,
+                         const LaunchEvents& ev = {});
 
 }  // namespace dio

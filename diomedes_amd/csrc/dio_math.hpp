@@ -17,6 +17,10 @@
 namespace dio {
 
 constexpr uint32_t kContainer = 0xFFFFFFFFu;
+// CSR peer code: the other body of a contact entry, with this flag set when
+// the entry's body is the contact's b; a container contact is kContainer
+// (its body is always a).
+constexpr uint32_t kPeerIsB = 0x80000000u;
 constexpr uint32_t kSphere = 0, kCube = 1, kTetrahedron = 2, kBox = 3, kHull = 4;
 constexpr uint32_t kSpring = 0, kRod = 1, kBallSocket = 2, kRope = 3;
 
