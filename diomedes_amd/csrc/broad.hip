@@ -1,9 +1,10 @@
-// broad.hip — LBVH broad phase for gfx950.
+// broad.hip — broad phase for gfx950.
 //
 // Replaces the reference's dense O(n^2) sweep (broadphase/src/lib.rs:154-216)
 // with: support-based AABBs (the same six support queries, same float
-// order), 30-bit Morton codes of the AABB centres, a stable LSD radix sort,
-// a Karras (2012) binary radix tree, an atomic bottom-up refit, and a
+// order), 30-bit Morton codes of the AABB centres on cubic cells, a stable LSD
+// radix sort (redone every few steps; in between the tree is refit over the
+// previous order), an implicit 8-ary BVH over the sorted bodies, and a
 // two-pass traversal (count, scan, write) in which each body reports the
 // overlapping bodies of higher index.  Leaves are tested with the reference's
 // closed-interval test on the reference's exact AABBs, so the pair set equals
@@ -19,9 +20,7 @@ namespace dio {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kStack = 64;
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
@@ -113,32 +112,59 @@ __global__ void __launch_bounds__(kBlock) k_morton(const float4* mn, const float
 }
 
 // ---------------------------------------------------------------------------
-// Implicit LBVH: a complete binary tree in heap layout over the bodies in
-// Morton order (perm[p] = body at sorted position p).  With N2 = the next
-// power of two >= n, node 0 is the root, node i has children 2i+1, 2i+2, and
-// sorted position p is the leaf N2 - 1 + p; positions p >= n are empty leaves
-// (inverted boxes that overlap nothing).  Boxes are refit bottom-up by
-// LDS reductions: chunks of C leaves per workgroup, then one workgroup for the
-// top levels -- no atomics, no inter-workgroup fences.  Leaf boxes are copies
-// of the bodies' AABBs, so the leaf test is the reference's exact closed-
-// interval test and the pair set equals the dense sweep's for any tree shape.
+// Implicit 8-ary BVH over the bodies in Morton order (perm[p] = body at sorted
+// position p).  With depth D (8^D >= n), node h has children 8h+1 .. 8h+8,
+// level L starts at heap index (8^L - 1) / 7, and sorted position p is leaf
+// (8^D - 1) / 7 + p.  A node whose first leaf position is >= n is empty: the
+// refit skips it and the traversal never reads it.  Leaf boxes are the bodies'
+// exact AABBs, so the leaf test is the reference's closed-interval test and
+// the pair set equals the dense sweep's for any tree shape; a stale Morton
+// order only loosens the boxes.  Refit: one workgroup per chunk of 512 leaves
+// (three levels reduced in LDS), then one workgroup for the levels above -- no
+// atomics.  Traversal: a group of LANES lanes per query tests the 8 children
+// of a node together (8 / LANES each) and keeps the group's stack in LDS, so
+// each step costs one round of independent loads; a shallow tree (5 levels at
+// 8192 bodies) keeps the chain of dependent steps short.
 
-__device__ __forceinline__ int ilog2(uint32_t x) { return 31 - __clz(x); }
+constexpr uint32_t kChunkLeaves = 512;  // 8^3 leaves per refit workgroup
+constexpr uint32_t kTopNodes = 585;     // levels 0..3, kept in LDS by the traversal
+constexpr int kTravBlock = 128;
+constexpr int kStackDepth = 64;         // >= 7 D + 1 for D <= 7
 
-// leaves and the lower levels of one chunk of C <= 1024 sorted positions
-constexpr uint32_t kMaxChunk = 1024;
-__global__ void __launch_bounds__(256) k_bvh_lower(const uint32_t* perm, uint32_t n, uint32_t n2, uint32_t chunk,
-                                                   const float4* mn, const float4* mx, float4* nmn, float4* nmx) {
-    __shared__ float4 lo[kMaxChunk], hi[kMaxChunk];
+__host__ __device__ __forceinline__ uint32_t level_start(uint32_t L) { return ((1u << (3 * L)) - 1u) / 7u; }
+
+__host__ __device__ __forceinline__ uint32_t tree_depth(uint32_t n) {
+    uint32_t d = 0;
+    while ((1u << (3 * d)) < n) ++d;
+    return d;
+}
+
+__device__ __forceinline__ float4 box_union_lo(float4 a, float4 b) {
+    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+}
+__device__ __forceinline__ float4 box_union_hi(float4 a, float4 b) {
+    return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), 0.0f);
+}
+
+// Leaves (the bodies' AABBs, computed here and also stored per body for the
+// queries) and the levels of one chunk of `chunk` sorted positions.
+__global__ void __launch_bounds__(256) k_tree_lower(BodyView bv, const uint32_t* perm, uint32_t n, uint32_t D,
+                                                    uint32_t chunk, float4* amn, float4* amx, float4* nmn,
+                                                    float4* nmx) {
+    __shared__ float4 lo[kChunkLeaves], hi[kChunkLeaves];
     const uint32_t base = blockIdx.x * chunk;
-    const uint32_t leaf0 = n2 - 1;
+    const uint32_t leaf0 = level_start(D);
     for (uint32_t t = threadIdx.x; t < chunk; t += 256) {
         const uint32_t p = base + t;
         float4 a = make_float4(3.4e38f, 3.4e38f, 3.4e38f, 0.0f), b = make_float4(-3.4e38f, -3.4e38f, -3.4e38f, 0.0f);
         if (p < n) {
             const uint32_t body = perm[p];
-            a = mn[body];
-            b = mx[body];
+            f3 u, w;
+            shape_aabb(load_shape(bv, body), HullPool{bv.hull}, u, w);
+            a = f4(u, 0.0f);
+            b = f4(w, 0.0f);
+            amn[body] = a;
+            amx[body] = b;
         }
         lo[t] = a;
         hi[t] = b;
@@ -146,134 +172,130 @@ __global__ void __launch_bounds__(256) k_bvh_lower(const uint32_t* perm, uint32_
         nmx[leaf0 + p] = b;
     }
     __syncthreads();
-    // level above the leaves: `width` nodes of this chunk starting at heap index
-    // level_first + blockIdx.x * width
-    uint32_t width = chunk >> 1;
-    uint32_t level_first = (leaf0 + 1) / 2 - 1;
+    uint32_t width = chunk / 8;  // this chunk's nodes on the level being built
+    uint32_t L = D;
     while (width >= 1) {
-        float4 r0[kMaxChunk / 512], r1[kMaxChunk / 512];
-        uint32_t k = 0;
-        for (uint32_t t = threadIdx.x; t < width; t += 256, ++k) {
-            const float4 a0 = lo[2 * t], a1 = hi[2 * t], b0 = lo[2 * t + 1], b1 = hi[2 * t + 1];
-            r0[k] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.0f);
-            r1[k] = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.0f);
-            nmn[level_first + blockIdx.x * width + t] = r0[k];
-            nmx[level_first + blockIdx.x * width + t] = r1[k];
-        }
-        __syncthreads();
-        k = 0;
-        for (uint32_t t = threadIdx.x; t < width; t += 256, ++k) {
-            lo[t] = r0[k];
-            hi[t] = r1[k];
-        }
-        __syncthreads();
-        width >>= 1;
-        level_first = (level_first + 1) / 2 - 1;
-    }
-}
-
-// the top levels: reduce the n2 / chunk chunk roots to the root (one workgroup)
-__global__ void __launch_bounds__(1024) k_bvh_top(uint32_t n2, uint32_t chunk, float4* nmn, float4* nmx) {
-    __shared__ float4 lo[1024], hi[1024];
-    uint32_t width = n2 / chunk;  // number of chunk roots
-    if (width <= 1) return;
-    uint32_t first = width - 1;  // heap index of the first chunk root
-    for (uint32_t t = threadIdx.x; t < width; t += 1024) {
-        lo[t] = nmn[first + t];
-        hi[t] = nmx[first + t];
-    }
-    __syncthreads();
-    while (width > 1) {
-        width >>= 1;
-        first = (first + 1) / 2 - 1;
-        float4 u0, u1;
-        const bool act = threadIdx.x < width;
-        if (act) {
-            const float4 a0 = lo[2 * threadIdx.x], a1 = hi[2 * threadIdx.x];
-            const float4 b0 = lo[2 * threadIdx.x + 1], b1 = hi[2 * threadIdx.x + 1];
-            u0 = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.0f);
-            u1 = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.0f);
-            nmn[first + threadIdx.x] = u0;
-            nmx[first + threadIdx.x] = u1;
-        }
-        __syncthreads();
-        if (act) {
-            lo[threadIdx.x] = u0;
-            hi[threadIdx.x] = u1;
-        }
-        __syncthreads();
-    }
-}
-
-constexpr uint32_t kTopCached = 511;  // nodes of the top 9 levels kept in LDS
-constexpr int kTravBlock = 128;
-
-// Traverse the subtree of one start node for body a; emit(b) for each
-// overlapping body b > a.  False if the stack would overflow (never for the
-// <= 22-level trees this library builds; the check keeps accesses in bounds).
-template <class Emit>
-__device__ __forceinline__ bool traverse_sub(uint32_t a, uint32_t start, uint32_t n, uint32_t n2, const uint32_t* perm,
-                                             const float4* mn, const float4* mx, const float4* nmn,
-                                             const float4* nmx, const float4* clo, const float4* chi,
-                                             uint32_t ncached, Emit&& emit) {
-    const float4 qa = mn[a], qb = mx[a];
-    const uint32_t leaf0 = n2 - 1;
-    uint32_t stack[kStack];
-    int sp = 0;
-    auto test = [&](uint32_t node) -> bool {
-        const float4 lo = node < ncached ? clo[node] : nmn[node];
-        const float4 hi = node < ncached ? chi[node] : nmx[node];
-        return aabb_overlap(qa, qb, lo, hi);
-    };
-    auto visit = [&](uint32_t node) -> bool {
-        if (node >= leaf0) {
-            const uint32_t p = node - leaf0;
-            if (p < n) {
-                const uint32_t b = perm[p];
-                if (b > a && aabb_overlap(qa, qb, mn[b], mx[b])) emit(b);
+        --L;
+        float4 u = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w = u;
+        const uint32_t t = threadIdx.x;  // width <= 64 < 256
+        if (t < width) {
+            u = lo[8 * t];
+            w = hi[8 * t];
+#pragma unroll
+            for (int j = 1; j < 8; ++j) {
+                u = box_union_lo(u, lo[8 * t + j]);
+                w = box_union_hi(w, hi[8 * t + j]);
             }
-            return true;
+            nmn[level_start(L) + blockIdx.x * width + t] = u;
+            nmx[level_start(L) + blockIdx.x * width + t] = w;
         }
-        if (!test(node)) return true;
-        if (sp >= kStack) return false;
-        stack[sp++] = node;
-        return true;
-    };
-    if (!visit(start)) return false;
+        __syncthreads();
+        if (t < width) {
+            lo[t] = u;
+            hi[t] = w;
+        }
+        __syncthreads();
+        width /= 8;
+    }
+}
+
+// Levels above the chunk roots (level D - 3), one workgroup; children whose
+// first leaf is past n were never built and are skipped.
+__global__ void __launch_bounds__(1024) k_tree_top(uint32_t n, uint32_t D, float4* nmn, float4* nmx) {
+    for (int L = (int)D - 4; L >= 0; --L) {
+        const uint32_t count = 1u << (3 * L);
+        const uint32_t span = 1u << (3 * (D - L - 1));  // leaves under one child
+        for (uint32_t k = threadIdx.x; k < count; k += 1024) {
+            const uint32_t c0 = level_start(L + 1) + 8 * k;
+            if ((uint64_t)(8 * k) * span >= n) continue;
+            float4 u = nmn[c0], w = nmx[c0];
+            for (uint32_t j = 1; j < 8; ++j) {
+                if ((uint64_t)(8 * k + j) * span >= n) break;
+                u = box_union_lo(u, nmn[c0 + j]);
+                w = box_union_hi(w, nmx[c0 + j]);
+            }
+            nmn[level_start(L) + k] = u;
+            nmx[level_start(L) + k] = w;
+        }
+        __syncthreads();
+    }
+}
+
+// lanes of one wavefront: order LDS / memory accesses of the group's lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the LANES bits of this lane's group in a wavefront ballot
+template <int LANES>
+__device__ __forceinline__ uint32_t group_bits(unsigned long long m) {
+    const uint32_t g = (threadIdx.x % 64u) / (uint32_t)LANES;
+    return (uint32_t)(m >> (g * LANES)) & ((1u << LANES) - 1u);
+}
+
+// Walk the tree for query body a (box qa..qb) with the LANES lanes of this
+// group.  hit(k, b, rank, total) is called on every lane for each child slot k
+// after each step: b is the partner found by this lane (or kNone), rank its
+// position among the group's hits of that slot, total their count.  Returns
+// false if the stack would overflow (never for D <= 7).
+template <int LANES, class Hit>
+__device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t n, uint32_t D, const uint32_t* perm,
+                                     const float4* nmn, const float4* nmx, const float4* clo, const float4* chi,
+                                     uint32_t ncached, uint32_t* stk, Hit&& hit) {
+    constexpr int kPer = 8 / LANES;
+    const uint32_t gl = threadIdx.x % (uint32_t)LANES;
+    const uint32_t below = (1u << gl) - 1u;
+    bool ok = true;
+    if (D == 0) return ok;
+    if (gl == 0) stk[0] = 0u;  // root, level 0
+    uint32_t sp = 1;
+    wave_sync();
     while (sp > 0) {
-        const uint32_t node = stack[--sp];
-        if (!visit(2 * node + 1)) return false;
-        if (!visit(2 * node + 2)) return false;
+        const uint32_t e = stk[sp - 1];
+        --sp;
+        wave_sync();
+        const uint32_t h = e & 0x00FFFFFFu, cl = (e >> 24) + 1u;  // children's level
+        const uint32_t span = 1u << (3 * (D - cl));
+        const uint32_t cs = level_start(cl);
+        const bool leaves = cl == D;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t c = 8 * h + 1 + gl + LANES * k;
+            const uint32_t first = (c - cs) * span;
+            bool ov = false;
+            if (first < n) {
+                const float4 lo = c < ncached ? clo[c] : nmn[c];
+                const float4 hi = c < ncached ? chi[c] : nmx[c];
+                ov = aabb_overlap(qa, qb, lo, hi);
+            }
+            uint32_t b = kNone;
+            if (leaves && ov) {
+                b = perm[first];
+                if (b <= a) b = kNone;
+            }
+            const uint32_t hm = group_bits<LANES>(__ballot(b != kNone));
+            hit(k, b, __popc(hm & below), __popc(hm));
+            const bool push = !leaves && ov;
+            const uint32_t pm = group_bits<LANES>(__ballot(push));
+            const uint32_t pos = sp + __popc(pm & below);
+            if (push) {
+                if (pos < (uint32_t)kStackDepth) stk[pos] = c | (cl << 24);
+                else ok = false;
+            }
+            sp += __popc(pm);
+            if (sp > (uint32_t)kStackDepth) sp = kStackDepth;
+        }
+        wave_sync();
     }
-    return true;
+    return ok;
 }
 
-// S lanes per body: lane s of the group walks the subtree of node S-1+s (the
-// S nodes of level log2 S).  Queries run in Morton order (group q takes body
-// perm[q]) so neighbouring lanes walk similar paths.
-//
-// Pass 1 (k_pairs_count) counts into counts[a*S + s] and keeps the first
-// kCache / S bodies it finds in cache[a*kCache + s*(kCache/S) ...].  Pass 2
-// (k_pairs_emit), after the scan, places each body's pairs by rank (b is
-// distinct within a body's list, so the rank among the list is its sorted
-// position): from the cache in LDS when every lane of the body fit, otherwise
-// by walking again and sorting the body's segment in place.
-template <int S>
-__device__ __forceinline__ bool lane_start(uint32_t n, uint32_t n2, uint32_t sub, uint32_t& start) {
-    // the subtree this lane walks; a tree smaller than S nodes is walked by lane 0
-    start = 0;
-    if (n <= 1) return false;
-    if (S > 1) {
-        if (n2 >= (uint32_t)S) start = (uint32_t)S - 1 + sub;
-        else return sub == 0;
-    }
-    return true;
-}
-
-__device__ __forceinline__ void load_top(const float4* nmn, const float4* nmx, uint32_t n2, float4* clo, float4* chi,
+__device__ __forceinline__ void load_top(const float4* nmn, const float4* nmx, uint32_t D, float4* clo, float4* chi,
                                          uint32_t& ncached) {
-    const uint32_t nnodes = 2 * n2 - 1;
-    ncached = nnodes < kTopCached ? nnodes : kTopCached;
+    const uint32_t nodes = level_start(D + 1);
+    ncached = nodes < kTopNodes ? nodes : kTopNodes;
     for (uint32_t t = threadIdx.x; t < ncached; t += kTravBlock) {
         clo[t] = nmn[t];
         chi[t] = nmx[t];
@@ -281,92 +303,80 @@ __device__ __forceinline__ void load_top(const float4* nmn, const float4* nmx, u
     __syncthreads();
 }
 
-template <int S>
-__global__ void __launch_bounds__(kTravBlock) k_pairs_count(uint32_t n, uint32_t n2, const uint32_t* perm,
-                                                            const float4* mn, const float4* mx, const float4* nmn,
-                                                            const float4* nmx, uint32_t* counts, uint32_t* cache,
-                                                            uint32_t* err) {
-    constexpr uint32_t C = kPairCache / S;
-    __shared__ float4 clo[kTopCached], chi[kTopCached];
+// Pass 1: count body a's partners b > a into counts[a] and keep the first
+// kPairCache of them in cache[a * kPairCache ...].  Groups take bodies in
+// Morton order, so the groups of a wavefront walk similar paths.
+template <int LANES>
+__global__ void __launch_bounds__(kTravBlock) k_tree_count(uint32_t n, uint32_t D, const uint32_t* perm,
+                                                           const float4* amn, const float4* amx, const float4* nmn,
+                                                           const float4* nmx, uint32_t* counts, uint32_t* cache,
+                                                           uint32_t* err) {
+    __shared__ float4 clo[kTopNodes], chi[kTopNodes];
+    __shared__ uint32_t stack[kTravBlock / LANES][kStackDepth];
     uint32_t ncached;
-    load_top(nmn, nmx, n2, clo, chi, ncached);
+    load_top(nmn, nmx, D, clo, chi, ncached);
     const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
-    const uint32_t q = gid / S, sub = gid % S;
-    if (q >= n) return;
+    const uint32_t q = gid / LANES;
+    if (q >= n) return;  // whole groups leave together (LANES divides 64)
     const uint32_t a = perm[q];
-    uint32_t start;
-    uint32_t c = 0;
-    uint32_t* mine = cache + (size_t)a * kPairCache + sub * C;
-    if (lane_start<S>(n, n2, sub, start) &&
-        !traverse_sub(a, start, n, n2, perm, mn, mx, nmn, nmx, clo, chi, ncached, [&](uint32_t b) {
-            if (c < C) mine[c] = b;
-            ++c;
-        }))
-        atomicOr(err, 1u);
-    counts[a * S + sub] = c;
+    uint32_t cnt = 0;
+    uint32_t* mine = cache + (size_t)a * kPairCache;
+    const bool ok = walk<LANES>(a, amn[a], amx[a], n, D, perm, nmn, nmx, clo, chi, ncached,
+                                stack[threadIdx.x / LANES], [&](int, uint32_t b, uint32_t rank, uint32_t total) {
+                                    if (b != kNone && cnt + rank < kPairCache) mine[cnt + rank] = b;
+                                    cnt += total;
+                                });
+    if (!ok) atomicOr(err, 1u);
+    if (threadIdx.x % LANES == 0) counts[a] = cnt;
 }
 
-template <int S>
-__global__ void __launch_bounds__(kTravBlock) k_pairs_emit(uint32_t n, uint32_t n2, const uint32_t* perm,
-                                                           const float4* mn, const float4* mx, const float4* nmn,
-                                                           const float4* nmx, const uint32_t* offsets,
-                                                           const uint32_t* cache, uint32_t p_max, uint2* pairs) {
-    constexpr uint32_t C = kPairCache / S;
-    constexpr int kBodies = kTravBlock / S;
-    __shared__ float4 clo[kTopCached], chi[kTopCached];
-    __shared__ uint32_t list[kBodies][kPairCache];
-    __shared__ uint32_t spill[kBodies];
-    const uint32_t slot = threadIdx.x / S;
-    if (threadIdx.x < (uint32_t)kBodies) spill[threadIdx.x] = 0u;
+// Pass 2: write body a's pairs at offsets[a], sorted by b.  When all of them
+// are cached, each one is placed at its rank in the list (the b's of one body
+// are distinct); otherwise the group walks again and sorts the segment.
+template <int LANES>
+__global__ void __launch_bounds__(kTravBlock) k_tree_emit(uint32_t n, uint32_t D, const uint32_t* perm,
+                                                          const float4* amn, const float4* amx, const float4* nmn,
+                                                          const float4* nmx, const uint32_t* offsets,
+                                                          const uint32_t* cache, uint32_t p_max, uint2* pairs) {
+    __shared__ float4 clo[kTopNodes], chi[kTopNodes];
+    __shared__ uint32_t stack[kTravBlock / LANES][kStackDepth];
+    __shared__ uint32_t list[kTravBlock / LANES][kPairCache];
     uint32_t ncached;
-    load_top(nmn, nmx, n2, clo, chi, ncached);  // includes a barrier
+    load_top(nmn, nmx, D, clo, chi, ncached);
     const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
-    const uint32_t q = gid / S, sub = gid % S;
-    const bool valid = q < n;
-    const uint32_t a = valid ? perm[q] : 0u;
-    uint32_t seg = 0, base = 0, cnt = 0;
-    if (valid) {
-        seg = offsets[a * S];
-        base = offsets[a * S + sub];
-        cnt = offsets[a * S + sub + 1] - base;
-        if (cnt > C) spill[slot] = 1u;
-    }
-    __syncthreads();
-    const bool fast = valid && spill[slot] == 0u;
-    const uint32_t* mine = cache + (size_t)a * kPairCache + sub * C;
-    if (fast) {
-        for (uint32_t j = 0; j < cnt; ++j) list[slot][base - seg + j] = mine[j];
-    } else if (valid) {
-        uint32_t start;
-        if (cnt > C) {
-            uint32_t k = 0;
-            if (lane_start<S>(n, n2, sub, start))
-                traverse_sub(a, start, n, n2, perm, mn, mx, nmn, nmx, clo, chi, ncached, [&](uint32_t b) {
-                    if (base + k < p_max) pairs[base + k] = make_uint2(a, b);
-                    ++k;
-                });
-        } else {
-            for (uint32_t j = 0; j < cnt; ++j)
-                if (base + j < p_max) pairs[base + j] = make_uint2(a, mine[j]);
-        }
-    }
-    __syncthreads();
-    if (fast) {
-        const uint32_t total = offsets[a * S + S] - seg;
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const uint32_t b = list[slot][base - seg + j];
+    const uint32_t q = gid / LANES, gl = threadIdx.x % LANES;
+    if (q >= n) return;
+    const uint32_t a = perm[q];
+    const uint32_t seg = offsets[a];
+    const uint32_t total = offsets[a + 1] - seg;
+    const uint32_t grp = threadIdx.x / LANES;
+    if (total <= kPairCache) {
+        const uint32_t* mine = cache + (size_t)a * kPairCache;
+        for (uint32_t j = gl; j < total; j += LANES) list[grp][j] = mine[j];
+        wave_sync();
+        for (uint32_t j = gl; j < total; j += LANES) {
+            const uint32_t b = list[grp][j];
             uint32_t rank = 0;
-            for (uint32_t x = 0; x < total; ++x) rank += list[slot][x] < b ? 1u : 0u;
+            for (uint32_t x = 0; x < total; ++x) rank += list[grp][x] < b ? 1u : 0u;
             if (seg + rank < p_max) pairs[seg + rank] = make_uint2(a, b);
         }
-    } else if (valid && sub == 0) {
-        const uint32_t b0 = seg;
-        uint32_t e = offsets[a * S + S];
+        return;
+    }
+    uint32_t k = 0;
+    walk<LANES>(a, amn[a], amx[a], n, D, perm, nmn, nmx, clo, chi, ncached, stack[grp],
+                [&](int, uint32_t b, uint32_t rank, uint32_t cnt) {
+                    if (b != kNone && seg + k + rank < p_max) pairs[seg + k + rank] = make_uint2(a, b);
+                    k += cnt;
+                });
+    __threadfence_block();
+    wave_sync();
+    if (gl == 0) {
+        uint32_t e = seg + total;
         if (e > p_max) e = p_max;
-        for (uint32_t x = b0 + 1; x < e; ++x) {
+        for (uint32_t x = seg + 1; x < e; ++x) {
             const uint2 v = pairs[x];
             uint32_t y = x;
-            while (y > b0 && pairs[y - 1].y > v.y) {
+            while (y > seg && pairs[y - 1].y > v.y) {
                 pairs[y] = pairs[y - 1];
                 --y;
             }
@@ -405,33 +415,28 @@ void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, 
 
 size_t BroadBuffers::sort_scratch_words(uint32_t n_cap) { return radix_scratch_size(n_cap); }
 
-uint32_t BroadBuffers::pow2_ceil(uint32_t n) {
-    uint32_t p = 1;
-    while (p < n) p <<= 1;
-    return p;
-}
+size_t BroadBuffers::tree_nodes(uint32_t n_cap) { return level_start(tree_depth(n_cap > 0 ? n_cap : 1) + 1); }
 
 namespace {
-template <int S>
-void launch_pairs(int mode, uint32_t n, uint32_t n2, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
-    const uint32_t blocks = (n * S + kTravBlock - 1) / kTravBlock;
+template <int LANES>
+void launch_tree_pairs(int mode, uint32_t n, uint32_t D, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
+    const uint32_t blocks = (n * LANES + kTravBlock - 1) / kTravBlock;
     if (mode == 0)
-        hipLaunchKernelGGL((k_pairs_count<S>), dim3(blocks), dim3(kTravBlock), 0, s, n, n2, (const uint32_t*)b.perm,
+        hipLaunchKernelGGL((k_tree_count<LANES>), dim3(blocks), dim3(kTravBlock), 0, s, n, D, (const uint32_t*)b.perm,
                            (const float4*)b.aabb_min, (const float4*)b.aabb_max, (const float4*)b.node_min,
                            (const float4*)b.node_max, b.counts, b.pair_cache, b.err);
     else
-        hipLaunchKernelGGL((k_pairs_emit<S>), dim3(blocks), dim3(kTravBlock), 0, s, n, n2, (const uint32_t*)b.perm,
+        hipLaunchKernelGGL((k_tree_emit<LANES>), dim3(blocks), dim3(kTravBlock), 0, s, n, D, (const uint32_t*)b.perm,
                            (const float4*)b.aabb_min, (const float4*)b.aabb_max, (const float4*)b.node_min,
-                           (const float4*)b.node_max, (const uint32_t*)b.offsets, (const uint32_t*)b.pair_cache, p_max,
-                           b.pairs);
+                           (const float4*)b.node_max, (const uint32_t*)b.offsets, (const uint32_t*)b.pair_cache,
+                           p_max, b.pairs);
 }
-void pairs_pass(int lanes, int mode, uint32_t n, uint32_t n2, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
+void tree_pairs(int lanes, int mode, uint32_t n, uint32_t D, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
     switch (lanes) {
-        case 1: launch_pairs<1>(mode, n, n2, b, p_max, s); break;
-        case 2: launch_pairs<2>(mode, n, n2, b, p_max, s); break;
-        case 8: launch_pairs<8>(mode, n, n2, b, p_max, s); break;
-        case 16: launch_pairs<16>(mode, n, n2, b, p_max, s); break;
-        default: launch_pairs<4>(mode, n, n2, b, p_max, s); break;
+        case 1: launch_tree_pairs<1>(mode, n, D, b, p_max, s); break;
+        case 2: launch_tree_pairs<2>(mode, n, D, b, p_max, s); break;
+        case 4: launch_tree_pairs<4>(mode, n, D, b, p_max, s); break;
+        default: launch_tree_pairs<8>(mode, n, D, b, p_max, s); break;
     }
 }
 }  // namespace
@@ -443,26 +448,24 @@ void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p
         hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets, p_max, b.p_dev, b.overflow);
         return;
     }
-    if (lanes != 1 && lanes != 2 && lanes != 8 && lanes != 16) lanes = 4;
-    hipLaunchKernelGGL(k_aabb, dim3(blocks_for(n)), dim3(kBlock), 0, s, bv, n, b.aabb_min, b.aabb_max);
     if (resort) {
+        hipLaunchKernelGGL(k_aabb, dim3(blocks_for(n)), dim3(kBlock), 0, s, bv, n, b.aabb_min, b.aabb_max);
         hipLaunchKernelGGL(k_bounds, dim3(1), dim3(1024), 0, s, b.aabb_min, b.aabb_max, n, b.bounds);
         hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.aabb_min, b.aabb_max, n, b.bounds,
                            b.keys0, b.perm);
         // 30-bit keys: four 8-bit passes, so the sorted values land back in perm
         radix_sort_pairs(b.keys0, b.perm, b.keys1, b.vals1, n, 30, b.sort_scratch, s);
     }
-    const uint32_t n2 = BroadBuffers::pow2_ceil(n);
-    uint32_t chunk = n2 < 512u ? n2 : 512u;
-    while (n2 / chunk > 1024u) chunk <<= 1;
-    hipLaunchKernelGGL(k_bvh_lower, dim3(n2 / chunk), dim3(256), 0, s, (const uint32_t*)b.perm, n, n2, chunk,
-                       (const float4*)b.aabb_min, (const float4*)b.aabb_max, b.node_min, b.node_max);
-    hipLaunchKernelGGL(k_bvh_top, dim3(1), dim3(1024), 0, s, n2, chunk, b.node_min, b.node_max);
-    const uint32_t nc = n * (uint32_t)lanes;
-    pairs_pass(lanes, 0, n, n2, b, p_max, s);
-    launch_scan_exclusive(b.counts, b.offsets, nc, nullptr, b.scan_partials, s);
-    pairs_pass(lanes, 1, n, n2, b, p_max, s);
-    hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets + nc, p_max, b.p_dev, b.overflow);
+    const uint32_t D = tree_depth(n);
+    const uint32_t full = 1u << (3 * D);
+    const uint32_t chunk = full < kChunkLeaves ? full : kChunkLeaves;
+    hipLaunchKernelGGL(k_tree_lower, dim3((n + chunk - 1) / chunk), dim3(256), 0, s, bv, (const uint32_t*)b.perm, n, D,
+                       chunk, b.aabb_min, b.aabb_max, b.node_min, b.node_max);
+    if (D > 3) hipLaunchKernelGGL(k_tree_top, dim3(1), dim3(1024), 0, s, n, D, b.node_min, b.node_max);
+    tree_pairs(lanes, 0, n, D, b, p_max, s);
+    launch_scan_exclusive(b.counts, b.offsets, n, nullptr, b.scan_partials, s);
+    tree_pairs(lanes, 1, n, D, b, p_max, s);
+    hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets + n, p_max, b.p_dev, b.overflow);
 }
 
 }  // namespace dio

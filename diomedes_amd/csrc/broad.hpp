@@ -27,31 +27,30 @@ struct BroadBuffers {
     uint32_t *keys0, *keys1, *vals1;          // n_cap each (radix sort ping-pong)
     uint32_t* perm;                           // n_cap: body at each Morton-sorted position
     uint32_t* sort_scratch;                   // sort_scratch_words(n_cap)
-    float4 *node_min, *node_max;              // 2 * pow2_ceil(n_cap) - 1 heap nodes
-    uint32_t* counts;                         // kMaxTraversalLanes * n_cap
-    uint32_t* offsets;                        // kMaxTraversalLanes * n_cap + 1
-    uint32_t* scan_partials;                  // scan_partials_size(kMaxTraversalLanes * n_cap + 1)
+    float4 *node_min, *node_max;              // tree_nodes(n_cap) nodes of the implicit 8-ary tree
+    uint32_t* counts;                         // n_cap: partners of higher index per body
+    uint32_t* offsets;                        // n_cap + 1
+    uint32_t* scan_partials;                  // scan_partials_size(n_cap + 1)
     uint32_t* pair_cache;                     // kPairCache * n_cap: partners found by the counting pass
     uint2* pairs;                             // p_max
     uint32_t* p_dev;                          // pair count min(total, p_max)
     uint32_t* overflow;                       // largest total that exceeded p_max
     uint32_t* err;                            // traversal stack overflow flag
     static size_t sort_scratch_words(uint32_t n_cap);
-    static uint32_t pow2_ceil(uint32_t n);
+    static size_t tree_nodes(uint32_t n_cap);
 };
 
-constexpr uint32_t kMaxTraversalLanes = 16;
 constexpr uint32_t kPairCache = 32;  // partners per body kept between the two traversal passes
-constexpr uint32_t kMaxBroadBodies = 1u << 20;  // two-level refit: 1024 chunks of <= 1024 leaves
+constexpr uint32_t kMaxBroadBodies = 1u << 20;  // tree depth <= 7: the top refit workgroup covers 4 levels
 
 void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, float4* rot, uint32_t* tag,
                               float4* corners, hipStream_t s);
 
-// AABBs, LBVH and the sorted pair list of n bodies; the pair count stays on
+// AABBs, BVH and the sorted pair list of n bodies; the pair count stays on
 // the device (b.p_dev).  resort: recompute Morton codes and the sorted order
 // (otherwise the tree is refit over the previous order, which keeps the pair
-// set exact and only changes the traversal cost).  lanes: traversal lanes per
-// body (1, 2, 4, 8 or 16).
+// set exact and only changes the traversal cost).  lanes: lanes per query in
+// the traversal (1, 2, 4 or 8; each tests 8 / lanes children per step).
 void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s, bool resort,
                int lanes);
 

@@ -84,8 +84,8 @@ struct PinnedArena {
 
 BroadBuffers alloc_broad(DevArena& A, uint32_t n_cap, uint32_t p_max) {
     BroadBuffers b{};
-    const size_t nodes = 2 * (size_t)BroadBuffers::pow2_ceil(n_cap) - 1;
-    const size_t nc = (size_t)kMaxTraversalLanes * n_cap;
+    const size_t nodes = BroadBuffers::tree_nodes(n_cap);
+    const size_t nc = n_cap;
     b.aabb_min = A.alloc<float4>(n_cap);
     b.aabb_max = A.alloc<float4>(n_cap);
     b.bounds = A.alloc<float4>(2);
@@ -210,7 +210,7 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{32, 1, 4, {0, 0, 0, 0, 0}};
+    dio_tuning tuning{32, 8, 4, {0, 0, 0, 0, 0}};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -516,7 +516,7 @@ dio_status dio_create(int device, const dio_limits* limits, dio_ctx** out) {
         c->m_cap = limits->max_contacts ? limits->max_contacts : 4u * c->p_max + 24u * c->n_cap;
         c->c_cap = limits->max_constraints ? limits->max_constraints : std::max<uint32_t>(16u, c->n_cap);
         c->h_cap = limits->max_hull_vertices ? limits->max_hull_vertices : 4096u;
-        const uint32_t big = std::max({c->n_cap * kMaxTraversalLanes + 1, c->p_max + 1, c->m_cap + 1});
+        const uint32_t big = std::max({c->n_cap + 1, c->p_max + 1, c->m_cap + 1});
         c->scan_partials = c->dev.alloc<uint32_t>(scan_partials_size(big));
         return DIO_OK;
     });
@@ -572,8 +572,7 @@ dio_status dio_broad_count(dio_ctx* ctx, uint32_t* count) {
         if (!count) invalid("count is null");
         if (!ctx->g_broad_submitted) throw StatusError{DIO_NOT_READY, "no broad phase submitted", 0};
         uint32_t v[3] = {0, 0, 0};
-        DIO_HIP_CHECK(hipMemcpyAsync(&v[0], ctx->gb.offsets + (size_t)ctx->g_n * ctx->g_lanes, 4, hipMemcpyDeviceToHost,
-                                     ctx->stream));
+        DIO_HIP_CHECK(hipMemcpyAsync(&v[0], ctx->gb.offsets + ctx->g_n, 4, hipMemcpyDeviceToHost, ctx->stream));
         DIO_HIP_CHECK(hipMemcpyAsync(&v[1], ctx->gb.overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
         DIO_HIP_CHECK(hipMemcpyAsync(&v[2], ctx->gb.err, 4, hipMemcpyDeviceToHost, ctx->stream));
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1164,13 +1163,13 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         if (!t) invalid("tuning is null");
         dio_tuning v = *t;
         if (v.primal_lanes == 0) v.primal_lanes = 32;
-        if (v.broad_lanes == 0) v.broad_lanes = 1;
+        if (v.broad_lanes == 0) v.broad_lanes = 8;
         if (v.resort_interval == 0) v.resort_interval = 4;
         if (v.primal_lanes != 1 && v.primal_lanes != 4 && v.primal_lanes != 8 && v.primal_lanes != 16 &&
             v.primal_lanes != 32)
             invalid("primal_lanes must be 1, 4, 8, 16 or 32");
-        if (v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 && v.broad_lanes != 8 && v.broad_lanes != 16)
-            invalid("broad_lanes must be 1, 2, 4, 8 or 16");
+        if (v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 && v.broad_lanes != 8)
+            invalid("broad_lanes must be 1, 2, 4 or 8");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
@@ -1224,9 +1223,9 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
         // re-sorts of the fused step)
         ensure_granular_broad(ctx);
         const BodyView bv{ctx->w_pos, ctx->w_rot, ctx->w_tag, ctx->w_corners, ctx->w_hull};
-        const int blanes[DIO_TUNE_BROAD_SHAPES] = {1, 2, 4, 8, 16};
+        const int blanes[DIO_TUNE_BROAD_SHAPES] = {1, 2, 4, 8};
         float bbest = 3.4e38f;
-        int best_blanes = 4;
+        int best_blanes = 8;
         run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, true, 4);
         for (int k = 0; k < DIO_TUNE_BROAD_SHAPES; ++k) {
             run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, false, blanes[k]);

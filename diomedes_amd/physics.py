@@ -484,14 +484,14 @@ class PhysicsContext:
         world, keep the fastest of each; returns the choice and the per-shape
         microseconds."""
         t = R.Tuning()
-        us = (ctypes.c_float * 10)()
+        us = (ctypes.c_float * 9)()
         o = options.to_c()
         self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
         return {
             "primal_lanes": t.primal_lanes,
             "broad_lanes": t.broad_lanes,
             "us_per_launch": dict(zip((1, 4, 8, 16, 32), list(us)[:5])),
-            "broad_us_per_pass": dict(zip((1, 2, 4, 8, 16), list(us)[5:])),
+            "broad_us_per_pass": dict(zip((1, 2, 4, 8), list(us)[5:])),
         }
 
     def stats(self) -> R.StepStats:

@@ -25,7 +25,7 @@ def _print(*a):
 
 
 # ------------------------------------------------------------ broad phase
-@pytest.mark.parametrize("blanes", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("blanes", [1, 2, 4, 8])
 @pytest.mark.parametrize("nx,ny,nz,seed", [(4, 2, 4, 1), (8, 2, 8, 2), (32, 8, 32, 0), (1, 1, 3, 5)])
 def test_broad_pairs_match_dense_sweep(oracle, nx, ny, nz, seed, blanes):
     _print("broad", nx, ny, nz, seed, blanes)
@@ -195,7 +195,7 @@ def test_box_stack_300_steps_bitexact(oracle):
 
 @pytest.mark.parametrize(
     "lag,lanes,blanes,resort",
-    [(1, 8, 4, 4), (0, 8, 4, 4), (1, 4, 2, 1), (1, 16, 8, 1000), (1, 1, 1, 4), (0, 4, 8, 1000), (1, 32, 16, 4)],
+    [(1, 8, 4, 4), (0, 8, 4, 4), (1, 4, 2, 1), (1, 16, 8, 1000), (1, 1, 1, 4), (0, 4, 8, 1000), (1, 32, 1, 4)],
 )
 def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort):
     """resort: Morton re-sort interval of the fused step; 1000 refits the first
@@ -222,7 +222,7 @@ def test_autotune_keeps_state_and_parity(oracle):
         res = ctx.autotune(scene.options, reps=5)
         _print("autotune", res)
         assert res["primal_lanes"] in (1, 4, 8, 16, 32)
-        assert res["broad_lanes"] in (1, 2, 4, 8, 16)
+        assert res["broad_lanes"] in (1, 2, 4, 8)
         _compare_world(ctx, w, 40)
         ctx.step(scene.options, 20)
         w.step(nsteps=20)
