@@ -34,17 +34,22 @@ HBM_PEAK_GBS = 8000.0
 
 
 def avbd_bytes(n_dyn: int, n: int, iters: int, m: float, k: float) -> float:
-    """Algorithmic (compulsory) HBM bytes of one AVBD solve:
-    init 152 N_dyn + I (primal 136 N_dyn + 4 K + 100 M + dual 124 M + 56 N)
-    + recover 92 N_dyn."""
-    return 152 * n_dyn + iters * (136 * n_dyn + 4 * k + 100 * m + 124 * m + 56 * n) + 92 * n_dyn
+    """Algorithmic (compulsory) HBM bytes of one AVBD solve: init 152 N_dyn;
+    I primal launches of 136 N_dyn + 8 K + 100 M, plus 24 M for the fused
+    dual update's lambda / penalty store in iterations 1 .. I-1; the final
+    dual + recover 124 M + 56 N + 92 N_dyn."""
+    return (152 * n_dyn + iters * (136 * n_dyn + 8 * k + 100 * m) + (iters - 1) * 24 * m
+            + 124 * m + 56 * n + 92 * n_dyn)
 
 
-def primal_bytes(n_dyn: int, m: float, k: float) -> float:
-    """Primal kernel, one launch: 136 B per dynamic body (state, snapshot,
-    initial, inertial, CSR offsets, new pose) + 4 B per CSR entry + 100 B per
-    contact (its record's used lanes); gathers of neighbour poses hit cache."""
-    return 136 * n_dyn + 4 * k + 100 * m
+def primal_bytes(n_dyn: int, m: float, k: float, iters: int) -> float:
+    """Primal kernel, average over the I launches of a solve: 136 B per
+    dynamic body (state, snapshot, initial, inertial, CSR offsets, new pose),
+    8 B per CSR entry (contact index, peer code), 100 B per contact (its
+    record's used lanes) and, in I - 1 of the I launches, the 24 B lambda /
+    penalty store of the fused dual update.  Gathers of neighbour poses are
+    counted once per body (they hit cache)."""
+    return 136 * n_dyn + 8 * k + 100 * m + 24 * m * (iters - 1) / iters
 
 
 def cpu_baseline(scene, warmup: int, steps: int, budget_s: float = 25.0) -> dict:
@@ -95,6 +100,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-steps", type=int, default=20, help="steps of the per-kernel timing pass")
+    ap.add_argument("--tune-steps", type=int, default=150,
+                    help="steps of a separate run of the same scene after which dio_autotune picks the launch "
+                         "shapes and dual schedule (0: library defaults)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (reference for HBM counters)")
     args = ap.parse_args()
@@ -123,7 +131,19 @@ def main() -> int:
     opts = scene.options
     n, n_dyn, iters = scene.n, scene.n_dynamic, opts.iterations
 
+    tuning = None
+    if args.tune_steps:
+        # launch shapes chosen on the settled scene in a separate context
+        with PhysicsContext(n) as tctx:
+            tctx.upload(scene)
+            tctx.step(opts, args.tune_steps)
+            tctx.autotune(opts, reps=20)
+            tuning = tctx.tuning()
+
     with PhysicsContext(n) as ctx:
+        if tuning:
+            ctx.set_tuning(**tuning)
+        tuning = ctx.tuning()
         ctx.upload(scene)
         if args.warmup:
             ctx.step(opts, args.warmup)
@@ -148,7 +168,7 @@ def main() -> int:
     primal_ms = pst.ms_primal / p_launches
     pm = pst.sum_contacts / max(pst.steps, 1)
     pk = pst.sum_csr_nnz / max(pst.steps, 1)
-    pbytes = primal_bytes(n_dyn, pm, pk)
+    pbytes = primal_bytes(n_dyn, pm, pk, iters)
     achieved = pbytes / (primal_ms / 1000.0) / 1e9
     step_bytes = avbd_bytes(n_dyn, n, iters, avg_m, avg_k)
     line = {
@@ -171,6 +191,7 @@ def main() -> int:
             "seed": args.seed,
             "mode": "jacobi (reference parity)",
         },
+        "tuning": tuning,
         "ms_per_step_avbd": ms_avbd / k,
         "body_iters_per_s_full_step": n_dyn * iters * k / wall,
         "avg_contacts": avg_m,
@@ -193,8 +214,7 @@ def main() -> int:
         "kernel_avg_us": {
             "init": 1000.0 * pst.ms_init / max(pst.n_init, 1),
             "primal": 1000.0 * primal_ms,
-            "dual": 1000.0 * pst.ms_dual / max(pst.n_dual, 1),
-            "recover": 1000.0 * pst.ms_recover / max(pst.n_recover, 1),
+            "dual_recover": 1000.0 * pst.ms_recover / max(pst.n_recover, 1),
         },
     }
     if not args.no_cpu:

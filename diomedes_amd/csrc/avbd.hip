@@ -140,6 +140,43 @@ __device__ __forceinline__ Force3 contact_force(float4 pen, float4 lam, float fr
     return r;
 }
 
+// avbd_dual (avbd/src/lib.rs:565-663) for one contact: the constraint value
+// from both bodies' displacements and world lever arms (the b terms are zero
+// for a container contact), the clamped force as the new lambda, and the
+// penalty ramp.  r0..r2 are the contact's normal and tangents as stored (a's
+// convention).
+struct DualState {
+    float4 lam, pen;
+};
+__device__ __forceinline__ DualState dual_update(const AvbdParams& p, float kk, f3 r0, f3 r1, f3 r2, float friction,
+                                                 f3 raw, f3 rbw, f3 dal, f3 daa, f3 dbl, f3 dba, float4 c04,
+                                                 float4 pen, float4 lam) {
+    float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
+    c0 = c0 + (dot3(r0, dal) + dot3(cross3(raw, r0), daa) - dot3(r0, dbl) - dot3(cross3(rbw, r0), dba));
+    c1 = c1 + (dot3(r1, dal) + dot3(cross3(raw, r1), daa) - dot3(r1, dbl) - dot3(cross3(rbw, r1), dba));
+    c2 = c2 + (dot3(r2, dal) + dot3(cross3(raw, r2), daa) - dot3(r2, dbl) - dot3(cross3(rbw, r2), dba));
+    const Force3 F = contact_force(pen, lam, friction, c0, c1, c2);
+    DualState st;
+    st.lam = make_float4(F.f0, F.f1, F.f2, 0.0f);
+    if (F.f0 < 0.0f) pen.x = rmin(pen.x + p.beta_lin * fabsf(c0), p.penalty_max);
+    if (F.tang <= F.bounds) {
+        pen.y = rmin(pen.y + p.beta_lin * fabsf(c1), p.penalty_max);
+        pen.z = rmin(pen.z + p.beta_lin * fabsf(c2), p.penalty_max);
+    }
+    st.pen = pen;
+    return st;
+}
+
+// The multipliers after j dual updates live in buffer j % 2 (0: the contact
+// set's own lam / pen, 1: the scratch pair).  Primal iteration k >= 1 first
+// performs dual update k - 1 for each of its contacts (both bodies of a
+// contact evaluate it identically; body a stores it, or body b when a is
+// static), so a solve is init, I primal launches and one final dual +
+// recover launch.  A contact between two static bodies is read by no primal;
+// the final launch applies its I updates at once (its poses do not move).
+__device__ __forceinline__ float4* lam_buf(const SolveView& v, uint32_t j) { return (j & 1u) ? v.c_lam_alt : v.c_lam; }
+__device__ __forceinline__ float4* pen_buf(const SolveView& v, uint32_t j) { return (j & 1u) ? v.c_pen_alt : v.c_pen; }
+
 __global__ void __launch_bounds__(kBlock) k_avbd_init(SolveView v, AvbdParams p) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= p.n) return;
@@ -177,7 +214,14 @@ __global__ void __launch_bounds__(kBlock) k_avbd_init(SolveView v, AvbdParams p)
     v.rot[0][i] = f4q(wr);
 }
 
-__global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams p, uint32_t parity) {
+template <bool FUSED>
+__global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams p, uint32_t iter) {
+    const uint32_t parity = iter & 1u;
+    const bool fuse = FUSED && iter > 0;
+    const float4* __restrict__ lam_in = lam_buf(v, fuse ? iter - 1 : 0);
+    const float4* __restrict__ pen_in = pen_buf(v, fuse ? iter - 1 : 0);
+    float4* lam_out = lam_buf(v, iter);
+    float4* pen_out = pen_buf(v, iter);
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= p.n) return;
     const float4 imm = v.inv_mm[i];
@@ -239,8 +283,21 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
         c0 = c0 + (dot3(r0, dsl) + dot3(j0, dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
         c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
         c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
-        const float4 pen = v.c_pen[ci];
-        const Force3 F = contact_force(pen, v.c_lam[ci], n4.w, c0, c1, c2);
+        float4 pen = pen_in[ci];
+        float4 lam = lam_in[ci];
+        if (fuse) {
+            const DualState d = is_a ? dual_update(p, kk, n0, t1, t2, n4.w, r_world, r_other, dsl, dsa, dol, doa,
+                                                   c04, pen, lam)
+                                     : dual_update(p, kk, n0, t1, t2, n4.w, r_other, r_world, dol, doa, dsl, dsa,
+                                                   c04, pen, lam);
+            pen = d.pen;
+            lam = d.lam;
+            if (is_a || v.inv_mm[other].w == 0.0f) {
+                lam_out[ci] = lam;
+                pen_out[ci] = pen;
+            }
+        }
+        const Force3 F = contact_force(pen, lam, n4.w, c0, c1, c2);
         B.add_lin(r0, pen.x); B.add_lin(r1, pen.y); B.add_lin(r2, pen.z);
         B.add_ang(j0, pen.x); B.add_ang(j1, pen.y); B.add_ang(j2, pen.z);
         B.add_cross(j0, r0, pen.x); B.add_cross(j1, r1, pen.y); B.add_cross(j2, r2, pen.z);
@@ -319,8 +376,14 @@ constexpr int kRowFloats = 8;
 constexpr int kItemFloats = 3 * kRowFloats + 1;  // 3 rows + row count
 }  // namespace
 
-template <int G>
-__global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t parity) {
+template <int G, bool FUSED>
+__global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t iter) {
+    const uint32_t parity = iter & 1u;
+    const bool fuse = FUSED && iter > 0;
+    const float4* __restrict__ lam_in = lam_buf(v, fuse ? iter - 1 : 0);
+    const float4* __restrict__ pen_in = pen_buf(v, fuse ? iter - 1 : 0);
+    float4* lam_out = lam_buf(v, iter);
+    float4* pen_out = pen_buf(v, iter);
     constexpr int kBodies = kBlock / G;
     constexpr int kOwned = (27 + G - 1) / G;
     __shared__ float stage[kBodies][G][kItemFloats];
@@ -425,8 +488,20 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
                 c0 = c0 + (dot3(r0, dsl) + dot3(j0, dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
                 c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
                 c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
-                const float4 pen = v.c_pen[ci];
-                const Force3 F = contact_force(pen, v.c_lam[ci], n4.w, c0, c1, c2);
+                float4 pen = pen_in[ci];
+                float4 lam = lam_in[ci];
+                if (fuse) {
+                    const DualState d =
+                        is_a ? dual_update(p, kk, n0, t1, t2, n4.w, r_world, r_other, dsl, dsa, dol, doa, c04, pen, lam)
+                             : dual_update(p, kk, n0, t1, t2, n4.w, r_other, r_world, dol, doa, dsl, dsa, c04, pen, lam);
+                    pen = d.pen;
+                    lam = d.lam;
+                    if (is_a || v.inv_mm[other].w == 0.0f) {
+                        lam_out[ci] = lam;
+                        pen_out[ci] = pen;
+                    }
+                }
+                const Force3 F = contact_force(pen, lam, n4.w, c0, c1, c2);
                 const f3 jl[3] = {r0, r1, r2};
                 const f3 ja[3] = {j0, j1, j2};
                 const float fr[3] = {F.f0, F.f1, F.f2};
@@ -535,47 +610,74 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_avbd_dual(SolveView v, AvbdParams p, uint32_t parity) {
-    const uint32_t m = v.m_dev ? *v.m_dev : v.m_host;
+// One dual update of contact ci from the Jacobi poses written by a primal of
+// parity `parity` (pos[1 - parity]): multipliers read from buffer src, stored
+// to buffer dst.  static_updates > 0 (fused schedule, final launch): a
+// contact between two static bodies starts from buffer 0 and takes that many
+// updates instead (its poses never move, no primal carried its updates).
+__device__ __forceinline__ void dual_one(const SolveView& v, const AvbdParams& p, uint32_t ci, uint32_t parity,
+                                         uint32_t src, uint32_t dst, uint32_t static_updates) {
     const float4* __restrict__ wp = v.pos[1 - parity];
     const float4* __restrict__ wr = v.rot[1 - parity];
     const float kk = 1.0f - p.alpha;
     const f3 zero{0.0f, 0.0f, 0.0f};
-    for (uint32_t ci = blockIdx.x * kBlock + threadIdx.x; ci < m; ci += gridDim.x * kBlock) {
-        const uint2 ab = v.c_ab[ci];
-        const q4 rot_a = as_q(wr[ab.x]);
-        const f3 dal = sub(xyz(wp[ab.x]), xyz(v.init_lin[ab.x]));
-        const f3 daa = ang_diff(rot_a, as_q(v.init_ang[ab.x]));
-        f3 dbl = zero, dba = zero, rbw = zero;
-        if (ab.y != kContainer) {
-            const q4 rot_b = as_q(wr[ab.y]);
-            dbl = sub(xyz(wp[ab.y]), xyz(v.init_lin[ab.y]));
-            dba = ang_diff(rot_b, as_q(v.init_ang[ab.y]));
-            rbw = qrot(rot_b, xyz(v.c_rb[ci]));
-        }
-        const f3 raw = qrot(rot_a, xyz(v.c_ra[ci]));
-        const float4 n4 = v.c_n[ci];
-        const f3 r0 = xyz(n4), r1 = xyz(v.c_t1[ci]), r2 = xyz(v.c_t2[ci]);
-        const float4 c04 = v.c_c0[ci];
-        float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
-        c0 = c0 + (dot3(r0, dal) + dot3(cross3(raw, r0), daa) - dot3(r0, dbl) - dot3(cross3(rbw, r0), dba));
-        c1 = c1 + (dot3(r1, dal) + dot3(cross3(raw, r1), daa) - dot3(r1, dbl) - dot3(cross3(rbw, r1), dba));
-        c2 = c2 + (dot3(r2, dal) + dot3(cross3(raw, r2), daa) - dot3(r2, dbl) - dot3(cross3(rbw, r2), dba));
-        float4 pen = v.c_pen[ci];
-        const Force3 F = contact_force(pen, v.c_lam[ci], n4.w, c0, c1, c2);
-        v.c_lam[ci] = make_float4(F.f0, F.f1, F.f2, 0.0f);
-        if (F.f0 < 0.0f) pen.x = rmin(pen.x + p.beta_lin * fabsf(c0), p.penalty_max);
-        if (F.tang <= F.bounds) {
-            pen.y = rmin(pen.y + p.beta_lin * fabsf(c1), p.penalty_max);
-            pen.z = rmin(pen.z + p.beta_lin * fabsf(c2), p.penalty_max);
-        }
-        v.c_pen[ci] = pen;
+    const uint2 ab = v.c_ab[ci];
+    const q4 rot_a = as_q(wr[ab.x]);
+    const f3 dal = sub(xyz(wp[ab.x]), xyz(v.init_lin[ab.x]));
+    const f3 daa = ang_diff(rot_a, as_q(v.init_ang[ab.x]));
+    f3 dbl = zero, dba = zero, rbw = zero;
+    if (ab.y != kContainer) {
+        const q4 rot_b = as_q(wr[ab.y]);
+        dbl = sub(xyz(wp[ab.y]), xyz(v.init_lin[ab.y]));
+        dba = ang_diff(rot_b, as_q(v.init_ang[ab.y]));
+        rbw = qrot(rot_b, xyz(v.c_rb[ci]));
     }
+    const f3 raw = qrot(rot_a, xyz(v.c_ra[ci]));
+    const float4 n4 = v.c_n[ci];
+    const f3 r0 = xyz(n4), r1 = xyz(v.c_t1[ci]), r2 = xyz(v.c_t2[ci]);
+    const float4 c04 = v.c_c0[ci];
+    uint32_t reps = 1, from = src;
+    if (static_updates > 0u) {
+        const bool moving = v.inv_mm[ab.x].w != 0.0f || (ab.y != kContainer && v.inv_mm[ab.y].w != 0.0f);
+        if (!moving) {
+            reps = static_updates;
+            from = 0;
+        }
+    }
+    DualState d;
+    d.lam = lam_buf(v, from)[ci];
+    d.pen = pen_buf(v, from)[ci];
+    for (uint32_t k = 0; k < reps; ++k)
+        d = dual_update(p, kk, r0, r1, r2, n4.w, raw, rbw, dal, daa, dbl, dba, c04, d.pen, d.lam);
+    lam_buf(v, dst)[ci] = d.lam;
+    pen_buf(v, dst)[ci] = d.pen;
 }
 
-__global__ void __launch_bounds__(kBlock) k_avbd_recover(SolveView v, AvbdParams p, uint32_t parity) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= p.n) return;
+// standalone dual update (unfused schedule)
+__global__ void __launch_bounds__(kBlock) k_avbd_dual(SolveView v, AvbdParams p, uint32_t parity, uint32_t src,
+                                                      uint32_t dst) {
+    const uint32_t m = v.m_dev ? *v.m_dev : v.m_host;
+    for (uint32_t ci = blockIdx.x * kBlock + threadIdx.x; ci < m; ci += gridDim.x * kBlock)
+        dual_one(v, p, ci, parity, src, dst, 0u);
+}
+
+__device__ __forceinline__ void recover_one(const SolveView& v, const AvbdParams& p, uint32_t i, uint32_t parity);
+
+// The last dual update (into the contact set's own lam / pen) and recover in
+// one launch: the dual reads only the final Jacobi poses and the initial
+// poses, which recover does not write.
+__global__ void __launch_bounds__(kBlock) k_avbd_dual_recover(SolveView v, AvbdParams p, uint32_t iters,
+                                                              uint32_t fused) {
+    const uint32_t m = v.m_dev ? *v.m_dev : v.m_host;
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t parity = (iters - 1) & 1u;
+    const uint32_t src = fused ? parity : 0u;
+    for (uint32_t ci = blockIdx.x * kBlock + threadIdx.x; ci < m; ci += stride)
+        dual_one(v, p, ci, parity, src, 0u, fused ? iters : 0u);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < p.n; i += stride) recover_one(v, p, i, parity);
+}
+
+__device__ __forceinline__ void recover_one(const SolveView& v, const AvbdParams& p, uint32_t i, uint32_t parity) {
     if (v.inv_mm[i].w == 0.0f) return;
     const float4 pos4 = v.pos[1 - parity][i];
     const float4 rot4 = v.rot[1 - parity][i];
@@ -613,40 +715,51 @@ void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t s, co
     if (p.n == 0) return;
     launch_k(k_avbd_init, dim3(blocks_for(p.n)), dim3(kBlock), s, ev, v, p);
 }
-void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t s, int lanes,
-                        const LaunchEvents& ev) {
-    if (p.n == 0) return;
+namespace {
+template <bool FUSED>
+void launch_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, hipStream_t s, int lanes,
+                   const LaunchEvents& ev) {
     switch (lanes) {
         case 4:
-            launch_k(k_avbd_primal_grouped<4>, dim3((p.n + kBlock / 4 - 1) / (kBlock / 4)), dim3(kBlock), s, ev, v, p,
-                     parity);
+            launch_k(k_avbd_primal_grouped<4, FUSED>, dim3((p.n + kBlock / 4 - 1) / (kBlock / 4)), dim3(kBlock), s, ev,
+                     v, p, iter);
             break;
         case 8:
-            launch_k(k_avbd_primal_grouped<8>, dim3((p.n + kBlock / 8 - 1) / (kBlock / 8)), dim3(kBlock), s, ev, v, p,
-                     parity);
+            launch_k(k_avbd_primal_grouped<8, FUSED>, dim3((p.n + kBlock / 8 - 1) / (kBlock / 8)), dim3(kBlock), s, ev,
+                     v, p, iter);
             break;
         case 16:
-            launch_k(k_avbd_primal_grouped<16>, dim3((p.n + kBlock / 16 - 1) / (kBlock / 16)), dim3(kBlock), s, ev, v,
-                     p, parity);
+            launch_k(k_avbd_primal_grouped<16, FUSED>, dim3((p.n + kBlock / 16 - 1) / (kBlock / 16)), dim3(kBlock), s,
+                     ev, v, p, iter);
             break;
         case 32:
-            launch_k(k_avbd_primal_grouped<32>, dim3((p.n + kBlock / 32 - 1) / (kBlock / 32)), dim3(kBlock), s, ev, v,
-                     p, parity);
+            launch_k(k_avbd_primal_grouped<32, FUSED>, dim3((p.n + kBlock / 32 - 1) / (kBlock / 32)), dim3(kBlock), s,
+                     ev, v, p, iter);
             break;
         default:
-            launch_k(k_avbd_primal, dim3(blocks_for(p.n)), dim3(kBlock), s, ev, v, p, parity);
+            launch_k(k_avbd_primal<FUSED>, dim3(blocks_for(p.n)), dim3(kBlock), s, ev, v, p, iter);
     }
 }
-void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t m_grid, hipStream_t s,
-                      const LaunchEvents& ev) {
-    uint32_t blocks = blocks_for(m_grid == 0 ? 1 : m_grid);
-    if (blocks > 4096) blocks = 4096;
-    launch_k(k_avbd_dual, dim3(blocks), dim3(kBlock), s, ev, v, p, parity);
-}
-void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t s,
-                         const LaunchEvents& ev) {
+}  // namespace
+
+void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, bool fused, hipStream_t s, int lanes,
+                        const LaunchEvents& ev) {
     if (p.n == 0) return;
-    launch_k(k_avbd_recover, dim3(blocks_for(p.n)), dim3(kBlock), s, ev, v, p, parity);
+    if (fused && iter > 0) launch_primal<true>(v, p, iter, s, lanes, ev);
+    else launch_primal<false>(v, p, iter, s, lanes, ev);
+}
+static uint32_t contact_blocks(uint32_t work) {
+    uint32_t blocks = blocks_for(work == 0 ? 1 : work);
+    return blocks > 4096 ? 4096 : blocks;
+}
+void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t src, uint32_t dst,
+                      uint32_t m_grid, hipStream_t s, const LaunchEvents& ev) {
+    launch_k(k_avbd_dual, dim3(contact_blocks(m_grid)), dim3(kBlock), s, ev, v, p, parity, src, dst);
+}
+void launch_avbd_dual_recover(const SolveView& v, const AvbdParams& p, uint32_t iters, bool fused, uint32_t m_grid,
+                              hipStream_t s, const LaunchEvents& ev) {
+    launch_k(k_avbd_dual_recover, dim3(contact_blocks(m_grid > p.n ? m_grid : p.n)), dim3(kBlock), s, ev, v, p, iters,
+             fused ? 1u : 0u);
 }
 
 }  // namespace dio

@@ -207,10 +207,11 @@ struct dio_ctx {
     float4 *a_pos[2] = {nullptr, nullptr}, *a_rot[2] = {nullptr, nullptr};
     float4 *a_init_lin = nullptr, *a_init_ang = nullptr, *a_iner_lin = nullptr, *a_iner_ang = nullptr;
     ContactArrays a_c{};
+    float4 *a_lam_alt = nullptr, *a_pen_alt = nullptr;  // scratch multipliers of a solve
     uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{32, 8, 4, {0, 0, 0, 0, 0}};
+    dio_tuning tuning{32, 8, 4, 0, {0, 0, 0, 0}};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -245,6 +246,7 @@ struct dio_ctx {
     float4 *s_init_lin = nullptr, *s_init_ang = nullptr, *s_iner_lin = nullptr, *s_iner_ang = nullptr;
     uint32_t* w_overflow = nullptr;
     unsigned long long* w_acc = nullptr;  // sums of pairs, contacts, CSR nnz over a dio_step call
+    float4 *w_lam_alt = nullptr, *w_pen_alt = nullptr;  // scratch multipliers of a solve
     std::vector<hipEvent_t> prof_events;
     uint64_t w_step = 0;
     dio_step_stats stats{};
@@ -350,6 +352,8 @@ void ensure_granular_avbd(dio_ctx* c) {
     c->a_iner_lin = A.alloc<float4>(n);
     c->a_iner_ang = A.alloc<float4>(n);
     c->a_c = alloc_contacts(A, m);
+    c->a_lam_alt = A.alloc<float4>(m);
+    c->a_pen_alt = A.alloc<float4>(m);
     c->a_off = A.alloc<uint32_t>((size_t)n + 1);
     c->a_idx = A.alloc<uint32_t>(2 * (size_t)m);
     c->a_peer = A.alloc<uint32_t>(2 * (size_t)m);
@@ -448,18 +452,23 @@ struct KernelTimer {
     }
 };
 
+// One AVBD solve (gpu_physics.rs:541-550): init; I x (primal, dual), the
+// dual fused into the next primal when `fused`; the last dual with recover.
 void schedule_avbd(const SolveView& v, const AvbdParams& p, uint32_t iters, uint32_t m_grid, hipStream_t s,
-                   KernelTimer* kt = nullptr, int lanes = 8) {
+                   KernelTimer* kt, const dio_tuning& t) {
+    const int lanes = (int)t.primal_lanes;
+    const bool fused = t.fuse_dual != 0u;
     auto run = [&](int kind, auto&& launch) {
         if (kt) kt->time(kind, launch);
         else launch(LaunchEvents{});
     };
     run(0, [&](const LaunchEvents& ev) { launch_avbd_init(v, p, s, ev); });
     for (uint32_t k = 0; k < iters; ++k) {
-        run(1, [&](const LaunchEvents& ev) { launch_avbd_primal(v, p, k % 2, s, lanes, ev); });
-        run(2, [&](const LaunchEvents& ev) { launch_avbd_dual(v, p, k % 2, m_grid, s, ev); });
+        run(1, [&](const LaunchEvents& ev) { launch_avbd_primal(v, p, k, fused, s, lanes, ev); });
+        if (!fused && k + 1 < iters)
+            run(2, [&](const LaunchEvents& ev) { launch_avbd_dual(v, p, k & 1u, 0u, 0u, m_grid, s, ev); });
     }
-    run(3, [&](const LaunchEvents& ev) { launch_avbd_recover(v, p, (iters - 1) % 2, s, ev); });
+    run(3, [&](const LaunchEvents& ev) { launch_avbd_dual_recover(v, p, iters, fused, m_grid, s, ev); });
 }
 
 __global__ void k_stats_accum(const uint32_t* pairs, const uint32_t* m, const uint32_t* nnz, unsigned long long* acc) {
@@ -759,6 +768,8 @@ dio_status dio_avbd_submit(dio_ctx* ctx, const dio_avbd_body_state* state, const
         v.c_ab = ctx->a_c.ab; v.c_n = ctx->a_c.n; v.c_t1 = ctx->a_c.t1; v.c_t2 = ctx->a_c.t2;
         v.c_ra = ctx->a_c.ra; v.c_rb = ctx->a_c.rb; v.c_c0 = ctx->a_c.c0; v.c_pen = ctx->a_c.pen;
         v.c_lam = ctx->a_c.lam;
+        v.c_lam_alt = ctx->a_lam_alt;
+        v.c_pen_alt = ctx->a_pen_alt;
         v.m_dev = nullptr;
         v.m_host = m;
         v.offsets = ctx->a_off;
@@ -771,7 +782,7 @@ dio_status dio_avbd_submit(dio_ctx* ctx, const dio_avbd_body_state* state, const
         v.out_rot = nullptr;
         const AvbdParams p = params_of(options->dt, options->gravity, options->alpha, options->beta_lin,
                                        options->penalty_max, n);
-        schedule_avbd(v, p, options->iterations, m, s, nullptr, (int)ctx->tuning.primal_lanes);
+        schedule_avbd(v, p, options->iterations, m, s, nullptr, ctx->tuning);
         DIO_HIP_CHECK(hipGetLastError());
         return DIO_OK;
     });
@@ -864,6 +875,8 @@ void ensure_world(dio_ctx* c) {
     c->s_iner_ang = A.alloc<float4>(n);
     c->w_overflow = A.alloc<uint32_t>(1);
     c->w_acc = A.alloc<unsigned long long>(4);
+    c->w_lam_alt = A.alloc<float4>(m);
+    c->w_pen_alt = A.alloc<float4>(m);
     c->w_stage_b = A.alloc<dio_gpu_body>(n);
     c->w_stage_s = A.alloc<dio_avbd_body_state>(n);
     c->w_stage_c = A.alloc<dio_avbd_constraint>(c->c_cap);
@@ -908,7 +921,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
     } else {
         SolveView v = world_solve_view(c, s);
         const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
-        schedule_avbd(v, p, o.iterations, c->m_cap, st, kt, (int)c->tuning.primal_lanes);
+        schedule_avbd(v, p, o.iterations, c->m_cap, st, kt, c->tuning);
         const int bn_pend = so.contact_lag ? 1 - s : s;
         hipLaunchKernelGGL(k_stats_accum, dim3(1), dim3(64), 0, st, (const uint32_t*)c->pend[bn_pend].p_dev,
                            (const uint32_t*)c->cs[s].m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
@@ -928,6 +941,7 @@ SolveView world_solve_view(dio_ctx* c, int s) {
         const ContactSet& cs = c->cs[s];
         v.c_ab = cs.c.ab; v.c_n = cs.c.n; v.c_t1 = cs.c.t1; v.c_t2 = cs.c.t2; v.c_ra = cs.c.ra; v.c_rb = cs.c.rb;
         v.c_c0 = cs.c.c0; v.c_pen = cs.c.pen; v.c_lam = cs.c.lam;
+        v.c_lam_alt = c->w_lam_alt; v.c_pen_alt = c->w_pen_alt;
         v.m_dev = cs.m_dev;
         v.m_host = 0;
         v.offsets = c->csr.off;
@@ -1170,6 +1184,7 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
             invalid("primal_lanes must be 1, 4, 8, 16 or 32");
         if (v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 && v.broad_lanes != 8)
             invalid("broad_lanes must be 1, 2, 4 or 8");
+        if (v.fuse_dual > 1) invalid("fuse_dual must be 0 or 1");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
@@ -1204,9 +1219,11 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
         DIO_HIP_CHECK(hipEventCreate(&e0));
         DIO_HIP_CHECK(hipEventCreate(&e1));
         for (int k = 0; k < DIO_TUNE_PRIMAL_SHAPES; ++k) {
-            launch_avbd_primal(v, p, 0, s, shapes[k]);  // warm the code path
+            // iteration 1 writes only the Jacobi pose buffer 0 (and, fused, the
+            // scratch multipliers), which the next solve overwrites before reading
+            launch_avbd_primal(v, p, 1, false, s, shapes[k]);  // warm the code path
             DIO_HIP_CHECK(hipEventRecord(e0, s));
-            for (uint32_t r = 0; r < reps; ++r) launch_avbd_primal(v, p, 0, s, shapes[k]);
+            for (uint32_t r = 0; r < reps; ++r) launch_avbd_primal(v, p, 1, false, s, shapes[k]);
             DIO_HIP_CHECK(hipEventRecord(e1, s));
             DIO_HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.0f;
@@ -1243,6 +1260,28 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
             }
         }
         ctx->g_broad_submitted = false;  // the granular broad buffers were used as scratch
+        // dual schedule, at the chosen primal shape: a fused primal against an
+        // unfused primal plus a standalone dual (into the scratch multipliers)
+        float sched_us[2] = {0.0f, 0.0f};
+        for (int f = 0; f < 2; ++f) {
+            auto once = [&]() {
+                launch_avbd_primal(v, p, 1, f == 1, s, best_lanes);
+                if (f == 0) launch_avbd_dual(v, p, 1u, 0u, 1u, ctx->m_cap, s);
+            };
+            once();
+            DIO_HIP_CHECK(hipEventRecord(e0, s));
+            for (uint32_t r = 0; r < reps; ++r) once();
+            DIO_HIP_CHECK(hipEventRecord(e1, s));
+            DIO_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            DIO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            sched_us[f] = 1000.0f * ms / (float)reps;
+        }
+        if (us_per_launch) {
+            us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES] = sched_us[0];
+            us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 1] = sched_us[1];
+        }
+        ctx->tuning.fuse_dual = sched_us[1] < sched_us[0] ? 1u : 0u;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         ctx->tuning.primal_lanes = (uint32_t)best_lanes;

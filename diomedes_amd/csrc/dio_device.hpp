@@ -21,7 +21,7 @@ struct AvbdParams {
     uint32_t n;  // body count
 };
 
-// Everything the four AVBD kernels touch (gpu_physics.rs bindings 0-10).
+// Everything the AVBD kernels touch (gpu_physics.rs bindings 0-10).
 struct SolveView {
     // current pose S_k (read by init) and the two Jacobi parity buffers
     const float4* cur_pos;
@@ -47,6 +47,8 @@ struct SolveView {
     const float4* c_c0;
     float4* c_pen;
     float4* c_lam;
+    float4* c_pen_alt;      // scratch multipliers (odd dual updates within a solve)
+    float4* c_lam_alt;
     const uint32_t* m_dev;  // contact count in device memory, or null: m_host
     uint32_t m_host;
     // per-body contact CSR (physics/gpu.rs:286-310)
@@ -71,14 +73,19 @@ struct LaunchEvents {
     hipEvent_t stop = nullptr;
 };
 
-// Launchers (avbd.hip).  All run on `stream`.
+// Launchers (avbd.hip).  All run on `stream`.  A solve with I iterations is
+// init; primal(k) for k = 0 .. I-1, each followed (k < I-1) by dual(k % 2,
+// 0, 0) in the unfused schedule, while in the fused schedule primal(k >= 1)
+// applies dual update k - 1 itself; then dual_recover(I) (the last dual
+// update, then recover).  Multipliers after j updates of a fused solve live
+// in buffer j % 2 (0: the contact set's lam / pen, 1: the scratch pair).
 void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t stream, const LaunchEvents& ev = {});
-// lanes: lanes per body (1 = one lane per body; 4, 8, 16 = grouped kernel)
-void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream, int lanes = 8,
-                        const LaunchEvents& ev = {});
-void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t m_grid, hipStream_t stream,
-                      const LaunchEvents& ev = {});
-void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream,
-                         const LaunchEvents& ev = {});
+// lanes: lanes per body (1 = one lane per body; 4, 8, 16, 32 = grouped kernel)
+void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, bool fused, hipStream_t stream,
+                        int lanes = 32, const LaunchEvents& ev = {});
+void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t src, uint32_t dst,
+                      uint32_t m_grid, hipStream_t stream, const LaunchEvents& ev = {});
+void launch_avbd_dual_recover(const SolveView& v, const AvbdParams& p, uint32_t iters, bool fused, uint32_t m_grid,
+                              hipStream_t stream, const LaunchEvents& ev = {});
 
 }  // namespace dio

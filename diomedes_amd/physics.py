@@ -465,33 +465,39 @@ class PhysicsContext:
         o = options.to_c()
         self._check(self._lib.dio_step(self._h, ctypes.byref(o), ctypes.byref(so), nsteps))
 
-    def set_tuning(self, primal_lanes: int = 0, broad_lanes: int = 0, resort_interval: int = 0) -> None:
-        """Launch shapes (0 keeps the current value)."""
+    def set_tuning(self, primal_lanes: int = 0, broad_lanes: int = 0, resort_interval: int = 0,
+                   fuse_dual: int | None = None) -> None:
+        """Launch shapes and schedule (0 / None keeps the current value)."""
         cur = self.tuning()
         t = R.Tuning()
         t.primal_lanes = primal_lanes or cur["primal_lanes"]
         t.broad_lanes = broad_lanes or cur["broad_lanes"]
         t.resort_interval = resort_interval or cur["resort_interval"]
+        t.fuse_dual = cur["fuse_dual"] if fuse_dual is None else int(fuse_dual)
         self._check(self._lib.dio_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
         t = R.Tuning()
         self._check(self._lib.dio_get_tuning(self._h, ctypes.byref(t)))
-        return {"primal_lanes": t.primal_lanes, "broad_lanes": t.broad_lanes, "resort_interval": t.resort_interval}
+        return {"primal_lanes": t.primal_lanes, "broad_lanes": t.broad_lanes, "resort_interval": t.resort_interval,
+                "fuse_dual": t.fuse_dual}
 
     def autotune(self, options: AvbdOptions, reps: int = 20) -> dict:
-        """Time every primal and broad-phase launch shape on the current
-        world, keep the fastest of each; returns the choice and the per-shape
-        microseconds."""
+        """Time every primal and broad-phase launch shape and both dual
+        schedules on the current world, keep the fastest; returns the choice
+        and the per-shape microseconds."""
         t = R.Tuning()
-        us = (ctypes.c_float * 9)()
+        us = (ctypes.c_float * 11)()
         o = options.to_c()
         self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
+        u = list(us)
         return {
             "primal_lanes": t.primal_lanes,
             "broad_lanes": t.broad_lanes,
-            "us_per_launch": dict(zip((1, 4, 8, 16, 32), list(us)[:5])),
-            "broad_us_per_pass": dict(zip((1, 2, 4, 8), list(us)[5:])),
+            "fuse_dual": t.fuse_dual,
+            "us_per_launch": dict(zip((1, 4, 8, 16, 32), u[:5])),
+            "broad_us_per_pass": dict(zip((1, 2, 4, 8), u[5:9])),
+            "dual_schedule_us": {"unfused": u[9], "fused": u[10]},
         }
 
     def stats(self) -> R.StepStats:

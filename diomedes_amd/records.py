@@ -172,7 +172,8 @@ assert ctypes.sizeof(StepStats) == 88
 
 class Tuning(ctypes.Structure):
     _fields_ = [("primal_lanes", ctypes.c_uint32), ("broad_lanes", ctypes.c_uint32),
-                ("resort_interval", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
+                ("resort_interval", ctypes.c_uint32), ("fuse_dual", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
 
 
 def f32(x) -> float:

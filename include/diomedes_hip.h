@@ -195,8 +195,11 @@ typedef struct dio_step_stats {
     uint64_t sum_contacts;
     uint64_t sum_csr_nnz;
     float ms_total;            /* GPU time of the whole call (events around all steps) */
-    float ms_avbd;             /* summed AVBD time: init + I x (primal + dual) + recover */
-    /* profile mode only: summed kernel times and launch counts */
+    float ms_avbd;             /* summed AVBD time: init, I primal launches (dual updates fused into
+                                  iterations 1 .. I-1), final dual + recover */
+    /* profile mode only: summed kernel times and launch counts.  ms_dual / n_dual
+       count standalone dual launches (none: the dual updates are fused), ms_recover /
+       n_recover the final dual + recover launch. */
     float ms_init, ms_primal, ms_dual, ms_recover;
     uint32_t n_init, n_primal, n_dual, n_recover;
 } dio_step_stats;
@@ -204,12 +207,16 @@ typedef struct dio_step_stats {
 /* Kernel launch shapes (the autotuner's search space). */
 #define DIO_TUNE_PRIMAL_SHAPES 5
 #define DIO_TUNE_BROAD_SHAPES 4
+#define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 2)
 typedef struct dio_tuning {
     uint32_t primal_lanes;     /* lanes per body in the primal: 1, 4, 8, 16 or 32 (0 = default 32) */
     uint32_t broad_lanes;      /* lanes per query in the BVH traversal: 1, 2, 4 or 8 (0 = default 8) */
     uint32_t resort_interval;  /* fused step: Morton re-sort every this many steps; the tree is refit
                                   over the previous order in between (0 = default 4) */
-    uint32_t reserved[5];
+    uint32_t fuse_dual;        /* 1: each primal iteration k >= 1 applies dual update k - 1 itself
+                                  (fewer launches, both bodies of a contact evaluate it); 0: a
+                                  dual launch after each primal (default) */
+    uint32_t reserved[4];
 } dio_tuning;
 
 typedef struct dio_ctx dio_ctx;
@@ -279,9 +286,10 @@ dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out);
 /* Autotuner: after at least one dio_step, time every primal shape on the
    current contact set and every broad-phase traversal shape on the current
    poses (reps launches each, HIP events), keep the fastest of each, and report
-   the average microseconds per launch in us_per_launch[0 ..
-   DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES - 1]: primal lanes 1, 4, 8,
-   16, 32, then broad lanes 1, 2, 4, 8 (a refit + traversal pass).  The
+   the average microseconds per launch in us_per_launch[0 .. DIO_TUNE_SLOTS - 1]:
+   primal lanes 1, 4, 8, 16, 32; broad lanes 1, 2, 4, 8 (a refit + traversal
+   pass); then, at the chosen primal shape, an unfused primal plus a dual
+   launch and a fused primal (fuse_dual is set to the faster).  The
    primal with a fixed parity is idempotent and the broad timing runs in the
    granular broad buffers, so the world state is unchanged; a granular broad
    result submitted before the call must be submitted again. */

@@ -106,8 +106,8 @@ def test_narrow_box_manifolds_bitexact(oracle):
 
 
 # ------------------------------------------------------------ AVBD solve
-@pytest.mark.parametrize("lanes", [1, 4, 8, 16, 32])
-def test_avbd_granular_solve_bitexact(oracle, lanes):
+@pytest.mark.parametrize("lanes,fuse", [(1, 0), (4, 1), (8, 0), (16, 1), (32, 0), (32, 1), (1, 1)])
+def test_avbd_granular_solve_bitexact(oracle, lanes, fuse):
     scene, w, _ = _settled_field(oracle, steps=30)
     # the oracle world's current contacts / CSR / state feed one granular solve
     c = w.contacts()
@@ -124,7 +124,7 @@ def test_avbd_granular_solve_bitexact(oracle, lanes):
     _print("granular avbd contacts", len(con))
     ref = oracle.avbd_run(state, pos, rot, con, scene.constraints, off, idx, opts.run_options())
     with PhysicsContext(scene.n) as ctx:
-        ctx.set_tuning(primal_lanes=lanes)
+        ctx.set_tuning(primal_lanes=lanes, fuse_dual=fuse)
         ctx.gpu_physics_submit(state, pos, rot, con, scene.constraints, off, idx, opts)
         got = ctx.gpu_physics_read()
     for k in ("positions", "orientations", "velocities", "angular_velocities", "prev_velocities", "lambda", "penalty"):
@@ -194,16 +194,18 @@ def test_box_stack_300_steps_bitexact(oracle):
 
 
 @pytest.mark.parametrize(
-    "lag,lanes,blanes,resort",
-    [(1, 8, 4, 4), (0, 8, 4, 4), (1, 4, 2, 1), (1, 16, 8, 1000), (1, 1, 1, 4), (0, 4, 8, 1000), (1, 32, 1, 4)],
+    "lag,lanes,blanes,resort,fuse",
+    [(1, 8, 4, 4, 0), (0, 8, 4, 4, 1), (1, 4, 2, 1, 0), (1, 16, 8, 1000, 1), (1, 1, 1, 4, 1), (0, 4, 8, 1000, 0),
+     (1, 32, 1, 4, 1), (1, 32, 8, 4, 0)],
 )
-def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort):
+def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort, fuse):
     """resort: Morton re-sort interval of the fused step; 1000 refits the first
-    step's order for the whole run, which must not change the pair set."""
+    step's order for the whole run, which must not change the pair set.
+    fuse: dual updates applied inside the next primal launch."""
     scene = scenes.cube_field(7, nx=8, ny=2, nz=8)
     w = oracle.OracleWorld(scene, 32)
     with PhysicsContext(scene.n) as ctx:
-        ctx.set_tuning(primal_lanes=lanes, broad_lanes=blanes, resort_interval=resort)
+        ctx.set_tuning(primal_lanes=lanes, broad_lanes=blanes, resort_interval=resort, fuse_dual=fuse)
         ctx.upload(scene)
         for k in range(6):
             ctx.step(scene.options, 20, lag=lag)
@@ -223,6 +225,7 @@ def test_autotune_keeps_state_and_parity(oracle):
         _print("autotune", res)
         assert res["primal_lanes"] in (1, 4, 8, 16, 32)
         assert res["broad_lanes"] in (1, 2, 4, 8)
+        assert res["fuse_dual"] in (0, 1)
         _compare_world(ctx, w, 40)
         ctx.step(scene.options, 20)
         w.step(nsteps=20)
@@ -362,3 +365,31 @@ def test_wall_throw_small_bitexact(oracle, resort):
             w.step(nsteps=15)
             _compare_world(ctx, w, 15 * (k + 1))
             _print("wall step", 15 * (k + 1), "pairs", ctx.stats().pairs, "contacts", ctx.stats().contacts)
+
+
+@pytest.mark.parametrize("fuse,lanes", [(0, 32), (1, 32), (1, 1)])
+def test_static_contacts_bitexact(oracle, fuse, lanes):
+    """Static bodies in contact with each other and with dynamic bodies: in the
+    fused schedule a static body runs no primal, so its contacts' dual updates
+    are stored by the dynamic side, and a static-static contact takes all its
+    updates in the final launch."""
+    from diomedes_amd.physics import AvbdBody, Scene
+
+    bodies = [
+        AvbdBody.cube((0.0, 0.5, 0.0), 0.5, 0.0),      # static
+        AvbdBody.cube((0.8, 0.5, 0.1), 0.5, 0.0),      # static, overlapping the first
+        AvbdBody.cube((0.1, 1.45, 0.0), 0.45, 2.0),    # resting on both
+        AvbdBody.cube((1.6, 0.45, 0.0), 0.45, 1.0),    # dynamic b > static a
+        AvbdBody.cube((-1.2, 1.5, 0.3), 0.4, 1.0),
+    ]
+    bodies[4].velocity = (3.0, -1.0, 0.0)
+    opts = AvbdOptions(iterations=5)
+    scene = Scene.from_bodies(bodies, (), scenes.floor_container(20.0), opts)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.set_tuning(primal_lanes=lanes, fuse_dual=fuse)
+        ctx.upload(scene)
+        for k in range(4):
+            ctx.step(opts, 15)
+            w.step(opts, nsteps=15)
+            _compare_world(ctx, w, 15 * (k + 1))
