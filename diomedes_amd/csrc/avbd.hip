@@ -387,7 +387,9 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     constexpr int kBodies = kBlock / G;
     constexpr int kOwned = (27 + G - 1) / G;
     __shared__ float stage[kBodies][G][kItemFloats];
-    __shared__ float gathered[kBodies][28];
+    __shared__ float gathered[kBodies][28];  // initial entries, later the accumulated block
+    __shared__ float4 body_f[kBodies][4];    // snapshot pos (.w: active), rot, d_lin, d_ang
+    __shared__ uint4 body_u[kBodies];        // CSR start / count of contacts, then of joints
     const int slot = threadIdx.x / G;
     const int lane = threadIdx.x % G;
     const uint32_t i = blockIdx.x * kBodies + slot;
@@ -395,51 +397,67 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     const float4* __restrict__ rr = v.rot[parity];
     const float dt = p.dt;
 
-    bool active = i < p.n;
-    float4 imm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (active) {
-        imm = v.inv_mm[i];
-        active = imm.w != 0.0f;
-    }
-    f3 snap_pos{0.0f, 0.0f, 0.0f}, dsl{0.0f, 0.0f, 0.0f}, dsa{0.0f, 0.0f, 0.0f};
-    q4 snap_rot{0.0f, 0.0f, 0.0f, 1.0f};
-    uint32_t jb = 0, ncontacts = 0, kb = 0, njoints = 0;
-    float acc[kOwned];
-    if (active) {
-        snap_pos = xyz(rp[i]);
-        snap_rot = as_q(rr[i]);
-        dsl = sub(snap_pos, xyz(v.init_lin[i]));
-        dsa = ang_diff(snap_rot, as_q(v.init_ang[i]));
-        jb = v.offsets[i];
-        ncontacts = v.offsets[i + 1] - jb;
-        if (v.j_off) {
-            kb = v.j_off[i];
-            njoints = v.j_off[i + 1] - kb;
+    // ---- per-body set-up by one thread per body (all in wave 0): the
+    // snapshot, the item ranges and the initial block, shared through LDS
+    if (threadIdx.x < (unsigned)kBodies) {
+        const int t = threadIdx.x;
+        const uint32_t bi = blockIdx.x * kBodies + t;
+        bool act = bi < p.n;
+        float4 imm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (act) {
+            imm = v.inv_mm[bi];
+            act = imm.w != 0.0f;
         }
-        // initial block: diag(M / dt^2, I / dt^2) and the inertial gradient
-        const float mass = 1.0f / imm.w;
-        const f3 moment{1.0f / imm.x, 1.0f / imm.y, 1.0f / imm.z};
-        const float dt2 = dt * dt;
-        const float m_dt2 = mass / dt2;
-        const f3 i_dt2 = divs(moment, dt2);
-        const f3 dl = sub(snap_pos, xyz(v.iner_lin[i]));
-        const f3 ad = ang_diff(snap_rot, as_q(v.iner_ang[i]));
+        uint4 u = make_uint4(0u, 0u, 0u, 0u);
+        f3 sp{0.0f, 0.0f, 0.0f}, sl{0.0f, 0.0f, 0.0f}, sa{0.0f, 0.0f, 0.0f};
+        q4 sr{0.0f, 0.0f, 0.0f, 1.0f};
+        if (act) {
+            sp = xyz(rp[bi]);
+            sr = as_q(rr[bi]);
+            sl = sub(sp, xyz(v.init_lin[bi]));
+            sa = ang_diff(sr, as_q(v.init_ang[bi]));
+            u.x = v.offsets[bi];
+            u.y = v.offsets[bi + 1] - u.x;
+            if (v.j_off) {
+                u.z = v.j_off[bi];
+                u.w = v.j_off[bi + 1] - u.z;
+            }
+            // initial block: diag(M / dt^2, I / dt^2) and the inertial gradient
+            const float mass = 1.0f / imm.w;
+            const f3 moment{1.0f / imm.x, 1.0f / imm.y, 1.0f / imm.z};
+            const float dt2 = dt * dt;
+            const float m_dt2 = mass / dt2;
+            const f3 i_dt2 = divs(moment, dt2);
+            const f3 dl = sub(sp, xyz(v.iner_lin[bi]));
+            const f3 ad = ang_diff(sr, as_q(v.iner_ang[bi]));
+            float* g0 = gathered[t];
 #pragma unroll
-        for (int t = 0; t < kOwned; ++t) {
-            const int e = lane + G * t;
-            float x = 0.0f;
-            if (e == 0 || e == 3 || e == 5) x = m_dt2;
-            else if (e == 6) x = i_dt2.x;
-            else if (e == 9) x = i_dt2.y;
-            else if (e == 11) x = i_dt2.z;
-            else if (e == 21) x = m_dt2 * dl.x;
-            else if (e == 22) x = m_dt2 * dl.y;
-            else if (e == 23) x = m_dt2 * dl.z;
-            else if (e == 24) x = i_dt2.x * ad.x;
-            else if (e == 25) x = i_dt2.y * ad.y;
-            else if (e == 26) x = i_dt2.z * ad.z;
-            acc[t] = x;
+            for (int e = 0; e < 27; ++e) g0[e] = 0.0f;
+            g0[0] = m_dt2; g0[3] = m_dt2; g0[5] = m_dt2;
+            g0[6] = i_dt2.x; g0[9] = i_dt2.y; g0[11] = i_dt2.z;
+            g0[21] = m_dt2 * dl.x; g0[22] = m_dt2 * dl.y; g0[23] = m_dt2 * dl.z;
+            g0[24] = i_dt2.x * ad.x; g0[25] = i_dt2.y * ad.y; g0[26] = i_dt2.z * ad.z;
         }
+        body_f[t][0] = f4(sp, act ? 1.0f : 0.0f);
+        body_f[t][1] = f4q(sr);
+        body_f[t][2] = f4(sl, 0.0f);
+        body_f[t][3] = f4(sa, 0.0f);
+        body_u[t] = u;
+    }
+    __syncthreads();
+    const float4 bf0 = body_f[slot][0];
+    const bool active = bf0.w != 0.0f;
+    const f3 snap_pos = xyz(bf0);
+    const q4 snap_rot = as_q(body_f[slot][1]);
+    const f3 dsl = xyz(body_f[slot][2]);
+    const f3 dsa = xyz(body_f[slot][3]);
+    const uint4 bu = body_u[slot];
+    const uint32_t jb = bu.x, ncontacts = bu.y, kb = bu.z, njoints = bu.w;
+    float acc[kOwned];
+#pragma unroll
+    for (int t = 0; t < kOwned; ++t) {
+        const int e = lane + G * t;
+        acc[t] = (active && e < 27) ? gathered[slot][e] : 0.0f;
     }
     const uint32_t nitems = ncontacts + njoints;
     const float kk = 1.0f - p.alpha;
@@ -582,7 +600,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
         const bool more = active && base + G < nitems;
         if (!__syncthreads_or(more)) break;
     }
-    // ---- gather the 27 entries and solve
+    // ---- gather the 27 entries; one thread per body solves (wave 0)
     if (active) {
 #pragma unroll
         for (int t = 0; t < kOwned; ++t) {
@@ -591,8 +609,12 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
         }
     }
     __syncthreads();
-    if (!active) return;
-    const float* gs = gathered[slot];
+    if (threadIdx.x >= (unsigned)kBodies) return;
+    const int t = threadIdx.x;
+    const float4 f0 = body_f[t][0];
+    if (f0.w == 0.0f) return;
+    const uint32_t bi = blockIdx.x * kBodies + t;
+    const float* gs = gathered[t];
     Block6 B;
 #pragma unroll
     for (int k = 0; k < 6; ++k) { B.l[k] = gs[k]; B.a[k] = gs[6 + k]; }
@@ -600,14 +622,13 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     for (int k = 0; k < 9; ++k) B.c[k] = gs[12 + k];
 #pragma unroll
     for (int k = 0; k < 3; ++k) { B.rl[k] = gs[21 + k]; B.ra[k] = gs[24 + k]; }
-    if (lane == 0) {
-        float x[6];
-        solve6(B, x);
-        const f3 new_pos = add(snap_pos, f3{x[0], x[1], x[2]});
-        const q4 new_rot = quat_step(snap_rot, f3{x[3], x[4], x[5]});
-        v.pos[1 - parity][i] = f4(new_pos, 0.0f);
-        v.rot[1 - parity][i] = f4q(new_rot);
-    }
+    float x[6];
+    solve6(B, x);
+    const q4 sr = as_q(body_f[t][1]);
+    const f3 new_pos = add(xyz(f0), f3{x[0], x[1], x[2]});
+    const q4 new_rot = quat_step(sr, f3{x[3], x[4], x[5]});
+    v.pos[1 - parity][bi] = f4(new_pos, 0.0f);
+    v.rot[1 - parity][bi] = f4q(new_rot);
 }
 
 // One dual update of contact ci from the Jacobi poses written by a primal of
@@ -734,6 +755,10 @@ void launch_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, hipSt
             break;
         case 32:
             launch_k(k_avbd_primal_grouped<32, FUSED>, dim3((p.n + kBlock / 32 - 1) / (kBlock / 32)), dim3(kBlock), s,
+                     ev, v, p, iter);
+            break;
+        case 64:
+            launch_k(k_avbd_primal_grouped<64, FUSED>, dim3((p.n + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), s,
                      ev, v, p, iter);
             break;
         default:

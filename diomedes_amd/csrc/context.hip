@@ -1180,8 +1180,8 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         if (v.broad_lanes == 0) v.broad_lanes = 8;
         if (v.resort_interval == 0) v.resort_interval = 4;
         if (v.primal_lanes != 1 && v.primal_lanes != 4 && v.primal_lanes != 8 && v.primal_lanes != 16 &&
-            v.primal_lanes != 32)
-            invalid("primal_lanes must be 1, 4, 8, 16 or 32");
+            v.primal_lanes != 32 && v.primal_lanes != 64)
+            invalid("primal_lanes must be 1, 4, 8, 16, 32 or 64");
         if (v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 && v.broad_lanes != 8)
             invalid("broad_lanes must be 1, 2, 4 or 8");
         if (v.fuse_dual > 1) invalid("fuse_dual must be 0 or 1");
@@ -1212,7 +1212,7 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
         const SolveView v = world_solve_view(ctx, par);
         const AvbdParams p = params_of(options->dt, options->gravity, options->alpha, options->beta_lin,
                                        options->penalty_max, ctx->w_n);
-        const int shapes[DIO_TUNE_PRIMAL_SHAPES] = {1, 4, 8, 16, 32};
+        const int shapes[DIO_TUNE_PRIMAL_SHAPES] = {1, 4, 8, 16, 32, 64};
         float best = 3.4e38f;
         int best_lanes = 8;
         hipEvent_t e0, e1;

@@ -487,7 +487,7 @@ class PhysicsContext:
         schedules on the current world, keep the fastest; returns the choice
         and the per-shape microseconds."""
         t = R.Tuning()
-        us = (ctypes.c_float * 11)()
+        us = (ctypes.c_float * 12)()
         o = options.to_c()
         self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
         u = list(us)
@@ -495,9 +495,9 @@ class PhysicsContext:
             "primal_lanes": t.primal_lanes,
             "broad_lanes": t.broad_lanes,
             "fuse_dual": t.fuse_dual,
-            "us_per_launch": dict(zip((1, 4, 8, 16, 32), u[:5])),
-            "broad_us_per_pass": dict(zip((1, 2, 4, 8), u[5:9])),
-            "dual_schedule_us": {"unfused": u[9], "fused": u[10]},
+            "us_per_launch": dict(zip((1, 4, 8, 16, 32, 64), u[:6])),
+            "broad_us_per_pass": dict(zip((1, 2, 4, 8), u[6:10])),
+            "dual_schedule_us": {"unfused": u[10], "fused": u[11]},
         }
 
     def stats(self) -> R.StepStats:

@@ -106,7 +106,7 @@ def test_narrow_box_manifolds_bitexact(oracle):
 
 
 # ------------------------------------------------------------ AVBD solve
-@pytest.mark.parametrize("lanes,fuse", [(1, 0), (4, 1), (8, 0), (16, 1), (32, 0), (32, 1), (1, 1)])
+@pytest.mark.parametrize("lanes,fuse", [(1, 0), (4, 1), (8, 0), (16, 1), (32, 0), (32, 1), (1, 1), (64, 0), (64, 1)])
 def test_avbd_granular_solve_bitexact(oracle, lanes, fuse):
     scene, w, _ = _settled_field(oracle, steps=30)
     # the oracle world's current contacts / CSR / state feed one granular solve
@@ -131,7 +131,7 @@ def test_avbd_granular_solve_bitexact(oracle, lanes, fuse):
         assert_bits(got[k], ref[k], k)
 
 
-@pytest.mark.parametrize("lanes", [1, 8])
+@pytest.mark.parametrize("lanes", [1, 8, 64])
 def test_avbd_granular_joints_bitexact(oracle, lanes):
     """Springs, rods, a ball socket and a rope (the constraints demo's set,
     demos/constraints/src/main.rs:110-146) through one solve."""
@@ -196,7 +196,7 @@ def test_box_stack_300_steps_bitexact(oracle):
 @pytest.mark.parametrize(
     "lag,lanes,blanes,resort,fuse",
     [(1, 8, 4, 4, 0), (0, 8, 4, 4, 1), (1, 4, 2, 1, 0), (1, 16, 8, 1000, 1), (1, 1, 1, 4, 1), (0, 4, 8, 1000, 0),
-     (1, 32, 1, 4, 1), (1, 32, 8, 4, 0)],
+     (1, 32, 1, 4, 1), (1, 32, 8, 4, 0), (1, 64, 8, 4, 1)],
 )
 def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort, fuse):
     """resort: Morton re-sort interval of the fused step; 1000 refits the first
@@ -223,7 +223,7 @@ def test_autotune_keeps_state_and_parity(oracle):
         w.step(nsteps=40)
         res = ctx.autotune(scene.options, reps=5)
         _print("autotune", res)
-        assert res["primal_lanes"] in (1, 4, 8, 16, 32)
+        assert res["primal_lanes"] in (1, 4, 8, 16, 32, 64)
         assert res["broad_lanes"] in (1, 2, 4, 8)
         assert res["fuse_dual"] in (0, 1)
         _compare_world(ctx, w, 40)
