@@ -52,6 +52,28 @@ def primal_bytes(n_dyn: int, m: float, k: float, iters: int) -> float:
     return 136 * n_dyn + 8 * k + 100 * m + 24 * m * (iters - 1) / iters
 
 
+PROFILE_MD = ROOT / "profiles" / "cube_field_8192_kernels.md"
+
+
+def measured_traffic(kernel: str) -> dict | None:
+    """HBM bytes per primal launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy kernel), when they were
+    taken on the kernel this run uses."""
+    try:
+        for line in PROFILE_MD.read_text().splitlines():
+            if line.startswith("primal_traffic_json:"):
+                t = json.loads(line.split(":", 1)[1])
+                return t if t.get("kernel") == kernel else None
+    except (OSError, ValueError):
+        return None
+    return None
+
+
+def _progress(msg: str) -> None:
+    """Progress lines go to stderr; stdout carries only the JSON result."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(scene, warmup: int, steps: int, budget_s: float = 25.0) -> dict:
     """FP32 oracle on the host cores over the same warm-up and steps."""
     from oracle import pyoracle
@@ -100,7 +122,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-steps", type=int, default=20, help="steps of the per-kernel timing pass")
-    ap.add_argument("--tune-steps", type=int, default=150,
+    ap.add_argument("--tune-steps", type=int, default=0,
                     help="steps of a separate run of the same scene after which dio_autotune picks the launch "
                          "shapes and dual schedule (0: library defaults)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -147,9 +169,11 @@ def main() -> int:
         ctx.upload(scene)
         if args.warmup:
             ctx.step(opts, args.warmup)
+        _progress(f"warm-up done ({args.warmup} steps)")
         ctx.step(opts, 0)  # synchronise (a zero-step call syncs the stream)
         t0 = time.perf_counter()
         ctx.step(opts, args.steps)  # returns after the stream is synchronised
+        _progress(f"timed region done ({args.steps} steps)")
         wall = time.perf_counter() - t0
         st = ctx.stats()
         ms_avbd = st.ms_avbd
@@ -157,6 +181,7 @@ def main() -> int:
         last = {"pairs": st.pairs, "contacts": st.contacts, "csr_nnz": st.csr_nnz}
         # per-kernel timing pass (eager launches, HIP events around each AVBD kernel)
         ctx.step(opts, args.profile_steps, profile=True)
+        _progress(f"profile pass done ({args.profile_steps} steps)")
         pst = ctx.stats()
 
     k = args.steps
@@ -171,6 +196,10 @@ def main() -> int:
     pbytes = primal_bytes(n_dyn, pm, pk, iters)
     achieved = pbytes / (primal_ms / 1000.0) / 1e9
     step_bytes = avbd_bytes(n_dyn, n, iters, avg_m, avg_k)
+    lanes = tuning["primal_lanes"]
+    kname = (f"k_avbd_primal_grouped<{lanes}, {'true' if tuning['fuse_dual'] else 'false'}>" if lanes > 1
+             else f"k_avbd_primal<{'true' if tuning['fuse_dual'] else 'false'}>")
+    traffic = measured_traffic(kname)
     line = {
         "metric": "AVBD body-iters/s & ms/step, 8k boxes",
         "value": value,
@@ -200,13 +229,15 @@ def main() -> int:
         "last_step": last,
         "avbd_segment_gbs": step_bytes * k / (ms_avbd / 1000.0) / 1e9,
         "roofline": {
-            "kernel": "k_avbd_primal",
+            "kernel": kname,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic["traffic_bytes"] if traffic else None,
+            "traffic_source": (f"{PROFILE_MD.relative_to(ROOT)}: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, "
+                               f"{traffic['window']}") if traffic else None,
             "avg_launch_us": primal_ms * 1000.0,
             "bytes_per_launch": pbytes,
             "launches_timed": pst.n_primal,

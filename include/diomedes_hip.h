@@ -214,8 +214,8 @@ typedef struct dio_tuning {
     uint32_t resort_interval;  /* fused step: Morton re-sort every this many steps; the tree is refit
                                   over the previous order in between (0 = default 4) */
     uint32_t fuse_dual;        /* 1: each primal iteration k >= 1 applies dual update k - 1 itself
-                                  (fewer launches, both bodies of a contact evaluate it); 0: a
-                                  dual launch after each primal (default) */
+                                  (fewer launches, both bodies of a contact evaluate it; default);
+                                  0: a dual launch after each primal */
     uint32_t reserved[4];
 } dio_tuning;
 
