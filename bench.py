@@ -69,6 +69,75 @@ def measured_traffic(kernel: str) -> dict | None:
     return None
 
 
+def batched_leg(args, tuning: dict) -> dict:
+    """The batched-scenes workload: every scene of the table stepped by the
+    same launches (one world with a scene table, the step captured in HIP
+    graphs), against a sample of the same scenes stepped one plain context
+    each, one after the other.  Both report AVBD body-iters/s (HIP events
+    around the AVBD segment) and full-step body-iters/s (wall clock)."""
+    from diomedes_amd import PhysicsContext, scenes
+
+    batch = scenes.batched_scenes(args.seed, args.batch_scenes)
+    opts = batch.options
+    iters = opts.iterations
+    k = args.steps
+    n_dyn = batch.n_dynamic
+    with PhysicsContext(batch.n, max_pairs=8 * batch.n, max_contacts=16 * batch.n,
+                        max_hull_vertices=len(batch.hull)) as ctx:
+        ctx.set_tuning(**tuning)
+        ctx.batch_upload(batch)
+        if args.warmup:
+            ctx.step(opts, args.warmup)
+        ctx.step(opts, 0)
+        t0 = time.perf_counter()
+        ctx.step(opts, k)
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+    _progress(f"batched leg: {len(batch.scenes)} scenes, {k} steps in {wall:.2f} s")
+    out = {
+        "workload": f"{len(batch.scenes)} scenes, sizes round(exp(U(ln 8, ln 512))), unit cubes and r 0.5 "
+                    f"icosphere hulls dropped on per-scene floors, I={iters}, dt=1/60, friction 0.5, seed {args.seed}",
+        "bodies": batch.n,
+        "value": n_dyn * iters * k / (st.ms_avbd / 1000.0),
+        "unit": "body-iters/s",
+        "ms_per_step": 1000.0 * wall / k,
+        "ms_per_step_avbd": st.ms_avbd / k,
+        "body_iters_per_s_full_step": n_dyn * iters * k / wall,
+        "last_step": {"pairs": st.pairs, "contacts": st.contacts, "csr_nnz": st.csr_nnz},
+    }
+    # plain: a sample of the same scenes, each in its own context, stepped in turn
+    step_ = max(1, len(batch.scenes) // max(1, args.plain_sample))
+    sample = list(range(0, len(batch.scenes), step_))[: args.plain_sample]
+    bi = 0.0
+    t_avbd = 0.0
+    t_wall = 0.0
+    for s_i in sample:
+        sc = batch.scenes[s_i]
+        with PhysicsContext(sc.n, max_hull_vertices=len(sc.hull)) as one:
+            one.set_tuning(**tuning)
+            one.upload(sc)
+            if args.warmup:
+                one.step(opts, args.warmup)
+            one.step(opts, 0)
+            t0 = time.perf_counter()
+            one.step(opts, k)
+            t_wall += time.perf_counter() - t0
+            t_avbd += one.stats().ms_avbd / 1000.0
+        bi += sc.n_dynamic * iters * k
+    _progress(f"plain sample: {len(sample)} scenes in {t_wall:.2f} s")
+    plain_avbd = bi / t_avbd
+    plain_full = bi / t_wall
+    out["plain"] = {
+        "sample": f"every {step_}th scene of the table ({len(sample)} scenes, "
+                  f"{sum(batch.scenes[i].n for i in sample)} bodies), one context each, {k} steps each",
+        "value": plain_avbd,
+        "body_iters_per_s_full_step": plain_full,
+    }
+    out["batched_over_plain"] = out["value"] / plain_avbd
+    out["batched_over_plain_full_step"] = out["body_iters_per_s_full_step"] / plain_full
+    return out
+
+
 def _progress(msg: str) -> None:
     """Progress lines go to stderr; stdout carries only the JSON result."""
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
@@ -121,6 +190,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batch-scenes", type=int, default=4096,
+                    help="scenes of the batched leg (0: skip it); steps and warm-up as for the main workload")
+    ap.add_argument("--plain-sample", type=int, default=64,
+                    help="scenes of the batch stepped one context each for the plain comparison")
     ap.add_argument("--profile-steps", type=int, default=20, help="steps of the per-kernel timing pass")
     ap.add_argument("--tune-steps", type=int, default=0,
                     help="steps of a separate run of the same scene after which dio_autotune picks the launch "
@@ -248,6 +321,8 @@ def main() -> int:
             "dual_recover": 1000.0 * pst.ms_recover / max(pst.n_recover, 1),
         },
     }
+    if args.batch_scenes:
+        line["batched"] = batched_leg(args, tuning)
     if not args.no_cpu:
         cb = cpu_baseline(scenes.cube_field(args.seed), args.warmup, k)
         line["cpu_baseline"] = cb

@@ -41,6 +41,9 @@ SIGNATURES = {
     ),
     "dio_avbd_read": (c_int, [_VP, _FP, _FP, _FP, _FP, _FP, _FP, _FP]),
     "dio_world_upload": (c_int, [_VP, _VP, _VP, c_uint32, _VP, c_uint32, POINTER(R.Container)]),
+    "dio_batch_upload": (
+        c_int, [_VP, _VP, _VP, c_uint32, _VP, c_uint32, _U32P, c_uint32, POINTER(R.Container)]
+    ),
     "dio_world_upload_hulls": (c_int, [_VP, _FP, c_uint32]),
     "dio_step": (c_int, [_VP, POINTER(R.Options), POINTER(R.StepOpts), c_uint32]),
     "dio_step_get_stats": (c_int, [_VP, POINTER(R.StepStats)]),

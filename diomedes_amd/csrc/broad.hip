@@ -98,7 +98,8 @@ __device__ __forceinline__ uint32_t quantize(float c, float lo, float e) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_morton(const float4* mn, const float4* mx, uint32_t n,
-                                                   const float4* bounds, uint32_t* keys, uint32_t* vals) {
+                                                   const float4* bounds, uint32_t* keys, uint32_t* vals,
+                                                   SceneTable scenes) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 lo = bounds[0], hi = bounds[1];
@@ -107,7 +108,9 @@ __global__ void __launch_bounds__(kBlock) k_morton(const float4* mn, const float
     const uint32_t qx = quantize((a.x + b.x) * 0.5f, lo.x, e);
     const uint32_t qy = quantize((a.y + b.y) * 0.5f, lo.y, e);
     const uint32_t qz = quantize((a.z + b.z) * 0.5f, lo.z, e);
-    keys[i] = (expand_bits(qx) << 2) | (expand_bits(qy) << 1) | expand_bits(qz);
+    const uint32_t code = (expand_bits(qx) << 2) | (expand_bits(qy) << 1) | expand_bits(qz);
+    // with a scene table: scene id in the high bits, the code's top bits below
+    keys[i] = scenes.bits ? (scenes.of[i] << (30 - scenes.bits)) | (code >> scenes.bits) : code;
     vals[i] = i;
 }
 
@@ -236,14 +239,15 @@ __device__ __forceinline__ uint32_t group_bits(unsigned long long m) {
 }
 
 // Walk the tree for query body a (box qa..qb) with the LANES lanes of this
-// group.  hit(k, b, rank, total) is called on every lane for each child slot k
+// group, over the sorted positions lo_pos .. hi_pos - 1 (all bodies, or the
+// query's scene).  hit(k, b, rank, total) is called on every lane for each child slot k
 // after each step: b is the partner found by this lane (or kNone), rank its
 // position among the group's hits of that slot, total their count.  Returns
 // false if the stack would overflow (never for D <= 7).
 template <int LANES, class Hit>
-__device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t n, uint32_t D, const uint32_t* perm,
-                                     const float4* nmn, const float4* nmx, const float4* clo, const float4* chi,
-                                     uint32_t ncached, uint32_t* stk, Hit&& hit) {
+__device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t lo_pos, uint32_t hi_pos, uint32_t D,
+                                     const uint32_t* perm, const float4* nmn, const float4* nmx, const float4* clo,
+                                     const float4* chi, uint32_t ncached, uint32_t* stk, Hit&& hit) {
     constexpr int kPer = 8 / LANES;
     const uint32_t gl = threadIdx.x % (uint32_t)LANES;
     const uint32_t below = (1u << gl) - 1u;
@@ -265,7 +269,7 @@ __device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t 
             const uint32_t c = 8 * h + 1 + gl + LANES * k;
             const uint32_t first = (c - cs) * span;
             bool ov = false;
-            if (first < n) {
+            if (first < hi_pos && first + span > lo_pos) {
                 const float4 lo = c < ncached ? clo[c] : nmn[c];
                 const float4 hi = c < ncached ? chi[c] : nmx[c];
                 ov = aabb_overlap(qa, qb, lo, hi);
@@ -306,11 +310,22 @@ __device__ __forceinline__ void load_top(const float4* nmn, const float4* nmx, u
 // Pass 1: count body a's partners b > a into counts[a] and keep the first
 // kPairCache of them in cache[a * kPairCache ...].  Groups take bodies in
 // Morton order, so the groups of a wavefront walk similar paths.
+__device__ __forceinline__ void query_range(const SceneTable& st, uint32_t a, uint32_t n, uint32_t& lo,
+                                            uint32_t& hi) {
+    lo = 0;
+    hi = n;
+    if (st.bits) {
+        const uint32_t s = st.of[a];
+        lo = st.off[s];
+        hi = st.off[s + 1];
+    }
+}
+
 template <int LANES>
 __global__ void __launch_bounds__(kTravBlock) k_tree_count(uint32_t n, uint32_t D, const uint32_t* perm,
                                                            const float4* amn, const float4* amx, const float4* nmn,
                                                            const float4* nmx, uint32_t* counts, uint32_t* cache,
-                                                           uint32_t* err) {
+                                                           uint32_t* err, SceneTable scenes) {
     __shared__ float4 clo[kTopNodes], chi[kTopNodes];
     __shared__ uint32_t stack[kTravBlock / LANES][kStackDepth];
     uint32_t ncached;
@@ -321,7 +336,9 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_count(uint32_t n, uint32_t 
     const uint32_t a = perm[q];
     uint32_t cnt = 0;
     uint32_t* mine = cache + (size_t)a * kPairCache;
-    const bool ok = walk<LANES>(a, amn[a], amx[a], n, D, perm, nmn, nmx, clo, chi, ncached,
+    uint32_t lo, hi;
+    query_range(scenes, a, n, lo, hi);
+    const bool ok = walk<LANES>(a, amn[a], amx[a], lo, hi, D, perm, nmn, nmx, clo, chi, ncached,
                                 stack[threadIdx.x / LANES], [&](int, uint32_t b, uint32_t rank, uint32_t total) {
                                     if (b != kNone && cnt + rank < kPairCache) mine[cnt + rank] = b;
                                     cnt += total;
@@ -337,7 +354,8 @@ template <int LANES>
 __global__ void __launch_bounds__(kTravBlock) k_tree_emit(uint32_t n, uint32_t D, const uint32_t* perm,
                                                           const float4* amn, const float4* amx, const float4* nmn,
                                                           const float4* nmx, const uint32_t* offsets,
-                                                          const uint32_t* cache, uint32_t p_max, uint2* pairs) {
+                                                          const uint32_t* cache, uint32_t p_max, uint2* pairs,
+                                                          SceneTable scenes) {
     __shared__ float4 clo[kTopNodes], chi[kTopNodes];
     __shared__ uint32_t stack[kTravBlock / LANES][kStackDepth];
     __shared__ uint32_t list[kTravBlock / LANES][kPairCache];
@@ -363,7 +381,9 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_emit(uint32_t n, uint32_t D
         return;
     }
     uint32_t k = 0;
-    walk<LANES>(a, amn[a], amx[a], n, D, perm, nmn, nmx, clo, chi, ncached, stack[grp],
+    uint32_t lo, hi;
+    query_range(scenes, a, n, lo, hi);
+    walk<LANES>(a, amn[a], amx[a], lo, hi, D, perm, nmn, nmx, clo, chi, ncached, stack[grp],
                 [&](int, uint32_t b, uint32_t rank, uint32_t cnt) {
                     if (b != kNone && seg + k + rank < p_max) pairs[seg + k + rank] = make_uint2(a, b);
                     k += cnt;
@@ -419,30 +439,32 @@ size_t BroadBuffers::tree_nodes(uint32_t n_cap) { return level_start(tree_depth(
 
 namespace {
 template <int LANES>
-void launch_tree_pairs(int mode, uint32_t n, uint32_t D, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
+void launch_tree_pairs(int mode, uint32_t n, uint32_t D, const BroadBuffers& b, uint32_t p_max, hipStream_t s,
+                       const SceneTable& st) {
     const uint32_t blocks = (n * LANES + kTravBlock - 1) / kTravBlock;
     if (mode == 0)
         hipLaunchKernelGGL((k_tree_count<LANES>), dim3(blocks), dim3(kTravBlock), 0, s, n, D, (const uint32_t*)b.perm,
                            (const float4*)b.aabb_min, (const float4*)b.aabb_max, (const float4*)b.node_min,
-                           (const float4*)b.node_max, b.counts, b.pair_cache, b.err);
+                           (const float4*)b.node_max, b.counts, b.pair_cache, b.err, st);
     else
         hipLaunchKernelGGL((k_tree_emit<LANES>), dim3(blocks), dim3(kTravBlock), 0, s, n, D, (const uint32_t*)b.perm,
                            (const float4*)b.aabb_min, (const float4*)b.aabb_max, (const float4*)b.node_min,
                            (const float4*)b.node_max, (const uint32_t*)b.offsets, (const uint32_t*)b.pair_cache,
-                           p_max, b.pairs);
+                           p_max, b.pairs, st);
 }
-void tree_pairs(int lanes, int mode, uint32_t n, uint32_t D, const BroadBuffers& b, uint32_t p_max, hipStream_t s) {
+void tree_pairs(int lanes, int mode, uint32_t n, uint32_t D, const BroadBuffers& b, uint32_t p_max, hipStream_t s,
+                const SceneTable& st) {
     switch (lanes) {
-        case 1: launch_tree_pairs<1>(mode, n, D, b, p_max, s); break;
-        case 2: launch_tree_pairs<2>(mode, n, D, b, p_max, s); break;
-        case 4: launch_tree_pairs<4>(mode, n, D, b, p_max, s); break;
-        default: launch_tree_pairs<8>(mode, n, D, b, p_max, s); break;
+        case 1: launch_tree_pairs<1>(mode, n, D, b, p_max, s, st); break;
+        case 2: launch_tree_pairs<2>(mode, n, D, b, p_max, s, st); break;
+        case 4: launch_tree_pairs<4>(mode, n, D, b, p_max, s, st); break;
+        default: launch_tree_pairs<8>(mode, n, D, b, p_max, s, st); break;
     }
 }
 }  // namespace
 
 void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s, bool resort,
-               int lanes) {
+               int lanes, const SceneTable& scenes) {
     if (n == 0) {
         launch_fill_u32(b.offsets, 0u, 1, s);
         hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets, p_max, b.p_dev, b.overflow);
@@ -452,7 +474,7 @@ void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p
         hipLaunchKernelGGL(k_aabb, dim3(blocks_for(n)), dim3(kBlock), 0, s, bv, n, b.aabb_min, b.aabb_max);
         hipLaunchKernelGGL(k_bounds, dim3(1), dim3(1024), 0, s, b.aabb_min, b.aabb_max, n, b.bounds);
         hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.aabb_min, b.aabb_max, n, b.bounds,
-                           b.keys0, b.perm);
+                           b.keys0, b.perm, scenes);
         // 30-bit keys: four 8-bit passes, so the sorted values land back in perm
         radix_sort_pairs(b.keys0, b.perm, b.keys1, b.vals1, n, 30, b.sort_scratch, s);
     }
@@ -462,9 +484,9 @@ void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p
     hipLaunchKernelGGL(k_tree_lower, dim3((n + chunk - 1) / chunk), dim3(256), 0, s, bv, (const uint32_t*)b.perm, n, D,
                        chunk, b.aabb_min, b.aabb_max, b.node_min, b.node_max);
     if (D > 3) hipLaunchKernelGGL(k_tree_top, dim3(1), dim3(1024), 0, s, n, D, b.node_min, b.node_max);
-    tree_pairs(lanes, 0, n, D, b, p_max, s);
+    tree_pairs(lanes, 0, n, D, b, p_max, s, scenes);
     launch_scan_exclusive(b.counts, b.offsets, n, nullptr, b.scan_partials, s);
-    tree_pairs(lanes, 1, n, D, b, p_max, s);
+    tree_pairs(lanes, 1, n, D, b, p_max, s, scenes);
     hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets + n, p_max, b.p_dev, b.overflow);
 }
 

@@ -43,6 +43,15 @@ struct BroadBuffers {
 constexpr uint32_t kPairCache = 32;  // partners per body kept between the two traversal passes
 constexpr uint32_t kMaxBroadBodies = 1u << 20;  // tree depth <= 7: the top refit workgroup covers 4 levels
 
+// A batched world's scene table: bodies of scene s are offsets off[s] ..
+// off[s+1] - 1 and never pair with another scene's.  bits = bits of the scene
+// id in the sort key (0: a single scene, no table).
+struct SceneTable {
+    const uint32_t* of = nullptr;   // scene of each body
+    const uint32_t* off = nullptr;  // num_scenes + 1 body offsets
+    uint32_t bits = 0;
+};
+
 void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, float4* rot, uint32_t* tag,
                               float4* corners, hipStream_t s);
 
@@ -51,7 +60,10 @@ void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, 
 // (otherwise the tree is refit over the previous order, which keeps the pair
 // set exact and only changes the traversal cost).  lanes: lanes per query in
 // the traversal (1, 2, 4 or 8; each tests 8 / lanes children per step).
+// scenes: with a table, the sort key's high bits are the scene id (so each
+// scene is a contiguous range of sorted positions, equal to its body range)
+// and each query walks only its scene's range.
 void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s, bool resort,
-               int lanes);
+               int lanes, const SceneTable& scenes = SceneTable{});
 
 }  // namespace dio

@@ -175,8 +175,9 @@ __global__ void __launch_bounds__(kBlock) k_container_count(WorldView w, Contain
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= w.n) return;
     uint32_t c = 0;
-    if (ct.enabled && w.inv_mm[i].w != 0.0f)
-        wall_contacts(w, i, ct, o, [&](f3, float, f3, f3, f3, const ShapeRef&) { ++c; });
+    const ContainerParams cp = w.scene_ct ? w.scene_ct[w.scene_of[i]] : ct;
+    if (cp.enabled && w.inv_mm[i].w != 0.0f)
+        wall_contacts(w, i, cp, o, [&](f3, float, f3, f3, f3, const ShapeRef&) { ++c; });
     cnt[i] = c;
 }
 
@@ -204,7 +205,8 @@ __global__ void __launch_bounds__(kBlock) k_container_write(WorldView w, Contain
     }
     uint32_t ci = mn + dst.cont_off[i];
     bool first = true;
-    wall_contacts(w, i, ct, o, [&](f3 p, float depth, f3 normal, f3 t1, f3 t2, const ShapeRef& s) {
+    const ContainerParams cp = w.scene_ct ? w.scene_ct[w.scene_of[i]] : ct;
+    wall_contacts(w, i, cp, o, [&](f3 p, float depth, f3 normal, f3 t1, f3 t2, const ShapeRef& s) {
         if (ci < dst.m_cap) {
             float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float4 pen = make_float4(o.penalty_min, o.penalty_min, o.penalty_min, 0.0f);

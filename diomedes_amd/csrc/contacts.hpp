@@ -37,6 +37,13 @@ struct NarrowSource {
     const float4* pb;
 };
 
+struct ContainerParams {
+    float cx, cy, cz;
+    float qx, qy, qz, qw;
+    float half;
+    uint32_t enabled;
+};
+
 struct WorldView {
     const float4* pos;
     const float4* rot;
@@ -46,13 +53,10 @@ struct WorldView {
     const float4* corners;
     const float4* hull;
     uint32_t n;
-};
-
-struct ContainerParams {
-    float cx, cy, cz;
-    float qx, qy, qz, qw;
-    float half;
-    uint32_t enabled;
+    // batched world: each body's scene and each scene's container (null: the
+    // single container passed to run_contact_build)
+    const uint32_t* scene_of = nullptr;
+    const ContainerParams* scene_ct = nullptr;
 };
 
 // AvbdOptions fields used by the contact build

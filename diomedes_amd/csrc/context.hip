@@ -233,6 +233,12 @@ struct dio_ctx {
     dio_avbd_constraint* w_stage_c = nullptr;
     JointDev w_j;
     ContainerParams w_ct{};
+    // batched world (dio_batch_upload): scene of each body, body offsets per
+    // scene, per-scene containers; w_scene_bits = 0 for a single scene
+    uint32_t* w_scene_of = nullptr;
+    uint32_t* w_scene_off = nullptr;
+    ContainerParams* w_scene_ct = nullptr;
+    uint32_t w_num_scenes = 1, w_scene_bits = 0;
     BroadBuffers wb{};
     struct Pending {
         uint2* pairs;
@@ -835,6 +841,9 @@ void ensure_world(dio_ctx* c) {
     c->w_tag = A.alloc<uint32_t>(n);
     c->w_corners = A.alloc<float4>((size_t)n * 4);
     c->w_hull = A.alloc<float4>(c->h_cap);
+    c->w_scene_of = A.alloc<uint32_t>(n);
+    c->w_scene_off = A.alloc<uint32_t>((size_t)n + 1);
+    c->w_scene_ct = A.alloc<ContainerParams>(n);
     c->w_j.abk = A.alloc<uint4>(c->c_cap);
     c->w_j.anc_a = A.alloc<float4>(c->c_cap);
     c->w_j.anc_b = A.alloc<float4>(c->c_cap);
@@ -896,6 +905,14 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
     hipStream_t st = c->stream;
     const uint32_t n = c->w_n;
     WorldView wv{c->w_pos, c->w_rot, c->w_vel, c->w_inv_mm, c->w_tag, c->w_corners, c->w_hull, n};
+    SceneTable scenes;
+    if (c->w_scene_bits) {
+        wv.scene_of = c->w_scene_of;
+        wv.scene_ct = c->w_scene_ct;
+        scenes.of = c->w_scene_of;
+        scenes.off = c->w_scene_off;
+        scenes.bits = c->w_scene_bits;
+    }
     BodyView bv{c->w_pos, c->w_rot, c->w_tag, c->w_corners, c->w_hull};
     BuildOptions bo{o.dt, o.alpha, o.margin, o.penalty_min, o.penalty_max, o.gamma, o.friction};
     const int lag = so.contact_lag ? 1 : 0;
@@ -910,7 +927,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
             b.pairs = c->pend[dst_pend].pairs;
             b.p_dev = c->pend[dst_pend].p_dev;
             b.overflow = c->w_overflow;
-            run_broad(bv, n, b, c->p_max, st, resort, (int)c->tuning.broad_lanes);
+            run_broad(bv, n, b, c->p_max, st, resort, (int)c->tuning.broad_lanes, scenes);
             run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, st);
         };
         if (!lag) broad_narrow();
@@ -989,59 +1006,114 @@ void build_graphs(dio_ctx* c, const dio_avbd_options& o, const dio_step_opts& so
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+ContainerParams container_params(const dio_container* container) {
+    ContainerParams ct{};
+    if (container && container->enabled) {
+        ct.cx = container->center[0]; ct.cy = container->center[1]; ct.cz = container->center[2];
+        ct.qx = container->rotation[0]; ct.qy = container->rotation[1];
+        ct.qz = container->rotation[2]; ct.qw = container->rotation[3];
+        ct.half = container->half_extent;
+        ct.enabled = 1;
+    }
+    return ct;
+}
+
+// The world upload; scene_offsets / containers describe a batched world
+// (num_scenes > 1) or are null (one scene, one optional container).
+dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd_body_state* state, uint32_t n,
+                        const dio_avbd_constraint* constraints, uint32_t c, const uint32_t* scene_offsets,
+                        uint32_t num_scenes, const dio_container* containers) {
+    if (n > ctx->n_cap) capacity("too many bodies for the context", n);
+    if (c > ctx->c_cap) capacity("too many constraints for the context", c);
+    if ((n > 0 && (!bodies || !state)) || (c > 0 && !constraints)) invalid("null input");
+    std::vector<uint32_t> scene_of;
+    if (scene_offsets) {
+        if (num_scenes == 0 || num_scenes > 65536u || num_scenes > ctx->n_cap) invalid("num_scenes out of range");
+        if (!containers) invalid("containers is null");
+        if (scene_offsets[0] != 0 || scene_offsets[num_scenes] != n) invalid("scene offsets must run from 0 to n");
+        scene_of.resize(n);
+        for (uint32_t k = 0; k < num_scenes; ++k) {
+            if (scene_offsets[k + 1] < scene_offsets[k]) invalid("scene offsets must not decrease");
+            for (uint32_t i = scene_offsets[k]; i < scene_offsets[k + 1]; ++i) scene_of[i] = k;
+        }
+        for (uint32_t k = 0; k < c; ++k) {
+            const uint32_t a = constraints[k].a, b = constraints[k].b;
+            if (a >= n || b >= n || scene_of[a] != scene_of[b]) invalid("a constraint joins two scenes");
+        }
+    }
+    ensure_world(ctx);
+    hipStream_t s = ctx->stream;
+    DIO_HIP_CHECK(hipStreamSynchronize(s));
+    ctx->drop_graphs();
+    ctx->w_n = n;
+    ctx->w_step = 0;
+    ctx->w_sort_pending = true;
+    ctx->w_friction.resize(n);
+    ctx->w_n_dyn = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        ctx->w_friction[i] = state[i].friction;
+        if (state[i].inv_mass != 0.0f) ctx->w_n_dyn += 1;
+    }
+    if (n > 0) {
+        dio_gpu_body* stage_b = ctx->w_stage_b;
+        dio_avbd_body_state* stage_s = ctx->w_stage_s;
+        DIO_HIP_CHECK(hipMemcpy(stage_b, bodies, (size_t)n * sizeof(dio_gpu_body), hipMemcpyHostToDevice));
+        DIO_HIP_CHECK(hipMemcpy(stage_s, state, (size_t)n * sizeof(dio_avbd_body_state), hipMemcpyHostToDevice));
+        launch_unpack_gpu_bodies(stage_b, n, ctx->w_pos, ctx->w_rot, ctx->w_tag, ctx->w_corners, s);
+        hipLaunchKernelGGL(k_unpack_state, dim3(blocks256(n)), dim3(256), 0, s, stage_s, (const float*)nullptr, n,
+                           ctx->w_vel, ctx->w_ang, ctx->w_prev, ctx->w_inv_mm, (float4*)nullptr, (float4*)nullptr);
+        DIO_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    upload_joints(ctx, ctx->w_j, constraints, c, n, ctx->w_stage_c);
+    ctx->w_num_scenes = 1;
+    ctx->w_scene_bits = 0;
+    if (scene_offsets && num_scenes > 1) {
+        std::vector<ContainerParams> cts(num_scenes);
+        for (uint32_t k = 0; k < num_scenes; ++k) cts[k] = container_params(&containers[k]);
+        if (n > 0) DIO_HIP_CHECK(hipMemcpy(ctx->w_scene_of, scene_of.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+        DIO_HIP_CHECK(hipMemcpy(ctx->w_scene_off, scene_offsets, ((size_t)num_scenes + 1) * 4, hipMemcpyHostToDevice));
+        DIO_HIP_CHECK(hipMemcpy(ctx->w_scene_ct, cts.data(), (size_t)num_scenes * sizeof(ContainerParams),
+                                hipMemcpyHostToDevice));
+        uint32_t bits = 1;
+        while ((1u << bits) < num_scenes) ++bits;
+        ctx->w_num_scenes = num_scenes;
+        ctx->w_scene_bits = bits;
+        ctx->w_ct = ContainerParams{};
+    } else {
+        ctx->w_ct = container_params(scene_offsets ? &containers[0] : containers);
+    }
+    // empty lagged narrow output and empty previous contact sets
+    for (int k = 0; k < 2; ++k) {
+        launch_fill_u32(ctx->pend[k].p_dev, 0u, 1, s);
+        launch_fill_u32(ctx->pend[k].out.off, 0u, 1, s);
+        launch_fill_u32(ctx->cs[k].m_dev, 0u, 1, s);
+        launch_fill_u32(ctx->cs[k].m_narrow, 0u, 1, s);
+        launch_fill_u32(ctx->cs[k].cont_off, 0u, n + 1, s);
+    }
+    launch_fill_u32(ctx->w_overflow, 0u, 1, s);
+    launch_fill_u32(ctx->wb.err, 0u, 1, s);
+    DIO_HIP_CHECK(hipStreamSynchronize(s));
+    std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+    return DIO_OK;
+}
+}  // namespace
+
+extern "C" {
+
 dio_status dio_world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd_body_state* state, uint32_t n,
                             const dio_avbd_constraint* constraints, uint32_t c, const dio_container* container) {
+    return guarded(ctx, [&]() { return world_upload(ctx, bodies, state, n, constraints, c, nullptr, 1, container); });
+}
+
+dio_status dio_batch_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd_body_state* state, uint32_t n,
+                            const dio_avbd_constraint* constraints, uint32_t c, const uint32_t* scene_offsets,
+                            uint32_t num_scenes, const dio_container* containers) {
     return guarded(ctx, [&]() {
-        if (n > ctx->n_cap) capacity("too many bodies for the context", n);
-        if (c > ctx->c_cap) capacity("too many constraints for the context", c);
-        if ((n > 0 && (!bodies || !state)) || (c > 0 && !constraints)) invalid("null input");
-        ensure_world(ctx);
-        hipStream_t s = ctx->stream;
-        DIO_HIP_CHECK(hipStreamSynchronize(s));
-        ctx->drop_graphs();
-        ctx->w_n = n;
-        ctx->w_step = 0;
-        ctx->w_sort_pending = true;
-        ctx->w_friction.resize(n);
-        ctx->w_n_dyn = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            ctx->w_friction[i] = state[i].friction;
-            if (state[i].inv_mass != 0.0f) ctx->w_n_dyn += 1;
-        }
-        if (n > 0) {
-            dio_gpu_body* stage_b = ctx->w_stage_b;
-            dio_avbd_body_state* stage_s = ctx->w_stage_s;
-            DIO_HIP_CHECK(hipMemcpy(stage_b, bodies, (size_t)n * sizeof(dio_gpu_body), hipMemcpyHostToDevice));
-            DIO_HIP_CHECK(hipMemcpy(stage_s, state, (size_t)n * sizeof(dio_avbd_body_state), hipMemcpyHostToDevice));
-            launch_unpack_gpu_bodies(stage_b, n, ctx->w_pos, ctx->w_rot, ctx->w_tag, ctx->w_corners, s);
-            hipLaunchKernelGGL(k_unpack_state, dim3(blocks256(n)), dim3(256), 0, s, stage_s, (const float*)nullptr, n,
-                               ctx->w_vel, ctx->w_ang, ctx->w_prev, ctx->w_inv_mm, (float4*)nullptr,
-                               (float4*)nullptr);
-            DIO_HIP_CHECK(hipStreamSynchronize(s));
-        }
-        upload_joints(ctx, ctx->w_j, constraints, c, n, ctx->w_stage_c);
-        ContainerParams ct{};
-        if (container && container->enabled) {
-            ct.cx = container->center[0]; ct.cy = container->center[1]; ct.cz = container->center[2];
-            ct.qx = container->rotation[0]; ct.qy = container->rotation[1];
-            ct.qz = container->rotation[2]; ct.qw = container->rotation[3];
-            ct.half = container->half_extent;
-            ct.enabled = 1;
-        }
-        ctx->w_ct = ct;
-        // empty lagged narrow output and empty previous contact sets
-        for (int k = 0; k < 2; ++k) {
-            launch_fill_u32(ctx->pend[k].p_dev, 0u, 1, s);
-            launch_fill_u32(ctx->pend[k].out.off, 0u, 1, s);
-            launch_fill_u32(ctx->cs[k].m_dev, 0u, 1, s);
-            launch_fill_u32(ctx->cs[k].m_narrow, 0u, 1, s);
-            launch_fill_u32(ctx->cs[k].cont_off, 0u, n + 1, s);
-        }
-        launch_fill_u32(ctx->w_overflow, 0u, 1, s);
-        launch_fill_u32(ctx->wb.err, 0u, 1, s);
-        DIO_HIP_CHECK(hipStreamSynchronize(s));
-        std::memset(&ctx->stats, 0, sizeof(ctx->stats));
-        return DIO_OK;
+        if (!scene_offsets) invalid("scene_offsets is null");
+        return world_upload(ctx, bodies, state, n, constraints, c, scene_offsets, num_scenes, containers);
     });
 }
 

@@ -460,6 +460,28 @@ class PhysicsContext:
         )
         self._n = len(bodies)
 
+    def batch_upload(self, batch) -> None:
+        """dio_batch_upload: a scene table (``scenes.BatchedScenes`` or any
+        object with bodies, state, offsets, containers, hull and optional
+        constraints in global body indices)."""
+        bodies = np.ascontiguousarray(batch.bodies, R.GPU_BODY)
+        state = np.ascontiguousarray(batch.state, R.BODY_STATE)
+        cons = np.ascontiguousarray(getattr(batch, "constraints", np.zeros(0, R.AVBD_CONSTRAINT)), R.AVBD_CONSTRAINT)
+        offs = np.ascontiguousarray(batch.offsets, np.uint32)
+        ns = len(offs) - 1
+        if len(batch.hull):
+            hull = np.ascontiguousarray(batch.hull, np.float32)
+            self._check(self._lib.dio_world_upload_hulls(self._h, _lib.ptr(hull, ctypes.c_float), len(hull)))
+        cts = (R.Container * ns)()
+        for k, c in enumerate(batch.containers):
+            if c is not None:
+                cts[k] = c.to_c()
+        self._check(
+            self._lib.dio_batch_upload(self._h, _lib.ptr(bodies), _lib.ptr(state), len(bodies), _lib.ptr(cons),
+                                       len(cons), _lib.ptr(offs, ctypes.c_uint32), ns, cts)
+        )
+        self._n = len(bodies)
+
     def step(self, options: AvbdOptions, nsteps: int = 1, lag: int = 1, profile: bool = False) -> None:
         so = R.StepOpts(1 if lag else 0, 0, 1 if profile else 0)
         o = options.to_c()

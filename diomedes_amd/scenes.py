@@ -26,8 +26,9 @@ AvbdContainer whose other walls are out of reach):
   (k 1e4, rest = initial distance) between random grid neighbours;
   dt = 1/120, 8 iterations.
 - ``batched_scenes``: a scene table of independent drops; scene sizes
-  round(exp(U(ln 8, ln 512))); each scene's cubes / spheres fall on its own
-  floor (see BatchedScenes).
+  round(exp(U(ln 8, ln 512))); each scene's unit cubes (h 0.5, mass 1) and
+  subdivision-1 icosphere hulls (r 0.5, density 1) fall on its own floor
+  (see ``BatchedScenes``).
 - ``wall_throw``: a 64 x 32 wall of bricks (0.5 x 0.25 x 0.25, density 1) on
   the floor, hit by a sphere (r 0.5, mass 20) thrown with v0 ~ (15 +- 2,
   2 +- 0.5, 0); dt = 1/60, 6 iterations.
@@ -314,3 +315,75 @@ def wall_throw(seed: int = 0, columns: int = 64, rows: int = 32) -> Scene:
     state["friction"] = np.float32(0.8)
     opts = AvbdOptions(dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), iterations=6)
     return Scene(bodies, state, container=floor_container(40.0), options=opts, name=f"wall_throw{n}_s{seed}")
+
+
+# ------------------------------------------------------------- batched scenes
+class BatchedScenes:
+    """A scene table: scene s owns bodies offsets[s] .. offsets[s+1] - 1 of the
+    concatenated arrays and has its own container (floor).  All scenes share
+    one hull vertex pool (the r = 0.5 icosphere) and one set of options."""
+
+    def __init__(self, scenes: list, hull: np.ndarray, options: AvbdOptions):
+        self.scenes = scenes
+        self.hull = hull
+        self.options = options
+        self.offsets = np.zeros(len(scenes) + 1, np.uint32)
+        self.offsets[1:] = np.cumsum([sc.n for sc in scenes])
+        self.bodies = np.concatenate([sc.bodies for sc in scenes])
+        self.state = np.concatenate([sc.state for sc in scenes])
+        self.containers = [sc.container for sc in scenes]
+
+    @property
+    def n(self) -> int:
+        return int(self.offsets[-1])
+
+    @property
+    def n_dynamic(self) -> int:
+        return int(np.count_nonzero(self.state["inv_mass"] != 0))
+
+    def scene_slice(self, s: int) -> slice:
+        return slice(int(self.offsets[s]), int(self.offsets[s + 1]))
+
+
+def batched_scenes(seed: int = 0, num_scenes: int = 4096, n_min: int = 8, n_max: int = 512) -> BatchedScenes:
+    """Draw order per scene: the body count (1), kinds (n), position jitter
+    (3n), orientations (Shoemake, 3n).  Bodies sit on a square grid of
+    spacing 1.5 (side ceil(sqrt(n / 2)), so about two layers), the lowest
+    layer at y = 1.0, with U(-0.1, 0.1) jitter; the container's floor is y = 0
+    and its walls stand 3 m beyond the grid.  g = -9.81, dt = 1/60, 4
+    iterations, friction 0.5."""
+    rng = SplitMix64(seed)
+    ico = (icosphere(1) * 0.5).astype(np.float32)
+    r = np.float32(0.5)
+    h = np.float32(0.5)
+    m_hull = np.float32(4.0 / 3.0 * np.pi) * r * r * r
+    inv_i_hull = np.float32(1.0) / (np.float32(0.4) * m_hull * r * r)
+    m_cube = np.float32(1.0)
+    inv_i_cube = np.float32(1.0) / (np.float32(2.0 / 3.0) * m_cube * h * h)
+    opts = AvbdOptions(dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), iterations=4, friction=0.5)
+    scenes = []
+    for s in range(num_scenes):
+        u = rng.uniform(0.0, 1.0, 1)[0]
+        n = int(round(math.exp(math.log(n_min) + u * (math.log(n_max) - math.log(n_min)))))
+        kinds = rng.uniform(0.0, 1.0, n) < 0.5  # True: icosphere hull
+        jit = rng.uniform(-0.1, 0.1, 3 * n).reshape(n, 3)
+        q = shoemake(rng, n)
+        side = int(math.ceil(math.sqrt(n / 2.0)))
+        idx = np.arange(n)
+        gx, gz, gy = idx % side, (idx // side) % side, idx // (side * side)
+        pos = np.stack([(gx - (side - 1) / 2.0) * 1.5, 1.0 + gy * 1.5, (gz - (side - 1) / 2.0) * 1.5], axis=1) + jit
+        bodies = np.zeros(n, R.GPU_BODY)
+        state = np.zeros(n, R.BODY_STATE)
+        bodies["position"][:, :3] = pos.astype(np.float32)
+        bodies["orientation"] = q
+        bodies["tag"] = np.where(kinds, R.HULL, R.CUBE)
+        bodies["corners"][kinds, 0, 0] = np.float32(0.0)
+        bodies["corners"][kinds, 0, 1] = np.float32(len(ico))
+        bodies["corners"][~kinds, 0, :3] = h
+        state["inv_mass"] = np.where(kinds, np.float32(1.0) / m_hull, np.float32(1.0) / m_cube)
+        state["inv_moment"][:, :3] = np.where(kinds, inv_i_hull, inv_i_cube)[:, None]
+        state["friction"] = np.float32(0.5)
+        half = float(max(8.0, (side - 1) * 0.75 + 3.0))
+        cont = AvbdContainer(center=(0.0, half, 0.0), rotation=(0.0, 0.0, 0.0, 1.0), half_extent=half)
+        scenes.append(Scene(bodies, state, container=cont, hull=ico, options=opts, name=f"batch{s}_n{n}_s{seed}"))
+    return BatchedScenes(scenes, ico, opts)

@@ -274,6 +274,20 @@ dio_status dio_world_upload(dio_ctx* ctx, const dio_gpu_body* bodies,
                             const dio_avbd_body_state* state, uint32_t n,
                             const dio_avbd_constraint* constraints, uint32_t c,
                             const dio_container* container);
+/* Batched world: num_scenes independent scenes in one world.  Scene s owns
+   bodies scene_offsets[s] .. scene_offsets[s+1] - 1 (offsets run from 0 to
+   n) and has its own container, containers[s].  Bodies of different scenes
+   never form pairs and constraints may not join two scenes (DIO_INVALID).
+   dio_step then steps every scene in the same launches, and each scene's
+   trajectory is bit-identical to stepping it alone; dio_world_download and
+   dio_world_contacts return the concatenation.  num_scenes <= 65536.  No
+   counterpart in the reference (its demos step one scene per Renderer);
+   replaces one plain context per scene. */
+dio_status dio_batch_upload(dio_ctx* ctx, const dio_gpu_body* bodies,
+                            const dio_avbd_body_state* state, uint32_t n,
+                            const dio_avbd_constraint* constraints, uint32_t c,
+                            const uint32_t* scene_offsets, uint32_t num_scenes,
+                            const dio_container* containers);
 /* Vertex pool for DIO_SHAPE_HULL bodies (x, y, z per vertex). */
 dio_status dio_world_upload_hulls(dio_ctx* ctx, const float* vertices, uint32_t count);
 /* Run nsteps physics steps on the device without host round trips. */

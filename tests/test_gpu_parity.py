@@ -393,3 +393,60 @@ def test_static_contacts_bitexact(oracle, fuse, lanes):
             ctx.step(opts, 15)
             w.step(opts, nsteps=15)
             _compare_world(ctx, w, 15 * (k + 1))
+
+
+# ------------------------------------------------------------ batched world
+def _scene_contacts(gc, lo, hi):
+    """The contacts of bodies lo .. hi - 1 in a batched contact list, with
+    scene-local indices (the container stays CONTAINER)."""
+    sel = (gc["a"] >= lo) & (gc["a"] < hi)
+    c = gc[sel].copy()
+    c["a"] -= lo
+    notc = c["b"] != 0xFFFFFFFF
+    c["b"][notc] -= lo
+    return c
+
+
+def test_batched_scenes_match_oracle_per_scene(oracle):
+    """Every scene of a batched world steps bit-identically to the FP32 oracle
+    stepping that scene alone (boxes and icosphere hulls, own floors)."""
+    batch = scenes.batched_scenes(3, num_scenes=6, n_min=8, n_max=48)
+    opts = batch.options
+    worlds = [oracle.OracleWorld(sc, 32) for sc in batch.scenes]
+    with PhysicsContext(batch.n, max_hull_vertices=len(batch.hull)) as ctx:
+        ctx.batch_upload(batch)
+        for k in range(3):
+            ctx.step(opts, 20)
+            got = ctx.download()
+            gc = ctx.contacts()
+            for s, w in enumerate(worlds):
+                w.step(nsteps=20)
+                sl = batch.scene_slice(s)
+                ref = w.state()
+                for key in ("positions", "orientations", "velocities", "angular_velocities", "prev_velocities"):
+                    assert_bits(got[key][sl], ref[key], f"scene {s} step {20 * (k + 1)} {key}")
+                c = _scene_contacts(gc, sl.start, sl.stop)
+                rc = w.contacts()
+                assert len(c) == len(rc["a"]), (s, len(c), len(rc["a"]))
+                assert np.array_equal(c["a"], rc["a"]) and np.array_equal(c["b"], rc["b"])
+                assert_bits(c["lambda_"][:, :3], rc["lambda"], f"scene {s} lambda")
+            _print("batched step", 20 * (k + 1), "contacts", len(gc))
+
+
+def test_batched_scenes_match_plain_contexts():
+    """A 24-scene batch (up to 512 bodies per scene) against each scene stepped
+    in a plain context of its own, GPU against GPU."""
+    batch = scenes.batched_scenes(5, num_scenes=24, n_min=8, n_max=512)
+    opts = batch.options
+    with PhysicsContext(batch.n, max_hull_vertices=len(batch.hull)) as ctx:
+        ctx.batch_upload(batch)
+        ctx.step(opts, 45)
+        got = ctx.download()
+    for s, sc in enumerate(batch.scenes):
+        with PhysicsContext(sc.n, max_hull_vertices=len(sc.hull)) as one:
+            one.upload(sc)
+            one.step(opts, 45)
+            ref = one.download()
+        sl = batch.scene_slice(s)
+        for key in ("positions", "orientations", "velocities", "angular_velocities"):
+            assert_bits(got[key][sl], ref[key], f"scene {s} ({sc.n} bodies) {key}")
