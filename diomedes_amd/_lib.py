@@ -7,7 +7,7 @@ when it is missing: there is no CPU fallback for the physics step.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_char_p, c_float, c_int, c_uint32, c_uint64, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
 from . import records as R
@@ -50,6 +50,13 @@ SIGNATURES = {
     "dio_set_tuning": (c_int, [_VP, POINTER(R.Tuning)]),
     "dio_get_tuning": (c_int, [_VP, POINTER(R.Tuning)]),
     "dio_autotune": (c_int, [_VP, POINTER(R.Options), c_uint32, POINTER(R.Tuning), _FP]),
+    "dio_adjoint_begin": (c_int, [_VP, c_uint32, c_uint32, c_uint64]),
+    "dio_adjoint_end": (c_int, [_VP]),
+    "dio_adjoint_steps": (c_int, [_VP, _U32P]),
+    "dio_adjoint_replay": (c_int, [_VP, POINTER(R.Options), _VP, _FP, _FP]),
+    "dio_adjoint_backward": (
+        c_int, [_VP, POINTER(R.Options), _VP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, POINTER(c_double)]
+    ),
     "dio_world_download": (c_int, [_VP, _FP, _FP, _FP, _FP, _FP]),
     "dio_world_contacts": (c_int, [_VP, _VP, c_uint32, _U32P]),
     "dio_world_pairs": (c_int, [_VP, _VP, c_uint32, _U32P]),

@@ -687,6 +687,12 @@ __device__ __forceinline__ void recover_one(const SolveView& v, const AvbdParams
 // The last dual update (into the contact set's own lam / pen) and recover in
 // one launch: the dual reads only the final Jacobi poses and the initial
 // poses, which recover does not write.
+// recover alone (the replay keeps every iteration's multipliers)
+__global__ void __launch_bounds__(kBlock) k_avbd_recover(SolveView v, AvbdParams p, uint32_t parity) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < p.n) recover_one(v, p, i, parity);
+}
+
 __global__ void __launch_bounds__(kBlock) k_avbd_dual_recover(SolveView v, AvbdParams p, uint32_t iters,
                                                               uint32_t fused) {
     const uint32_t m = v.m_dev ? *v.m_dev : v.m_host;
@@ -780,6 +786,10 @@ static uint32_t contact_blocks(uint32_t work) {
 void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t src, uint32_t dst,
                       uint32_t m_grid, hipStream_t s, const LaunchEvents& ev) {
     launch_k(k_avbd_dual, dim3(contact_blocks(m_grid)), dim3(kBlock), s, ev, v, p, parity, src, dst);
+}
+void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t s) {
+    if (p.n == 0) return;
+    hipLaunchKernelGGL(k_avbd_recover, dim3(blocks_for(p.n)), dim3(kBlock), 0, s, v, p, parity);
 }
 void launch_avbd_dual_recover(const SolveView& v, const AvbdParams& p, uint32_t iters, bool fused, uint32_t m_grid,
                               hipStream_t s, const LaunchEvents& ev) {

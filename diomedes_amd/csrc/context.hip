@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/diomedes_hip.h"
+#include "adjoint.hpp"
 #include "broad.hpp"
 #include "contacts.hpp"
 #include "dio_device.hpp"
@@ -211,7 +212,7 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{32, 8, 4, 1, {0, 0, 0, 0}};
+    dio_tuning tuning{32, 8, 4, 1, 8, {0, 0, 0}};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -274,8 +275,23 @@ struct dio_ctx {
                 }
         graphs_valid = false;
     }
+    // adjoint: the recording of dio_step calls and the replay / backward scratch
+    AdjRecord adj_rec{};
+    AdjWork adj_work{};
+    void* adj_rec_base = nullptr;
+    bool adj_on = false;
+    std::vector<uint8_t> adj_sweep_key;  // what w.ck was swept from (empty: nothing)
+    void free_adjoint() {
+        if (adj_rec_base) (void)hipFree(adj_rec_base);
+        adj_rec_base = nullptr;
+        adj_rec = AdjRecord{};
+        adj_work_free(adj_work);
+        adj_on = false;
+        adj_sweep_key.clear();
+    }
     ~dio_ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
+        free_adjoint();
         drop_graphs();
         for (auto e : events) (void)hipEventDestroy(e);
         for (auto e : prof_events) (void)hipEventDestroy(e);
@@ -293,6 +309,7 @@ template <class F> dio_status guarded(dio_ctx* ctx, F&& f) {
         return f();
     } catch (const HipError& e) {
         ctx->err = std::string(e.what) + ": " + hipGetErrorString(e.code);
+        (void)hipGetLastError();  // a failed call leaves no error for the next one to report
         return DIO_HIP;
     } catch (const StatusError& e) {
         ctx->err = e.msg;
@@ -934,6 +951,9 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         run_contact_build(wv, c->w_ct, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]),
                           c->cs[1 - s], c->scan_partials, c->w_overflow, st);
         run_csr(c->cs[s], n, c->csr, c->w_overflow, st);
+        if (c->adj_on)
+            launch_record_step(c->adj_rec, c->cs[s], c->csr.off, c->csr.idx, c->csr.peer,
+                               BodyState{c->w_pos, c->w_rot, c->w_vel, c->w_ang, c->w_prev}, n, c->m_cap, st);
         if (lag) broad_narrow();
     } else {
         SolveView v = world_solve_view(c, s);
@@ -1048,6 +1068,7 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
     hipStream_t s = ctx->stream;
     DIO_HIP_CHECK(hipStreamSynchronize(s));
     ctx->drop_graphs();
+    ctx->free_adjoint();  // a recording belongs to the world it was made in
     ctx->w_n = n;
     ctx->w_step = 0;
     ctx->w_sort_pending = true;
@@ -1251,12 +1272,16 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         if (v.primal_lanes == 0) v.primal_lanes = 32;
         if (v.broad_lanes == 0) v.broad_lanes = 8;
         if (v.resort_interval == 0) v.resort_interval = 4;
+        if (v.adjoint_lanes == 0) v.adjoint_lanes = 8;
         if (v.primal_lanes != 1 && v.primal_lanes != 4 && v.primal_lanes != 8 && v.primal_lanes != 16 &&
             v.primal_lanes != 32 && v.primal_lanes != 64)
             invalid("primal_lanes must be 1, 4, 8, 16, 32 or 64");
         if (v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 && v.broad_lanes != 8)
             invalid("broad_lanes must be 1, 2, 4 or 8");
         if (v.fuse_dual > 1) invalid("fuse_dual must be 0 or 1");
+        if (v.adjoint_lanes != 1 && v.adjoint_lanes != 2 && v.adjoint_lanes != 4 && v.adjoint_lanes != 8 &&
+            v.adjoint_lanes != 16)
+            invalid("adjoint_lanes must be 1, 2, 4, 8 or 16");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
@@ -1461,4 +1486,287 @@ dio_status dio_world_pairs(dio_ctx* ctx, dio_gpu_pair* out, uint32_t cap, uint32
     });
 }
 
+
+// ------------------------------------------------------------------ adjoint
+}  // extern "C"
+
+namespace {
+AdjHostIndex adj_index(dio_ctx* ctx) {
+    if (!ctx->adj_rec_base) throw StatusError{DIO_NOT_READY, "no recording (dio_adjoint_begin)", 0};
+    AdjHostIndex h;
+    uint32_t cnt[kRecCounters] = {};
+    DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    DIO_HIP_CHECK(hipMemcpy(cnt, ctx->adj_rec.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    h.steps = cnt[kRecSteps];
+    h.overflow = cnt[kRecOverflow];
+    h.coff.resize(h.steps + 1);
+    h.ioff.resize(h.steps + 1);
+    h.m.resize(h.steps);
+    if (h.steps > 0) {
+        DIO_HIP_CHECK(hipMemcpy(h.coff.data(), ctx->adj_rec.step_coff, (h.steps + 1) * 4, hipMemcpyDeviceToHost));
+        DIO_HIP_CHECK(hipMemcpy(h.ioff.data(), ctx->adj_rec.step_ioff, (h.steps + 1) * 4, hipMemcpyDeviceToHost));
+        DIO_HIP_CHECK(hipMemcpy(h.m.data(), ctx->adj_rec.step_m, h.steps * 4, hipMemcpyDeviceToHost));
+    } else {
+        h.coff[0] = 0;
+        h.ioff[0] = 0;
+    }
+    return h;
+}
+
+// the problem of a replay / backward, with the masses of `state` when given
+AdjProblem adj_problem(dio_ctx* ctx, const dio_avbd_options* o, const dio_avbd_body_state* state) {
+    AdjProblem pb;
+    pb.n = ctx->w_n;
+    pb.p = params_of(o->dt, o->gravity, o->alpha, o->beta_lin, o->penalty_max, ctx->w_n);
+    pb.iters = o->iterations;
+    pb.lanes = ctx->tuning.primal_lanes;
+    pb.rev_lanes = ctx->tuning.adjoint_lanes;
+    pb.mu = o->friction;
+    pb.inv_mm = ctx->w_inv_mm;
+    if (state) {
+        std::vector<float4> imm(ctx->w_n);
+        for (uint32_t i = 0; i < ctx->w_n; ++i)
+            imm[i] = make_float4(state[i].inv_moment[0], state[i].inv_moment[1], state[i].inv_moment[2],
+                                 state[i].inv_mass);
+        if (ctx->w_n > 0)
+            DIO_HIP_CHECK(hipMemcpy(ctx->adj_work.inv_mm, imm.data(), (size_t)ctx->w_n * 16, hipMemcpyHostToDevice));
+        pb.inv_mm = ctx->adj_work.inv_mm;
+    }
+    return pb;
+}
+
+void adj_ready(dio_ctx* ctx, const dio_avbd_options* o, const AdjHostIndex& h) {
+    if (!ctx->adj_rec_base) throw StatusError{DIO_NOT_READY, "no recording (dio_adjoint_begin)", 0};
+    if (!o) invalid("options is null");
+    if (o->iterations == 0) invalid("iterations must be positive");
+    if (h.overflow & 1u) capacity("the recording ran out of contact storage", h.coff.back());
+    if (h.overflow & 2u) throw StatusError{DIO_CAPACITY, "more steps than the recording holds", h.steps};
+    if (h.steps == 0) throw StatusError{DIO_NOT_READY, "no recorded steps (dio_step after dio_adjoint_begin)", 0};
+    uint32_t m_max = 1;
+    for (uint32_t v : h.m) m_max = std::max(m_max, v);
+    adj_work_reserve(ctx->adj_work, ctx->n_cap, ctx->adj_rec.every, o->iterations, m_max,
+                     ctx->adj_rec.max_steps / ctx->adj_rec.every + 1);
+}
+
+// The key of a sweep: the recorded step count, the options and state0.
+std::vector<uint8_t> adj_key(const AdjHostIndex& h, const dio_avbd_options* o, const dio_avbd_body_state* state0,
+                             uint32_t n) {
+    std::vector<uint8_t> k(sizeof(uint32_t) + sizeof(dio_avbd_options));
+    std::memcpy(k.data(), &h.steps, sizeof(uint32_t));
+    std::memcpy(k.data() + sizeof(uint32_t), o, sizeof(dio_avbd_options));
+    if (state0) {
+        const uint8_t* b = reinterpret_cast<const uint8_t*>(state0);
+        k.insert(k.end(), b, b + (size_t)n * sizeof(dio_avbd_body_state));
+    }
+    return k;
+}
+
+// Forward sweep over every recorded step from the first checkpoint, with
+// state0's velocities (and, through pb, masses) when given, saving each
+// segment's starting state in w.ck.  The working state ends as the final state.
+void adj_sweep(dio_ctx* ctx, const AdjHostIndex& h, const AdjProblem& pb, const dio_avbd_body_state* state0) {
+    hipStream_t s = ctx->stream;
+    AdjWork& w = ctx->adj_work;
+    const uint32_t n = ctx->w_n;
+    adj_restore(ctx->adj_rec.ck, w, 0, n, s);
+    if (state0 && n > 0) {
+        // the state's velocities replace the checkpoint's
+        std::vector<float4> v(3 * (size_t)n);
+        for (uint32_t i = 0; i < n; ++i) {
+            v[i] = make_float4(state0[i].vel[0], state0[i].vel[1], state0[i].vel[2], 0.0f);
+            v[n + i] = make_float4(state0[i].ang[0], state0[i].ang[1], state0[i].ang[2], 0.0f);
+            v[2 * (size_t)n + i] = make_float4(state0[i].prev_vel[0], state0[i].prev_vel[1], state0[i].prev_vel[2], 0.0f);
+        }
+        DIO_HIP_CHECK(hipMemcpyAsync(w.st.vel, v.data(), (size_t)n * 16, hipMemcpyHostToDevice, s));
+        DIO_HIP_CHECK(hipMemcpyAsync(w.st.ang, v.data() + n, (size_t)n * 16, hipMemcpyHostToDevice, s));
+        DIO_HIP_CHECK(hipMemcpyAsync(w.st.prev, v.data() + 2 * (size_t)n, (size_t)n * 16, hipMemcpyHostToDevice, s));
+        DIO_HIP_CHECK(hipStreamSynchronize(s));  // v is released on return
+    }
+    adj_set_friction(ctx->adj_rec, h, pb.mu, s);
+    const uint32_t E = ctx->adj_rec.every;
+    for (uint32_t first = 0, k = 0; first < h.steps; first += E, ++k) {
+        adj_save(w, k, n, s);
+        adj_replay(ctx->adj_rec, h, w, pb, first, std::min(h.steps, first + E), s);
+    }
+    DIO_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+extern "C" {
+
+dio_status dio_adjoint_begin(dio_ctx* ctx, uint32_t max_steps, uint32_t checkpoint_every,
+                             uint64_t max_recorded_contacts) {
+    return guarded(ctx, [&]() {
+        if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
+        if (max_steps == 0 || checkpoint_every == 0) invalid("max_steps and checkpoint_every must be positive");
+        if (ctx->w_j.c > 0) invalid("the adjoint does not differentiate joints: the world has constraints");
+        if (max_recorded_contacts == 0) max_recorded_contacts = (uint64_t)max_steps * ctx->m_cap;
+        if (max_recorded_contacts > 0x7FFFFFFFull) invalid("max_recorded_contacts above 2^31 - 1");
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->free_adjoint();
+        const uint64_t rc = max_recorded_contacts, ic = 2 * max_recorded_contacts;
+        const uint32_t n = ctx->n_cap, nck = max_steps / checkpoint_every + 1;
+        size_t bytes = 0;
+        auto take = [&](size_t b) {
+            const size_t at = bytes;
+            bytes += (b + 255) & ~(size_t)255;
+            return at;
+        };
+        const size_t o_ab = take(rc * 8), o_c = take(rc * 16 * 8);
+        const size_t o_coff = take(((size_t)max_steps + 1) * 4), o_ioff = take(((size_t)max_steps + 1) * 4);
+        const size_t o_m = take((size_t)max_steps * 4), o_off = take((size_t)max_steps * (n + 1) * 4);
+        const size_t o_idx = take(ic * 4), o_peer = take(ic * 4), o_cnt = take(kRecCounters * 4);
+        const size_t o_ck = take((size_t)nck * 5 * n * 16);
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess)
+            throw StatusError{DIO_CAPACITY, "the recording does not fit in device memory", bytes};
+        ctx->adj_rec_base = p;
+        char* b = static_cast<char*>(p);
+        AdjRecord& r = ctx->adj_rec;
+        r.max_steps = max_steps;
+        r.every = checkpoint_every;
+        r.n_cap = n;
+        r.rec_cap = (uint32_t)rc;
+        r.idx_cap = (uint32_t)std::min<uint64_t>(ic, 0xFFFFFFFFull);
+        r.c.ab = reinterpret_cast<uint2*>(b + o_ab);
+        float4* cf = reinterpret_cast<float4*>(b + o_c);
+        r.c.n = cf; r.c.t1 = cf + rc; r.c.t2 = cf + 2 * rc; r.c.ra = cf + 3 * rc;
+        r.c.rb = cf + 4 * rc; r.c.c0 = cf + 5 * rc; r.c.pen = cf + 6 * rc; r.c.lam = cf + 7 * rc;
+        r.step_coff = reinterpret_cast<uint32_t*>(b + o_coff);
+        r.step_ioff = reinterpret_cast<uint32_t*>(b + o_ioff);
+        r.step_m = reinterpret_cast<uint32_t*>(b + o_m);
+        r.csr_off = reinterpret_cast<uint32_t*>(b + o_off);
+        r.csr_idx = reinterpret_cast<uint32_t*>(b + o_idx);
+        r.csr_peer = reinterpret_cast<uint32_t*>(b + o_peer);
+        r.counters = reinterpret_cast<uint32_t*>(b + o_cnt);
+        r.ck = reinterpret_cast<float4*>(b + o_ck);
+        launch_fill_u32(r.counters, 0u, kRecCounters, ctx->stream);
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->adj_on = true;
+        ctx->drop_graphs();
+        return DIO_OK;
+    });
+}
+
+dio_status dio_adjoint_end(dio_ctx* ctx) {
+    return guarded(ctx, [&]() {
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->free_adjoint();
+        ctx->drop_graphs();
+        return DIO_OK;
+    });
+}
+
+dio_status dio_adjoint_steps(dio_ctx* ctx, uint32_t* steps) {
+    return guarded(ctx, [&]() {
+        if (!steps) invalid("steps is null");
+        if (!ctx->adj_rec_base) throw StatusError{DIO_NOT_READY, "no recording (dio_adjoint_begin)", 0};
+        *steps = adj_index(ctx).steps;
+        return DIO_OK;
+    });
+}
+
+dio_status dio_adjoint_replay(dio_ctx* ctx, const dio_avbd_options* options, const dio_avbd_body_state* state0,
+                              float* positions, float* orientations) {
+    return guarded(ctx, [&]() {
+        const AdjHostIndex h = adj_index(ctx);
+        adj_ready(ctx, options, h);
+        hipStream_t s = ctx->stream;
+        AdjWork& w = ctx->adj_work;
+        const AdjProblem pb = adj_problem(ctx, options, state0);
+        const uint32_t n = ctx->w_n;
+        ctx->adj_sweep_key.clear();
+        adj_sweep(ctx, h, pb, state0);
+        DIO_HIP_CHECK(hipStreamSynchronize(s));
+        ctx->adj_sweep_key = adj_key(h, options, state0, n);
+        std::vector<float4> buf(n);
+        if (positions && n > 0) {
+            DIO_HIP_CHECK(hipMemcpy(buf.data(), w.st.pos, (size_t)n * 16, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; ++i) {
+                positions[3 * i] = buf[i].x; positions[3 * i + 1] = buf[i].y; positions[3 * i + 2] = buf[i].z;
+            }
+        }
+        if (orientations && n > 0)
+            DIO_HIP_CHECK(hipMemcpy(orientations, w.st.rot, (size_t)n * 16, hipMemcpyDeviceToHost));
+        return DIO_OK;
+    });
+}
+
+dio_status dio_adjoint_backward(dio_ctx* ctx, const dio_avbd_options* options,
+                                const dio_avbd_body_state* state0, const float* d_positions,
+                                const float* d_orientations, const float* d_velocities, const float* d_angular,
+                                float* g_positions, float* g_orientations, float* g_velocities, float* g_angular,
+                                float* g_prev_velocities, float* g_inv_mass, float* g_inv_moment,
+                                double* g_friction) {
+    return guarded(ctx, [&]() {
+        const AdjHostIndex h = adj_index(ctx);
+        adj_ready(ctx, options, h);
+        hipStream_t s = ctx->stream;
+        AdjWork& w = ctx->adj_work;
+        const AdjProblem pb = adj_problem(ctx, options, state0);
+        const uint32_t n = ctx->w_n;
+        // segment checkpoints: the recording's (the recorded run), or those
+        // of a sweep from state0 (kept from the last replay when it matches)
+        const float4* ck = ctx->adj_rec.ck;
+        if (state0) {
+            const std::vector<uint8_t> key = adj_key(h, options, state0, n);
+            if (key != ctx->adj_sweep_key) {
+                ctx->adj_sweep_key.clear();
+                adj_sweep(ctx, h, pb, state0);
+                ctx->adj_sweep_key = key;
+            }
+            ck = w.ck;
+        }
+        std::vector<float4> sb(5 * (size_t)n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (d_positions) sb[i] = make_float4(d_positions[3 * i], d_positions[3 * i + 1], d_positions[3 * i + 2], 0.0f);
+            if (d_orientations)
+                sb[n + i] = make_float4(d_orientations[4 * i], d_orientations[4 * i + 1], d_orientations[4 * i + 2],
+                                        d_orientations[4 * i + 3]);
+            if (d_velocities)
+                sb[2 * (size_t)n + i] = make_float4(d_velocities[3 * i], d_velocities[3 * i + 1], d_velocities[3 * i + 2], 0.0f);
+            if (d_angular)
+                sb[3 * (size_t)n + i] = make_float4(d_angular[3 * i], d_angular[3 * i + 1], d_angular[3 * i + 2], 0.0f);
+        }
+        for (int f = 0; f < 5; ++f)
+            if (n > 0)
+                DIO_HIP_CHECK(hipMemcpy(w.sbar + (size_t)f * w.n_cap, sb.data() + (size_t)f * n, (size_t)n * 16,
+                                        hipMemcpyHostToDevice));
+        adj_set_friction(ctx->adj_rec, h, pb.mu, s);
+        adj_backward(ctx->adj_rec, h, w, pb, ck, s);
+        DIO_HIP_CHECK(hipGetLastError());
+        DIO_HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<float4> out(5 * (size_t)n), mb(n);
+        for (int f = 0; f < 5; ++f)
+            if (n > 0)
+                DIO_HIP_CHECK(hipMemcpy(out.data() + (size_t)f * n, w.sbar + (size_t)f * w.n_cap, (size_t)n * 16,
+                                        hipMemcpyDeviceToHost));
+        if (n > 0) DIO_HIP_CHECK(hipMemcpy(mb.data(), w.mbar, (size_t)n * 16, hipMemcpyDeviceToHost));
+        double mu_total = 0.0;
+        DIO_HIP_CHECK(hipMemcpy(&mu_total, w.mubar_total, 8, hipMemcpyDeviceToHost));
+        auto put3 = [&](float* dst, int f) {
+            if (!dst) return;
+            for (uint32_t i = 0; i < n; ++i) {
+                const float4 v = out[(size_t)f * n + i];
+                dst[3 * i] = v.x; dst[3 * i + 1] = v.y; dst[3 * i + 2] = v.z;
+            }
+        };
+        put3(g_positions, 0);
+        if (g_orientations)
+            for (uint32_t i = 0; i < n; ++i) {
+                const float4 v = out[n + i];
+                g_orientations[4 * i] = v.x; g_orientations[4 * i + 1] = v.y;
+                g_orientations[4 * i + 2] = v.z; g_orientations[4 * i + 3] = v.w;
+            }
+        put3(g_velocities, 2);
+        put3(g_angular, 3);
+        put3(g_prev_velocities, 4);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (g_inv_mass) g_inv_mass[i] = mb[i].w;
+            if (g_inv_moment) { g_inv_moment[3 * i] = mb[i].x; g_inv_moment[3 * i + 1] = mb[i].y; g_inv_moment[3 * i + 2] = mb[i].z; }
+        }
+        if (g_friction) *g_friction = mu_total;
+        return DIO_OK;
+    });
+}
 }  // extern "C"

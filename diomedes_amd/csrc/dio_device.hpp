@@ -87,5 +87,7 @@ void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, 
                       uint32_t m_grid, hipStream_t stream, const LaunchEvents& ev = {});
 void launch_avbd_dual_recover(const SolveView& v, const AvbdParams& p, uint32_t iters, bool fused, uint32_t m_grid,
                               hipStream_t stream, const LaunchEvents& ev = {});
+// recover alone, reading the final Jacobi buffer pos[1 - parity]
+void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream);
 
 }  // namespace dio

@@ -482,13 +482,62 @@ class PhysicsContext:
         )
         self._n = len(bodies)
 
+    # ---- adjoint (frozen-contact replay of recorded steps)
+    def adjoint_begin(self, max_steps: int, checkpoint_every: int = 8, max_recorded_contacts: int = 0) -> None:
+        """Record the following steps for replay and reverse mode."""
+        self._check(self._lib.dio_adjoint_begin(self._h, max_steps, checkpoint_every, max_recorded_contacts))
+
+    def adjoint_end(self) -> None:
+        self._check(self._lib.dio_adjoint_end(self._h))
+
+    def adjoint_steps(self) -> int:
+        v = ctypes.c_uint32()
+        self._check(self._lib.dio_adjoint_steps(self._h, ctypes.byref(v)))
+        return v.value
+
+    def adjoint_replay(self, options: AvbdOptions, state0: np.ndarray | None = None):
+        """Replay the recording (optionally from other velocities / masses):
+        final positions (n x 3) and orientations (n x 4)."""
+        n = self._n
+        pos = np.zeros((n, 3), np.float32)
+        rot = np.zeros((n, 4), np.float32)
+        st = None if state0 is None else np.ascontiguousarray(state0, R.BODY_STATE)
+        o = options.to_c()
+        self._check(self._lib.dio_adjoint_replay(self._h, ctypes.byref(o), _lib.ptr(st),
+                                                 _lib.ptr(pos, ctypes.c_float), _lib.ptr(rot, ctypes.c_float)))
+        return pos, rot
+
+    def adjoint_backward(self, options: AvbdOptions, d_positions=None, d_orientations=None, d_velocities=None,
+                         d_angular=None, state0: np.ndarray | None = None) -> dict:
+        """Reverse mode through the recording (the replay from state0, or the
+        recorded run when None): adjoints of the initial state, the per-body
+        inverse masses / moments and the friction, given the adjoint of the
+        final state."""
+        n = self._n
+        st = None if state0 is None else np.ascontiguousarray(state0, R.BODY_STATE)
+        f = lambda a, k: None if a is None else np.ascontiguousarray(a, np.float32).reshape(n, k)
+        ins = [f(d_positions, 3), f(d_orientations, 4), f(d_velocities, 3), f(d_angular, 3)]
+        out = {"positions": np.zeros((n, 3), np.float32), "orientations": np.zeros((n, 4), np.float32),
+               "velocities": np.zeros((n, 3), np.float32), "angular_velocities": np.zeros((n, 3), np.float32),
+               "prev_velocities": np.zeros((n, 3), np.float32), "inv_mass": np.zeros(n, np.float32),
+               "inv_moment": np.zeros((n, 3), np.float32)}
+        mu = ctypes.c_double()
+        o = options.to_c()
+        fp = lambda a: _lib.ptr(a, ctypes.c_float)
+        self._check(self._lib.dio_adjoint_backward(
+            self._h, ctypes.byref(o), _lib.ptr(st), *[fp(a) for a in ins], fp(out["positions"]), fp(out["orientations"]),
+            fp(out["velocities"]), fp(out["angular_velocities"]), fp(out["prev_velocities"]), fp(out["inv_mass"]),
+            fp(out["inv_moment"]), ctypes.byref(mu)))
+        out["friction"] = mu.value
+        return out
+
     def step(self, options: AvbdOptions, nsteps: int = 1, lag: int = 1, profile: bool = False) -> None:
         so = R.StepOpts(1 if lag else 0, 0, 1 if profile else 0)
         o = options.to_c()
         self._check(self._lib.dio_step(self._h, ctypes.byref(o), ctypes.byref(so), nsteps))
 
     def set_tuning(self, primal_lanes: int = 0, broad_lanes: int = 0, resort_interval: int = 0,
-                   fuse_dual: int | None = None) -> None:
+                   fuse_dual: int | None = None, adjoint_lanes: int = 0) -> None:
         """Launch shapes and schedule (0 / None keeps the current value)."""
         cur = self.tuning()
         t = R.Tuning()
@@ -496,13 +545,14 @@ class PhysicsContext:
         t.broad_lanes = broad_lanes or cur["broad_lanes"]
         t.resort_interval = resort_interval or cur["resort_interval"]
         t.fuse_dual = cur["fuse_dual"] if fuse_dual is None else int(fuse_dual)
+        t.adjoint_lanes = adjoint_lanes or cur["adjoint_lanes"]
         self._check(self._lib.dio_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
         t = R.Tuning()
         self._check(self._lib.dio_get_tuning(self._h, ctypes.byref(t)))
         return {"primal_lanes": t.primal_lanes, "broad_lanes": t.broad_lanes, "resort_interval": t.resort_interval,
-                "fuse_dual": t.fuse_dual}
+                "fuse_dual": t.fuse_dual, "adjoint_lanes": t.adjoint_lanes}
 
     def autotune(self, options: AvbdOptions, reps: int = 20) -> dict:
         """Time every primal and broad-phase launch shape and both dual

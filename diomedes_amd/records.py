@@ -173,7 +173,7 @@ assert ctypes.sizeof(StepStats) == 88
 class Tuning(ctypes.Structure):
     _fields_ = [("primal_lanes", ctypes.c_uint32), ("broad_lanes", ctypes.c_uint32),
                 ("resort_interval", ctypes.c_uint32), ("fuse_dual", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("adjoint_lanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 def f32(x) -> float:

@@ -31,7 +31,9 @@ AvbdContainer whose other walls are out of reach):
   (see ``BatchedScenes``).
 - ``wall_throw``: a 64 x 32 wall of bricks (0.5 x 0.25 x 0.25, density 1) on
   the floor, hit by a sphere (r 0.5, mass 20) thrown with v0 ~ (15 +- 2,
-  2 +- 0.5, 0); dt = 1/60, 6 iterations.
+  2 +- 0.5, 0); dt = 1/60, 6 iterations, the reference demo's other options
+  (beta 5000, margin 0.002, penalty_max 1e6, gamma 0.995, friction 0.25,
+  g = -10).
 """
 
 from __future__ import annotations
@@ -313,7 +315,9 @@ def wall_throw(seed: int = 0, columns: int = 64, rows: int = 32) -> Scene:
     state[n]["inv_moment"][:3] = np.float32(1.0) / (np.float32(0.4) * m * r * r)
     state[n]["vel"][:3] = (vx, vy, 0.0)
     state["friction"] = np.float32(0.8)
-    opts = AvbdOptions(dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), iterations=6)
+    # the reference demo's preset (demos/wall_throw/src/main.rs:61-68) with 6 iterations
+    opts = AvbdOptions(dt=1.0 / 60.0, gravity=(0.0, -10.0, 0.0), iterations=6, alpha=0.99, beta_lin=5000.0,
+                       margin=0.002, penalty_max=1.0e6, gamma=0.995, friction=0.25)
     return Scene(bodies, state, container=floor_container(40.0), options=opts, name=f"wall_throw{n}_s{seed}")
 
 
@@ -387,3 +391,12 @@ def batched_scenes(seed: int = 0, num_scenes: int = 4096, n_min: int = 8, n_max:
         cont = AvbdContainer(center=(0.0, half, 0.0), rotation=(0.0, 0.0, 0.0, 1.0), half_extent=half)
         scenes.append(Scene(bodies, state, container=cont, hull=ico, options=opts, name=f"batch{s}_n{n}_s{seed}"))
     return BatchedScenes(scenes, ico, opts)
+
+
+def wall_throw_target(scene: Scene, seed: int) -> np.ndarray:
+    """The seeded target field of the differentiable wall throw: each brick's
+    initial centre plus U(-0.5, 0.5) per axis (draws after the scene's own,
+    from stream seed + 1), shape (bricks, 3), float64."""
+    rng = SplitMix64(seed + 1)
+    nb = scene.n - 1
+    return scene.bodies["position"][:nb, :3].astype(np.float64) + rng.uniform(-0.5, 0.5, 3 * nb).reshape(nb, 3)

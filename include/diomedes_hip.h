@@ -216,7 +216,9 @@ typedef struct dio_tuning {
     uint32_t fuse_dual;        /* 1: each primal iteration k >= 1 applies dual update k - 1 itself
                                   (fewer launches, both bodies of a contact evaluate it; default);
                                   0: a dual launch after each primal */
-    uint32_t reserved[4];
+    uint32_t adjoint_lanes;    /* lanes per body in the adjoint's reverse primal: 1, 2, 4, 8 or 16
+                                  (0 = default 8) */
+    uint32_t reserved[3];
 } dio_tuning;
 
 typedef struct dio_ctx dio_ctx;
@@ -309,6 +311,49 @@ dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out);
    result submitted before the call must be submitted again. */
 dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t reps, dio_tuning* chosen,
                         float* us_per_launch);
+/* ---- adjoint of the AVBD step (no counterpart in the reference) -------
+   The differentiated function is the frozen-contact replay: each recorded
+   step's contact set (geometry and the warm-start lambda / penalty as the
+   solve received them) and CSR are kept, and only the AVBD solve runs, with
+   every contact's friction set to options->friction.  At the recorded
+   parameters it reproduces the recorded run bit for bit.  Worlds with joints
+   are rejected (DIO_INVALID).
+
+   dio_adjoint_begin: record the following dio_step calls (at most max_steps
+   steps and max_recorded_contacts contacts over all of them; 0 = max_steps x
+   max_contacts), with a state checkpoint every checkpoint_every steps.
+   dio_adjoint_end frees the recording. */
+dio_status dio_adjoint_begin(dio_ctx* ctx, uint32_t max_steps, uint32_t checkpoint_every,
+                             uint64_t max_recorded_contacts);
+dio_status dio_adjoint_end(dio_ctx* ctx);
+dio_status dio_adjoint_steps(dio_ctx* ctx, uint32_t* steps);
+/* Replay and backward need at least one recorded step (DIO_NOT_READY
+   otherwise) and a recording that did not overflow (DIO_CAPACITY).
+   Replay every recorded step from the first checkpoint.  state0 (n records
+   or null) replaces the velocities and masses of the recorded start.  Final
+   positions (n x 3) and orientations (n x 4); the world is not modified. */
+dio_status dio_adjoint_replay(dio_ctx* ctx, const dio_avbd_options* options,
+                              const dio_avbd_body_state* state0, float* positions,
+                              float* orientations);
+/* Reverse mode through every recorded step (checkpointed: each segment is
+   replayed once to tape its iterations), at the replay from state0 (null:
+   the recorded run itself, whose checkpoints the recording holds; otherwise
+   a forward sweep from state0 saves the segment checkpoints first, unless the
+   last dio_adjoint_replay ran from the same state0, options and steps).
+   Inputs: the adjoint of the final state (any pointer may be null = 0; n x 3
+   or n x 4 floats).  Outputs (any may be null): the adjoint of the initial
+   positions, orientations, velocities, angular and previous velocities, of
+   each body's inverse mass (n) and inverse moments (n x 3), and of the
+   friction coefficient.  A new dio_world_upload / dio_batch_upload frees the
+   recording. */
+dio_status dio_adjoint_backward(dio_ctx* ctx, const dio_avbd_options* options,
+                                const dio_avbd_body_state* state0,
+                                const float* d_positions, const float* d_orientations,
+                                const float* d_velocities, const float* d_angular,
+                                float* g_positions, float* g_orientations, float* g_velocities,
+                                float* g_angular, float* g_prev_velocities, float* g_inv_mass,
+                                float* g_inv_moment, double* g_friction);
+
 /* Download body state (any pointer may be null): n x 3 / n x 4 floats. */
 dio_status dio_world_download(dio_ctx* ctx, float* positions, float* orientations,
                               float* velocities, float* angular_velocities,

@@ -57,6 +57,17 @@ template <class T> struct Q { T x, y, z, w; };
 template <class T> inline V3<T> v3(T x, T y, T z) { return V3<T>{x, y, z}; }
 template <class T> inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 template <class T> inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+// sqrt / |x| found by argument-dependent lookup, so the templates also run on
+// the forward-mode dual numbers below (std::sqrt / std::fabs for float and
+// double, unchanged)
+template <class T> inline T osqrt(T x) {
+    using std::sqrt;
+    return sqrt(x);
+}
+template <class T> inline T ofabs(T x) {
+    using std::fabs;
+    return fabs(x);
+}
 template <class T> inline V3<T> operator-(V3<T> a) { return {-a.x, -a.y, -a.z}; }
 template <class T> inline V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
 template <class T> inline V3<T> operator/(V3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
@@ -67,7 +78,7 @@ template <class T> inline V3<T> cross(V3<T> a, V3<T> b) {
     return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
 }
 template <class T> inline T length_squared(V3<T> a) { return dot(a, a); }
-template <class T> inline T length(V3<T> a) { return std::sqrt(dot(a, a)); }
+template <class T> inline T length(V3<T> a) { return osqrt(dot(a, a)); }
 template <class T> inline V3<T> normalize(V3<T> a) { return a * (T(1) / length(a)); }
 template <class T> inline V3<T> normalize_or_zero(V3<T> a) {
     T rcp = T(1) / length(a);
@@ -94,7 +105,7 @@ template <class T> inline Q<T> qinv(Q<T> q) { return {-q.x, -q.y, -q.z, q.w}; }
 template <class T> inline Q<T> qadd(Q<T> a, Q<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
 template <class T> inline Q<T> qscale(Q<T> a, T s) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
 template <class T> inline Q<T> qnormalize(Q<T> q) {
-    T len = std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    T len = osqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     return qscale(q, T(1) / len);
 }
 // glam Quat * Vec3: v (w^2 - b.b) + b (2 v.b) + (b x v)(2 w)
@@ -313,9 +324,9 @@ template <class T> inline void body_axes(const Body<T>& b, V3<T> ax[3]) {
 template <class T> inline T box_radius(const Body<T>& b, const V3<T> ax[3], V3<T> axis) {
     if (b.tag == CUBE)
         return b.corners[0][0] *
-               (std::fabs(dot(axis, ax[0])) + std::fabs(dot(axis, ax[1])) + std::fabs(dot(axis, ax[2])));
-    return b.corners[0][0] * std::fabs(dot(axis, ax[0])) + b.corners[0][1] * std::fabs(dot(axis, ax[1])) +
-           b.corners[0][2] * std::fabs(dot(axis, ax[2]));
+               (ofabs(dot(axis, ax[0])) + ofabs(dot(axis, ax[1])) + ofabs(dot(axis, ax[2])));
+    return b.corners[0][0] * ofabs(dot(axis, ax[0])) + b.corners[0][1] * ofabs(dot(axis, ax[1])) +
+           b.corners[0][2] * ofabs(dot(axis, ax[2]));
 }
 template <class T> inline T box_half(const Body<T>& b, uint32_t axis) {
     return b.tag == CUBE ? b.corners[0][0] : b.corners[0][axis];
@@ -335,7 +346,7 @@ inline bool test_axis(V3<T> raw, uint32_t kind, uint32_t index, const Body<T>& a
     V3<T> axis = normalize(raw);
     T ra = box_radius(a, aa, axis);
     T rb = box_radius(b, ba, axis);
-    T overlap = ra + rb - std::fabs(dot(delta, axis));
+    T overlap = ra + rb - ofabs(dot(delta, axis));
     if (overlap < T(0)) return false;
     if (overlap < best.overlap) {
         best.overlap = overlap;
@@ -395,9 +406,9 @@ template <class T> inline uint32_t incident_axis(const Body<T>& body, V3<T> n) {
     V3<T> ax[3];
     body_axes(body, ax);
     uint32_t best_axis = 0;
-    T best_dot = std::fabs(dot(ax[0], n));
+    T best_dot = ofabs(dot(ax[0], n));
     for (uint32_t i = 1; i < 3; ++i) {
-        T d = std::fabs(dot(ax[i], n));
+        T d = ofabs(dot(ax[i], n));
         if (d > best_dot) { best_dot = d; best_axis = i; }
     }
     return best_axis;
@@ -417,7 +428,7 @@ inline uint32_t clip_plane(const V3<T> in[8], uint32_t in_count, V3<T> n, T offs
         bool cur_in = cur_d <= T(1.0e-5);
         if (prev_in != cur_in) {
             T den = prev_d - cur_d;
-            if (std::fabs(den) > T(1.0e-8) && count < 8) {
+            if (ofabs(den) > T(1.0e-8) && count < 8) {
                 T t = rclamp(prev_d / den, T(0), T(1));
                 out[count++] = prev + (cur - prev) * t;
             }
@@ -530,8 +541,8 @@ template <class T> inline V3<T> triple_product(V3<T> a, V3<T> b, V3<T> c) { retu
 // gjk.rs:214-223
 template <class T> inline V3<T> any_perpendicular(V3<T> v) {
     V3<T> axis;
-    if (std::fabs(v.x) <= std::fabs(v.y) && std::fabs(v.x) <= std::fabs(v.z)) axis = {T(1), T(0), T(0)};
-    else if (std::fabs(v.y) <= std::fabs(v.z)) axis = {T(0), T(1), T(0)};
+    if (ofabs(v.x) <= ofabs(v.y) && ofabs(v.x) <= ofabs(v.z)) axis = {T(1), T(0), T(0)};
+    else if (ofabs(v.y) <= ofabs(v.z)) axis = {T(0), T(1), T(0)};
     else axis = {T(0), T(0), T(1)};
     return normalize_or_zero(cross(axis, v));
 }
@@ -640,7 +651,7 @@ template <class T> inline Penetration<T> face_contact(const Face<T>& f, const Po
     T d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
     T denom = d00 * d11 - d01 * d01;
     T w1, w2, w3;
-    if (std::fabs(denom) > T(1e-12)) {
+    if (ofabs(denom) > T(1e-12)) {
         w2 = (d11 * d20 - d01 * d21) / denom;
         w3 = (d00 * d21 - d01 * d20) / denom;
         w1 = T(1) - w2 - w3;
@@ -784,7 +795,7 @@ std::vector<RawContact<T>> narrow(const std::vector<Body<T>>& bodies, const std:
 // --------------------------------------------------- host contact building
 // physics/gpu.rs:406-414
 template <class T> inline void tangents(V3<T> n, V3<T>& t1, V3<T>& t2) {
-    V3<T> c = std::fabs(n.x) > T(0.9) ? cross(n, V3<T>{T(0), T(1), T(0)}) : cross(n, V3<T>{T(1), T(0), T(0)});
+    V3<T> c = ofabs(n.x) > T(0.9) ? cross(n, V3<T>{T(0), T(1), T(0)}) : cross(n, V3<T>{T(1), T(0), T(0)});
     t1 = normalize(c);
     t2 = cross(n, t1);
 }
@@ -815,9 +826,9 @@ template <class T> inline void cube_support_face(const Body<T>& body, V3<T> outw
     V3<T> ax[3];
     body_axes(body, ax);
     uint32_t ai = 0;
-    T best = std::fabs(dot(ax[0], outward));
+    T best = ofabs(dot(ax[0], outward));
     for (uint32_t i = 1; i < 3; ++i) {
-        T al = std::fabs(dot(ax[i], outward));
+        T al = ofabs(dot(ax[i], outward));
         if (al > best) { ai = i; best = al; }
     }
     V3<T> fa = ax[ai];
@@ -1140,8 +1151,8 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
                 T f0 = rmin(ct.penalty.x * c0 + ct.lambda.x, T(0));
                 T f1r = ct.penalty.y * c1 + ct.lambda.y;
                 T f2r = ct.penalty.z * c2 + ct.lambda.z;
-                T bounds = std::fabs(f0) * ct.friction;
-                T tm = std::sqrt(f1r * f1r + f2r * f2r);
+                T bounds = ofabs(f0) * ct.friction;
+                T tm = osqrt(f1r * f1r + f2r * f2r);
                 T f1 = f1r, f2 = f2r;
                 if (tm > bounds && tm > T(0)) {
                     T s = bounds / tm;
@@ -1233,8 +1244,8 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
             T f0 = rmin(ct.penalty.x * c0 + ct.lambda.x, T(0));
             T f1r = ct.penalty.y * c1 + ct.lambda.y;
             T f2r = ct.penalty.z * c2 + ct.lambda.z;
-            T bounds = std::fabs(f0) * ct.friction;
-            T tm = std::sqrt(f1r * f1r + f2r * f2r);
+            T bounds = ofabs(f0) * ct.friction;
+            T tm = osqrt(f1r * f1r + f2r * f2r);
             T f1 = f1r, f2 = f2r;
             if (tm > bounds && tm > T(0)) {
                 T s = bounds / tm;
@@ -1242,10 +1253,10 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
                 f2 = f2r * s;
             }
             ct.lambda = V3<T>{f0, f1, f2};
-            if (f0 < T(0)) ct.penalty.x = rmin(ct.penalty.x + pc.beta_lin * std::fabs(c0), pc.penalty_max);
+            if (f0 < T(0)) ct.penalty.x = rmin(ct.penalty.x + pc.beta_lin * ofabs(c0), pc.penalty_max);
             if (tm <= bounds) {
-                ct.penalty.y = rmin(ct.penalty.y + pc.beta_lin * std::fabs(c1), pc.penalty_max);
-                ct.penalty.z = rmin(ct.penalty.z + pc.beta_lin * std::fabs(c2), pc.penalty_max);
+                ct.penalty.y = rmin(ct.penalty.y + pc.beta_lin * ofabs(c1), pc.penalty_max);
+                ct.penalty.z = rmin(ct.penalty.z + pc.beta_lin * ofabs(c2), pc.penalty_max);
             }
         }
         double tc = now_s();
@@ -1288,6 +1299,10 @@ template <class T> struct World {
     std::vector<Pair> last_pairs;
     bool broad_primed = false;
     StepTimes times;
+    // with keep_presolve, the contacts as the last solve received them (warm
+    // λ / penalty, before the dual updates): the frozen contact set of a replay
+    bool keep_presolve = false;
+    std::vector<Contact<T>> presolve;
 
     // prepare_contacts (physics/gpu.rs:188-197)
     void prepare_contacts(std::vector<Contact<T>> body_contacts, const Options<T>& o) {
@@ -1362,6 +1377,7 @@ template <class T> struct World {
         RunOptions<T> pc{o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, o.iterations};
         std::vector<V3<T>> op;
         std::vector<Q<T>> orr;
+        if (keep_presolve) presolve = contacts;
         avbd_solve(S, contacts, constraints, offsets, indices, pc, op, orr, &times.solve);
         for (size_t i = 0; i < n; ++i) {
             Body<T>& b = bodies[i];
@@ -1371,5 +1387,125 @@ template <class T> struct World {
         }
     }
 };
+
+// ------------------------------------------------- frozen-contact replay (AD)
+// The differentiable step of the adjoint: each step's contact set (geometry
+// and warm λ / penalty, as the forward's solve received it) and CSR are
+// replayed instead of being found again; only the AVBD solve runs, with the
+// contacts' friction replaced by the parameter μ.  Instantiated with
+// Dual<5>, it gives forward-mode derivatives (the FP64 oracle gradient);
+// with Dual<5, float> the values are the float replay's, bit for bit, and
+// the derivatives (in double) are those of the piece the float run lies on.
+
+// Forward-mode dual number: value (type V) and N directional derivatives
+// (double).  Branches (min / max / clamps, the friction cone) follow the
+// value, so the result is the derivative of the piece the value lies on.
+template <int N, class V = double> struct Dual {
+    V v = V(0);
+    double d[N] = {};
+    Dual() = default;
+    Dual(double x) : v(V(x)) {}  // NOLINT: constants convert implicitly
+    explicit operator double() const { return v; }
+    explicit operator float() const { return (float)v; }
+    friend Dual operator+(const Dual& a, const Dual& b) {
+        Dual r(a.v + b.v);
+        for (int k = 0; k < N; ++k) r.d[k] = a.d[k] + b.d[k];
+        return r;
+    }
+    friend Dual operator-(const Dual& a, const Dual& b) {
+        Dual r(a.v - b.v);
+        for (int k = 0; k < N; ++k) r.d[k] = a.d[k] - b.d[k];
+        return r;
+    }
+    friend Dual operator-(const Dual& a) {
+        Dual r(-a.v);
+        for (int k = 0; k < N; ++k) r.d[k] = -a.d[k];
+        return r;
+    }
+    friend Dual operator*(const Dual& a, const Dual& b) {
+        Dual r(a.v * b.v);
+        for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+        return r;
+    }
+    friend Dual operator/(const Dual& a, const Dual& b) {
+        Dual r(a.v / b.v);
+        for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+        return r;
+    }
+    friend bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+    friend bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+    friend bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+    friend bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+    friend bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+    friend bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
+    friend Dual sqrt(const Dual& a) {
+        Dual r;
+        r.v = std::sqrt(a.v);
+        for (int k = 0; k < N; ++k) r.d[k] = r.v > V(0) ? a.d[k] / (2.0 * (double)r.v) : 0.0;
+        return r;
+    }
+    friend Dual fabs(const Dual& a) { return a.v < 0.0 ? -a : a; }
+};
+
+// One AVBD solve of `bodies` on a given contact set and CSR (World::solve
+// without joints); contacts' λ / penalty are updated in place.
+template <class T>
+void solve_bodies(std::vector<Body<T>>& bodies, std::vector<Contact<T>>& contacts,
+                  const std::vector<uint32_t>& offsets, const std::vector<uint32_t>& indices, const Options<T>& o) {
+    const size_t n = bodies.size();
+    if (n == 0) return;
+    SolveState<T> S;
+    S.vel.resize(n); S.ang.resize(n); S.prev_vel.resize(n); S.inv_moment.resize(n); S.inv_mass.resize(n);
+    S.pos[0].resize(n); S.rot[0].resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        const Body<T>& b = bodies[i];
+        S.vel[i] = b.velocity; S.ang[i] = b.angular_velocity; S.prev_vel[i] = b.prev_velocity;
+        S.inv_moment[i] = b.inv_moment; S.inv_mass[i] = b.inv_mass;
+        S.pos[0][i] = b.position; S.rot[0][i] = b.orientation;
+    }
+    S.pos[1] = S.pos[0];
+    S.rot[1] = S.rot[0];
+    RunOptions<T> pc{o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, o.iterations};
+    std::vector<V3<T>> op;
+    std::vector<Q<T>> orr;
+    const std::vector<Constraint<T>> none;
+    avbd_solve(S, contacts, none, offsets, indices, pc, op, orr);
+    for (size_t i = 0; i < n; ++i) {
+        Body<T>& b = bodies[i];
+        if (b.is_static()) continue;
+        b.position = op[i]; b.orientation = orr[i];
+        b.velocity = S.vel[i]; b.angular_velocity = S.ang[i]; b.prev_velocity = S.prev_vel[i];
+    }
+}
+
+// A recorded step of the forward: the contact set its solve received and the CSR.
+struct ReplayStep {
+    std::vector<Contact<double>> contacts;
+    std::vector<uint32_t> offsets, indices;
+};
+
+template <class T> Contact<T> contact_cast(const Contact<double>& c, T friction) {
+    auto v = [](V3<double> a) { return V3<T>{T(a.x), T(a.y), T(a.z)}; };
+    Contact<T> r;
+    r.a = c.a;
+    r.b = c.b;
+    r.friction = friction;
+    r.normal = v(c.normal); r.tangent1 = v(c.tangent1); r.tangent2 = v(c.tangent2);
+    r.r_a = v(c.r_a); r.r_b = v(c.r_b); r.c0 = v(c.c0);
+    r.penalty = v(c.penalty); r.lambda = v(c.lambda);
+    return r;
+}
+
+// Replay the recorded steps from `bodies` (their state at the first step)
+// with friction μ; the contact values are exact in float and double.
+template <class T>
+void replay(std::vector<Body<T>>& bodies, const std::vector<ReplayStep>& steps, const Options<T>& o, T friction) {
+    for (const ReplayStep& st : steps) {
+        std::vector<Contact<T>> cs;
+        cs.reserve(st.contacts.size());
+        for (const auto& c : st.contacts) cs.push_back(contact_cast<T>(c, friction));
+        solve_bodies(bodies, cs, st.offsets, st.indices, o);
+    }
+}
 
 }  // namespace oracle

@@ -5,6 +5,7 @@
 // product library.  `precision` selects the FP32 parity reference (32) or the
 // FP64 oracle (64).  Record layouts are the reference's #[repr(C)] records as
 // declared in include/diomedes_hip.h.
+#include <type_traits>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -492,3 +493,154 @@ OAPI void oracle_support(int precision, int host, const dio_gpu_body* body, cons
     }
 }
 
+
+// ------------------------------------------------- frozen-contact replay (AD)
+namespace {
+using D5 = Dual<5>;
+using D5f = Dual<5, float>;
+
+std::vector<ReplayStep> replay_steps(uint32_t n, uint32_t nsteps, const dio_avbd_contact* contacts,
+                                     const uint64_t* coff, const uint32_t* csr_off, const uint32_t* csr_idx,
+                                     const uint64_t* ioff) {
+    std::vector<ReplayStep> steps(nsteps);
+    for (uint32_t s = 0; s < nsteps; ++s) {
+        ReplayStep& st = steps[s];
+        for (uint64_t c = coff[s]; c < coff[s + 1]; ++c) st.contacts.push_back(contact_from<double>(contacts[c]));
+        st.offsets.assign(csr_off + (size_t)s * (n + 1), csr_off + (size_t)(s + 1) * (n + 1));
+        st.indices.assign(csr_idx + ioff[s], csr_idx + ioff[s + 1]);
+    }
+    return steps;
+}
+
+template <class T> void put_poses(const std::vector<Body<T>>& b, double* pos, double* rot) {
+    for (size_t i = 0; i < b.size(); ++i) {
+        put3(pos + 3 * i, b[i].position);
+        put4(rot + 4 * i, b[i].orientation);
+    }
+}
+}  // namespace
+
+// Replay nsteps recorded steps (frozen contact sets) from the given bodies /
+// state.  params (5 doubles, or null for the state's values and `friction`):
+// the initial velocity (x, y, z) and the mass m of body seed_body (inverse
+// mass 1/m, inverse moments scaled as 1/m) and the friction μ of every
+// contact.  precision 32 or 64: plain replay.  precision 65: FP64 with
+// forward-mode derivatives along those 5 parameters; dpos (n x 3 x 5) and
+// drot (n x 4 x 5) receive d(final pose)/d(parameter).  precision 33: the
+// same derivatives (in double) along the float replay's values and branches.
+OAPI void oracle_replay(int precision, const dio_gpu_body* bodies, const dio_avbd_body_state* st, uint32_t n,
+                        uint32_t nsteps, const dio_avbd_contact* contacts, const uint64_t* coff,
+                        const uint32_t* csr_off, const uint32_t* csr_idx, const uint64_t* ioff,
+                        const dio_avbd_options* o, const double* params, double friction, uint32_t seed_body,
+                        double* pos, double* rot, double* dpos, double* drot) {
+    const std::vector<ReplayStep> steps = replay_steps(n, nsteps, contacts, coff, csr_off, csr_idx, ioff);
+    double pv[5] = {0.0, 0.0, 0.0, 0.0, friction};
+    const bool has_seed = seed_body < n;
+    double c_mom[3] = {0.0, 0.0, 0.0};
+    if (has_seed) {
+        const dio_avbd_body_state& sb = st[seed_body];
+        for (int k = 0; k < 3; ++k) pv[k] = (double)sb.vel[k];
+        pv[3] = sb.inv_mass != 0.0f ? 1.0 / (double)sb.inv_mass : 0.0;
+        for (int k = 0; k < 3; ++k) c_mom[k] = sb.inv_mass != 0.0f ? (double)sb.inv_moment[k] / (double)sb.inv_mass : 0.0;
+    }
+    if (params)
+        for (int k = 0; k < 5; ++k) pv[k] = params[k];
+    auto setup = [&](auto& b, auto mk) {
+        using T = std::decay_t<decltype(mk(0, 0.0))>;
+        for (uint32_t i = 0; i < n; ++i) b[i] = body_from<T>(bodies[i], &st[i]);
+        if (has_seed && params) {
+            auto& sb = b[seed_body];
+            sb.velocity = {mk(0, pv[0]), mk(1, pv[1]), mk(2, pv[2])};
+            if (pv[3] != 0.0) {
+                const T m = mk(3, pv[3]);
+                sb.inv_mass = T(1.0) / m;
+                sb.inv_moment = {T(c_mom[0]) / m, T(c_mom[1]) / m, T(c_mom[2]) / m};
+            }
+        }
+        return mk(4, pv[4]);
+    };
+    if (precision == 32 || precision == 64) {
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            std::vector<Body<T>> b(n);
+            const T mu = setup(b, [](int, double x) { return T(x); });
+            replay(b, steps, options_from<T>(o), mu);
+            put_poses(b, pos, rot);
+        };
+        if (precision == 32) run(float{});
+        else run(double{});
+        return;
+    }
+    auto run_dual = [&](auto tag) {
+        using DT = decltype(tag);
+        std::vector<Body<DT>> b(n);
+        const DT mu = setup(b, [&](int k, double x) {
+            DT r(x);
+            if (has_seed || k == 4) r.d[k] = 1.0;
+            return r;
+        });
+        if (has_seed && !params) {
+            // seed the state's own values
+            Body<DT>& sb = b[seed_body];
+            sb.velocity.x.d[0] = 1.0;
+            sb.velocity.y.d[1] = 1.0;
+            sb.velocity.z.d[2] = 1.0;
+            if (pv[3] != 0.0) {
+                DT m(pv[3]);
+                m.d[3] = 1.0;
+                sb.inv_mass = DT(1.0) / m;
+                sb.inv_moment = {DT(c_mom[0]) / m, DT(c_mom[1]) / m, DT(c_mom[2]) / m};
+            }
+        }
+        replay(b, steps, options_from<DT>(o), mu);
+        for (uint32_t i = 0; i < n; ++i) {
+            const V3<DT>& p = b[i].position;
+            const Q<DT>& q = b[i].orientation;
+            pos[3 * i] = p.x.v; pos[3 * i + 1] = p.y.v; pos[3 * i + 2] = p.z.v;
+            rot[4 * i] = q.x.v; rot[4 * i + 1] = q.y.v; rot[4 * i + 2] = q.z.v; rot[4 * i + 3] = q.w.v;
+            for (int k = 0; k < 5; ++k) {
+                if (dpos) {
+                    dpos[(3 * i + 0) * 5 + k] = p.x.d[k];
+                    dpos[(3 * i + 1) * 5 + k] = p.y.d[k];
+                    dpos[(3 * i + 2) * 5 + k] = p.z.d[k];
+                }
+                if (drot) {
+                    drot[(4 * i + 0) * 5 + k] = q.x.d[k];
+                    drot[(4 * i + 1) * 5 + k] = q.y.d[k];
+                    drot[(4 * i + 2) * 5 + k] = q.z.d[k];
+                    drot[(4 * i + 3) * 5 + k] = q.w.d[k];
+                }
+            }
+        }
+    };
+    if (precision == 33) run_dual(D5f{});
+    else run_dual(D5{});
+}
+
+// The world's contacts as its last solve received them (see keep_presolve).
+OAPI void oracle_world_keep_presolve(void* p, int on) {
+    with_world(static_cast<WorldHandle*>(p), [&](auto& w) {
+        w.keep_presolve = on != 0;
+        return 0;
+    });
+}
+OAPI uint32_t oracle_world_presolve_count(void* p) {
+    return with_world(static_cast<WorldHandle*>(p), [&](auto& w) { return (uint32_t)w.presolve.size(); });
+}
+OAPI void oracle_world_get_presolve(void* p, dio_avbd_contact* out) {
+    with_world(static_cast<WorldHandle*>(p), [&](auto& w) {
+        for (size_t i = 0; i < w.presolve.size(); ++i) {
+            const auto& c = w.presolve[i];
+            dio_avbd_contact& r = out[i];
+            std::memset(&r, 0, sizeof(r));
+            r.a = c.a;
+            r.b = c.b;
+            r.friction = float(c.friction);
+            auto put = [](float* d, const auto& v) { d[0] = float(v.x); d[1] = float(v.y); d[2] = float(v.z); };
+            put(r.normal, c.normal); put(r.tangent1, c.tangent1); put(r.tangent2, c.tangent2);
+            put(r.r_a, c.r_a); put(r.r_b, c.r_b); put(r.c0, c.c0);
+            put(r.penalty, c.penalty); put(r.lambda, c.lambda);
+        }
+        return 0;
+    });
+}

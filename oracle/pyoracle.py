@@ -27,6 +27,9 @@ _U = POINTER(c_uint32)
 _F = POINTER(c_float)
 
 
+_U64 = POINTER(ctypes.c_uint64)
+
+
 def load() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
@@ -66,6 +69,11 @@ def load() -> ctypes.CDLL:
         "oracle_qrot": (None, [c_int, _D, _D, _D]),
         "oracle_qmul": (None, [c_int, _D, _D, _D]),
         "oracle_support": (None, [c_int, c_int, c_void_p, _D, _D]),
+        "oracle_replay": (None, [c_int, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, _U64, _U, _U, _U64,
+                                 POINTER(R.Options), _D, c_double, c_uint32, _D, _D, _D, _D]),
+        "oracle_world_keep_presolve": (None, [c_void_p, c_int]),
+        "oracle_world_presolve_count": (c_uint32, [c_void_p]),
+        "oracle_world_get_presolve": (None, [c_void_p, c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -170,8 +178,85 @@ class OracleWorld:
     def reset_times(self) -> None:
         load().oracle_world_reset_times(self._h)
 
+    def keep_presolve(self, on: bool = True) -> None:
+        load().oracle_world_keep_presolve(self._h, 1 if on else 0)
 
-# ---------------------------------------------------------------- unit level
+    def presolve_contacts(self) -> np.ndarray:
+        """The contact records (AVBD_CONTACT) the last solve received."""
+        lib = load()
+        m = lib.oracle_world_presolve_count(self._h)
+        out = np.zeros(m, R.AVBD_CONTACT)
+        lib.oracle_world_get_presolve(self._h, _p(out))
+        return out
+
+
+# ------------------------------------------------- frozen-contact replay
+class Recording:
+    """Per-step frozen contact sets (AVBD_CONTACT records as the solve received
+    them) and CSRs of a forward run, packed for ``replay``."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.contacts, self.offsets, self.indices = [], [], []
+
+    def add(self, contacts: np.ndarray, offsets: np.ndarray, indices: np.ndarray) -> None:
+        self.contacts.append(np.ascontiguousarray(contacts, R.AVBD_CONTACT))
+        self.offsets.append(np.ascontiguousarray(offsets, np.uint32))
+        self.indices.append(np.ascontiguousarray(indices, np.uint32))
+
+    @property
+    def steps(self) -> int:
+        return len(self.contacts)
+
+    def packed(self):
+        coff = np.zeros(self.steps + 1, np.uint64)
+        coff[1:] = np.cumsum([len(c) for c in self.contacts])
+        ioff = np.zeros(self.steps + 1, np.uint64)
+        ioff[1:] = np.cumsum([len(i) for i in self.indices])
+        cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
+        return (cat(self.contacts, R.AVBD_CONTACT), coff, cat(self.offsets, np.uint32), cat(self.indices, np.uint32),
+                ioff)
+
+
+def record_world(scene, nsteps: int, precision: int = 32, lag: int = 1) -> Recording:
+    """Step an oracle world and record each step's frozen contact set."""
+    w = OracleWorld(scene, precision)
+    w.keep_presolve(True)
+    rec = Recording(scene.n)
+    for _ in range(nsteps):
+        w.step(nsteps=1, lag=lag)
+        off, idx = w.csr()
+        rec.add(w.presolve_contacts(), off, idx)
+    return rec
+
+
+def replay(bodies, state, rec: Recording, options, friction: float, precision: int = 64, seed_body: int = -1,
+           params=None):
+    """Replay a recording from (bodies, state).  params: (v0 x, y, z, mass) of
+    seed_body and the friction, in float64, overriding the state (None: the
+    state's values and `friction`).  precision 32 / 64: final positions
+    (n x 3) and orientations (n x 4).  precision 65: also their derivatives
+    (n x 3 x 5, n x 4 x 5) with respect to those five parameters."""
+    lib = load()
+    n = rec.n
+    bodies = np.ascontiguousarray(bodies, R.GPU_BODY)
+    state = np.ascontiguousarray(state, R.BODY_STATE)
+    con, coff, csr_off, csr_idx, ioff = rec.packed()
+    pos = np.zeros((n, 3))
+    rot = np.zeros((n, 4))
+    dual = precision in (33, 65)
+    dpos = np.zeros((n, 3, 5)) if dual else None
+    drot = np.zeros((n, 4, 5)) if dual else None
+    pv = None if params is None else np.ascontiguousarray(params, np.float64)
+    o = options.to_c()
+    sb = n if seed_body is None or seed_body < 0 else seed_body
+    lib.oracle_replay(precision, _p(bodies), _p(state), n, rec.steps, _p(con), _p(coff, ctypes.c_uint64),
+                      _p(csr_off, c_uint32), _p(csr_idx, c_uint32), _p(ioff, ctypes.c_uint64), ctypes.byref(o),
+                      _p(pv, c_double), float(friction), sb, _p(pos, c_double), _p(rot, c_double),
+                      _p(dpos, c_double), _p(drot, c_double))
+    return (pos, rot, dpos, drot) if dual else (pos, rot)
+
+
 def aabbs(bodies: np.ndarray, hull=None, precision: int = 32) -> np.ndarray:
     bodies = np.ascontiguousarray(bodies, R.GPU_BODY)
     hull = np.zeros((0, 3), np.float32) if hull is None else np.ascontiguousarray(hull, np.float32)
