@@ -175,6 +175,12 @@ __device__ __forceinline__ DualState dual_update(const AvbdParams& p, float kk, 
 // recover launch.  A contact between two static bodies is read by no primal;
 // the final launch applies its I updates at once (its poses do not move).
 __device__ __forceinline__ float4* lam_buf(const SolveView& v, uint32_t j) { return (j & 1u) ? v.c_lam_alt : v.c_lam; }
+// the body of work item k: k itself, or entry k of a colour's body list (p.n
+// past its end: no body)
+__device__ __forceinline__ uint32_t body_at(const SolveView& v, const AvbdParams& p, uint32_t k) {
+    if (!v.body_list) return k;
+    return k < v.list_count ? v.body_list[k] : p.n;
+}
 __device__ __forceinline__ float4* pen_buf(const SolveView& v, uint32_t j) { return (j & 1u) ? v.c_pen_alt : v.c_pen; }
 
 __global__ void __launch_bounds__(kBlock) k_avbd_init(SolveView v, AvbdParams p) {
@@ -222,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
     const float4* __restrict__ pen_in = pen_buf(v, fuse ? iter - 1 : 0);
     float4* lam_out = lam_buf(v, iter);
     float4* pen_out = pen_buf(v, iter);
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = body_at(v, p, blockIdx.x * kBlock + threadIdx.x);
     if (i >= p.n) return;
     const float4 imm = v.inv_mm[i];
     if (imm.w == 0.0f) return;
@@ -392,7 +398,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     __shared__ uint4 body_u[kBodies];        // CSR start / count of contacts, then of joints
     const int slot = threadIdx.x / G;
     const int lane = threadIdx.x % G;
-    const uint32_t i = blockIdx.x * kBodies + slot;
+    const uint32_t i = body_at(v, p, blockIdx.x * kBodies + slot);
     const float4* __restrict__ rp = v.pos[parity];
     const float4* __restrict__ rr = v.rot[parity];
     const float dt = p.dt;
@@ -401,7 +407,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     // snapshot, the item ranges and the initial block, shared through LDS
     if (threadIdx.x < (unsigned)kBodies) {
         const int t = threadIdx.x;
-        const uint32_t bi = blockIdx.x * kBodies + t;
+        const uint32_t bi = body_at(v, p, blockIdx.x * kBodies + t);
         bool act = bi < p.n;
         float4 imm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (act) {
@@ -613,7 +619,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     const int t = threadIdx.x;
     const float4 f0 = body_f[t][0];
     if (f0.w == 0.0f) return;
-    const uint32_t bi = blockIdx.x * kBodies + t;
+    const uint32_t bi = body_at(v, p, blockIdx.x * kBodies + t);
     const float* gs = gathered[t];
     Block6 B;
 #pragma unroll
@@ -745,30 +751,30 @@ void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t s, co
 namespace {
 template <bool FUSED>
 void launch_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, hipStream_t s, int lanes,
-                   const LaunchEvents& ev) {
+                   const LaunchEvents& ev, uint32_t work) {
     switch (lanes) {
         case 4:
-            launch_k(k_avbd_primal_grouped<4, FUSED>, dim3((p.n + kBlock / 4 - 1) / (kBlock / 4)), dim3(kBlock), s, ev,
+            launch_k(k_avbd_primal_grouped<4, FUSED>, dim3((work + kBlock / 4 - 1) / (kBlock / 4)), dim3(kBlock), s, ev,
                      v, p, iter);
             break;
         case 8:
-            launch_k(k_avbd_primal_grouped<8, FUSED>, dim3((p.n + kBlock / 8 - 1) / (kBlock / 8)), dim3(kBlock), s, ev,
+            launch_k(k_avbd_primal_grouped<8, FUSED>, dim3((work + kBlock / 8 - 1) / (kBlock / 8)), dim3(kBlock), s, ev,
                      v, p, iter);
             break;
         case 16:
-            launch_k(k_avbd_primal_grouped<16, FUSED>, dim3((p.n + kBlock / 16 - 1) / (kBlock / 16)), dim3(kBlock), s,
+            launch_k(k_avbd_primal_grouped<16, FUSED>, dim3((work + kBlock / 16 - 1) / (kBlock / 16)), dim3(kBlock), s,
                      ev, v, p, iter);
             break;
         case 32:
-            launch_k(k_avbd_primal_grouped<32, FUSED>, dim3((p.n + kBlock / 32 - 1) / (kBlock / 32)), dim3(kBlock), s,
+            launch_k(k_avbd_primal_grouped<32, FUSED>, dim3((work + kBlock / 32 - 1) / (kBlock / 32)), dim3(kBlock), s,
                      ev, v, p, iter);
             break;
         case 64:
-            launch_k(k_avbd_primal_grouped<64, FUSED>, dim3((p.n + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), s,
+            launch_k(k_avbd_primal_grouped<64, FUSED>, dim3((work + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), s,
                      ev, v, p, iter);
             break;
         default:
-            launch_k(k_avbd_primal<FUSED>, dim3(blocks_for(p.n)), dim3(kBlock), s, ev, v, p, iter);
+            launch_k(k_avbd_primal<FUSED>, dim3(blocks_for(work)), dim3(kBlock), s, ev, v, p, iter);
     }
 }
 }  // namespace
@@ -776,8 +782,13 @@ void launch_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, hipSt
 void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, bool fused, hipStream_t s, int lanes,
                         const LaunchEvents& ev) {
     if (p.n == 0) return;
-    if (fused && iter > 0) launch_primal<true>(v, p, iter, s, lanes, ev);
-    else launch_primal<false>(v, p, iter, s, lanes, ev);
+    if (fused && iter > 0) launch_primal<true>(v, p, iter, s, lanes, ev, p.n);
+    else launch_primal<false>(v, p, iter, s, lanes, ev, p.n);
+}
+void launch_avbd_primal_colour(const SolveView& v, const AvbdParams& p, uint32_t iter, uint32_t count, hipStream_t s,
+                               int lanes, const LaunchEvents& ev) {
+    if (count == 0) return;
+    launch_primal<false>(v, p, iter, s, lanes, ev, count);
 }
 static uint32_t contact_blocks(uint32_t work) {
     uint32_t blocks = blocks_for(work == 0 ? 1 : work);

@@ -28,6 +28,7 @@
 #include "../../include/diomedes_hip.h"
 #include "adjoint.hpp"
 #include "broad.hpp"
+#include "colour.hpp"
 #include "contacts.hpp"
 #include "dio_device.hpp"
 #include "narrow.hpp"
@@ -280,6 +281,11 @@ struct dio_ctx {
     AdjWork adj_work{};
     void* adj_rec_base = nullptr;
     bool adj_on = false;
+    // coloured Gauss-Seidel mode: colouring buffers, a pinned read-back area
+    // and the last step's colouring summary
+    ColourBuffers wc{};
+    uint32_t* wc_host = nullptr;  // kColourRoundSlots + kMaxColours + 2 words
+    uint32_t last_colours = 0, last_rounds = 0;
     std::vector<uint8_t> adj_sweep_key;  // what w.ck was swept from (empty: nothing)
     void free_adjoint() {
         if (adj_rec_base) (void)hipFree(adj_rec_base);
@@ -492,6 +498,36 @@ void schedule_avbd(const SolveView& v, const AvbdParams& p, uint32_t iters, uint
             run(2, [&](const LaunchEvents& ev) { launch_avbd_dual(v, p, k & 1u, 0u, 0u, m_grid, s, ev); });
     }
     run(3, [&](const LaunchEvents& ev) { launch_avbd_dual_recover(v, p, iters, fused, m_grid, s, ev); });
+}
+
+// The coloured Gauss-Seidel solve (no counterpart in the reference): init;
+// per iteration one primal launch per colour over that colour's body list,
+// poses updated in place (v.pos[0] == v.pos[1]), then the dual update; the
+// last dual update with recover.  start[c] .. start[c + 1] index the list.
+void schedule_avbd_coloured(SolveView v, const AvbdParams& p, uint32_t iters, uint32_t m_grid, hipStream_t s,
+                            KernelTimer* kt, const dio_tuning& t, const uint32_t* list,
+                            const std::vector<uint32_t>& start) {
+    const int lanes = (int)t.primal_lanes;
+    auto run = [&](int kind, auto&& launch) {
+        if (kt) kt->time(kind, launch);
+        else launch(LaunchEvents{});
+    };
+    v.pos[1] = v.pos[0];
+    v.rot[1] = v.rot[0];
+    run(0, [&](const LaunchEvents& ev) { launch_avbd_init(v, p, s, ev); });
+    for (uint32_t k = 0; k < iters; ++k) {
+        for (size_t c = 0; c + 1 < start.size(); ++c) {
+            const uint32_t count = start[c + 1] - start[c];
+            if (count == 0) continue;
+            SolveView vc = v;
+            vc.body_list = list + start[c];
+            vc.list_count = count;
+            run(1, [&](const LaunchEvents& ev) { launch_avbd_primal_colour(vc, p, k, count, s, lanes, ev); });
+        }
+        if (k + 1 < iters)
+            run(2, [&](const LaunchEvents& ev) { launch_avbd_dual(v, p, k & 1u, 0u, 0u, m_grid, s, ev); });
+    }
+    run(3, [&](const LaunchEvents& ev) { launch_avbd_dual_recover(v, p, iters, false, m_grid, s, ev); });
 }
 
 __global__ void k_stats_accum(const uint32_t* pairs, const uint32_t* m, const uint32_t* nnz, unsigned long long* acc) {
@@ -906,6 +942,8 @@ void ensure_world(dio_ctx* c) {
     c->w_stage_b = A.alloc<dio_gpu_body>(n);
     c->w_stage_s = A.alloc<dio_avbd_body_state>(n);
     c->w_stage_c = A.alloc<dio_avbd_constraint>(c->c_cap);
+    colour_buffers_bind(c->wc, A.alloc<uint32_t>(colour_buffer_words(n)), n);
+    c->wc_host = c->pin.alloc<uint32_t>(kColourRoundSlots + kMaxColours + 2);
     c->w_ready = true;
 }
 
@@ -963,6 +1001,61 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         hipLaunchKernelGGL(k_stats_accum, dim3(1), dim3(64), 0, st, (const uint32_t*)c->pend[bn_pend].p_dev,
                            (const uint32_t*)c->cs[s].m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
     }
+}
+
+// The AVBD segment of a coloured step: colour the step's body graph, read
+// back the colour counts (the one host synchronisation of the mode), then
+// launch the solve colour by colour.
+void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const dio_step_opts& so, KernelTimer* kt) {
+    hipStream_t st = c->stream;
+    const uint32_t n = c->w_n;
+    ColourGraph g;
+    g.n = n;
+    g.inv_mm = c->w_inv_mm;
+    g.off = c->csr.off;
+    g.peer = c->csr.peer;
+    if (c->w_j.c > 0) {
+        g.j_off = c->w_j.off;
+        g.j_idx = c->w_j.idx;
+        g.j_abk = c->w_j.abk;
+    }
+    if (c->w_scene_bits) {
+        g.scene_of = c->w_scene_of;
+        g.scene_off = c->w_scene_off;
+    }
+    launch_colour_init(g, c->wc, st);
+    uint32_t r = 0, batch = 24;
+    uint32_t* h = c->wc_host;
+    for (;;) {
+        if (r + batch > kColourRoundSlots - 1) capacity("the colouring needs more rounds than it can track", r);
+        launch_colour_rounds(g, c->wc, r, batch, st);
+        DIO_HIP_CHECK(hipMemcpyAsync(h, c->wc.pending + r, (size_t)batch * 4, hipMemcpyDeviceToHost, st));
+        DIO_HIP_CHECK(hipStreamSynchronize(st));
+        uint32_t k = 0;
+        while (k < batch && h[k] != 0u) ++k;
+        r += k;
+        if (k < batch) break;
+        batch = 16;
+    }
+    c->last_rounds = n > 0 ? r + 1 : 0;
+    const int lb = launch_colour_lists(g, c->wc, st);
+    DIO_HIP_CHECK(hipMemcpyAsync(h, c->wc.counts, (kMaxColours + 1) * 4, hipMemcpyDeviceToHost, st));
+    DIO_HIP_CHECK(hipMemcpyAsync(h + kMaxColours + 1, c->wc.pending + kColourRoundSlots - 1, 4,
+                                 hipMemcpyDeviceToHost, st));
+    DIO_HIP_CHECK(hipStreamSynchronize(st));
+    if (h[kMaxColours + 1]) capacity("a body needs more than 254 colours", kMaxColours);
+    uint32_t ncol = 0;
+    for (uint32_t k = 0; k < kMaxColours; ++k)
+        if (h[k]) ncol = k + 1;
+    std::vector<uint32_t> start(ncol + 1, 0);
+    for (uint32_t k = 0; k < ncol; ++k) start[k + 1] = start[k] + h[k];
+    c->last_colours = ncol;
+    SolveView v = world_solve_view(c, s);
+    const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
+    schedule_avbd_coloured(v, p, o.iterations, c->m_cap, st, kt, c->tuning, c->wc.vals[lb], start);
+    const int bn_pend = so.contact_lag ? 1 - s : s;
+    hipLaunchKernelGGL(k_stats_accum, dim3(1), dim3(64), 0, st, (const uint32_t*)c->pend[bn_pend].p_dev,
+                       (const uint32_t*)c->cs[s].m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
 }
 
 SolveView world_solve_view(dio_ctx* c, int s) {
@@ -1163,7 +1256,9 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         if (options->iterations == 0) invalid("iterations must be at least 1");
         dio_step_opts so{1, 0, 0};
         if (opts) so = *opts;
-        if (so.mode != 0) invalid("only the Jacobi mode (0) is available");
+        if (so.mode > 1) invalid("mode must be 0 (Jacobi) or 1 (coloured Gauss-Seidel)");
+        const bool coloured = so.mode == 1;
+        if (coloured && ctx->adj_on) invalid("the adjoint records Jacobi steps (mode 0) only");
         ctx->last_lag = so.contact_lag ? 1 : 0;
         hipStream_t s = ctx->stream;
         DIO_HIP_CHECK(hipSetDevice(ctx->device));
@@ -1193,13 +1288,15 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
                 const auto a = std::chrono::steady_clock::now();
                 DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[resort ? 2 : 0][par], s));
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
-                DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[1][par], s));
+                if (coloured) run_coloured_segment(ctx, par, *options, so, nullptr);
+                else DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[1][par], s));
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
                 h_launch += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
             } else {
                 enqueue_segment(ctx, 0, par, *options, so, nullptr, resort);
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
-                enqueue_segment(ctx, 1, par, *options, so, so.profile ? &kt : nullptr);
+                if (coloured) run_coloured_segment(ctx, par, *options, so, so.profile ? &kt : nullptr);
+                else enqueue_segment(ctx, 1, par, *options, so, so.profile ? &kt : nullptr);
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
             }
             ctx->w_step += 1;
@@ -1219,6 +1316,8 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         std::memset(&st, 0, sizeof(st));
         st.steps = nsteps;
         st.dynamic_bodies = ctx->w_n_dyn;
+        st.colours = coloured ? ctx->last_colours : 0;
+        st.colour_rounds = coloured ? ctx->last_rounds : 0;
         if (ctx->w_n > 0 && nsteps > 0) {
             const int par = (int)((ctx->w_step - 1) % 2);
             const int lag = so.contact_lag ? 1 : 0;
@@ -1459,6 +1558,19 @@ dio_status dio_world_contacts(dio_ctx* ctx, dio_avbd_contact* out, uint32_t cap,
                 dst[k][3] = 0.0f;
             }
         }
+        return DIO_OK;
+    });
+}
+
+dio_status dio_world_colours(dio_ctx* ctx, uint32_t* colour, uint32_t cap, uint32_t* num_colours) {
+    return guarded(ctx, [&]() {
+        if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
+        if (num_colours) *num_colours = ctx->last_colours;
+        if (!colour) return DIO_OK;
+        if (cap < ctx->w_n) capacity("colour buffer too small", ctx->w_n);
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->w_n > 0)
+            DIO_HIP_CHECK(hipMemcpy(colour, ctx->wc.colour, (size_t)ctx->w_n * 4, hipMemcpyDeviceToHost));
         return DIO_OK;
     });
 }

@@ -64,6 +64,10 @@ struct SolveView {
     // fused path: recover writes the final pose back into the world pose
     float4* out_pos;
     float4* out_rot;
+    // coloured Gauss-Seidel: the primal runs over body_list[0 .. list_count)
+    // (one colour) instead of every body; pos[0] / pos[1] then alias
+    const uint32_t* body_list;
+    uint32_t list_count;
 };
 
 // Optional timing events of one launch: recorded by the dispatch itself at the
@@ -83,6 +87,10 @@ void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t strea
 // lanes: lanes per body (1 = one lane per body; 4, 8, 16, 32 = grouped kernel)
 void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, bool fused, hipStream_t stream,
                         int lanes = 32, const LaunchEvents& ev = {});
+// The primal over one colour's bodies (v.body_list, count entries): unfused,
+// poses read and written in place (v.pos[0] == v.pos[1]).
+void launch_avbd_primal_colour(const SolveView& v, const AvbdParams& p, uint32_t iter, uint32_t count,
+                               hipStream_t s, int lanes = 32, const LaunchEvents& ev = LaunchEvents{});
 void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t src, uint32_t dst,
                       uint32_t m_grid, hipStream_t stream, const LaunchEvents& ev = {});
 void launch_avbd_dual_recover(const SolveView& v, const AvbdParams& p, uint32_t iters, bool fused, uint32_t m_grid,

@@ -531,8 +531,10 @@ class PhysicsContext:
         out["friction"] = mu.value
         return out
 
-    def step(self, options: AvbdOptions, nsteps: int = 1, lag: int = 1, profile: bool = False) -> None:
-        so = R.StepOpts(1 if lag else 0, 0, 1 if profile else 0)
+    def step(self, options: AvbdOptions, nsteps: int = 1, lag: int = 1, profile: bool = False,
+             mode: int = 0) -> None:
+        """dio_step: nsteps frames; mode 0 Jacobi (the reference), 1 coloured Gauss-Seidel."""
+        so = R.StepOpts(1 if lag else 0, int(mode), 1 if profile else 0)
         o = options.to_c()
         self._check(self._lib.dio_step(self._h, ctypes.byref(o), ctypes.byref(so), nsteps))
 
@@ -602,6 +604,14 @@ class PhysicsContext:
         out = np.zeros(m.value, R.AVBD_CONTACT)
         self._check(self._lib.dio_world_contacts(self._h, _lib.ptr(out), m.value, ctypes.byref(m)))
         return out
+
+    def colours(self) -> tuple:
+        """The last coloured step's colour per body (0xFFFFFFFE: static) and
+        the number of colours."""
+        out = np.zeros(self._n, np.uint32)
+        nc = ctypes.c_uint32()
+        self._check(self._lib.dio_world_colours(self._h, _lib.ptr(out, ctypes.c_uint32), self._n, ctypes.byref(nc)))
+        return out, nc.value
 
     def pairs(self) -> np.ndarray:
         n = ctypes.c_uint32()

@@ -164,10 +164,12 @@ class StepStats(ctypes.Structure):
         ("n_primal", ctypes.c_uint32),
         ("n_dual", ctypes.c_uint32),
         ("n_recover", ctypes.c_uint32),
+        ("colours", ctypes.c_uint32),
+        ("colour_rounds", ctypes.c_uint32),
     ]
 
 
-assert ctypes.sizeof(StepStats) == 88
+assert ctypes.sizeof(StepStats) == 96
 
 
 class Tuning(ctypes.Structure):

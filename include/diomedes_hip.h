@@ -179,7 +179,12 @@ typedef struct dio_limits {
 typedef struct dio_step_opts {
     uint32_t contact_lag; /* 1: the reference demos' protocol (contacts found on the
                              previous state, lever arms on the current one); 0: synchronous */
-    uint32_t mode;        /* 0: Jacobi sweep with ping-pong pose buffers (reference) */
+    uint32_t mode;        /* 0: Jacobi sweep with ping-pong pose buffers (reference);
+                             1: coloured Gauss-Seidel (no counterpart in the reference): the
+                             step's body graph (contacts and joints between dynamic bodies) is
+                             coloured, and each iteration runs the primal colour by colour on
+                             one pose buffer updated in place, then the dual update.  The step
+                             reads the colour counts back once (not graph-captured). */
     uint32_t profile;     /* 1: launch eagerly and time every AVBD kernel with HIP events */
 } dio_step_opts;
 
@@ -202,6 +207,8 @@ typedef struct dio_step_stats {
        n_recover the final dual + recover launch. */
     float ms_init, ms_primal, ms_dual, ms_recover;
     uint32_t n_init, n_primal, n_dual, n_recover;
+    uint32_t colours;          /* mode 1: colours and colouring rounds of the last step */
+    uint32_t colour_rounds;
 } dio_step_stats;
 
 /* Kernel launch shapes (the autotuner's search space). */
@@ -361,6 +368,10 @@ dio_status dio_world_download(dio_ctx* ctx, float* positions, float* orientation
 /* Contacts of the last step with their solved lambda / penalty (the reference's
    AvbdSolver::contacts() after sync_bodies_from_gpu). */
 dio_status dio_world_contacts(dio_ctx* ctx, dio_avbd_contact* out, uint32_t cap, uint32_t* m);
+/* The last coloured step's colour per body (0xFFFFFFFE: static, not in the
+   graph; cap >= n) and the number of colours.  No counterpart in the
+   reference. */
+dio_status dio_world_colours(dio_ctx* ctx, uint32_t* colour, uint32_t cap, uint32_t* num_colours);
 /* Candidate pairs of the last step's broad phase (sorted, a < b). */
 dio_status dio_world_pairs(dio_ctx* ctx, dio_gpu_pair* out, uint32_t cap, uint32_t* n);
 

@@ -1073,13 +1073,133 @@ template <class T> void avbd_init_only(SolveState<T>& S, const RunOptions<T>& pc
     }
 }
 
+// ------------------------------------------------------ graph colouring
+// No counterpart in the reference (its solve is Jacobi).  Vertices are the
+// dynamic bodies; two are adjacent when they share a contact or a joint.
+// Jones-Plassmann rounds with hashed priorities: in round r every uncoloured
+// body whose priority beats that of each uncoloured neighbour takes the
+// smallest colour no neighbour coloured in an earlier round holds.  The
+// decisions of a round read only the state before it, so the colouring does
+// not depend on the order of evaluation.  Colours are < kMaxColours (255).
+constexpr uint32_t kUncoloured = 0xFFFFFFFFu, kNotInGraph = 0xFFFFFFFEu, kMaxColours = 255;
+
+// 32-bit integer hash (C. Wellons' "lowbias32" finaliser constants)
+inline uint32_t colour_priority(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+inline bool colour_beats(uint32_t j, uint32_t i) {
+    const uint32_t pj = colour_priority(j), pi = colour_priority(i);
+    return pj > pi || (pj == pi && j > i);
+}
+
+struct ColourPlan {
+    std::vector<uint32_t> colour;  // per body (kNotInGraph for static bodies)
+    std::vector<uint32_t> order;   // dynamic bodies by colour, ascending index within a colour
+    std::vector<uint32_t> start;   // colours + 1 offsets into order
+    uint32_t rounds = 0;
+    bool overflow = false;         // a body found all kMaxColours colours taken
+};
+
+// neighbours(i, f): calls f(j) for every dynamic neighbour j of dynamic body i
+template <class Neighbours>
+ColourPlan colour_graph(uint32_t n, const std::vector<bool>& dynamic, Neighbours&& neighbours) {
+    ColourPlan P;
+    P.colour.assign(n, kNotInGraph);
+    std::vector<uint32_t> round_of(n, kNotInGraph);
+    uint32_t left = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (dynamic[i]) { round_of[i] = kUncoloured; ++left; }
+    std::vector<std::pair<uint32_t, uint32_t>> taken;
+    for (uint32_t r = 0; left > 0; ++r) {
+        taken.clear();
+        for (uint32_t i = 0; i < n; ++i) {
+            if (round_of[i] != kUncoloured) continue;
+            uint32_t mask[8] = {0, 0, 0, 0, 0, 0, 0, 0x80000000u};  // colour 255 is never free
+            bool best = true;
+            neighbours(i, [&](uint32_t j) {
+                if (!best) return;
+                if (round_of[j] < r) mask[P.colour[j] >> 5] |= 1u << (P.colour[j] & 31);
+                else if (colour_beats(j, i)) best = false;
+            });
+            if (!best) continue;
+            uint32_t c = kMaxColours - 1;
+            bool found = false;
+            for (uint32_t w = 0; w < 8 && !found; ++w)
+                if (mask[w] != 0xFFFFFFFFu) {
+                    uint32_t b = 0;
+                    while (mask[w] & (1u << b)) ++b;
+                    c = 32 * w + b;
+                    found = true;
+                }
+            if (!found) P.overflow = true;
+            taken.push_back({i, c});
+        }
+        for (const auto& t : taken) {
+            P.colour[t.first] = t.second;
+            round_of[t.first] = r;
+        }
+        left -= (uint32_t)taken.size();
+        P.rounds = r + 1;
+    }
+    uint32_t ncol = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (dynamic[i]) ncol = std::max(ncol, P.colour[i] + 1);
+    P.start.assign(ncol + 1, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        if (dynamic[i]) P.start[P.colour[i] + 1] += 1;
+    for (uint32_t c = 0; c < ncol; ++c) P.start[c + 1] += P.start[c];
+    P.order.assign(P.start[ncol], 0);
+    std::vector<uint32_t> fill(P.start.begin(), P.start.end() - 1);
+    for (uint32_t i = 0; i < n; ++i)
+        if (dynamic[i]) P.order[fill[P.colour[i]]++] = i;
+    return P;
+}
+
+// The solve's graph: contacts through the CSR, joints through their bodies.
+template <class T>
+ColourPlan colour_bodies(const std::vector<T>& inv_mass, const std::vector<Contact<T>>& contacts,
+                         const std::vector<Constraint<T>>& cons, const std::vector<uint32_t>& offsets,
+                         const std::vector<uint32_t>& indices) {
+    const uint32_t n = (uint32_t)inv_mass.size();
+    std::vector<bool> dyn(n);
+    for (uint32_t i = 0; i < n; ++i) dyn[i] = inv_mass[i] != T(0);
+    std::vector<std::vector<uint32_t>> jn(n);
+    for (const auto& c : cons)
+        if (c.a < n && c.b < n && c.a != c.b) {
+            jn[c.a].push_back(c.b);
+            jn[c.b].push_back(c.a);
+        }
+    return colour_graph(n, dyn, [&](uint32_t i, auto&& f) {
+        for (uint32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+            const Contact<T>& ct = contacts[indices[e]];
+            const uint32_t o = ct.a == i ? ct.b : ct.a;
+            if (o != CONTAINER && o != i && dyn[o]) f(o);
+        }
+        for (uint32_t o : jn[i])
+            if (dyn[o]) f(o);
+    });
+}
+
 // gpu_physics.rs:439-622 + avbd/src/lib.rs:202-699.  Inputs: per-body state,
 // poses, contacts (λ / penalty updated in place), joints, CSR.  Outputs: the
 // pose of the final parity buffer and the recovered velocities.
+//
+// With a colour plan the primal runs as coloured Gauss-Seidel instead (no
+// counterpart in the reference): one pose buffer, updated in place, colour
+// after colour; a body reads the poses its neighbours of earlier colours
+// wrote in the same iteration.  Bodies of one colour share no contact or
+// joint, so within a colour the order does not matter.  Each iteration's
+// dual update follows its last colour, and recover reads the same buffer.
 template <class T>
 void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::vector<Constraint<T>>& cons,
                 const std::vector<uint32_t>& offsets, const std::vector<uint32_t>& indices, const RunOptions<T>& pc,
-                std::vector<V3<T>>& out_pos, std::vector<Q<T>>& out_rot, SolveTimes* times = nullptr) {
+                std::vector<V3<T>>& out_pos, std::vector<Q<T>>& out_rot, SolveTimes* times = nullptr,
+                const ColourPlan* plan = nullptr) {
     const int64_t n = (int64_t)S.inv_mass.size();
     const int64_t m = (int64_t)contacts.size();
     out_pos.assign(n, V3<T>{});
@@ -1100,16 +1220,15 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
     }
     const V3<T> zero{T(0), T(0), T(0)};
     for (uint32_t k = 0; k < pc.iterations; ++k) {
-        const uint32_t parity = k % 2;
+        const uint32_t parity = plan ? 0u : k % 2;
         std::vector<V3<T>>& rp = S.pos[parity];
         std::vector<Q<T>>& rr = S.rot[parity];
-        std::vector<V3<T>>& wpv = S.pos[1 - parity];
-        std::vector<Q<T>>& wrv = S.rot[1 - parity];
+        std::vector<V3<T>>& wpv = S.pos[plan ? 0u : 1 - parity];
+        std::vector<Q<T>>& wrv = S.rot[plan ? 0u : 1 - parity];
         double ta = now_s();
         // avbd_primal (avbd/src/lib.rs:258-551)
-#pragma omp parallel for schedule(dynamic, 64)
-        for (int64_t i = 0; i < n; ++i) {
-            if (S.inv_mass[i] == T(0)) continue;
+        auto primal_body = [&](int64_t i) {
+            if (S.inv_mass[i] == T(0)) return;
             V3<T> snap_pos = rp[i];
             Q<T> snap_rot = rr[i];
             T mass = T(1) / S.inv_mass[i];
@@ -1216,6 +1335,16 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
             solve6(B.lin, B.ang, B.cr, bl, ba, x);
             wpv[i] = snap_pos + V3<T>{x[0], x[1], x[2]};
             wrv[i] = quat_step(snap_rot, V3<T>{x[3], x[4], x[5]});
+        };
+        if (plan) {
+            for (size_t c = 0; c + 1 < plan->start.size(); ++c) {
+                const int64_t b0 = plan->start[c], b1 = plan->start[c + 1];
+#pragma omp parallel for schedule(dynamic, 64)
+                for (int64_t t = b0; t < b1; ++t) primal_body((int64_t)plan->order[t]);
+            }
+        } else {
+#pragma omp parallel for schedule(dynamic, 64)
+            for (int64_t i = 0; i < n; ++i) primal_body(i);
         }
         double tb = now_s();
         // avbd_dual (avbd/src/lib.rs:565-663): reads the just-written buffer
@@ -1266,8 +1395,8 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
     // avbd_recover (avbd/src/lib.rs:667-699) with parity (I-1)%2: reads the
     // buffer the last primal wrote
     const uint32_t fp = (pc.iterations - 1) % 2;
-    std::vector<V3<T>>& fpos = S.pos[1 - fp];
-    std::vector<Q<T>>& frot = S.rot[1 - fp];
+    std::vector<V3<T>>& fpos = S.pos[plan ? 0u : 1 - fp];
+    std::vector<Q<T>>& frot = S.rot[plan ? 0u : 1 - fp];
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         if (S.inv_mass[i] != T(0)) {
@@ -1303,6 +1432,9 @@ template <class T> struct World {
     // λ / penalty, before the dual updates): the frozen contact set of a replay
     bool keep_presolve = false;
     std::vector<Contact<T>> presolve;
+    // 0: Jacobi (the reference); 1: coloured Gauss-Seidel (colour_bodies)
+    int mode = 0;
+    ColourPlan plan;
 
     // prepare_contacts (physics/gpu.rs:188-197)
     void prepare_contacts(std::vector<Contact<T>> body_contacts, const Options<T>& o) {
@@ -1378,7 +1510,8 @@ template <class T> struct World {
         std::vector<V3<T>> op;
         std::vector<Q<T>> orr;
         if (keep_presolve) presolve = contacts;
-        avbd_solve(S, contacts, constraints, offsets, indices, pc, op, orr, &times.solve);
+        if (mode == 1) plan = colour_bodies(S.inv_mass, contacts, constraints, offsets, indices);
+        avbd_solve(S, contacts, constraints, offsets, indices, pc, op, orr, &times.solve, mode == 1 ? &plan : nullptr);
         for (size_t i = 0; i < n; ++i) {
             Body<T>& b = bodies[i];
             if (b.is_static()) continue;

@@ -225,6 +225,48 @@ OAPI void oracle_world_get_csr(void* p, uint32_t* offsets, uint32_t* indices) {
     });
 }
 
+// 0: Jacobi (the reference); 1: coloured Gauss-Seidel
+OAPI void oracle_world_set_mode(void* p, int mode) {
+    with_world(static_cast<WorldHandle*>(p), [&](auto& w) {
+        w.mode = mode;
+        return 0;
+    });
+}
+
+// The last coloured step's colouring: colour per body (kNotInGraph for static
+// bodies); returns the number of colours, *rounds the Jones-Plassmann rounds.
+OAPI uint32_t oracle_world_get_colours(void* p, uint32_t* colour, uint32_t* rounds) {
+    return with_world(static_cast<WorldHandle*>(p), [&](auto& w) {
+        for (size_t i = 0; i < w.plan.colour.size(); ++i) colour[i] = w.plan.colour[i];
+        if (rounds) *rounds = w.plan.rounds;
+        return (uint32_t)(w.plan.start.empty() ? 0 : w.plan.start.size() - 1);
+    });
+}
+
+// Colour an arbitrary graph: n vertices (dynamic[i] != 0 takes part), edges
+// (a, b) pairs.  Colour per vertex out; returns the number of colours, or
+// 0xFFFFFFFF when a vertex found all colours taken.
+OAPI uint32_t oracle_colour_graph(uint32_t n, const uint8_t* dynamic, const uint32_t* edges, uint32_t ne,
+                                  uint32_t* colour, uint32_t* rounds) {
+    std::vector<bool> dyn(n);
+    for (uint32_t i = 0; i < n; ++i) dyn[i] = dynamic[i] != 0;
+    std::vector<std::vector<uint32_t>> adj(n);
+    for (uint32_t e = 0; e < ne; ++e) {
+        const uint32_t a = edges[2 * e], b = edges[2 * e + 1];
+        if (a < n && b < n && a != b) {
+            adj[a].push_back(b);
+            adj[b].push_back(a);
+        }
+    }
+    const ColourPlan P = colour_graph(n, dyn, [&](uint32_t i, auto&& f) {
+        for (uint32_t o : adj[i])
+            if (dyn[o]) f(o);
+    });
+    for (uint32_t i = 0; i < n; ++i) colour[i] = P.colour[i];
+    if (rounds) *rounds = P.rounds;
+    return P.overflow ? 0xFFFFFFFFu : (uint32_t)(P.start.empty() ? 0 : P.start.size() - 1);
+}
+
 // broad, narrow, contacts, avbd, total, init, primal, dual, recover (seconds)
 OAPI void oracle_world_times(void* p, double* out) {
     with_world(static_cast<WorldHandle*>(p), [&](auto& w) {

@@ -56,6 +56,9 @@ def load() -> ctypes.CDLL:
         "oracle_broad_pairs": (c_uint32, [c_int, c_void_p, c_uint32, _F, c_uint32, _U, c_uint32]),
         "oracle_narrow_pair": (c_uint32, [c_int, c_void_p, c_void_p, _F, c_uint32, _D]),
         "oracle_gjk": (c_int, [c_int, c_void_p, c_void_p, _F, c_uint32]),
+        "oracle_world_set_mode": (None, [c_void_p, c_int]),
+        "oracle_world_get_colours": (c_uint32, [c_void_p, _U, _U]),
+        "oracle_colour_graph": (c_uint32, [c_uint32, ctypes.POINTER(ctypes.c_uint8), _U, c_uint32, _U, _U]),
         "oracle_solve6": (None, [c_int, _D, _D, _D, _D, _D]),
         "oracle_container_contacts": (c_uint32, [c_int, c_void_p, c_void_p, POINTER(R.Container), POINTER(R.Options),
                                                  c_uint32, _F, c_uint32, _U, _D]),
@@ -136,6 +139,18 @@ class OracleWorld:
     def step(self, options=None, nsteps: int = 1, lag: int = 1) -> None:
         o = (options or self.scene.options).to_c()
         load().oracle_world_step(self._h, ctypes.byref(o), lag, nsteps)
+
+    def set_mode(self, mode: int) -> None:
+        """0: Jacobi (the reference); 1: coloured Gauss-Seidel."""
+        load().oracle_world_set_mode(self._h, int(mode))
+
+    def colours(self) -> tuple:
+        """The last coloured step's colour per body (0xFFFFFFFE for static
+        bodies), the number of colours and of Jones-Plassmann rounds."""
+        col = np.zeros(self.n, np.uint32)
+        rounds = ctypes.c_uint32()
+        nc = load().oracle_world_get_colours(self._h, _p(col, c_uint32), ctypes.byref(rounds))
+        return col, int(nc), rounds.value
 
     def state(self) -> dict:
         n = self.n
@@ -397,3 +412,15 @@ def support(body, direction, host: bool = False, precision: int = 32) -> np.ndar
     out = np.zeros(3)
     load().oracle_support(precision, 1 if host else 0, _p(body), _p(d, c_double), _p(out, c_double))
     return out
+
+
+def colour_graph(n: int, dynamic, edges) -> tuple:
+    """Jones-Plassmann colouring of an arbitrary graph (the solve's colouring
+    rule): colour per vertex, number of colours (None on overflow), rounds."""
+    dyn = np.ascontiguousarray(np.asarray(dynamic, np.uint8))
+    e = np.ascontiguousarray(np.asarray(edges, np.uint32).reshape(-1, 2))
+    col = np.zeros(n, np.uint32)
+    rounds = ctypes.c_uint32()
+    nc = load().oracle_colour_graph(n, _p(dyn, ctypes.c_uint8), _p(e, c_uint32), len(e), _p(col, c_uint32),
+                                    ctypes.byref(rounds))
+    return col, (None if nc == 0xFFFFFFFF else int(nc)), rounds.value
