@@ -120,6 +120,8 @@ NarrowOut alloc_narrow(DevArena& A, uint32_t p_max) {
     o.scan_partials = A.alloc<uint32_t>(scan_partials_size(p_max + 1));
     o.gjk_list = A.alloc<uint32_t>(p_max);
     o.gjk_count = A.alloc<uint32_t>(1);
+    o.epa_retry_list = A.alloc<uint32_t>(p_max);
+    o.epa_retry_count = A.alloc<uint32_t>(1);
     o.p_max = p_max;
     return o;
 }

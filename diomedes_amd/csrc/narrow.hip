@@ -25,9 +25,13 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
+// EPA caps of epa.rs (256 vertices, 512 faces), and the smaller polytope of
+// the first pass: a pair that would outgrow it is run again at full size, so
+// every pair's result is the full-size algorithm's
 constexpr int kMaxV = 256;
 constexpr int kMaxF = 512;
-constexpr int kFacesPerLane = kMaxF / kWave;  // 8
+constexpr int kSmallV = 64;
+constexpr int kSmallF = 128;
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
@@ -199,18 +203,53 @@ __global__ void __launch_bounds__(kBlock) k_narrow(BodyView bv, NarrowOut o, flo
 }
 
 // ------------------------------------------------------------------ EPA
-struct EpaLds {
-    float vx[kMaxV], vy[kMaxV], vz[kMaxV];
-    float wx[kMaxV], wy[kMaxV], wz[kMaxV];
-    uint32_t fa[kMaxF], fb[kMaxF], fc[kMaxF];
-    float fnx[kMaxF], fny[kMaxF], fnz[kMaxF], fd[kMaxF];
-    uint32_t vis[kMaxF];
-    uint32_t ha[kMaxF], hb[kMaxF];
+template <int MV, int MF> struct EpaLds {
+    float vx[MV], vy[MV], vz[MV];
+    float wx[MV], wy[MV], wz[MV];
+    uint32_t fa[MF], fb[MF], fc[MF];
+    float fnx[MF], fny[MF], fnz[MF], fd[MF];
+    uint8_t vis[MF];
+    uint16_t vlist[MF];  // the visible faces, in index order
+    uint32_t ha[MF], hb[MF];
     uint32_t count;  // broadcast word
 };
 
-__device__ __forceinline__ f3 vtx(const EpaLds& L, uint32_t i) { return f3{L.vx[i], L.vy[i], L.vz[i]}; }
-__device__ __forceinline__ f3 wit(const EpaLds& L, uint32_t i) { return f3{L.wx[i], L.wy[i], L.wz[i]}; }
+// support_local of a hull with the wavefront's lanes splitting the vertices:
+// the same dot products, and the vertex the sequential scan keeps (the first
+// maximal one, skipping NaNs; a NaN at vertex 0 is never replaced).
+__device__ __forceinline__ f3 support_local_wave(const ShapeRef& s, f3 d, const HullPool& hp, int lane) {
+    if (s.tag != kHull) return support_local(s, d, hp);
+    const uint32_t first = (uint32_t)s.c[0].x, count = (uint32_t)s.c[0].y;
+    float bd = 0.0f;
+    uint32_t bi = 0;
+    bool have = false;
+    for (uint32_t i = (uint32_t)lane; i < count; i += kWave) {
+        const float dd = dot3(xyz(hp.v[first + i]), d);
+        if (dd == dd && (!have || dd > bd)) { bd = dd; bi = i; have = true; }
+    }
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const float od = __shfl_xor(bd, off, kWave);
+        const uint32_t oi = __shfl_xor(bi, off, kWave);
+        const bool oh = __shfl_xor((int)have, off, kWave) != 0;
+        if (oh && (!have || od > bd || (od == bd && oi < bi))) { bd = od; bi = oi; have = true; }
+    }
+    const float d0 = dot3(xyz(hp.v[first]), d);
+    if (!have || !(d0 == d0)) bi = 0;
+    return xyz(hp.v[first + bi]);
+}
+
+// support_witness (gjk.rs:67-73) with wave-parallel hull supports
+__device__ __forceinline__ void support_witness_wave(const ShapeRef& a, const ShapeRef& b, f3 d, const HullPool& hp,
+                                                     int lane, f3& point, f3& witness) {
+    const f3 ap = add(qrot(a.rot, support_local_wave(a, qrot_inv(a.rot, d), hp, lane)), a.pos);
+    const f3 bp = add(qrot(b.rot, support_local_wave(b, qrot_inv(b.rot, neg(d)), hp, lane)), b.pos);
+    point = sub(ap, bp);
+    witness = ap;
+}
+
+template <class L_t> __device__ __forceinline__ f3 vtx(const L_t& L, uint32_t i) { return f3{L.vx[i], L.vy[i], L.vz[i]}; }
+template <class L_t> __device__ __forceinline__ f3 wit(const L_t& L, uint32_t i) { return f3{L.wx[i], L.wy[i], L.wz[i]}; }
 
 struct FaceR {
     uint32_t a, b, c;
@@ -221,7 +260,7 @@ struct FaceR {
 
 // Polytope::add_face (epa.rs:58-85) without the store: the face and whether
 // it is non-degenerate
-__device__ __forceinline__ FaceR make_face(const EpaLds& L, uint32_t a, uint32_t b, uint32_t c) {
+template <class L_t> __device__ __forceinline__ FaceR make_face(const L_t& L, uint32_t a, uint32_t b, uint32_t c) {
     FaceR f;
     const f3 va = vtx(L, a);
     f3 n = cross3(sub(vtx(L, b), va), sub(vtx(L, c), va));
@@ -239,21 +278,9 @@ __device__ __forceinline__ FaceR make_face(const EpaLds& L, uint32_t a, uint32_t
     }
     return f;
 }
-__device__ __forceinline__ void store_face(EpaLds& L, uint32_t slot, const FaceR& f) {
+template <class L_t> __device__ __forceinline__ void store_face(L_t& L, uint32_t slot, const FaceR& f) {
     L.fa[slot] = f.a; L.fb[slot] = f.b; L.fc[slot] = f.c;
     L.fnx[slot] = f.n.x; L.fny[slot] = f.n.y; L.fnz[slot] = f.n.z; L.fd[slot] = f.d;
-}
-
-__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t& total) {
-    const int lane = threadIdx.x & (kWave - 1);
-    uint32_t inc = x;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t y = __shfl_up(inc, off, kWave);
-        if (lane >= off) inc += y;
-    }
-    total = __shfl(inc, kWave - 1, kWave);
-    return inc - x;
 }
 
 __device__ __forceinline__ bool edge_in(uint32_t ea, uint32_t eb, uint32_t fa, uint32_t fb, uint32_t fc) {
@@ -262,7 +289,8 @@ __device__ __forceinline__ bool edge_in(uint32_t ea, uint32_t eb, uint32_t fa, u
 }
 
 // epa.rs:193-240
-__device__ void face_contact(const EpaLds& L, uint32_t fi, f3& normal, float& depth, f3& pa, f3& pb) {
+template <class L_t>
+__device__ void face_contact(const L_t& L, uint32_t fi, f3& normal, float& depth, f3& pa, f3& pb) {
     const uint32_t ia = L.fa[fi], ib = L.fb[fi], ic = L.fc[fi];
     const f3 fn{L.fnx[fi], L.fny[fi], L.fnz[fi]};
     const float fdist = L.fd[fi];
@@ -293,15 +321,21 @@ __device__ void face_contact(const EpaLds& L, uint32_t fi, f3& normal, float& de
     pb = sub(pw, mk);
 }
 
-// One 64-lane workgroup per GJK-intersecting pair (epa.rs:89-189).
+// One 64-lane workgroup per GJK-intersecting pair (epa.rs:89-189), with a
+// polytope of MV vertices and MF faces.  FIRST: the pairs of the GJK list; a
+// pair that would pass MV or MF (where the caps of epa.rs, 256 and 512, are
+// not yet reached) goes to the retry list instead of giving a result.
+// Otherwise (MV, MF = the caps of epa.rs): the pairs of the retry list.
+template <int MV, int MF, bool FIRST>
 __global__ void __launch_bounds__(kWave) k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
-    __shared__ EpaLds L;
+    constexpr int kFacesPerLane = MF / kWave;
+    __shared__ EpaLds<MV, MF> L;
     const int lane = threadIdx.x;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
     const HullPool hp{bv.hull};
-    const uint32_t jobs = *o.gjk_count;
+    const uint32_t jobs = FIRST ? *o.gjk_count : *o.epa_retry_count;
     for (uint32_t job = blockIdx.x; job < jobs; job += gridDim.x) {
-        const uint32_t p = o.gjk_list[job];
+        const uint32_t p = FIRST ? o.gjk_list[job] : o.epa_retry_list[job];
         const uint2 pr = o.pairs[p];
         const ShapeRef a = load_shape(bv, pr.x);
         const ShapeRef b = load_shape(bv, pr.y);
@@ -328,6 +362,7 @@ __global__ void __launch_bounds__(kWave) k_epa(BodyView bv, NarrowOut o, const f
         uint32_t face_count = L.count;
         uint32_t vertex_count = 4;
         int result_face = -1;
+        bool retry = false;
         for (int it = 0; it < 64; ++it) {
             // closest face: first minimal distance in index order (strict <)
             float md = 0.0f;
@@ -356,55 +391,66 @@ __global__ void __launch_bounds__(kWave) k_epa(BodyView bv, NarrowOut o, const f
             const f3 mn{L.fnx[mi], L.fny[mi], L.fnz[mi]};
             const float mdist = L.fd[mi];
             f3 sp, sw;
-            support_witness(a, b, mn, hp, sp, sw);
+            support_witness_wave(a, b, mn, hp, lane, sp, sw);
             const float sd = dot3(mn, sp);
             if (sd - mdist < 1e-4f) { result_face = (int)mi; break; }
             const uint32_t ni = vertex_count;
-            if (ni >= (uint32_t)kMaxV) { result_face = (int)mi; break; }
+            if (ni >= (uint32_t)MV) {
+                if (MV < kMaxV) retry = true;
+                else result_face = (int)mi;
+                break;
+            }
             if (lane == 0) {
                 L.vx[ni] = sp.x; L.vy[ni] = sp.y; L.vz[ni] = sp.z;
                 L.wx[ni] = sw.x; L.wy[ni] = sw.y; L.wz[ni] = sw.z;
             }
             vertex_count += 1;
-            // visibility of each face from the new vertex
+            // visibility of each face from the new vertex, and the list of the
+            // visible faces in index order
+            uint32_t nvis = 0;
             for (int k = 0; k < kFacesPerLane; ++k) {
                 const uint32_t f = (uint32_t)(k * kWave + lane);
+                bool v = false;
                 if (f < face_count) {
                     const f3 fn{L.fnx[f], L.fny[f], L.fnz[f]};
-                    L.vis[f] = dot3(fn, sub(sp, vtx(L, L.fa[f]))) > 0.0f ? 1u : 0u;
+                    v = dot3(fn, sub(sp, vtx(L, L.fa[f]))) > 0.0f;
+                    L.vis[f] = v ? 1u : 0u;
                 }
+                const uint64_t bal = __ballot(v);
+                if (v) L.vlist[nvis + (uint32_t)__popcll(bal & lt)] = (uint16_t)f;
+                nvis += (uint32_t)__popcll(bal);
             }
             __syncthreads();
             // horizon: edges of visible faces that no other visible face has,
-            // in (face, edge) order, at most kMaxF of them
+            // in (face, edge) order, at most kMaxF of them.  Candidate c is
+            // edge c % 3 of visible face c / 3; only visible faces can share it.
             uint32_t hbase = 0;
-            for (int k = 0; k < kFacesPerLane; ++k) {
-                const uint32_t f = (uint32_t)(k * kWave + lane);
-                uint32_t e[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-                uint32_t hmask = 0;
-                if (f < face_count && L.vis[f]) {
+            for (uint32_t cb = 0; cb < 3 * nvis; cb += kWave) {
+                const uint32_t c = cb + (uint32_t)lane;
+                bool hz = false;
+                uint32_t ea = 0, eb = 0;
+                if (c < 3 * nvis) {
+                    const uint32_t f = L.vlist[c / 3], q = c % 3;
                     const uint32_t fa = L.fa[f], fb = L.fb[f], fcc = L.fc[f];
-                    e[0][0] = fa; e[0][1] = fb; e[1][0] = fb; e[1][1] = fcc; e[2][0] = fcc; e[2][1] = fa;
-                    for (int q = 0; q < 3; ++q) {
-                        bool shared = false;
-                        for (uint32_t g = 0; g < face_count && !shared; ++g)
-                            if (g != f && L.vis[g] && edge_in(e[q][0], e[q][1], L.fa[g], L.fb[g], L.fc[g]))
-                                shared = true;
-                        if (!shared) hmask |= 1u << q;
+                    ea = q == 0 ? fa : (q == 1 ? fb : fcc);
+                    eb = q == 0 ? fb : (q == 1 ? fcc : fa);
+                    bool shared = false;
+                    for (uint32_t t = 0; t < nvis && !shared; ++t) {
+                        const uint32_t g = L.vlist[t];
+                        if (g != f && edge_in(ea, eb, L.fa[g], L.fb[g], L.fc[g])) shared = true;
                     }
+                    hz = !shared;
                 }
-                uint32_t tot;
-                const uint32_t pre = wave_prefix((uint32_t)__popc(hmask), tot);
-                uint32_t pos = hbase + pre;
-                for (int q = 0; q < 3; ++q) {
-                    if (hmask & (1u << q)) {
-                        if (pos < (uint32_t)kMaxF) { L.ha[pos] = e[q][0]; L.hb[pos] = e[q][1]; }
-                        ++pos;
-                    }
-                }
-                hbase += tot;
+                const uint64_t bal = __ballot(hz);
+                const uint32_t pos = hbase + (uint32_t)__popcll(bal & lt);
+                if (hz && pos < (uint32_t)MF) { L.ha[pos] = ea; L.hb[pos] = eb; }
+                hbase += (uint32_t)__popcll(bal);
             }
-            const uint32_t hcount = hbase < (uint32_t)kMaxF ? hbase : (uint32_t)kMaxF;
+            if (MF < kMaxF && hbase > (uint32_t)MF) {
+                retry = true;
+                break;
+            }
+            const uint32_t hcount = hbase < (uint32_t)MF ? hbase : (uint32_t)MF;
             // compact the kept (non-visible) faces in index order
             FaceR keep[kFacesPerLane];
             uint32_t kflag = 0;
@@ -435,12 +481,15 @@ __global__ void __launch_bounds__(kWave) k_epa(BodyView bv, NarrowOut o, const f
                 if (h < hcount) f = make_face(L, ni, L.ha[h], L.hb[h]);
                 const uint64_t bal = __ballot(f.ok);
                 const uint32_t slot = face_count + (uint32_t)__popcll(bal & lt);
-                if (f.ok && slot < (uint32_t)kMaxF) store_face(L, slot, f);
+                if (f.ok && slot < (uint32_t)MF) store_face(L, slot, f);
                 const uint32_t added = face_count + (uint32_t)__popcll(bal);
-                face_count = added < (uint32_t)kMaxF ? added : (uint32_t)kMaxF;
+                if (MF < kMaxF && added > (uint32_t)MF) retry = true;
+                face_count = added < (uint32_t)MF ? added : (uint32_t)MF;
             }
             __syncthreads();
+            if (retry) break;
         }
+        if (retry && lane == 0) o.epa_retry_list[atomicAdd(o.epa_retry_count, 1u)] = p;
         if (result_face >= 0 && lane == 0) {
             f3 n, pa, pb;
             float depth;
@@ -482,9 +531,13 @@ void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, h
     uint32_t blocks = blocks_for(o.p_max > 0 ? o.p_max : 1);
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_narrow, dim3(blocks), dim3(kBlock), 0, s, bv, o, simplex_store);
+    launch_fill_u32(o.epa_retry_count, 0u, 1, s);
     uint32_t eblocks = o.p_max > 0 ? o.p_max : 1;
-    if (eblocks > 2048) eblocks = 2048;
-    hipLaunchKernelGGL(k_epa, dim3(eblocks), dim3(kWave), 0, s, bv, o, (const float4*)simplex_store);
+    if (eblocks > 8192) eblocks = 8192;
+    hipLaunchKernelGGL((k_epa<kSmallV, kSmallF, true>), dim3(eblocks), dim3(kWave), 0, s, bv, o,
+                       (const float4*)simplex_store);
+    hipLaunchKernelGGL((k_epa<kMaxV, kMaxF, false>), dim3(eblocks < 512 ? eblocks : 512), dim3(kWave), 0, s, bv, o,
+                       (const float4*)simplex_store);
     launch_scan_exclusive(o.cnt, o.off, o.p_max, o.p_dev, o.scan_partials, s);
 }
 

@@ -20,6 +20,8 @@ struct NarrowOut {
     uint32_t* scan_partials;
     uint32_t* gjk_list;   // pairs routed to GJK + EPA
     uint32_t* gjk_count;
+    uint32_t* epa_retry_list;   // pairs whose polytope outgrew the first EPA pass
+    uint32_t* epa_retry_count;
     const uint2* pairs;   // the pair list these slots belong to
     const uint32_t* p_dev;
     uint32_t p_max;

@@ -338,6 +338,50 @@ def test_narrow_convex_gjk_epa_bitexact(oracle, seed):
     assert hits > 50
 
 
+def test_narrow_epa_large_polytopes_bitexact(oracle):
+    """Deeply overlapping 642-vertex icosphere hulls: EPA polytopes outgrow
+    the first pass (64 vertices, 128 faces) and many reach the 256-vertex cap
+    of epa.rs; the retried pairs must give the full-size result."""
+    from diomedes_amd.physics import AvbdBody, Shape, gpu_bodies
+    from diomedes_amd import records as RR
+    from diomedes_amd.physics import Scene
+
+    rng = np.random.default_rng(7)
+    ico = scenes.icosphere(3).astype(np.float32)
+    nv = len(ico)
+    hull = np.concatenate([ico * np.float32(0.5), ico * np.float32(0.7)])
+    bodies = []
+    for k in range(64):
+        q = rng.normal(size=(2, 4))
+        q /= np.linalg.norm(q, axis=1, keepdims=True)
+        pa = rng.normal(size=3) * 3.0
+        off = rng.normal(size=3)
+        off *= rng.uniform(0.05, 0.9) / np.linalg.norm(off)
+        bodies.append(AvbdBody(Shape.hull(0, nv), tuple(pa), tuple(q[0])))
+        bodies.append(AvbdBody(Shape.hull(nv, nv), tuple(pa + off), tuple(q[1])))
+    bodies = gpu_bodies(bodies)
+    n = len(bodies)
+    pairs = np.zeros(n // 2, RR.GPU_PAIR)
+    pairs["a"] = np.arange(0, n, 2)
+    pairs["b"] = np.arange(1, n, 2)
+    with PhysicsContext(n, max_hull_vertices=len(hull)) as ctx:
+        ctx.upload(Scene(bodies, np.zeros(n, RR.BODY_STATE), hull=hull))
+        raw = ctx.narrow_phase_run(bodies, pairs)
+    hits = 0
+    for i in range(len(pairs)):
+        ref = oracle.narrow_pair(bodies[2 * i], bodies[2 * i + 1], hull=hull)
+        slots = raw[4 * i:4 * i + 4]
+        assert int(np.sum(slots["valid"])) == len(ref), i
+        for k, c in enumerate(ref):
+            assert_bits(slots[k]["normal"][:3], c["normal"], f"normal {i}")
+            assert_bits(slots[k]["depth"], c["depth"], f"depth {i}")
+            assert_bits(slots[k]["point_a"][:3], c["point_a"], f"pa {i}")
+            assert_bits(slots[k]["point_b"][:3], c["point_b"], f"pb {i}")
+            hits += 1
+    _print("large-polytope EPA contacts", hits)
+    assert hits >= len(pairs) // 2  # the rest end EPA's 64 iterations without converging: no contact
+
+
 def test_mixed_field_small_bitexact(oracle):
     """Hulls, tetrahedra, boxes, ropes and springs through the fused step."""
     scene = scenes.mixed_field(5, n=300, ropes=4, rope_links=8, springs=16, extent=8.0)
