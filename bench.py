@@ -138,6 +138,37 @@ def batched_leg(args, tuning: dict) -> dict:
     return out
 
 
+def coloured_leg(args, tuning: dict) -> dict:
+    """The same cube field stepped in the coloured Gauss-Seidel mode (dio_step
+    mode 1): the step's body graph is coloured on the device and each
+    iteration runs the primal colour by colour.  t_AVBD here includes the
+    colouring and its one host read-back per step."""
+    from diomedes_amd import PhysicsContext, scenes
+
+    scene = scenes.cube_field(args.seed)
+    opts = scene.options
+    k = args.steps
+    with PhysicsContext(scene.n) as ctx:
+        ctx.set_tuning(**tuning)
+        ctx.upload(scene)
+        if args.warmup:
+            ctx.step(opts, args.warmup, mode=1)
+        ctx.step(opts, 0)
+        t0 = time.perf_counter()
+        ctx.step(opts, k, mode=1)
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+    _progress(f"coloured leg: {k} steps in {wall:.2f} s")
+    return {
+        "workload": "the same cube field, coloured Gauss-Seidel primal (mode 1)",
+        "value": scene.n_dynamic * opts.iterations * k / (st.ms_avbd / 1000.0),
+        "unit": "body-iters/s",
+        "ms_per_step": 1000.0 * wall / k,
+        "ms_per_step_avbd": st.ms_avbd / k,
+        "last_step": {"colours": st.colours, "colour_rounds": st.colour_rounds, "contacts": st.contacts},
+    }
+
+
 def _wall_loss_grad(pos, targets, offsets):
     """Sum over seeds of the mean squared distance of the bricks' final
     centres to their targets, and its adjoint with respect to the final
@@ -347,6 +378,7 @@ def main() -> int:
     ap.add_argument("--tune-steps", type=int, default=0,
                     help="steps of a separate run of the same scene after which dio_autotune picks the launch "
                          "shapes and dual schedule (0: library defaults)")
+    ap.add_argument("--no-coloured", action="store_true", help="skip the coloured Gauss-Seidel leg")
     ap.add_argument("--adjoint-seeds", type=int, default=64,
                     help="wall throws of the adjoint leg (0: skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=120, help="steps of the adjoint leg")
@@ -473,6 +505,8 @@ def main() -> int:
             "dual_recover": 1000.0 * pst.ms_recover / max(pst.n_recover, 1),
         },
     }
+    if not args.no_coloured:
+        line["coloured"] = coloured_leg(args, tuning)
     if args.batch_scenes:
         line["batched"] = batched_leg(args, tuning)
     if args.adjoint_seeds:

@@ -183,30 +183,26 @@ __device__ __forceinline__ uint32_t body_at(const SolveView& v, const AvbdParams
 }
 __device__ __forceinline__ float4* pen_buf(const SolveView& v, uint32_t j) { return (j & 1u) ? v.c_pen_alt : v.c_pen; }
 
-__global__ void __launch_bounds__(kBlock) k_avbd_init(SolveView v, AvbdParams p) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= p.n) return;
+// avbd_init of one body (avbd/src/lib.rs:202-254) from its pose and
+// velocities, into target t
+__device__ __forceinline__ void init_body(const InitTarget& t, const AvbdParams& p, uint32_t i, float4 pos4,
+                                          float4 rot4, f3 vel, f3 angv, f3 prev, float inv_mass) {
     const float dt = p.dt;
     const f3 g{p.gx, p.gy, p.gz};
-    const float inv_mass = v.inv_mm[i].w;
-    const float4 pos4 = v.cur_pos[i];
-    const float4 rot4 = v.cur_rot[i];
     const f3 pos = xyz(pos4);
     const q4 rot = as_q(rot4);
-    const f3 vel = xyz(v.vel[i]);
-    const f3 angv = xyz(v.ang[i]);
-    v.init_lin[i] = f4(pos, 0.0f);
-    v.init_ang[i] = rot4;
+    t.init_lin[i] = f4(pos, 0.0f);
+    t.init_ang[i] = rot4;
     f3 ip = add(pos, scale(vel, dt));
     if (inv_mass > 0.0f) ip = add(ip, scale(g, dt * dt));
-    v.iner_lin[i] = f4(ip, 0.0f);
+    t.iner_lin[i] = f4(ip, 0.0f);
     const q4 irot = quat_step(rot, scale(angv, dt));
-    v.iner_ang[i] = f4q(irot);
+    t.iner_ang[i] = f4q(irot);
     // adaptive gravity warm start (avbd/src/lib.rs:235-251)
     f3 wp = pos;
     q4 wr = rot;
     if (inv_mass > 0.0f) {
-        const f3 accel = divs(sub(vel, xyz(v.prev_vel[i])), dt);
+        const f3 accel = divs(sub(vel, prev), dt);
         const float gm = len3(g);
         const float weight = gm > 0.0f ? rclamp(dot3(accel, divs(g, gm)) / gm, 0.0f, 1.0f) : 0.0f;
         wp = add(add(pos, scale(vel, dt)), scale(g, weight * dt * dt));
@@ -214,10 +210,17 @@ __global__ void __launch_bounds__(kBlock) k_avbd_init(SolveView v, AvbdParams p)
     }
     // both parity buffers start from S_k (gpu_physics.rs:477-482); buffer 0
     // then takes the warm start
-    v.pos[1][i] = f4(pos, 0.0f);
-    v.rot[1][i] = rot4;
-    v.pos[0][i] = f4(wp, 0.0f);
-    v.rot[0][i] = f4q(wr);
+    t.pos[1][i] = f4(pos, 0.0f);
+    t.rot[1][i] = rot4;
+    t.pos[0][i] = f4(wp, 0.0f);
+    t.rot[0][i] = f4q(wr);
+}
+
+__global__ void __launch_bounds__(kBlock) k_avbd_init(SolveView v, AvbdParams p) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= p.n) return;
+    const InitTarget t{{v.pos[0], v.pos[1]}, {v.rot[0], v.rot[1]}, v.init_lin, v.init_ang, v.iner_lin, v.iner_ang};
+    init_body(t, p, i, v.cur_pos[i], v.cur_rot[i], xyz(v.vel[i]), xyz(v.ang[i]), xyz(v.prev_vel[i]), v.inv_mm[i].w);
 }
 
 template <bool FUSED>
@@ -382,8 +385,11 @@ constexpr int kRowFloats = 8;
 constexpr int kItemFloats = 3 * kRowFloats + 1;  // 3 rows + row count
 }  // namespace
 
+// Compiled for four waves per SIMD (at most 128 VGPRs): the fused variant
+// then keeps a few values in scratch, which measures faster than three waves.
 template <int G, bool FUSED>
-__global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t iter) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t iter) {
     const uint32_t parity = iter & 1u;
     const bool fuse = FUSED && iter > 0;
     const float4* __restrict__ lam_in = lam_buf(v, fuse ? iter - 1 : 0);
@@ -393,9 +399,8 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     constexpr int kBodies = kBlock / G;
     constexpr int kOwned = (27 + G - 1) / G;
     __shared__ float stage[kBodies][G][kItemFloats];
-    __shared__ float gathered[kBodies][28];  // initial entries, later the accumulated block
-    __shared__ float4 body_f[kBodies][4];    // snapshot pos (.w: active), rot, d_lin, d_ang
-    __shared__ uint4 body_u[kBodies];        // CSR start / count of contacts, then of joints
+    __shared__ float gathered[kBodies][28];  // the accumulated block
+    __shared__ float4 body_f[kBodies][2];    // snapshot pos (.w: active), rot: for the solving thread
     const int slot = threadIdx.x / G;
     const int lane = threadIdx.x % G;
     const uint32_t i = body_at(v, p, blockIdx.x * kBodies + slot);
@@ -403,67 +408,71 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     const float4* __restrict__ rr = v.rot[parity];
     const float dt = p.dt;
 
-    // ---- per-body set-up by one thread per body (all in wave 0): the
-    // snapshot, the item ranges and the initial block, shared through LDS
-    if (threadIdx.x < (unsigned)kBodies) {
-        const int t = threadIdx.x;
-        const uint32_t bi = body_at(v, p, blockIdx.x * kBodies + t);
-        bool act = bi < p.n;
-        float4 imm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (act) {
-            imm = v.inv_mm[bi];
-            act = imm.w != 0.0f;
+    // ---- every lane of the group loads its body's snapshot and item ranges
+    // and computes the initial block entries it owns (no set-up barrier)
+    bool active = i < p.n;
+    float4 imm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 sp4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sr4 = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    float4 il4 = sp4, ia4 = sr4, nl4 = sp4, na4 = sr4;
+    uint32_t jb = 0, ncontacts = 0, kb = 0, njoints = 0;
+    if (active) {
+        imm = v.inv_mm[i];
+        sp4 = rp[i];
+        sr4 = rr[i];
+        il4 = v.init_lin[i];
+        ia4 = v.init_ang[i];
+        nl4 = v.iner_lin[i];
+        na4 = v.iner_ang[i];
+        jb = v.offsets[i];
+        ncontacts = v.offsets[i + 1] - jb;
+        if (v.j_off) {
+            kb = v.j_off[i];
+            njoints = v.j_off[i + 1] - kb;
         }
-        uint4 u = make_uint4(0u, 0u, 0u, 0u);
-        f3 sp{0.0f, 0.0f, 0.0f}, sl{0.0f, 0.0f, 0.0f}, sa{0.0f, 0.0f, 0.0f};
-        q4 sr{0.0f, 0.0f, 0.0f, 1.0f};
-        if (act) {
-            sp = xyz(rp[bi]);
-            sr = as_q(rr[bi]);
-            sl = sub(sp, xyz(v.init_lin[bi]));
-            sa = ang_diff(sr, as_q(v.init_ang[bi]));
-            u.x = v.offsets[bi];
-            u.y = v.offsets[bi + 1] - u.x;
-            if (v.j_off) {
-                u.z = v.j_off[bi];
-                u.w = v.j_off[bi + 1] - u.z;
-            }
-            // initial block: diag(M / dt^2, I / dt^2) and the inertial gradient
+        active = imm.w != 0.0f;
+    }
+    if (!active) { ncontacts = 0; njoints = 0; }
+    const f3 snap_pos = xyz(sp4);
+    const q4 snap_rot = as_q(sr4);
+    const f3 dsl = active ? sub(snap_pos, xyz(il4)) : f3{0.0f, 0.0f, 0.0f};
+    const f3 dsa = active ? ang_diff(snap_rot, as_q(ia4)) : f3{0.0f, 0.0f, 0.0f};
+    if (lane == 0) {
+        body_f[slot][0] = f4(snap_pos, active ? 1.0f : 0.0f);
+        body_f[slot][1] = sr4;
+    }
+    float acc[kOwned];
+    {
+        // initial block: diag(M / dt^2, I / dt^2) and the inertial gradient
+        // (entry order: lin 0-5, ang 6-11, cross 12-20, rhs lin 21-23, rhs ang 24-26)
+        float m_dt2 = 0.0f;
+        f3 i_dt2{0.0f, 0.0f, 0.0f}, gl{0.0f, 0.0f, 0.0f}, ga{0.0f, 0.0f, 0.0f};
+        if (active) {
             const float mass = 1.0f / imm.w;
             const f3 moment{1.0f / imm.x, 1.0f / imm.y, 1.0f / imm.z};
             const float dt2 = dt * dt;
-            const float m_dt2 = mass / dt2;
-            const f3 i_dt2 = divs(moment, dt2);
-            const f3 dl = sub(sp, xyz(v.iner_lin[bi]));
-            const f3 ad = ang_diff(sr, as_q(v.iner_ang[bi]));
-            float* g0 = gathered[t];
-#pragma unroll
-            for (int e = 0; e < 27; ++e) g0[e] = 0.0f;
-            g0[0] = m_dt2; g0[3] = m_dt2; g0[5] = m_dt2;
-            g0[6] = i_dt2.x; g0[9] = i_dt2.y; g0[11] = i_dt2.z;
-            g0[21] = m_dt2 * dl.x; g0[22] = m_dt2 * dl.y; g0[23] = m_dt2 * dl.z;
-            g0[24] = i_dt2.x * ad.x; g0[25] = i_dt2.y * ad.y; g0[26] = i_dt2.z * ad.z;
+            m_dt2 = mass / dt2;
+            i_dt2 = divs(moment, dt2);
+            const f3 dl = sub(snap_pos, xyz(nl4));
+            const f3 ad = ang_diff(snap_rot, as_q(na4));
+            gl = f3{m_dt2 * dl.x, m_dt2 * dl.y, m_dt2 * dl.z};
+            ga = f3{i_dt2.x * ad.x, i_dt2.y * ad.y, i_dt2.z * ad.z};
         }
-        body_f[t][0] = f4(sp, act ? 1.0f : 0.0f);
-        body_f[t][1] = f4q(sr);
-        body_f[t][2] = f4(sl, 0.0f);
-        body_f[t][3] = f4(sa, 0.0f);
-        body_u[t] = u;
-    }
-    __syncthreads();
-    const float4 bf0 = body_f[slot][0];
-    const bool active = bf0.w != 0.0f;
-    const f3 snap_pos = xyz(bf0);
-    const q4 snap_rot = as_q(body_f[slot][1]);
-    const f3 dsl = xyz(body_f[slot][2]);
-    const f3 dsa = xyz(body_f[slot][3]);
-    const uint4 bu = body_u[slot];
-    const uint32_t jb = bu.x, ncontacts = bu.y, kb = bu.z, njoints = bu.w;
-    float acc[kOwned];
 #pragma unroll
-    for (int t = 0; t < kOwned; ++t) {
-        const int e = lane + G * t;
-        acc[t] = (active && e < 27) ? gathered[slot][e] : 0.0f;
+        for (int t = 0; t < kOwned; ++t) {
+            const int e = lane + G * t;
+            float x = 0.0f;
+            x = (e == 0 || e == 3 || e == 5) ? m_dt2 : x;
+            x = e == 6 ? i_dt2.x : x;
+            x = e == 9 ? i_dt2.y : x;
+            x = e == 11 ? i_dt2.z : x;
+            x = e == 21 ? gl.x : x;
+            x = e == 22 ? gl.y : x;
+            x = e == 23 ? gl.z : x;
+            x = e == 24 ? ga.x : x;
+            x = e == 25 ? ga.y : x;
+            x = e == 26 ? ga.z : x;
+            acc[t] = x;
+        }
     }
     const uint32_t nitems = ncontacts + njoints;
     const float kk = 1.0f - p.alpha;
@@ -711,7 +720,13 @@ __global__ void __launch_bounds__(kBlock) k_avbd_dual_recover(SolveView v, AvbdP
 }
 
 __device__ __forceinline__ void recover_one(const SolveView& v, const AvbdParams& p, uint32_t i, uint32_t parity) {
-    if (v.inv_mm[i].w == 0.0f) return;
+    const float inv_mass = v.inv_mm[i].w;
+    if (inv_mass == 0.0f) {
+        if (v.next.init_lin)
+            init_body(v.next, p, i, v.cur_pos[i], v.cur_rot[i], xyz(v.vel[i]), xyz(v.ang[i]), xyz(v.prev_vel[i]),
+                      inv_mass);
+        return;
+    }
     const float4 pos4 = v.pos[1 - parity][i];
     const float4 rot4 = v.rot[1 - parity][i];
     const float dt = p.dt;
@@ -720,11 +735,13 @@ __device__ __forceinline__ void recover_one(const SolveView& v, const AvbdParams
     const f3 nv = divs(sub(xyz(pos4), xyz(v.init_lin[i])), dt);
     v.vel[i] = f4(nv, 0.0f);
     const q4 rel = qmul(as_q(rot4), qconj(as_q(v.init_ang[i])));
-    v.ang[i] = make_float4(2.0f * rel.x / dt, 2.0f * rel.y / dt, 2.0f * rel.z / dt, 0.0f);
+    const float4 nang = make_float4(2.0f * rel.x / dt, 2.0f * rel.y / dt, 2.0f * rel.z / dt, 0.0f);
+    v.ang[i] = nang;
     if (v.out_pos) {
         v.out_pos[i] = pos4;
         v.out_rot[i] = rot4;
     }
+    if (v.next.init_lin) init_body(v.next, p, i, pos4, rot4, nv, xyz(nang), xyz(vel), inv_mass);
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }

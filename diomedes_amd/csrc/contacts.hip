@@ -312,8 +312,7 @@ void run_contact_build(const WorldView& w, const ContainerParams& ct, const Buil
 }
 
 void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s) {
-    launch_fill_u32(b.cnt, 0u, n, s);
-    launch_fill_u32(b.cur, 0u, n, s);
+    launch_fill2_u32(b.cnt, b.cur, 0u, n, s);
     uint32_t mb = blocks_for(cs.m_cap > 0 ? cs.m_cap : 1);
     if (mb > 2048) mb = 2048;
     hipLaunchKernelGGL(k_csr_count, dim3(mb), dim3(kBlock), 0, s, cs, b.cnt);

@@ -252,8 +252,13 @@ struct dio_ctx {
     } pend[2]{};
     ContactSet cs[2]{};
     CsrBuffers csr{};
-    float4 *s_pos[2] = {nullptr, nullptr}, *s_rot[2] = {nullptr, nullptr};
-    float4 *s_init_lin = nullptr, *s_init_ang = nullptr, *s_iner_lin = nullptr, *s_iner_ang = nullptr;
+    // two sets of solver buffers, used by alternate steps: the final launch
+    // of a Jacobi step runs the next step's init into the other set.
+    // init_set: the set holding a prepared init (-1: none), for init_dt /
+    // init_g (the options the init depends on)
+    InitTarget ss[2]{};
+    int init_set = -1;
+    float init_dt = 0.0f, init_g[3] = {0.0f, 0.0f, 0.0f};
     uint32_t* w_overflow = nullptr;
     unsigned long long* w_acc = nullptr;  // sums of pairs, contacts, CSR nnz over a dio_step call
     float4 *w_lam_alt = nullptr, *w_pen_alt = nullptr;  // scratch multipliers of a solve
@@ -486,14 +491,14 @@ struct KernelTimer {
 // One AVBD solve (gpu_physics.rs:541-550): init; I x (primal, dual), the
 // dual fused into the next primal when `fused`; the last dual with recover.
 void schedule_avbd(const SolveView& v, const AvbdParams& p, uint32_t iters, uint32_t m_grid, hipStream_t s,
-                   KernelTimer* kt, const dio_tuning& t) {
+                   KernelTimer* kt, const dio_tuning& t, bool run_init = true) {
     const int lanes = (int)t.primal_lanes;
     const bool fused = t.fuse_dual != 0u;
     auto run = [&](int kind, auto&& launch) {
         if (kt) kt->time(kind, launch);
         else launch(LaunchEvents{});
     };
-    run(0, [&](const LaunchEvents& ev) { launch_avbd_init(v, p, s, ev); });
+    if (run_init) run(0, [&](const LaunchEvents& ev) { launch_avbd_init(v, p, s, ev); });
     for (uint32_t k = 0; k < iters; ++k) {
         run(1, [&](const LaunchEvents& ev) { launch_avbd_primal(v, p, k, fused, s, lanes, ev); });
         if (!fused && k + 1 < iters)
@@ -929,14 +934,17 @@ void ensure_world(dio_ctx* c) {
     c->csr.tmp = A.alloc<uint32_t>(2 * (size_t)m);
     c->csr.scan_partials = A.alloc<uint32_t>(scan_partials_size(n + 1));
     c->csr.idx_cap = 2 * m;
-    for (int k = 0; k < 2; ++k) {
-        c->s_pos[k] = A.alloc<float4>(n);
-        c->s_rot[k] = A.alloc<float4>(n);
+    for (int set = 0; set < 2; ++set) {
+        InitTarget& t = c->ss[set];
+        for (int k = 0; k < 2; ++k) {
+            t.pos[k] = A.alloc<float4>(n);
+            t.rot[k] = A.alloc<float4>(n);
+        }
+        t.init_lin = A.alloc<float4>(n);
+        t.init_ang = A.alloc<float4>(n);
+        t.iner_lin = A.alloc<float4>(n);
+        t.iner_ang = A.alloc<float4>(n);
     }
-    c->s_init_lin = A.alloc<float4>(n);
-    c->s_init_ang = A.alloc<float4>(n);
-    c->s_iner_lin = A.alloc<float4>(n);
-    c->s_iner_ang = A.alloc<float4>(n);
     c->w_overflow = A.alloc<uint32_t>(1);
     c->w_acc = A.alloc<unsigned long long>(4);
     c->w_lam_alt = A.alloc<float4>(m);
@@ -953,7 +961,9 @@ NarrowSource source_of(const dio_ctx::Pending& p) {
     return NarrowSource{p.pairs, p.out.off, p.p_dev, p.out.n, p.out.pa, p.out.pb};
 }
 
-SolveView world_solve_view(dio_ctx* c, int s);
+// The world's solve over buffer set s; prepare_next: the final launch also
+// runs the next step's init into set 1 - s.
+SolveView world_solve_view(dio_ctx* c, int s, bool prepare_next = false);
 
 // Enqueue segment `seg` of one step with parity s (0: contacts + broad +
 // narrow, 1: AVBD).
@@ -996,9 +1006,11 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
                                BodyState{c->w_pos, c->w_rot, c->w_vel, c->w_ang, c->w_prev}, n, c->m_cap, st);
         if (lag) broad_narrow();
     } else {
-        SolveView v = world_solve_view(c, s);
+        // the init of this step ran already (the previous step's final launch,
+        // or dio_step before this segment)
+        SolveView v = world_solve_view(c, s, true);
         const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
-        schedule_avbd(v, p, o.iterations, c->m_cap, st, kt, c->tuning);
+        schedule_avbd(v, p, o.iterations, c->m_cap, st, kt, c->tuning, false);
         const int bn_pend = so.contact_lag ? 1 - s : s;
         hipLaunchKernelGGL(k_stats_accum, dim3(1), dim3(64), 0, st, (const uint32_t*)c->pend[bn_pend].p_dev,
                            (const uint32_t*)c->cs[s].m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
@@ -1060,15 +1072,31 @@ void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const di
                        (const uint32_t*)c->cs[s].m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
 }
 
-SolveView world_solve_view(dio_ctx* c, int s) {
+// Whether buffer set s holds this step's init, prepared for these options.
+bool init_prepared(const dio_ctx* c, int s, const dio_avbd_options& o) {
+    return c->init_set == s && c->init_dt == o.dt && c->init_g[0] == o.gravity[0] && c->init_g[1] == o.gravity[1] &&
+           c->init_g[2] == o.gravity[2];
+}
+
+// The standalone init of buffer set s from the world state.
+void run_world_init(dio_ctx* c, int s, const dio_avbd_options& o, KernelTimer* kt) {
+    const SolveView v = world_solve_view(c, s);
+    const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, c->w_n);
+    if (kt) kt->time(0, [&](const LaunchEvents& ev) { launch_avbd_init(v, p, c->stream, ev); });
+    else launch_avbd_init(v, p, c->stream);
+}
+
+SolveView world_solve_view(dio_ctx* c, int s, bool prepare_next) {
     {
         SolveView v{};
         v.cur_pos = c->w_pos;
         v.cur_rot = c->w_rot;
-        v.pos[0] = c->s_pos[0]; v.pos[1] = c->s_pos[1];
-        v.rot[0] = c->s_rot[0]; v.rot[1] = c->s_rot[1];
-        v.init_lin = c->s_init_lin; v.init_ang = c->s_init_ang;
-        v.iner_lin = c->s_iner_lin; v.iner_ang = c->s_iner_ang;
+        const InitTarget& t = c->ss[s];
+        v.pos[0] = t.pos[0]; v.pos[1] = t.pos[1];
+        v.rot[0] = t.rot[0]; v.rot[1] = t.rot[1];
+        v.init_lin = t.init_lin; v.init_ang = t.init_ang;
+        v.iner_lin = t.iner_lin; v.iner_ang = t.iner_ang;
+        if (prepare_next) v.next = c->ss[1 - s];
         v.vel = c->w_vel; v.ang = c->w_ang; v.prev_vel = c->w_prev; v.inv_mm = c->w_inv_mm;
         const ContactSet& cs = c->cs[s];
         v.c_ab = cs.c.ab; v.c_n = cs.c.n; v.c_t1 = cs.c.t1; v.c_t2 = cs.c.t2; v.c_ra = cs.c.ra; v.c_rb = cs.c.rb;
@@ -1164,6 +1192,7 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
     DIO_HIP_CHECK(hipStreamSynchronize(s));
     ctx->drop_graphs();
     ctx->free_adjoint();  // a recording belongs to the world it was made in
+    ctx->init_set = -1;
     ctx->w_n = n;
     ctx->w_step = 0;
     ctx->w_sort_pending = true;
@@ -1286,10 +1315,15 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
                 ctx->w_sorted_at = ctx->w_step;
                 ctx->w_sort_pending = false;
             }
+            // a Jacobi step's init is the previous step's final launch when that
+            // prepared this buffer set with the same dt and gravity
+            const bool need_init = !coloured && !init_prepared(ctx, par, *options);
+            KernelTimer* tk = (!use_graph && so.profile) ? &kt : nullptr;
             if (use_graph) {
                 const auto a = std::chrono::steady_clock::now();
                 DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[resort ? 2 : 0][par], s));
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
+                if (need_init) run_world_init(ctx, par, *options, nullptr);
                 if (coloured) run_coloured_segment(ctx, par, *options, so, nullptr);
                 else DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[1][par], s));
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
@@ -1297,9 +1331,17 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
             } else {
                 enqueue_segment(ctx, 0, par, *options, so, nullptr, resort);
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
-                if (coloured) run_coloured_segment(ctx, par, *options, so, so.profile ? &kt : nullptr);
-                else enqueue_segment(ctx, 1, par, *options, so, so.profile ? &kt : nullptr);
+                if (need_init) run_world_init(ctx, par, *options, tk);
+                if (coloured) run_coloured_segment(ctx, par, *options, so, tk);
+                else enqueue_segment(ctx, 1, par, *options, so, tk);
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
+            }
+            if (coloured) {
+                ctx->init_set = -1;
+            } else {
+                ctx->init_set = 1 - par;
+                ctx->init_dt = options->dt;
+                for (int q = 0; q < 3; ++q) ctx->init_g[q] = options->gravity[q];
             }
             ctx->w_step += 1;
         }
@@ -1485,6 +1527,7 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
         ctx->tuning.primal_lanes = (uint32_t)best_lanes;
         ctx->tuning.broad_lanes = (uint32_t)best_blanes;
         ctx->drop_graphs();
+        ctx->init_set = -1;  // the timed launches wrote the solver buffers
         if (chosen) *chosen = ctx->tuning;
         return DIO_OK;
     });

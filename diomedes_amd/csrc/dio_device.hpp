@@ -21,6 +21,17 @@ struct AvbdParams {
     uint32_t n;  // body count
 };
 
+// Where an init writes (the solve's own buffers, or the other buffer set of a
+// world whose final launch prepares the next step's init).
+struct InitTarget {
+    float4* pos[2];
+    float4* rot[2];
+    float4* init_lin;
+    float4* init_ang;
+    float4* iner_lin;
+    float4* iner_ang;
+};
+
 // Everything the AVBD kernels touch (gpu_physics.rs bindings 0-10).
 struct SolveView {
     // current pose S_k (read by init) and the two Jacobi parity buffers
@@ -68,6 +79,9 @@ struct SolveView {
     // (one colour) instead of every body; pos[0] / pos[1] then alias
     const uint32_t* body_list;
     uint32_t list_count;
+    // when next.init_lin is set, the final launch (dual + recover) also runs
+    // the next step's init from the recovered state into these buffers
+    InitTarget next;
 };
 
 // Optional timing events of one launch: recorded by the dispatch itself at the

@@ -113,6 +113,46 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* in, u
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partials[tiles];
 }
 
+// One workgroup scans up to kSingleMax elements in tiles of kSingleTile with a
+// running carry: one launch instead of three for the per-body arrays.
+constexpr int kSingleThreads = 1024;
+constexpr int kSingleItems = 8;
+constexpr uint32_t kSingleTile = kSingleThreads * kSingleItems;
+constexpr uint32_t kSingleMax = 4 * kSingleTile;
+
+__global__ void __launch_bounds__(kSingleThreads) k_scan_single(const uint32_t* in, uint32_t* out, uint32_t n_cap,
+                                                                const uint32_t* n_dev) {
+    __shared__ uint32_t lds[kSingleThreads / kWave + 1];
+    const uint32_t n = read_n(n_cap, n_dev);
+    uint32_t carry = 0;
+    for (uint32_t tile = 0; tile < n; tile += kSingleTile) {
+        const uint32_t base = tile + threadIdx.x * kSingleItems;
+        uint32_t v[kSingleItems];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kSingleItems; ++k) {
+            v[k] = base + k < n ? in[base + k] : 0;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t ex = block_exclusive<kSingleThreads>(sum, lds, &total) + carry;
+#pragma unroll
+        for (int k = 0; k < kSingleItems; ++k) {
+            if (base + k < n) out[base + k] = ex;
+            ex += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_fill2_u32(uint32_t* a, uint32_t* b, uint32_t value, uint32_t n) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        a[i] = value;
+        b[i] = value;
+    }
+}
+
 // ------------------------------------------------------------ radix sort
 constexpr int kSortBlock = 256;
 constexpr int kSortItems = 8;
@@ -195,8 +235,19 @@ void launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s) {
 
 size_t scan_partials_size(uint32_t n_cap) { return ceil_div(n_cap > 0 ? n_cap : 1, kScanTile) + 1; }
 
+void launch_fill2_u32(uint32_t* a, uint32_t* b, uint32_t value, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    uint32_t blocks = ceil_div(n, 256);
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_fill2_u32, dim3(blocks), dim3(256), 0, s, a, b, value, n);
+}
+
 void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n_cap, const uint32_t* n_dev,
                            uint32_t* partials, hipStream_t s) {
+    if (n_cap <= kSingleMax) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kSingleThreads), 0, s, in, out, n_cap, n_dev);
+        return;
+    }
     const uint32_t tiles = ceil_div(n_cap > 0 ? n_cap : 1, kScanTile);
     hipLaunchKernelGGL(k_scan_partials, dim3(tiles), dim3(kScanBlock), 0, s, in, n_cap, n_dev, partials);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, partials, tiles);

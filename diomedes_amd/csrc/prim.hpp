@@ -11,10 +11,13 @@ namespace dio {
 
 // Fill n words (graph-safe replacement for hipMemsetAsync).
 void launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s);
+// Fill n words of two arrays in one launch.
+void launch_fill2_u32(uint32_t* a, uint32_t* b, uint32_t value, uint32_t n, hipStream_t s);
 
 // Exclusive scan of `in` (n elements; n read from *n_dev when n_dev is not
 // null, with n_cap the largest value it can take) into out[0..n]; out[n] is
-// the total.  `partials` needs scan_partials_size(n_cap) words.
+// the total.  `partials` needs scan_partials_size(n_cap) words.  Up to 32768
+// elements one workgroup scans in one launch; larger arrays take three.
 size_t scan_partials_size(uint32_t n_cap);
 void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n_cap, const uint32_t* n_dev,
                            uint32_t* partials, hipStream_t s);

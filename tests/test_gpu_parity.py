@@ -411,6 +411,33 @@ def test_wall_throw_small_bitexact(oracle, resort):
             _print("wall step", 15 * (k + 1), "pairs", ctx.stats().pairs, "contacts", ctx.stats().contacts)
 
 
+def test_options_change_between_calls(oracle):
+    """The final launch of a step prepares the next step's init for the same
+    dt and gravity; a call with other options, an autotune, or a coloured step
+    in between must not reuse it."""
+    import dataclasses
+
+    scene = scenes.cube_field(3, nx=5, ny=2, nz=5)
+    w = oracle.OracleWorld(scene, 32)
+    o1 = scene.options
+    o2 = dataclasses.replace(o1, dt=1.0 / 90.0)
+    o3 = dataclasses.replace(o1, gravity=(0.5, -9.0, 0.0))
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        for k, o in enumerate((o1, o2, o2, o3, o1, o1)):
+            ctx.step(o, 7)
+            w.step(o, nsteps=7)
+            _compare_world(ctx, w, 7 * (k + 1))
+            if k == 1:
+                ctx.autotune(o, reps=2)
+                ctx.set_tuning(primal_lanes=32, broad_lanes=8, fuse_dual=1)
+            if k == 3:
+                w.set_mode(1)
+                ctx.step(o, 3, mode=1)
+                w.step(o, nsteps=3)
+                w.set_mode(0)
+
+
 @pytest.mark.parametrize("fuse,lanes", [(0, 32), (1, 32), (1, 1)])
 def test_static_contacts_bitexact(oracle, fuse, lanes):
     """Static bodies in contact with each other and with dynamic bodies: in the
