@@ -714,6 +714,11 @@ __global__ void __launch_bounds__(kBlock) k_avbd_dual_recover(SolveView v, AvbdP
     const uint32_t stride = gridDim.x * kBlock;
     const uint32_t parity = (iters - 1) & 1u;
     const uint32_t src = fused ? parity : 0u;
+    if (v.stat_acc && blockIdx.x == 0 && threadIdx.x == 0) {
+        v.stat_acc[0] += *v.stat_pairs;
+        v.stat_acc[1] += m;
+        v.stat_acc[2] += v.offsets[p.n];
+    }
     for (uint32_t ci = blockIdx.x * kBlock + threadIdx.x; ci < m; ci += stride)
         dual_one(v, p, ci, parity, src, 0u, fused ? iters : 0u);
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < p.n; i += stride) recover_one(v, p, i, parity);

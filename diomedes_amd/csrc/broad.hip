@@ -355,10 +355,19 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_emit(uint32_t n, uint32_t D
                                                           const float4* amn, const float4* amx, const float4* nmn,
                                                           const float4* nmx, const uint32_t* offsets,
                                                           const uint32_t* cache, uint32_t p_max, uint2* pairs,
-                                                          SceneTable scenes) {
+                                                          SceneTable scenes, uint32_t* p_dev, uint32_t* overflow,
+                                                          uint32_t* clear0, uint32_t* clear1) {
     __shared__ float4 clo[kTopNodes], chi[kTopNodes];
     __shared__ uint32_t stack[kTravBlock / LANES][kStackDepth];
     __shared__ uint32_t list[kTravBlock / LANES][kPairCache];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the pair count min(total, p_max) for the narrow phase, and its counters
+        const uint32_t t = offsets[n];
+        *p_dev = t < p_max ? t : p_max;
+        if (t > p_max) atomicMax(overflow, t);
+        if (clear0) *clear0 = 0u;
+        if (clear1) *clear1 = 0u;
+    }
     uint32_t ncached;
     load_top(nmn, nmx, D, clo, chi, ncached);
     const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
@@ -450,7 +459,7 @@ void launch_tree_pairs(int mode, uint32_t n, uint32_t D, const BroadBuffers& b, 
         hipLaunchKernelGGL((k_tree_emit<LANES>), dim3(blocks), dim3(kTravBlock), 0, s, n, D, (const uint32_t*)b.perm,
                            (const float4*)b.aabb_min, (const float4*)b.aabb_max, (const float4*)b.node_min,
                            (const float4*)b.node_max, (const uint32_t*)b.offsets, (const uint32_t*)b.pair_cache,
-                           p_max, b.pairs, st);
+                           p_max, b.pairs, st, b.p_dev, b.overflow, b.clear_words[0], b.clear_words[1]);
 }
 void tree_pairs(int lanes, int mode, uint32_t n, uint32_t D, const BroadBuffers& b, uint32_t p_max, hipStream_t s,
                 const SceneTable& st) {
@@ -468,6 +477,8 @@ void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p
     if (n == 0) {
         launch_fill_u32(b.offsets, 0u, 1, s);
         hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets, p_max, b.p_dev, b.overflow);
+        if (b.clear_words[0]) launch_fill_u32(b.clear_words[0], 0u, 1, s);
+        if (b.clear_words[1]) launch_fill_u32(b.clear_words[1], 0u, 1, s);
         return;
     }
     if (resort) {
@@ -486,8 +497,7 @@ void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p
     if (D > 3) hipLaunchKernelGGL(k_tree_top, dim3(1), dim3(1024), 0, s, n, D, b.node_min, b.node_max);
     tree_pairs(lanes, 0, n, D, b, p_max, s, scenes);
     launch_scan_exclusive(b.counts, b.offsets, n, nullptr, b.scan_partials, s);
-    tree_pairs(lanes, 1, n, D, b, p_max, s, scenes);
-    hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets + n, p_max, b.p_dev, b.overflow);
+    tree_pairs(lanes, 1, n, D, b, p_max, s, scenes);  // also sets p_dev and clears clear_words
 }
 
 }  // namespace dio

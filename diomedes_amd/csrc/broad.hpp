@@ -36,6 +36,7 @@ struct BroadBuffers {
     uint32_t* p_dev;                          // pair count min(total, p_max)
     uint32_t* overflow;                       // largest total that exceeded p_max
     uint32_t* err;                            // traversal stack overflow flag
+    uint32_t* clear_words[2];                 // words the emit launch zeroes for the next stage (may be null)
     static size_t sort_scratch_words(uint32_t n_cap);
     static size_t tree_nodes(uint32_t n_cap);
 };

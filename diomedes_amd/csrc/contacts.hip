@@ -171,9 +171,13 @@ __device__ __forceinline__ void wall_contacts(const WorldView& w, uint32_t i, co
 }
 
 __global__ void __launch_bounds__(kBlock) k_container_count(WorldView w, ContainerParams ct, BuildOptions o,
-                                                            uint32_t* cnt) {
+                                                            uint32_t* cnt, uint32_t* zero_a, uint32_t* zero_b) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= w.n) return;
+    if (zero_a) {
+        zero_a[i] = 0u;
+        zero_b[i] = 0u;
+    }
     uint32_t c = 0;
     const ContainerParams cp = w.scene_ct ? w.scene_ct[w.scene_of[i]] : ct;
     if (cp.enabled && w.inv_mm[i].w != 0.0f)
@@ -301,18 +305,21 @@ __global__ void __launch_bounds__(kBlock) k_unpack_contacts(const dio_avbd_conta
 
 void run_contact_build(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const NarrowSource& src,
                        const ContactSet& dst, const NarrowSource& old_src, const ContactSet& old,
-                       uint32_t* scan_partials, uint32_t* overflow, hipStream_t s) {
+                       uint32_t* scan_partials, uint32_t* overflow, hipStream_t s, uint32_t* csr_cnt,
+                       uint32_t* csr_cur) {
     uint32_t pb = blocks_for(dst.m_cap > 0 ? dst.m_cap / 4 + 1 : 1);
     if (pb > 2048) pb = 2048;
     hipLaunchKernelGGL(k_contacts_narrow, dim3(pb), dim3(kBlock), 0, s, w, o, src, dst, old_src, old, overflow);
     const uint32_t nb = blocks_for(w.n > 0 ? w.n : 1);
-    hipLaunchKernelGGL(k_container_count, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst.cont_cnt);
+    hipLaunchKernelGGL(k_container_count, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst.cont_cnt, csr_cnt,
+                       csr_cnt ? csr_cur : nullptr);
     launch_scan_exclusive(dst.cont_cnt, dst.cont_off, w.n, nullptr, scan_partials, s);
     hipLaunchKernelGGL(k_container_write, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst, old, overflow);
 }
 
-void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s) {
-    launch_fill2_u32(b.cnt, b.cur, 0u, n, s);
+void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s,
+             bool counters_cleared) {
+    if (!counters_cleared) launch_fill2_u32(b.cnt, b.cur, 0u, n, s);
     uint32_t mb = blocks_for(cs.m_cap > 0 ? cs.m_cap : 1);
     if (mb > 2048) mb = 2048;
     hipLaunchKernelGGL(k_csr_count, dim3(mb), dim3(kBlock), 0, s, cs, b.cnt);

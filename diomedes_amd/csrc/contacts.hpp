@@ -83,10 +83,14 @@ void launch_csr_peers(uint32_t n, const uint32_t* off, const uint32_t* idx, cons
 // from `old` (built from `old_src`).  `overflow` records a capacity excess.
 void run_contact_build(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const NarrowSource& src,
                        const ContactSet& dst, const NarrowSource& old_src, const ContactSet& old,
-                       uint32_t* scan_partials, uint32_t* overflow, hipStream_t s);
+                       uint32_t* scan_partials, uint32_t* overflow, hipStream_t s, uint32_t* csr_cnt = nullptr,
+                       uint32_t* csr_cur = nullptr);  // csr_cnt / csr_cur (n words each): zeroed on the way
 
 // CSR of a contact set (physics/gpu.rs:286-310).
-void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s);
+// counters_cleared: b.cnt and b.cur were zeroed by run_contact_build (its
+// csr argument); otherwise they are zeroed here.
+void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s,
+             bool counters_cleared = false);
 
 // Unpack GpuAvbdContact records into SoA (granular path).
 void launch_unpack_contacts(const dio_avbd_contact* rec, uint32_t m, const ContactArrays& c, hipStream_t s);

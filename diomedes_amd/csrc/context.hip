@@ -537,13 +537,6 @@ void schedule_avbd_coloured(SolveView v, const AvbdParams& p, uint32_t iters, ui
     run(3, [&](const LaunchEvents& ev) { launch_avbd_dual_recover(v, p, iters, false, m_grid, s, ev); });
 }
 
-__global__ void k_stats_accum(const uint32_t* pairs, const uint32_t* m, const uint32_t* nnz, unsigned long long* acc) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        acc[0] += *pairs;
-        acc[1] += *m;
-        acc[2] += *nnz;
-    }
-}
 __global__ void k_stats_zero(unsigned long long* acc) {
     if (threadIdx.x < 4) acc[threadIdx.x] = 0ull;
 }
@@ -994,13 +987,15 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
             b.pairs = c->pend[dst_pend].pairs;
             b.p_dev = c->pend[dst_pend].p_dev;
             b.overflow = c->w_overflow;
+            b.clear_words[0] = c->pend[dst_pend].out.gjk_count;
+            b.clear_words[1] = c->pend[dst_pend].out.epa_retry_count;
             run_broad(bv, n, b, c->p_max, st, resort, (int)c->tuning.broad_lanes, scenes);
-            run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, st);
+            run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, st, true);
         };
         if (!lag) broad_narrow();
         run_contact_build(wv, c->w_ct, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]),
-                          c->cs[1 - s], c->scan_partials, c->w_overflow, st);
-        run_csr(c->cs[s], n, c->csr, c->w_overflow, st);
+                          c->cs[1 - s], c->scan_partials, c->w_overflow, st, c->csr.cnt, c->csr.cur);
+        run_csr(c->cs[s], n, c->csr, c->w_overflow, st, n > 0);
         if (c->adj_on)
             launch_record_step(c->adj_rec, c->cs[s], c->csr.off, c->csr.idx, c->csr.peer,
                                BodyState{c->w_pos, c->w_rot, c->w_vel, c->w_ang, c->w_prev}, n, c->m_cap, st);
@@ -1009,11 +1004,11 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         // the init of this step ran already (the previous step's final launch,
         // or dio_step before this segment)
         SolveView v = world_solve_view(c, s, true);
+        const int bn_pend = so.contact_lag ? 1 - s : s;
+        v.stat_pairs = c->pend[bn_pend].p_dev;
+        v.stat_acc = c->w_acc;
         const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
         schedule_avbd(v, p, o.iterations, c->m_cap, st, kt, c->tuning, false);
-        const int bn_pend = so.contact_lag ? 1 - s : s;
-        hipLaunchKernelGGL(k_stats_accum, dim3(1), dim3(64), 0, st, (const uint32_t*)c->pend[bn_pend].p_dev,
-                           (const uint32_t*)c->cs[s].m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
     }
 }
 
@@ -1065,11 +1060,11 @@ void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const di
     for (uint32_t k = 0; k < ncol; ++k) start[k + 1] = start[k] + h[k];
     c->last_colours = ncol;
     SolveView v = world_solve_view(c, s);
+    const int bn_pend = so.contact_lag ? 1 - s : s;
+    v.stat_pairs = c->pend[bn_pend].p_dev;
+    v.stat_acc = c->w_acc;
     const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
     schedule_avbd_coloured(v, p, o.iterations, c->m_cap, st, kt, c->tuning, c->wc.vals[lb], start);
-    const int bn_pend = so.contact_lag ? 1 - s : s;
-    hipLaunchKernelGGL(k_stats_accum, dim3(1), dim3(64), 0, st, (const uint32_t*)c->pend[bn_pend].p_dev,
-                       (const uint32_t*)c->cs[s].m_dev, (const uint32_t*)(c->csr.off + n), c->w_acc);
 }
 
 // Whether buffer set s holds this step's init, prepared for these options.

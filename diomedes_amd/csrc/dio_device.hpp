@@ -82,6 +82,10 @@ struct SolveView {
     // when next.init_lin is set, the final launch (dual + recover) also runs
     // the next step's init from the recovered state into these buffers
     InitTarget next;
+    // when stat_acc is set, the final launch adds the step's pair count
+    // (*stat_pairs), contact count and CSR entries to stat_acc[0..2]
+    const uint32_t* stat_pairs;
+    unsigned long long* stat_acc;
 };
 
 // Optional timing events of one launch: recorded by the dispatch itself at the

@@ -526,8 +526,8 @@ __global__ void __launch_bounds__(kBlock) k_pack_contact_out(NarrowOut o, uint32
 
 }  // namespace
 
-void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, hipStream_t s) {
-    launch_fill2_u32(o.gjk_count, o.epa_retry_count, 0u, 1, s);
+void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, hipStream_t s, bool counters_cleared) {
+    if (!counters_cleared) launch_fill2_u32(o.gjk_count, o.epa_retry_count, 0u, 1, s);
     uint32_t blocks = blocks_for(o.p_max > 0 ? o.p_max : 1);
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_narrow, dim3(blocks), dim3(kBlock), 0, s, bv, o, simplex_store);

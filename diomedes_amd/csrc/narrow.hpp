@@ -30,7 +30,10 @@ struct NarrowOut {
 // Narrow phase over pairs [0, *p_dev) and the scan of the per-pair counts;
 // the contact total is left in out.off[*p_dev].
 // simplex_store holds 8 float4 per pair (the GJK simplex handed to EPA).
-void run_narrow(const BodyView& bv, const NarrowOut& out, float4* simplex_store, hipStream_t s);
+// counters_cleared: gjk_count and epa_retry_count were zeroed by an earlier
+// launch on the stream (the broad phase's emit); otherwise they are zeroed here.
+void run_narrow(const BodyView& bv, const NarrowOut& out, float4* simplex_store, hipStream_t s,
+                bool counters_cleared = false);
 
 // Pack slots into the reference's GpuContactOut records (compute.rs:123-133).
 void launch_pack_contact_out(const NarrowOut& out, uint32_t pair_count, dio_gpu_contact_out* rec, hipStream_t s);
