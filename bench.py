@@ -105,6 +105,8 @@ def batched_leg(args, tuning: dict) -> dict:
         "body_iters_per_s_full_step": n_dyn * iters * k / wall,
         "last_step": {"pairs": st.pairs, "contacts": st.contacts, "csr_nnz": st.csr_nnz},
     }
+    if args.plain_sample <= 0:
+        return out
     # plain: a sample of the same scenes, each in its own context, stepped in turn
     step_ = max(1, len(batch.scenes) // max(1, args.plain_sample))
     sample = list(range(0, len(batch.scenes), step_))[: args.plain_sample]
@@ -379,6 +381,8 @@ def main() -> int:
                     help="steps of a separate run of the same scene after which dio_autotune picks the launch "
                          "shapes and dual schedule (0: library defaults)")
     ap.add_argument("--no-coloured", action="store_true", help="skip the coloured Gauss-Seidel leg")
+    ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint"), default=None,
+                    help="run only this leg and print its object (for profiling it alone)")
     ap.add_argument("--adjoint-seeds", type=int, default=64,
                     help="wall throws of the adjoint leg (0: skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=120, help="steps of the adjoint leg")
@@ -394,6 +398,12 @@ def main() -> int:
         return 2
 
     from diomedes_amd import PhysicsContext, scenes
+
+    if args.only_leg:
+        legs = {"coloured": lambda: coloured_leg(args, {}), "batched": lambda: batched_leg(args, {}),
+                "adjoint": lambda: adjoint_leg(args)}
+        print(json.dumps({"leg": args.only_leg, **legs[args.only_leg]()}), flush=True)
+        return 0
 
     if args.calibrate_copy_mib:
         import torch

@@ -286,19 +286,14 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
             doa = ang_diff(orr, as_q(v.init_ang[other]));
             r_other = qrot(orr, is_a ? rb : ra);
         }
-        const f3 j0 = cross3(r_world, r0), j1 = cross3(r_world, r1), j2 = cross3(r_world, r2);
         const float4 c04 = v.c_c0[ci];
-        float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
-        c0 = c0 + (dot3(r0, dsl) + dot3(j0, dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
-        c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
-        c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
         float4 pen = pen_in[ci];
         float4 lam = lam_in[ci];
         if (fuse) {
-            const DualState d = is_a ? dual_update(p, kk, n0, t1, t2, n4.w, r_world, r_other, dsl, dsa, dol, doa,
-                                                   c04, pen, lam)
-                                     : dual_update(p, kk, n0, t1, t2, n4.w, r_other, r_world, dol, doa, dsl, dsa,
-                                                   c04, pen, lam);
+            // dual update k - 1 of the contact, in its own (a, b) order
+            const DualState d = dual_update(p, kk, n0, t1, t2, n4.w, is_a ? r_world : r_other,
+                                            is_a ? r_other : r_world, is_a ? dsl : dol, is_a ? dsa : doa,
+                                            is_a ? dol : dsl, is_a ? doa : dsa, c04, pen, lam);
             pen = d.pen;
             lam = d.lam;
             if (is_a || v.inv_mm[other].w == 0.0f) {
@@ -306,6 +301,11 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
                 pen_out[ci] = pen;
             }
         }
+        const f3 j0 = cross3(r_world, r0), j1 = cross3(r_world, r1), j2 = cross3(r_world, r2);
+        float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
+        c0 = c0 + (dot3(r0, dsl) + dot3(j0, dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
+        c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
+        c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
         const Force3 F = contact_force(pen, lam, n4.w, c0, c1, c2);
         B.add_lin(r0, pen.x); B.add_lin(r1, pen.y); B.add_lin(r2, pen.z);
         B.add_ang(j0, pen.x); B.add_ang(j1, pen.y); B.add_ang(j2, pen.z);
@@ -385,11 +385,8 @@ constexpr int kRowFloats = 8;
 constexpr int kItemFloats = 3 * kRowFloats + 1;  // 3 rows + row count
 }  // namespace
 
-// Compiled for four waves per SIMD (at most 128 VGPRs): the fused variant
-// then keeps a few values in scratch, which measures faster than three waves.
 template <int G, bool FUSED>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t iter) {
+__global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t iter) {
     const uint32_t parity = iter & 1u;
     const bool fuse = FUSED && iter > 0;
     const float4* __restrict__ lam_in = lam_buf(v, fuse ? iter - 1 : 0);
@@ -515,18 +512,14 @@ k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t iter) {
                     doa = ang_diff(orr, as_q(v.init_ang[other]));
                     r_other = qrot(orr, is_a ? rb : ra);
                 }
-                const f3 j0 = cross3(r_world, r0), j1 = cross3(r_world, r1), j2 = cross3(r_world, r2);
                 const float4 c04 = v.c_c0[ci];
-                float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
-                c0 = c0 + (dot3(r0, dsl) + dot3(j0, dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
-                c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
-                c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
                 float4 pen = pen_in[ci];
                 float4 lam = lam_in[ci];
                 if (fuse) {
-                    const DualState d =
-                        is_a ? dual_update(p, kk, n0, t1, t2, n4.w, r_world, r_other, dsl, dsa, dol, doa, c04, pen, lam)
-                             : dual_update(p, kk, n0, t1, t2, n4.w, r_other, r_world, dol, doa, dsl, dsa, c04, pen, lam);
+                    // dual update k - 1 of the contact, in its own (a, b) order
+                    const DualState d = dual_update(p, kk, n0, t1, t2, n4.w, is_a ? r_world : r_other,
+                                                    is_a ? r_other : r_world, is_a ? dsl : dol, is_a ? dsa : doa,
+                                                    is_a ? dol : dsl, is_a ? doa : dsa, c04, pen, lam);
                     pen = d.pen;
                     lam = d.lam;
                     if (is_a || v.inv_mm[other].w == 0.0f) {
@@ -534,6 +527,11 @@ k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t iter) {
                         pen_out[ci] = pen;
                     }
                 }
+                const f3 j0 = cross3(r_world, r0), j1 = cross3(r_world, r1), j2 = cross3(r_world, r2);
+                float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
+                c0 = c0 + (dot3(r0, dsl) + dot3(j0, dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
+                c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
+                c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
                 const Force3 F = contact_force(pen, lam, n4.w, c0, c1, c2);
                 const f3 jl[3] = {r0, r1, r2};
                 const f3 ja[3] = {j0, j1, j2};
