@@ -464,8 +464,10 @@ def main() -> int:
     achieved = pbytes / (primal_ms / 1000.0) / 1e9
     step_bytes = avbd_bytes(n_dyn, n, iters, avg_m, avg_k)
     lanes = tuning["primal_lanes"]
-    kname = (f"k_avbd_primal_grouped<{lanes}, {'true' if tuning['fuse_dual'] else 'false'}>" if lanes > 1
-             else f"k_avbd_primal<{'true' if tuning['fuse_dual'] else 'false'}>")
+    # the step's primal launches: with the fused schedule the first of a step is
+    # <G, false> and the others <G, true>; the roofline averages over all of them
+    kname = (f"k_avbd_primal_grouped<{lanes}, {'*' if tuning['fuse_dual'] else 'false'}>" if lanes > 1
+             else f"k_avbd_primal<{'*' if tuning['fuse_dual'] else 'false'}>")
     traffic = measured_traffic(kname)
     line = {
         "metric": "AVBD body-iters/s & ms/step, 8k boxes",
