@@ -171,6 +171,38 @@ def coloured_leg(args, tuning: dict) -> dict:
     }
 
 
+def mixed_leg(args, tuning: dict) -> dict:
+    """The mixed workload: 20000 hulls (42-vertex icospheres), tetrahedra and
+    boxes with N(0, 2) velocities in a 60 m container, 128 ropes of 64 links and
+    512 springs; GJK + EPA for the convex pairs, dt = 1/120, 8 iterations."""
+    from diomedes_amd import PhysicsContext, scenes
+
+    scene = scenes.mixed_field(args.seed)
+    opts = scene.options
+    k = args.mixed_steps
+    with PhysicsContext(scene.n, max_hull_vertices=len(scene.hull), max_constraints=len(scene.constraints)) as ctx:
+        ctx.set_tuning(**tuning)
+        ctx.upload(scene)
+        if args.warmup:
+            ctx.step(opts, args.warmup)
+        ctx.step(opts, 0)
+        t0 = time.perf_counter()
+        ctx.step(opts, k)
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+    _progress(f"mixed leg: {k} steps in {wall:.2f} s")
+    return {
+        "workload": f"mixed field: 20000 convex bodies + 8192 rope links, {len(scene.constraints)} joints, "
+                    f"I={opts.iterations}, dt=1/120, {k} steps after {args.warmup}",
+        "bodies": scene.n,
+        "value": scene.n_dynamic * opts.iterations * k / (st.ms_avbd / 1000.0),
+        "unit": "body-iters/s",
+        "ms_per_step": 1000.0 * wall / k,
+        "ms_per_step_avbd": st.ms_avbd / k,
+        "last_step": {"pairs": st.pairs, "contacts": st.contacts, "narrow_contacts": st.narrow_contacts},
+    }
+
+
 def _wall_loss_grad(pos, targets, offsets):
     """Sum over seeds of the mean squared distance of the bricks' final
     centres to their targets, and its adjoint with respect to the final
@@ -381,7 +413,8 @@ def main() -> int:
                     help="steps of a separate run of the same scene after which dio_autotune picks the launch "
                          "shapes and dual schedule (0: library defaults)")
     ap.add_argument("--no-coloured", action="store_true", help="skip the coloured Gauss-Seidel leg")
-    ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint"), default=None,
+    ap.add_argument("--mixed-steps", type=int, default=300, help="steps of the mixed-field leg (0: skip it)")
+    ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint", "mixed"), default=None,
                     help="run only this leg and print its object (for profiling it alone)")
     ap.add_argument("--adjoint-seeds", type=int, default=64,
                     help="wall throws of the adjoint leg (0: skip it)")
@@ -401,7 +434,7 @@ def main() -> int:
 
     if args.only_leg:
         legs = {"coloured": lambda: coloured_leg(args, {}), "batched": lambda: batched_leg(args, {}),
-                "adjoint": lambda: adjoint_leg(args)}
+                "adjoint": lambda: adjoint_leg(args), "mixed": lambda: mixed_leg(args, {})}
         print(json.dumps({"leg": args.only_leg, **legs[args.only_leg]()}), flush=True)
         return 0
 
@@ -519,6 +552,8 @@ def main() -> int:
     }
     if not args.no_coloured:
         line["coloured"] = coloured_leg(args, tuning)
+    if args.mixed_steps:
+        line["mixed"] = mixed_leg(args, tuning)
     if args.batch_scenes:
         line["batched"] = batched_leg(args, tuning)
     if args.adjoint_seeds:

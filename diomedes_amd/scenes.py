@@ -215,8 +215,8 @@ def mixed_field(seed: int = 0, n: int = 20000, ropes: int = 128, rope_links: int
     for i in range(n):
         k = kinds[i]
         if k == 0:
+            # hull vertices are not scaled per body: each hull body has its own 42 vertices in the pool
             r = np.float32(0.3 + 0.3 * size[i])
-            first = len(hull_pool) * 0  # one shared unit pool scaled per body is not possible: store per body
             verts = (ico * r).astype(np.float32)
             bodies[i]["tag"] = R.HULL
             bodies[i]["corners"][0, 0] = np.float32(len(hull_pool) * 42)
@@ -225,7 +225,6 @@ def mixed_field(seed: int = 0, n: int = 20000, ropes: int = 128, rope_links: int
             mass = np.float32(4.0 / 3.0 * np.pi) * r * r * r
             inv_i = np.float32(1.0) / (np.float32(0.4) * mass * r * r)
             state[i]["inv_moment"][:3] = inv_i
-            del first
         elif k == 1:
             e = 0.5 + 0.5 * size[i]
             c = regular_tetrahedron(e).astype(np.float32)

@@ -245,6 +245,24 @@ def test_cube_field_8192_first_steps_bitexact(oracle):
             _print("8192 step", k, "pairs", s.pairs, "contacts", s.contacts)
 
 
+def test_mixed_field_20000_first_steps_bitexact(oracle):
+    """The full mixed workload: 20000 hulls, tetrahedra and boxes, 128 ropes of
+    64 links and 512 springs, GJK + EPA and joints."""
+    scene = scenes.mixed_field(0)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n, max_hull_vertices=len(scene.hull), max_constraints=len(scene.constraints)) as ctx:
+        ctx.upload(scene)
+        done = 0
+        for n in (30, 10, 1, 1):  # the bodies start apart; collisions begin after about 15 steps
+            ctx.step(scene.options, n)
+            w.step(nsteps=n)
+            done += n
+            _compare_world(ctx, w, done)
+            s = ctx.stats()
+            _print("mixed 20000 step", done, "pairs", s.pairs, "contacts", s.contacts)
+        assert s.contacts > 1000
+
+
 # ------------------------------------------------------------ golden fixtures
 def test_box_stack_matches_golden_fixture():
     """The committed FP32 trajectory (tests/golden/box_stack_s0.npz), bit-exact."""
