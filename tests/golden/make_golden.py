@@ -7,6 +7,8 @@ python tests/golden/make_golden.py
   step's contacts and pairs, from the FP32 and the FP64 oracle.
 - cube_field_8x2x8_s7.npz: 128 random cubes (seed 7), 120 steps, poses and
   velocities every 20 steps (FP32 and FP64).
+- box_stack_s0.dtraj: the 10-cube stack's FP32 trajectory as a scene and
+  trajectory file: frames every 10 steps with velocities and pairs.
 
 The fixtures pin the oracle build against regressions and let GPU tests check
 trajectories without recomputing them.  They are outputs of this build's
@@ -57,7 +59,29 @@ def write(name: str, scene, steps: int, every: int) -> None:
     print("wrote", HERE / name)
 
 
+def write_dtraj(name: str, scene, steps: int, every: int) -> None:
+    """The FP32 oracle's trajectory as a scene and trajectory file
+    (diomedes_amd.trajectory): frames every `every` steps with velocities and
+    the step's broad-phase pairs."""
+    from diomedes_amd import trajectory as T
+    from diomedes_amd import records as R
+
+    w = pyoracle.OracleWorld(scene, 32)
+    with T.Writer(HERE / name, scene, T.FLAG_VELOCITIES | T.FLAG_PAIRS) as out:
+        st = w.state()
+        out.frame(0, st["positions"], st["orientations"], st["velocities"], st["angular_velocities"],
+                  np.zeros((0, 2), np.uint32))
+        for k in range(steps // every):
+            w.step(nsteps=every)
+            st = w.state()
+            pr = w.pairs()
+            out.frame((k + 1) * every, st["positions"], st["orientations"], st["velocities"],
+                      st["angular_velocities"], pr)
+    print("wrote", HERE / name)
+
+
 if __name__ == "__main__":
     pyoracle.threads(1)
     write("box_stack_s0.npz", scenes.box_stack(0), 300, 10)
     write("cube_field_8x2x8_s7.npz", scenes.cube_field(7, nx=8, ny=2, nz=8), 120, 20)
+    write_dtraj("box_stack_s0.dtraj", scenes.box_stack(0), 300, 10)
