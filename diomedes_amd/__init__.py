@@ -14,6 +14,9 @@ from .physics import (
     AvbdOptions,
     AvbdSolver,
     BallSocket,
+    HardBallSocket,
+    HardRod,
+    HardRope,
     PhysicsContext,
     Rod,
     Rope,
@@ -33,6 +36,9 @@ __all__ = [
     "AvbdOptions",
     "AvbdSolver",
     "BallSocket",
+    "HardBallSocket",
+    "HardRod",
+    "HardRope",
     "PhysicsContext",
     "Rod",
     "Rope",

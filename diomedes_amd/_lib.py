@@ -61,6 +61,7 @@ SIGNATURES = {
     "dio_world_contacts": (c_int, [_VP, _VP, c_uint32, _U32P]),
     "dio_world_pairs": (c_int, [_VP, _VP, c_uint32, _U32P]),
     "dio_world_colours": (c_int, [_VP, _U32P, c_uint32, _U32P]),
+    "dio_world_joints": (c_int, [_VP, _FP, _FP, c_uint32, _U32P]),
 }
 
 _lib = None

@@ -167,6 +167,70 @@ __device__ __forceinline__ DualState dual_update(const AvbdParams& p, float kk, 
     return st;
 }
 
+// Hard joints (kHardRod, kHardSocket, kHardRope; no counterpart in the
+// reference's soft kinds).  The stabilised row values are C(x) - alpha C(x0)
+// from the anchor separation delta = p_a - p_b at the current poses and delta0
+// at the step's initial poses (a rope, an inequality C = |delta| - L <= 0,
+// subtracts only a violated start value); the row force is penalty * C^ +
+// lambda (kept >= 0 for a rope, which only pulls); a dual update sets lambda
+// to the row force and ramps an active row's penalty by beta |C^|, as contacts
+// do.  A rod or rope row is defined only when |delta| > 1e-6.
+struct HardRows {
+    f3 chat;  // C^ per row (x alone for a rod or rope)
+    f3 dir;   // delta / |delta| (rod, rope)
+    bool ok;
+};
+__device__ __forceinline__ HardRows hard_rows(uint32_t kind, f3 delta, f3 delta0, float rest, float alpha) {
+    HardRows r;
+    r.dir = f3{0.0f, 0.0f, 0.0f};
+    if (kind == kHardSocket) {
+        r.chat = f3{delta.x - alpha * delta0.x, delta.y - alpha * delta0.y, delta.z - alpha * delta0.z};
+        r.ok = true;
+    } else {
+        const float len = len3(delta);
+        const float len0 = len3(delta0);
+        r.ok = len > 1.0e-6f;
+        float c0 = len0 - rest;
+        if (kind == kHardRope) c0 = rmax(c0, 0.0f);
+        r.chat = f3{(len - rest) - alpha * c0, 0.0f, 0.0f};
+        if (r.ok) r.dir = divs(delta, len);
+    }
+    return r;
+}
+__device__ __forceinline__ float hard_force(uint32_t kind, float pen, float chat, float lam) {
+    const float f = pen * chat + lam;
+    return kind == kHardRope ? rmax(f, 0.0f) : f;
+}
+__device__ __forceinline__ void hard_row_update(const AvbdParams& p, uint32_t kind, float chat, float& lam,
+                                                float& pen) {
+    const float f = hard_force(kind, pen, chat, lam);
+    lam = f;
+    if (kind != kHardRope || f > 0.0f) pen = rmin(pen + p.beta_lin * fabsf(chat), p.penalty_max);
+}
+__device__ __forceinline__ void hard_update(const AvbdParams& p, uint32_t kind, const HardRows& r, float4& lam,
+                                            float4& pen) {
+    if (!r.ok) return;
+    hard_row_update(p, kind, r.chat.x, lam.x, pen.x);
+    if (kind == kHardSocket) {
+        hard_row_update(p, kind, r.chat.y, lam.y, pen.y);
+        hard_row_update(p, kind, r.chat.z, lam.z, pen.z);
+    }
+}
+// the warm start of a step (the contacts' rule, physics/gpu.rs:277-283):
+// lambda * alpha gamma, penalty * gamma clamped to [penalty_min, penalty_max]
+__device__ __forceinline__ void hard_warm(const AvbdParams& p, float4 lam, float4 pen, float4& lam_out,
+                                          float4& pen_out) {
+    const float ag = p.alpha * p.gamma;
+    lam_out = make_float4(lam.x * ag, lam.y * ag, lam.z * ag, 0.0f);
+    pen_out = make_float4(rmin(rmax(pen.x * p.gamma, p.penalty_min), p.penalty_max),
+                          rmin(rmax(pen.y * p.gamma, p.penalty_min), p.penalty_max),
+                          rmin(rmax(pen.z * p.gamma, p.penalty_min), p.penalty_max), 0.0f);
+}
+// anchor separation p_a - p_b of a joint from a pose buffer pair
+__device__ __forceinline__ f3 joint_delta(const float4* P, const float4* Q, uint4 abk, f3 aa, f3 ab) {
+    return sub(add(xyz(P[abk.x]), qrot(as_q(Q[abk.x]), aa)), add(xyz(P[abk.y]), qrot(as_q(Q[abk.y]), ab)));
+}
+
 // The multipliers after j dual updates live in buffer j % 2 (0: the contact
 // set's own lam / pen, 1: the scratch pair).  Primal iteration k >= 1 first
 // performs dual update k - 1 for each of its contacts (both bodies of a
@@ -216,8 +280,19 @@ __device__ __forceinline__ void init_body(const InitTarget& t, const AvbdParams&
     t.rot[0][i] = f4q(wr);
 }
 
+// the warm start of hard joint cj into buffer 0 (soft joints: nothing)
+__device__ __forceinline__ void warm_joint(const SolveView& v, const AvbdParams& p, uint32_t cj) {
+    if (!is_hard(v.j_abk[cj].z)) return;
+    float4 lam, pen;
+    hard_warm(p, v.j_st_lam[cj], v.j_st_pen[cj], lam, pen);
+    v.j_lam[0][cj] = lam;
+    v.j_pen[0][cj] = pen;
+}
+
 __global__ void __launch_bounds__(kBlock) k_avbd_init(SolveView v, AvbdParams p) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (v.j_hard)
+        for (uint32_t cj = i; cj < v.j_count; cj += gridDim.x * kBlock) warm_joint(v, p, cj);
     if (i >= p.n) return;
     const InitTarget t{{v.pos[0], v.pos[1]}, {v.rot[0], v.rot[1]}, v.init_lin, v.init_ang, v.iner_lin, v.iner_ang};
     init_body(t, p, i, v.cur_pos[i], v.cur_rot[i], xyz(v.vel[i]), xyz(v.ang[i]), xyz(v.prev_vel[i]), v.inv_mm[i].w);
@@ -231,6 +306,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
     const float4* __restrict__ pen_in = pen_buf(v, fuse ? iter - 1 : 0);
     float4* lam_out = lam_buf(v, iter);
     float4* pen_out = pen_buf(v, iter);
+    const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
     const uint32_t i = body_at(v, p, blockIdx.x * kBlock + threadIdx.x);
     if (i >= p.n) return;
     const float4 imm = v.inv_mm[i];
@@ -339,7 +415,34 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
             const f3 delta = sub(p_a, p_b);
             const f3 r_self = qrot(snap_rot, self_anchor);
             const float stiff = ab4.w;
-            if (abk.z == kBallSocket) {
+            if (is_hard(abk.z)) {
+                const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa, abn);
+                const HardRows hr = hard_rows(abk.z, delta, d0, aa4.w, p.alpha);
+                float4 lam = v.j_lam[jin][cidx];
+                float4 pen = v.j_pen[jin][cidx];
+                if (fuse) {
+                    // dual update k - 1 of the joint (the same on both bodies)
+                    hard_update(p, abk.z, hr, lam, pen);
+                    if (is_a || v.inv_mm[abk.x].w == 0.0f) {
+                        v.j_lam[iter & 1u][cidx] = lam;
+                        v.j_pen[iter & 1u][cidx] = pen;
+                    }
+                }
+                if (abk.z == kHardSocket) {
+                    const f3 jx = scale(f3{1.0f, 0.0f, 0.0f}, sign);
+                    const f3 jy = scale(f3{0.0f, 1.0f, 0.0f}, sign);
+                    const f3 jz = scale(f3{0.0f, 0.0f, 1.0f}, sign);
+                    B.row(jx, cross3(r_self, jx), hard_force(abk.z, pen.x, hr.chat.x, lam.x), pen.x);
+                    B.row(jy, cross3(r_self, jy), hard_force(abk.z, pen.y, hr.chat.y, lam.y), pen.y);
+                    B.row(jz, cross3(r_self, jz), hard_force(abk.z, pen.z, hr.chat.z, lam.z), pen.z);
+                } else if (hr.ok) {
+                    const float f = hard_force(abk.z, pen.x, hr.chat.x, lam.x);
+                    if (abk.z != kHardRope || f > 0.0f) {
+                        const f3 jl = scale(hr.dir, sign);
+                        B.row(jl, cross3(r_self, jl), f, pen.x);
+                    }
+                }
+            } else if (abk.z == kBallSocket) {
                 const f3 jx = scale(f3{1.0f, 0.0f, 0.0f}, sign);
                 const f3 jy = scale(f3{0.0f, 1.0f, 0.0f}, sign);
                 const f3 jz = scale(f3{0.0f, 0.0f, 1.0f}, sign);
@@ -385,7 +488,9 @@ constexpr int kRowFloats = 8;
 constexpr int kItemFloats = 3 * kRowFloats + 1;  // 3 rows + row count
 }  // namespace
 
-template <int G, bool FUSED>
+// HARD: the world has hard joints (their rows and fused dual updates are
+// compiled only into that instantiation).
+template <int G, bool FUSED, bool HARD>
 __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, AvbdParams p, uint32_t iter) {
     const uint32_t parity = iter & 1u;
     const bool fuse = FUSED && iter > 0;
@@ -559,7 +664,47 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
                 const f3 r_self = qrot(snap_rot, is_a ? aa : abn);
                 const float sign = is_a ? 1.0f : -1.0f;
                 const float stiff = ab4.w;
-                if (abk.z == kBallSocket) {
+                if (HARD && is_hard(abk.z)) {
+                    const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa, abn);
+                    const HardRows hr = hard_rows(abk.z, delta, d0, aa4.w, p.alpha);
+                    const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
+                    float4 lam = v.j_lam[jin][cidx];
+                    float4 pen = v.j_pen[jin][cidx];
+                    if (fuse) {
+                        // dual update k - 1 of the joint (the same on both bodies)
+                        hard_update(p, abk.z, hr, lam, pen);
+                        if (is_a || v.inv_mm[abk.x].w == 0.0f) {
+                            v.j_lam[iter & 1u][cidx] = lam;
+                            v.j_pen[iter & 1u][cidx] = pen;
+                        }
+                    }
+                    if (abk.z == kHardSocket) {
+                        const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
+                                          scale(f3{0.0f, 0.0f, 1.0f}, sign)};
+                        const float fs[3] = {hard_force(abk.z, pen.x, hr.chat.x, lam.x),
+                                             hard_force(abk.z, pen.y, hr.chat.y, lam.y),
+                                             hard_force(abk.z, pen.z, hr.chat.z, lam.z)};
+                        const float ks[3] = {pen.x, pen.y, pen.z};
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            const f3 ja = cross3(r_self, js[r]);
+                            st[8 * r + 0] = js[r].x; st[8 * r + 1] = js[r].y; st[8 * r + 2] = js[r].z;
+                            st[8 * r + 3] = ja.x; st[8 * r + 4] = ja.y; st[8 * r + 5] = ja.z;
+                            st[8 * r + 6] = fs[r]; st[8 * r + 7] = ks[r];
+                        }
+                        nrows = 3;
+                    } else if (hr.ok) {
+                        const float f = hard_force(abk.z, pen.x, hr.chat.x, lam.x);
+                        if (abk.z != kHardRope || f > 0.0f) {
+                            const f3 jl = scale(hr.dir, sign);
+                            const f3 ja = cross3(r_self, jl);
+                            st[0] = jl.x; st[1] = jl.y; st[2] = jl.z;
+                            st[3] = ja.x; st[4] = ja.y; st[5] = ja.z;
+                            st[6] = f; st[7] = pen.x;
+                            nrows = 1;
+                        }
+                    }
+                } else if (abk.z == kBallSocket) {
                     const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
                                       scale(f3{0.0f, 0.0f, 1.0f}, sign)};
                     const float fs[3] = {delta.x * stiff, delta.y * stiff, delta.z * stiff};
@@ -687,12 +832,45 @@ __device__ __forceinline__ void dual_one(const SolveView& v, const AvbdParams& p
     pen_buf(v, dst)[ci] = d.pen;
 }
 
+// One dual update of hard joint cj from the poses pos[1 - parity], from
+// multiplier buffer src (static_updates > 0: a joint between two static
+// bodies starts from buffer 0 and takes that many, as dual_one).
+struct JointMults {
+    float4 lam, pen;
+};
+__device__ __forceinline__ JointMults joint_dual_one(const SolveView& v, const AvbdParams& p, uint32_t cj,
+                                                     uint32_t parity, uint32_t src, uint32_t static_updates) {
+    const uint4 abk = v.j_abk[cj];
+    const f3 aa = xyz(v.j_anc_a[cj]);
+    const float4 ab4 = v.j_anc_b[cj];
+    const float rest = v.j_anc_a[cj].w;
+    const f3 ab = xyz(ab4);
+    const f3 delta = joint_delta(v.pos[1 - parity], v.rot[1 - parity], abk, aa, ab);
+    const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa, ab);
+    const HardRows hr = hard_rows(abk.z, delta, d0, rest, p.alpha);
+    uint32_t reps = 1, from = src;
+    if (static_updates > 0u && v.inv_mm[abk.x].w == 0.0f && v.inv_mm[abk.y].w == 0.0f) {
+        reps = static_updates;
+        from = 0;
+    }
+    JointMults m{v.j_lam[from][cj], v.j_pen[from][cj]};
+    for (uint32_t k = 0; k < reps; ++k) hard_update(p, abk.z, hr, m.lam, m.pen);
+    return m;
+}
+
 // standalone dual update (unfused schedule)
 __global__ void __launch_bounds__(kBlock) k_avbd_dual(SolveView v, AvbdParams p, uint32_t parity, uint32_t src,
                                                       uint32_t dst) {
     const uint32_t m = v.m_dev ? *v.m_dev : v.m_host;
     for (uint32_t ci = blockIdx.x * kBlock + threadIdx.x; ci < m; ci += gridDim.x * kBlock)
         dual_one(v, p, ci, parity, src, dst, 0u);
+    if (v.j_hard)
+        for (uint32_t cj = blockIdx.x * kBlock + threadIdx.x; cj < v.j_count; cj += gridDim.x * kBlock) {
+            if (!is_hard(v.j_abk[cj].z)) continue;
+            const JointMults jm = joint_dual_one(v, p, cj, parity, src, 0u);
+            v.j_lam[dst][cj] = jm.lam;
+            v.j_pen[dst][cj] = jm.pen;
+        }
 }
 
 __device__ __forceinline__ void recover_one(const SolveView& v, const AvbdParams& p, uint32_t i, uint32_t parity);
@@ -719,6 +897,21 @@ __global__ void __launch_bounds__(kBlock) k_avbd_dual_recover(SolveView v, AvbdP
     }
     for (uint32_t ci = blockIdx.x * kBlock + threadIdx.x; ci < m; ci += stride)
         dual_one(v, p, ci, parity, src, 0u, fused ? iters : 0u);
+    // hard joints: the last dual update into the state carried to the next
+    // step, and that step's warm start when this launch prepares its init
+    if (v.j_hard)
+        for (uint32_t cj = blockIdx.x * kBlock + threadIdx.x; cj < v.j_count; cj += stride) {
+            if (!is_hard(v.j_abk[cj].z)) continue;
+            const JointMults jm = joint_dual_one(v, p, cj, parity, src, fused ? iters : 0u);
+            v.j_st_lam[cj] = jm.lam;
+            v.j_st_pen[cj] = jm.pen;
+            if (v.next.init_lin) {
+                float4 lam, pen;
+                hard_warm(p, jm.lam, jm.pen, lam, pen);
+                v.j_lam[0][cj] = lam;
+                v.j_pen[0][cj] = pen;
+            }
+        }
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < p.n; i += stride) recover_one(v, p, i, parity);
 }
 
@@ -769,33 +962,39 @@ void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t s, co
     launch_k(k_avbd_init, dim3(blocks_for(p.n)), dim3(kBlock), s, ev, v, p);
 }
 namespace {
-template <bool FUSED>
-void launch_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, hipStream_t s, int lanes,
-                   const LaunchEvents& ev, uint32_t work) {
+template <bool FUSED, bool HARD>
+void launch_primal_t(const SolveView& v, const AvbdParams& p, uint32_t iter, hipStream_t s, int lanes,
+                     const LaunchEvents& ev, uint32_t work) {
     switch (lanes) {
         case 4:
-            launch_k(k_avbd_primal_grouped<4, FUSED>, dim3((work + kBlock / 4 - 1) / (kBlock / 4)), dim3(kBlock), s, ev,
+            launch_k(k_avbd_primal_grouped<4, FUSED, HARD>, dim3((work + kBlock / 4 - 1) / (kBlock / 4)), dim3(kBlock), s, ev,
                      v, p, iter);
             break;
         case 8:
-            launch_k(k_avbd_primal_grouped<8, FUSED>, dim3((work + kBlock / 8 - 1) / (kBlock / 8)), dim3(kBlock), s, ev,
+            launch_k(k_avbd_primal_grouped<8, FUSED, HARD>, dim3((work + kBlock / 8 - 1) / (kBlock / 8)), dim3(kBlock), s, ev,
                      v, p, iter);
             break;
         case 16:
-            launch_k(k_avbd_primal_grouped<16, FUSED>, dim3((work + kBlock / 16 - 1) / (kBlock / 16)), dim3(kBlock), s,
+            launch_k(k_avbd_primal_grouped<16, FUSED, HARD>, dim3((work + kBlock / 16 - 1) / (kBlock / 16)), dim3(kBlock), s,
                      ev, v, p, iter);
             break;
         case 32:
-            launch_k(k_avbd_primal_grouped<32, FUSED>, dim3((work + kBlock / 32 - 1) / (kBlock / 32)), dim3(kBlock), s,
+            launch_k(k_avbd_primal_grouped<32, FUSED, HARD>, dim3((work + kBlock / 32 - 1) / (kBlock / 32)), dim3(kBlock), s,
                      ev, v, p, iter);
             break;
         case 64:
-            launch_k(k_avbd_primal_grouped<64, FUSED>, dim3((work + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), s,
+            launch_k(k_avbd_primal_grouped<64, FUSED, HARD>, dim3((work + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), s,
                      ev, v, p, iter);
             break;
         default:
             launch_k(k_avbd_primal<FUSED>, dim3(blocks_for(work)), dim3(kBlock), s, ev, v, p, iter);
     }
+}
+template <bool FUSED>
+void launch_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, hipStream_t s, int lanes,
+                   const LaunchEvents& ev, uint32_t work) {
+    if (v.j_hard) launch_primal_t<FUSED, true>(v, p, iter, s, lanes, ev, work);
+    else launch_primal_t<FUSED, false>(v, p, iter, s, lanes, ev, work);
 }
 }  // namespace
 

@@ -164,6 +164,14 @@ struct JointDev {
     uint32_t* off = nullptr;
     uint32_t* idx = nullptr;
     uint32_t c = 0;
+    // hard joints (world only): multipliers of a solve (two buffers) and the
+    // state carried between steps; the hard joints, and those between two
+    // dynamic bodies
+    float4* lam[2] = {nullptr, nullptr};
+    float4* pen[2] = {nullptr, nullptr};
+    float4* st_lam = nullptr;
+    float4* st_pen = nullptr;
+    uint32_t hard = 0, hard_dynamic = 0;
 };
 
 }  // namespace
@@ -259,6 +267,8 @@ struct dio_ctx {
     InitTarget ss[2]{};
     int init_set = -1;
     float init_dt = 0.0f, init_g[3] = {0.0f, 0.0f, 0.0f};
+    // and, with hard joints, the options of their warm start
+    float init_alpha = 0.0f, init_gamma = 0.0f, init_pmin = 0.0f, init_pmax = 0.0f;
     uint32_t* w_overflow = nullptr;
     unsigned long long* w_acc = nullptr;  // sums of pairs, contacts, CSR nnz over a dio_step call
     float4 *w_lam_alt = nullptr, *w_pen_alt = nullptr;  // scratch multipliers of a solve
@@ -450,8 +460,11 @@ void upload_joints(dio_ctx* c, JointDev& J, const dio_avbd_constraint* cons, uin
     DIO_HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
-AvbdParams params_of(float dt, const float g[3], float alpha, float beta, float pmax, uint32_t n) {
+AvbdParams params_of(float dt, const float g[3], float alpha, float beta, float pmax, uint32_t n,
+                     float gamma = 0.999f, float pmin = 1.0f) {
     AvbdParams p;
+    p.gamma = gamma;
+    p.penalty_min = pmin;
     p.dt = dt;
     p.alpha = alpha;
     p.beta_lin = beta;
@@ -785,6 +798,9 @@ dio_status dio_avbd_submit(dio_ctx* ctx, const dio_avbd_body_state* state, const
         for (uint32_t k = 0; k < m; ++k)
             if (contacts[k].a >= n || (contacts[k].b != DIO_CONTAINER && contacts[k].b >= n))
                 invalid("contact body index out of range");
+        for (uint32_t k = 0; k < c; ++k)
+            if (is_hard(constraints[k].kind))
+                invalid("hard joints need a world (dio_world_upload): their multipliers persist between steps");
         for (uint32_t k = 0; k < nidx; ++k)
             if (indices[k] >= m) invalid("CSR index out of range");
         for (uint32_t i = 0; i < n; ++i)
@@ -902,6 +918,12 @@ void ensure_world(dio_ctx* c) {
     c->w_j.anc_b = A.alloc<float4>(c->c_cap);
     c->w_j.off = A.alloc<uint32_t>((size_t)n + 1);
     c->w_j.idx = A.alloc<uint32_t>(2 * (size_t)c->c_cap);
+    for (int k = 0; k < 2; ++k) {
+        c->w_j.lam[k] = A.alloc<float4>(c->c_cap);
+        c->w_j.pen[k] = A.alloc<float4>(c->c_cap);
+    }
+    c->w_j.st_lam = A.alloc<float4>(c->c_cap);
+    c->w_j.st_pen = A.alloc<float4>(c->c_cap);
     c->wb = alloc_broad(A, n, p);
     for (int k = 0; k < 2; ++k) {
         auto& pd = c->pend[k];
@@ -1007,7 +1029,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         const int bn_pend = so.contact_lag ? 1 - s : s;
         v.stat_pairs = c->pend[bn_pend].p_dev;
         v.stat_acc = c->w_acc;
-        const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
+        const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n, o.gamma, o.penalty_min);
         schedule_avbd(v, p, o.iterations, c->m_cap, st, kt, c->tuning, false);
     }
 }
@@ -1063,20 +1085,23 @@ void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const di
     const int bn_pend = so.contact_lag ? 1 - s : s;
     v.stat_pairs = c->pend[bn_pend].p_dev;
     v.stat_acc = c->w_acc;
-    const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n);
+    const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n, o.gamma, o.penalty_min);
     schedule_avbd_coloured(v, p, o.iterations, c->m_cap, st, kt, c->tuning, c->wc.vals[lb], start);
 }
 
 // Whether buffer set s holds this step's init, prepared for these options.
 bool init_prepared(const dio_ctx* c, int s, const dio_avbd_options& o) {
     return c->init_set == s && c->init_dt == o.dt && c->init_g[0] == o.gravity[0] && c->init_g[1] == o.gravity[1] &&
-           c->init_g[2] == o.gravity[2];
+           c->init_g[2] == o.gravity[2] &&
+           (c->w_j.hard == 0 || (c->init_alpha == o.alpha && c->init_gamma == o.gamma &&
+                                 c->init_pmin == o.penalty_min && c->init_pmax == o.penalty_max));
 }
 
 // The standalone init of buffer set s from the world state.
 void run_world_init(dio_ctx* c, int s, const dio_avbd_options& o, KernelTimer* kt) {
     const SolveView v = world_solve_view(c, s);
-    const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, c->w_n);
+    const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, c->w_n, o.gamma,
+                                   o.penalty_min);
     if (kt) kt->time(0, [&](const LaunchEvents& ev) { launch_avbd_init(v, p, c->stream, ev); });
     else launch_avbd_init(v, p, c->stream);
 }
@@ -1104,6 +1129,13 @@ SolveView world_solve_view(dio_ctx* c, int s, bool prepare_next) {
         v.peers = c->csr.peer;
         v.j_off = c->w_j.off; v.j_idx = c->w_j.idx; v.j_abk = c->w_j.abk;
         v.j_anc_a = c->w_j.anc_a; v.j_anc_b = c->w_j.anc_b;
+        v.j_count = c->w_j.c;
+        if (c->w_j.hard > 0) {
+            v.j_hard = 1;
+            v.j_lam[0] = c->w_j.lam[0]; v.j_lam[1] = c->w_j.lam[1];
+            v.j_pen[0] = c->w_j.pen[0]; v.j_pen[1] = c->w_j.pen[1];
+            v.j_st_lam = c->w_j.st_lam; v.j_st_pen = c->w_j.st_pen;
+        }
         v.out_pos = c->w_pos;
         v.out_rot = c->w_rot;
         return v;
@@ -1208,6 +1240,22 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
         DIO_HIP_CHECK(hipStreamSynchronize(s));
     }
     upload_joints(ctx, ctx->w_j, constraints, c, n, ctx->w_stage_c);
+    ctx->w_j.hard = 0;
+    ctx->w_j.hard_dynamic = 0;
+    for (uint32_t k = 0; k < c; ++k) {
+        if (!is_hard(constraints[k].kind)) continue;
+        ctx->w_j.hard += 1;
+        if (state[constraints[k].a].inv_mass != 0.0f && state[constraints[k].b].inv_mass != 0.0f &&
+            constraints[k].a != constraints[k].b)
+            ctx->w_j.hard_dynamic += 1;
+    }
+    if (c > 0) {
+        // hard joints start with no force and the minimum penalty (the warm
+        // start clamps a zero penalty up to penalty_min)
+        DIO_HIP_CHECK(hipMemsetAsync(ctx->w_j.st_lam, 0, (size_t)c * sizeof(float4), s));
+        DIO_HIP_CHECK(hipMemsetAsync(ctx->w_j.st_pen, 0, (size_t)c * sizeof(float4), s));
+        DIO_HIP_CHECK(hipStreamSynchronize(s));
+    }
     ctx->w_num_scenes = 1;
     ctx->w_scene_bits = 0;
     if (scene_offsets && num_scenes > 1) {
@@ -1285,6 +1333,9 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         if (so.mode > 1) invalid("mode must be 0 (Jacobi) or 1 (coloured Gauss-Seidel)");
         const bool coloured = so.mode == 1;
         if (coloured && ctx->adj_on) invalid("the adjoint records Jacobi steps (mode 0) only");
+        if (!coloured && ctx->w_j.hard_dynamic > 0)
+            invalid("hard joints between two dynamic bodies need the coloured mode (1): the Jacobi solve "
+                    "diverges on them");
         ctx->last_lag = so.contact_lag ? 1 : 0;
         hipStream_t s = ctx->stream;
         DIO_HIP_CHECK(hipSetDevice(ctx->device));
@@ -1337,6 +1388,10 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
                 ctx->init_set = 1 - par;
                 ctx->init_dt = options->dt;
                 for (int q = 0; q < 3; ++q) ctx->init_g[q] = options->gravity[q];
+                ctx->init_alpha = options->alpha;
+                ctx->init_gamma = options->gamma;
+                ctx->init_pmin = options->penalty_min;
+                ctx->init_pmax = options->penalty_max;
             }
             ctx->w_step += 1;
         }
@@ -1611,6 +1666,34 @@ dio_status dio_world_colours(dio_ctx* ctx, uint32_t* colour, uint32_t cap, uint3
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         if (ctx->w_n > 0)
             DIO_HIP_CHECK(hipMemcpy(colour, ctx->wc.colour, (size_t)ctx->w_n * 4, hipMemcpyDeviceToHost));
+        return DIO_OK;
+    });
+}
+
+dio_status dio_world_joints(dio_ctx* ctx, float* lambda, float* penalty, uint32_t cap, uint32_t* c_out) {
+    return guarded(ctx, [&]() {
+        if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
+        const uint32_t c = ctx->w_j.c;
+        if (c_out) *c_out = c;
+        if (!lambda && !penalty) return DIO_OK;
+        if (cap < c) capacity("joint buffer too small", c);
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        std::vector<float4> lam(c), pen(c);
+        if (c > 0) {
+            DIO_HIP_CHECK(hipMemcpy(lam.data(), ctx->w_j.st_lam, (size_t)c * sizeof(float4), hipMemcpyDeviceToHost));
+            DIO_HIP_CHECK(hipMemcpy(pen.data(), ctx->w_j.st_pen, (size_t)c * sizeof(float4), hipMemcpyDeviceToHost));
+        }
+        // the state arrays hold values for hard joints only
+        std::vector<uint4> abk(c);
+        if (c > 0)
+            DIO_HIP_CHECK(hipMemcpy(abk.data(), ctx->w_j.abk, (size_t)c * sizeof(uint4), hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < c; ++k) {
+            const bool h = is_hard(abk[k].z);
+            const float4 l = h ? lam[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float4 q = h ? pen[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (lambda) { lambda[3 * k] = l.x; lambda[3 * k + 1] = l.y; lambda[3 * k + 2] = l.z; }
+            if (penalty) { penalty[3 * k] = q.x; penalty[3 * k + 1] = q.y; penalty[3 * k + 2] = q.z; }
+        }
         return DIO_OK;
     });
 }

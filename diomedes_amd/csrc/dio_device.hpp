@@ -19,6 +19,9 @@ struct AvbdParams {
     float dt, alpha, beta_lin, penalty_max;
     float gx, gy, gz;
     uint32_t n;  // body count
+    // the warm start of hard joints' multipliers (AvbdOptions gamma and
+    // penalty_min, physics/gpu.rs:277-283)
+    float gamma, penalty_min;
 };
 
 // Where an init writes (the solve's own buffers, or the other buffer set of a
@@ -72,6 +75,16 @@ struct SolveView {
     const uint4* j_abk;      // a, b, kind, 0
     const float4* j_anc_a;   // anchor_a.xyz, rest / max length
     const float4* j_anc_b;   // anchor_b.xyz, stiffness
+    // hard joints' multipliers (.xyz: one per row), indexed by constraint:
+    // j_lam / j_pen[j % 2] after j dual updates of this step (as contacts'
+    // lam / pen and their scratch pair), and the state carried between steps
+    // (j_st_*, read by the init's warm start and written by the final launch)
+    float4* j_lam[2];
+    float4* j_pen[2];
+    float4* j_st_lam;
+    float4* j_st_pen;
+    uint32_t j_count;  // constraints
+    uint32_t j_hard;   // nonzero: some constraint is a hard joint
     // fused path: recover writes the final pose back into the world pose
     float4* out_pos;
     float4* out_rot;

@@ -23,6 +23,10 @@ constexpr uint32_t kContainer = 0xFFFFFFFFu;
 constexpr uint32_t kPeerIsB = 0x80000000u;
 constexpr uint32_t kSphere = 0, kCube = 1, kTetrahedron = 2, kBox = 3, kHull = 4;
 constexpr uint32_t kSpring = 0, kRod = 1, kBallSocket = 2, kRope = 3;
+// hard joints: rod, socket and rope geometry held by a multiplier and a
+// ramped penalty per row (no counterpart in the reference's soft kinds)
+constexpr uint32_t kHardRod = 4, kHardSocket = 5, kHardRope = 6;
+DIO_HD bool is_hard(uint32_t kind) { return kind >= kHardRod && kind <= kHardRope; }
 
 struct f3 {
     float x, y, z;

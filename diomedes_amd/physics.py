@@ -241,6 +241,46 @@ class Rope:
     name = "rope"
 
 
+@dataclass
+class HardRod:
+    """A rod held at rest_length by a multiplier (no counterpart in the
+    reference, whose Rod is a stiff spring)."""
+    body_a: int
+    body_b: int
+    anchor_a: tuple
+    anchor_b: tuple
+    rest_length: float
+    kind = R.HARD_ROD
+    name = "hard rod"
+    stiffness = 0.0
+
+
+@dataclass
+class HardBallSocket:
+    """Two anchors held together by three multipliers."""
+    body_a: int
+    body_b: int
+    anchor_a: tuple
+    anchor_b: tuple
+    kind = R.HARD_SOCKET
+    name = "hard ball and socket"
+    stiffness = 0.0
+
+
+@dataclass
+class HardRope:
+    """At most max_length between the anchors, held by a multiplier that only
+    pulls."""
+    body_a: int
+    body_b: int
+    anchor_a: tuple
+    anchor_b: tuple
+    max_length: float
+    kind = R.HARD_ROPE
+    name = "hard rope"
+    stiffness = 0.0
+
+
 def constraints_from_avbd(constraints: Iterable) -> np.ndarray:
     """gpu_physics.rs:149-193: anchor_a.w = rest / max length (0 for sockets),
     anchor_b.w = stiffness."""
@@ -250,9 +290,9 @@ def constraints_from_avbd(constraints: Iterable) -> np.ndarray:
         out[i]["a"], out[i]["b"], out[i]["kind"] = c.body_a, c.body_b, c.kind
         out[i]["anchor_a"][:3] = c.anchor_a
         out[i]["anchor_b"][:3] = c.anchor_b
-        if isinstance(c, (Spring, Rod)):
+        if isinstance(c, (Spring, Rod, HardRod)):
             out[i]["anchor_a"][3] = c.rest_length
-        elif isinstance(c, Rope):
+        elif isinstance(c, (Rope, HardRope)):
             out[i]["anchor_a"][3] = c.max_length
         out[i]["anchor_b"][3] = c.stiffness
     return out
@@ -612,6 +652,17 @@ class PhysicsContext:
         nc = ctypes.c_uint32()
         self._check(self._lib.dio_world_colours(self._h, _lib.ptr(out, ctypes.c_uint32), self._n, ctypes.byref(nc)))
         return out, nc.value
+
+    def joints(self) -> tuple:
+        """Each constraint's multipliers after the last step: lambda and
+        penalty per row, (c, 3) float32 each (zero for soft joints)."""
+        c = ctypes.c_uint32()
+        self._check(self._lib.dio_world_joints(self._h, None, None, 0, ctypes.byref(c)))
+        lam = np.zeros((c.value, 3), np.float32)
+        pen = np.zeros((c.value, 3), np.float32)
+        self._check(self._lib.dio_world_joints(self._h, _lib.ptr(lam, ctypes.c_float), _lib.ptr(pen, ctypes.c_float),
+                                               c.value, ctypes.byref(c)))
+        return lam, pen
 
     def pairs(self) -> np.ndarray:
         n = ctypes.c_uint32()

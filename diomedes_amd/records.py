@@ -15,6 +15,9 @@ CONTAINER = 0xFFFFFFFF
 
 SPHERE, CUBE, TETRAHEDRON, BOX, HULL = 0, 1, 2, 3, 4
 SPRING, ROD, BALL_SOCKET, ROPE = 0, 1, 2, 3
+# hard joints: the rod, socket and rope geometry held by a multiplier and a
+# ramped penalty per row (the stiffness field is unused)
+HARD_ROD, HARD_SOCKET, HARD_ROPE = 4, 5, 6
 
 GPU_BODY = np.dtype(
     [

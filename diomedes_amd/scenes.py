@@ -282,6 +282,68 @@ def mixed_field(seed: int = 0, n: int = 20000, ropes: int = 128, rope_links: int
     return Scene(bodies, state, cons_arr, container, hull, opts, name=f"mixed_field{n}_s{seed}")
 
 
+# ------------------------------------------------------------- hard chains
+def hard_chains(seed: int = 0, chains: int = 256, links: int = 32, half: float = 0.15, gap: float = 0.04,
+                pitch: float = 3.0) -> Scene:
+    """Chains of cubes on hard joints, each hung from a static anchor and
+    released horizontally, so that the chains swing into their neighbours and
+    their lower links pile on the floor.  Chain k uses hard sockets between
+    link ends (k % 3 == 0), hard rods (1) or hard ropes (2) between link
+    centres.  Anchors sit on a square grid `pitch` apart at a height of 0.8 of
+    a chain's length; bodies are the chains' anchors (0 .. chains - 1), then
+    the links chain by chain.  Draw order: directions (chains), jitter
+    (3 x chains x links).  Steps need the coloured mode (hard joints join
+    dynamic bodies)."""
+    rng = SplitMix64(seed)
+    dirs_u = rng.uniform(0.0, 4.0, chains)
+    jitter = rng.uniform(-1e-3, 1e-3, 3 * chains * links).reshape(chains * links, 3)
+    step = 2.0 * half + gap
+    height = 0.8 * links * step
+    side = int(np.ceil(np.sqrt(chains)))
+    n = chains + chains * links
+    bodies = np.zeros(n, R.GPU_BODY)
+    state = np.zeros(n, R.BODY_STATE)
+    bodies["orientation"][:, 3] = 1.0
+    axes = np.array([(1.0, 0.0, 0.0), (-1.0, 0.0, 0.0), (0.0, 0.0, 1.0), (0.0, 0.0, -1.0)])
+    anchors = np.zeros((chains, 3))
+    for k in range(chains):
+        anchors[k] = ((k % side - (side - 1) / 2.0) * pitch, height, (k // side - (side - 1) / 2.0) * pitch)
+    bodies["position"][:chains, :3] = anchors.astype(np.float32)
+    bodies["tag"][:chains] = R.SPHERE
+    bodies["corners"][:chains, 0, 0] = np.float32(0.1)
+    link_pos = np.zeros((chains * links, 3))
+    for k in range(chains):
+        d = axes[int(dirs_u[k]) % 4]
+        for i in range(links):
+            link_pos[k * links + i] = anchors[k] + d * step * (i + 1)
+    link_pos += jitter
+    lb, ls = cubes(link_pos, np.full(chains * links, half), np.ones(chains * links))
+    bodies[chains:] = lb
+    state[chains:] = ls
+    state["friction"] = np.float32(0.6)
+    cons = np.zeros(chains * links, R.AVBD_CONSTRAINT)
+    c = 0
+    for k in range(chains):
+        d = axes[int(dirs_u[k]) % 4].astype(np.float32)
+        kind = (R.HARD_SOCKET, R.HARD_ROD, R.HARD_ROPE)[k % 3]
+        for i in range(links):
+            a = k if i == 0 else chains + k * links + i - 1
+            b = chains + k * links + i
+            cons[c]["a"], cons[c]["b"], cons[c]["kind"] = a, b, kind
+            if kind == R.HARD_SOCKET:
+                # this link's near end to the previous link's far end (or the
+                # point half a link from the anchor)
+                cons[c]["anchor_a"][:3] = d * np.float32(step / 2.0)
+                cons[c]["anchor_b"][:3] = -d * np.float32(step / 2.0)
+            else:
+                cons[c]["anchor_a"][3] = np.float32(step)
+            c += 1
+    extent = side * pitch / 2.0 + links * step + 4.0
+    opts = AvbdOptions(dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), iterations=8, friction=0.6)
+    return Scene(bodies, state, cons, floor_container(extent), options=opts,
+                 name=f"hard_chains{chains}x{links}_s{seed}")
+
+
 # ------------------------------------------------------------- wall throw
 def wall_throw(seed: int = 0, columns: int = 64, rows: int = 32) -> Scene:
     """Bricks 0.5 (z) x 0.25 (y) x 0.25 (x) stacked in a wall in the y-z

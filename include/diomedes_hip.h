@@ -59,7 +59,17 @@ enum {
     DIO_CONSTRAINT_SPRING = 0,
     DIO_CONSTRAINT_ROD = 1,
     DIO_CONSTRAINT_BALL_SOCKET = 2,
-    DIO_CONSTRAINT_ROPE = 3
+    DIO_CONSTRAINT_ROPE = 3,
+    /* Hard joints: no counterpart in the reference, whose joints are the soft
+       kinds above (README.md:11 lists hard joints as missing).  The rod,
+       socket and rope geometry held to the constraint by a multiplier and a
+       penalty per row, with the contacts' augmented-Lagrangian updates and
+       warm start; the stiffness field is unused.  World path only (not
+       dio_avbd_submit); between two dynamic bodies they need dio_step
+       mode 1. */
+    DIO_CONSTRAINT_HARD_ROD = 4,
+    DIO_CONSTRAINT_HARD_BALL_SOCKET = 5,
+    DIO_CONSTRAINT_HARD_ROPE = 6
 };
 
 /* GpuShape, 80 bytes (compute.rs:28-34) */
@@ -372,6 +382,11 @@ dio_status dio_world_contacts(dio_ctx* ctx, dio_avbd_contact* out, uint32_t cap,
    graph; cap >= n) and the number of colours.  No counterpart in the
    reference. */
 dio_status dio_world_colours(dio_ctx* ctx, uint32_t* colour, uint32_t cap, uint32_t* num_colours);
+/* Each constraint's multipliers after the last step: lambda and penalty per
+   row, c x 3 floats each (x, y, z rows of a hard socket; x alone of a hard rod
+   or rope; zero for soft joints, which carry none).  cap >= the constraint
+   count, *c set to it.  No counterpart in the reference. */
+dio_status dio_world_joints(dio_ctx* ctx, float* lambda, float* penalty, uint32_t cap, uint32_t* c);
 /* Candidate pairs of the last step's broad phase (sorted, a < b). */
 dio_status dio_world_pairs(dio_ctx* ctx, dio_gpu_pair* out, uint32_t cap, uint32_t* n);
 

@@ -41,6 +41,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <stdexcept>
 #include <utility>
 #include <vector>
 
@@ -49,6 +50,11 @@ namespace oracle {
 constexpr uint32_t CONTAINER = 0xFFFFFFFFu;
 constexpr uint32_t SPHERE = 0, CUBE = 1, TETRAHEDRON = 2, BOX = 3, HULL = 4;
 constexpr uint32_t SPRING = 0, ROD = 1, BALL_SOCKET = 2, ROPE = 3;
+// hard joints (no counterpart in the reference, whose joints are the soft
+// kinds above): the same geometry, solved to the constraint with a
+// multiplier and a ramped penalty per row, as contacts are
+constexpr uint32_t HARD_ROD = 4, HARD_SOCKET = 5, HARD_ROPE = 6;
+inline bool is_hard(uint32_t kind) { return kind >= HARD_ROD && kind <= HARD_ROPE; }
 
 // ---------------------------------------------------------------- glam math
 template <class T> struct V3 { T x, y, z; };
@@ -1020,7 +1026,78 @@ template <class T> struct RunOptions {  // AvbdRunOptions (gpu_physics.rs:76-83)
     V3<T> gravity;
     T alpha, beta_lin, penalty_max;
     uint32_t iterations;
+    // the warm start of hard joints' multipliers (AvbdOptions gamma and
+    // penalty_min, as the contact warm start uses them: physics/gpu.rs:266-283)
+    T gamma = T(0.999), penalty_min = T(1);
 };
+
+// A hard joint's multipliers: lambda and penalty per row (x, y, z for a
+// socket; x alone for a rod or rope).
+template <class T> struct JointMult {
+    V3<T> lam{T(0), T(0), T(0)};
+    V3<T> pen{T(0), T(0), T(0)};
+};
+
+// The stabilised value of a hard joint's rows, C(x) - alpha C(x0), from the
+// anchor separation delta = p_a - p_b at the current poses and delta0 at the
+// step's initial poses (a rope, an inequality C <= 0, subtracts only a
+// violated start value: C(x) - alpha max(C(x0), 0)); dir is delta / |delta|
+// for a rod or rope, whose single row is defined only when |delta| > 1e-6
+// (ok).
+template <class T> struct HardRows {
+    V3<T> chat{T(0), T(0), T(0)};
+    V3<T> dir{T(0), T(0), T(0)};
+    bool ok = false;
+};
+template <class T> HardRows<T> hard_rows(uint32_t kind, V3<T> delta, V3<T> delta0, T rest, T alpha) {
+    HardRows<T> r;
+    if (kind == HARD_SOCKET) {
+        r.chat = V3<T>{delta.x - alpha * delta0.x, delta.y - alpha * delta0.y, delta.z - alpha * delta0.z};
+        r.ok = true;
+    } else {
+        T len = length(delta);
+        T len0 = length(delta0);
+        r.ok = len > T(1.0e-6);
+        T c0 = len0 - rest;
+        if (kind == HARD_ROPE) c0 = rmax(c0, T(0));
+        r.chat.x = (len - rest) - alpha * c0;
+        if (r.ok) r.dir = delta / len;
+    }
+    return r;
+}
+// The row force of a hard joint: penalty * C^ + lambda, kept >= 0 for a rope
+// (it only pulls).
+template <class T> T hard_force(uint32_t kind, T pen, T chat, T lam) {
+    T f = pen * chat + lam;
+    return kind == HARD_ROPE ? rmax(f, T(0)) : f;
+}
+// One dual update of a hard joint's rows (the augmented-Lagrangian step the
+// contacts take, avbd/src/lib.rs:565-663): lambda becomes the row force, and
+// the penalty of an active row ramps by beta |C^| up to penalty_max.
+template <class T> void hard_update(uint32_t kind, const HardRows<T>& r, JointMult<T>& m, T beta, T pmax) {
+    if (!r.ok) return;
+    const int rows = kind == HARD_SOCKET ? 3 : 1;
+    T* lam = &m.lam.x;
+    T* pen = &m.pen.x;
+    const T* ch = &r.chat.x;
+    for (int k = 0; k < rows; ++k) {
+        T f = hard_force(kind, pen[k], ch[k], lam[k]);
+        lam[k] = f;
+        if (kind != HARD_ROPE || f > T(0)) pen[k] = rmin(pen[k] + beta * ofabs(ch[k]), pmax);
+    }
+}
+// The warm start of a hard joint at the start of a step (the contacts' rule,
+// physics/gpu.rs:277-283): lambda * alpha gamma, penalty * gamma clamped to
+// [penalty_min, penalty_max].
+template <class T> JointMult<T> hard_warm(const JointMult<T>& s, const RunOptions<T>& pc) {
+    JointMult<T> m;
+    T ag = pc.alpha * pc.gamma;
+    m.lam = s.lam * ag;
+    V3<T> p = s.pen * pc.gamma;
+    m.pen = V3<T>{rmin(rmax(p.x, pc.penalty_min), pc.penalty_max), rmin(rmax(p.y, pc.penalty_min), pc.penalty_max),
+                  rmin(rmax(p.z, pc.penalty_min), pc.penalty_max)};
+    return m;
+}
 
 template <class T> struct SolveState {
     // body state (GpuAvbdBodyState)
@@ -1199,15 +1276,32 @@ template <class T>
 void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::vector<Constraint<T>>& cons,
                 const std::vector<uint32_t>& offsets, const std::vector<uint32_t>& indices, const RunOptions<T>& pc,
                 std::vector<V3<T>>& out_pos, std::vector<Q<T>>& out_rot, SolveTimes* times = nullptr,
-                const ColourPlan* plan = nullptr) {
+                const ColourPlan* plan = nullptr, std::vector<JointMult<T>>* jstate = nullptr) {
     const int64_t n = (int64_t)S.inv_mass.size();
     const int64_t m = (int64_t)contacts.size();
     out_pos.assign(n, V3<T>{});
     out_rot.assign(n, Q<T>{});
     if (n == 0) return;
+    // hard joints: multipliers warm-started from the state the last step
+    // left, updated after every primal, and stored back at the end
+    std::vector<uint32_t> hard;
+    for (uint32_t c = 0; c < cons.size(); ++c)
+        if (is_hard(cons[c].kind)) hard.push_back(c);
+    if (!hard.empty() && (!jstate || jstate->size() != cons.size()))
+        throw std::invalid_argument("hard joints need their multiplier state");
+    std::vector<JointMult<T>> jm(cons.size());
+    for (uint32_t c : hard) jm[c] = hard_warm((*jstate)[c], pc);
+
     const T dt = pc.dt;
     double t0 = now_s();
     avbd_init_only(S, pc);
+    // the anchor separation of each hard joint at the step's initial poses
+    std::vector<V3<T>> delta0(cons.size(), V3<T>{T(0), T(0), T(0)});
+    for (uint32_t ci : hard) {
+        const Constraint<T>& c = cons[ci];
+        delta0[ci] = (S.initial_lin[c.a] + qrot(S.initial_ang[c.a], c.anchor_a)) -
+                     (S.initial_lin[c.b] + qrot(S.initial_ang[c.b], c.anchor_b));
+    }
     double t1 = now_s();
     if (times) times->init += t1 - t0;
     // per-body joint lists in ascending constraint order (equivalent to the
@@ -1300,7 +1394,39 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
                 V3<T> delta = p_a - p_b;
                 V3<T> r_self = qrot(snap_rot, self_anchor);
                 T stiff = c.stiffness;
-                if (c.kind == BALL_SOCKET) {
+                if (is_hard(c.kind)) {
+                    const HardRows<T> hr = hard_rows(c.kind, delta, delta0[ci], c.length, pc.alpha);
+                    const JointMult<T>& mu = jm[ci];
+                    if (c.kind == HARD_SOCKET) {
+                        const V3<T> js[3] = {V3<T>{T(1), T(0), T(0)} * sign, V3<T>{T(0), T(1), T(0)} * sign,
+                                             V3<T>{T(0), T(0), T(1)} * sign};
+                        const T* ch = &hr.chat.x;
+                        const T* pn = &mu.pen.x;
+                        const T* lm = &mu.lam.x;
+                        for (int r = 0; r < 3; ++r) {
+                            V3<T> ja = cross(r_self, js[r]);
+                            T f = hard_force(c.kind, pn[r], ch[r], lm[r]);
+                            B.add_lin(js[r], pn[r]);
+                            B.add_ang(ja, pn[r]);
+                            B.add_cross(ja, js[r], pn[r]);
+                            B.add_rhs_lin(js[r], f);
+                            B.add_rhs_ang(ja, f);
+                        }
+                    } else if (hr.ok) {
+                        // a rope whose row force is zero (slack) adds nothing,
+                        // as the soft rope is inactive within its length
+                        T f = hard_force(c.kind, mu.pen.x, hr.chat.x, mu.lam.x);
+                        if (c.kind != HARD_ROPE || f > T(0)) {
+                            V3<T> jl = hr.dir * sign;
+                            V3<T> ja = cross(r_self, jl);
+                            B.add_lin(jl, mu.pen.x);
+                            B.add_ang(ja, mu.pen.x);
+                            B.add_cross(ja, jl, mu.pen.x);
+                            B.add_rhs_lin(jl, f);
+                            B.add_rhs_ang(ja, f);
+                        }
+                    }
+                } else if (c.kind == BALL_SOCKET) {
                     V3<T> jx = V3<T>{T(1), T(0), T(0)} * sign;
                     V3<T> jy = V3<T>{T(0), T(1), T(0)} * sign;
                     V3<T> jz = V3<T>{T(0), T(0), T(1)} * sign;
@@ -1388,9 +1514,17 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
                 ct.penalty.z = rmin(ct.penalty.z + pc.beta_lin * ofabs(c2), pc.penalty_max);
             }
         }
+        // the hard joints' dual update, from the same poses
+        for (uint32_t ci : hard) {
+            const Constraint<T>& c = cons[ci];
+            V3<T> delta = (wpv[c.a] + qrot(wrv[c.a], c.anchor_a)) - (wpv[c.b] + qrot(wrv[c.b], c.anchor_b));
+            hard_update(c.kind, hard_rows(c.kind, delta, delta0[ci], c.length, pc.alpha), jm[ci], pc.beta_lin,
+                        pc.penalty_max);
+        }
         double tc = now_s();
         if (times) { times->primal += tb - ta; times->dual += tc - tb; }
     }
+    for (uint32_t c : hard) (*jstate)[c] = jm[c];
     double td = now_s();
     // avbd_recover (avbd/src/lib.rs:667-699) with parity (I-1)%2: reads the
     // buffer the last primal wrote
@@ -1435,6 +1569,8 @@ template <class T> struct World {
     // 0: Jacobi (the reference); 1: coloured Gauss-Seidel (colour_bodies)
     int mode = 0;
     ColourPlan plan;
+    // hard joints' multipliers as the last step left them (zero at the start)
+    std::vector<JointMult<T>> joint_state;
 
     // prepare_contacts (physics/gpu.rs:188-197)
     void prepare_contacts(std::vector<Contact<T>> body_contacts, const Options<T>& o) {
@@ -1506,12 +1642,14 @@ template <class T> struct World {
         }
         S.pos[1] = S.pos[0];
         S.rot[1] = S.rot[0];
-        RunOptions<T> pc{o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, o.iterations};
+        RunOptions<T> pc{o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, o.iterations, o.gamma, o.penalty_min};
         std::vector<V3<T>> op;
         std::vector<Q<T>> orr;
         if (keep_presolve) presolve = contacts;
         if (mode == 1) plan = colour_bodies(S.inv_mass, contacts, constraints, offsets, indices);
-        avbd_solve(S, contacts, constraints, offsets, indices, pc, op, orr, &times.solve, mode == 1 ? &plan : nullptr);
+        joint_state.resize(constraints.size());
+        avbd_solve(S, contacts, constraints, offsets, indices, pc, op, orr, &times.solve, mode == 1 ? &plan : nullptr,
+                   &joint_state);
         for (size_t i = 0; i < n; ++i) {
             Body<T>& b = bodies[i];
             if (b.is_static()) continue;

@@ -225,6 +225,23 @@ OAPI void oracle_world_get_csr(void* p, uint32_t* offsets, uint32_t* indices) {
     });
 }
 
+// Each constraint's multipliers after the last step: lambda and penalty per
+// row, (c, 3) each (zero for soft joints, which carry none).
+OAPI uint32_t oracle_world_get_joints(void* p, double* lam, double* pen) {
+    return with_world(static_cast<WorldHandle*>(p), [&](auto& w) {
+        const size_t c = w.constraints.size();
+        for (size_t i = 0; i < c; ++i) {
+            if (i < w.joint_state.size()) {
+                put3(lam + 3 * i, w.joint_state[i].lam);
+                put3(pen + 3 * i, w.joint_state[i].pen);
+            } else {
+                for (int k = 0; k < 3; ++k) lam[3 * i + k] = pen[3 * i + k] = 0.0;
+            }
+        }
+        return (uint32_t)c;
+    });
+}
+
 // 0: Jacobi (the reference); 1: coloured Gauss-Seidel
 OAPI void oracle_world_set_mode(void* p, int mode) {
     with_world(static_cast<WorldHandle*>(p), [&](auto& w) {

@@ -58,6 +58,7 @@ def load() -> ctypes.CDLL:
         "oracle_gjk": (c_int, [c_int, c_void_p, c_void_p, _F, c_uint32]),
         "oracle_world_set_mode": (None, [c_void_p, c_int]),
         "oracle_world_get_colours": (c_uint32, [c_void_p, _U, _U]),
+        "oracle_world_get_joints": (c_uint32, [c_void_p, _D, _D]),
         "oracle_colour_graph": (c_uint32, [c_uint32, ctypes.POINTER(ctypes.c_uint8), _U, c_uint32, _U, _U]),
         "oracle_solve6": (None, [c_int, _D, _D, _D, _D, _D]),
         "oracle_container_contacts": (c_uint32, [c_int, c_void_p, c_void_p, POINTER(R.Container), POINTER(R.Options),
@@ -151,6 +152,14 @@ class OracleWorld:
         rounds = ctypes.c_uint32()
         nc = load().oracle_world_get_colours(self._h, _p(col, c_uint32), ctypes.byref(rounds))
         return col, int(nc), rounds.value
+
+    def joints(self) -> tuple:
+        """Each constraint's multipliers after the last step: lambda and
+        penalty per row, (c, 3) float64 each (zero for soft joints)."""
+        c = len(self.scene.constraints)
+        lam, pen = np.zeros((c, 3)), np.zeros((c, 3))
+        load().oracle_world_get_joints(self._h, _p(lam, c_double), _p(pen, c_double))
+        return lam, pen
 
     def state(self) -> dict:
         n = self.n
