@@ -203,6 +203,62 @@ def mixed_leg(args, tuning: dict) -> dict:
     }
 
 
+def _joint_error(scene, pos, rot) -> float:
+    """Worst violation over the scene's joints: |p_a - p_b| for a socket,
+    ||delta| - L| for a rod, max(|delta| - L, 0) for a rope (metres)."""
+    import numpy as np
+
+    c = scene.constraints
+
+    def world(idx, anchor):
+        q = rot[idx].astype(np.float64)
+        u, w = q[:, :3], q[:, 3:4]
+        v = anchor.astype(np.float64)
+        return pos[idx].astype(np.float64) + v + 2.0 * np.cross(u, np.cross(u, v) + w * v)
+
+    delta = world(c["a"], c["anchor_a"][:, :3]) - world(c["b"], c["anchor_b"][:, :3])
+    d = np.linalg.norm(delta, axis=1)
+    length = c["anchor_a"][:, 3].astype(np.float64)
+    err = np.where(c["kind"] == 5, d, np.where(c["kind"] == 6, np.maximum(d - length, 0.0), np.abs(d - length)))
+    return float(err.max()) if len(err) else 0.0
+
+
+def joints_leg(args, tuning: dict) -> dict:
+    """Hard joints: 256 chains of 32 cubes (8,448 bodies, 8,192 hard sockets,
+    rods and ropes) released horizontally from static anchors, swinging into
+    each other and piling on the floor; coloured mode (mode 1, which hard
+    joints between dynamic bodies need), I = 8, dt = 1/60."""
+    from diomedes_amd import PhysicsContext, scenes
+
+    scene = scenes.hard_chains(args.seed)
+    opts = scene.options
+    k = args.joint_steps
+    with PhysicsContext(scene.n, max_constraints=len(scene.constraints)) as ctx:
+        ctx.set_tuning(**tuning)
+        ctx.upload(scene)
+        if args.warmup:
+            ctx.step(opts, args.warmup, mode=1)
+        ctx.step(opts, 0, mode=1)
+        t0 = time.perf_counter()
+        ctx.step(opts, k, mode=1)
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+        d = ctx.download()
+    err = _joint_error(scene, d["positions"], d["orientations"])
+    _progress(f"joints leg: {k} steps in {wall:.2f} s, worst joint error {err:.3e} m")
+    return {
+        "workload": f"hard chains: 256 x 32 cubes on {len(scene.constraints)} hard joints, coloured mode, "
+                    f"I={opts.iterations}, dt=1/60, {k} steps after {args.warmup}",
+        "bodies": scene.n,
+        "value": scene.n_dynamic * opts.iterations * k / (st.ms_avbd / 1000.0),
+        "unit": "body-iters/s",
+        "ms_per_step": 1000.0 * wall / k,
+        "ms_per_step_avbd": st.ms_avbd / k,
+        "worst_joint_error_m": err,
+        "last_step": {"colours": st.colours, "colour_rounds": st.colour_rounds, "contacts": st.contacts},
+    }
+
+
 def _wall_loss_grad(pos, targets, offsets):
     """Sum over seeds of the mean squared distance of the bricks' final
     centres to their targets, and its adjoint with respect to the final
@@ -357,13 +413,15 @@ def _progress(msg: str) -> None:
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, warmup: int, steps: int, budget_s: float = 25.0) -> dict:
-    """FP32 oracle on the host cores over the same warm-up and steps."""
+def cpu_baseline(scene, warmup: int, steps: int, budget_s: float = 25.0, mode: int = 0) -> dict:
+    """FP32 oracle on the host cores over the same warm-up and steps (mode 1:
+    its coloured Gauss-Seidel solve)."""
     from oracle import pyoracle
 
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     pyoracle.threads(cores)
     w = pyoracle.OracleWorld(scene, 32)
+    w.set_mode(mode)
     w.step(nsteps=warmup)
     w.reset_times()
     done = 0
@@ -414,7 +472,8 @@ def main() -> int:
                          "shapes and dual schedule (0: library defaults)")
     ap.add_argument("--no-coloured", action="store_true", help="skip the coloured Gauss-Seidel leg")
     ap.add_argument("--mixed-steps", type=int, default=300, help="steps of the mixed-field leg (0: skip it)")
-    ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint", "mixed"), default=None,
+    ap.add_argument("--joint-steps", type=int, default=120, help="steps of the hard-joints leg (0: skip it)")
+    ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint", "mixed", "joints"), default=None,
                     help="run only this leg and print its object (for profiling it alone)")
     ap.add_argument("--adjoint-seeds", type=int, default=64,
                     help="wall throws of the adjoint leg (0: skip it)")
@@ -434,7 +493,8 @@ def main() -> int:
 
     if args.only_leg:
         legs = {"coloured": lambda: coloured_leg(args, {}), "batched": lambda: batched_leg(args, {}),
-                "adjoint": lambda: adjoint_leg(args), "mixed": lambda: mixed_leg(args, {})}
+                "adjoint": lambda: adjoint_leg(args), "mixed": lambda: mixed_leg(args, {}),
+                "joints": lambda: joints_leg(args, {})}
         print(json.dumps({"leg": args.only_leg, **legs[args.only_leg]()}), flush=True)
         return 0
 
@@ -554,6 +614,8 @@ def main() -> int:
         line["coloured"] = coloured_leg(args, tuning)
     if args.mixed_steps:
         line["mixed"] = mixed_leg(args, tuning)
+    if args.joint_steps:
+        line["joints"] = joints_leg(args, tuning)
     if args.batch_scenes:
         line["batched"] = batched_leg(args, tuning)
     if args.adjoint_seeds:
@@ -563,6 +625,11 @@ def main() -> int:
         line["cpu_baseline"] = cb
         if cb.get("value"):
             line["gpu_over_cpu"] = value / cb["value"]
+        if args.joint_steps:
+            jc = cpu_baseline(scenes.hard_chains(args.seed), args.warmup, args.joint_steps, budget_s=10.0, mode=1)
+            line["joints"]["cpu_baseline"] = jc
+            if jc.get("value"):
+                line["joints"]["gpu_over_cpu"] = line["joints"]["value"] / jc["value"]
     print(json.dumps(line), flush=True)
     return 0
 

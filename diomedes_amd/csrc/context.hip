@@ -1333,7 +1333,7 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         if (so.mode > 1) invalid("mode must be 0 (Jacobi) or 1 (coloured Gauss-Seidel)");
         const bool coloured = so.mode == 1;
         if (coloured && ctx->adj_on) invalid("the adjoint records Jacobi steps (mode 0) only");
-        if (!coloured && ctx->w_j.hard_dynamic > 0)
+        if (!coloured && nsteps > 0 && ctx->w_j.hard_dynamic > 0)
             invalid("hard joints between two dynamic bodies need the coloured mode (1): the Jacobi solve "
                     "diverges on them");
         ctx->last_lag = so.contact_lag ? 1 : 0;

@@ -284,38 +284,44 @@ def mixed_field(seed: int = 0, n: int = 20000, ropes: int = 128, rope_links: int
 
 # ------------------------------------------------------------- hard chains
 def hard_chains(seed: int = 0, chains: int = 256, links: int = 32, half: float = 0.15, gap: float = 0.04,
-                pitch: float = 3.0) -> Scene:
+                row_pitch: float = 1.0) -> Scene:
     """Chains of cubes on hard joints, each hung from a static anchor and
-    released horizontally, so that the chains swing into their neighbours and
-    their lower links pile on the floor.  Chain k uses hard sockets between
-    link ends (k % 3 == 0), hard rods (1) or hard ropes (2) between link
-    centres.  Anchors sit on a square grid `pitch` apart at a height of 0.8 of
-    a chain's length; bodies are the chains' anchors (0 .. chains - 1), then
-    the links chain by chain.  Draw order: directions (chains), jitter
-    (3 x chains x links).  Steps need the coloured mode (hard joints join
-    dynamic bodies)."""
+    released horizontally: they swing down in parallel planes `row_pitch`
+    apart, neighbouring rows in opposite directions, and their lower links
+    pile on the floor into one another.  No two bodies overlap at the start:
+    anchors sit in rows (z) of up to 8 chains whose spans (x) do not overlap,
+    at a height of 0.8 of a chain's length.  Chain k uses hard sockets
+    between link ends (k % 3 == 0), hard rods (1) or hard ropes (2) between
+    link centres.  Bodies are the anchors (0 .. chains - 1), then the links
+    chain by chain.  Draw order: jitter (3 x chains x links).  Steps need the
+    coloured mode (hard joints join dynamic bodies)."""
     rng = SplitMix64(seed)
-    dirs_u = rng.uniform(0.0, 4.0, chains)
     jitter = rng.uniform(-1e-3, 1e-3, 3 * chains * links).reshape(chains * links, 3)
     step = 2.0 * half + gap
-    height = 0.8 * links * step
-    side = int(np.ceil(np.sqrt(chains)))
+    span = links * step
+    height = 0.8 * span
+    cols = min(8, chains)
+    rows = (chains + cols - 1) // cols
     n = chains + chains * links
     bodies = np.zeros(n, R.GPU_BODY)
     state = np.zeros(n, R.BODY_STATE)
     bodies["orientation"][:, 3] = 1.0
-    axes = np.array([(1.0, 0.0, 0.0), (-1.0, 0.0, 0.0), (0.0, 0.0, 1.0), (0.0, 0.0, -1.0)])
     anchors = np.zeros((chains, 3))
+    dirs = np.zeros((chains, 3))
     for k in range(chains):
-        anchors[k] = ((k % side - (side - 1) / 2.0) * pitch, height, (k // side - (side - 1) / 2.0) * pitch)
+        r, c = k // cols, k % cols
+        # even rows point +x from the span's left end, odd rows -x from its right end
+        sgn = 1.0 if r % 2 == 0 else -1.0
+        x0 = (c - (cols - 1) / 2.0) * (span + 1.0) - sgn * span / 2.0
+        anchors[k] = (x0, height, (r - (rows - 1) / 2.0) * row_pitch)
+        dirs[k] = (sgn, 0.0, 0.0)
     bodies["position"][:chains, :3] = anchors.astype(np.float32)
     bodies["tag"][:chains] = R.SPHERE
     bodies["corners"][:chains, 0, 0] = np.float32(0.1)
     link_pos = np.zeros((chains * links, 3))
     for k in range(chains):
-        d = axes[int(dirs_u[k]) % 4]
         for i in range(links):
-            link_pos[k * links + i] = anchors[k] + d * step * (i + 1)
+            link_pos[k * links + i] = anchors[k] + dirs[k] * step * (i + 1)
     link_pos += jitter
     lb, ls = cubes(link_pos, np.full(chains * links, half), np.ones(chains * links))
     bodies[chains:] = lb
@@ -324,7 +330,7 @@ def hard_chains(seed: int = 0, chains: int = 256, links: int = 32, half: float =
     cons = np.zeros(chains * links, R.AVBD_CONSTRAINT)
     c = 0
     for k in range(chains):
-        d = axes[int(dirs_u[k]) % 4].astype(np.float32)
+        d = dirs[k].astype(np.float32)
         kind = (R.HARD_SOCKET, R.HARD_ROD, R.HARD_ROPE)[k % 3]
         for i in range(links):
             a = k if i == 0 else chains + k * links + i - 1
@@ -338,7 +344,7 @@ def hard_chains(seed: int = 0, chains: int = 256, links: int = 32, half: float =
             else:
                 cons[c]["anchor_a"][3] = np.float32(step)
             c += 1
-    extent = side * pitch / 2.0 + links * step + 4.0
+    extent = max(cols * (span + 1.0), rows * row_pitch) / 2.0 + span + 4.0
     opts = AvbdOptions(dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), iterations=8, friction=0.6)
     return Scene(bodies, state, cons, floor_container(extent), options=opts,
                  name=f"hard_chains{chains}x{links}_s{seed}")
