@@ -99,13 +99,25 @@ __global__ void __launch_bounds__(kBlock) k_colour_round(ColourGraph g, ColourBu
     b.round_of[i] = r;
 }
 
+// Sort keys (colour, or kMaxColours for a body not in the graph) and the
+// count per key: a block histogram in LDS, then one global add per (block,
+// key) -- a handful of colours would otherwise put every thread's atomic on
+// the same few words.
+static_assert(kMaxColours + 1 == kBlock, "one histogram bin per thread");
 __global__ void __launch_bounds__(kBlock) k_colour_keys(ColourGraph g, ColourBuffers b) {
+    __shared__ uint32_t hist[kMaxColours + 1];
+    hist[threadIdx.x] = 0u;
+    __syncthreads();
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= g.n) return;
-    const uint32_t key = b.round_of[i] == kColourNotInGraph ? kMaxColours : b.colour[i];
-    b.keys[0][i] = key;
-    b.vals[0][i] = i;
-    atomicAdd(&b.counts[key], 1u);
+    if (i < g.n) {
+        const uint32_t key = b.round_of[i] == kColourNotInGraph ? kMaxColours : b.colour[i];
+        b.keys[0][i] = key;
+        b.vals[0][i] = i;
+        atomicAdd(&hist[key], 1u);
+    }
+    __syncthreads();
+    const uint32_t h = hist[threadIdx.x];
+    if (h) atomicAdd(&b.counts[threadIdx.x], h);
 }
 
 }  // namespace

@@ -108,11 +108,18 @@ __global__ void __launch_bounds__(kBlock) k_contacts_narrow(WorldView w, BuildOp
     }
 }
 
-// container_contacts (physics/gpu.rs:352-404) for one body; emit(point, depth,
-// normal, t1, t2) for each contact in wall / point order.
-template <class Emit>
-__device__ __forceinline__ void wall_contacts(const WorldView& w, uint32_t i, const ContainerParams& ct,
-                                              const BuildOptions& o, Emit&& emit) {
+// container_contacts (physics/gpu.rs:352-404), one wall at a time: the
+// wall's candidate points (a cube or box's face towards it, cube_support_face
+// physics/gpu.rs:314-350; else one support point), each point's depth
+// max(current, predicted) and the wall's contact frame.  A point is a contact
+// when its depth is positive; a body's contacts run in wall, then point order.
+struct WallPoints {
+    f3 pts[4];
+    float depth[4];
+    uint32_t count;
+    f3 normal, t1, t2;
+};
+__device__ __forceinline__ ShapeRef body_shape(const WorldView& w, uint32_t i) {
     ShapeRef s;
     s.pos = xyz(w.pos[i]);
     s.rot = as_q(w.rot[i]);
@@ -121,108 +128,154 @@ __device__ __forceinline__ void wall_contacts(const WorldView& w, uint32_t i, co
     s.c[1] = w.corners[4 * i + 1];
     s.c[2] = w.corners[4 * i + 2];
     s.c[3] = w.corners[4 * i + 3];
-    const f3 vel = xyz(w.vel[i]);
+    return s;
+}
+__device__ __forceinline__ void wall_points(const WorldView& w, const ShapeRef& s, f3 vel, const ContainerParams& ct,
+                                            const BuildOptions& o, int wall, WallPoints& out) {
     const q4 crot{ct.qx, ct.qy, ct.qz, ct.qw};
     const f3 center{ct.cx, ct.cy, ct.cz};
-    const f3 axes[3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
-    for (int wall = 0; wall < 6; ++wall) {
-        const float sign = (wall & 1) ? -1.0f : 1.0f;
-        const f3 outward = qrot(crot, scale(axes[wall >> 1], sign));
-        const float plane = dot3(center, outward) + ct.half;
-        f3 pts[4];
-        uint32_t count;
-        if (s.tag == kCube || s.tag == kBox) {
-            // cube_support_face (physics/gpu.rs:314-350)
-            f3 ax[3];
-            box_axes(s.rot, ax);
-            uint32_t ai = 0;
-            float best = fabsf(dot3(ax[0], outward));
-            for (uint32_t k = 1; k < 3; ++k) {
-                const float al = fabsf(dot3(ax[k], outward));
-                if (al > best) { ai = k; best = al; }
-            }
-            const f3 fa = ax[ai];
-            const f3 fnrm = scale(fa, dot3(fa, outward) >= 0.0f ? 1.0f : -1.0f);
-            const uint32_t iu = ai == 0 ? 1u : 0u;
-            const uint32_t iv = ai == 2 ? 1u : 2u;
-            const float hn = box_half(s, ai), hu = box_half(s, iu), hv = box_half(s, iv);
-            const f3 c = add(s.pos, scale(fnrm, hn));
-            const f3 u = ax[iu], v = ax[iv];
-            pts[0] = add(add(c, scale(u, hu)), scale(v, hv));
-            pts[1] = add(sub(c, scale(u, hu)), scale(v, hv));
-            pts[2] = sub(sub(c, scale(u, hu)), scale(v, hv));
-            pts[3] = sub(add(c, scale(u, hu)), scale(v, hv));
-            count = 4;
-        } else {
-            const f3 local = qrot_inv(s.rot, outward);
-            pts[0] = add(qrot(s.rot, support_host(s, local, HullPool{w.hull})), s.pos);
-            count = 1;
-        }
-        const f3 normal = neg(outward);
-        f3 t1, t2;
-        tangents(normal, t1, t2);
-        for (uint32_t k = 0; k < count; ++k) {
-            const float cur = dot3(pts[k], outward) - plane;
-            const float pred = dot3(add(pts[k], scale(vel, o.dt)), outward) - plane;
-            const float depth = rmax(cur, pred);
-            if (depth > 0.0f) emit(pts[k], depth, normal, t1, t2, s);
+    const float sign = (wall & 1) ? -1.0f : 1.0f;
+    const int axis = wall >> 1;
+    const f3 unit{axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f};
+    const f3 outward = qrot(crot, scale(unit, sign));
+    const float plane = dot3(center, outward) + ct.half;
+    if (s.tag == kCube || s.tag == kBox) {
+        const f3 ax0 = qrot(s.rot, f3{1.0f, 0.0f, 0.0f});
+        const f3 ax1 = qrot(s.rot, f3{0.0f, 1.0f, 0.0f});
+        const f3 ax2 = qrot(s.rot, f3{0.0f, 0.0f, 1.0f});
+        uint32_t ai = 0;
+        float best = fabsf(dot3(ax0, outward));
+        const float al1 = fabsf(dot3(ax1, outward));
+        if (al1 > best) { ai = 1; best = al1; }
+        const float al2 = fabsf(dot3(ax2, outward));
+        if (al2 > best) { ai = 2; best = al2; }
+        const f3 fa = sel3(ax0, ax1, ax2, ai);
+        const f3 fnrm = scale(fa, dot3(fa, outward) >= 0.0f ? 1.0f : -1.0f);
+        const uint32_t iu = ai == 0 ? 1u : 0u;
+        const uint32_t iv = ai == 2 ? 1u : 2u;
+        const float hn = box_half(s, ai), hu = box_half(s, iu), hv = box_half(s, iv);
+        const f3 c = add(s.pos, scale(fnrm, hn));
+        const f3 u = sel3(ax0, ax1, ax2, iu), v = sel3(ax0, ax1, ax2, iv);
+        out.pts[0] = add(add(c, scale(u, hu)), scale(v, hv));
+        out.pts[1] = add(sub(c, scale(u, hu)), scale(v, hv));
+        out.pts[2] = sub(sub(c, scale(u, hu)), scale(v, hv));
+        out.pts[3] = sub(add(c, scale(u, hu)), scale(v, hv));
+        out.count = 4;
+    } else {
+        const f3 local = qrot_inv(s.rot, outward);
+        out.pts[0] = add(qrot(s.rot, support_host(s, local, HullPool{w.hull})), s.pos);
+        // the unused slots too: a struct left partly undefined on one path
+        // stays in scratch memory
+        out.pts[1] = out.pts[0];
+        out.pts[2] = out.pts[0];
+        out.pts[3] = out.pts[0];
+        out.count = 1;
+    }
+    out.normal = neg(outward);
+    tangents(out.normal, out.t1, out.t2);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        out.depth[k] = 0.0f;
+        if (k < out.count) {
+            const float cur = dot3(out.pts[k], outward) - plane;
+            const float pred = dot3(add(out.pts[k], scale(vel, o.dt)), outward) - plane;
+            out.depth[k] = rmax(cur, pred);
         }
     }
 }
+__device__ __forceinline__ uint32_t wall_contact_count(const WallPoints& wp) {
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) c += (k < wp.count && wp.depth[k] > 0.0f) ? 1u : 0u;
+    return c;
+}
+
+// Container contacts run kWallLanes lanes per body, lane l on wall l (lanes 6
+// and 7 idle): six times the parallelism of a lane per body, with each wall's
+// arithmetic unchanged.
+constexpr int kWallLanes = 8;
 
 __global__ void __launch_bounds__(kBlock) k_container_count(WorldView w, ContainerParams ct, BuildOptions o,
                                                             uint32_t* cnt, uint32_t* zero_a, uint32_t* zero_b) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= w.n) return;
-    if (zero_a) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = t / kWallLanes;
+    const int wall = (int)(t % kWallLanes);
+    if (i >= w.n) return;  // whole groups of kWallLanes lanes leave together
+    if (zero_a && wall == 0) {
         zero_a[i] = 0u;
         zero_b[i] = 0u;
     }
     uint32_t c = 0;
     const ContainerParams cp = w.scene_ct ? w.scene_ct[w.scene_of[i]] : ct;
-    if (cp.enabled && w.inv_mm[i].w != 0.0f)
-        wall_contacts(w, i, cp, o, [&](f3, float, f3, f3, f3, const ShapeRef&) { ++c; });
-    cnt[i] = c;
+    if (wall < 6 && cp.enabled && w.inv_mm[i].w != 0.0f) {
+        WallPoints wp;
+        wall_points(w, body_shape(w, i), xyz(w.vel[i]), cp, o, wall, wp);
+        c = wall_contact_count(wp);
+    }
+    c += __shfl_xor(c, 1, kWallLanes);
+    c += __shfl_xor(c, 2, kWallLanes);
+    c += __shfl_xor(c, 4, kWallLanes);
+    if (wall == 0) cnt[i] = c;
 }
 
 __global__ void __launch_bounds__(kBlock) k_container_write(WorldView w, ContainerParams ct, BuildOptions o,
                                                             ContactSet dst, ContactSet old, uint32_t* overflow) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= w.n) return;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = t / kWallLanes;
+    const int wall = (int)(t % kWallLanes);
+    if (i >= w.n) return;  // whole groups of kWallLanes lanes leave together
     const uint32_t mn = *dst.m_narrow;
-    if (i == 0) {
-        const uint32_t t = mn + dst.cont_off[w.n];
-        *dst.m_dev = t < dst.m_cap ? t : dst.m_cap;
-        if (t > dst.m_cap) atomicMax(overflow, t);
+    if (t == 0) {
+        const uint32_t tot = mn + dst.cont_off[w.n];
+        *dst.m_dev = tot < dst.m_cap ? tot : dst.m_cap;
+        if (tot > dst.m_cap) atomicMax(overflow, tot);
     }
-    if (dst.cont_off[i + 1] == dst.cont_off[i]) return;
-    // warm value of key (i, CONTAINER): the last old container contact of body i
+    if (dst.cont_off[i + 1] == dst.cont_off[i]) return;  // the whole group: body i has no wall contact
+    WallPoints wp;
+    wp.count = 0;
+    const ShapeRef s = body_shape(w, i);
+    const ContainerParams cp = w.scene_ct ? w.scene_ct[w.scene_of[i]] : ct;
+    if (wall < 6) wall_points(w, s, xyz(w.vel[i]), cp, o, wall, wp);
+    const uint32_t mine = wall < 6 ? wall_contact_count(wp) : 0u;
+    // exclusive prefix of the counts over the group's walls
+    uint32_t incl = mine;
+#pragma unroll
+    for (int d = 1; d < kWallLanes; d <<= 1) {
+        const uint32_t up = __shfl_up(incl, d, kWallLanes);
+        if (wall >= d) incl += up;
+    }
+    if (mine == 0) return;
+    uint32_t before = incl - mine;
+    // warm value of key (i, CONTAINER): the last old container contact of body
+    // i, taken by the body's first new one
     float4 wl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float4 wpn = make_float4(o.penalty_min, o.penalty_min, o.penalty_min, 0.0f);
-    const uint32_t o0 = old.cont_off[i], o1 = old.cont_off[i + 1];
-    if (o1 > o0) {
-        const uint32_t q = *old.m_narrow + o1 - 1;
-        if (q < *old.m_dev) {
-            wl = old.c.lam[q];
-            wpn = old.c.pen[q];
+    if (before == 0) {
+        const uint32_t o0 = old.cont_off[i], o1 = old.cont_off[i + 1];
+        if (o1 > o0) {
+            const uint32_t q = *old.m_narrow + o1 - 1;
+            if (q < *old.m_dev) {
+                wl = old.c.lam[q];
+                wpn = old.c.pen[q];
+            }
         }
     }
-    uint32_t ci = mn + dst.cont_off[i];
-    bool first = true;
-    const ContainerParams cp = w.scene_ct ? w.scene_ct[w.scene_of[i]] : ct;
-    wall_contacts(w, i, cp, o, [&](f3 p, float depth, f3 normal, f3 t1, f3 t2, const ShapeRef& s) {
+    uint32_t ci = mn + dst.cont_off[i] + before;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k >= wp.count || !(wp.depth[k] > 0.0f)) continue;
         if (ci < dst.m_cap) {
             float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float4 pen = make_float4(o.penalty_min, o.penalty_min, o.penalty_min, 0.0f);
-            if (first) { lam = wl; pen = wpn; }
+            if (before == 0) { lam = wl; pen = wpn; }
             warm_scale(o, lam, pen);
-            const f3 ra = qrot_inv(s.rot, sub(p, s.pos));
-            put_contact(dst.c, ci, i, kContainer, normal, t1, t2, ra, f3{0.0f, 0.0f, 0.0f}, o.margin - depth,
-                        o.friction, lam, pen);
+            const f3 ra = qrot_inv(s.rot, sub(wp.pts[k], s.pos));
+            put_contact(dst.c, ci, i, kContainer, wp.normal, wp.t1, wp.t2, ra, f3{0.0f, 0.0f, 0.0f},
+                        o.margin - wp.depth[k], o.friction, lam, pen);
         }
-        first = false;
+        ++before;
         ++ci;
-    });
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_csr_count(ContactSet cs, uint32_t* cnt) {
@@ -310,7 +363,7 @@ void run_contact_build(const WorldView& w, const ContainerParams& ct, const Buil
     uint32_t pb = blocks_for(dst.m_cap > 0 ? dst.m_cap / 4 + 1 : 1);
     if (pb > 2048) pb = 2048;
     hipLaunchKernelGGL(k_contacts_narrow, dim3(pb), dim3(kBlock), 0, s, w, o, src, dst, old_src, old, overflow);
-    const uint32_t nb = blocks_for(w.n > 0 ? w.n : 1);
+    const uint32_t nb = blocks_for((w.n > 0 ? w.n : 1) * (uint32_t)kWallLanes);
     hipLaunchKernelGGL(k_container_count, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst.cont_cnt, csr_cnt,
                        csr_cnt ? csr_cur : nullptr);
     launch_scan_exclusive(dst.cont_cnt, dst.cont_off, w.n, nullptr, scan_partials, s);

@@ -1055,7 +1055,9 @@ void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const di
         g.scene_off = c->w_scene_off;
     }
     launch_colour_init(g, c->wc, st);
-    uint32_t r = 0, batch = 24;
+    // the first batch covers the previous coloured step's rounds and two more
+    // (the graph changes little from step to step); later batches 16 each
+    uint32_t r = 0, batch = c->last_rounds ? std::min<uint32_t>(24u, std::max<uint32_t>(4u, c->last_rounds + 2)) : 24u;
     uint32_t* h = c->wc_host;
     for (;;) {
         if (r + batch > kColourRoundSlots - 1) capacity("the colouring needs more rounds than it can track", r);

@@ -57,18 +57,29 @@ __device__ __forceinline__ void support_witness(const ShapeRef& a, const ShapeRe
     witness = ap;
 }
 
+// The shifts store every slot with its new or old value (unrolled selects): a
+// runtime index, or conditional stores the compiler merges into one, would
+// place the simplex in scratch memory.
 struct Simplex {
     f3 p[4], w[4];
     uint32_t len;
     __device__ void push_front(f3 pt, f3 wt) {
-        for (uint32_t i = len; i > 0; --i)
-            if (i < 4) { p[i] = p[i - 1]; w[i] = w[i - 1]; }
+#pragma unroll
+        for (uint32_t i = 3; i > 0; --i) {
+            p[i] = sel_f3(i <= len, p[i - 1], p[i]);
+            w[i] = sel_f3(i <= len, w[i - 1], w[i]);
+        }
         p[0] = pt;
         w[0] = wt;
         len = len + 1 < 4 ? len + 1 : 4;
     }
     __device__ void remove(uint32_t index) {
-        for (uint32_t i = index; i + 1 < len; ++i) { p[i] = p[i + 1]; w[i] = w[i + 1]; }
+#pragma unroll
+        for (uint32_t i = 0; i < 3; ++i) {
+            const bool shift = i >= index && i + 1 < len;
+            p[i] = sel_f3(shift, p[i + 1], p[i]);
+            w[i] = sel_f3(shift, w[i + 1], w[i]);
+        }
         len -= 1;
     }
     __device__ void swap12() {
@@ -172,9 +183,11 @@ __global__ void __launch_bounds__(kBlock) k_narrow(BodyView bv, NarrowOut o, flo
         uint32_t count = 0;
         const bool boxa = a.tag == kCube || a.tag == kBox, boxb = b.tag == kCube || b.tag == kBox;
         if (boxa && boxb) {
-            Manifold m;
+            Manifold m{};
             if (box_manifold(a, b, m)) {
-                for (uint32_t k = 0; k < m.count; ++k) write_slot(o, p * 4 + k, m.normal, m.depth, m.pa[k], m.pb[k]);
+#pragma unroll
+                for (uint32_t k = 0; k < 4; ++k)
+                    if (k < m.count) write_slot(o, p * 4 + k, m.normal, m.depth, m.pa[k], m.pb[k]);
                 count = m.count;
             }
         } else if (a.tag == kSphere && b.tag == kSphere) {

@@ -71,13 +71,16 @@ DIO_HD f3 support_host(const ShapeRef& s, f3 d, const HullPool& hp) {
         return f3{d.x >= 0.0f ? h : -h, d.y >= 0.0f ? h : -h, d.z >= 0.0f ? h : -h};
     }
     if (s.tag == kTetrahedron) {
-        int best = 0;
-        int32_t bk = total_key(dot3(xyz(s.c[0]), d));
+        // the winning corner itself, not its index (a runtime index into the
+        // corners would place the shape in scratch memory)
+        f3 best = xyz(s.c[0]);
+        int32_t bk = total_key(dot3(best, d));
         for (int i = 1; i < 4; ++i) {
-            const int32_t k = total_key(dot3(xyz(s.c[i]), d));
-            if (k >= bk) { bk = k; best = i; }
+            const f3 cc = xyz(s.c[i]);
+            const int32_t k = total_key(dot3(cc, d));
+            if (k >= bk) { bk = k; best = cc; }
         }
-        return xyz(s.c[best]);
+        return best;
     }
     return support_local(s, d, hp);
 }
@@ -118,8 +121,26 @@ DIO_HD void box_axes(q4 r, f3 ax[3]) {
     ax[1] = qrot(r, f3{0.0f, 1.0f, 0.0f});
     ax[2] = qrot(r, f3{0.0f, 0.0f, 1.0f});
 }
+// One of three values for a runtime k, by selects on the values (a runtime
+// index into a local array would place the array in scratch memory).
+DIO_HD f3 sel3(f3 a0, f3 a1, f3 a2, uint32_t k) {
+    f3 r = a2;
+    r.x = k == 1 ? a1.x : r.x; r.y = k == 1 ? a1.y : r.y; r.z = k == 1 ? a1.z : r.z;
+    r.x = k == 0 ? a0.x : r.x; r.y = k == 0 ? a0.y : r.y; r.z = k == 0 ? a0.z : r.z;
+    return r;
+}
+DIO_HD f3 pick3(const f3 a[3], uint32_t k) { return sel3(a[0], a[1], a[2], k); }
+// The conditionals select values, not lvalues: a ?: over struct members is an
+// lvalue in C++, compiled to a load through a selected address, which keeps
+// the struct in scratch memory.
 DIO_HD float box_half(const ShapeRef& s, uint32_t axis) {
-    return s.tag == kCube ? s.c[0].x : (axis == 0 ? s.c[0].x : (axis == 1 ? s.c[0].y : s.c[0].z));
+    const float4 c0 = s.c[0];
+    const float hx = c0.x, hy = c0.y, hz = c0.z;
+    float r = hz;
+    r = axis == 1 ? hy : r;
+    r = axis == 0 ? hx : r;
+    r = s.tag == kCube ? hx : r;
+    return r;
 }
 // cuboid.rs:37-40
 DIO_HD float box_radius(const ShapeRef& s, const f3 ax[3], f3 axis) {
@@ -178,14 +199,14 @@ DIO_HD void box_face(const ShapeRef& s, uint32_t ai, f3 outward, f3 verts[4], f3
                      float& ev) {
     f3 ax[3];
     box_axes(s.rot, ax);
-    const f3 axis = ax[ai];
+    const f3 axis = pick3(ax, ai);
     const float hn = box_half(s, ai);
     const f3 fnrm = scale(axis, dot3(axis, outward) >= 0.0f ? 1.0f : -1.0f);
     center = add(s.pos, scale(fnrm, hn));
     const uint32_t iu = ai == 0 ? 1u : 0u;
     const uint32_t iv = ai == 2 ? 1u : 2u;
-    u = ax[iu];
-    v = ax[iv];
+    u = pick3(ax, iu);
+    v = pick3(ax, iv);
     eu = box_half(s, iu);
     ev = box_half(s, iv);
     verts[0] = add(add(center, scale(u, eu)), scale(v, ev));
@@ -207,27 +228,52 @@ DIO_HD uint32_t box_incident_axis(const ShapeRef& s, f3 n) {
     return best;
 }
 
-// cuboid.rs:216-250
-DIO_HD uint32_t clip_plane(const f3* in, uint32_t in_count, f3 n, float offset, f3* out) {
+// Fixed 8-slot polygons kept in registers: every loop below is unrolled so
+// each slot index is static, and a point goes to slot k by selects (a runtime
+// index into a local array would place the array in scratch memory).
+struct Poly8 {
+    f3 v[8];
+};
+DIO_HD float fsel(bool c, float a, float b) { return c ? a : b; }
+DIO_HD f3 sel_f3(bool c, f3 a, f3 b) { return f3{fsel(c, a.x, b.x), fsel(c, a.y, b.y), fsel(c, a.z, b.z)}; }
+// every slot is stored, with the new or its old value: conditional stores to
+// the slots would be merged into one store at a runtime address
+DIO_HD void put8(Poly8& p, uint32_t k, f3 x) {
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) p.v[j] = sel_f3(j == k, x, p.v[j]);
+}
+
+// cuboid.rs:216-250: Sutherland-Hodgman against one plane, at most 8 points out
+DIO_HD uint32_t clip_plane(const Poly8& in, uint32_t in_count, f3 n, float offset, Poly8& out) {
     if (in_count == 0) return 0;
     uint32_t count = 0;
-    f3 prev = in[in_count - 1];
+    f3 prev = in.v[0];
+#pragma unroll
+    for (uint32_t j = 1; j < 8; ++j)
+        if (j == in_count - 1) prev = in.v[j];
     float pd = dot3(n, prev) - offset;
-    for (uint32_t i = 0; i < in_count; ++i) {
-        const f3 cur = in[i];
-        const float cd = dot3(n, cur) - offset;
-        const bool pin = pd <= 1.0e-5f;
-        const bool cin = cd <= 1.0e-5f;
-        if (pin != cin) {
-            const float den = pd - cd;
-            if (fabsf(den) > 1.0e-8f && count < 8) {
-                const float t = rclamp(pd / den, 0.0f, 1.0f);
-                out[count++] = add(prev, scale(sub(cur, prev), t));
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        if (i < in_count) {
+            const f3 cur = in.v[i];
+            const float cd = dot3(n, cur) - offset;
+            const bool pin = pd <= 1.0e-5f;
+            const bool cin = cd <= 1.0e-5f;
+            if (pin != cin) {
+                const float den = pd - cd;
+                if (fabsf(den) > 1.0e-8f && count < 8) {
+                    const float t = rclamp(pd / den, 0.0f, 1.0f);
+                    put8(out, count, add(prev, scale(sub(cur, prev), t)));
+                    ++count;
+                }
             }
+            if (cin && count < 8) {
+                put8(out, count, cur);
+                ++count;
+            }
+            prev = cur;
+            pd = cd;
         }
-        if (cin && count < 8) out[count++] = cur;
-        prev = cur;
-        pd = cd;
     }
     return count;
 }
@@ -253,6 +299,21 @@ DIO_HD void centered_contact(const ShapeRef& a, const ShapeRef& b, f3 n, float d
     m.count = 1;
 }
 
+// c ? a : b member by member, on values (a ?: over two struct references is
+// an lvalue, compiled to a load through a selected address)
+DIO_HD ShapeRef sel_shape(bool c, const ShapeRef& a, const ShapeRef& b) {
+    ShapeRef r;
+    r.pos = f3{fsel(c, a.pos.x, b.pos.x), fsel(c, a.pos.y, b.pos.y), fsel(c, a.pos.z, b.pos.z)};
+    r.rot = q4{fsel(c, a.rot.x, b.rot.x), fsel(c, a.rot.y, b.rot.y), fsel(c, a.rot.z, b.rot.z),
+               fsel(c, a.rot.w, b.rot.w)};
+    r.tag = c ? a.tag + 0u : b.tag + 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r.c[k] = make_float4(fsel(c, a.c[k].x, b.c[k].x), fsel(c, a.c[k].y, b.c[k].y), fsel(c, a.c[k].z, b.c[k].z),
+                             fsel(c, a.c[k].w, b.c[k].w));
+    return r;
+}
+
 // cuboid.rs:276-377
 DIO_HD bool box_manifold(const ShapeRef& a, const ShapeRef& b, Manifold& m) {
     SatBest best;
@@ -265,16 +326,21 @@ DIO_HD bool box_manifold(const ShapeRef& a, const ShapeRef& b, Manifold& m) {
     }
     const bool ref_is_a = best.kind == 0;
     const f3 rn = ref_is_a ? normal : neg(normal);
-    const ShapeRef& rb = ref_is_a ? a : b;
-    const ShapeRef& ib = ref_is_a ? b : a;
+    const ShapeRef rb = sel_shape(ref_is_a, a, b);
+    const ShapeRef ib = sel_shape(ref_is_a, b, a);
     const uint32_t inc = box_incident_axis(ib, rn);
     f3 rface[4], center, u, v;
     float eu, ev;
     box_face(rb, best.index, rn, rface, center, u, v, eu, ev);
-    f3 pa[8], pb[8], iface[4], c2, u2, v2;
+    f3 iface[4], c2, u2, v2;
     float eu2, ev2;
     box_face(ib, inc, neg(rn), iface, c2, u2, v2, eu2, ev2);
-    for (int i = 0; i < 4; ++i) pa[i] = iface[i];
+    Poly8 pa, pb;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        pa.v[i] = i < 4 ? iface[i & 3] : f3{0.0f, 0.0f, 0.0f};
+        pb.v[i] = f3{0.0f, 0.0f, 0.0f};
+    }
     uint32_t count = 4;
     count = clip_plane(pa, count, u, dot3(u, center) + eu, pb);
     count = clip_plane(pb, count, neg(u), dot3(neg(u), center) + eu, pa);
@@ -286,15 +352,22 @@ DIO_HD bool box_manifold(const ShapeRef& a, const ShapeRef& b, Manifold& m) {
     }
     const float rplane = dot3(rn, center);
     uint32_t cc = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        if (cc >= 4) break;
-        const f3 ip = pa[i];
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        // point i is kept when it exists, fewer than 4 are kept, and it lies
+        // no more than 1e-5 above the reference face
+        const f3 ip = pa.v[i];
         const float pd = rplane - dot3(rn, ip);
-        if (pd < -1.0e-5f) continue;
+        const bool take = i < count && cc < 4 && !(pd < -1.0e-5f);
         const f3 rp = add(ip, scale(rn, rmax(pd, 0.0f)));
-        m.pa[cc] = ref_is_a ? rp : ip;
-        m.pb[cc] = ref_is_a ? ip : rp;
-        ++cc;
+        const f3 xa = sel_f3(ref_is_a, rp, ip);
+        const f3 xb = sel_f3(ref_is_a, ip, rp);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            m.pa[j] = sel_f3(take && j == cc, xa, m.pa[j]);
+            m.pb[j] = sel_f3(take && j == cc, xb, m.pb[j]);
+        }
+        cc += take ? 1u : 0u;
     }
     if (cc == 0) {
         centered_contact(a, b, normal, depth, m);
