@@ -171,6 +171,37 @@ def coloured_leg(args, tuning: dict) -> dict:
     }
 
 
+def overlap_leg(args, tuning: dict) -> dict:
+    """The same cube field with the next step's broad and narrow phases on a
+    second stream beside the contact build and CSR (tuning overlap_broad):
+    a shorter step, while the second hardware queue slows the dispatch of
+    the AVBD launches that follow."""
+    from diomedes_amd import PhysicsContext, scenes
+
+    scene = scenes.cube_field(args.seed)
+    opts = scene.options
+    k = args.steps
+    with PhysicsContext(scene.n) as ctx:
+        ctx.set_tuning(**tuning)
+        ctx.set_tuning(overlap_broad=1)
+        ctx.upload(scene)
+        if args.warmup:
+            ctx.step(opts, args.warmup)
+        ctx.step(opts, 0)
+        t0 = time.perf_counter()
+        ctx.step(opts, k)
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+    _progress(f"overlap leg: {k} steps in {wall:.2f} s")
+    return {
+        "workload": "the same cube field, broad + narrow overlapped with the contact build (overlap_broad = 1)",
+        "value": scene.n_dynamic * opts.iterations * k / (st.ms_avbd / 1000.0),
+        "unit": "body-iters/s",
+        "ms_per_step": 1000.0 * wall / k,
+        "ms_per_step_avbd": st.ms_avbd / k,
+    }
+
+
 def mixed_leg(args, tuning: dict) -> dict:
     """The mixed workload: 20000 hulls (42-vertex icospheres), tetrahedra and
     boxes with N(0, 2) velocities in a 60 m container, 128 ropes of 64 links and
@@ -473,7 +504,8 @@ def main() -> int:
     ap.add_argument("--no-coloured", action="store_true", help="skip the coloured Gauss-Seidel leg")
     ap.add_argument("--mixed-steps", type=int, default=300, help="steps of the mixed-field leg (0: skip it)")
     ap.add_argument("--joint-steps", type=int, default=120, help="steps of the hard-joints leg (0: skip it)")
-    ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint", "mixed", "joints"), default=None,
+    ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint", "mixed", "joints", "overlap"),
+                    default=None,
                     help="run only this leg and print its object (for profiling it alone)")
     ap.add_argument("--adjoint-seeds", type=int, default=64,
                     help="wall throws of the adjoint leg (0: skip it)")
@@ -494,7 +526,7 @@ def main() -> int:
     if args.only_leg:
         legs = {"coloured": lambda: coloured_leg(args, {}), "batched": lambda: batched_leg(args, {}),
                 "adjoint": lambda: adjoint_leg(args), "mixed": lambda: mixed_leg(args, {}),
-                "joints": lambda: joints_leg(args, {})}
+                "joints": lambda: joints_leg(args, {}), "overlap": lambda: overlap_leg(args, {})}
         print(json.dumps({"leg": args.only_leg, **legs[args.only_leg]()}), flush=True)
         return 0
 
@@ -610,6 +642,7 @@ def main() -> int:
             "dual_recover": 1000.0 * pst.ms_recover / max(pst.n_recover, 1),
         },
     }
+    line["overlap"] = overlap_leg(args, tuning)
     if not args.no_coloured:
         line["coloured"] = coloured_leg(args, tuning)
     if args.mixed_steps:

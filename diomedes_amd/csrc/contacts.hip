@@ -356,13 +356,16 @@ __global__ void __launch_bounds__(kBlock) k_unpack_contacts(const dio_avbd_conta
 
 }  // namespace
 
-void run_contact_build(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const NarrowSource& src,
-                       const ContactSet& dst, const NarrowSource& old_src, const ContactSet& old,
-                       uint32_t* scan_partials, uint32_t* overflow, hipStream_t s, uint32_t* csr_cnt,
-                       uint32_t* csr_cur) {
+void run_contact_pairs(const WorldView& w, const BuildOptions& o, const NarrowSource& src, const ContactSet& dst,
+                       const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow, hipStream_t s) {
     uint32_t pb = blocks_for(dst.m_cap > 0 ? dst.m_cap / 4 + 1 : 1);
     if (pb > 2048) pb = 2048;
     hipLaunchKernelGGL(k_contacts_narrow, dim3(pb), dim3(kBlock), 0, s, w, o, src, dst, old_src, old, overflow);
+}
+
+void run_contact_walls(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const ContactSet& dst,
+                       const ContactSet& old, uint32_t* scan_partials, uint32_t* overflow, hipStream_t s,
+                       uint32_t* csr_cnt, uint32_t* csr_cur) {
     const uint32_t nb = blocks_for((w.n > 0 ? w.n : 1) * (uint32_t)kWallLanes);
     hipLaunchKernelGGL(k_container_count, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst.cont_cnt, csr_cnt,
                        csr_cnt ? csr_cur : nullptr);

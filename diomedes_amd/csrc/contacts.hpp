@@ -54,7 +54,7 @@ struct WorldView {
     const float4* hull;
     uint32_t n;
     // batched world: each body's scene and each scene's container (null: the
-    // single container passed to run_contact_build)
+    // single container passed to run_contact_walls)
     const uint32_t* scene_of = nullptr;
     const ContainerParams* scene_ct = nullptr;
 };
@@ -80,14 +80,18 @@ void launch_csr_peers(uint32_t n, const uint32_t* off, const uint32_t* idx, cons
                       hipStream_t s);
 
 // Build `dst` from the narrow output `src` and the container, warm-started
-// from `old` (built from `old_src`).  `overflow` records a capacity excess.
-void run_contact_build(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const NarrowSource& src,
-                       const ContactSet& dst, const NarrowSource& old_src, const ContactSet& old,
-                       uint32_t* scan_partials, uint32_t* overflow, hipStream_t s, uint32_t* csr_cnt = nullptr,
+// from `old` (built from `old_src`); `overflow` records a capacity excess.
+// Two parts: the narrow-phase contacts (the only part that reads old_src, the
+// previous step's pair list and offsets), then the container contacts.
+void run_contact_pairs(const WorldView& w, const BuildOptions& o, const NarrowSource& src, const ContactSet& dst,
+                       const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow, hipStream_t s);
+void run_contact_walls(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const ContactSet& dst,
+                       const ContactSet& old, uint32_t* scan_partials, uint32_t* overflow, hipStream_t s,
+                       uint32_t* csr_cnt = nullptr,
                        uint32_t* csr_cur = nullptr);  // csr_cnt / csr_cur (n words each): zeroed on the way
 
 // CSR of a contact set (physics/gpu.rs:286-310).
-// counters_cleared: b.cnt and b.cur were zeroed by run_contact_build (its
+// counters_cleared: b.cnt and b.cur were zeroed by run_contact_walls (its
 // csr argument); otherwise they are zeroed here.
 void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s,
              bool counters_cleared = false);

@@ -179,6 +179,10 @@ struct JointDev {
 struct dio_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the second stream of a step (the next step's broad and narrow phases
+    // beside the contact build and CSR) and its fork / join events
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     uint64_t required = 0;
     dio_limits lim{};
@@ -223,7 +227,7 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{32, 8, 4, 1, 8, {0, 0, 0}};
+    dio_tuning tuning{32, 8, 4, 1, 8, 0, {0, 0}};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -314,12 +318,16 @@ struct dio_ctx {
     }
     ~dio_ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (stream2) (void)hipStreamSynchronize(stream2);
         free_adjoint();
         drop_graphs();
         for (auto e : events) (void)hipEventDestroy(e);
         for (auto e : prof_events) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         dev.release();
         pin.release();
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -590,6 +598,9 @@ dio_status dio_create(int device, const dio_limits* limits, dio_ctx** out) {
         c->device = device;
         DIO_HIP_CHECK(hipSetDevice(device));
         DIO_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        DIO_HIP_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        DIO_HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        DIO_HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         c->lim = *limits;
         if (limits->max_bodies > kMaxBroadBodies) capacity("max_bodies exceeds the broad phase limit", kMaxBroadBodies);
         c->n_cap = limits->max_bodies;
@@ -1004,24 +1015,43 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         // step's narrow output goes to pend[s] and is used at once.
         const int src = s;
         const int dst_pend = lag ? 1 - s : s;
-        auto broad_narrow = [&]() {
+        auto broad_narrow = [&](hipStream_t bs) {
             BroadBuffers b = c->wb;
             b.pairs = c->pend[dst_pend].pairs;
             b.p_dev = c->pend[dst_pend].p_dev;
             b.overflow = c->w_overflow;
             b.clear_words[0] = c->pend[dst_pend].out.gjk_count;
             b.clear_words[1] = c->pend[dst_pend].out.epa_retry_count;
-            run_broad(bv, n, b, c->p_max, st, resort, (int)c->tuning.broad_lanes, scenes);
-            run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, st, true);
+            run_broad(bv, n, b, c->p_max, bs, resort, (int)c->tuning.broad_lanes, scenes);
+            run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, bs, true);
         };
-        if (!lag) broad_narrow();
-        run_contact_build(wv, c->w_ct, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]),
-                          c->cs[1 - s], c->scan_partials, c->w_overflow, st, c->csr.cnt, c->csr.cur);
+        if (!lag) broad_narrow(st);
+        run_contact_pairs(wv, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]), c->cs[1 - s],
+                          c->w_overflow, st);
+        // With lag 1, the next step's broad and narrow phases read only the
+        // poses, which nothing before the AVBD segment writes, and write
+        // pend[1 - s], which the contact build has finished reading: with
+        // overlap_broad they run on the second stream beside the container
+        // contacts and CSR (both chains are short launches on few
+        // workgroups), joined before the segment ends.
+        const bool fork = lag && c->tuning.overlap_broad;
+        if (fork) {
+            DIO_HIP_CHECK(hipEventRecord(c->ev_fork, st));
+            DIO_HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+            broad_narrow(c->stream2);
+        }
+        run_contact_walls(wv, c->w_ct, bo, c->cs[s], c->cs[1 - s], c->scan_partials, c->w_overflow, st, c->csr.cnt,
+                          c->csr.cur);
         run_csr(c->cs[s], n, c->csr, c->w_overflow, st, n > 0);
         if (c->adj_on)
             launch_record_step(c->adj_rec, c->cs[s], c->csr.off, c->csr.idx, c->csr.peer,
                                BodyState{c->w_pos, c->w_rot, c->w_vel, c->w_ang, c->w_prev}, n, c->m_cap, st);
-        if (lag) broad_narrow();
+        if (fork) {
+            DIO_HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+            DIO_HIP_CHECK(hipStreamWaitEvent(st, c->ev_join, 0));
+        } else if (lag) {
+            broad_narrow(st);
+        }
     } else {
         // the init of this step ran already (the previous step's final launch,
         // or dio_step before this segment)
@@ -1477,6 +1507,7 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         if (v.adjoint_lanes != 1 && v.adjoint_lanes != 2 && v.adjoint_lanes != 4 && v.adjoint_lanes != 8 &&
             v.adjoint_lanes != 16)
             invalid("adjoint_lanes must be 1, 2, 4, 8 or 16");
+        if (v.overlap_broad > 1) invalid("overlap_broad must be 0 or 1");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();

@@ -579,7 +579,7 @@ class PhysicsContext:
         self._check(self._lib.dio_step(self._h, ctypes.byref(o), ctypes.byref(so), nsteps))
 
     def set_tuning(self, primal_lanes: int = 0, broad_lanes: int = 0, resort_interval: int = 0,
-                   fuse_dual: int | None = None, adjoint_lanes: int = 0) -> None:
+                   fuse_dual: int | None = None, adjoint_lanes: int = 0, overlap_broad: int | None = None) -> None:
         """Launch shapes and schedule (0 / None keeps the current value)."""
         cur = self.tuning()
         t = R.Tuning()
@@ -588,13 +588,14 @@ class PhysicsContext:
         t.resort_interval = resort_interval or cur["resort_interval"]
         t.fuse_dual = cur["fuse_dual"] if fuse_dual is None else int(fuse_dual)
         t.adjoint_lanes = adjoint_lanes or cur["adjoint_lanes"]
+        t.overlap_broad = cur["overlap_broad"] if overlap_broad is None else int(overlap_broad)
         self._check(self._lib.dio_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
         t = R.Tuning()
         self._check(self._lib.dio_get_tuning(self._h, ctypes.byref(t)))
         return {"primal_lanes": t.primal_lanes, "broad_lanes": t.broad_lanes, "resort_interval": t.resort_interval,
-                "fuse_dual": t.fuse_dual, "adjoint_lanes": t.adjoint_lanes}
+                "fuse_dual": t.fuse_dual, "adjoint_lanes": t.adjoint_lanes, "overlap_broad": t.overlap_broad}
 
     def autotune(self, options: AvbdOptions, reps: int = 20) -> dict:
         """Time every primal and broad-phase launch shape and both dual

@@ -235,7 +235,11 @@ typedef struct dio_tuning {
                                   0: a dual launch after each primal */
     uint32_t adjoint_lanes;    /* lanes per body in the adjoint's reverse primal: 1, 2, 4, 8 or 16
                                   (0 = default 8) */
-    uint32_t reserved[3];
+    uint32_t overlap_broad;    /* 1: with contact lag 1, the next step's broad and narrow phases run
+                                  on a second stream beside this step's contact build and CSR (a
+                                  shorter step; the second hardware queue slows the dispatch of the
+                                  AVBD launches that follow); 0 (default): after them, on one stream */
+    uint32_t reserved[2];
 } dio_tuning;
 
 typedef struct dio_ctx dio_ctx;

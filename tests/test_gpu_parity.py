@@ -456,6 +456,29 @@ def test_options_change_between_calls(oracle):
                 w.set_mode(0)
 
 
+@pytest.mark.parametrize("name", ["field", "mixed"])
+def test_overlapped_broad_phase_bitexact(oracle, name):
+    """overlap_broad: the next step's broad and narrow phases on a second
+    stream beside the contact build; the step stays the oracle's, bit for bit
+    (graph-captured, and switched off and on between calls)."""
+    if name == "field":
+        scene = scenes.cube_field(5, nx=10, ny=3, nz=10)
+    else:
+        scene = scenes.mixed_field(4, n=600, ropes=4, rope_links=8, springs=16)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n, max_hull_vertices=len(scene.hull), max_constraints=max(16, len(scene.constraints))) as ctx:
+        ctx.set_tuning(overlap_broad=1)
+        assert ctx.tuning()["overlap_broad"] == 1
+        ctx.upload(scene)
+        for k, overlap in enumerate((1, 1, 0, 1)):
+            ctx.set_tuning(overlap_broad=overlap)
+            ctx.step(scene.options, 12)
+            w.step(nsteps=12)
+            _compare_world(ctx, w, 12 * (k + 1))
+            _print(name, "overlap", overlap, "step", 12 * (k + 1), "pairs", ctx.stats().pairs, "contacts",
+                   ctx.stats().contacts)
+
+
 @pytest.mark.parametrize("fuse,lanes", [(0, 32), (1, 32), (1, 1)])
 def test_static_contacts_bitexact(oracle, fuse, lanes):
     """Static bodies in contact with each other and with dynamic bodies: in the
