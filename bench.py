@@ -502,6 +502,7 @@ def main() -> int:
                     help="steps of a separate run of the same scene after which dio_autotune picks the launch "
                          "shapes and dual schedule (0: library defaults)")
     ap.add_argument("--no-coloured", action="store_true", help="skip the coloured Gauss-Seidel leg")
+    ap.add_argument("--no-overlap", action="store_true", help="skip the overlapped broad-phase leg")
     ap.add_argument("--mixed-steps", type=int, default=300, help="steps of the mixed-field leg (0: skip it)")
     ap.add_argument("--joint-steps", type=int, default=120, help="steps of the hard-joints leg (0: skip it)")
     ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint", "mixed", "joints", "overlap"),
@@ -642,7 +643,8 @@ def main() -> int:
             "dual_recover": 1000.0 * pst.ms_recover / max(pst.n_recover, 1),
         },
     }
-    line["overlap"] = overlap_leg(args, tuning)
+    if not args.no_overlap:
+        line["overlap"] = overlap_leg(args, tuning)
     if not args.no_coloured:
         line["coloured"] = coloured_leg(args, tuning)
     if args.mixed_steps:
