@@ -235,11 +235,13 @@ def mixed_leg(args, tuning: dict) -> dict:
 
 
 def _joint_error(scene, pos, rot) -> float:
-    """Worst violation over the scene's joints: |p_a - p_b| for a socket,
-    ||delta| - L| for a rod, max(|delta| - L, 0) for a rope (metres)."""
+    """Worst violation over the scene's positional joints: |p_a - p_b| for a
+    socket, ||delta| - L| for a rod, max(|delta| - L, 0) for a rope (metres;
+    bending joints, whose anchor field holds a rest rotation, are left out)."""
     import numpy as np
 
     c = scene.constraints
+    c = c[(c["kind"] != 7) & (c["kind"] != 8)]
 
     def world(idx, anchor):
         q = rot[idx].astype(np.float64)

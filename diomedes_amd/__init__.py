@@ -14,7 +14,9 @@ from .physics import (
     AvbdOptions,
     AvbdSolver,
     BallSocket,
+    Bend,
     HardBallSocket,
+    HardBend,
     HardRod,
     HardRope,
     PhysicsContext,
@@ -36,7 +38,9 @@ __all__ = [
     "AvbdOptions",
     "AvbdSolver",
     "BallSocket",
+    "Bend",
     "HardBallSocket",
+    "HardBend",
     "HardRod",
     "HardRope",
     "PhysicsContext",

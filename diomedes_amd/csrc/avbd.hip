@@ -183,7 +183,7 @@ struct HardRows {
 __device__ __forceinline__ HardRows hard_rows(uint32_t kind, f3 delta, f3 delta0, float rest, float alpha) {
     HardRows r;
     r.dir = f3{0.0f, 0.0f, 0.0f};
-    if (kind == kHardSocket) {
+    if (three_rows(kind)) {
         r.chat = f3{delta.x - alpha * delta0.x, delta.y - alpha * delta0.y, delta.z - alpha * delta0.z};
         r.ok = true;
     } else {
@@ -211,7 +211,7 @@ __device__ __forceinline__ void hard_update(const AvbdParams& p, uint32_t kind, 
                                             float4& pen) {
     if (!r.ok) return;
     hard_row_update(p, kind, r.chat.x, lam.x, pen.x);
-    if (kind == kHardSocket) {
+    if (three_rows(kind)) {
         hard_row_update(p, kind, r.chat.y, lam.y, pen.y);
         hard_row_update(p, kind, r.chat.z, lam.z, pen.z);
     }
@@ -226,9 +226,11 @@ __device__ __forceinline__ void hard_warm(const AvbdParams& p, float4 lam, float
                           rmin(rmax(pen.y * p.gamma, p.penalty_min), p.penalty_max),
                           rmin(rmax(pen.z * p.gamma, p.penalty_min), p.penalty_max), 0.0f);
 }
-// anchor separation p_a - p_b of a joint from a pose buffer pair
-__device__ __forceinline__ f3 joint_delta(const float4* P, const float4* Q, uint4 abk, f3 aa, f3 ab) {
-    return sub(add(xyz(P[abk.x]), qrot(as_q(Q[abk.x]), aa)), add(xyz(P[abk.y]), qrot(as_q(Q[abk.y]), ab)));
+// the value vector of a hard joint from a pose buffer pair: the anchor
+// separation p_a - p_b, or a bending joint's error (aa4 = the rest rotation)
+__device__ __forceinline__ f3 joint_delta(const float4* P, const float4* Q, uint4 abk, float4 aa4, f3 ab) {
+    if (is_bend(abk.z)) return bend_error(as_q(Q[abk.x]), as_q(aa4), as_q(Q[abk.y]));
+    return sub(add(xyz(P[abk.x]), qrot(as_q(Q[abk.x]), xyz(aa4))), add(xyz(P[abk.y]), qrot(as_q(Q[abk.y]), ab)));
 }
 
 // The multipliers after j dual updates live in buffer j % 2 (0: the contact
@@ -415,8 +417,32 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
             const f3 delta = sub(p_a, p_b);
             const f3 r_self = qrot(snap_rot, self_anchor);
             const float stiff = ab4.w;
-            if (is_hard(abk.z)) {
-                const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa, abn);
+            if (is_bend(abk.z)) {
+                // three angular rows: j_lin = 0, j_ang = +-e_r
+                const f3 err = bend_error(qa, as_q(aa4), qb);
+                const f3 zl{0.0f, 0.0f, 0.0f};
+                float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pen = make_float4(stiff, stiff, stiff, 0.0f);
+                f3 fr{err.x * stiff, err.y * stiff, err.z * stiff};
+                if (abk.z == kHardBend) {
+                    const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, abn);
+                    const HardRows hr = hard_rows(abk.z, err, d0, aa4.w, p.alpha);
+                    lam = v.j_lam[jin][cidx];
+                    pen = v.j_pen[jin][cidx];
+                    if (fuse) {
+                        hard_update(p, abk.z, hr, lam, pen);
+                        if (is_a || v.inv_mm[abk.x].w == 0.0f) {
+                            v.j_lam[iter & 1u][cidx] = lam;
+                            v.j_pen[iter & 1u][cidx] = pen;
+                        }
+                    }
+                    fr = f3{hard_force(abk.z, pen.x, hr.chat.x, lam.x), hard_force(abk.z, pen.y, hr.chat.y, lam.y),
+                            hard_force(abk.z, pen.z, hr.chat.z, lam.z)};
+                }
+                B.row(zl, scale(f3{1.0f, 0.0f, 0.0f}, sign), fr.x, pen.x);
+                B.row(zl, scale(f3{0.0f, 1.0f, 0.0f}, sign), fr.y, pen.y);
+                B.row(zl, scale(f3{0.0f, 0.0f, 1.0f}, sign), fr.z, pen.z);
+            } else if (is_hard(abk.z)) {
+                const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, abn);
                 const HardRows hr = hard_rows(abk.z, delta, d0, aa4.w, p.alpha);
                 float4 lam = v.j_lam[jin][cidx];
                 float4 pen = v.j_pen[jin][cidx];
@@ -664,8 +690,41 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
                 const f3 r_self = qrot(snap_rot, is_a ? aa : abn);
                 const float sign = is_a ? 1.0f : -1.0f;
                 const float stiff = ab4.w;
-                if (HARD && is_hard(abk.z)) {
-                    const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa, abn);
+                if (HARD && is_bend(abk.z)) {
+                    // three angular rows: j_lin = 0, j_ang = +-e_r
+                    const f3 err = bend_error(qa, as_q(aa4), qb);
+                    float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pen = make_float4(stiff, stiff, stiff, 0.0f);
+                    float fs[3] = {err.x * stiff, err.y * stiff, err.z * stiff};
+                    if (abk.z == kHardBend) {
+                        const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, abn);
+                        const HardRows hr = hard_rows(abk.z, err, d0, aa4.w, p.alpha);
+                        const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
+                        lam = v.j_lam[jin][cidx];
+                        pen = v.j_pen[jin][cidx];
+                        if (fuse) {
+                            hard_update(p, abk.z, hr, lam, pen);
+                            if (is_a || v.inv_mm[abk.x].w == 0.0f) {
+                                v.j_lam[iter & 1u][cidx] = lam;
+                                v.j_pen[iter & 1u][cidx] = pen;
+                            }
+                        }
+                        fs[0] = hard_force(abk.z, pen.x, hr.chat.x, lam.x);
+                        fs[1] = hard_force(abk.z, pen.y, hr.chat.y, lam.y);
+                        fs[2] = hard_force(abk.z, pen.z, hr.chat.z, lam.z);
+                    }
+                    const float ks[3] = {pen.x, pen.y, pen.z};
+                    // e_r * sign, as the one-lane loop forms it (zeros keep the sign's sign)
+                    const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
+                                      scale(f3{0.0f, 0.0f, 1.0f}, sign)};
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        st[8 * r + 0] = 0.0f; st[8 * r + 1] = 0.0f; st[8 * r + 2] = 0.0f;
+                        st[8 * r + 3] = js[r].x; st[8 * r + 4] = js[r].y; st[8 * r + 5] = js[r].z;
+                        st[8 * r + 6] = fs[r]; st[8 * r + 7] = ks[r];
+                    }
+                    nrows = 3;
+                } else if (HARD && is_hard(abk.z)) {
+                    const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, abn);
                     const HardRows hr = hard_rows(abk.z, delta, d0, aa4.w, p.alpha);
                     const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
                     float4 lam = v.j_lam[jin][cidx];
@@ -841,13 +900,11 @@ struct JointMults {
 __device__ __forceinline__ JointMults joint_dual_one(const SolveView& v, const AvbdParams& p, uint32_t cj,
                                                      uint32_t parity, uint32_t src, uint32_t static_updates) {
     const uint4 abk = v.j_abk[cj];
-    const f3 aa = xyz(v.j_anc_a[cj]);
-    const float4 ab4 = v.j_anc_b[cj];
-    const float rest = v.j_anc_a[cj].w;
-    const f3 ab = xyz(ab4);
-    const f3 delta = joint_delta(v.pos[1 - parity], v.rot[1 - parity], abk, aa, ab);
-    const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa, ab);
-    const HardRows hr = hard_rows(abk.z, delta, d0, rest, p.alpha);
+    const float4 aa4 = v.j_anc_a[cj];
+    const f3 ab = xyz(v.j_anc_b[cj]);
+    const f3 delta = joint_delta(v.pos[1 - parity], v.rot[1 - parity], abk, aa4, ab);
+    const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, ab);
+    const HardRows hr = hard_rows(abk.z, delta, d0, aa4.w, p.alpha);
     uint32_t reps = 1, from = src;
     if (static_updates > 0u && v.inv_mm[abk.x].w == 0.0f && v.inv_mm[abk.y].w == 0.0f) {
         reps = static_updates;

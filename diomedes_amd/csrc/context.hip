@@ -172,6 +172,7 @@ struct JointDev {
     float4* st_lam = nullptr;
     float4* st_pen = nullptr;
     uint32_t hard = 0, hard_dynamic = 0;
+    uint32_t extended = 0;  // joints of kinds past the reference's four (hard and bending joints)
 };
 
 }  // namespace
@@ -810,8 +811,9 @@ dio_status dio_avbd_submit(dio_ctx* ctx, const dio_avbd_body_state* state, const
             if (contacts[k].a >= n || (contacts[k].b != DIO_CONTAINER && contacts[k].b >= n))
                 invalid("contact body index out of range");
         for (uint32_t k = 0; k < c; ++k)
-            if (is_hard(constraints[k].kind))
-                invalid("hard joints need a world (dio_world_upload): their multipliers persist between steps");
+            if (constraints[k].kind > kRope)
+                invalid("hard and bending joints need a world (dio_world_upload): hard joints' multipliers persist "
+                        "between steps");
         for (uint32_t k = 0; k < nidx; ++k)
             if (indices[k] >= m) invalid("CSR index out of range");
         for (uint32_t i = 0; i < n; ++i)
@@ -1162,8 +1164,8 @@ SolveView world_solve_view(dio_ctx* c, int s, bool prepare_next) {
         v.j_off = c->w_j.off; v.j_idx = c->w_j.idx; v.j_abk = c->w_j.abk;
         v.j_anc_a = c->w_j.anc_a; v.j_anc_b = c->w_j.anc_b;
         v.j_count = c->w_j.c;
+        v.j_hard = c->w_j.extended > 0 ? 1u : 0u;  // the primal with the hard and bending rows
         if (c->w_j.hard > 0) {
-            v.j_hard = 1;
             v.j_lam[0] = c->w_j.lam[0]; v.j_lam[1] = c->w_j.lam[1];
             v.j_pen[0] = c->w_j.pen[0]; v.j_pen[1] = c->w_j.pen[1];
             v.j_st_lam = c->w_j.st_lam; v.j_st_pen = c->w_j.st_pen;
@@ -1274,7 +1276,9 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
     upload_joints(ctx, ctx->w_j, constraints, c, n, ctx->w_stage_c);
     ctx->w_j.hard = 0;
     ctx->w_j.hard_dynamic = 0;
+    ctx->w_j.extended = 0;
     for (uint32_t k = 0; k < c; ++k) {
+        if (constraints[k].kind > kRope) ctx->w_j.extended += 1;
         if (!is_hard(constraints[k].kind)) continue;
         ctx->w_j.hard += 1;
         if (state[constraints[k].a].inv_mass != 0.0f && state[constraints[k].b].inv_mass != 0.0f &&

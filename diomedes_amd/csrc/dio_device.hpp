@@ -84,7 +84,7 @@ struct SolveView {
     float4* j_st_lam;
     float4* j_st_pen;
     uint32_t j_count;  // constraints
-    uint32_t j_hard;   // nonzero: some constraint is a hard joint
+    uint32_t j_hard;   // nonzero: some constraint is a hard or bending joint (kinds past the reference's four)
     // fused path: recover writes the final pose back into the world pose
     float4* out_pos;
     float4* out_rot;

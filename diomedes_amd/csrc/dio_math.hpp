@@ -26,7 +26,14 @@ constexpr uint32_t kSpring = 0, kRod = 1, kBallSocket = 2, kRope = 3;
 // hard joints: rod, socket and rope geometry held by a multiplier and a
 // ramped penalty per row (no counterpart in the reference's soft kinds)
 constexpr uint32_t kHardRod = 4, kHardSocket = 5, kHardRope = 6;
-DIO_HD bool is_hard(uint32_t kind) { return kind >= kHardRod && kind <= kHardRope; }
+// bending joints: b's orientation held at a's times a rest rotation (the rest
+// quaternion in anchor_a), softly (kBend, stiffness in anchor_b.w) or with
+// multipliers (kHardBend)
+constexpr uint32_t kBend = 7, kHardBend = 8;
+DIO_HD bool is_hard(uint32_t kind) { return (kind >= kHardRod && kind <= kHardRope) || kind == kHardBend; }
+DIO_HD bool is_bend(uint32_t kind) { return kind == kBend || kind == kHardBend; }
+// three rows per joint (socket and bend kinds), else one
+DIO_HD bool three_rows(uint32_t kind) { return kind == kHardSocket || kind == kHardBend; }
 
 struct f3 {
     float x, y, z;
@@ -93,6 +100,15 @@ DIO_HD q4 quat_step(q4 q, f3 d) {
 DIO_HD f3 ang_diff(q4 to, q4 from) {
     q4 rel = qmul(to, qconj(from));
     return f3{2.0f * rel.x, 2.0f * rel.y, 2.0f * rel.z};
+}
+
+// A bending joint's error: the rotation from b's orientation to a's times the
+// rest rotation, 2 vec(qa rest qb^-1) on the shorter arc (+d when a turns by a
+// small world rotation d, -d when b does).
+DIO_HD f3 bend_error(q4 qa, q4 rest, q4 qb) {
+    const q4 rel = qmul(qmul(qa, rest), qconj(qb));
+    const float s = rel.w < 0.0f ? -1.0f : 1.0f;
+    return f3{(2.0f * rel.x) * s, (2.0f * rel.y) * s, (2.0f * rel.z) * s};
 }
 
 // float4 <-> f3 / q4 views of the SoA arrays

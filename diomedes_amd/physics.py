@@ -281,6 +281,40 @@ class HardRope:
     stiffness = 0.0
 
 
+def rest_rotation(orientation_a, orientation_b) -> tuple:
+    """The rest rotation of a bending joint that holds two bodies at their
+    current relative orientation: conj(q_a) * q_b in float32 (x, y, z, w)."""
+    ax, ay, az, aw = (np.float32(v) for v in orientation_a)
+    bx, by, bz, bw = (np.float32(v) for v in orientation_b)
+    ax, ay, az = -ax, -ay, -az
+    return (float(aw * bx + ax * bw + ay * bz - az * by), float(aw * by - ax * bz + ay * bw + az * bx),
+            float(aw * bz + ax * by - ay * bx + az * bw), float(aw * bw - ax * bx - ay * by - az * bz))
+
+
+@dataclass
+class Bend:
+    """Holds b's orientation at a's times `rest` with an angular spring of
+    the given stiffness (three angular rows; no counterpart in the reference,
+    whose ropes do not bend)."""
+    body_a: int
+    body_b: int
+    rest: tuple = (0.0, 0.0, 0.0, 1.0)
+    stiffness: float = DEFAULT_ROD_STIFFNESS
+    kind = R.BEND
+    name = "bend"
+
+
+@dataclass
+class HardBend:
+    """Holds b's orientation at a's times `rest` with three multipliers."""
+    body_a: int
+    body_b: int
+    rest: tuple = (0.0, 0.0, 0.0, 1.0)
+    kind = R.HARD_BEND
+    name = "hard bend"
+    stiffness = 0.0
+
+
 def constraints_from_avbd(constraints: Iterable) -> np.ndarray:
     """gpu_physics.rs:149-193: anchor_a.w = rest / max length (0 for sockets),
     anchor_b.w = stiffness."""
@@ -288,6 +322,10 @@ def constraints_from_avbd(constraints: Iterable) -> np.ndarray:
     out = np.zeros(len(cons), R.AVBD_CONSTRAINT)
     for i, c in enumerate(cons):
         out[i]["a"], out[i]["b"], out[i]["kind"] = c.body_a, c.body_b, c.kind
+        if isinstance(c, (Bend, HardBend)):
+            out[i]["anchor_a"][:] = c.rest
+            out[i]["anchor_b"][3] = c.stiffness
+            continue
         out[i]["anchor_a"][:3] = c.anchor_a
         out[i]["anchor_b"][:3] = c.anchor_b
         if isinstance(c, (Spring, Rod, HardRod)):

@@ -18,6 +18,9 @@ SPRING, ROD, BALL_SOCKET, ROPE = 0, 1, 2, 3
 # hard joints: the rod, socket and rope geometry held by a multiplier and a
 # ramped penalty per row (the stiffness field is unused)
 HARD_ROD, HARD_SOCKET, HARD_ROPE = 4, 5, 6
+# bending joints: b's orientation held at a's times a rest rotation (anchor_a
+# = the rest quaternion x, y, z, w), softly (stiffness in anchor_b.w) or hard
+BEND, HARD_BEND = 7, 8
 
 GPU_BODY = np.dtype(
     [

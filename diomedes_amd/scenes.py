@@ -284,7 +284,7 @@ def mixed_field(seed: int = 0, n: int = 20000, ropes: int = 128, rope_links: int
 
 # ------------------------------------------------------------- hard chains
 def hard_chains(seed: int = 0, chains: int = 256, links: int = 32, half: float = 0.15, gap: float = 0.04,
-                row_pitch: float = 1.0) -> Scene:
+                row_pitch: float = 1.0, bend: str = "", bend_stiffness: float = 5.0) -> Scene:
     """Chains of cubes on hard joints, each hung from a static anchor and
     released horizontally: they swing down in parallel planes `row_pitch`
     apart, neighbouring rows in opposite directions, and their lower links
@@ -292,9 +292,12 @@ def hard_chains(seed: int = 0, chains: int = 256, links: int = 32, half: float =
     anchors sit in rows (z) of up to 8 chains whose spans (x) do not overlap,
     at a height of 0.8 of a chain's length.  Chain k uses hard sockets
     between link ends (k % 3 == 0), hard rods (1) or hard ropes (2) between
-    link centres.  Bodies are the anchors (0 .. chains - 1), then the links
-    chain by chain.  Draw order: jitter (3 x chains x links).  Steps need the
-    coloured mode (hard joints join dynamic bodies)."""
+    link centres.  bend "soft" or "hard" adds a bending joint beside each hard
+    socket (an angular spring of bend_stiffness, or multipliers), holding each
+    link's orientation to its predecessor's.  Bodies are the anchors (0 ..
+    chains - 1), then the links chain by chain.  Draw order: jitter (3 x
+    chains x links).  Steps need the coloured mode (hard joints join dynamic
+    bodies)."""
     rng = SplitMix64(seed)
     jitter = rng.uniform(-1e-3, 1e-3, 3 * chains * links).reshape(chains * links, 3)
     step = 2.0 * half + gap
@@ -327,7 +330,10 @@ def hard_chains(seed: int = 0, chains: int = 256, links: int = 32, half: float =
     bodies[chains:] = lb
     state[chains:] = ls
     state["friction"] = np.float32(0.6)
-    cons = np.zeros(chains * links, R.AVBD_CONSTRAINT)
+    if bend not in ("", "soft", "hard"):
+        raise ValueError("bend must be '', 'soft' or 'hard'")
+    n_bend = links * ((chains + 2) // 3) if bend else 0
+    cons = np.zeros(chains * links + n_bend, R.AVBD_CONSTRAINT)
     c = 0
     for k in range(chains):
         d = dirs[k].astype(np.float32)
@@ -344,10 +350,18 @@ def hard_chains(seed: int = 0, chains: int = 256, links: int = 32, half: float =
             else:
                 cons[c]["anchor_a"][3] = np.float32(step)
             c += 1
+            if bend and kind == R.HARD_SOCKET:
+                # every body starts unrotated: the rest rotation is the identity
+                cons[c]["a"], cons[c]["b"] = a, b
+                cons[c]["kind"] = R.BEND if bend == "soft" else R.HARD_BEND
+                cons[c]["anchor_a"][3] = 1.0
+                cons[c]["anchor_b"][3] = np.float32(bend_stiffness) if bend == "soft" else 0.0
+                c += 1
+    assert c == len(cons)
     extent = max(cols * (span + 1.0), rows * row_pitch) / 2.0 + span + 4.0
     opts = AvbdOptions(dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), iterations=8, friction=0.6)
     return Scene(bodies, state, cons, floor_container(extent), options=opts,
-                 name=f"hard_chains{chains}x{links}_s{seed}")
+                 name=f"hard_chains{chains}x{links}{('_' + bend + '_bend') if bend else ''}_s{seed}")
 
 
 # ------------------------------------------------------------- wall throw

@@ -69,7 +69,14 @@ enum {
        mode 1. */
     DIO_CONSTRAINT_HARD_ROD = 4,
     DIO_CONSTRAINT_HARD_BALL_SOCKET = 5,
-    DIO_CONSTRAINT_HARD_ROPE = 6
+    DIO_CONSTRAINT_HARD_ROPE = 6,
+    /* Bending joints (no counterpart in the reference; README.md:11 lists
+       bending as missing): hold b's orientation at a's times a rest rotation,
+       whose quaternion (x, y, z, w) is anchor_a; three angular rows, with the
+       stiffness anchor_b[3] (BEND) or multipliers (HARD_BEND, which also needs
+       mode 1 between two dynamic bodies).  World path only. */
+    DIO_CONSTRAINT_BEND = 7,
+    DIO_CONSTRAINT_HARD_BEND = 8
 };
 
 /* GpuShape, 80 bytes (compute.rs:28-34) */
@@ -387,8 +394,8 @@ dio_status dio_world_contacts(dio_ctx* ctx, dio_avbd_contact* out, uint32_t cap,
    reference. */
 dio_status dio_world_colours(dio_ctx* ctx, uint32_t* colour, uint32_t cap, uint32_t* num_colours);
 /* Each constraint's multipliers after the last step: lambda and penalty per
-   row, c x 3 floats each (x, y, z rows of a hard socket; x alone of a hard rod
-   or rope; zero for soft joints, which carry none).  cap >= the constraint
+   row, c x 3 floats each (x, y, z rows of a hard socket or hard bend; x alone
+   of a hard rod or rope; zero for soft joints, which carry none).  cap >= the constraint
    count, *c set to it.  No counterpart in the reference. */
 dio_status dio_world_joints(dio_ctx* ctx, float* lambda, float* penalty, uint32_t cap, uint32_t* c);
 /* Candidate pairs of the last step's broad phase (sorted, a < b). */

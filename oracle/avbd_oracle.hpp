@@ -54,7 +54,14 @@ constexpr uint32_t SPRING = 0, ROD = 1, BALL_SOCKET = 2, ROPE = 3;
 // kinds above): the same geometry, solved to the constraint with a
 // multiplier and a ramped penalty per row, as contacts are
 constexpr uint32_t HARD_ROD = 4, HARD_SOCKET = 5, HARD_ROPE = 6;
-inline bool is_hard(uint32_t kind) { return kind >= HARD_ROD && kind <= HARD_ROPE; }
+// bending joints (no counterpart either): keep b's orientation at a's times a
+// rest rotation (anchor_a holds the rest quaternion), softly (BEND, stiffness in
+// anchor_b.w) or with multipliers (HARD_BEND)
+constexpr uint32_t BEND = 7, HARD_BEND = 8;
+inline bool is_hard(uint32_t kind) { return (kind >= HARD_ROD && kind <= HARD_ROPE) || kind == HARD_BEND; }
+inline bool is_bend(uint32_t kind) { return kind == BEND || kind == HARD_BEND; }
+// three rows per joint (socket and bend kinds), else one
+inline bool three_rows(uint32_t kind) { return kind == HARD_SOCKET || kind == HARD_BEND; }
 
 // ---------------------------------------------------------------- glam math
 template <class T> struct V3 { T x, y, z; };
@@ -131,6 +138,14 @@ template <class T> inline Q<T> quat_step(Q<T> q, V3<T> d) {
 template <class T> inline V3<T> ang_diff(Q<T> to, Q<T> from) {
     Q<T> rel = qmul(to, qinv(from));
     return {T(2) * rel.x, T(2) * rel.y, T(2) * rel.z};
+}
+// A bending joint's error: the rotation from b's orientation to a's times the
+// rest rotation, 2 vec(qa rest qb^-1) on the shorter arc.  Turning a by a
+// small world rotation d changes it by +d, turning b by d changes it by -d.
+template <class T> inline V3<T> bend_error(Q<T> qa, Q<T> rest, Q<T> qb) {
+    Q<T> rel = qmul(qmul(qa, rest), qinv(qb));
+    T s = rel.w < T(0) ? T(-1) : T(1);
+    return {(T(2) * rel.x) * s, (T(2) * rel.y) * s, (T(2) * rel.z) * s};
 }
 
 // ---------------------------------------------------------------- records
@@ -1051,7 +1066,7 @@ template <class T> struct HardRows {
 };
 template <class T> HardRows<T> hard_rows(uint32_t kind, V3<T> delta, V3<T> delta0, T rest, T alpha) {
     HardRows<T> r;
-    if (kind == HARD_SOCKET) {
+    if (three_rows(kind)) {
         r.chat = V3<T>{delta.x - alpha * delta0.x, delta.y - alpha * delta0.y, delta.z - alpha * delta0.z};
         r.ok = true;
     } else {
@@ -1076,7 +1091,7 @@ template <class T> T hard_force(uint32_t kind, T pen, T chat, T lam) {
 // the penalty of an active row ramps by beta |C^| up to penalty_max.
 template <class T> void hard_update(uint32_t kind, const HardRows<T>& r, JointMult<T>& m, T beta, T pmax) {
     if (!r.ok) return;
-    const int rows = kind == HARD_SOCKET ? 3 : 1;
+    const int rows = three_rows(kind) ? 3 : 1;
     T* lam = &m.lam.x;
     T* pen = &m.pen.x;
     const T* ch = &r.chat.x;
@@ -1299,8 +1314,12 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
     std::vector<V3<T>> delta0(cons.size(), V3<T>{T(0), T(0), T(0)});
     for (uint32_t ci : hard) {
         const Constraint<T>& c = cons[ci];
-        delta0[ci] = (S.initial_lin[c.a] + qrot(S.initial_ang[c.a], c.anchor_a)) -
-                     (S.initial_lin[c.b] + qrot(S.initial_ang[c.b], c.anchor_b));
+        if (is_bend(c.kind))
+            delta0[ci] = bend_error(S.initial_ang[c.a], Q<T>{c.anchor_a.x, c.anchor_a.y, c.anchor_a.z, c.length},
+                                    S.initial_ang[c.b]);
+        else
+            delta0[ci] = (S.initial_lin[c.a] + qrot(S.initial_ang[c.a], c.anchor_a)) -
+                         (S.initial_lin[c.b] + qrot(S.initial_ang[c.b], c.anchor_b));
     }
     double t1 = now_s();
     if (times) times->init += t1 - t0;
@@ -1394,7 +1413,35 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
                 V3<T> delta = p_a - p_b;
                 V3<T> r_self = qrot(snap_rot, self_anchor);
                 T stiff = c.stiffness;
-                if (is_hard(c.kind)) {
+                if (is_bend(c.kind)) {
+                    // three angular rows: j_lin = 0, j_ang = +-e_r
+                    const V3<T> err = bend_error(qa, Q<T>{c.anchor_a.x, c.anchor_a.y, c.anchor_a.z, c.length}, qb);
+                    const V3<T> zl{T(0), T(0), T(0)};
+                    const V3<T> js[3] = {V3<T>{T(1), T(0), T(0)} * sign, V3<T>{T(0), T(1), T(0)} * sign,
+                                         V3<T>{T(0), T(0), T(1)} * sign};
+                    const T* e = &err.x;
+                    T fs[3], ks[3];
+                    if (c.kind == HARD_BEND) {
+                        const HardRows<T> hr = hard_rows(c.kind, err, delta0[ci], c.length, pc.alpha);
+                        const JointMult<T>& mu = jm[ci];
+                        for (int r = 0; r < 3; ++r) {
+                            fs[r] = hard_force(c.kind, (&mu.pen.x)[r], (&hr.chat.x)[r], (&mu.lam.x)[r]);
+                            ks[r] = (&mu.pen.x)[r];
+                        }
+                    } else {
+                        for (int r = 0; r < 3; ++r) {
+                            fs[r] = e[r] * stiff;
+                            ks[r] = stiff;
+                        }
+                    }
+                    for (int r = 0; r < 3; ++r) {
+                        B.add_lin(zl, ks[r]);
+                        B.add_ang(js[r], ks[r]);
+                        B.add_cross(js[r], zl, ks[r]);
+                        B.add_rhs_lin(zl, fs[r]);
+                        B.add_rhs_ang(js[r], fs[r]);
+                    }
+                } else if (is_hard(c.kind)) {
                     const HardRows<T> hr = hard_rows(c.kind, delta, delta0[ci], c.length, pc.alpha);
                     const JointMult<T>& mu = jm[ci];
                     if (c.kind == HARD_SOCKET) {
@@ -1517,7 +1564,9 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
         // the hard joints' dual update, from the same poses
         for (uint32_t ci : hard) {
             const Constraint<T>& c = cons[ci];
-            V3<T> delta = (wpv[c.a] + qrot(wrv[c.a], c.anchor_a)) - (wpv[c.b] + qrot(wrv[c.b], c.anchor_b));
+            V3<T> delta = is_bend(c.kind)
+                              ? bend_error(wrv[c.a], Q<T>{c.anchor_a.x, c.anchor_a.y, c.anchor_a.z, c.length}, wrv[c.b])
+                              : (wpv[c.a] + qrot(wrv[c.a], c.anchor_a)) - (wpv[c.b] + qrot(wrv[c.b], c.anchor_b));
             hard_update(c.kind, hard_rows(c.kind, delta, delta0[ci], c.length, pc.alpha), jm[ci], pc.beta_lin,
                         pc.penalty_max);
         }

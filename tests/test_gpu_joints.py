@@ -180,6 +180,39 @@ def test_hard_chains_full_size_bitexact(oracle):
         assert st.contacts > 1000
 
 
+@pytest.mark.parametrize("bend,lanes", [("soft", 32), ("hard", 32), ("hard", 1)])
+def test_bending_chains_bitexact(oracle, bend, lanes):
+    """Bending joints beside the socket chains' hard sockets (kinds 7 and 8),
+    coloured mode: poses, velocities and multipliers bit-identical."""
+    sc = scenes.hard_chains(3, chains=12, links=10, bend=bend)
+    w = oracle.OracleWorld(sc, 32)
+    w.set_mode(1)
+    with PhysicsContext(sc.n, max_constraints=len(sc.constraints)) as ctx:
+        ctx.set_tuning(primal_lanes=lanes)
+        ctx.upload(sc)
+        for k in range(3):
+            ctx.step(sc.options, 20, mode=1)
+            w.step(nsteps=20)
+            _compare(ctx, w, f"{bend} bend lanes {lanes} step {20 * (k + 1)}")
+        _print(bend, "bend chains lanes", lanes, "contacts", ctx.stats().contacts, "colours", ctx.stats().colours)
+
+
+@pytest.mark.parametrize("lanes,fuse", [(32, 1), (1, 1), (32, 0)])
+def test_cantilevers_jacobi_bitexact(oracle, lanes, fuse):
+    """Single links held from static anchors by a hard socket and a hard bend
+    (no joint between dynamic bodies: the Jacobi mode's fused and unfused
+    schedules)."""
+    sc = scenes.hard_chains(4, chains=12, links=1, bend="hard")
+    w = oracle.OracleWorld(sc, 32)
+    with PhysicsContext(sc.n, max_constraints=len(sc.constraints)) as ctx:
+        ctx.set_tuning(primal_lanes=lanes, fuse_dual=fuse)
+        ctx.upload(sc)
+        for k in range(3):
+            ctx.step(sc.options, 20, mode=0)
+            w.step(nsteps=20)
+            _compare(ctx, w, f"cantilevers lanes {lanes} fuse {fuse} step {20 * (k + 1)}")
+
+
 def test_hard_joint_errors():
     sc = scenes.hard_chains(2, chains=3, links=4)
     with PhysicsContext(sc.n, max_constraints=len(sc.constraints)) as ctx:

@@ -13,7 +13,12 @@ formulas:
 - pendulums on static anchors share no joint between dynamic bodies, so the
   coloured Gauss-Seidel solve is the Jacobi solve, bit for bit;
 - a chain on hard rods keeps its link lengths far better than on the
-  reference's soft rods, in FP64 and FP32 alike.
+  reference's soft rods, in FP64 and FP32 alike;
+- bending joints (kinds 7, 8): a body on a soft bend is a linear torsional
+  oscillator whose angle follows backward Euler's recurrence
+  th_{n+1} = (2 th_n - th_{n-1}) / (1 + w^2 dt^2), w^2 = k / I; a cantilever of
+  N links on hard sockets and hard bends carries, at joint j, the shear
+  m g (N - j) and the moment m g s (N - j)^2 / 2; a bend moves no body.
 """
 
 from __future__ import annotations
@@ -21,7 +26,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from diomedes_amd.physics import AvbdBody, AvbdOptions, HardBallSocket, HardRod, HardRope, Rod, Scene
+from diomedes_amd import scenes
+from diomedes_amd.physics import AvbdBody, AvbdOptions, Bend, HardBallSocket, HardRod, HardRope, Rod, Scene
 
 G = 10.0
 DT = float(np.float32(1.0 / 60.0))  # the options record holds dt in float32
@@ -169,3 +175,48 @@ def test_hard_chain_holds_its_length(oracle):
     # FP32 follows FP64 (a smooth swing: no contacts)
     assert np.max(np.abs(final[True, 32] - final[True, 64])) < 1e-2
     assert worst[True, 32] < 0.03
+
+
+def test_soft_bend_is_a_backward_euler_oscillator(oracle):
+    """A cube on a soft bend to a static body, no gravity, started at a
+    small angle about z: one body, a quadratic energy in the angle, so the
+    step is backward Euler on I th'' = -k th."""
+    th0, k = 0.04, 1.0
+    bodies = [AvbdBody.sphere((5.0, 0.0, 0.0), 0.1, 0.0), AvbdBody.cube((0.0, 0.0, 0.0), 0.1, 1.0)]
+    bodies[1].orientation = (0.0, 0.0, float(np.sin(th0 / 2)), float(np.cos(th0 / 2)))
+    sc = Scene.from_bodies(bodies, [Bend(0, 1, stiffness=k)], None, AvbdOptions(iterations=4, gravity=(0.0, 0.0, 0.0)))
+    w2dt2 = k / (1.0 / float(sc.state["inv_moment"][1][0])) * DT * DT
+    w = oracle.OracleWorld(sc, 64)
+    th = [th0]
+    for _ in range(60):
+        w.step(nsteps=1)
+        q = w.state()["orientations"][1]
+        th.append(2.0 * np.arctan2(q[2], q[3]))
+        assert np.all(w.state()["positions"][1] == 0.0)  # the bend's rows are angular only
+    ref = [th0, th0 / (1.0 + w2dt2)]
+    for _ in range(59):
+        ref.append((2.0 * ref[-1] - ref[-2]) / (1.0 + w2dt2))
+    # the error is the bend error's own nonlinearity, 2 sin(th / 2) against th
+    assert np.max(np.abs(np.array(th) - np.array(ref))) < 2e-4 * th0
+    assert min(th) < -0.5 * th0  # it swings through
+
+
+def test_cantilever_shear_and_moment(oracle):
+    """Six unit-mass links on hard sockets and hard bends, held horizontally
+    from a static anchor: at equilibrium joint j carries the weight of the
+    links beyond it and their moment about it."""
+    n_links, s = 6, 0.34
+    sc = scenes.hard_chains(0, chains=1, links=n_links, bend="hard")
+    w = oracle.OracleWorld(sc, 64)
+    w.set_mode(1)
+    w.step(nsteps=600)
+    lam, _ = w.joints()
+    kinds = sc.constraints["kind"]
+    g = 9.81
+    shear = np.abs(lam[kinds == 5][:, 1])
+    moment = np.abs(lam[kinds == 8][:, 2])
+    for j in range(n_links):
+        assert shear[j] == pytest.approx(g * (n_links - j), rel=2e-3), j
+        assert moment[j] == pytest.approx(g * s * (n_links - j) ** 2 / 2.0, rel=2e-3), j
+    assert np.max(np.abs(w.state()["velocities"])) < 1e-3
+
