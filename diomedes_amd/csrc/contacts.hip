@@ -48,14 +48,44 @@ __device__ __forceinline__ void warm_scale(const BuildOptions& o, float4& lam, f
                       rmin(rmax(pen.z * o.gamma, o.penalty_min), o.penalty_max), 0.0f);
 }
 
-// index of the last old contact with key (a, b) among the narrow prefix, or -1
-__device__ __forceinline__ int64_t find_old_pair(const NarrowSource& old_src, uint32_t a, uint32_t b) {
+__device__ __forceinline__ bool pair_less(uint2 q, uint32_t a, uint32_t b) { return q.x < a || (q.x == a && q.y < b); }
+
+// index of the last old contact with key (a, b) among the narrow prefix, or -1.
+// The old and new pair lists are both sorted and change little from step to
+// step, so the lower bound of (a, b) among the old pairs is searched by
+// galloping out from `hint` (the new pair's own index) and then bisecting the
+// bracket found: a few dependent loads instead of log2(np).
+__device__ __forceinline__ int64_t find_old_pair(const NarrowSource& old_src, uint32_t a, uint32_t b, uint32_t hint) {
     const uint32_t np = *old_src.p_dev;
+    // bracket [lo, hi) with pairs[lo - 1] < (a, b) <= pairs[hi] (ends open)
     uint32_t lo = 0, hi = np;
+    if (np > 0) {
+        uint32_t h = hint < np ? hint : np - 1;
+        if (pair_less(old_src.pairs[h], a, b)) {
+            lo = h + 1;
+            for (uint32_t step = 1; lo < np; step <<= 1) {
+                const uint32_t probe = lo + step - 1 < np ? lo + step - 1 : np - 1;
+                if (!pair_less(old_src.pairs[probe], a, b)) {
+                    hi = probe;
+                    break;
+                }
+                lo = probe + 1;
+            }
+        } else {
+            hi = h;
+            for (uint32_t step = 1; hi > 0; step <<= 1) {
+                const uint32_t probe = hi >= step ? hi - step : 0;
+                if (pair_less(old_src.pairs[probe], a, b)) {
+                    lo = probe + 1;
+                    break;
+                }
+                hi = probe;
+            }
+        }
+    }
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        const uint2 q = old_src.pairs[mid];
-        if (q.x < a || (q.x == a && q.y < b)) lo = mid + 1;
+        if (pair_less(old_src.pairs[mid], a, b)) lo = mid + 1;
         else hi = mid;
     }
     if (lo < np) {
@@ -96,7 +126,7 @@ __global__ void __launch_bounds__(kBlock) k_contacts_narrow(WorldView w, BuildOp
             float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float4 pen = make_float4(o.penalty_min, o.penalty_min, o.penalty_min, 0.0f);
             if (k == 0) {
-                const int64_t q = find_old_pair(old_src, pr.x, pr.y);
+                const int64_t q = find_old_pair(old_src, pr.x, pr.y, p);
                 if (q >= 0 && (uint32_t)q < old_m) {
                     lam = old.c.lam[q];
                     pen = old.c.pen[q];
@@ -316,16 +346,26 @@ __global__ void __launch_bounds__(kBlock) k_csr_rank(uint32_t n, const uint32_t*
     const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     if (g == 0 && off[n] > idx_cap) atomicMax(overflow, off[n]);
     const uint32_t i = g / kCsrLanes, lane = g % kCsrLanes;
-    if (i >= n) return;
+    if (i >= n) return;  // whole groups of kCsrLanes lanes leave together
     const uint32_t b = off[i];
     uint32_t e = off[i + 1];
     if (e > idx_cap) e = idx_cap;
-    for (uint32_t x = b + lane; x < e; x += kCsrLanes) {
-        const uint32_t c = tmp[x];
+    // The group walks the body's entries kCsrLanes at a time: each lane loads
+    // one and the others read it by shuffles, so a chunk costs one load.
+    for (uint32_t x0 = b; x0 < e; x0 += kCsrLanes) {
+        const uint32_t x = x0 + lane;
+        const uint32_t c = x < e ? tmp[x] : 0xFFFFFFFFu;
         uint32_t r = 0;
-        for (uint32_t y = b; y < e; ++y) r += tmp[y] < c ? 1u : 0u;
-        idx[b + r] = c;
-        peer[b + r] = peer_code(ab[c], i);
+        for (uint32_t y0 = b; y0 < e; y0 += kCsrLanes) {
+            const uint32_t y = y0 + lane;
+            const uint32_t mine = y < e ? tmp[y] : 0xFFFFFFFFu;  // past the end: ranks nothing
+#pragma unroll
+            for (int t = 0; t < kCsrLanes; ++t) r += __shfl(mine, t, kCsrLanes) < c ? 1u : 0u;
+        }
+        if (x < e) {
+            idx[b + r] = c;
+            peer[b + r] = peer_code(ab[c], i);
+        }
     }
 }
 

@@ -37,14 +37,13 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t x, uint32_t* lds_wa
     uint32_t inc = wave_inclusive(x);
     if (lane == kWave - 1) lds_waves[wave] = inc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            uint32_t t = lds_waves[w];
-            lds_waves[w] = run;
-            run += t;
-        }
-        lds_waves[kWaves] = run;
+    // wave 0 scans the wave totals (each lane reads its slot before any lane
+    // of the wave writes)
+    if (wave == 0) {
+        const uint32_t t = lane < kWaves ? lds_waves[lane] : 0u;
+        const uint32_t ti = wave_inclusive(t);
+        if (lane < kWaves) lds_waves[lane] = ti - t;
+        if (lane == kWaves - 1) lds_waves[kWaves] = ti;
     }
     __syncthreads();
     uint32_t r = lds_waves[wave] + inc - x;
