@@ -228,7 +228,7 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{32, 8, 4, 1, 8, 0, {0, 0}};
+    dio_tuning tuning{32, 8, 16, 1, 8, 0, {0, 0}};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -1500,7 +1500,7 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         dio_tuning v = *t;
         if (v.primal_lanes == 0) v.primal_lanes = 32;
         if (v.broad_lanes == 0) v.broad_lanes = 8;
-        if (v.resort_interval == 0) v.resort_interval = 4;
+        if (v.resort_interval == 0) v.resort_interval = 16;
         if (v.adjoint_lanes == 0) v.adjoint_lanes = 8;
         if (v.primal_lanes != 1 && v.primal_lanes != 4 && v.primal_lanes != 8 && v.primal_lanes != 16 &&
             v.primal_lanes != 32 && v.primal_lanes != 64)

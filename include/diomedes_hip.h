@@ -236,7 +236,7 @@ typedef struct dio_tuning {
     uint32_t primal_lanes;     /* lanes per body in the primal: 1, 4, 8, 16, 32 or 64 (0 = default 32) */
     uint32_t broad_lanes;      /* lanes per query in the BVH traversal: 1, 2, 4 or 8 (0 = default 8) */
     uint32_t resort_interval;  /* fused step: Morton re-sort every this many steps; the tree is refit
-                                  over the previous order in between (0 = default 4) */
+                                  over the previous order in between (0 = default 16) */
     uint32_t fuse_dual;        /* 1: each primal iteration k >= 1 applies dual update k - 1 itself
                                   (fewer launches, both bodies of a contact evaluate it; default);
                                   0: a dual launch after each primal */
