@@ -52,6 +52,40 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t x, uint32_t* lds_wa
     return r;
 }
 
+// Eight consecutive words from base: two 16-byte loads when all are in range
+// and the array is 16-byte aligned (base is a multiple of 8), else guarded
+// scalar loads (zero past n).
+__device__ __forceinline__ void load8(const uint32_t* in, uint32_t base, uint32_t n, bool aligned, uint32_t v[8]) {
+    if (aligned && base + 8 <= n) {
+        const uint4 a = *reinterpret_cast<const uint4*>(in + base);
+        const uint4 b = *reinterpret_cast<const uint4*>(in + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = base + k < n ? in[base + k] : 0u;
+    }
+}
+// out[base + k] = ex + v[0] + ... + v[k-1] for the k < 8 with base + k < n
+__device__ __forceinline__ void store8_exclusive(uint32_t* out, uint32_t base, uint32_t n, bool aligned, uint32_t ex,
+                                                 const uint32_t v[8]) {
+    uint32_t e[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        e[k] = ex;
+        ex += v[k];
+    }
+    if (aligned && base + 8 <= n) {
+        *reinterpret_cast<uint4*>(out + base) = make_uint4(e[0], e[1], e[2], e[3]);
+        *reinterpret_cast<uint4*>(out + base + 4) = make_uint4(e[4], e[5], e[6], e[7]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (base + k < n) out[base + k] = e[k];
+    }
+}
+__device__ __forceinline__ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0u; }
+
 __device__ __forceinline__ uint32_t read_n(uint32_t n_cap, const uint32_t* n_dev) {
     if (!n_dev) return n_cap;
     uint32_t n = *n_dev;
@@ -64,10 +98,11 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_partials(const uint32_t* in
     __shared__ uint32_t lds[kScanBlock / kWave + 1];
     const uint32_t n = read_n(n_cap, n_dev);
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    load8(in, base, n, aligned16(in), v);
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < n) s += in[base + k];
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     uint32_t total;
     block_exclusive<kScanBlock>(s, lds, &total);
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
@@ -95,20 +130,15 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* in, u
     __shared__ uint32_t lds[kScanBlock / kWave + 1];
     const uint32_t n = read_n(n_cap, n_dev);
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const bool al = aligned16(in) && aligned16(out);
     uint32_t v[kScanItems];
+    load8(in, base, n, al, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = base + k < n ? in[base + k] : 0;
-        s += v[k];
-    }
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     uint32_t total;
-    uint32_t ex = block_exclusive<kScanBlock>(s, lds, &total) + partials[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < n) out[base + k] = ex;
-        ex += v[k];
-    }
+    const uint32_t ex = block_exclusive<kScanBlock>(s, lds, &total) + partials[blockIdx.x];
+    store8_exclusive(out, base, n, al, ex, v);
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partials[tiles];
 }
 
@@ -116,6 +146,7 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* in, u
 // running carry: one launch instead of three for the per-body arrays.
 constexpr int kSingleThreads = 1024;
 constexpr int kSingleItems = 8;
+static_assert(kScanItems == 8 && kSingleItems == 8, "load8 / store8_exclusive move eight words per thread");
 constexpr uint32_t kSingleTile = kSingleThreads * kSingleItems;
 constexpr uint32_t kSingleMax = 4 * kSingleTile;
 
@@ -123,23 +154,18 @@ __global__ void __launch_bounds__(kSingleThreads) k_scan_single(const uint32_t* 
                                                                 const uint32_t* n_dev) {
     __shared__ uint32_t lds[kSingleThreads / kWave + 1];
     const uint32_t n = read_n(n_cap, n_dev);
+    const bool al = aligned16(in) && aligned16(out);
     uint32_t carry = 0;
     for (uint32_t tile = 0; tile < n; tile += kSingleTile) {
         const uint32_t base = tile + threadIdx.x * kSingleItems;
         uint32_t v[kSingleItems];
+        load8(in, base, n, al, v);
         uint32_t sum = 0;
 #pragma unroll
-        for (int k = 0; k < kSingleItems; ++k) {
-            v[k] = base + k < n ? in[base + k] : 0;
-            sum += v[k];
-        }
+        for (int k = 0; k < kSingleItems; ++k) sum += v[k];
         uint32_t total;
-        uint32_t ex = block_exclusive<kSingleThreads>(sum, lds, &total) + carry;
-#pragma unroll
-        for (int k = 0; k < kSingleItems; ++k) {
-            if (base + k < n) out[base + k] = ex;
-            ex += v[k];
-        }
+        const uint32_t ex = block_exclusive<kSingleThreads>(sum, lds, &total) + carry;
+        store8_exclusive(out, base, n, al, ex, v);
         carry += total;
     }
     if (threadIdx.x == 0) out[n] = carry;
