@@ -245,6 +245,9 @@ struct dio_ctx {
     uint32_t w_hull_count = 0;
     uint32_t last_lag = 1;
     uint32_t w_n_dyn = 0;
+    // false when every body is a box (or cube), or every body a sphere: no
+    // pair can reach GJK / EPA
+    bool w_gjk_possible = true;
     dio_gpu_body* w_stage_b = nullptr;
     dio_avbd_body_state* w_stage_s = nullptr;
     dio_avbd_constraint* w_stage_c = nullptr;
@@ -1025,7 +1028,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
             b.clear_words[0] = c->pend[dst_pend].out.gjk_count;
             b.clear_words[1] = c->pend[dst_pend].out.epa_retry_count;
             run_broad(bv, n, b, c->p_max, bs, resort, (int)c->tuning.broad_lanes, scenes);
-            run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, bs, true);
+            run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, bs, true, c->w_gjk_possible);
         };
         if (!lag) broad_narrow(st);
         run_contact_pairs(wv, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]), c->cs[1 - s],
@@ -1259,10 +1262,15 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
     ctx->w_sort_pending = true;
     ctx->w_friction.resize(n);
     ctx->w_n_dyn = 0;
+    bool all_box = true, all_sphere = true;
     for (uint32_t i = 0; i < n; ++i) {
         ctx->w_friction[i] = state[i].friction;
         if (state[i].inv_mass != 0.0f) ctx->w_n_dyn += 1;
+        const uint32_t tag = bodies[i].shape.tag;
+        all_box = all_box && (tag == kCube || tag == kBox);
+        all_sphere = all_sphere && tag == kSphere;
     }
+    ctx->w_gjk_possible = !(all_box || all_sphere);
     if (n > 0) {
         dio_gpu_body* stage_b = ctx->w_stage_b;
         dio_avbd_body_state* stage_s = ctx->w_stage_s;

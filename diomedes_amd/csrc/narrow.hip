@@ -539,17 +539,20 @@ __global__ void __launch_bounds__(kBlock) k_pack_contact_out(NarrowOut o, uint32
 
 }  // namespace
 
-void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, hipStream_t s, bool counters_cleared) {
+void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, hipStream_t s, bool counters_cleared,
+                bool gjk_possible) {
     if (!counters_cleared) launch_fill2_u32(o.gjk_count, o.epa_retry_count, 0u, 1, s);
     uint32_t blocks = blocks_for(o.p_max > 0 ? o.p_max : 1);
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_narrow, dim3(blocks), dim3(kBlock), 0, s, bv, o, simplex_store);
-    uint32_t eblocks = o.p_max > 0 ? o.p_max : 1;
-    if (eblocks > 8192) eblocks = 8192;
-    hipLaunchKernelGGL((k_epa<kSmallV, kSmallF, true>), dim3(eblocks), dim3(kWave), 0, s, bv, o,
-                       (const float4*)simplex_store);
-    hipLaunchKernelGGL((k_epa<kMaxV, kMaxF, false>), dim3(eblocks < 512 ? eblocks : 512), dim3(kWave), 0, s, bv, o,
-                       (const float4*)simplex_store);
+    if (gjk_possible) {
+        uint32_t eblocks = o.p_max > 0 ? o.p_max : 1;
+        if (eblocks > 8192) eblocks = 8192;
+        hipLaunchKernelGGL((k_epa<kSmallV, kSmallF, true>), dim3(eblocks), dim3(kWave), 0, s, bv, o,
+                           (const float4*)simplex_store);
+        hipLaunchKernelGGL((k_epa<kMaxV, kMaxF, false>), dim3(eblocks < 512 ? eblocks : 512), dim3(kWave), 0, s, bv,
+                           o, (const float4*)simplex_store);
+    }
     launch_scan_exclusive(o.cnt, o.off, o.p_max, o.p_dev, o.scan_partials, s);
 }
 

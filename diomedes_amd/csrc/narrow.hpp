@@ -32,8 +32,11 @@ struct NarrowOut {
 // simplex_store holds 8 float4 per pair (the GJK simplex handed to EPA).
 // counters_cleared: gjk_count and epa_retry_count were zeroed by an earlier
 // launch on the stream (the broad phase's emit); otherwise they are zeroed here.
+// gjk_possible false: every pair is box-box or sphere-sphere (a world of boxes
+// only or of spheres only), so no pair reaches GJK and the EPA launches are left
+// out.
 void run_narrow(const BodyView& bv, const NarrowOut& out, float4* simplex_store, hipStream_t s,
-                bool counters_cleared = false);
+                bool counters_cleared = false, bool gjk_possible = true);
 
 // Pack slots into the reference's GpuContactOut records (compute.rs:123-133).
 void launch_pack_contact_out(const NarrowOut& out, uint32_t pair_count, dio_gpu_contact_out* rec, hipStream_t s);
