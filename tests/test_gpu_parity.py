@@ -179,6 +179,30 @@ def _compare_world(ctx, w, step, check_contacts=True):
         assert_bits(gc["r_a"][:, :3], rc["r_a"], f"step {step} r_a")
 
 
+def test_sphere_heap_bitexact(oracle):
+    """A world of spheres only (no pair reaches GJK, so the step leaves out
+    the EPA launches): 400 spheres of two radii dropped into a container."""
+    from diomedes_amd.physics import AvbdBody, AvbdOptions, Scene
+
+    rng = np.random.default_rng(11)
+    bodies = []
+    for i in range(400):
+        x, z = (i % 20) * 1.1 - 10.0, (i // 20) * 1.1 - 10.0
+        r = 0.3 if i % 3 else 0.5
+        bodies.append(AvbdBody.sphere((x + rng.uniform(-0.05, 0.05), 1.0 + (i % 7) * 1.2, z), r, 1.0))
+        bodies[-1].velocity = tuple(rng.normal(0.0, 1.0, 3))
+    scene = Scene.from_bodies(bodies, (), scenes.floor_container(14.0), AvbdOptions(iterations=6))
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        for k in range(4):
+            ctx.step(scene.options, 15)
+            w.step(nsteps=15)
+            _compare_world(ctx, w, 15 * (k + 1))
+            _print("sphere heap step", 15 * (k + 1), "pairs", ctx.stats().pairs, "contacts", ctx.stats().contacts)
+        assert ctx.stats().contacts > 200
+
+
 def test_box_stack_300_steps_bitexact(oracle):
     scene = scenes.box_stack(0)
     w = oracle.OracleWorld(scene, 32)
