@@ -2,7 +2,8 @@
 
 ``python -m diomedes_amd.build`` compiles ``diomedes_amd/csrc/*.hip`` into
 ``diomedes_amd/libdiomedes_hip.so`` with hipcc for gfx950 and, as test
-infrastructure, ``oracle/liboracle.so`` with g++.
+infrastructure, ``oracle/liboracle.so`` with g++ and the plain C caller
+``tests/capi/replay_trajectory`` with gcc.
 
 Floating point: every file is built with ``-ffp-contract=off`` and without
 fast-math, so each float operation rounds once in source order (the CPU
@@ -24,6 +25,8 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libdiomedes_hip.so"
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+CAPI_DIR = ROOT / "tests" / "capi"
+CAPI_EXE = CAPI_DIR / "replay_trajectory"
 BUILD_DIR = ROOT / "_build"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -85,11 +88,25 @@ def build_oracle(verbose: bool = True) -> Path:
     return ORACLE_LIB
 
 
+def build_capi_caller(verbose: bool = True) -> Path:
+    """Compile the plain C99 caller of the C ABI (tests/capi), strict and
+    warning-free, linked against the library built beside it."""
+    tmp = CAPI_EXE.with_name(CAPI_EXE.name + ".tmp")
+    _run(["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", "-pedantic", f"-I{ROOT / 'include'}",
+          str(CAPI_DIR / "replay_trajectory.c"), "-o", str(tmp), f"-L{PKG}", "-ldiomedes_hip",
+          "-Wl,-rpath,$ORIGIN/../../diomedes_amd"])
+    os.replace(tmp, CAPI_EXE)
+    if verbose:
+        print(f"built {CAPI_EXE.relative_to(ROOT)}", file=sys.stderr)
+    return CAPI_EXE
+
+
 def build_all(verbose: bool = True) -> None:
     if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     build_hip(verbose)
     build_oracle(verbose)
+    build_capi_caller(verbose)
 
 
 if __name__ == "__main__":
