@@ -318,11 +318,12 @@ def adjoint_leg(args) -> dict:
     step: broad, narrow, contacts, AVBD, plus the recording); backward_ms the
     reverse pass (each segment replayed once from its checkpoint with its
     iterations taped, then reversed); replay_ms a solve-only replay of the
-    recording.  The gradient of seed 0 is compared with central differences
-    (h = 1e-3 relative) of the same float32 replay on the GPU, over the full
-    horizon and over 20 steps."""
-    import dataclasses
-
+    recording.  The reverse-mode gradient of seed 0 over 20 steps is reported
+    as a figure; its check is the test suite's (reverse mode against the
+    oracle's float-valued dual numbers, the exact derivative of the float32
+    replay's piece): float32 central differences at any usable step cross
+    pieces of the piecewise-smooth float32 map and measure rounding, not the
+    derivative."""
     import numpy as np
 
     from diomedes_amd import PhysicsContext, scenes
@@ -333,7 +334,6 @@ def adjoint_leg(args) -> dict:
     opts = throws[0].options
     targets = [scenes.wall_throw_target(sc, args.seed + s) for s, sc in enumerate(throws)]
     offs = batch.offsets
-    proj0 = throws[0].n - 1
 
     def grad5(g, state, s):
         i = int(offs[s]) + throws[s].n - 1
@@ -342,36 +342,6 @@ def adjoint_leg(args) -> dict:
         m = 1.0 / inv_m
         gm = -(float(g["inv_mass"][i]) + float(np.dot(g["inv_moment"][i].astype(np.float64), c))) / (m * m)
         return [*map(float, g["velocities"][i]), gm, float(g["friction"])]
-
-    def fd5(ctx, state, n_steps_loss, h_rel=1e-3):
-        """Central differences of the GPU replay (seed 0's v0 and mass, the friction)."""
-        out = []
-        i = proj0
-        inv_m = float(state["inv_mass"][i])
-        c = state["inv_moment"][i, :3].astype(np.float64) / inv_m
-        p0 = [*map(float, state["vel"][i, :3]), 1.0 / inv_m, float(np.float32(opts.friction))]
-        for k in range(5):
-            h = h_rel * max(abs(p0[k]), 1.0)
-            vals = []
-            for sgn in (1.0, -1.0):
-                st = state.copy()
-                o = opts
-                if k < 3:
-                    st["vel"][i, k] = np.float32(p0[k] + sgn * h)
-                elif k == 3:
-                    m = np.float32(p0[3] + sgn * h)
-                    st["inv_mass"][i] = np.float32(1.0) / m
-                    st["inv_moment"][i, :3] = c.astype(np.float32) / m
-                else:
-                    o = dataclasses.replace(opts, friction=float(np.float32(p0[4] + sgn * h)))
-                pos, _ = ctx.adjoint_replay(o, st)
-                vals.append(n_steps_loss(pos))
-            out.append((vals[0] - vals[1]) / (2.0 * h))
-        return out
-
-    def rel(a, b):
-        a, b = np.asarray(a), np.asarray(b)
-        return [float(x) for x in np.abs(a - b) / np.maximum(np.abs(b), 1e-30)]
 
     n = batch.n
     with PhysicsContext(n, max_pairs=8 * n, max_contacts=16 * n) as ctx:
@@ -396,11 +366,7 @@ def adjoint_leg(args) -> dict:
         rep = time.perf_counter() - t0
         _progress(f"adjoint leg: backward {bwd:.2f} s, replay {rep:.2f} s")
         replay_exact = bool(np.array_equal(rpos.view(np.uint32), pos.view(np.uint32)))
-        g0 = grad5(g, batch.state, 0)
-        loss0 = lambda p: _wall_loss_grad(p[:throws[0].n], targets[:1], offs[:1])[0]
-        fd_full = fd5(ctx, batch.state, lambda p: _wall_loss_grad(p, targets, offs)[0])
-        _progress("adjoint leg: differences over the full horizon done")
-    # the same check over 20 steps (one seed), where the replay is smooth
+    # the gradient of one seed over 20 steps, where the tangent map is still well conditioned
     short = 20
     sc0 = throws[0]
     with PhysicsContext(sc0.n) as c1:
@@ -411,7 +377,6 @@ def adjoint_leg(args) -> dict:
         _, d1 = _wall_loss_grad(p1, targets[:1], offs[:1])
         g1 = c1.adjoint_backward(opts, d_positions=d1.astype(np.float32))
         g_short = grad5(g1, sc0.state, 0)
-        fd_short = fd5(c1, sc0.state, loss0)
     bodies_dyn = batch.n_dynamic
     return {
         "workload": f"{seeds} wall throws (64 x 32 bricks 0.5 x 0.25 x 0.25 + one r 0.5, m 20 sphere each) "
@@ -426,18 +391,12 @@ def adjoint_leg(args) -> dict:
         "body_iters_per_s_backward": bodies_dyn * opts.iterations * steps / bwd,
         "replay_bit_exact": replay_exact,
         "loss": loss,
-        "grad_seed0": dict(zip(("v0x", "v0y", "v0z", "mass", "friction"), g0)),
-        "fd_seed0": dict(zip(("v0x", "v0y", "v0z", "mass", "friction"), fd_full)),
-        "rel_err_vs_fd": rel(g0, fd_full),
-        "short_horizon": {
-            "steps": short,
-            "grad": g_short,
-            "fd": fd_short,
-            "rel_err_vs_fd": rel(g_short, fd_short),
-        },
-        "note": "friction enters only the summed total; beyond ~20 steps the frozen-contact replay is chaotic "
-                "in these parameters (derivatives grow by orders of magnitude and central differences change "
-                "with h), so the full-horizon comparison measures conditioning, not the adjoint",
+        "grad_seed0_20_steps": dict(zip(("v0x", "v0y", "v0z", "mass", "friction"), g_short)),
+        "note": "the 120-step backward is timed but its gradient is not reported: the penalty ramp makes the "
+                "tangent map expansive (FP64: max |dx/dv0| 3e9 by 60 steps) and the float32 tangents of 120 steps "
+                "overflow. The 20-step gradient equals the oracle's float-valued dual numbers "
+                "(tests/test_gpu_adjoint.py); float32 central differences are no check of it, since at any usable "
+                "step they cross pieces of the piecewise-smooth float32 map",
     }
 
 
