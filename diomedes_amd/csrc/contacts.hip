@@ -104,7 +104,8 @@ __global__ void __launch_bounds__(kBlock) k_contacts_narrow(WorldView w, BuildOp
     const uint32_t np = *src.p_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t t = src.off[np];
-        *dst.m_narrow = t < dst.m_cap ? t : dst.m_cap;
+        dst.m_narrow[0] = t < dst.m_cap ? t : dst.m_cap;
+        dst.m_narrow[1] = t;
         if (t > dst.m_cap) atomicMax(overflow, t);
     }
     const uint32_t old_m = *old.m_dev;
@@ -258,7 +259,8 @@ __global__ void __launch_bounds__(kBlock) k_container_write(WorldView w, Contain
     if (t == 0) {
         const uint32_t tot = mn + dst.cont_off[w.n];
         *dst.m_dev = tot < dst.m_cap ? tot : dst.m_cap;
-        if (tot > dst.m_cap) atomicMax(overflow, tot);
+        const uint32_t need = dst.m_narrow[1] + dst.cont_off[w.n];  // with the narrow contacts cut off by m_cap
+        if (need > dst.m_cap) atomicMax(overflow, need);
     }
     if (dst.cont_off[i + 1] == dst.cont_off[i]) return;  // the whole group: body i has no wall contact
     WallPoints wp;

@@ -21,7 +21,7 @@ struct ContactArrays {
 struct ContactSet {
     ContactArrays c;
     uint32_t* m_dev;     // total contact count
-    uint32_t* m_narrow;  // count of the narrow prefix
+    uint32_t* m_narrow;  // [0] count of the narrow prefix (at most m_cap), [1] narrow contacts found
     uint32_t* cont_cnt;  // per body (n_cap)
     uint32_t* cont_off;  // n_cap + 1
     uint32_t m_cap;

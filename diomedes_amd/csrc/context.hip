@@ -277,7 +277,7 @@ struct dio_ctx {
     float init_dt = 0.0f, init_g[3] = {0.0f, 0.0f, 0.0f};
     // and, with hard joints, the options of their warm start
     float init_alpha = 0.0f, init_gamma = 0.0f, init_pmin = 0.0f, init_pmax = 0.0f;
-    uint32_t* w_overflow = nullptr;
+    uint32_t* w_overflow = nullptr;  // [0] candidate pairs past max_pairs, [1] contacts past max_contacts
     unsigned long long* w_acc = nullptr;  // sums of pairs, contacts, CSR nnz over a dio_step call
     float4 *w_lam_alt = nullptr, *w_pen_alt = nullptr;  // scratch multipliers of a solve
     std::vector<hipEvent_t> prof_events;
@@ -952,7 +952,7 @@ void ensure_world(dio_ctx* c) {
         ContactSet& cs = c->cs[k];
         cs.c = alloc_contacts(A, m);
         cs.m_dev = A.alloc<uint32_t>(1);
-        cs.m_narrow = A.alloc<uint32_t>(1);
+        cs.m_narrow = A.alloc<uint32_t>(2);
         cs.cont_cnt = A.alloc<uint32_t>(n);
         cs.cont_off = A.alloc<uint32_t>((size_t)n + 1);
         cs.m_cap = m;
@@ -976,7 +976,7 @@ void ensure_world(dio_ctx* c) {
         t.iner_lin = A.alloc<float4>(n);
         t.iner_ang = A.alloc<float4>(n);
     }
-    c->w_overflow = A.alloc<uint32_t>(1);
+    c->w_overflow = A.alloc<uint32_t>(2);
     c->w_acc = A.alloc<unsigned long long>(4);
     c->w_lam_alt = A.alloc<float4>(m);
     c->w_pen_alt = A.alloc<float4>(m);
@@ -1032,7 +1032,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         };
         if (!lag) broad_narrow(st);
         run_contact_pairs(wv, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]), c->cs[1 - s],
-                          c->w_overflow, st);
+                          c->w_overflow + 1, st);
         // With lag 1, the next step's broad and narrow phases read only the
         // poses, which nothing before the AVBD segment writes, and write
         // pend[1 - s], which the contact build has finished reading: with
@@ -1045,9 +1045,9 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
             DIO_HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
             broad_narrow(c->stream2);
         }
-        run_contact_walls(wv, c->w_ct, bo, c->cs[s], c->cs[1 - s], c->scan_partials, c->w_overflow, st, c->csr.cnt,
+        run_contact_walls(wv, c->w_ct, bo, c->cs[s], c->cs[1 - s], c->scan_partials, c->w_overflow + 1, st, c->csr.cnt,
                           c->csr.cur);
-        run_csr(c->cs[s], n, c->csr, c->w_overflow, st, n > 0);
+        run_csr(c->cs[s], n, c->csr, c->w_overflow + 1, st, n > 0);
         if (c->adj_on)
             launch_record_step(c->adj_rec, c->cs[s], c->csr.off, c->csr.idx, c->csr.peer,
                                BodyState{c->w_pos, c->w_rot, c->w_vel, c->w_ang, c->w_prev}, n, c->m_cap, st);
@@ -1322,10 +1322,10 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
         launch_fill_u32(ctx->pend[k].p_dev, 0u, 1, s);
         launch_fill_u32(ctx->pend[k].out.off, 0u, 1, s);
         launch_fill_u32(ctx->cs[k].m_dev, 0u, 1, s);
-        launch_fill_u32(ctx->cs[k].m_narrow, 0u, 1, s);
+        launch_fill_u32(ctx->cs[k].m_narrow, 0u, 2, s);
         launch_fill_u32(ctx->cs[k].cont_off, 0u, n + 1, s);
     }
-    launch_fill_u32(ctx->w_overflow, 0u, 1, s);
+    launch_fill_u32(ctx->w_overflow, 0u, 2, s);
     launch_fill_u32(ctx->wb.err, 0u, 1, s);
     DIO_HIP_CHECK(hipStreamSynchronize(s));
     std::memset(&ctx->stats, 0, sizeof(ctx->stats));
@@ -1460,12 +1460,13 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
             const int par = (int)((ctx->w_step - 1) % 2);
             const int lag = so.contact_lag ? 1 : 0;
             const int bn_pend = lag ? 1 - par : par;
-            uint32_t w[5] = {0, 0, 0, 0, 0};
+            uint32_t w[4] = {0, 0, 0, 0};
+            uint32_t ov[2] = {0, 0};
             DIO_HIP_CHECK(hipMemcpy(&w[0], ctx->pend[bn_pend].p_dev, 4, hipMemcpyDeviceToHost));
             DIO_HIP_CHECK(hipMemcpy(&w[1], ctx->cs[par].m_dev, 4, hipMemcpyDeviceToHost));
             DIO_HIP_CHECK(hipMemcpy(&w[2], ctx->csr.off + ctx->w_n, 4, hipMemcpyDeviceToHost));
             DIO_HIP_CHECK(hipMemcpy(&w[3], ctx->cs[par].m_narrow, 4, hipMemcpyDeviceToHost));
-            DIO_HIP_CHECK(hipMemcpy(&w[4], ctx->w_overflow, 4, hipMemcpyDeviceToHost));
+            DIO_HIP_CHECK(hipMemcpy(ov, ctx->w_overflow, sizeof(ov), hipMemcpyDeviceToHost));
             uint32_t terr = 0;
             DIO_HIP_CHECK(hipMemcpy(&terr, ctx->wb.err, 4, hipMemcpyDeviceToHost));
             st.pairs = w[0];
@@ -1496,7 +1497,14 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
                 *cnt[mk.first] += 1;
             }
             if (terr) throw StatusError{DIO_HIP, "LBVH build / traversal error (malformed tree or stack overflow)", 0};
-            if (w[4]) capacity("a step exceeded max_pairs / max_contacts; results are truncated", w[4]);
+            if (ov[0])
+                capacity("a step needed " + std::to_string(ov[0]) + " candidate pairs, more than max_pairs (" +
+                             std::to_string(ctx->p_max) + "); its results are truncated",
+                         ov[0]);
+            if (ov[1])
+                capacity("a step needed " + std::to_string(ov[1]) + " contacts, more than max_contacts (" +
+                             std::to_string(ctx->m_cap) + "); its results are truncated",
+                         ov[1]);
         }
         return DIO_OK;
     });

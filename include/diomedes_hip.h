@@ -320,7 +320,12 @@ dio_status dio_batch_upload(dio_ctx* ctx, const dio_gpu_body* bodies,
                             const dio_container* containers);
 /* Vertex pool for DIO_SHAPE_HULL bodies (x, y, z per vertex). */
 dio_status dio_world_upload_hulls(dio_ctx* ctx, const float* vertices, uint32_t count);
-/* Run nsteps physics steps on the device without host round trips. */
+/* Run nsteps physics steps on the device without host round trips.
+   DIO_CAPACITY when a step needed more candidate pairs than max_pairs or more
+   contacts than max_contacts: dio_required_capacity gives the largest count
+   needed (pairs are reported before contacts) and dio_last_error names the
+   limit.  The step then ran on truncated lists and no longer follows the
+   reference; every later dio_step reports the same until the next upload. */
 dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_step_opts* opts,
                     uint32_t nsteps);
 dio_status dio_step_get_stats(dio_ctx* ctx, dio_step_stats* out);

@@ -179,6 +179,41 @@ def _compare_world(ctx, w, step, check_contacts=True):
         assert_bits(gc["r_a"][:, :3], rc["r_a"], f"step {step} r_a")
 
 
+@pytest.mark.parametrize("limit", ["pairs", "contacts"])
+def test_world_step_capacity(oracle, limit):
+    """A fused step that needs more candidate pairs than max_pairs, or more
+    contacts than max_contacts: every step before it stays bit-exact, that
+    step returns CAPACITY with the count the oracle finds at that step and
+    names the limit, later calls report it again, and an upload clears it.
+    Every device write stays within the limits (the lists are truncated)."""
+    from diomedes_amd._lib import DioError
+
+    scene = scenes.cube_field(4, nx=6, ny=3, nz=6)  # 45 pairs at first, growing as the field collapses
+    cap = 50 if limit == "pairs" else 40
+    kw = {"max_pairs": cap} if limit == "pairs" else {"max_contacts": cap}
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n, **kw) as ctx:
+        ctx.upload(scene)
+        for step in range(1, 81):
+            w.step(nsteps=1)
+            need = len(w.pairs()) if limit == "pairs" else len(w.contacts()["a"])
+            if need > cap:
+                break
+            ctx.step(scene.options, 1)
+            _compare_world(ctx, w, step)
+        assert need > cap, "the scene never reached the limit"
+        _print(limit, "limit", cap, "first exceeded at step", step, "needing", need)
+        with pytest.raises(DioError) as e:
+            ctx.step(scene.options, 1)
+        assert e.value.status == "CAPACITY" and e.value.required == need
+        assert f"max_{limit} ({cap})" in str(e.value)
+        with pytest.raises(DioError) as e:
+            ctx.step(scene.options, 2)
+        assert e.value.status == "CAPACITY"
+        ctx.upload(scene)
+        ctx.step(scene.options, 1)
+
+
 def test_sphere_heap_bitexact(oracle):
     """A world of spheres only (no pair reaches GJK, so the step leaves out
     the EPA launches): 400 spheres of two radii dropped into a container."""
