@@ -566,6 +566,90 @@ def test_static_contacts_bitexact(oracle, fuse, lanes):
             _compare_world(ctx, w, 15 * (k + 1))
 
 
+# ------------------------------------------------------------ edge worlds
+def test_empty_world():
+    """A world of no bodies uploads, steps (nothing runs) and downloads."""
+    from diomedes_amd.physics import Scene
+
+    empty = Scene.from_bodies([], (), scenes.floor_container(5.0), AvbdOptions(iterations=3))
+    with PhysicsContext(4) as ctx:
+        ctx.upload(empty)
+        ctx.step(empty.options, 5)
+        got = ctx.download()
+        assert all(len(v) == 0 for v in got.values())
+        assert len(ctx.contacts()) == 0 and len(ctx.pairs()) == 0
+        assert ctx.stats().dynamic_bodies == 0
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_one_body_bitexact(oracle, mode):
+    """A single cube dropped tilted into a container: wall contacts only, no
+    pairs, 150 steps."""
+    from diomedes_amd.physics import AvbdBody, Scene
+
+    b = AvbdBody.cube((0.3, 2.0, -0.2), 0.5, 1.5)
+    b.orientation = (0.2, 0.1, 0.05, float(np.sqrt(1.0 - 0.0525)))
+    b.velocity = (1.0, 0.0, 0.5)
+    scene = Scene.from_bodies([b], (), scenes.floor_container(3.0), AvbdOptions(iterations=6))
+    w = oracle.OracleWorld(scene, 32)
+    w.set_mode(mode)
+    with PhysicsContext(1) as ctx:
+        ctx.upload(scene)
+        for k in range(3):
+            ctx.step(scene.options, 50, mode=mode)
+            w.step(nsteps=50)
+            _compare_world(ctx, w, 50 * (k + 1))
+        assert ctx.stats().pairs == 0 and ctx.stats().contacts > 0
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_static_world_bitexact(oracle, mode):
+    """Only static bodies, overlapping each other and the floor: no primal
+    has work, the poses never change, and the static-static contacts take
+    every dual update in the final launch."""
+    from diomedes_amd.physics import AvbdBody, Scene
+
+    bodies = [AvbdBody.cube((0.7 * (i % 4), 0.4 + 0.3 * (i // 4), 0.1 * i), 0.5, 0.0) for i in range(12)]
+    scene = Scene.from_bodies(bodies, (), scenes.floor_container(8.0), AvbdOptions(iterations=4))
+    w = oracle.OracleWorld(scene, 32)
+    w.set_mode(mode)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        ctx.step(scene.options, 20, mode=mode)
+        w.step(nsteps=20)
+        _compare_world(ctx, w, 20)
+        got = ctx.download()
+        assert np.array_equal(got["positions"], scene.bodies["position"][:, :3])
+        assert ctx.stats().dynamic_bodies == 0 and ctx.stats().contacts > 0
+
+
+def test_batched_table_with_empty_and_single_scenes(oracle):
+    """A scene table holding empty scenes and a one-body scene between
+    ordinary ones: each non-empty scene steps as the oracle steps it alone."""
+    from diomedes_amd.physics import Scene
+
+    base = scenes.batched_scenes(7, num_scenes=3, n_min=8, n_max=30).scenes
+    one = base[0]
+    single = Scene(one.bodies[:1].copy(), one.state[:1].copy(), container=one.container, hull=one.hull,
+                   options=one.options)
+    none = Scene(one.bodies[:0].copy(), one.state[:0].copy(), container=one.container, hull=one.hull,
+                 options=one.options)
+    parts = [none, base[0], none, single, base[1], none, base[2], none]
+    batch = scenes.BatchedScenes(parts, one.hull, one.options)
+    worlds = {s: oracle.OracleWorld(sc, 32) for s, sc in enumerate(parts) if sc.n > 0}
+    with PhysicsContext(batch.n, max_hull_vertices=len(batch.hull)) as ctx:
+        ctx.batch_upload(batch)
+        for k in range(2):
+            ctx.step(batch.options, 25)
+            got = ctx.download()
+            for s, w in worlds.items():
+                w.step(nsteps=25)
+                ref = w.state()
+                sl = batch.scene_slice(s)
+                for key in ("positions", "orientations", "velocities", "angular_velocities"):
+                    assert_bits(got[key][sl], ref[key], f"scene {s} step {25 * (k + 1)} {key}")
+
+
 # ------------------------------------------------------------ batched world
 def _scene_contacts(gc, lo, hi):
     """The contacts of bodies lo .. hi - 1 in a batched contact list, with
