@@ -160,8 +160,9 @@ __global__ void __launch_bounds__(256) k_tree_lower(BodyView bv, const uint32_t*
     for (uint32_t t = threadIdx.x; t < chunk; t += 256) {
         const uint32_t p = base + t;
         float4 a = make_float4(3.4e38f, 3.4e38f, 3.4e38f, 0.0f), b = make_float4(-3.4e38f, -3.4e38f, -3.4e38f, 0.0f);
+        uint32_t body = kNone;
         if (p < n) {
-            const uint32_t body = perm[p];
+            body = perm[p];
             f3 u, w;
             shape_aabb(load_shape(bv, body), HullPool{bv.hull}, u, w);
             a = f4(u, 0.0f);
@@ -171,7 +172,9 @@ __global__ void __launch_bounds__(256) k_tree_lower(BodyView bv, const uint32_t*
         }
         lo[t] = a;
         hi[t] = b;
-        nmn[leaf0 + p] = a;
+        // a leaf carries its body's index in the unused w, so the traversal
+        // gets it with the box instead of by a second, dependent load
+        nmn[leaf0 + p] = make_float4(a.x, a.y, a.z, __uint_as_float(body));
         nmx[leaf0 + p] = b;
     }
     __syncthreads();
@@ -246,8 +249,8 @@ __device__ __forceinline__ uint32_t group_bits(unsigned long long m) {
 // false if the stack would overflow (never for D <= 7).
 template <int LANES, class Hit>
 __device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t lo_pos, uint32_t hi_pos, uint32_t D,
-                                     const uint32_t* perm, const float4* nmn, const float4* nmx, const float4* clo,
-                                     const float4* chi, uint32_t ncached, uint32_t* stk, Hit&& hit) {
+                                     const float4* nmn, const float4* nmx, const float4* clo, const float4* chi,
+                                     uint32_t ncached, uint32_t* stk, Hit&& hit) {
     constexpr int kPer = 8 / LANES;
     const uint32_t gl = threadIdx.x % (uint32_t)LANES;
     const uint32_t below = (1u << gl) - 1u;
@@ -269,14 +272,16 @@ __device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t 
             const uint32_t c = 8 * h + 1 + gl + LANES * k;
             const uint32_t first = (c - cs) * span;
             bool ov = false;
+            uint32_t leaf_body = kNone;
             if (first < hi_pos && first + span > lo_pos) {
                 const float4 lo = c < ncached ? clo[c] : nmn[c];
                 const float4 hi = c < ncached ? chi[c] : nmx[c];
                 ov = aabb_overlap(qa, qb, lo, hi);
+                leaf_body = __float_as_uint(lo.w);  // a leaf's body (perm[first]); 0 for inner nodes
             }
             uint32_t b = kNone;
             if (leaves && ov) {
-                b = perm[first];
+                b = leaf_body;
                 if (b <= a) b = kNone;
             }
             const uint32_t hm = group_bits<LANES>(__ballot(b != kNone));
@@ -338,7 +343,7 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_count(uint32_t n, uint32_t 
     uint32_t* mine = cache + (size_t)a * kPairCache;
     uint32_t lo, hi;
     query_range(scenes, a, n, lo, hi);
-    const bool ok = walk<LANES>(a, amn[a], amx[a], lo, hi, D, perm, nmn, nmx, clo, chi, ncached,
+    const bool ok = walk<LANES>(a, amn[a], amx[a], lo, hi, D, nmn, nmx, clo, chi, ncached,
                                 stack[threadIdx.x / LANES], [&](int, uint32_t b, uint32_t rank, uint32_t total) {
                                     if (b != kNone && cnt + rank < kPairCache) mine[cnt + rank] = b;
                                     cnt += total;
@@ -392,7 +397,7 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_emit(uint32_t n, uint32_t D
     uint32_t k = 0;
     uint32_t lo, hi;
     query_range(scenes, a, n, lo, hi);
-    walk<LANES>(a, amn[a], amx[a], lo, hi, D, perm, nmn, nmx, clo, chi, ncached, stack[grp],
+    walk<LANES>(a, amn[a], amx[a], lo, hi, D, nmn, nmx, clo, chi, ncached, stack[grp],
                 [&](int, uint32_t b, uint32_t rank, uint32_t cnt) {
                     if (b != kNone && seg + k + rank < p_max) pairs[seg + k + rank] = make_uint2(a, b);
                     k += cnt;
