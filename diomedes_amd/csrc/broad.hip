@@ -301,15 +301,42 @@ __device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t 
     return ok;
 }
 
-__device__ __forceinline__ void load_top(const float4* nmn, const float4* nmx, uint32_t D, float4* clo, float4* chi,
-                                         uint32_t& ncached) {
+// The top levels 0..3 into LDS.  The refit's chunks write every level from
+// D - 3 down; above that, k_tree_top writes the levels when D > kLocalTopMax,
+// and for D = 4 or 5 each traversal workgroup forms them here from level
+// D - 3 (9 nodes at most), which saves that launch.  Unions are exact, so
+// both give the same boxes; nodes past the last body stay unset and are
+// never read.
+constexpr uint32_t kLocalTopMax = 5;
+__device__ __forceinline__ void load_top(const float4* nmn, const float4* nmx, uint32_t n, uint32_t D, float4* clo,
+                                         float4* chi, uint32_t& ncached) {
     const uint32_t nodes = level_start(D + 1);
     ncached = nodes < kTopNodes ? nodes : kTopNodes;
-    for (uint32_t t = threadIdx.x; t < ncached; t += kTravBlock) {
+    const bool local_top = D >= 4 && D <= kLocalTopMax;
+    const uint32_t from = local_top ? level_start(D - 3) : 0u;
+    for (uint32_t t = from + threadIdx.x; t < ncached; t += kTravBlock) {
         clo[t] = nmn[t];
         chi[t] = nmx[t];
     }
     __syncthreads();
+    if (!local_top) return;
+    for (int L = (int)D - 4; L >= 0; --L) {
+        const uint32_t count = 1u << (3 * L);            // 8 or 1 < kTravBlock
+        const uint32_t span = 1u << (3 * (D - L - 1));  // leaves under one child
+        const uint32_t k = threadIdx.x;
+        if (k < count && (uint64_t)(8 * k) * span < n) {
+            const uint32_t c0 = level_start(L + 1) + 8 * k;
+            float4 u = clo[c0], w = chi[c0];
+            for (uint32_t j = 1; j < 8; ++j) {
+                if ((uint64_t)(8 * k + j) * span >= n) break;
+                u = box_union_lo(u, clo[c0 + j]);
+                w = box_union_hi(w, chi[c0 + j]);
+            }
+            clo[level_start(L) + k] = u;
+            chi[level_start(L) + k] = w;
+        }
+        __syncthreads();
+    }
 }
 
 // Pass 1: count body a's partners b > a into counts[a] and keep the first
@@ -334,7 +361,7 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_count(uint32_t n, uint32_t 
     __shared__ float4 clo[kTopNodes], chi[kTopNodes];
     __shared__ uint32_t stack[kTravBlock / LANES][kStackDepth];
     uint32_t ncached;
-    load_top(nmn, nmx, D, clo, chi, ncached);
+    load_top(nmn, nmx, n, D, clo, chi, ncached);
     const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
     const uint32_t q = gid / LANES;
     if (q >= n) return;  // whole groups leave together (LANES divides 64)
@@ -374,7 +401,7 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_emit(uint32_t n, uint32_t D
         if (clear1) *clear1 = 0u;
     }
     uint32_t ncached;
-    load_top(nmn, nmx, D, clo, chi, ncached);
+    load_top(nmn, nmx, n, D, clo, chi, ncached);
     const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
     const uint32_t q = gid / LANES, gl = threadIdx.x % LANES;
     if (q >= n) return;
@@ -499,7 +526,7 @@ void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p
     const uint32_t chunk = full < kChunkLeaves ? full : kChunkLeaves;
     hipLaunchKernelGGL(k_tree_lower, dim3((n + chunk - 1) / chunk), dim3(256), 0, s, bv, (const uint32_t*)b.perm, n, D,
                        chunk, b.aabb_min, b.aabb_max, b.node_min, b.node_max);
-    if (D > 3) hipLaunchKernelGGL(k_tree_top, dim3(1), dim3(1024), 0, s, n, D, b.node_min, b.node_max);
+    if (D > kLocalTopMax) hipLaunchKernelGGL(k_tree_top, dim3(1), dim3(1024), 0, s, n, D, b.node_min, b.node_max);
     tree_pairs(lanes, 0, n, D, b, p_max, s, scenes);
     launch_scan_exclusive(b.counts, b.offsets, n, nullptr, b.scan_partials, s);
     tree_pairs(lanes, 1, n, D, b, p_max, s, scenes);  // also sets p_dev and clears clear_words

@@ -26,7 +26,8 @@ def _print(*a):
 
 # ------------------------------------------------------------ broad phase
 @pytest.mark.parametrize("blanes", [1, 2, 4, 8])
-@pytest.mark.parametrize("nx,ny,nz,seed", [(4, 2, 4, 1), (8, 2, 8, 2), (32, 8, 32, 0), (1, 1, 3, 5)])
+@pytest.mark.parametrize("nx,ny,nz,seed", [(4, 2, 4, 1), (8, 2, 8, 2), (32, 8, 32, 0), (1, 1, 3, 5), (16, 4, 16, 6),
+                                          (64, 10, 64, 6)])
 def test_broad_pairs_match_dense_sweep(oracle, nx, ny, nz, seed, blanes):
     _print("broad", nx, ny, nz, seed, blanes)
     scene = scenes.cube_field(seed, nx=nx, ny=ny, nz=nz, spacing=1.2)
