@@ -400,14 +400,17 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_emit(uint32_t n, uint32_t D
         if (clear0) *clear0 = 0u;
         if (clear1) *clear1 = 0u;
     }
-    uint32_t ncached;
-    load_top(nmn, nmx, n, D, clo, chi, ncached);
     const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
     const uint32_t q = gid / LANES, gl = threadIdx.x % LANES;
-    if (q >= n) return;
-    const uint32_t a = perm[q];
-    const uint32_t seg = offsets[a];
-    const uint32_t total = offsets[a + 1] - seg;
+    const bool active = q < n;
+    const uint32_t a = active ? perm[q] : 0u;
+    const uint32_t seg = active ? offsets[a] : 0u;
+    const uint32_t total = active ? offsets[a + 1] - seg : 0u;
+    // only a body with more than kPairCache partners walks the tree again:
+    // a workgroup without one skips the top levels' load
+    uint32_t ncached = 0;
+    if (__syncthreads_or(active && total > kPairCache)) load_top(nmn, nmx, n, D, clo, chi, ncached);
+    if (!active) return;
     const uint32_t grp = threadIdx.x / LANES;
     if (total <= kPairCache) {
         const uint32_t* mine = cache + (size_t)a * kPairCache;
