@@ -125,14 +125,20 @@ __global__ void __launch_bounds__(kBlock) k_morton(const float4* mn, const float
 // order only loosens the boxes.  Refit: one workgroup per chunk of 512 leaves
 // (three levels reduced in LDS), then one workgroup for the levels above -- no
 // atomics.  Traversal: a group of LANES lanes per query tests the 8 children
-// of a node together (8 / LANES each) and keeps the group's stack in LDS, so
-// each step costs one round of independent loads; a shallow tree (5 levels at
-// 8192 bodies) keeps the chain of dependent steps short.
+// of a node together (8 / LANES each; with 16 or 32 lanes, each 8 lanes take
+// a node of their own, so a step pops 2 or 4 nodes) and keeps the group's
+// stack in LDS, so each step costs one round of independent loads; a shallow
+// tree (5 levels at 8192 bodies) keeps the chain of dependent steps short.
 
 constexpr uint32_t kChunkLeaves = 512;  // 8^3 leaves per refit workgroup
 constexpr uint32_t kTopNodes = 585;     // levels 0..3, kept in LDS by the traversal
 constexpr int kTravBlock = 128;
 constexpr int kStackDepth = 64;         // >= 7 D + 1 for D <= 7
+// popping P nodes a step needs at most 7 P D + P entries: 64 P covers D <= 7
+template <int LANES>
+constexpr int stack_depth() {
+    return kStackDepth * (LANES > 8 ? LANES / 8 : 1);
+}
 
 __host__ __device__ __forceinline__ uint32_t level_start(uint32_t L) { return ((1u << (3 * L)) - 1u) / 7u; }
 
@@ -238,7 +244,7 @@ __device__ __forceinline__ void wave_sync() {
 template <int LANES>
 __device__ __forceinline__ uint32_t group_bits(unsigned long long m) {
     const uint32_t g = (threadIdx.x % 64u) / (uint32_t)LANES;
-    return (uint32_t)(m >> (g * LANES)) & ((1u << LANES) - 1u);
+    return (uint32_t)(m >> (g * LANES)) & (LANES >= 32 ? 0xFFFFFFFFu : ((1u << (LANES % 32)) - 1u));
 }
 
 // Walk the tree for query body a (box qa..qb) with the LANES lanes of this
@@ -246,13 +252,18 @@ __device__ __forceinline__ uint32_t group_bits(unsigned long long m) {
 // query's scene).  hit(k, b, rank, total) is called on every lane for each child slot k
 // after each step: b is the partner found by this lane (or kNone), rank its
 // position among the group's hits of that slot, total their count.  Returns
-// false if the stack would overflow (never for D <= 7).
+// false if the stack would overflow (never for D <= 7).  The partners come in
+// no particular order; both passes place them by rank.
 template <int LANES, class Hit>
 __device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t lo_pos, uint32_t hi_pos, uint32_t D,
                                      const float4* nmn, const float4* nmx, const float4* clo, const float4* chi,
                                      uint32_t ncached, uint32_t* stk, Hit&& hit) {
-    constexpr int kPer = 8 / LANES;
+    constexpr int kPer = LANES >= 8 ? 1 : 8 / LANES;        // child slots per lane
+    constexpr uint32_t kPop = LANES > 8 ? LANES / 8 : 1u;  // nodes popped per step
+    constexpr uint32_t kDepth = (uint32_t)stack_depth<LANES>();
     const uint32_t gl = threadIdx.x % (uint32_t)LANES;
+    const uint32_t oct = LANES > 8 ? gl / 8u : 0u;  // which popped node this lane tests
+    const uint32_t slot = LANES > 8 ? gl % 8u : gl;
     const uint32_t below = (1u << gl) - 1u;
     bool ok = true;
     if (D == 0) return ok;
@@ -260,8 +271,10 @@ __device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t 
     uint32_t sp = 1;
     wave_sync();
     while (sp > 0) {
-        const uint32_t e = stk[sp - 1];
-        --sp;
+        const uint32_t popped = sp < kPop ? sp : kPop;
+        const bool live = oct < popped;
+        const uint32_t e = stk[live ? sp - 1 - oct : sp - 1];
+        sp -= popped;
         wave_sync();
         const uint32_t h = e & 0x00FFFFFFu, cl = (e >> 24) + 1u;  // children's level
         const uint32_t span = 1u << (3 * (D - cl));
@@ -269,11 +282,11 @@ __device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t 
         const bool leaves = cl == D;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            const uint32_t c = 8 * h + 1 + gl + LANES * k;
+            const uint32_t c = 8 * h + 1 + slot + LANES * k;
             const uint32_t first = (c - cs) * span;
             bool ov = false;
             uint32_t leaf_body = kNone;
-            if (first < hi_pos && first + span > lo_pos) {
+            if (live && first < hi_pos && first + span > lo_pos) {
                 const float4 lo = c < ncached ? clo[c] : nmn[c];
                 const float4 hi = c < ncached ? chi[c] : nmx[c];
                 ov = aabb_overlap(qa, qb, lo, hi);
@@ -290,11 +303,11 @@ __device__ __forceinline__ bool walk(uint32_t a, float4 qa, float4 qb, uint32_t 
             const uint32_t pm = group_bits<LANES>(__ballot(push));
             const uint32_t pos = sp + __popc(pm & below);
             if (push) {
-                if (pos < (uint32_t)kStackDepth) stk[pos] = c | (cl << 24);
+                if (pos < kDepth) stk[pos] = c | (cl << 24);
                 else ok = false;
             }
             sp += __popc(pm);
-            if (sp > (uint32_t)kStackDepth) sp = kStackDepth;
+            if (sp > kDepth) sp = kDepth;
         }
         wave_sync();
     }
@@ -359,7 +372,7 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_count(uint32_t n, uint32_t 
                                                            const float4* nmx, uint32_t* counts, uint32_t* cache,
                                                            uint32_t* err, SceneTable scenes) {
     __shared__ float4 clo[kTopNodes], chi[kTopNodes];
-    __shared__ uint32_t stack[kTravBlock / LANES][kStackDepth];
+    __shared__ uint32_t stack[kTravBlock / LANES][stack_depth<LANES>()];
     uint32_t ncached;
     load_top(nmn, nmx, n, D, clo, chi, ncached);
     const uint32_t gid = blockIdx.x * kTravBlock + threadIdx.x;
@@ -390,7 +403,7 @@ __global__ void __launch_bounds__(kTravBlock) k_tree_emit(uint32_t n, uint32_t D
                                                           SceneTable scenes, uint32_t* p_dev, uint32_t* overflow,
                                                           uint32_t* clear0, uint32_t* clear1) {
     __shared__ float4 clo[kTopNodes], chi[kTopNodes];
-    __shared__ uint32_t stack[kTravBlock / LANES][kStackDepth];
+    __shared__ uint32_t stack[kTravBlock / LANES][stack_depth<LANES>()];
     __shared__ uint32_t list[kTravBlock / LANES][kPairCache];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // the pair count min(total, p_max) for the narrow phase, and its counters
@@ -502,6 +515,8 @@ void tree_pairs(int lanes, int mode, uint32_t n, uint32_t D, const BroadBuffers&
         case 1: launch_tree_pairs<1>(mode, n, D, b, p_max, s, st); break;
         case 2: launch_tree_pairs<2>(mode, n, D, b, p_max, s, st); break;
         case 4: launch_tree_pairs<4>(mode, n, D, b, p_max, s, st); break;
+        case 16: launch_tree_pairs<16>(mode, n, D, b, p_max, s, st); break;
+        case 32: launch_tree_pairs<32>(mode, n, D, b, p_max, s, st); break;
         default: launch_tree_pairs<8>(mode, n, D, b, p_max, s, st); break;
     }
 }

@@ -38,6 +38,15 @@ using namespace dio;
 
 namespace {
 
+// Traversal lanes per query: the tuned value, or by world size when it is 0.
+// Up to 64k bodies the step is short of work and the chain of dependent tree
+// steps sets the time, so 16 lanes pop two nodes a step; above, the GPU is
+// full and 8 lanes do less work per query.
+constexpr uint32_t kWideTraversalMaxBodies = 65536;
+inline uint32_t traversal_lanes(uint32_t tuned, uint32_t n) {
+    return tuned ? tuned : (n <= kWideTraversalMaxBodies ? 16u : 8u);
+}
+
 struct HipError {
     hipError_t code;
     const char* what;
@@ -228,7 +237,7 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{32, 8, 16, 1, 8, 0, {0, 0}};
+    dio_tuning tuning{32, 0, 16, 1, 8, 0, {0, 0}};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -655,7 +664,7 @@ dio_status dio_broad_submit(dio_ctx* ctx, const dio_gpu_body* bodies, uint32_t n
         launch_fill_u32(ctx->gb.overflow, 0u, 1, ctx->stream);
         launch_fill_u32(ctx->gb.err, 0u, 1, ctx->stream);
         BodyView bv{ctx->g_pos, ctx->g_rot, ctx->g_tag, ctx->g_corners, ctx->g_hull};
-        ctx->g_lanes = ctx->tuning.broad_lanes;
+        ctx->g_lanes = traversal_lanes(ctx->tuning.broad_lanes, n);
         run_broad(bv, n, ctx->gb, ctx->p_max, ctx->stream, true, (int)ctx->g_lanes);
         DIO_HIP_CHECK(hipGetLastError());
         ctx->g_broad_submitted = true;
@@ -1027,7 +1036,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
             b.overflow = c->w_overflow;
             b.clear_words[0] = c->pend[dst_pend].out.gjk_count;
             b.clear_words[1] = c->pend[dst_pend].out.epa_retry_count;
-            run_broad(bv, n, b, c->p_max, bs, resort, (int)c->tuning.broad_lanes, scenes);
+            run_broad(bv, n, b, c->p_max, bs, resort, (int)traversal_lanes(c->tuning.broad_lanes, n), scenes);
             run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, bs, true, c->w_gjk_possible);
         };
         if (!lag) broad_narrow(st);
@@ -1515,14 +1524,14 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         if (!t) invalid("tuning is null");
         dio_tuning v = *t;
         if (v.primal_lanes == 0) v.primal_lanes = 32;
-        if (v.broad_lanes == 0) v.broad_lanes = 8;
         if (v.resort_interval == 0) v.resort_interval = 16;
         if (v.adjoint_lanes == 0) v.adjoint_lanes = 8;
         if (v.primal_lanes != 1 && v.primal_lanes != 4 && v.primal_lanes != 8 && v.primal_lanes != 16 &&
             v.primal_lanes != 32 && v.primal_lanes != 64)
             invalid("primal_lanes must be 1, 4, 8, 16, 32 or 64");
-        if (v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 && v.broad_lanes != 8)
-            invalid("broad_lanes must be 1, 2, 4 or 8");
+        if (v.broad_lanes != 0 && v.broad_lanes != 1 && v.broad_lanes != 2 && v.broad_lanes != 4 &&
+            v.broad_lanes != 8 && v.broad_lanes != 16 && v.broad_lanes != 32)
+            invalid("broad_lanes must be 0 (by world size), 1, 2, 4, 8, 16 or 32");
         if (v.fuse_dual > 1) invalid("fuse_dual must be 0 or 1");
         if (v.adjoint_lanes != 1 && v.adjoint_lanes != 2 && v.adjoint_lanes != 4 && v.adjoint_lanes != 8 &&
             v.adjoint_lanes != 16)
@@ -1583,7 +1592,7 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
         // re-sorts of the fused step)
         ensure_granular_broad(ctx);
         const BodyView bv{ctx->w_pos, ctx->w_rot, ctx->w_tag, ctx->w_corners, ctx->w_hull};
-        const int blanes[DIO_TUNE_BROAD_SHAPES] = {1, 2, 4, 8};
+        const int blanes[DIO_TUNE_BROAD_SHAPES] = {1, 2, 4, 8, 16, 32};
         float bbest = 3.4e38f;
         int best_blanes = 8;
         run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, true, 4);

@@ -640,7 +640,7 @@ class PhysicsContext:
         schedules on the current world, keep the fastest; returns the choice
         and the per-shape microseconds."""
         t = R.Tuning()
-        us = (ctypes.c_float * 12)()
+        us = (ctypes.c_float * 14)()
         o = options.to_c()
         self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
         u = list(us)
@@ -649,8 +649,8 @@ class PhysicsContext:
             "broad_lanes": t.broad_lanes,
             "fuse_dual": t.fuse_dual,
             "us_per_launch": dict(zip((1, 4, 8, 16, 32, 64), u[:6])),
-            "broad_us_per_pass": dict(zip((1, 2, 4, 8), u[6:10])),
-            "dual_schedule_us": {"unfused": u[10], "fused": u[11]},
+            "broad_us_per_pass": dict(zip((1, 2, 4, 8, 16, 32), u[6:12])),
+            "dual_schedule_us": {"unfused": u[12], "fused": u[13]},
         }
 
     def stats(self) -> R.StepStats:

@@ -230,11 +230,14 @@ typedef struct dio_step_stats {
 
 /* Kernel launch shapes (the autotuner's search space). */
 #define DIO_TUNE_PRIMAL_SHAPES 6
-#define DIO_TUNE_BROAD_SHAPES 4
+#define DIO_TUNE_BROAD_SHAPES 6
 #define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 2)
 typedef struct dio_tuning {
     uint32_t primal_lanes;     /* lanes per body in the primal: 1, 4, 8, 16, 32 or 64 (0 = default 32) */
-    uint32_t broad_lanes;      /* lanes per query in the BVH traversal: 1, 2, 4 or 8 (0 = default 8) */
+    uint32_t broad_lanes;      /* lanes per query in the BVH traversal: 1, 2, 4, 8, 16 or 32; with 16
+                                  or 32 a step pops 2 or 4 nodes.  0 (default): by world size, 16 up
+                                  to 65,536 bodies (a short chain of dependent steps), 8 above (the
+                                  GPU is full and extra lanes only add work) */
     uint32_t resort_interval;  /* fused step: Morton re-sort every this many steps; the tree is refit
                                   over the previous order in between (0 = default 16) */
     uint32_t fuse_dual;        /* 1: each primal iteration k >= 1 applies dual update k - 1 itself
@@ -336,7 +339,7 @@ dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out);
    current contact set and every broad-phase traversal shape on the current
    poses (reps launches each, HIP events), keep the fastest of each, and report
    the average microseconds per launch in us_per_launch[0 .. DIO_TUNE_SLOTS - 1]:
-   primal lanes 1, 4, 8, 16, 32, 64; broad lanes 1, 2, 4, 8 (a refit + traversal
+   primal lanes 1, 4, 8, 16, 32, 64; broad lanes 1, 2, 4, 8, 16, 32 (a refit + traversal
    pass); then, at the chosen primal shape, an unfused primal plus a dual
    launch and a fused primal (fuse_dual is set to the faster).  The
    primal with a fixed parity is idempotent and the broad timing runs in the

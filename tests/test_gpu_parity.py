@@ -25,7 +25,7 @@ def _print(*a):
 
 
 # ------------------------------------------------------------ broad phase
-@pytest.mark.parametrize("blanes", [1, 2, 4, 8])
+@pytest.mark.parametrize("blanes", [1, 2, 4, 8, 16, 32])
 @pytest.mark.parametrize("nx,ny,nz,seed", [(4, 2, 4, 1), (8, 2, 8, 2), (32, 8, 32, 0), (1, 1, 3, 5), (16, 4, 16, 6),
                                           (64, 10, 64, 6)])
 def test_broad_pairs_match_dense_sweep(oracle, nx, ny, nz, seed, blanes):
@@ -44,11 +44,14 @@ def test_broad_pairs_match_dense_sweep(oracle, nx, ny, nz, seed, blanes):
     assert np.array_equal(got, ref)
 
 
-def test_broad_degenerate_inputs(oracle):
-    # one body, two coincident bodies, many identical bodies (equal Morton codes)
-    for n in (1, 2, 50):
+@pytest.mark.parametrize("blanes", [8, 16, 32])
+def test_broad_degenerate_inputs(oracle, blanes):
+    # one body, two coincident bodies, many identical bodies (equal Morton
+    # codes; every node overlaps every query, the traversal stack's worst case)
+    for n in (1, 2, 50, 600):
         scene = scenes.cube_field(3, nx=n, ny=1, nz=1, spacing=0.0)
         with PhysicsContext(max(n, 1), max_pairs=n * n + 16) as ctx:
+            ctx.set_tuning(broad_lanes=blanes)
             ctx.broad_phase_submit(scene.bodies)
             pairs = ctx.broad_phase_pairs()
         ref = oracle.broad_pairs(scene.bodies)
@@ -256,7 +259,7 @@ def test_box_stack_300_steps_bitexact(oracle):
 @pytest.mark.parametrize(
     "lag,lanes,blanes,resort,fuse",
     [(1, 8, 4, 4, 0), (0, 8, 4, 4, 1), (1, 4, 2, 1, 0), (1, 16, 8, 1000, 1), (1, 1, 1, 4, 1), (0, 4, 8, 1000, 0),
-     (1, 32, 1, 4, 1), (1, 32, 8, 4, 0), (1, 64, 8, 4, 1)],
+     (1, 32, 1, 4, 1), (1, 32, 8, 4, 0), (1, 64, 8, 4, 1), (1, 32, 16, 16, 1), (0, 32, 32, 4, 1)],
 )
 def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort, fuse):
     """resort: Morton re-sort interval of the fused step; 1000 refits the first
@@ -284,7 +287,7 @@ def test_autotune_keeps_state_and_parity(oracle):
         res = ctx.autotune(scene.options, reps=5)
         _print("autotune", res)
         assert res["primal_lanes"] in (1, 4, 8, 16, 32, 64)
-        assert res["broad_lanes"] in (1, 2, 4, 8)
+        assert res["broad_lanes"] in (1, 2, 4, 8, 16, 32)
         assert res["fuse_dual"] in (0, 1)
         _compare_world(ctx, w, 40)
         ctx.step(scene.options, 20)
