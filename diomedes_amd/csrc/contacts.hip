@@ -100,7 +100,7 @@ __device__ __forceinline__ int64_t find_old_pair(const NarrowSource& old_src, ui
 
 __global__ void __launch_bounds__(kBlock) k_contacts_narrow(WorldView w, BuildOptions o, NarrowSource src,
                                                             ContactSet dst, NarrowSource old_src, ContactSet old,
-                                                            uint32_t* overflow) {
+                                                            uint32_t* overflow, uint32_t* csr_cnt) {
     const uint32_t np = *src.p_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t t = src.off[np];
@@ -135,6 +135,12 @@ __global__ void __launch_bounds__(kBlock) k_contacts_narrow(WorldView w, BuildOp
             }
             warm_scale(o, lam, pen);
             put_contact(dst.c, ci, pr.x, pr.y, neg(normal), t1, t2, ra, rb, o.margin - n4.w, o.friction, lam, pen);
+        }
+        // each written contact is a CSR entry of both bodies
+        const uint32_t written = base < dst.m_cap ? min(cnt, dst.m_cap - base) : 0u;
+        if (written) {
+            atomicAdd(&csr_cnt[pr.x], written);
+            atomicAdd(&csr_cnt[pr.y], written);
         }
     }
 }
@@ -227,15 +233,11 @@ __device__ __forceinline__ uint32_t wall_contact_count(const WallPoints& wp) {
 constexpr int kWallLanes = 8;
 
 __global__ void __launch_bounds__(kBlock) k_container_count(WorldView w, ContainerParams ct, BuildOptions o,
-                                                            uint32_t* cnt, uint32_t* zero_a, uint32_t* zero_b) {
+                                                            uint32_t* cnt) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t i = t / kWallLanes;
     const int wall = (int)(t % kWallLanes);
     if (i >= w.n) return;  // whole groups of kWallLanes lanes leave together
-    if (zero_a && wall == 0) {
-        zero_a[i] = 0u;
-        zero_b[i] = 0u;
-    }
     uint32_t c = 0;
     const ContainerParams cp = w.scene_ct ? w.scene_ct[w.scene_of[i]] : ct;
     if (wall < 6 && cp.enabled && w.inv_mm[i].w != 0.0f) {
@@ -250,7 +252,8 @@ __global__ void __launch_bounds__(kBlock) k_container_count(WorldView w, Contain
 }
 
 __global__ void __launch_bounds__(kBlock) k_container_write(WorldView w, ContainerParams ct, BuildOptions o,
-                                                            ContactSet dst, ContactSet old, uint32_t* overflow) {
+                                                            ContactSet dst, ContactSet old, uint32_t* overflow,
+                                                            uint32_t* csr_cnt) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t i = t / kWallLanes;
     const int wall = (int)(t % kWallLanes);
@@ -293,6 +296,7 @@ __global__ void __launch_bounds__(kBlock) k_container_write(WorldView w, Contain
         }
     }
     uint32_t ci = mn + dst.cont_off[i] + before;
+    uint32_t written = 0;  // this lane's CSR entries of body i
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         if (k >= wp.count || !(wp.depth[k] > 0.0f)) continue;
@@ -304,19 +308,12 @@ __global__ void __launch_bounds__(kBlock) k_container_write(WorldView w, Contain
             const f3 ra = qrot_inv(s.rot, sub(wp.pts[k], s.pos));
             put_contact(dst.c, ci, i, kContainer, wp.normal, wp.t1, wp.t2, ra, f3{0.0f, 0.0f, 0.0f},
                         o.margin - wp.depth[k], o.friction, lam, pen);
+            ++written;
         }
         ++before;
         ++ci;
     }
-}
-
-__global__ void __launch_bounds__(kBlock) k_csr_count(ContactSet cs, uint32_t* cnt) {
-    const uint32_t m = *cs.m_dev;
-    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < m; c += gridDim.x * kBlock) {
-        const uint2 ab = cs.c.ab[c];
-        atomicAdd(&cnt[ab.x], 1u);
-        if (ab.y != kContainer) atomicAdd(&cnt[ab.y], 1u);
-    }
+    if (written) atomicAdd(&csr_cnt[i], written);
 }
 
 __global__ void __launch_bounds__(kBlock) k_csr_fill(ContactSet cs, const uint32_t* off, uint32_t* cur, uint32_t* tmp,
@@ -344,11 +341,16 @@ __device__ __forceinline__ uint32_t peer_code(uint2 ab, uint32_t body) {
 constexpr int kCsrLanes = 8;
 __global__ void __launch_bounds__(kBlock) k_csr_rank(uint32_t n, const uint32_t* off, const uint32_t* tmp,
                                                      const uint2* ab, uint32_t* idx, uint32_t* peer, uint32_t idx_cap,
-                                                     uint32_t* overflow) {
+                                                     uint32_t* overflow, uint32_t* cnt, uint32_t* cur) {
     const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     if (g == 0 && off[n] > idx_cap) atomicMax(overflow, off[n]);
     const uint32_t i = g / kCsrLanes, lane = g % kCsrLanes;
     if (i >= n) return;  // whole groups of kCsrLanes lanes leave together
+    if (lane == 0) {
+        // the counters' last reader was the fill: zero them for the next step's writers
+        cnt[i] = 0u;
+        cur[i] = 0u;
+    }
     const uint32_t b = off[i];
     uint32_t e = off[i + 1];
     if (e > idx_cap) e = idx_cap;
@@ -399,33 +401,31 @@ __global__ void __launch_bounds__(kBlock) k_unpack_contacts(const dio_avbd_conta
 }  // namespace
 
 void run_contact_pairs(const WorldView& w, const BuildOptions& o, const NarrowSource& src, const ContactSet& dst,
-                       const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow, hipStream_t s) {
+                       const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow, hipStream_t s,
+                       uint32_t* csr_cnt) {
     uint32_t pb = blocks_for(dst.m_cap > 0 ? dst.m_cap / 4 + 1 : 1);
     if (pb > 2048) pb = 2048;
-    hipLaunchKernelGGL(k_contacts_narrow, dim3(pb), dim3(kBlock), 0, s, w, o, src, dst, old_src, old, overflow);
+    hipLaunchKernelGGL(k_contacts_narrow, dim3(pb), dim3(kBlock), 0, s, w, o, src, dst, old_src, old, overflow,
+                       csr_cnt);
 }
 
 void run_contact_walls(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const ContactSet& dst,
                        const ContactSet& old, uint32_t* scan_partials, uint32_t* overflow, hipStream_t s,
-                       uint32_t* csr_cnt, uint32_t* csr_cur) {
+                       uint32_t* csr_cnt) {
     const uint32_t nb = blocks_for((w.n > 0 ? w.n : 1) * (uint32_t)kWallLanes);
-    hipLaunchKernelGGL(k_container_count, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst.cont_cnt, csr_cnt,
-                       csr_cnt ? csr_cur : nullptr);
+    hipLaunchKernelGGL(k_container_count, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst.cont_cnt);
     launch_scan_exclusive(dst.cont_cnt, dst.cont_off, w.n, nullptr, scan_partials, s);
-    hipLaunchKernelGGL(k_container_write, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst, old, overflow);
+    hipLaunchKernelGGL(k_container_write, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst, old, overflow, csr_cnt);
 }
 
-void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s,
-             bool counters_cleared) {
-    if (!counters_cleared) launch_fill2_u32(b.cnt, b.cur, 0u, n, s);
+void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s) {
     uint32_t mb = blocks_for(cs.m_cap > 0 ? cs.m_cap : 1);
     if (mb > 2048) mb = 2048;
-    hipLaunchKernelGGL(k_csr_count, dim3(mb), dim3(kBlock), 0, s, cs, b.cnt);
     launch_scan_exclusive(b.cnt, b.off, n, nullptr, b.scan_partials, s);
     hipLaunchKernelGGL(k_csr_fill, dim3(mb), dim3(kBlock), 0, s, cs, (const uint32_t*)b.off, b.cur, b.tmp, b.idx_cap);
     const uint32_t rb = blocks_for((n > 0 ? n : 1) * (uint32_t)kCsrLanes);
     hipLaunchKernelGGL(k_csr_rank, dim3(rb), dim3(kBlock), 0, s, n, (const uint32_t*)b.off, (const uint32_t*)b.tmp,
-                       (const uint2*)cs.c.ab, b.idx, b.peer, b.idx_cap, overflow);
+                       (const uint2*)cs.c.ab, b.idx, b.peer, b.idx_cap, overflow, b.cnt, b.cur);
 }
 
 void launch_csr_peers(uint32_t n, const uint32_t* off, const uint32_t* idx, const uint2* ab, uint32_t* peer,

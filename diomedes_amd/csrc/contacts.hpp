@@ -82,19 +82,18 @@ void launch_csr_peers(uint32_t n, const uint32_t* off, const uint32_t* idx, cons
 // Build `dst` from the narrow output `src` and the container, warm-started
 // from `old` (built from `old_src`); `overflow` records a capacity excess.
 // Two parts: the narrow-phase contacts (the only part that reads old_src, the
-// previous step's pair list and offsets), then the container contacts.
+// previous step's pair list and offsets), then the container contacts.  Both
+// add each written contact to csr_cnt (n words, zero on entry) of its bodies.
 void run_contact_pairs(const WorldView& w, const BuildOptions& o, const NarrowSource& src, const ContactSet& dst,
-                       const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow, hipStream_t s);
+                       const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow, hipStream_t s,
+                       uint32_t* csr_cnt);
 void run_contact_walls(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const ContactSet& dst,
                        const ContactSet& old, uint32_t* scan_partials, uint32_t* overflow, hipStream_t s,
-                       uint32_t* csr_cnt = nullptr,
-                       uint32_t* csr_cur = nullptr);  // csr_cnt / csr_cur (n words each): zeroed on the way
+                       uint32_t* csr_cnt);
 
-// CSR of a contact set (physics/gpu.rs:286-310).
-// counters_cleared: b.cnt and b.cur were zeroed by run_contact_walls (its
-// csr argument); otherwise they are zeroed here.
-void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s,
-             bool counters_cleared = false);
+// CSR of a contact set (physics/gpu.rs:286-310) from the per-body counts the
+// contact build left in b.cnt; leaves b.cnt and b.cur zero for the next build.
+void run_csr(const ContactSet& cs, uint32_t n, const CsrBuffers& b, uint32_t* overflow, hipStream_t s);
 
 // Unpack GpuAvbdContact records into SoA (granular path).
 void launch_unpack_contacts(const dio_avbd_contact* rec, uint32_t m, const ContactArrays& c, hipStream_t s);

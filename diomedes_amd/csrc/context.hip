@@ -1041,7 +1041,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         };
         if (!lag) broad_narrow(st);
         run_contact_pairs(wv, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]), c->cs[1 - s],
-                          c->w_overflow + 1, st);
+                          c->w_overflow + 1, st, c->csr.cnt);
         // With lag 1, the next step's broad and narrow phases read only the
         // poses, which nothing before the AVBD segment writes, and write
         // pend[1 - s], which the contact build has finished reading: with
@@ -1054,9 +1054,9 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
             DIO_HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
             broad_narrow(c->stream2);
         }
-        run_contact_walls(wv, c->w_ct, bo, c->cs[s], c->cs[1 - s], c->scan_partials, c->w_overflow + 1, st, c->csr.cnt,
-                          c->csr.cur);
-        run_csr(c->cs[s], n, c->csr, c->w_overflow + 1, st, n > 0);
+        run_contact_walls(wv, c->w_ct, bo, c->cs[s], c->cs[1 - s], c->scan_partials, c->w_overflow + 1, st,
+                          c->csr.cnt);
+        run_csr(c->cs[s], n, c->csr, c->w_overflow + 1, st);
         if (c->adj_on)
             launch_record_step(c->adj_rec, c->cs[s], c->csr.off, c->csr.idx, c->csr.peer,
                                BodyState{c->w_pos, c->w_rot, c->w_vel, c->w_ang, c->w_prev}, n, c->m_cap, st);
@@ -1335,6 +1335,8 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
         launch_fill_u32(ctx->cs[k].cont_off, 0u, n + 1, s);
     }
     launch_fill_u32(ctx->w_overflow, 0u, 2, s);
+    // the CSR counters start at zero; every step's CSR build leaves them so
+    if (ctx->n_cap > 0) launch_fill2_u32(ctx->csr.cnt, ctx->csr.cur, 0u, ctx->n_cap, s);
     launch_fill_u32(ctx->wb.err, 0u, 1, s);
     DIO_HIP_CHECK(hipStreamSynchronize(s));
     std::memset(&ctx->stats, 0, sizeof(ctx->stats));
