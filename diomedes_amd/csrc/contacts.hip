@@ -98,18 +98,20 @@ __device__ __forceinline__ int64_t find_old_pair(const NarrowSource& old_src, ui
     return -1;
 }
 
-__global__ void __launch_bounds__(kBlock) k_contacts_narrow(WorldView w, BuildOptions o, NarrowSource src,
-                                                            ContactSet dst, NarrowSource old_src, ContactSet old,
-                                                            uint32_t* overflow, uint32_t* csr_cnt) {
+// The narrow-phase contacts, on workgroups bid = 0 .. nblk - 1 of the launch.
+__device__ __forceinline__ void contacts_narrow_part(uint32_t bid, uint32_t nblk, const WorldView& w,
+                                                     const BuildOptions& o, const NarrowSource& src,
+                                                     const ContactSet& dst, const NarrowSource& old_src,
+                                                     const ContactSet& old, uint32_t* overflow, uint32_t* csr_cnt) {
     const uint32_t np = *src.p_dev;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         const uint32_t t = src.off[np];
         dst.m_narrow[0] = t < dst.m_cap ? t : dst.m_cap;
         dst.m_narrow[1] = t;
         if (t > dst.m_cap) atomicMax(overflow, t);
     }
     const uint32_t old_m = *old.m_dev;
-    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < np; p += gridDim.x * kBlock) {
+    for (uint32_t p = bid * kBlock + threadIdx.x; p < np; p += nblk * kBlock) {
         const uint32_t base = src.off[p], cnt = src.off[p + 1] - base;
         if (cnt == 0) continue;
         const uint2 pr = src.pairs[p];
@@ -232,9 +234,10 @@ __device__ __forceinline__ uint32_t wall_contact_count(const WallPoints& wp) {
 // arithmetic unchanged.
 constexpr int kWallLanes = 8;
 
-__global__ void __launch_bounds__(kBlock) k_container_count(WorldView w, ContainerParams ct, BuildOptions o,
-                                                            uint32_t* cnt) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+// Each body's container contact count, on workgroup bid of its part.
+__device__ __forceinline__ void container_count_part(uint32_t bid, const WorldView& w, const ContainerParams& ct,
+                                                     const BuildOptions& o, uint32_t* cnt) {
+    const uint32_t t = bid * kBlock + threadIdx.x;
     const uint32_t i = t / kWallLanes;
     const int wall = (int)(t % kWallLanes);
     if (i >= w.n) return;  // whole groups of kWallLanes lanes leave together
@@ -249,6 +252,17 @@ __global__ void __launch_bounds__(kBlock) k_container_count(WorldView w, Contain
     c += __shfl_xor(c, 2, kWallLanes);
     c += __shfl_xor(c, 4, kWallLanes);
     if (wall == 0) cnt[i] = c;
+}
+
+// The narrow-phase contacts and the container counts are independent: one
+// launch runs both, the first pb workgroups on the pairs, the rest on the
+// bodies (kWallLanes lanes each, whole workgroups per part).
+__global__ void __launch_bounds__(kBlock) k_contacts_build(WorldView w, BuildOptions o, NarrowSource src,
+                                                           ContactSet dst, NarrowSource old_src, ContactSet old,
+                                                           uint32_t* overflow, uint32_t* csr_cnt, ContainerParams ct,
+                                                           uint32_t pb) {
+    if (blockIdx.x < pb) contacts_narrow_part(blockIdx.x, pb, w, o, src, dst, old_src, old, overflow, csr_cnt);
+    else container_count_part(blockIdx.x - pb, w, ct, o, dst.cont_cnt);
 }
 
 __global__ void __launch_bounds__(kBlock) k_container_write(WorldView w, ContainerParams ct, BuildOptions o,
@@ -400,20 +414,20 @@ __global__ void __launch_bounds__(kBlock) k_unpack_contacts(const dio_avbd_conta
 
 }  // namespace
 
-void run_contact_pairs(const WorldView& w, const BuildOptions& o, const NarrowSource& src, const ContactSet& dst,
-                       const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow, hipStream_t s,
-                       uint32_t* csr_cnt) {
+void run_contact_pairs(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const NarrowSource& src,
+                       const ContactSet& dst, const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow,
+                       hipStream_t s, uint32_t* csr_cnt) {
     uint32_t pb = blocks_for(dst.m_cap > 0 ? dst.m_cap / 4 + 1 : 1);
     if (pb > 2048) pb = 2048;
-    hipLaunchKernelGGL(k_contacts_narrow, dim3(pb), dim3(kBlock), 0, s, w, o, src, dst, old_src, old, overflow,
-                       csr_cnt);
+    const uint32_t nb = blocks_for((w.n > 0 ? w.n : 1) * (uint32_t)kWallLanes);
+    hipLaunchKernelGGL(k_contacts_build, dim3(pb + nb), dim3(kBlock), 0, s, w, o, src, dst, old_src, old, overflow,
+                       csr_cnt, ct, pb);
 }
 
 void run_contact_walls(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const ContactSet& dst,
                        const ContactSet& old, uint32_t* scan_partials, uint32_t* overflow, hipStream_t s,
                        uint32_t* csr_cnt) {
     const uint32_t nb = blocks_for((w.n > 0 ? w.n : 1) * (uint32_t)kWallLanes);
-    hipLaunchKernelGGL(k_container_count, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst.cont_cnt);
     launch_scan_exclusive(dst.cont_cnt, dst.cont_off, w.n, nullptr, scan_partials, s);
     hipLaunchKernelGGL(k_container_write, dim3(nb), dim3(kBlock), 0, s, w, ct, o, dst, old, overflow, csr_cnt);
 }

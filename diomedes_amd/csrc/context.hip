@@ -1040,7 +1040,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
             run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, bs, true, c->w_gjk_possible);
         };
         if (!lag) broad_narrow(st);
-        run_contact_pairs(wv, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]), c->cs[1 - s],
+        run_contact_pairs(wv, c->w_ct, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]), c->cs[1 - s],
                           c->w_overflow + 1, st, c->csr.cnt);
         // With lag 1, the next step's broad and narrow phases read only the
         // poses, which nothing before the AVBD segment writes, and write
