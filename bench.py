@@ -477,8 +477,9 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            f"bench.py: --gpus {args.gpus} is not supported: this build targets one MI355X; the batched and "
-            "single-scene sharding designs are described in DESIGN.md and not run here.",
+            f"bench.py: --gpus {args.gpus} is not supported: the step runs on one MI355X. A scene table would "
+            "split across GPUs by scene with no exchange; one large scene would split by Morton range with a "
+            "pose halo per iteration. Neither is run here.",
             file=sys.stderr,
         )
         return 2
