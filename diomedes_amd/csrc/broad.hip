@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "broad.hpp"
+#include "tree_refit.hpp"
 #include "prim.hpp"
 
 namespace dio {
@@ -23,18 +24,6 @@ constexpr int kBlock = 256;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
-
-__device__ __forceinline__ ShapeRef load_shape(const BodyView& bv, uint32_t i) {
-    ShapeRef s;
-    s.pos = xyz(bv.pos[i]);
-    s.rot = as_q(bv.rot[i]);
-    s.tag = bv.tag[i];
-    s.c[0] = bv.corners[4 * i];
-    s.c[1] = bv.corners[4 * i + 1];
-    s.c[2] = bv.corners[4 * i + 2];
-    s.c[3] = bv.corners[4 * i + 3];
-    return s;
-}
 
 __global__ void __launch_bounds__(kBlock) k_aabb(BodyView bv, uint32_t n, float4* mn, float4* mx) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -130,7 +119,6 @@ __global__ void __launch_bounds__(kBlock) k_morton(const float4* mn, const float
 // stack in LDS, so each step costs one round of independent loads; a shallow
 // tree (5 levels at 8192 bodies) keeps the chain of dependent steps short.
 
-constexpr uint32_t kChunkLeaves = 512;  // 8^3 leaves per refit workgroup
 constexpr uint32_t kTopNodes = 585;     // levels 0..3, kept in LDS by the traversal
 constexpr int kTravBlock = 128;
 constexpr int kStackDepth = 64;         // >= 7 D + 1 for D <= 7
@@ -140,79 +128,9 @@ constexpr int stack_depth() {
     return kStackDepth * (LANES > 8 ? LANES / 8 : 1);
 }
 
-__host__ __device__ __forceinline__ uint32_t level_start(uint32_t L) { return ((1u << (3 * L)) - 1u) / 7u; }
+// Leaves and the levels of each chunk of kChunkLeaves sorted positions.
+__global__ void __launch_bounds__(256) k_tree_lower(TreeLower tl) { tree_lower_part(blockIdx.x, tl); }
 
-__host__ __device__ __forceinline__ uint32_t tree_depth(uint32_t n) {
-    uint32_t d = 0;
-    while ((1u << (3 * d)) < n) ++d;
-    return d;
-}
-
-__device__ __forceinline__ float4 box_union_lo(float4 a, float4 b) {
-    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
-}
-__device__ __forceinline__ float4 box_union_hi(float4 a, float4 b) {
-    return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), 0.0f);
-}
-
-// Leaves (the bodies' AABBs, computed here and also stored per body for the
-// queries) and the levels of one chunk of `chunk` sorted positions.
-__global__ void __launch_bounds__(256) k_tree_lower(BodyView bv, const uint32_t* perm, uint32_t n, uint32_t D,
-                                                    uint32_t chunk, float4* amn, float4* amx, float4* nmn,
-                                                    float4* nmx) {
-    __shared__ float4 lo[kChunkLeaves], hi[kChunkLeaves];
-    const uint32_t base = blockIdx.x * chunk;
-    const uint32_t leaf0 = level_start(D);
-    for (uint32_t t = threadIdx.x; t < chunk; t += 256) {
-        const uint32_t p = base + t;
-        float4 a = make_float4(3.4e38f, 3.4e38f, 3.4e38f, 0.0f), b = make_float4(-3.4e38f, -3.4e38f, -3.4e38f, 0.0f);
-        uint32_t body = kNone;
-        if (p < n) {
-            body = perm[p];
-            f3 u, w;
-            shape_aabb(load_shape(bv, body), HullPool{bv.hull}, u, w);
-            a = f4(u, 0.0f);
-            b = f4(w, 0.0f);
-            amn[body] = a;
-            amx[body] = b;
-        }
-        lo[t] = a;
-        hi[t] = b;
-        // a leaf carries its body's index in the unused w, so the traversal
-        // gets it with the box instead of by a second, dependent load
-        nmn[leaf0 + p] = make_float4(a.x, a.y, a.z, __uint_as_float(body));
-        nmx[leaf0 + p] = b;
-    }
-    __syncthreads();
-    uint32_t width = chunk / 8;  // this chunk's nodes on the level being built
-    uint32_t L = D;
-    while (width >= 1) {
-        --L;
-        float4 u = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w = u;
-        const uint32_t t = threadIdx.x;  // width <= 64 < 256
-        if (t < width) {
-            u = lo[8 * t];
-            w = hi[8 * t];
-#pragma unroll
-            for (int j = 1; j < 8; ++j) {
-                u = box_union_lo(u, lo[8 * t + j]);
-                w = box_union_hi(w, hi[8 * t + j]);
-            }
-            nmn[level_start(L) + blockIdx.x * width + t] = u;
-            nmx[level_start(L) + blockIdx.x * width + t] = w;
-        }
-        __syncthreads();
-        if (t < width) {
-            lo[t] = u;
-            hi[t] = w;
-        }
-        __syncthreads();
-        width /= 8;
-    }
-}
-
-// Levels above the chunk roots (level D - 3), one workgroup; children whose
-// first leaf is past n were never built and are skipped.
 __global__ void __launch_bounds__(1024) k_tree_top(uint32_t n, uint32_t D, float4* nmn, float4* nmx) {
     for (int L = (int)D - 4; L >= 0; --L) {
         const uint32_t count = 1u << (3 * L);
@@ -523,7 +441,7 @@ void tree_pairs(int lanes, int mode, uint32_t n, uint32_t D, const BroadBuffers&
 }  // namespace
 
 void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s, bool resort,
-               int lanes, const SceneTable& scenes) {
+               int lanes, const SceneTable& scenes, bool refit_done) {
     if (n == 0) {
         launch_fill_u32(b.offsets, 0u, 1, s);
         hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, b.offsets, p_max, b.p_dev, b.overflow);
@@ -539,11 +457,9 @@ void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p
         // 30-bit keys: four 8-bit passes, so the sorted values land back in perm
         radix_sort_pairs(b.keys0, b.perm, b.keys1, b.vals1, n, 30, b.sort_scratch, s);
     }
-    const uint32_t D = tree_depth(n);
-    const uint32_t full = 1u << (3 * D);
-    const uint32_t chunk = full < kChunkLeaves ? full : kChunkLeaves;
-    hipLaunchKernelGGL(k_tree_lower, dim3((n + chunk - 1) / chunk), dim3(256), 0, s, bv, (const uint32_t*)b.perm, n, D,
-                       chunk, b.aabb_min, b.aabb_max, b.node_min, b.node_max);
+    const TreeLower tl = tree_lower_args(bv, n, b);
+    const uint32_t D = tl.D;
+    if (resort || !refit_done) hipLaunchKernelGGL(k_tree_lower, dim3(tl.blocks), dim3(256), 0, s, tl);
     if (D > kLocalTopMax) hipLaunchKernelGGL(k_tree_top, dim3(1), dim3(1024), 0, s, n, D, b.node_min, b.node_max);
     tree_pairs(lanes, 0, n, D, b, p_max, s, scenes);
     launch_scan_exclusive(b.counts, b.offsets, n, nullptr, b.scan_partials, s);

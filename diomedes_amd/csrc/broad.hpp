@@ -60,11 +60,12 @@ void launch_unpack_gpu_bodies(const dio_gpu_body* rec, uint32_t n, float4* pos, 
 // the device (b.p_dev).  resort: recompute Morton codes and the sorted order
 // (otherwise the tree is refit over the previous order, which keeps the pair
 // set exact and only changes the traversal cost).  lanes: lanes per query in
-// the traversal (1, 2, 4 or 8; each tests 8 / lanes children per step).
-// scenes: with a table, the sort key's high bits are the scene id (so each
-// scene is a contiguous range of sorted positions, equal to its body range)
-// and each query walks only its scene's range.
+// the traversal (1, 2, 4, 8, 16 or 32).  scenes: with a table, the sort key's
+// high bits are the scene id (so each scene is a contiguous range of sorted
+// positions, equal to its body range) and each query walks only its scene's
+// range.  refit_done (refit steps only): the first refit launch already ran
+// on this stream as part of another launch (tree_refit.hpp).
 void run_broad(const BodyView& bv, uint32_t n, const BroadBuffers& b, uint32_t p_max, hipStream_t s, bool resort,
-               int lanes, const SceneTable& scenes = SceneTable{});
+               int lanes, const SceneTable& scenes = SceneTable{}, bool refit_done = false);
 
 }  // namespace dio

@@ -18,6 +18,7 @@
 
 #include "contacts.hpp"
 #include "prim.hpp"
+#include "tree_refit.hpp"
 
 namespace dio {
 
@@ -254,15 +255,18 @@ __device__ __forceinline__ void container_count_part(uint32_t bid, const WorldVi
     if (wall == 0) cnt[i] = c;
 }
 
-// The narrow-phase contacts and the container counts are independent: one
-// launch runs both, the first pb workgroups on the pairs, the rest on the
-// bodies (kWallLanes lanes each, whole workgroups per part).
+// The narrow-phase contacts, the container counts and (on refit steps) the
+// tree refit's chunks are independent: one launch runs them all, the first pb
+// workgroups on the pairs, the next nb on the bodies (kWallLanes lanes each),
+// the rest on the refit chunks; whole workgroups per part.
 __global__ void __launch_bounds__(kBlock) k_contacts_build(WorldView w, BuildOptions o, NarrowSource src,
                                                            ContactSet dst, NarrowSource old_src, ContactSet old,
                                                            uint32_t* overflow, uint32_t* csr_cnt, ContainerParams ct,
-                                                           uint32_t pb) {
+                                                           uint32_t pb, uint32_t nb, TreeLower refit) {
+    static_assert(kBlock == 256, "the refit part runs on 256 threads");
     if (blockIdx.x < pb) contacts_narrow_part(blockIdx.x, pb, w, o, src, dst, old_src, old, overflow, csr_cnt);
-    else container_count_part(blockIdx.x - pb, w, ct, o, dst.cont_cnt);
+    else if (blockIdx.x < pb + nb) container_count_part(blockIdx.x - pb, w, ct, o, dst.cont_cnt);
+    else tree_lower_part(blockIdx.x - pb - nb, refit);
 }
 
 __global__ void __launch_bounds__(kBlock) k_container_write(WorldView w, ContainerParams ct, BuildOptions o,
@@ -416,12 +420,14 @@ __global__ void __launch_bounds__(kBlock) k_unpack_contacts(const dio_avbd_conta
 
 void run_contact_pairs(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const NarrowSource& src,
                        const ContactSet& dst, const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow,
-                       hipStream_t s, uint32_t* csr_cnt) {
+                       hipStream_t s, uint32_t* csr_cnt, const TreeLower* refit) {
     uint32_t pb = blocks_for(dst.m_cap > 0 ? dst.m_cap / 4 + 1 : 1);
     if (pb > 2048) pb = 2048;
     const uint32_t nb = blocks_for((w.n > 0 ? w.n : 1) * (uint32_t)kWallLanes);
-    hipLaunchKernelGGL(k_contacts_build, dim3(pb + nb), dim3(kBlock), 0, s, w, o, src, dst, old_src, old, overflow,
-                       csr_cnt, ct, pb);
+    TreeLower tl{};
+    if (refit) tl = *refit;
+    hipLaunchKernelGGL(k_contacts_build, dim3(pb + nb + tl.blocks), dim3(kBlock), 0, s, w, o, src, dst, old_src, old,
+                       overflow, csr_cnt, ct, pb, nb, tl);
 }
 
 void run_contact_walls(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const ContactSet& dst,

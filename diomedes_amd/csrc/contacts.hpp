@@ -84,10 +84,12 @@ void launch_csr_peers(uint32_t n, const uint32_t* off, const uint32_t* idx, cons
 // Two parts: the narrow-phase contacts (the only part that reads old_src, the
 // previous step's pair list and offsets) with each body's container contact
 // count, then the container contacts.  Both add each written contact to
-// csr_cnt (n words, zero on entry) of its bodies.
+// csr_cnt (n words, zero on entry) of its bodies.  refit (optional): the
+// broad phase's first refit launch, carried by the first part's launch.
+struct TreeLower;
 void run_contact_pairs(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const NarrowSource& src,
                        const ContactSet& dst, const NarrowSource& old_src, const ContactSet& old, uint32_t* overflow,
-                       hipStream_t s, uint32_t* csr_cnt);
+                       hipStream_t s, uint32_t* csr_cnt, const TreeLower* refit = nullptr);
 void run_contact_walls(const WorldView& w, const ContainerParams& ct, const BuildOptions& o, const ContactSet& dst,
                        const ContactSet& old, uint32_t* scan_partials, uint32_t* overflow, hipStream_t s,
                        uint32_t* csr_cnt);

@@ -28,6 +28,7 @@
 #include "../../include/diomedes_hip.h"
 #include "adjoint.hpp"
 #include "broad.hpp"
+#include "tree_refit.hpp"
 #include "colour.hpp"
 #include "contacts.hpp"
 #include "dio_device.hpp"
@@ -1029,6 +1030,11 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         // step's narrow output goes to pend[s] and is used at once.
         const int src = s;
         const int dst_pend = lag ? 1 - s : s;
+        const bool fork = lag && c->tuning.overlap_broad;
+        // On a refit step with lag 1 on one stream, the broad phase's first
+        // refit launch rides on the contact build's first launch: it reads only
+        // the poses and the sorted order, and writes only boxes.
+        const bool carry_refit = lag && !fork && !resort && n > 0;
         auto broad_narrow = [&](hipStream_t bs) {
             BroadBuffers b = c->wb;
             b.pairs = c->pend[dst_pend].pairs;
@@ -1036,19 +1042,20 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
             b.overflow = c->w_overflow;
             b.clear_words[0] = c->pend[dst_pend].out.gjk_count;
             b.clear_words[1] = c->pend[dst_pend].out.epa_retry_count;
-            run_broad(bv, n, b, c->p_max, bs, resort, (int)traversal_lanes(c->tuning.broad_lanes, n), scenes);
+            run_broad(bv, n, b, c->p_max, bs, resort, (int)traversal_lanes(c->tuning.broad_lanes, n), scenes,
+                      carry_refit);
             run_narrow(bv, c->pend[dst_pend].out, c->pend[dst_pend].simplex, bs, true, c->w_gjk_possible);
         };
         if (!lag) broad_narrow(st);
+        const TreeLower refit = tree_lower_args(bv, n, c->wb);
         run_contact_pairs(wv, c->w_ct, bo, source_of(c->pend[src]), c->cs[s], source_of(c->pend[1 - s]), c->cs[1 - s],
-                          c->w_overflow + 1, st, c->csr.cnt);
+                          c->w_overflow + 1, st, c->csr.cnt, carry_refit ? &refit : nullptr);
         // With lag 1, the next step's broad and narrow phases read only the
         // poses, which nothing before the AVBD segment writes, and write
         // pend[1 - s], which the contact build has finished reading: with
         // overlap_broad they run on the second stream beside the container
         // contacts and CSR (both chains are short launches on few
         // workgroups), joined before the segment ends.
-        const bool fork = lag && c->tuning.overlap_broad;
         if (fork) {
             DIO_HIP_CHECK(hipEventRecord(c->ev_fork, st));
             DIO_HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
