@@ -21,7 +21,8 @@ def oracle():
     from oracle import pyoracle
 
     pyoracle.load()
-    pyoracle.threads(int(os.environ.get("ORACLE_THREADS", "8")))
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 8)
+    pyoracle.threads(int(os.environ.get("ORACLE_THREADS", str(min(16, cores)))))
     return pyoracle
 
 
