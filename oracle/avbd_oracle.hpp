@@ -20,19 +20,29 @@
 // baseline; double is the FP64 oracle.  Build with -ffp-contract=off and
 // without fast-math so every operation rounds once, as written.
 //
-// Parity status per function (what pins it; see tests/):
-//   support / aabb / dense sweep     pinned: shape.rs:53-94 tests, brute force
-//   SAT manifold                     pinned: cuboid.rs:404-422 tests, FP64 brute-force SAT
-//   sphere-sphere                    pinned: closed form
-//   GJK / EPA                        pinned: FP64 Minkowski-difference hull (scipy)
-//   raw_contact / tangents           pinned: closed-form identities (orthonormal frame)
-//   container_contacts               pinned: gpu.rs:421 test, closed form
-//   warm start / CSR                 pinned: brute-force dictionary restatement
-//   solve6                           pinned: numpy.linalg.solve on random SPD blocks
-//   init / recover                   pinned: free fall and torque-free spin closed forms
-//   primal / dual                    pinned: resting-stack invariants, FP64 agreement,
-//                                            single-contact closed form
-//   quaternion product / rotation    pinned: constraints.rs:158 test + rotation identities
+// Parity status per function (what pins it; see tests/test_oracle_*.py):
+//   support (sphere, box, tet)      pinned: shape.rs:53-94 tests (both tie rules)
+//   hull support / hull aabb        pinned: numpy first argmax over the pool; rotated-vertex extent
+//   aabb / dense sweep              pinned: brute force pair set
+//   SAT axis, depth, normal         pinned: cuboid.rs:404-422 tests, FP64 brute-force 15-axis SAT
+//   SAT manifold points             pinned: clipped face as a half-plane intersection, projection
+//                                           onto the reference plane, centred contact closed form,
+//                                           two resting-cube closed forms
+//   sphere-sphere                   pinned: closed form
+//   GJK / EPA depth                 pinned: FP64 Minkowski-difference hull (scipy), tets, boxes, hulls
+//   EPA witness points              pinned: containment, (pa - pb).n = depth, support planes,
+//                                           sphere resting on a box
+//   raw_contact / tangents          pinned: orthonormal frame, lever arms, axis-normal closed forms
+//   container_contacts              pinned: gpu.rs:421 test, closed form incl. predicted depth
+//   warm start / CSR                pinned: dictionary restatement of the HashMap semantics
+//   solve6                          pinned: numpy.linalg.solve on random SPD blocks
+//   init / recover                  pinned: free fall, adaptive-gravity cases, torque-free spin
+//   primal / dual                   pinned: resting and stacked force balance, sliding friction mu g,
+//                                           spinning-neighbour sign, spring / socket / slack rope
+//   hard joints, bending, colouring pinned: closed forms and invariants (no reference counterpart)
+//   quaternion product / rotation   pinned: constraints.rs:158 test + rotation identities
+// tests/test_oracle_mutations.py plants 41 plausible mistakes in copies of
+// this file; each fails at least one of the designed pins above.
 #pragma once
 
 #include <algorithm>

@@ -537,19 +537,20 @@ OAPI void oracle_qmul(int precision, const double* a, const double* b, double* o
     else put4(out, qmul(Q<float>{float(a[0]), float(a[1]), float(a[2]), float(a[3])},
                         Q<float>{float(b[0]), float(b[1]), float(b[2]), float(b[3])}));
 }
-// Shape::support (host, last-max) and the kernels' support (first-max)
-OAPI void oracle_support(int precision, int host, const dio_gpu_body* body, const double* d, double* out) {
-    if (precision == 64) {
-        Body<double> b = body_from<double>(*body, nullptr);
-        V3<double> dd{d[0], d[1], d[2]};
-        std::vector<V3<double>> h;
-        put3(out, host ? support_cpu(b, dd, h) : support_gpu(b, dd, h));
-    } else {
-        Body<float> b = body_from<float>(*body, nullptr);
-        V3<float> dd{float(d[0]), float(d[1]), float(d[2])};
-        std::vector<V3<float>> h;
-        put3(out, host ? support_cpu(b, dd, h) : support_gpu(b, dd, h));
-    }
+// Shape::support (host, last-max) and the kernels' support (first-max); a
+// hull body reads its vertices from the pool hull[0 .. hc)
+template <class T>
+static void support_t(int host, const dio_gpu_body* body, const double* d, const float* hull, uint32_t hc,
+                      double* out) {
+    Body<T> b = body_from<T>(*body, nullptr);
+    V3<T> dd{T(d[0]), T(d[1]), T(d[2])};
+    auto h = hull_from<T>(hull, hc);
+    put3(out, host ? support_cpu(b, dd, h) : support_gpu(b, dd, h));
+}
+OAPI void oracle_support(int precision, int host, const dio_gpu_body* body, const double* d, const float* hull,
+                         uint32_t hc, double* out) {
+    if (precision == 64) support_t<double>(host, body, d, hull, hc, out);
+    else support_t<float>(host, body, d, hull, hc, out);
 }
 
 

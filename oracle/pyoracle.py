@@ -72,7 +72,7 @@ def load() -> ctypes.CDLL:
         "oracle_avbd_init": (None, [c_int, c_void_p, _F, _F, c_uint32, POINTER(R.RunOptions), _D, _D, _D, _D]),
         "oracle_qrot": (None, [c_int, _D, _D, _D]),
         "oracle_qmul": (None, [c_int, _D, _D, _D]),
-        "oracle_support": (None, [c_int, c_int, c_void_p, _D, _D]),
+        "oracle_support": (None, [c_int, c_int, c_void_p, _D, _F, c_uint32, _D]),
         "oracle_replay": (None, [c_int, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, _U64, _U, _U, _U64,
                                  POINTER(R.Options), _D, c_double, c_uint32, _D, _D, _D, _D]),
         "oracle_world_keep_presolve": (None, [c_void_p, c_int]),
@@ -415,11 +415,16 @@ def qmul(a, b, precision: int = 32) -> np.ndarray:
     return out
 
 
-def support(body, direction, host: bool = False, precision: int = 32) -> np.ndarray:
+def support(body, direction, host: bool = False, precision: int = 32, hull=None) -> np.ndarray:
+    """Local support point of one body: the kernels' rule (first maximal
+    vertex) or, with host=True, Shape::support's (last maximal corner of a
+    tetrahedron)."""
     body = np.ascontiguousarray(np.atleast_1d(body), R.GPU_BODY)
     d = np.ascontiguousarray(direction, np.float64)
+    hull = np.zeros((0, 3), np.float32) if hull is None else np.ascontiguousarray(hull, np.float32)
     out = np.zeros(3)
-    load().oracle_support(precision, 1 if host else 0, _p(body), _p(d, c_double), _p(out, c_double))
+    load().oracle_support(precision, 1 if host else 0, _p(body), _p(d, c_double), _p(hull, c_float), len(hull),
+                          _p(out, c_double))
     return out
 
 

@@ -392,6 +392,42 @@ def test_torque_free_spin_decay(oracle):
     assert np.allclose(w.state()["angular_velocities"][0], om, rtol=1e-6)
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_spinning_neighbour_pushes_only_when_its_contact_point_approaches(oracle, precision):
+    """The constraint value of a contact row moves with the other body's
+    rotation as its contact point does: c = c0 (1 - alpha) + n.dx_self +
+    (r_self x n).dtheta_self - n.dx_other - (r_other x n).dtheta_other
+    (avbd/src/lib.rs:258-551).  Body a rests on body b (stored normal +y, from
+    b to a, at zero depth past the margin, so c0 = 0); b spins about z with
+    its contact point at r_b = (0.5, 0.5, 0), whose velocity w x r_b has the
+    vertical component 0.5 w.  In one iteration without gravity, a is pushed
+    up when that point moves into it (w > 0: c < 0, f = k c < 0) and does not
+    move at all when it moves away (w < 0: c > 0, f = min(k c, 0) = 0)."""
+    opts = AvbdOptions(gravity=(0.0, 0.0, 0.0), iterations=1)
+    for w in (3.0, -3.0):
+        state = np.zeros(2, R.BODY_STATE)
+        state["inv_mass"] = 1.0
+        state["inv_moment"][:, :3] = 6.0
+        state[1]["ang"][:3] = (0.0, 0.0, w)
+        pos = np.float32([[0.0, 1.0, 0.0], [0.0, 0.0, 0.0]])
+        rot = np.float32([[0, 0, 0, 1], [0, 0, 0, 1]])
+        con = np.zeros(1, R.AVBD_CONTACT)
+        con["a"], con["b"], con["friction"] = 0, 1, 0.5
+        con["normal"][0, :3] = (0.0, 1.0, 0.0)
+        con["tangent1"][0, :3] = (0.0, 0.0, -1.0)
+        con["tangent2"][0, :3] = (-1.0, 0.0, 0.0)
+        con["r_a"][0, :3] = (0.0, -0.5, 0.0)
+        con["r_b"][0, :3] = (0.5, 0.5, 0.0)
+        con["penalty"][0, :3] = 1000.0
+        out = oracle.avbd_run(state, pos, rot, con, np.zeros(0, R.AVBD_CONSTRAINT), np.uint32([0, 1, 2]),
+                              np.uint32([0, 0]), opts.run_options(), precision=precision)
+        dy = out["positions"][0][1] - 1.0
+        if w > 0:
+            assert dy > 1e-4, dy
+        else:
+            assert dy == 0.0 and out["lambda"][0][0] == 0.0, (dy, out["lambda"][0])
+
+
 def test_resting_cube_normal_force_balances_weight(oracle):
     body = AvbdBody.cube((0.0, 0.5, 0.0), 0.5, 3.0)
     opts = AvbdOptions(gravity=(0.0, -9.81, 0.0), iterations=10)
@@ -454,19 +490,21 @@ def test_stacked_dynamic_cubes_impulse_balance(oracle):
     assert abs(np.mean(sums) - (g + (v1 - v0) / T)) < 0.05 * g
 
 
-def test_sliding_friction_decelerates_at_mu_g(oracle):
+@pytest.mark.parametrize("mu", [0.3, 0.6])
+def test_sliding_friction_decelerates_at_mu_g(oracle, mu):
     """The friction cone (|f_t| <= mu |f_n|, avbd/src/lib.rs:391-398): a cube
-    sliding on the floor slows at about mu g."""
+    sliding on the floor slows at about mu g.  The window is one in which the
+    cube slides throughout (Coulomb's law holds only while it slides)."""
     body = AvbdBody.cube((0.0, 0.5, 0.0), 0.5, 1.0)
-    body.velocity = (4.0, 0.0, 0.0)
-    mu = 0.3
+    body.velocity = (8.0, 0.0, 0.0)
     opts = AvbdOptions(gravity=(0.0, -9.81, 0.0), iterations=10, friction=mu)
     w = oracle.OracleWorld(_single(body, scenes.floor_container(50.0), opts), 64)
     w.step(nsteps=20)
     v0 = w.state()["velocities"][0][0]
-    w.step(nsteps=40)
+    w.step(nsteps=20)
     v1 = w.state()["velocities"][0][0]
-    decel = (v0 - v1) / (40 * float(np.float32(opts.dt)))
+    assert v1 > 1.0, "the cube stopped inside the window"
+    decel = (v0 - v1) / (20 * float(np.float32(opts.dt)))
     assert abs(decel - mu * 9.81) < 0.1 * mu * 9.81
 
 
@@ -480,6 +518,31 @@ def test_hanging_spring_equilibrium(oracle):
     w.step(nsteps=1500)
     y = w.state()["positions"][1][1]
     assert abs(y - (5.0 - rest - 2.0 * 9.81 / k)) < 2e-3
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_slack_soft_rope_exerts_nothing(oracle, precision):
+    """The reference's rope row is active only beyond its length
+    (avbd/src/lib.rs:422-539): a bob hanging 0.5 below its anchor on a rope of
+    length 1.5 falls exactly as a free body does, bit for bit, until the rope
+    is taut (after 1 m, about 27 steps); then it comes to rest at the rope's
+    length plus the stretch m g / k."""
+    from diomedes_amd.physics import Rope
+
+    anchor = AvbdBody.cube((0.0, 5.0, 0.0), 0.2, 0.0)
+    bob = AvbdBody.cube((0.0, 4.5, 0.0), 0.2, 2.0)
+    opts = AvbdOptions(gravity=(0.0, -9.81, 0.0), iterations=8)
+    k, length = 1000.0, 1.5
+    roped = oracle.OracleWorld(Scene.from_bodies([anchor, bob], [Rope(0, 1, (0, 0, 0), (0, 0, 0), length)], None,
+                                                 opts), precision)
+    free = oracle.OracleWorld(Scene.from_bodies([anchor, bob], (), None, opts), precision)
+    for _ in range(20):
+        roped.step()
+        free.step()
+        assert np.array_equal(roped.state()["positions"], free.state()["positions"])
+    roped.step(nsteps=1500)
+    y = roped.state()["positions"][1][1]
+    assert abs(y - (5.0 - length - 2.0 * 9.81 / k)) < 2e-3
 
 
 def test_ball_socket_holds_the_anchor(oracle):
