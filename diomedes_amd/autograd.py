@@ -37,19 +37,24 @@ class _Replay(torch.autograd.Function):
         st["inv_moment"][:, :3] = _host(inv_moment)
         opts = dataclasses.replace(options, friction=float(friction))
         pos, rot = ctx.adjoint_replay(opts, st)
-        fctx.saved = (ctx, opts, st)
+        # friction may be a Python number (no gradient) or a one-element
+        # tensor of any shape (its gradient takes that shape)
+        fctx.saved = (ctx, opts, st, tuple(friction.shape) if torch.is_tensor(friction) else None)
         dev = velocities.device
         return torch.from_numpy(pos).to(dev), torch.from_numpy(rot).to(dev)
 
     @staticmethod
     def backward(fctx, g_pos, g_rot):
-        ctx, opts, st = fctx.saved
+        ctx, opts, st, friction_shape = fctx.saved
         g = ctx.adjoint_backward(opts, d_positions=None if g_pos is None else _host(g_pos),
                                  d_orientations=None if g_rot is None else _host(g_rot), state0=st)
         dev = g_pos.device if g_pos is not None else g_rot.device
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        g_friction = None
+        if friction_shape is not None:
+            g_friction = torch.tensor(g["friction"], dtype=torch.float32, device=dev).reshape(friction_shape)
         return (None, None, None, t(g["velocities"]), t(g["angular_velocities"]), t(g["inv_mass"]),
-                t(g["inv_moment"]), torch.tensor(g["friction"], dtype=torch.float32, device=dev))
+                t(g["inv_moment"]), g_friction)
 
 
 def replay(ctx, options, state, velocities, angular_velocities, inv_mass, inv_moment, friction):

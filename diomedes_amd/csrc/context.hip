@@ -253,6 +253,9 @@ struct dio_ctx {
            *w_inv_mm = nullptr, *w_corners = nullptr, *w_hull = nullptr;
     uint32_t* w_tag = nullptr;
     uint32_t w_hull_count = 0;
+    // the hull pool size the world's hull bodies need (checked at each step,
+    // since the pool may be uploaded before or after the bodies)
+    uint64_t w_hull_need = 0;
     uint32_t last_lag = 1;
     uint32_t w_n_dyn = 0;
     // false when every body is a box (or cube), or every body a sphere: no
@@ -372,6 +375,28 @@ template <class F> dio_status guarded(dio_ctx* ctx, F&& f) {
 [[noreturn]] void invalid(const std::string& m) { throw StatusError{DIO_INVALID, m, 0}; }
 [[noreturn]] void capacity(const std::string& m, uint64_t need) { throw StatusError{DIO_CAPACITY, m, need}; }
 
+// A hull body names its vertices as a range of the hull pool: first in
+// corners[0].x and count in corners[0].y, both whole numbers, count >= 1.
+// The kernels read hull[first .. first + count) without a check of their own,
+// so every range is checked here: a malformed record is DIO_INVALID, and the
+// return value is the pool size the bodies need (the largest first + count),
+// which the caller compares with the pool uploaded when it launches.
+uint64_t hull_extent(const dio_gpu_body* bodies, uint32_t n) {
+    uint64_t need = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (bodies[i].shape.tag != kHull) continue;
+        const float first = bodies[i].shape.corners[0][0], count = bodies[i].shape.corners[0][1];
+        const bool whole = std::isfinite(first) && std::isfinite(count) && first >= 0.0f && count >= 1.0f &&
+                           first == std::floor(first) && count == std::floor(count) && first < 16777216.0f &&
+                           count < 16777216.0f;
+        if (!whole)
+            invalid("body " + std::to_string(i) + ": a hull's first vertex and vertex count must be whole numbers, "
+                    "the count at least 1");
+        need = std::max<uint64_t>(need, (uint64_t)first + (uint64_t)count);
+    }
+    return need;
+}
+
 void ensure_granular_broad(dio_ctx* c) {
     if (c->g_ready) return;
     DevArena& A = c->dev;
@@ -390,6 +415,16 @@ void ensure_granular_broad(dio_ctx* c) {
     c->g_np_dev = A.alloc<uint32_t>(1);
     c->g_out = A.alloc<dio_gpu_contact_out>((size_t)c->p_max * 4);
     c->g_ready = true;
+}
+
+// The granular broad and narrow phases read the pool dio_world_upload_hulls
+// uploaded (none before that call).
+void check_granular_hulls(dio_ctx* c, const dio_gpu_body* bodies, uint32_t n) {
+    const uint64_t have = (c->w_hull && c->g_hull == c->w_hull) ? c->w_hull_count : 0;
+    const uint64_t need = hull_extent(bodies, n);
+    if (need > have)
+        invalid("the hull bodies name " + std::to_string(need) + " hull vertices; the uploaded pool holds " +
+                std::to_string(have));
 }
 
 void ensure_granular_avbd(dio_ctx* c) {
@@ -652,6 +687,7 @@ dio_status dio_broad_submit(dio_ctx* ctx, const dio_gpu_body* bodies, uint32_t n
     return guarded(ctx, [&]() {
         if (n > 0 && !bodies) invalid("bodies is null");
         if (n > ctx->n_cap) capacity("too many bodies for the context", n);
+        check_granular_hulls(ctx, bodies, n);
         ensure_granular_broad(ctx);
         DIO_HIP_CHECK(hipSetDevice(ctx->device));
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // staging buffer reuse
@@ -756,6 +792,7 @@ dio_status dio_narrow_submit_pairs(dio_ctx* ctx, const dio_gpu_body* bodies, uin
         if ((n > 0 && !bodies) || (pair_count > 0 && !pairs)) invalid("null input");
         for (uint32_t k = 0; k < pair_count; ++k)
             if (pairs[k].a >= n || pairs[k].b >= n) invalid("pair index out of range");
+        check_granular_hulls(ctx, bodies, n);
         ensure_granular_broad(ctx);
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->g_n = n;
@@ -1000,6 +1037,27 @@ void ensure_world(dio_ctx* c) {
 
 NarrowSource source_of(const dio_ctx::Pending& p) {
     return NarrowSource{p.pairs, p.out.off, p.p_dev, p.out.n, p.out.pa, p.out.pb};
+}
+
+// Copy what the contact build reads of pending buffer `from` (the pair count,
+// pairs, per-pair contact offsets and the four slots of each pair) into `to`.
+void copy_pending(dio_ctx* c, int from, int to) {
+    hipStream_t st = c->stream;
+    const dio_ctx::Pending& a = c->pend[from];
+    const dio_ctx::Pending& b = c->pend[to];
+    uint32_t np = 0;
+    DIO_HIP_CHECK(hipStreamSynchronize(st));
+    DIO_HIP_CHECK(hipMemcpy(&np, a.p_dev, 4, hipMemcpyDeviceToHost));
+    np = std::min(np, c->p_max);
+    DIO_HIP_CHECK(hipMemcpyAsync(b.p_dev, a.p_dev, 4, hipMemcpyDeviceToDevice, st));
+    DIO_HIP_CHECK(hipMemcpyAsync(b.out.off, a.out.off, ((size_t)np + 1) * 4, hipMemcpyDeviceToDevice, st));
+    if (np > 0) {
+        DIO_HIP_CHECK(hipMemcpyAsync(b.pairs, a.pairs, (size_t)np * 8, hipMemcpyDeviceToDevice, st));
+        DIO_HIP_CHECK(hipMemcpyAsync(b.out.n, a.out.n, (size_t)np * 64, hipMemcpyDeviceToDevice, st));
+        DIO_HIP_CHECK(hipMemcpyAsync(b.out.pa, a.out.pa, (size_t)np * 64, hipMemcpyDeviceToDevice, st));
+        DIO_HIP_CHECK(hipMemcpyAsync(b.out.pb, a.out.pb, (size_t)np * 64, hipMemcpyDeviceToDevice, st));
+    }
+    DIO_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 // The world's solve over buffer set s; prepare_next: the final launch also
@@ -1267,6 +1325,7 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
             if (a >= n || b >= n || scene_of[a] != scene_of[b]) invalid("a constraint joins two scenes");
         }
     }
+    const uint64_t hull_need = hull_extent(bodies, n);
     ensure_world(ctx);
     hipStream_t s = ctx->stream;
     DIO_HIP_CHECK(hipStreamSynchronize(s));
@@ -1274,6 +1333,7 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
     ctx->free_adjoint();  // a recording belongs to the world it was made in
     ctx->init_set = -1;
     ctx->w_n = n;
+    ctx->w_hull_need = hull_need;
     ctx->w_step = 0;
     ctx->w_sort_pending = true;
     ctx->w_friction.resize(n);
@@ -1390,6 +1450,9 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
         if (!options) invalid("options is null");
         if (options->iterations == 0) invalid("iterations must be at least 1");
+        if (ctx->w_hull_need > ctx->w_hull_count)
+            invalid("the world's hull bodies name " + std::to_string(ctx->w_hull_need) +
+                    " hull vertices; the uploaded pool holds " + std::to_string(ctx->w_hull_count));
         dio_step_opts so{1, 0, 0};
         if (opts) so = *opts;
         if (so.mode > 1) invalid("mode must be 0 (Jacobi) or 1 (coloured Gauss-Seidel)");
@@ -1398,7 +1461,14 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         if (!coloured && nsteps > 0 && ctx->w_j.hard_dynamic > 0)
             invalid("hard joints between two dynamic bodies need the coloured mode (1): the Jacobi solve "
                     "diverges on them");
-        ctx->last_lag = so.contact_lag ? 1 : 0;
+        // After a lag-0 step, a lag-1 step builds its contacts from the narrow
+        // output that step found on the poses at its start, as if it had run
+        // with lag 1.  That output sits in pend[1 - s], where the step's warm
+        // start also looks up the old contacts' pairs; its source is pend[s],
+        // so the used part of the output is copied there once.
+        if (nsteps > 0 && ctx->w_step > 0 && ctx->last_lag == 0 && so.contact_lag && ctx->w_n > 0)
+            copy_pending(ctx, (int)(1 - ctx->w_step % 2), (int)(ctx->w_step % 2));
+        if (nsteps > 0) ctx->last_lag = so.contact_lag ? 1 : 0;
         hipStream_t s = ctx->stream;
         DIO_HIP_CHECK(hipSetDevice(ctx->device));
         const bool use_graph = !graphs_disabled() && !so.profile;

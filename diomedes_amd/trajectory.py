@@ -33,6 +33,8 @@ The writer appends frames as they come, so a long run need not fit in memory;
 
 from __future__ import annotations
 
+import dataclasses
+
 import ctypes
 import struct
 from dataclasses import dataclass, field
@@ -216,12 +218,14 @@ def read(path) -> Trajectory:
 def record(ctx, scene: Scene, path, steps: int, every: int = 1, velocities: bool = False, pairs: bool = False,
            options: Optional[AvbdOptions] = None) -> int:
     """Upload scene to a PhysicsContext, step it and write a frame after every
-    `every` steps (the initial state is frame 0).  Returns the frames written."""
+    `every` steps (the initial state is frame 0).  The file's header names the
+    options the frames were stepped with: `options` when given, else the
+    scene's own.  Returns the frames written."""
     flags = (FLAG_VELOCITIES if velocities else 0) | (FLAG_PAIRS if pairs else 0)
     opts = options or scene.options
     ctx.upload(scene)
     frames = 0
-    with Writer(path, scene, flags) as w:
+    with Writer(path, dataclasses.replace(scene, options=opts), flags) as w:
         def put(step: int) -> None:
             d = ctx.download()
             pr = None

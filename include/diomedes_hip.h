@@ -51,7 +51,11 @@ enum {
     DIO_SHAPE_BOX = 3,         /* corners[0][0..3] = per-axis half extents (extension) */
     DIO_SHAPE_HULL = 4         /* corners[0][0] = first vertex, [0][1] = vertex count, as
                                   floats holding exact integers, into the hull vertex pool
-                                  given to dio_world_upload_hulls (extension) */
+                                  given to dio_world_upload_hulls (extension).  A range
+                                  that is not whole, has a count below 1, or reaches
+                                  past the uploaded pool is DIO_INVALID (at upload for a
+                                  malformed range; at dio_step and the granular broad and
+                                  narrow submits for one past the pool), before any launch */
 };
 
 /* constraint kinds (avbd/src/lib.rs:76-79) */
@@ -195,7 +199,10 @@ typedef struct dio_limits {
 /* Step protocol options for the fused path. */
 typedef struct dio_step_opts {
     uint32_t contact_lag; /* 1: the reference demos' protocol (contacts found on the
-                             previous state, lever arms on the current one); 0: synchronous */
+                             previous state, lever arms on the current one); 0: synchronous.
+                             A lag-1 step after a lag-0 step builds its contacts from the
+                             narrow output that step found on the poses at its start, as
+                             if it had run with lag 1 */
     uint32_t mode;        /* 0: Jacobi sweep with ping-pong pose buffers (reference);
                              1: coloured Gauss-Seidel (no counterpart in the reference): the
                              step's body graph (contacts and joints between dynamic bodies) is

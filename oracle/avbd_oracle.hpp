@@ -1651,7 +1651,8 @@ template <class T> struct World {
 
     // One demo frame (demos/avbd_box/src/main.rs:189-272).  With lag 1 the
     // body contacts were found on the previous state; lag 0 finds them on the
-    // current one.
+    // current one.  (The reference always runs lag 1; lag 0 and switching
+    // between the two are extensions.)
     void step(const Options<T>& o, int lag = 1) {
         double t0 = now_s();
         double tb = 0, tn = 0, tc = 0;
@@ -1673,6 +1674,10 @@ template <class T> struct World {
             std::vector<RawContact<T>> raw = narrow(bodies, hull, last_pairs);
             double c = now_s();
             prepare_contacts(raw_contacts(raw, o), o);
+            // a lag-1 step after this one builds its contacts from this
+            // narrow output, as if this step had run with lag 1
+            pending = std::move(raw);
+            broad_primed = true;
             double d = now_s();
             tb += b - a; tn += c - b; tc += d - c;
         }

@@ -292,6 +292,16 @@ def test_torch_autograd_binding(oracle):
     assert_bits(vel.grad.numpy(), g["velocities"], "torch velocity grad")
     assert_bits(im.grad.numpy(), g["inv_mass"], "torch inverse-mass grad")
     assert abs(float(mu.grad) - np.float32(g["friction"])) == 0.0
+    # friction as a plain number takes no gradient; as a shape-(1,) tensor
+    # its gradient has that shape
+    vel2 = torch.from_numpy(sc.state["vel"][:, :3].copy()).requires_grad_()
+    pos2, _ = replay(ctx, sc.options, sc.state, vel2, ang.detach(), im.detach(), imo.detach(), float(sc.options.friction))
+    ((pos2[:nb].double() - target) ** 2).sum(dim=1).mean().backward()
+    assert_bits(vel2.grad.numpy(), g["velocities"], "torch velocity grad, friction a number")
+    mu1 = torch.tensor([float(sc.options.friction)], requires_grad=True)
+    pos3, _ = replay(ctx, sc.options, sc.state, vel.detach(), ang.detach(), im.detach(), imo.detach(), mu1)
+    ((pos3[:nb].double() - target) ** 2).sum(dim=1).mean().backward()
+    assert mu1.grad.shape == (1,) and float(mu1.grad[0]) == float(mu.grad)
     ctx.close()
 
 

@@ -276,6 +276,53 @@ def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort, fuse):
             _compare_world(ctx, w, 20 * (k + 1))
 
 
+def test_contact_lag_switches_bitexact(oracle):
+    """Lag 1, then lag 0, then lag 1 again on one world: after a lag-0 step the
+    next lag-1 step builds its contacts from the narrow output that step found
+    on the poses at its start, as if it had run with lag 1 (the oracle does
+    the same), and graph capture follows each switch."""
+    scene = scenes.cube_field(9, nx=8, ny=2, nz=8)
+    w = oracle.OracleWorld(scene, 32)
+    done = 0
+    with PhysicsContext(scene.n) as ctx:
+        ctx.upload(scene)
+        for lag, n in ((1, 25), (0, 7), (1, 1), (0, 1), (1, 20), (0, 10), (1, 15)):
+            ctx.step(scene.options, n, lag=lag)
+            w.step(nsteps=n, lag=lag)
+            done += n
+            _compare_world(ctx, w, done)
+            pairs = ctx.pairs()
+            assert np.array_equal(np.stack([pairs["a"], pairs["b"]], 1).reshape(-1, 2), w.pairs().reshape(-1, 2))
+
+
+def test_refit_carried_by_the_contact_build_at_depth_six(oracle):
+    """More than 32768 bodies (an 8-ary tree of depth 6, whose top levels take
+    a launch of their own after the chunk refit): with lag 1 on one stream the
+    refit steps carry the chunk refit on the contact build's first launch;
+    every step equals the oracle's, and the pairs equal those of the same run
+    with the broad phase on the second stream (a separate refit launch)."""
+    scene = scenes.cube_field(12, nx=64, ny=10, nz=64, spacing=1.6)
+    assert scene.n > 32768
+    w = oracle.OracleWorld(scene, 32)
+    runs = {}
+    for overlap in (0, 1):
+        with PhysicsContext(scene.n) as ctx:
+            ctx.set_tuning(resort_interval=4, overlap_broad=overlap)
+            ctx.upload(scene)
+            got = []
+            for k in range(3):
+                ctx.step(scene.options, 3)
+                if overlap == 0:
+                    w.step(nsteps=3)
+                    _compare_world(ctx, w, 3 * (k + 1))
+                got.append((ctx.download()["positions"].copy(), ctx.pairs().copy()))
+            runs[overlap] = got
+            _print("depth-6 field, overlap", overlap, "pairs", ctx.stats().pairs, "contacts", ctx.stats().contacts)
+    for (p0, q0), (p1, q1) in zip(runs[0], runs[1]):
+        assert_bits(p1, p0, "positions with and without the carried refit")
+        assert np.array_equal(q0, q1)
+
+
 def test_autotune_keeps_state_and_parity(oracle):
     """The autotuner times the primal shapes without changing the world."""
     scene = scenes.cube_field(11, nx=8, ny=2, nz=8)
@@ -625,6 +672,58 @@ def test_static_world_bitexact(oracle, mode):
         got = ctx.download()
         assert np.array_equal(got["positions"], scene.bodies["position"][:, :3])
         assert ctx.stats().dynamic_bodies == 0 and ctx.stats().contacts > 0
+
+
+def test_hull_ranges_are_checked_before_any_launch(oracle):
+    """A hull body names a range of the vertex pool that the kernels read
+    unchecked: a record whose range is malformed (count 0, a fractional or
+    negative first vertex) is rejected at upload, and one that reaches past
+    the uploaded pool is rejected by the step and by the granular broad and
+    narrow phases, all as DIO_INVALID before anything runs; a pool large
+    enough then makes the same world step as the oracle steps it."""
+    from diomedes_amd._lib import DioError
+    from diomedes_amd.physics import AvbdBody, Scene, Shape, gpu_bodies
+
+    ico = (scenes.icosphere(1) * 0.5).astype(np.float32)
+    opts = AvbdOptions(iterations=4)
+
+    def world(first, count, hull):
+        bodies = [AvbdBody.cube((0.0, 0.5, 0.0), 0.5, 1.0), AvbdBody(Shape.hull(0, 42), (0.1, 1.45, 0.0)),
+                  AvbdBody(Shape.hull(0, 42), (1.5, 0.6, 0.0))]
+        bodies[1].inv_mass, bodies[2].inv_mass = 1.0, 1.0
+        bodies[1].inv_moment = bodies[2].inv_moment = (10.0, 10.0, 10.0)
+        sc = Scene.from_bodies(bodies, (), scenes.floor_container(5.0), opts)
+        sc.bodies["corners"][2, 0, 0] = first
+        sc.bodies["corners"][2, 0, 1] = count
+        return Scene(sc.bodies, sc.state, container=sc.container, hull=hull, options=opts)
+
+    with PhysicsContext(8, max_hull_vertices=256) as ctx:
+        for first, count, what in ((0.0, 0.0, "count 0"), (0.5, 42.0, "fractional first"),
+                                   (-42.0, 42.0, "negative first"), (0.0, float("nan"), "NaN count")):
+            with pytest.raises(DioError) as e:
+                ctx.upload(world(first, count, ico))
+            assert e.value.status == "INVALID", what
+        # a range past the pool: accepted at upload (the pool may follow), refused by the step
+        ctx.upload(world(42.0, 42.0, ico))
+        with pytest.raises(DioError) as e:
+            ctx.step(opts, 1)
+        assert e.value.status == "INVALID" and "84 hull vertices" in str(e.value)
+        bad = world(42.0, 42.0, ico).bodies
+        with pytest.raises(DioError) as e:
+            ctx.broad_phase_submit(bad)
+        assert e.value.status == "INVALID"
+        pairs = np.zeros(1, R.GPU_PAIR)
+        pairs["a"], pairs["b"] = 0, 2
+        with pytest.raises(DioError) as e:
+            ctx.narrow_phase_run(bad, pairs)
+        assert e.value.status == "INVALID"
+        # with both hulls in the pool the same world steps as the oracle's
+        good = world(42.0, 42.0, np.concatenate([ico, ico]))
+        ctx.upload(good)
+        w = oracle.OracleWorld(good, 32)
+        ctx.step(opts, 20)
+        w.step(nsteps=20)
+        _compare_world(ctx, w, 20)
 
 
 def test_batched_table_with_empty_and_single_scenes(oracle):

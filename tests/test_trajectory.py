@@ -139,3 +139,25 @@ def test_gpu_records_the_golden_trajectory(tmp_path):
         for k in ("positions", "orientations", "velocities", "angular_velocities", "pairs"):
             assert np.array_equal(getattr(f, k), getattr(g, k)), (f.step, k)
 
+
+
+@pytest.mark.gpu
+def test_gpu_record_names_the_options_it_steps_with(tmp_path, oracle):
+    """trajectory.record with options other than the scene's writes those
+    options into the header, so the file replays: the oracle stepping the
+    file's own scene and options reproduces its frames."""
+    import dataclasses
+
+    from diomedes_amd import PhysicsContext
+
+    sc = scenes.box_stack(0)
+    opts = dataclasses.replace(sc.options, iterations=6, friction=0.3)
+    p = tmp_path / "opts.dtraj"
+    with PhysicsContext(sc.n) as ctx:
+        T.record(ctx, sc, p, 40, every=20, options=opts)
+    got = T.read(p)
+    assert bytes(got.scene.options.to_c()) == bytes(opts.to_c())
+    w = oracle.OracleWorld(got.scene, 32)
+    for f, k in zip(got.frames[1:], (20, 20)):
+        w.step(got.scene.options, nsteps=k)
+        assert np.array_equal(w.state()["positions"].astype(np.float32), f.positions)
