@@ -35,6 +35,7 @@ SIGNATURES = {
     "dio_narrow_submit": (c_int, [_VP, c_uint32]),
     "dio_narrow_submit_pairs": (c_int, [_VP, _VP, c_uint32, _VP, c_uint32]),
     "dio_narrow_read": (c_int, [_VP, _VP, c_uint32, _U32P]),
+    "dio_narrow_pairs_noalloc": (c_int, [c_int, _VP, c_uint32, _VP, c_uint32, _VP, c_uint32, _VP, c_uint32]),
     "dio_avbd_submit": (
         c_int,
         [_VP, _VP, _FP, _FP, c_uint32, _VP, c_uint32, _VP, c_uint32, _U32P, _U32P, c_uint32, POINTER(R.RunOptions)],

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/diomedes_hip.h"
+#include "abi_checks.hpp"
 #include "adjoint.hpp"
 #include "broad.hpp"
 #include "tree_refit.hpp"
@@ -375,26 +376,15 @@ template <class F> dio_status guarded(dio_ctx* ctx, F&& f) {
 [[noreturn]] void invalid(const std::string& m) { throw StatusError{DIO_INVALID, m, 0}; }
 [[noreturn]] void capacity(const std::string& m, uint64_t need) { throw StatusError{DIO_CAPACITY, m, need}; }
 
-// A hull body names its vertices as a range of the hull pool: first in
-// corners[0].x and count in corners[0].y, both whole numbers, count >= 1.
-// The kernels read hull[first .. first + count) without a check of their own,
-// so every range is checked here: a malformed record is DIO_INVALID, and the
-// return value is the pool size the bodies need (the largest first + count),
-// which the caller compares with the pool uploaded when it launches.
+// The hull pool size the bodies need (hull_pool_need); a malformed hull
+// range is DIO_INVALID.  The caller compares the size with the pool uploaded
+// when it launches.
 uint64_t hull_extent(const dio_gpu_body* bodies, uint32_t n) {
-    uint64_t need = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        if (bodies[i].shape.tag != kHull) continue;
-        const float first = bodies[i].shape.corners[0][0], count = bodies[i].shape.corners[0][1];
-        const bool whole = std::isfinite(first) && std::isfinite(count) && first >= 0.0f && count >= 1.0f &&
-                           first == std::floor(first) && count == std::floor(count) && first < 16777216.0f &&
-                           count < 16777216.0f;
-        if (!whole)
-            invalid("body " + std::to_string(i) + ": a hull's first vertex and vertex count must be whole numbers, "
-                    "the count at least 1");
-        need = std::max<uint64_t>(need, (uint64_t)first + (uint64_t)count);
-    }
-    return need;
+    const int64_t need = hull_pool_need(bodies, n);
+    if (need < 0)
+        invalid("body " + std::to_string(-1 - need) + ": a hull's first vertex and vertex count must be whole "
+                "numbers, the count at least 1");
+    return (uint64_t)need;
 }
 
 void ensure_granular_broad(dio_ctx* c) {

@@ -291,6 +291,29 @@ dio_status dio_narrow_submit_pairs(dio_ctx* ctx, const dio_gpu_body* bodies, uin
 dio_status dio_narrow_read(dio_ctx* ctx, dio_gpu_contact_out* slots, uint32_t cap_slots,
                            uint32_t* n_slots);
 
+/* No counterpart in the reference.  For callers that cannot allocate (tight
+   serving loops, code that must not call hipMalloc): dio_narrow_submit_pairs
+   followed by dio_narrow_read in one synchronous call that needs no context
+   and allocates nothing.  The same kernels give the same slots, bit for bit.
+   The caller passes host arrays: n bodies, pair_count pairs (a, b < n), the
+   hull vertex pool (x, y, z per vertex) and slots (cap_slots >= 4 * pair_count).
+   Every other buffer is one fixed device array per GPU, held from the
+   library's load until the process ends (DIO_NOALLOC_SCRATCH_BYTES, about
+   45 MiB).  Limits: DIO_CAPACITY beyond DIO_NOALLOC_MAX_BODIES bodies,
+   DIO_NOALLOC_MAX_PAIRS pairs or DIO_NOALLOC_MAX_HULL_VERTICES vertices, or
+   when cap_slots is short; DIO_INVALID for a null array, a device outside
+   0 .. 63, a pair index >= n, or a hull range that is malformed or reaches
+   past hull_count.  Calls on one device share its array: a mutex per device
+   serialises them, so calls from several threads do not overlap. */
+#define DIO_NOALLOC_MAX_BODIES 16384u
+#define DIO_NOALLOC_MAX_PAIRS 65536u
+#define DIO_NOALLOC_MAX_HULL_VERTICES 16384u
+#define DIO_NOALLOC_SCRATCH_BYTES 47453184u
+dio_status dio_narrow_pairs_noalloc(int device, const dio_gpu_body* bodies, uint32_t n,
+                                    const dio_gpu_pair* pairs, uint32_t pair_count,
+                                    const float* hull_vertices, uint32_t hull_count,
+                                    dio_gpu_contact_out* slots, uint32_t cap_slots);
+
 /* ---- AVBD solve (AvbdCompute, render/gpu_physics.rs:439-622) ---------- */
 /* Replaces AvbdCompute::submit.  positions: n x 3 floats, orientations: n x 4
    (x, y, z, w).  offsets: n + 1 CSR offsets, indices: nidx contact indices. */
