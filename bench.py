@@ -9,12 +9,20 @@ One "step" = broad phase + narrow phase + contact build + CSR + AVBD
 (init, 4 x (primal, dual), recover), fully on the device.
 
 value = N_dyn * I * K / t_AVBD, where t_AVBD is the summed GPU time of the
-AVBD segment (HIP events around it in every step).  ms_per_step is the wall
-time of the whole K-step timed region / K.  The roofline object describes the
-primal kernel (the dominant one), timed per launch with HIP events in a
-profiled pass over the same workload.  cpu_baseline is the FP32 oracle (the
-C++ restatement of the reference algorithms, OpenMP over the host cores)
-stepping the same scene for the same steps on the host.
+AVBD segment (HIP events around it in every step), over the K steps after W
+warm-up steps of the drop.  ms_per_step is the wall time of the whole K-step
+timed region / K.
+
+cube_field_300 runs the workload as specified, 300 steps from the drop, on
+the GPU and, over the same steps, the CPU baseline: the FP32 oracle (the C++
+restatement of the reference algorithms) on every host core (median of 3
+runs) and on one thread over the settled window 250..260 (median of 3).
+The roofline object describes the primal kernel (the dominant one) over one
+window, the 20 steps after those 300 (the settled pile): algorithmic bytes
+per launch from that window's own contact counts, the average launch time
+from HIP events around each launch, and the HBM traffic per launch from the
+committed counter runs of the same command (bench.py --only-leg roofline).
+cpu_baseline is the FP32 oracle over the headline's own K steps (median of 3).
 """
 
 from __future__ import annotations
@@ -55,15 +63,20 @@ def primal_bytes(n_dyn: int, m: float, k: float, iters: int) -> float:
 PROFILE_MD = ROOT / "profiles" / "cube_field_8192_kernels.md"
 
 
+ROOFLINE_WINDOW = "cube_field_8192_s0_steps_300_320"
+
+
 def measured_traffic(kernel: str) -> dict | None:
     """HBM bytes per primal launch from the committed rocprofv3 counter runs
     (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy kernel), when they were
-    taken on the kernel this run uses."""
+    taken on the kernel this run uses and on the roofline's own window (the
+    settled steps 300..320 of `bench.py --only-leg roofline`)."""
     try:
         for line in PROFILE_MD.read_text().splitlines():
             if line.startswith("primal_traffic_json:"):
                 t = json.loads(line.split(":", 1)[1])
-                return t if t.get("kernel") == kernel else None
+                ok = t.get("kernel") == kernel and t.get("window_id") == ROOFLINE_WINDOW
+                return t if ok else None
     except (OSError, ValueError):
         return None
     return None
@@ -74,7 +87,11 @@ def batched_leg(args, tuning: dict) -> dict:
     same launches (one world with a scene table, the step captured in HIP
     graphs), against a sample of the same scenes stepped one plain context
     each, one after the other.  Both report AVBD body-iters/s (HIP events
-    around the AVBD segment) and full-step body-iters/s (wall clock)."""
+    around the AVBD segment) and full-step body-iters/s (wall clock).  With
+    autotuning (the default) each world runs dio_autotune on its live state
+    after the warm-up, and the table's run is also timed with the library's
+    default launch shapes; the table then continues to step 300, the
+    workload's length, timed as one more window."""
     from diomedes_amd import PhysicsContext, scenes
 
     batch = scenes.batched_scenes(args.seed, args.batch_scenes)
@@ -82,17 +99,33 @@ def batched_leg(args, tuning: dict) -> dict:
     iters = opts.iterations
     k = args.steps
     n_dyn = batch.n_dynamic
-    with PhysicsContext(batch.n, max_pairs=8 * batch.n, max_contacts=16 * batch.n,
-                        max_hull_vertices=len(batch.hull)) as ctx:
-        ctx.set_tuning(**tuning)
-        ctx.batch_upload(batch)
-        if args.warmup:
-            ctx.step(opts, args.warmup)
-        ctx.step(opts, 0)
-        t0 = time.perf_counter()
-        ctx.step(opts, k)
-        wall = time.perf_counter() - t0
-        st = ctx.stats()
+    tune = not args.no_tune
+
+    def run(autotune: bool):
+        with PhysicsContext(batch.n, max_pairs=8 * batch.n, max_contacts=16 * batch.n,
+                            max_hull_vertices=len(batch.hull)) as ctx:
+            ctx.set_tuning(**tuning)
+            ctx.batch_upload(batch)
+            ctx.step(opts, max(args.warmup, 1))
+            chosen = ctx.autotune(opts, reps=10) if autotune else None
+            ctx.step(opts, 0)
+            t0 = time.perf_counter()
+            ctx.step(opts, k)
+            wall = time.perf_counter() - t0
+            st = ctx.stats()
+            rest = None
+            left = FULL_STEPS - max(args.warmup, 1) - k
+            if autotune and left > 0:
+                t1 = time.perf_counter()
+                ctx.step(opts, left)
+                rw = time.perf_counter() - t1
+                rs = ctx.stats()
+                rest = {"steps": f"{FULL_STEPS - left}..{FULL_STEPS}", "value": n_dyn * iters * left / (rs.ms_avbd / 1000.0),
+                        "ms_per_step": 1000.0 * rw / left, "ms_per_step_avbd": rs.ms_avbd / left,
+                        "last_step": {"pairs": rs.pairs, "contacts": rs.contacts, "csr_nnz": rs.csr_nnz}}
+            return wall, st, ctx.tuning(), chosen, rest
+
+    wall, st, used, chosen, rest = run(tune)
     _progress(f"batched leg: {len(batch.scenes)} scenes, {k} steps in {wall:.2f} s")
     out = {
         "workload": f"{len(batch.scenes)} scenes, sizes round(exp(U(ln 8, ln 512))), unit cubes and r 0.5 "
@@ -104,7 +137,15 @@ def batched_leg(args, tuning: dict) -> dict:
         "ms_per_step_avbd": st.ms_avbd / k,
         "body_iters_per_s_full_step": n_dyn * iters * k / wall,
         "last_step": {"pairs": st.pairs, "contacts": st.contacts, "csr_nnz": st.csr_nnz},
+        "tuning": used,
     }
+    if chosen:
+        out["autotune_us"] = {k2: chosen[k2] for k2 in ("us_per_launch", "dual_schedule_us", "primal_order_us")}
+        dwall, dst, _, _, _ = run(False)
+        out["library_defaults"] = {"value": n_dyn * iters * k / (dst.ms_avbd / 1000.0),
+                                   "ms_per_step": 1000.0 * dwall / k, "ms_per_step_avbd": dst.ms_avbd / k}
+    if rest:
+        out["to_step_300"] = rest
     if args.plain_sample <= 0:
         return out
     # plain: a sample of the same scenes, each in its own context, stepped in turn
@@ -118,8 +159,9 @@ def batched_leg(args, tuning: dict) -> dict:
         with PhysicsContext(sc.n, max_hull_vertices=len(sc.hull)) as one:
             one.set_tuning(**tuning)
             one.upload(sc)
-            if args.warmup:
-                one.step(opts, args.warmup)
+            one.step(opts, max(args.warmup, 1))
+            if tune:
+                one.autotune(opts, reps=10)
             one.step(opts, 0)
             t0 = time.perf_counter()
             one.step(opts, k)
@@ -131,7 +173,8 @@ def batched_leg(args, tuning: dict) -> dict:
     plain_full = bi / t_wall
     out["plain"] = {
         "sample": f"every {step_}th scene of the table ({len(sample)} scenes, "
-                  f"{sum(batch.scenes[i].n for i in sample)} bodies), one context each, {k} steps each",
+                  f"{sum(batch.scenes[i].n for i in sample)} bodies), one context each, {k} steps each"
+                  f"{', each autotuned after its warm-up' if tune else ''}",
         "value": plain_avbd,
         "body_iters_per_s_full_step": plain_full,
     }
@@ -214,8 +257,10 @@ def mixed_leg(args, tuning: dict) -> dict:
     with PhysicsContext(scene.n, max_hull_vertices=len(scene.hull), max_constraints=len(scene.constraints)) as ctx:
         ctx.set_tuning(**tuning)
         ctx.upload(scene)
-        if args.warmup:
-            ctx.step(opts, args.warmup)
+        ctx.step(opts, max(args.warmup, 1))
+        if not args.no_tune:
+            ctx.autotune(opts, reps=10)
+        used = ctx.tuning()
         ctx.step(opts, 0)
         t0 = time.perf_counter()
         ctx.step(opts, k)
@@ -231,6 +276,7 @@ def mixed_leg(args, tuning: dict) -> dict:
         "ms_per_step": 1000.0 * wall / k,
         "ms_per_step_avbd": st.ms_avbd / k,
         "last_step": {"pairs": st.pairs, "contacts": st.contacts, "narrow_contacts": st.narrow_contacts},
+        "tuning": used,
     }
 
 
@@ -405,46 +451,208 @@ def _progress(msg: str) -> None:
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, warmup: int, steps: int, budget_s: float = 25.0, mode: int = 0) -> dict:
-    """FP32 oracle on the host cores over the same warm-up and steps (mode 1:
-    its coloured Gauss-Seidel solve)."""
+FULL_STEPS = 300  # the plain cube field's length (300 steps from the drop)
+
+
+def cube_field_gpu(args, tuning: dict, profile_steps: int) -> dict:
+    """The plain cube field as its benchmark specifies it: 300 steps from the
+    drop (graph-captured), timed in the windows 0..250, 250..260 and
+    260..300; then `profile_steps` more steps of the settled pile launched
+    one kernel at a time, with HIP events around each AVBD launch, which give
+    the per-kernel averages and the roofline of the primal over that one
+    window."""
+    from diomedes_amd import PhysicsContext, scenes
+
+    scene = scenes.cube_field(args.seed)
+    opts = scene.options
+    n_dyn, iters = scene.n_dynamic, opts.iterations
+    windows = []
+    with PhysicsContext(scene.n) as ctx:
+        ctx.set_tuning(**tuning)
+        ctx.upload(scene)
+        ctx.step(opts, 0)
+        done = 0
+        for k in (250, 10, FULL_STEPS - 260):
+            t0 = time.perf_counter()
+            ctx.step(opts, k)
+            wall = time.perf_counter() - t0
+            st = ctx.stats()
+            windows.append({"steps": (done, done + k), "wall_s": wall, "ms_avbd": st.ms_avbd,
+                            "sum_contacts": st.sum_contacts, "sum_csr_nnz": st.sum_csr_nnz,
+                            "last": {"pairs": st.pairs, "contacts": st.contacts, "csr_nnz": st.csr_nnz}})
+            done += k
+        _progress(f"cube field: {FULL_STEPS} steps from the drop in {sum(w['wall_s'] for w in windows):.3f} s")
+        ctx.step(opts, profile_steps, profile=True)
+        pst = ctx.stats()
+    ms_avbd = sum(w["ms_avbd"] for w in windows)
+    wall = sum(w["wall_s"] for w in windows)
+    return {"windows": windows, "profile": pst, "value": n_dyn * iters * FULL_STEPS / (ms_avbd / 1000.0),
+            "ms_per_step": 1000.0 * wall / FULL_STEPS, "ms_per_step_avbd": ms_avbd / FULL_STEPS,
+            "settled_value": n_dyn * iters * 10 / (windows[1]["ms_avbd"] / 1000.0),
+            "last_step": windows[-1]["last"], "n_dyn": n_dyn, "iters": iters, "n": scene.n,
+            "profile_window": (FULL_STEPS, FULL_STEPS + profile_steps)}
+
+
+def roofline_of(g: dict, tuning: dict) -> dict:
+    """The primal's roofline over the settled profile window of
+    cube_field_gpu: algorithmic bytes per launch (primal_bytes with that
+    window's own contact and CSR counts) over the launches' average time from
+    HIP events; traffic = measured HBM bytes per launch from the committed
+    counter runs of the same window and command, when they exist."""
+    pst = g["profile"]
+    lanes = tuning["primal_lanes"]
+    kname = (f"k_avbd_primal_grouped<{lanes}, {'*' if tuning['fuse_dual'] else 'false'}>" if lanes > 1
+             else f"k_avbd_primal<{'*' if tuning['fuse_dual'] else 'false'}>")
+    p_launches = max(pst.n_primal, 1)
+    primal_ms = pst.ms_primal / p_launches
+    pm = pst.sum_contacts / max(pst.steps, 1)
+    pk = pst.sum_csr_nnz / max(pst.steps, 1)
+    pbytes = primal_bytes(g["n_dyn"], pm, pk, g["iters"])
+    achieved = pbytes / (primal_ms / 1000.0) / 1e9
+    traffic = measured_traffic(kname)
+    a, b = g["profile_window"]
+    return {
+        "kernel": kname,
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic["traffic_bytes"] if traffic else None,
+        "traffic_over_algorithmic": traffic["traffic_bytes"] / pbytes if traffic else None,
+        "traffic_source": (f"{PROFILE_MD.relative_to(ROOT)}: rocprofv3 FETCH_SIZE and WRITE_SIZE per launch, "
+                           f"{traffic['window']}") if traffic else None,
+        "window": f"steps {a}..{b} of the 8192-box field (the settled pile, after the 300 steps from the drop); "
+                  f"per step M = {pm:.0f} contacts, K = {pk:.0f} CSR entries",
+        "bytes_per_unit": "136 B per dynamic body + 8 B per CSR entry + 100 B per contact (+ 24 B per contact "
+                          "in the launches carrying the fused dual update, 3 of 4)",
+        "avg_launch_us": primal_ms * 1000.0,
+        "bytes_per_launch": pbytes,
+        "launches_timed": pst.n_primal,
+    }
+
+
+def cube_field_cpu(args, g: dict) -> dict:
+    """The CPU side of the plain cube field's comparison, over the same
+    steps: the FP32 oracle on every host core for the 300 steps from the
+    drop (median of 3 runs), and on one thread over the settled window
+    250..260 of the same trajectory (copies of an all-core run's world at
+    step 250, median of 3)."""
+    from diomedes_amd import scenes
     from oracle import pyoracle
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    scene = scenes.cube_field(args.seed)
+    n_dyn, iters = scene.n_dynamic, scene.options.iterations
+    allc = cpu_baseline(scene, 0, FULL_STEPS, budget_s=600.0, runs=3)
+    cores = allc["cores"]
     pyoracle.threads(cores)
     w = pyoracle.OracleWorld(scene, 32)
-    w.set_mode(mode)
-    w.step(nsteps=warmup)
+    w.step(nsteps=250)
+    pyoracle.threads(1)
+    rates, walls = [], []
+    for _ in range(3):
+        c = w.clone()
+        t = _time_oracle(c, 10, 600.0)
+        rates.append(n_dyn * iters * t["steps"] / t["avbd"])
+        walls.append(t["wall"] / t["steps"])
+        c.close()
+    w.close()
+    pyoracle.threads(cores)
+    med = sorted(rates)[1]
+    one = {"value": med, "unit": "AVBD body-iters/s", "cores": 1, "kind": "port",
+           "sample": "steps 250..260 of the drop (the settled pile): copies of the all-core run's world at step 250, "
+                     "one OpenMP thread, median of 3", "runs": rates, "ms_per_step": 1000.0 * sorted(walls)[1]}
+    _progress(f"cube field CPU: all cores {allc['value']:.3e}, one thread {med:.3e} body-iters/s")
+    return {
+        "workload": "plain cube field 32x32x8 = 8192 boxes, I=4, dt=1/60, friction 0.6, contact lag 1: the 300 "
+                    "steps from the drop, GPU and CPU over the same steps",
+        "gpu": {"value": g["value"], "unit": "body-iters/s", "ms_per_step": g["ms_per_step"],
+                "ms_per_step_avbd": g["ms_per_step_avbd"], "settled_window_value": g["settled_value"],
+                "last_step": g["last_step"]},
+        "cpu_all_cores": allc,
+        "cpu_one_thread": one,
+        "gpu_over_cpu_all_cores": g["value"] / allc["value"],
+        "gpu_over_cpu_one_thread_settled": g["settled_value"] / med,
+        "gpu_over_cpu_full_step": allc["ms_per_step"] / g["ms_per_step"],
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "the host CPU"
+
+
+def _host_cores() -> int:
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def _time_oracle(w, steps: int, budget_s: float) -> dict:
+    """Step an oracle world up to `steps` times (stopping once budget_s has
+    passed) and return its per-stage times."""
     w.reset_times()
     done = 0
     t0 = time.perf_counter()
     while done < steps and time.perf_counter() - t0 < budget_s:
         w.step(nsteps=1)
         done += 1
-    wall = time.perf_counter() - t0
     t = w.times()
+    t["wall"] = time.perf_counter() - t0
+    t["steps"] = done
+    return t
+
+
+def cpu_baseline(scene, warmup: int, steps: int, budget_s: float = 25.0, mode: int = 0, runs: int = 3,
+                 threads: int = 0) -> dict:
+    """FP32 oracle on the host cores over the same warm-up and steps (mode 1:
+    its coloured Gauss-Seidel solve); the median of `runs` runs, each from
+    the upload.  threads 0: every core the process may use."""
+    from oracle import pyoracle
+
+    cores = threads or _host_cores()
+    pyoracle.threads(cores)
     n_dyn = scene.n_dynamic
     iters = scene.options.iterations
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    results = []
+    for _ in range(runs):
+        w = pyoracle.OracleWorld(scene, 32)
+        w.set_mode(mode)
+        w.step(nsteps=warmup)
+        results.append(_time_oracle(w, steps, budget_s))
+        w.close()
+    rates = [n_dyn * iters * t["steps"] / t["avbd"] if t["avbd"] > 0 else 0.0 for t in results]
+    med = sorted(range(runs), key=lambda i: rates[i])[runs // 2]
+    t = results[med]
+    done = t["steps"]
     return {
-        "value": n_dyn * iters * done / t["avbd"] if t["avbd"] > 0 else None,
+        "value": rates[med] or None,
         "unit": "AVBD body-iters/s",
         "cores": cores,
         "kind": "port",
         "sample": f"{scene.name}: FP32 oracle, steps {warmup}..{warmup + done} after {warmup} warm-up steps, "
-                  f"{cores} OpenMP threads on {cpu_model or 'the host CPU'}",
-        "ms_per_step": 1000.0 * wall / max(done, 1),
+                  f"{cores} OpenMP thread{'s' if cores > 1 else ''} on {_cpu_model()}, median of {runs} runs",
+        "runs": rates,
+        "ms_per_step": 1000.0 * t["wall"] / max(done, 1),
         "ms_per_step_avbd": 1000.0 * t["avbd"] / max(done, 1),
         "ms_per_step_broad": 1000.0 * t["broad"] / max(done, 1),
         "steps_timed": done,
     }
+
+
+def _roofline_only(args) -> dict:
+    """The settled-window pass alone (the command the committed counter runs
+    profile): 300 steps from the drop, then the per-kernel pass."""
+    from diomedes_amd import PhysicsContext
+
+    with PhysicsContext(8192) as c:
+        tuning = c.tuning()
+    g = cube_field_gpu(args, default_tuning, args.profile_steps)
+    return {"roofline": roofline_of(g, tuning), "cube_field_300_gpu": {k: g[k] for k in ("value", "ms_per_step",
+                                                                                        "ms_per_step_avbd")}}
 
 
 def main() -> int:
@@ -459,14 +667,16 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="scenes of the batch stepped one context each for the plain comparison")
     ap.add_argument("--profile-steps", type=int, default=20, help="steps of the per-kernel timing pass")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default launch shapes (no dio_autotune on the live worlds)")
     ap.add_argument("--tune-steps", type=int, default=0,
                     help="steps of a separate run of the same scene after which dio_autotune picks the launch "
                          "shapes and dual schedule (0: library defaults)")
     ap.add_argument("--no-coloured", action="store_true", help="skip the coloured Gauss-Seidel leg")
     ap.add_argument("--no-overlap", action="store_true", help="skip the overlapped broad-phase leg")
-    ap.add_argument("--mixed-steps", type=int, default=300, help="steps of the mixed-field leg (0: skip it)")
+    ap.add_argument("--mixed-steps", type=int, default=600, help="steps of the mixed-field leg (0: skip it)")
     ap.add_argument("--joint-steps", type=int, default=120, help="steps of the hard-joints leg (0: skip it)")
-    ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint", "mixed", "joints", "overlap"),
+    ap.add_argument("--only-leg", choices=("coloured", "batched", "adjoint", "mixed", "joints", "overlap", "roofline"),
                     default=None,
                     help="run only this leg and print its object (for profiling it alone)")
     ap.add_argument("--adjoint-seeds", type=int, default=64,
@@ -489,7 +699,8 @@ def main() -> int:
     if args.only_leg:
         legs = {"coloured": lambda: coloured_leg(args, {}), "batched": lambda: batched_leg(args, {}),
                 "adjoint": lambda: adjoint_leg(args), "mixed": lambda: mixed_leg(args, {}),
-                "joints": lambda: joints_leg(args, {}), "overlap": lambda: overlap_leg(args, {})}
+                "joints": lambda: joints_leg(args, {}), "overlap": lambda: overlap_leg(args, {}),
+                "roofline": lambda: _roofline_only(args)}
         print(json.dumps({"leg": args.only_leg, **legs[args.only_leg]()}), flush=True)
         return 0
 
@@ -517,14 +728,20 @@ def main() -> int:
             tctx.autotune(opts, reps=20)
             tuning = tctx.tuning()
 
+    chosen = None
     with PhysicsContext(n) as ctx:
         if tuning:
             ctx.set_tuning(**tuning)
         tuning = ctx.tuning()
+        default_tuning = dict(tuning)
         ctx.upload(scene)
         if args.warmup:
             ctx.step(opts, args.warmup)
         _progress(f"warm-up done ({args.warmup} steps)")
+        if not args.no_tune and args.warmup and not args.tune_steps:
+            # the launch shapes for the timed window, chosen on the live world
+            chosen = ctx.autotune(opts, reps=20)
+            tuning = ctx.tuning()
         ctx.step(opts, 0)  # synchronise (a zero-step call syncs the stream)
         t0 = time.perf_counter()
         ctx.step(opts, args.steps)  # returns after the stream is synchronised
@@ -534,29 +751,16 @@ def main() -> int:
         ms_avbd = st.ms_avbd
         sum_m, sum_k, sum_p = st.sum_contacts, st.sum_csr_nnz, st.sum_pairs
         last = {"pairs": st.pairs, "contacts": st.contacts, "csr_nnz": st.csr_nnz}
-        # per-kernel timing pass (eager launches, HIP events around each AVBD kernel)
-        ctx.step(opts, args.profile_steps, profile=True)
-        _progress(f"profile pass done ({args.profile_steps} steps)")
-        pst = ctx.stats()
 
     k = args.steps
     value = n_dyn * iters * k / (ms_avbd / 1000.0)
     avg_m = sum_m / k
     avg_k = sum_k / k
-    # roofline of the primal kernel
-    p_launches = max(pst.n_primal, 1)
-    primal_ms = pst.ms_primal / p_launches
-    pm = pst.sum_contacts / max(pst.steps, 1)
-    pk = pst.sum_csr_nnz / max(pst.steps, 1)
-    pbytes = primal_bytes(n_dyn, pm, pk, iters)
-    achieved = pbytes / (primal_ms / 1000.0) / 1e9
     step_bytes = avbd_bytes(n_dyn, n, iters, avg_m, avg_k)
-    lanes = tuning["primal_lanes"]
-    # the step's primal launches: with the fused schedule the first of a step is
-    # <G, false> and the others <G, true>; the roofline averages over all of them
-    kname = (f"k_avbd_primal_grouped<{lanes}, {'*' if tuning['fuse_dual'] else 'false'}>" if lanes > 1
-             else f"k_avbd_primal<{'*' if tuning['fuse_dual'] else 'false'}>")
-    traffic = measured_traffic(kname)
+    # the workload's full length from the drop, with the per-kernel pass of
+    # the settled pile that gives the roofline
+    g = cube_field_gpu(args, tuning, args.profile_steps)
+    pst = g["profile"]
     line = {
         "metric": "AVBD body-iters/s & ms/step, 8k boxes",
         "value": value,
@@ -576,8 +780,13 @@ def main() -> int:
             "iterations": iters,
             "seed": args.seed,
             "mode": "jacobi (reference parity)",
+            "window": f"steps {args.warmup}..{args.warmup + k} of the drop",
         },
         "tuning": tuning,
+        "tuning_source": ("dio_autotune on the live world after the warm-up" if chosen else
+                          "library defaults" if not args.tune_steps else f"dio_autotune after {args.tune_steps} steps"),
+        "autotune_us": ({k2: chosen[k2] for k2 in ("us_per_launch", "broad_us_per_pass", "dual_schedule_us",
+                                                  "primal_order_us")} if chosen else None),
         "ms_per_step_avbd": ms_avbd / k,
         "body_iters_per_s_full_step": n_dyn * iters * k / wall,
         "avg_contacts": avg_m,
@@ -585,36 +794,24 @@ def main() -> int:
         "avg_pairs": sum_p / k,
         "last_step": last,
         "avbd_segment_gbs": step_bytes * k / (ms_avbd / 1000.0) / 1e9,
-        "roofline": {
-            "kernel": kname,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic["traffic_bytes"] if traffic else None,
-            "traffic_source": (f"{PROFILE_MD.relative_to(ROOT)}: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, "
-                               f"{traffic['window']}") if traffic else None,
-            "avg_launch_us": primal_ms * 1000.0,
-            "bytes_per_launch": pbytes,
-            "launches_timed": pst.n_primal,
-        },
+        "roofline": roofline_of(g, default_tuning),
         "kernel_avg_us": {
+            "window": "the settled profile window of the roofline",
             "init": 1000.0 * pst.ms_init / max(pst.n_init, 1),
-            "primal": 1000.0 * primal_ms,
+            "primal": 1000.0 * pst.ms_primal / max(pst.n_primal, 1),
             "dual_recover": 1000.0 * pst.ms_recover / max(pst.n_recover, 1),
         },
     }
     if not args.no_overlap:
-        line["overlap"] = overlap_leg(args, tuning)
+        line["overlap"] = overlap_leg(args, default_tuning)
     if not args.no_coloured:
-        line["coloured"] = coloured_leg(args, tuning)
+        line["coloured"] = coloured_leg(args, default_tuning)
     if args.mixed_steps:
-        line["mixed"] = mixed_leg(args, tuning)
+        line["mixed"] = mixed_leg(args, default_tuning)
     if args.joint_steps:
-        line["joints"] = joints_leg(args, tuning)
+        line["joints"] = joints_leg(args, default_tuning)
     if args.batch_scenes:
-        line["batched"] = batched_leg(args, tuning)
+        line["batched"] = batched_leg(args, default_tuning)
     if args.adjoint_seeds:
         line["adjoint"] = adjoint_leg(args)
     if not args.no_cpu:
@@ -622,11 +819,18 @@ def main() -> int:
         line["cpu_baseline"] = cb
         if cb.get("value"):
             line["gpu_over_cpu"] = value / cb["value"]
+        line["cube_field_300"] = cube_field_cpu(args, g)
         if args.joint_steps:
-            jc = cpu_baseline(scenes.hard_chains(args.seed), args.warmup, args.joint_steps, budget_s=10.0, mode=1)
+            jc = cpu_baseline(scenes.hard_chains(args.seed), args.warmup, args.joint_steps, budget_s=4.0, mode=1)
+            jc["note"] = ("the oracle gathers each body's joint rows from a per-body list in ascending constraint "
+                          "order, the rows and order of the reference's scan of every constraint per body "
+                          "(avbd/src/lib.rs:422-539) in O(N + C) instead of O(N C): faster than the reference")
             line["joints"]["cpu_baseline"] = jc
             if jc.get("value"):
                 line["joints"]["gpu_over_cpu"] = line["joints"]["value"] / jc["value"]
+    else:
+        line["cube_field_300"] = {"gpu": {"value": g["value"], "ms_per_step": g["ms_per_step"],
+                                          "ms_per_step_avbd": g["ms_per_step_avbd"]}}
     print(json.dumps(line), flush=True)
     return 0
 

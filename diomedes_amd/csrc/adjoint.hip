@@ -154,6 +154,7 @@ SolveView step_view(const AdjRecord& r, const AdjHostIndex& h, const AdjWork& w,
     v.c_ra = r.c.ra + c0;
     v.c_rb = r.c.rb + c0;
     v.c_c0 = r.c.c0 + c0;
+    v.derived_frame = 1;  // recorded from the world's contact build (ra.w holds c0)
     v.m_dev = nullptr;
     v.m_host = h.m[s];
     v.offsets = r.csr_off + (size_t)s * (r.n_cap + 1);

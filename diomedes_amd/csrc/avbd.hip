@@ -140,6 +140,30 @@ __device__ __forceinline__ Force3 contact_force(float4 pen, float4 lam, float fr
     return r;
 }
 
+// A contact's tangent frame and c0 (physics/gpu.rs:239-264,352-404).  For
+// contacts the device built (v.derived_frame) they are recomputed instead of
+// read: the frame is tangents() of the narrow normal, which the build stored
+// negated, or for a wall contact of the stored normal itself; c0 is
+// (margin - depth, 0, 0) with its first component in ra.w.  Both come out
+// bit-identical to the stored values.
+struct ContactFrame {
+    f3 t1, t2;
+    float4 c0;
+};
+__device__ __forceinline__ ContactFrame contact_frame(const SolveView& v, uint32_t ci, f3 n0, float4 ra4,
+                                                      bool wall) {
+    ContactFrame fr;
+    if (v.derived_frame) {
+        tangents(wall ? n0 : neg(n0), fr.t1, fr.t2);
+        fr.c0 = make_float4(ra4.w, 0.0f, 0.0f, 0.0f);
+    } else {
+        fr.t1 = xyz(v.c_t1[ci]);
+        fr.t2 = xyz(v.c_t2[ci]);
+        fr.c0 = v.c_c0[ci];
+    }
+    return fr;
+}
+
 // avbd_dual (avbd/src/lib.rs:565-663) for one contact: the constraint value
 // from both bodies' displacements and world lever arms (the b terms are zero
 // for a container contact), the clamped force as the new lambda, and the
@@ -248,6 +272,18 @@ __device__ __forceinline__ uint32_t body_at(const SolveView& v, const AvbdParams
     return k < v.list_count ? v.body_list[k] : p.n;
 }
 __device__ __forceinline__ float4* pen_buf(const SolveView& v, uint32_t j) { return (j & 1u) ? v.c_pen_alt : v.c_pen; }
+// The chunk of work items workgroup b takes.  Workgroups are handed to the 8
+// XCDs in turn (b % 8), each XCD with an L2 of its own; this mapping gives
+// XCD x one contiguous range of chunks, so bodies that are neighbours in the
+// work order (the Morton order of the broad phase) share an L2, and with it
+// the contact records both bodies of a contact read and each other's poses.
+// It is a bijection of [0, nb).
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t per = nb / kXcds, extra = nb % kXcds;
+    const uint32_t x = b % kXcds, local = b / kXcds;
+    return x < extra ? x * (per + 1) + local : extra * (per + 1) + (x - extra) * per + local;
+}
 
 // avbd_init of one body (avbd/src/lib.rs:202-254) from its pose and
 // velocities, into target t
@@ -309,7 +345,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
     float4* lam_out = lam_buf(v, iter);
     float4* pen_out = pen_buf(v, iter);
     const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
-    const uint32_t i = body_at(v, p, blockIdx.x * kBlock + threadIdx.x);
+    const uint32_t i = body_at(v, p, xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
     if (i >= p.n) return;
     const float4 imm = v.inv_mm[i];
     if (imm.w == 0.0f) return;
@@ -349,10 +385,11 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
         const uint32_t other = pc == kContainer ? kContainer : (pc & ~kPeerIsB);
         const float4 n4 = v.c_n[ci];
         const f3 n0 = xyz(n4);
-        const f3 t1 = xyz(v.c_t1[ci]);
-        const f3 t2 = xyz(v.c_t2[ci]);
-        const f3 ra = xyz(v.c_ra[ci]);
-        const f3 rb = xyz(v.c_rb[ci]);
+        const float4 ra4 = v.c_ra[ci];
+        const f3 ra = xyz(ra4);
+        const f3 rb = other != kContainer ? xyz(v.c_rb[ci]) : zero;  // a wall contact's r_b is zero
+        const ContactFrame cf = contact_frame(v, ci, n0, ra4, other == kContainer);
+        const f3 t1 = cf.t1, t2 = cf.t2;
         const f3 r0 = is_a ? n0 : neg(n0);
         const f3 r1 = is_a ? t1 : neg(t1);
         const f3 r2 = is_a ? t2 : neg(t2);
@@ -364,7 +401,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
             doa = ang_diff(orr, as_q(v.init_ang[other]));
             r_other = qrot(orr, is_a ? rb : ra);
         }
-        const float4 c04 = v.c_c0[ci];
+        const float4 c04 = cf.c0;
         float4 pen = pen_in[ci];
         float4 lam = lam_in[ci];
         if (fuse) {
@@ -531,7 +568,8 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     __shared__ float4 body_f[kBodies][2];    // snapshot pos (.w: active), rot: for the solving thread
     const int slot = threadIdx.x / G;
     const int lane = threadIdx.x % G;
-    const uint32_t i = body_at(v, p, blockIdx.x * kBodies + slot);
+    const uint32_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const uint32_t i = body_at(v, p, chunk * kBodies + slot);
     const float4* __restrict__ rp = v.pos[parity];
     const float4* __restrict__ rr = v.rot[parity];
     const float dt = p.dt;
@@ -628,10 +666,11 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
                 const uint32_t other = pc == kContainer ? kContainer : (pc & ~kPeerIsB);
                 const float4 n4 = v.c_n[ci];
                 const f3 n0 = xyz(n4);
-                const f3 t1 = xyz(v.c_t1[ci]);
-                const f3 t2 = xyz(v.c_t2[ci]);
-                const f3 ra = xyz(v.c_ra[ci]);
-                const f3 rb = xyz(v.c_rb[ci]);
+                const float4 ra4 = v.c_ra[ci];
+                const f3 ra = xyz(ra4);
+                const f3 rb = other != kContainer ? xyz(v.c_rb[ci]) : zero;  // a wall contact's r_b is zero
+                const ContactFrame cf = contact_frame(v, ci, n0, ra4, other == kContainer);
+                const f3 t1 = cf.t1, t2 = cf.t2;
                 const f3 r0 = is_a ? n0 : neg(n0);
                 const f3 r1 = is_a ? t1 : neg(t1);
                 const f3 r2 = is_a ? t2 : neg(t2);
@@ -643,7 +682,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
                     doa = ang_diff(orr, as_q(v.init_ang[other]));
                     r_other = qrot(orr, is_a ? rb : ra);
                 }
-                const float4 c04 = v.c_c0[ci];
+                const float4 c04 = cf.c0;
                 float4 pen = pen_in[ci];
                 float4 lam = lam_in[ci];
                 if (fuse) {
@@ -830,7 +869,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     const int t = threadIdx.x;
     const float4 f0 = body_f[t][0];
     if (f0.w == 0.0f) return;
-    const uint32_t bi = body_at(v, p, blockIdx.x * kBodies + t);
+    const uint32_t bi = body_at(v, p, chunk * kBodies + t);
     const float* gs = gathered[t];
     Block6 B;
 #pragma unroll
@@ -870,10 +909,13 @@ __device__ __forceinline__ void dual_one(const SolveView& v, const AvbdParams& p
         dba = ang_diff(rot_b, as_q(v.init_ang[ab.y]));
         rbw = qrot(rot_b, xyz(v.c_rb[ci]));
     }
-    const f3 raw = qrot(rot_a, xyz(v.c_ra[ci]));
+    const float4 ra4 = v.c_ra[ci];
+    const f3 raw = qrot(rot_a, xyz(ra4));
     const float4 n4 = v.c_n[ci];
-    const f3 r0 = xyz(n4), r1 = xyz(v.c_t1[ci]), r2 = xyz(v.c_t2[ci]);
-    const float4 c04 = v.c_c0[ci];
+    const f3 r0 = xyz(n4);
+    const ContactFrame cf = contact_frame(v, ci, r0, ra4, ab.y == kContainer);
+    const f3 r1 = cf.t1, r2 = cf.t2;
+    const float4 c04 = cf.c0;
     uint32_t reps = 1, from = src;
     if (static_updates > 0u) {
         const bool moving = v.inv_mm[ab.x].w != 0.0f || (ab.y != kContainer && v.inv_mm[ab.y].w != 0.0f);

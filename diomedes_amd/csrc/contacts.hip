@@ -33,7 +33,7 @@ __device__ __forceinline__ void put_contact(const ContactArrays& c, uint32_t i, 
     c.n[i] = f4(n, friction);
     c.t1[i] = f4(t1, 0.0f);
     c.t2[i] = f4(t2, 0.0f);
-    c.ra[i] = f4(ra, 0.0f);
+    c.ra[i] = f4(ra, c0x);  // c0 rides in ra.w for the solve
     c.rb[i] = f4(rb, 0.0f);
     c.c0[i] = make_float4(c0x, 0.0f, 0.0f, 0.0f);
     c.lam[i] = lam;

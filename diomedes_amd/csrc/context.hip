@@ -239,7 +239,7 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{32, 0, 16, 1, 8, 0, {0, 0}};
+    dio_tuning tuning{32, 0, 16, 1, 8, 0, 0, 0};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -1222,6 +1222,7 @@ SolveView world_solve_view(dio_ctx* c, int s, bool prepare_next) {
         const ContactSet& cs = c->cs[s];
         v.c_ab = cs.c.ab; v.c_n = cs.c.n; v.c_t1 = cs.c.t1; v.c_t2 = cs.c.t2; v.c_ra = cs.c.ra; v.c_rb = cs.c.rb;
         v.c_c0 = cs.c.c0; v.c_pen = cs.c.pen; v.c_lam = cs.c.lam;
+        v.derived_frame = 1;  // the world's contacts come from the device's contact build
         v.c_lam_alt = c->w_lam_alt; v.c_pen_alt = c->w_pen_alt;
         v.m_dev = cs.m_dev;
         v.m_host = 0;
@@ -1239,6 +1240,14 @@ SolveView world_solve_view(dio_ctx* c, int s, bool prepare_next) {
         }
         v.out_pos = c->w_pos;
         v.out_rot = c->w_rot;
+        // with primal_order 1 the Jacobi primal takes the bodies in the broad
+        // phase's Morton order (each body's result does not depend on the
+        // order), so bodies in contact run close together (a coloured step
+        // replaces the list)
+        if (c->w_n > 0 && c->tuning.primal_order == 1) {
+            v.body_list = c->wb.perm;
+            v.list_count = c->w_n;
+        }
         return v;
     }
 }
@@ -1606,6 +1615,7 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
             v.adjoint_lanes != 16)
             invalid("adjoint_lanes must be 1, 2, 4, 8 or 16");
         if (v.overlap_broad > 1) invalid("overlap_broad must be 0 or 1");
+        if (v.primal_order > 1) invalid("primal_order must be 0 (index) or 1 (Morton)");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
@@ -1661,14 +1671,21 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
         // re-sorts of the fused step)
         ensure_granular_broad(ctx);
         const BodyView bv{ctx->w_pos, ctx->w_rot, ctx->w_tag, ctx->w_corners, ctx->w_hull};
+        // a batched world's traversal keeps to each scene's own bodies, as the step's does
+        SceneTable scenes;
+        if (ctx->w_scene_bits) {
+            scenes.of = ctx->w_scene_of;
+            scenes.off = ctx->w_scene_off;
+            scenes.bits = ctx->w_scene_bits;
+        }
         const int blanes[DIO_TUNE_BROAD_SHAPES] = {1, 2, 4, 8, 16, 32};
         float bbest = 3.4e38f;
         int best_blanes = 8;
-        run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, true, 4);
+        run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, true, 4, scenes);
         for (int k = 0; k < DIO_TUNE_BROAD_SHAPES; ++k) {
-            run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, false, blanes[k]);
+            run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, false, blanes[k], scenes);
             DIO_HIP_CHECK(hipEventRecord(e0, s));
-            for (uint32_t r = 0; r < reps; ++r) run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, false, blanes[k]);
+            for (uint32_t r = 0; r < reps; ++r) run_broad(bv, ctx->w_n, ctx->gb, ctx->p_max, s, false, blanes[k], scenes);
             DIO_HIP_CHECK(hipEventRecord(e1, s));
             DIO_HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.0f;
@@ -1703,6 +1720,27 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
             us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 1] = sched_us[1];
         }
         ctx->tuning.fuse_dual = sched_us[1] < sched_us[0] ? 1u : 0u;
+        // body order, at the chosen shape and schedule
+        float order_us[2] = {0.0f, 0.0f};
+        for (int ord = 0; ord < 2; ++ord) {
+            SolveView vo = v;
+            vo.body_list = ord == 1 && ctx->w_n > 0 ? ctx->wb.perm : nullptr;
+            vo.list_count = ord == 1 ? ctx->w_n : 0;
+            const bool fz = ctx->tuning.fuse_dual == 1;
+            launch_avbd_primal(vo, p, 1, fz, s, best_lanes);
+            DIO_HIP_CHECK(hipEventRecord(e0, s));
+            for (uint32_t r = 0; r < reps; ++r) launch_avbd_primal(vo, p, 1, fz, s, best_lanes);
+            DIO_HIP_CHECK(hipEventRecord(e1, s));
+            DIO_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            DIO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            order_us[ord] = 1000.0f * ms / (float)reps;
+        }
+        if (us_per_launch) {
+            us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 2] = order_us[0];
+            us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 3] = order_us[1];
+        }
+        ctx->tuning.primal_order = order_us[1] < order_us[0] ? 1u : 0u;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         ctx->tuning.primal_lanes = (uint32_t)best_lanes;

@@ -59,6 +59,13 @@ struct SolveView {
     const float4* c_ra;
     const float4* c_rb;
     const float4* c_c0;
+    // nonzero: the contacts were built by the device's contact build, so the
+    // tangent frame is a function of the stored normal (the narrow normal's
+    // frame, negated by the build, or for a wall contact the stored normal's,
+    // physics/gpu.rs:250,379) and c0 is (ra.w, 0, 0) (gpu.rs:259,395); the
+    // solve then reads neither c_t1 / c_t2 nor c_c0.  Zero for contacts a
+    // caller passes in (the granular solve), whose frame and c0 are read.
+    uint32_t derived_frame;
     float4* c_pen;
     float4* c_lam;
     float4* c_pen_alt;      // scratch multipliers (odd dual updates within a solve)

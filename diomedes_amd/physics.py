@@ -617,7 +617,8 @@ class PhysicsContext:
         self._check(self._lib.dio_step(self._h, ctypes.byref(o), ctypes.byref(so), nsteps))
 
     def set_tuning(self, primal_lanes: int = 0, broad_lanes: int = 0, resort_interval: int = 0,
-                   fuse_dual: int | None = None, adjoint_lanes: int = 0, overlap_broad: int | None = None) -> None:
+                   fuse_dual: int | None = None, adjoint_lanes: int = 0, overlap_broad: int | None = None,
+                   primal_order: int | None = None) -> None:
         """Launch shapes and schedule (0 / None keeps the current value)."""
         cur = self.tuning()
         t = R.Tuning()
@@ -627,20 +628,22 @@ class PhysicsContext:
         t.fuse_dual = cur["fuse_dual"] if fuse_dual is None else int(fuse_dual)
         t.adjoint_lanes = adjoint_lanes or cur["adjoint_lanes"]
         t.overlap_broad = cur["overlap_broad"] if overlap_broad is None else int(overlap_broad)
+        t.primal_order = cur["primal_order"] if primal_order is None else int(primal_order)
         self._check(self._lib.dio_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
         t = R.Tuning()
         self._check(self._lib.dio_get_tuning(self._h, ctypes.byref(t)))
         return {"primal_lanes": t.primal_lanes, "broad_lanes": t.broad_lanes, "resort_interval": t.resort_interval,
-                "fuse_dual": t.fuse_dual, "adjoint_lanes": t.adjoint_lanes, "overlap_broad": t.overlap_broad}
+                "fuse_dual": t.fuse_dual, "adjoint_lanes": t.adjoint_lanes, "overlap_broad": t.overlap_broad,
+                "primal_order": t.primal_order}
 
     def autotune(self, options: AvbdOptions, reps: int = 20) -> dict:
         """Time every primal and broad-phase launch shape and both dual
         schedules on the current world, keep the fastest; returns the choice
         and the per-shape microseconds."""
         t = R.Tuning()
-        us = (ctypes.c_float * 14)()
+        us = (ctypes.c_float * 16)()
         o = options.to_c()
         self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
         u = list(us)
@@ -651,6 +654,8 @@ class PhysicsContext:
             "us_per_launch": dict(zip((1, 4, 8, 16, 32, 64), u[:6])),
             "broad_us_per_pass": dict(zip((1, 2, 4, 8, 16, 32), u[6:12])),
             "dual_schedule_us": {"unfused": u[12], "fused": u[13]},
+            "primal_order": t.primal_order,
+            "primal_order_us": {"index": u[14], "morton": u[15]},
         }
 
     def stats(self) -> R.StepStats:

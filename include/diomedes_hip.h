@@ -238,7 +238,7 @@ typedef struct dio_step_stats {
 /* Kernel launch shapes (the autotuner's search space). */
 #define DIO_TUNE_PRIMAL_SHAPES 6
 #define DIO_TUNE_BROAD_SHAPES 6
-#define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 2)
+#define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4)
 typedef struct dio_tuning {
     uint32_t primal_lanes;     /* lanes per body in the primal: 1, 4, 8, 16, 32 or 64 (0 = default 32) */
     uint32_t broad_lanes;      /* lanes per query in the BVH traversal: 1, 2, 4, 8, 16 or 32; with 16
@@ -256,7 +256,11 @@ typedef struct dio_tuning {
                                   on a second stream beside this step's contact build and CSR (a
                                   shorter step; the second hardware queue slows the dispatch of the
                                   AVBD launches that follow); 0 (default): after them, on one stream */
-    uint32_t reserved[2];
+    uint32_t primal_order;     /* the order the Jacobi primal takes the bodies in: 0, body index; 1,
+                                  the broad phase's Morton order with each XCD given one contiguous
+                                  range, so bodies in contact share an L2 (every body's result is the
+                                  same either way).  The coloured mode takes each colour's list */
+    uint32_t reserved;
 } dio_tuning;
 
 typedef struct dio_ctx dio_ctx;
@@ -371,7 +375,9 @@ dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out);
    the average microseconds per launch in us_per_launch[0 .. DIO_TUNE_SLOTS - 1]:
    primal lanes 1, 4, 8, 16, 32, 64; broad lanes 1, 2, 4, 8, 16, 32 (a refit + traversal
    pass); then, at the chosen primal shape, an unfused primal plus a dual
-   launch and a fused primal (fuse_dual is set to the faster).  The
+   launch and a fused primal (fuse_dual is set to the faster); then that
+   primal in index order and in Morton order (primal_order is set to the
+   faster; the shapes above are timed in the order current at the call).  The
    primal with a fixed parity is idempotent and the broad timing runs in the
    granular broad buffers, so the world state is unchanged; a granular broad
    result submitted before the call must be submitted again. */

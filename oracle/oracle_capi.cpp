@@ -167,6 +167,17 @@ OAPI void* oracle_world_create(int precision, const dio_gpu_body* bodies, const 
 
 OAPI void oracle_world_destroy(void* p) { delete static_cast<WorldHandle*>(p); }
 
+// A copy of a world in its current state (bodies, contacts with their
+// multipliers, the pending narrow output, joints, timers): stepping the copy
+// continues the same trajectory.
+OAPI void* oracle_world_clone(void* p) {
+    auto* h = static_cast<WorldHandle*>(p);
+    auto* c = new WorldHandle{h->precision, nullptr, nullptr};
+    if (h->d) c->d.reset(new World<double>(*h->d));
+    if (h->f) c->f.reset(new World<float>(*h->f));
+    return c;
+}
+
 OAPI void oracle_world_step(void* p, const dio_avbd_options* o, int lag, uint32_t nsteps) {
     auto* h = static_cast<WorldHandle*>(p);
     with_world(h, [&](auto& w) {

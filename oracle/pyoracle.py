@@ -42,6 +42,7 @@ def load() -> ctypes.CDLL:
         "oracle_world_create": (c_void_p, [c_int, c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, _F,
                                            c_uint32]),
         "oracle_world_destroy": (None, [c_void_p]),
+        "oracle_world_clone": (c_void_p, [c_void_p]),
         "oracle_world_step": (None, [c_void_p, POINTER(R.Options), c_int, c_uint32]),
         "oracle_world_get_state": (None, [c_void_p, _D, _D, _D, _D, _D]),
         "oracle_world_contact_count": (c_uint32, [c_void_p]),
@@ -125,6 +126,14 @@ class OracleWorld:
         self.n = len(bodies)
         self._h = lib.oracle_world_create(precision, _p(bodies), _p(state), len(bodies), _p(cons), len(cons),
                                           ctypes.byref(ct) if ct is not None else None, _p(hull, c_float), len(hull))
+
+    def clone(self) -> "OracleWorld":
+        """A copy in the current state: stepping it continues the same
+        trajectory."""
+        c = OracleWorld.__new__(OracleWorld)
+        c.scene, c.precision, c.n = self.scene, self.precision, self.n
+        c._h = load().oracle_world_clone(self._h)
+        return c
 
     def close(self):
         if getattr(self, "_h", None):

@@ -323,6 +323,65 @@ def test_refit_carried_by_the_contact_build_at_depth_six(oracle):
         assert np.array_equal(q0, q1)
 
 
+@pytest.mark.parametrize("lanes,fuse", [(32, 1), (1, 0), (8, 1)])
+def test_primal_in_morton_order_bitexact(oracle, lanes, fuse):
+    """primal_order 1: the Jacobi primal takes the bodies in the broad phase's
+    Morton order, one contiguous range per XCD; every body's result is the
+    oracle's, as in index order."""
+    scene = scenes.cube_field(13, nx=12, ny=3, nz=12)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n) as ctx:
+        ctx.set_tuning(primal_lanes=lanes, fuse_dual=fuse, primal_order=1)
+        assert ctx.tuning()["primal_order"] == 1
+        ctx.upload(scene)
+        for k in range(4):
+            ctx.step(scene.options, 20)
+            w.step(nsteps=20)
+            _compare_world(ctx, w, 20 * (k + 1))
+
+
+def test_batched_scenes_in_morton_order_bitexact(oracle):
+    """The Morton-ordered primal over a scene table: each scene as the oracle
+    steps it alone."""
+    batch = scenes.batched_scenes(4, num_scenes=12, n_min=8, n_max=200)
+    worlds = [oracle.OracleWorld(sc, 32) for sc in batch.scenes]
+    with PhysicsContext(batch.n, max_hull_vertices=len(batch.hull)) as ctx:
+        ctx.set_tuning(primal_lanes=1, fuse_dual=0, primal_order=1)
+        ctx.batch_upload(batch)
+        for k in range(2):
+            ctx.step(batch.options, 25)
+            got = ctx.download()
+            for s, w in enumerate(worlds):
+                w.step(nsteps=25)
+                sl = batch.scene_slice(s)
+                for key in ("positions", "orientations", "velocities", "angular_velocities"):
+                    assert_bits(got[key][sl], w.state()[key], f"scene {s} step {25 * (k + 1)} {key}")
+
+
+def test_autotune_on_a_batched_table_keeps_parity(oracle):
+    """The autotuner on a scene table: its broad-phase timing keeps to each
+    scene's bodies (scenes overlap in space), it leaves the world unchanged,
+    and the tuned steps stay each scene's own."""
+    import time
+
+    batch = scenes.batched_scenes(6, num_scenes=40, n_min=8, n_max=120)
+    worlds = [oracle.OracleWorld(sc, 32) for sc in batch.scenes]
+    with PhysicsContext(batch.n, max_hull_vertices=len(batch.hull)) as ctx:
+        ctx.batch_upload(batch)
+        ctx.step(batch.options, 15)
+        t0 = time.perf_counter()
+        res = ctx.autotune(batch.options, reps=5)
+        _print("batched autotune", round(time.perf_counter() - t0, 3), "s:", res["primal_lanes"], res["broad_lanes"],
+               res["fuse_dual"], res["primal_order"])
+        ctx.step(batch.options, 15)
+        got = ctx.download()
+        for s, w in enumerate(worlds):
+            w.step(nsteps=30)
+            sl = batch.scene_slice(s)
+            for key in ("positions", "orientations", "velocities", "angular_velocities"):
+                assert_bits(got[key][sl], w.state()[key], f"scene {s} step 30 {key}")
+
+
 def test_autotune_keeps_state_and_parity(oracle):
     """The autotuner times the primal shapes without changing the world."""
     scene = scenes.cube_field(11, nx=8, ny=2, nz=8)
@@ -336,6 +395,7 @@ def test_autotune_keeps_state_and_parity(oracle):
         assert res["primal_lanes"] in (1, 4, 8, 16, 32, 64)
         assert res["broad_lanes"] in (1, 2, 4, 8, 16, 32)
         assert res["fuse_dual"] in (0, 1)
+        assert res["primal_order"] in (0, 1) and ctx.tuning()["primal_order"] == res["primal_order"]
         _compare_world(ctx, w, 40)
         ctx.step(scene.options, 20)
         w.step(nsteps=20)
