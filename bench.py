@@ -82,7 +82,7 @@ def measured_traffic(kernel: str) -> dict | None:
     return None
 
 
-def batched_leg(args, tuning: dict) -> dict:
+def batched_leg(args, tuning: dict, compare_defaults: bool = True) -> dict:
     """The batched-scenes workload: every scene of the table stepped by the
     same launches (one world with a scene table, the step captured in HIP
     graphs), against a sample of the same scenes stepped one plain context
@@ -141,6 +141,7 @@ def batched_leg(args, tuning: dict) -> dict:
     }
     if chosen:
         out["autotune_us"] = {k2: chosen[k2] for k2 in ("us_per_launch", "dual_schedule_us", "primal_order_us")}
+    if chosen and compare_defaults:
         dwall, dst, _, _, _ = run(False)
         out["library_defaults"] = {"value": n_dyn * iters * k / (dst.ms_avbd / 1000.0),
                                    "ms_per_step": 1000.0 * dwall / k, "ms_per_step_avbd": dst.ms_avbd / k}
@@ -650,7 +651,7 @@ def _roofline_only(args) -> dict:
 
     with PhysicsContext(8192) as c:
         tuning = c.tuning()
-    g = cube_field_gpu(args, default_tuning, args.profile_steps)
+    g = cube_field_gpu(args, tuning, args.profile_steps)
     return {"roofline": roofline_of(g, tuning), "cube_field_300_gpu": {k: g[k] for k in ("value", "ms_per_step",
                                                                                         "ms_per_step_avbd")}}
 
@@ -696,14 +697,6 @@ def main() -> int:
 
     from diomedes_amd import PhysicsContext, scenes
 
-    if args.only_leg:
-        legs = {"coloured": lambda: coloured_leg(args, {}), "batched": lambda: batched_leg(args, {}),
-                "adjoint": lambda: adjoint_leg(args), "mixed": lambda: mixed_leg(args, {}),
-                "joints": lambda: joints_leg(args, {}), "overlap": lambda: overlap_leg(args, {}),
-                "roofline": lambda: _roofline_only(args)}
-        print(json.dumps({"leg": args.only_leg, **legs[args.only_leg]()}), flush=True)
-        return 0
-
     if args.calibrate_copy_mib:
         import torch
 
@@ -714,6 +707,14 @@ def main() -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+
+    if args.only_leg:
+        legs = {"coloured": lambda: coloured_leg(args, {}), "batched": lambda: batched_leg(args, {}, compare_defaults=False),
+                "adjoint": lambda: adjoint_leg(args), "mixed": lambda: mixed_leg(args, {}),
+                "joints": lambda: joints_leg(args, {}), "overlap": lambda: overlap_leg(args, {}),
+                "roofline": lambda: _roofline_only(args)}
+        print(json.dumps({"leg": args.only_leg, **legs[args.only_leg]()}), flush=True)
+        return 0
 
     scene = scenes.cube_field(args.seed)
     opts = scene.options
@@ -759,7 +760,7 @@ def main() -> int:
     step_bytes = avbd_bytes(n_dyn, n, iters, avg_m, avg_k)
     # the workload's full length from the drop, with the per-kernel pass of
     # the settled pile that gives the roofline
-    g = cube_field_gpu(args, tuning, args.profile_steps)
+    g = cube_field_gpu(args, default_tuning, args.profile_steps)
     pst = g["profile"]
     line = {
         "metric": "AVBD body-iters/s & ms/step, 8k boxes",

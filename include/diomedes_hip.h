@@ -256,10 +256,12 @@ typedef struct dio_tuning {
                                   on a second stream beside this step's contact build and CSR (a
                                   shorter step; the second hardware queue slows the dispatch of the
                                   AVBD launches that follow); 0 (default): after them, on one stream */
-    uint32_t primal_order;     /* the order the Jacobi primal takes the bodies in: 0, body index; 1,
-                                  the broad phase's Morton order with each XCD given one contiguous
-                                  range, so bodies in contact share an L2 (every body's result is the
-                                  same either way).  The coloured mode takes each colour's list */
+    uint32_t primal_order;     /* the order the Jacobi primal takes the bodies in: 0, body index; 1
+                                  (default), the broad phase's Morton order with each XCD given one
+                                  contiguous range, so bodies in contact share an L2 (every body's
+                                  result is the same either way; on the settled 8192-box field the
+                                  primal's L2 misses halve).  The coloured mode takes each colour's
+                                  list */
     uint32_t reserved;
 } dio_tuning;
 

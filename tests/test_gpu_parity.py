@@ -323,16 +323,17 @@ def test_refit_carried_by_the_contact_build_at_depth_six(oracle):
         assert np.array_equal(q0, q1)
 
 
-@pytest.mark.parametrize("lanes,fuse", [(32, 1), (1, 0), (8, 1)])
-def test_primal_in_morton_order_bitexact(oracle, lanes, fuse):
-    """primal_order 1: the Jacobi primal takes the bodies in the broad phase's
-    Morton order, one contiguous range per XCD; every body's result is the
-    oracle's, as in index order."""
+@pytest.mark.parametrize("order,lanes,fuse", [(1, 32, 1), (1, 1, 0), (1, 8, 1), (0, 32, 1), (0, 1, 0)])
+def test_primal_body_order_bitexact(oracle, order, lanes, fuse):
+    """primal_order 1 (the default): the Jacobi primal takes the bodies in the
+    broad phase's Morton order, one contiguous range per XCD; 0: in index
+    order.  Either way every body's result is the oracle's."""
     scene = scenes.cube_field(13, nx=12, ny=3, nz=12)
     w = oracle.OracleWorld(scene, 32)
     with PhysicsContext(scene.n) as ctx:
-        ctx.set_tuning(primal_lanes=lanes, fuse_dual=fuse, primal_order=1)
         assert ctx.tuning()["primal_order"] == 1
+        ctx.set_tuning(primal_lanes=lanes, fuse_dual=fuse, primal_order=order)
+        assert ctx.tuning()["primal_order"] == order
         ctx.upload(scene)
         for k in range(4):
             ctx.step(scene.options, 20)
