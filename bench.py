@@ -259,8 +259,9 @@ def mixed_leg(args, tuning: dict) -> dict:
         ctx.set_tuning(**tuning)
         ctx.upload(scene)
         ctx.step(opts, max(args.warmup, 1))
-        if not args.no_tune:
-            ctx.autotune(opts, reps=10)
+        # library defaults: after the warm-up the bodies are still apart, and
+        # shapes tuned on that state (one lane per body) lose once the ropes
+        # and collisions fill the CSR (0.54 against 0.26 ms AVBD per step)
         used = ctx.tuning()
         ctx.step(opts, 0)
         t0 = time.perf_counter()
