@@ -54,6 +54,7 @@ SIGNATURES = {
     "dio_adjoint_begin": (c_int, [_VP, c_uint32, c_uint32, c_uint64]),
     "dio_adjoint_end": (c_int, [_VP]),
     "dio_adjoint_steps": (c_int, [_VP, _U32P]),
+    "dio_adjoint_set_mode": (c_int, [_VP, c_uint32]),
     "dio_adjoint_replay": (c_int, [_VP, POINTER(R.Options), _VP, _FP, _FP]),
     "dio_adjoint_backward": (
         c_int, [_VP, POINTER(R.Options), _VP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, POINTER(c_double)]

@@ -170,6 +170,7 @@ SolveView step_view(const AdjRecord& r, const AdjHostIndex& h, const AdjWork& w,
 // One recorded step's data for the reverse kernels.
 struct AdjStepView {
     uint32_t n, m, n_cap, m_max, iters;
+    uint32_t frozen;  // the dual updates' outputs are constants: no dual reverse, no DPOSE
     AvbdParams p;
     float mu;
     const float4* inv_mm;
@@ -504,10 +505,12 @@ __device__ __forceinline__ void gather_pose(const AdjStepView& v, uint32_t e0, u
         const uint32_t pc = v.peer[e];
         const bool is_b = pc != kContainer && (pc & kPeerIsB);
         const uint32_t sl = 2 * ci + (is_b ? 1u : 0u);
-        gy = add3(gy, DPOSE(v, sl, 0));
-        gr = add4(gr, DPOSE(v, sl, 1));
-        gx0 = add3(gx0, DPOSE(v, sl, 2));
-        gq0 = add4(gq0, DPOSE(v, sl, 3));
+        if (!v.frozen) {
+            gy = add3(gy, DPOSE(v, sl, 0));
+            gr = add4(gr, DPOSE(v, sl, 1));
+            gx0 = add3(gx0, DPOSE(v, sl, 2));
+            gq0 = add4(gq0, DPOSE(v, sl, 3));
+        }
         if (pc == kContainer || v.inv_mm[pc & ~kPeerIsB].w == 0.0f) continue;
         const uint32_t so = 2 * ci + (is_b ? 0u : 1u);
         gy = add3(gy, OPOSE(v, so, 0));
@@ -967,6 +970,7 @@ void adj_backward(const AdjRecord& r, const AdjHostIndex& h, AdjWork& w, const A
             v.n_cap = w.n_cap;
             v.m_max = w.m_max;
             v.iters = I;
+            v.frozen = pb.frozen;
             v.p = pb.p;
             v.mu = pb.mu;
             v.inv_mm = pb.inv_mm;
@@ -1004,7 +1008,8 @@ void adj_backward(const AdjRecord& r, const AdjHostIndex& h, AdjWork& w, const A
             uint32_t cur = 0;
             for (int k = (int)I - 1; k >= 0; --k) {
                 const bool dual = k + 1 < (int)I;
-                if (dual && m > 0)
+                // frozen: dual k's outputs are constants, so nothing flows back through it
+                if (dual && m > 0 && !pb.frozen)
                     hipLaunchKernelGGL(k_adj_dual_rev, dim3(grid_for(m)), dim3(kBlock), 0, s, v, (uint32_t)k);
                 launch_adj_primal(pb.rev_lanes, v, (uint32_t)k, cur, 1u - cur, dual ? 1u : 0u, s);
                 cur = 1u - cur;

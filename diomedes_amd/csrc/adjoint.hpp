@@ -96,6 +96,7 @@ struct AdjProblem {
     int lanes = 1;            // primal launch shape of the tape replay (1, 4, 8, 16, 32, 64)
     int rev_lanes = 8;        // lanes per body of the reverse primal (1, 2, 4, 8, 16)
     float mu = 0.0f;          // friction of every contact
+    uint32_t frozen = 0;      // 1: the multipliers and penalties each dual update writes are constants
     const float4* inv_mm = nullptr;
 };
 

@@ -320,6 +320,7 @@ struct dio_ctx {
     AdjWork adj_work{};
     void* adj_rec_base = nullptr;
     bool adj_on = false;
+    uint32_t adj_mode = DIO_ADJOINT_EXACT;  // dio_adjoint_set_mode
     // coloured Gauss-Seidel mode: colouring buffers, a pinned read-back area
     // and the last step's colouring summary
     ColourBuffers wc{};
@@ -1926,6 +1927,7 @@ AdjProblem adj_problem(dio_ctx* ctx, const dio_avbd_options* o, const dio_avbd_b
     pb.lanes = ctx->tuning.primal_lanes;
     pb.rev_lanes = ctx->tuning.adjoint_lanes;
     pb.mu = o->friction;
+    pb.frozen = ctx->adj_mode == DIO_ADJOINT_FROZEN_MULTIPLIERS ? 1u : 0u;
     pb.inv_mm = ctx->w_inv_mm;
     if (state) {
         std::vector<float4> imm(ctx->w_n);
@@ -2057,6 +2059,15 @@ dio_status dio_adjoint_end(dio_ctx* ctx) {
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->free_adjoint();
         ctx->drop_graphs();
+        return DIO_OK;
+    });
+}
+
+dio_status dio_adjoint_set_mode(dio_ctx* ctx, uint32_t mode) {
+    return guarded(ctx, [&]() {
+        if (mode != DIO_ADJOINT_EXACT && mode != DIO_ADJOINT_FROZEN_MULTIPLIERS)
+            invalid("adjoint mode must be DIO_ADJOINT_EXACT (0) or DIO_ADJOINT_FROZEN_MULTIPLIERS (1)");
+        ctx->adj_mode = mode;
         return DIO_OK;
     });
 }

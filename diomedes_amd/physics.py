@@ -568,6 +568,16 @@ class PhysicsContext:
     def adjoint_end(self) -> None:
         self._check(self._lib.dio_adjoint_end(self._h))
 
+    def adjoint_set_mode(self, mode: str) -> None:
+        """What adjoint_backward differentiates: "exact" (the replay itself) or
+        "frozen_multipliers" (the replay with the multipliers and penalties
+        each dual update writes held as constants; bounded over long
+        contact-rich runs where the exact derivative is not)."""
+        code = {"exact": 0, "frozen_multipliers": 1}.get(mode)
+        if code is None:
+            raise ValueError(f"unknown adjoint mode {mode!r}: 'exact' or 'frozen_multipliers'")
+        self._check(self._lib.dio_adjoint_set_mode(self._h, code))
+
     def adjoint_steps(self) -> int:
         v = ctypes.c_uint32()
         self._check(self._lib.dio_adjoint_steps(self._h, ctypes.byref(v)))

@@ -1054,6 +1054,16 @@ template <class T> struct RunOptions {  // AvbdRunOptions (gpu_physics.rs:76-83)
     // the warm start of hard joints' multipliers (AvbdOptions gamma and
     // penalty_min, as the contact warm start uses them: physics/gpu.rs:266-283)
     T gamma = T(0.999), penalty_min = T(1);
+    // A fixed penalty schedule (no counterpart in the reference; the map the
+    // frozen-penalty gradient differentiates): when pen_in is set, contact c's
+    // penalty after the dual update of iteration k is pen_in[(k m + c) 3 + r]
+    // (m contacts) instead of the ramped value, a constant of every dual
+    // number.  pen_out, when set, receives the penalties after each dual
+    // update in the same layout.
+    const double* pen_in = nullptr;
+    double* pen_out = nullptr;
+    const double* lam_in = nullptr;
+    double* lam_out = nullptr;
 };
 
 // A hard joint's multipliers: lambda and penalty per row (x, y, z for a
@@ -1565,10 +1575,27 @@ void avbd_solve(SolveState<T>& S, std::vector<Contact<T>>& contacts, const std::
                 f2 = f2r * s;
             }
             ct.lambda = V3<T>{f0, f1, f2};
-            if (f0 < T(0)) ct.penalty.x = rmin(ct.penalty.x + pc.beta_lin * ofabs(c0), pc.penalty_max);
-            if (tm <= bounds) {
-                ct.penalty.y = rmin(ct.penalty.y + pc.beta_lin * ofabs(c1), pc.penalty_max);
-                ct.penalty.z = rmin(ct.penalty.z + pc.beta_lin * ofabs(c2), pc.penalty_max);
+            if (pc.lam_in) {
+                const double* fixed = pc.lam_in + ((size_t)k * (size_t)m + (size_t)ci) * 3;
+                ct.lambda = V3<T>{T(fixed[0]), T(fixed[1]), T(fixed[2])};
+            }
+            if (pc.lam_out) {
+                double* rec = pc.lam_out + ((size_t)k * (size_t)m + (size_t)ci) * 3;
+                rec[0] = (double)ct.lambda.x; rec[1] = (double)ct.lambda.y; rec[2] = (double)ct.lambda.z;
+            }
+            if (pc.pen_in) {
+                const double* fixed = pc.pen_in + ((size_t)k * (size_t)m + (size_t)ci) * 3;
+                ct.penalty = V3<T>{T(fixed[0]), T(fixed[1]), T(fixed[2])};
+            } else {
+                if (f0 < T(0)) ct.penalty.x = rmin(ct.penalty.x + pc.beta_lin * ofabs(c0), pc.penalty_max);
+                if (tm <= bounds) {
+                    ct.penalty.y = rmin(ct.penalty.y + pc.beta_lin * ofabs(c1), pc.penalty_max);
+                    ct.penalty.z = rmin(ct.penalty.z + pc.beta_lin * ofabs(c2), pc.penalty_max);
+                }
+            }
+            if (pc.pen_out) {
+                double* rec = pc.pen_out + ((size_t)k * (size_t)m + (size_t)ci) * 3;
+                rec[0] = (double)ct.penalty.x; rec[1] = (double)ct.penalty.y; rec[2] = (double)ct.penalty.z;
             }
         }
         // the hard joints' dual update, from the same poses
@@ -1783,10 +1810,14 @@ template <int N, class V = double> struct Dual {
 };
 
 // One AVBD solve of `bodies` on a given contact set and CSR (World::solve
-// without joints); contacts' λ / penalty are updated in place.
+// without joints); contacts' λ / penalty are updated in place.  pen_in /
+// lam_in: fixed penalty / multiplier schedules to read, pen_out / lam_out:
+// the schedules to write (RunOptions), I m 3 values each.
 template <class T>
 void solve_bodies(std::vector<Body<T>>& bodies, std::vector<Contact<T>>& contacts,
-                  const std::vector<uint32_t>& offsets, const std::vector<uint32_t>& indices, const Options<T>& o) {
+                  const std::vector<uint32_t>& offsets, const std::vector<uint32_t>& indices, const Options<T>& o,
+                  const double* pen_in = nullptr, double* pen_out = nullptr, const double* lam_in = nullptr,
+                  double* lam_out = nullptr) {
     const size_t n = bodies.size();
     if (n == 0) return;
     SolveState<T> S;
@@ -1801,6 +1832,10 @@ void solve_bodies(std::vector<Body<T>>& bodies, std::vector<Contact<T>>& contact
     S.pos[1] = S.pos[0];
     S.rot[1] = S.rot[0];
     RunOptions<T> pc{o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, o.iterations};
+    pc.pen_in = pen_in;
+    pc.pen_out = pen_out;
+    pc.lam_in = lam_in;
+    pc.lam_out = lam_out;
     std::vector<V3<T>> op;
     std::vector<Q<T>> orr;
     const std::vector<Constraint<T>> none;
@@ -1833,13 +1868,31 @@ template <class T> Contact<T> contact_cast(const Contact<double>& c, T friction)
 
 // Replay the recorded steps from `bodies` (their state at the first step)
 // with friction μ; the contact values are exact in float and double.
+// sched_in / sched_out: the schedules of every step to hold fixed, or to
+// write (step after step, the penalties then the multipliers, each I m 3
+// values in solve_bodies' layout; sched_out is resized to fit).  freeze bit
+// 0 holds the penalties of sched_in fixed, bit 1 the multipliers.  Holding a
+// run's schedules fixed gives a map with the same values at that run's
+// parameters, whose derivatives treat them as constants.
 template <class T>
-void replay(std::vector<Body<T>>& bodies, const std::vector<ReplayStep>& steps, const Options<T>& o, T friction) {
+void replay(std::vector<Body<T>>& bodies, const std::vector<ReplayStep>& steps, const Options<T>& o, T friction,
+            const std::vector<double>* sched_in = nullptr, std::vector<double>* sched_out = nullptr,
+            int freeze = 0) {
+    size_t need = 0;
+    for (const ReplayStep& st : steps) need += (size_t)o.iterations * st.contacts.size() * 6;
+    if (sched_in && sched_in->size() != need) throw std::invalid_argument("schedule of the wrong size");
+    if (sched_out) sched_out->assign(need, 0.0);
+    size_t at = 0;
     for (const ReplayStep& st : steps) {
         std::vector<Contact<T>> cs;
         cs.reserve(st.contacts.size());
         for (const auto& c : st.contacts) cs.push_back(contact_cast<T>(c, friction));
-        solve_bodies(bodies, cs, st.offsets, st.indices, o);
+        const size_t blk = (size_t)o.iterations * st.contacts.size() * 3;
+        const double* in = sched_in ? sched_in->data() + at : nullptr;
+        double* out = sched_out ? sched_out->data() + at : nullptr;
+        solve_bodies(bodies, cs, st.offsets, st.indices, o, (freeze & 1) && in ? in : nullptr, out,
+                     (freeze & 2) && in ? in + blk : nullptr, out ? out + blk : nullptr);
+        at += 2 * blk;
     }
 }
 

@@ -599,12 +599,11 @@ template <class T> void put_poses(const std::vector<Body<T>>& b, double* pos, do
 // forward-mode derivatives along those 5 parameters; dpos (n x 3 x 5) and
 // drot (n x 4 x 5) receive d(final pose)/d(parameter).  precision 33: the
 // same derivatives (in double) along the float replay's values and branches.
-OAPI void oracle_replay(int precision, const dio_gpu_body* bodies, const dio_avbd_body_state* st, uint32_t n,
-                        uint32_t nsteps, const dio_avbd_contact* contacts, const uint64_t* coff,
-                        const uint32_t* csr_off, const uint32_t* csr_idx, const uint64_t* ioff,
-                        const dio_avbd_options* o, const double* params, double friction, uint32_t seed_body,
-                        double* pos, double* rot, double* dpos, double* drot) {
-    const std::vector<ReplayStep> steps = replay_steps(n, nsteps, contacts, coff, csr_off, csr_idx, ioff);
+namespace {
+void replay_impl(int precision, const dio_gpu_body* bodies, const dio_avbd_body_state* st, uint32_t n,
+                 const std::vector<ReplayStep>& steps, const dio_avbd_options* o, const double* params,
+                 double friction, uint32_t seed_body, double* pos, double* rot, double* dpos, double* drot,
+                 const std::vector<double>* sched_in, std::vector<double>* sched_out, int freeze) {
     double pv[5] = {0.0, 0.0, 0.0, 0.0, friction};
     const bool has_seed = seed_body < n;
     double c_mom[3] = {0.0, 0.0, 0.0};
@@ -635,7 +634,7 @@ OAPI void oracle_replay(int precision, const dio_gpu_body* bodies, const dio_avb
             using T = decltype(tag);
             std::vector<Body<T>> b(n);
             const T mu = setup(b, [](int, double x) { return T(x); });
-            replay(b, steps, options_from<T>(o), mu);
+            replay(b, steps, options_from<T>(o), mu, sched_in, sched_out, freeze);
             put_poses(b, pos, rot);
         };
         if (precision == 32) run(float{});
@@ -663,7 +662,7 @@ OAPI void oracle_replay(int precision, const dio_gpu_body* bodies, const dio_avb
                 sb.inv_moment = {DT(c_mom[0]) / m, DT(c_mom[1]) / m, DT(c_mom[2]) / m};
             }
         }
-        replay(b, steps, options_from<DT>(o), mu);
+        replay(b, steps, options_from<DT>(o), mu, sched_in, sched_out, freeze);
         for (uint32_t i = 0; i < n; ++i) {
             const V3<DT>& p = b[i].position;
             const Q<DT>& q = b[i].orientation;
@@ -686,6 +685,40 @@ OAPI void oracle_replay(int precision, const dio_gpu_body* bodies, const dio_avb
     };
     if (precision == 33) run_dual(D5f{});
     else run_dual(D5{});
+}
+}  // namespace
+
+OAPI void oracle_replay(int precision, const dio_gpu_body* bodies, const dio_avbd_body_state* st, uint32_t n,
+                        uint32_t nsteps, const dio_avbd_contact* contacts, const uint64_t* coff,
+                        const uint32_t* csr_off, const uint32_t* csr_idx, const uint64_t* ioff,
+                        const dio_avbd_options* o, const double* params, double friction, uint32_t seed_body,
+                        double* pos, double* rot, double* dpos, double* drot) {
+    const std::vector<ReplayStep> steps = replay_steps(n, nsteps, contacts, coff, csr_off, csr_idx, ioff);
+    replay_impl(precision, bodies, st, n, steps, o, params, friction, seed_body, pos, rot, dpos, drot, nullptr,
+                nullptr, 0);
+}
+
+// The frozen replay: first the replay in sched_precision (32 or 64) at
+// sched_params (5 doubles as params, or null for the state's values and
+// `friction`) writes its schedules of penalties and multipliers, then the
+// replay in `precision` (32, 64, 33 or 65, as oracle_replay) at params runs
+// with them held fixed: freeze bit 0 the penalties, bit 1 the multipliers
+// after each dual update.  At params = sched_params and precision =
+// sched_precision it gives the plain replay's values bit for bit; its dual
+// numbers are the frozen derivatives, and its central differences
+// (precision 64) check them.
+OAPI void oracle_replay_frozen(int precision, int sched_precision, int freeze, const dio_gpu_body* bodies,
+                               const dio_avbd_body_state* st, uint32_t n, uint32_t nsteps,
+                               const dio_avbd_contact* contacts, const uint64_t* coff, const uint32_t* csr_off,
+                               const uint32_t* csr_idx, const uint64_t* ioff, const dio_avbd_options* o,
+                               const double* sched_params, const double* params, double friction,
+                               uint32_t seed_body, double* pos, double* rot, double* dpos, double* drot) {
+    const std::vector<ReplayStep> steps = replay_steps(n, nsteps, contacts, coff, csr_off, csr_idx, ioff);
+    std::vector<double> sched, p0((size_t)n * 3), r0((size_t)n * 4);
+    replay_impl(sched_precision == 64 ? 64 : 32, bodies, st, n, steps, o, sched_params, friction, seed_body,
+                p0.data(), r0.data(), nullptr, nullptr, nullptr, &sched, 0);
+    replay_impl(precision, bodies, st, n, steps, o, params, friction, seed_body, pos, rot, dpos, drot, &sched,
+                nullptr, freeze);
 }
 
 // The world's contacts as its last solve received them (see keep_presolve).

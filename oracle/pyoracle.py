@@ -76,6 +76,8 @@ def load() -> ctypes.CDLL:
         "oracle_support": (None, [c_int, c_int, c_void_p, _D, _F, c_uint32, _D]),
         "oracle_replay": (None, [c_int, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, _U64, _U, _U, _U64,
                                  POINTER(R.Options), _D, c_double, c_uint32, _D, _D, _D, _D]),
+        "oracle_replay_frozen": (None, [c_int, c_int, c_int, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, _U64, _U,
+                                        _U, _U64, POINTER(R.Options), _D, _D, c_double, c_uint32, _D, _D, _D, _D]),
         "oracle_world_keep_presolve": (None, [c_void_p, c_int]),
         "oracle_world_presolve_count": (c_uint32, [c_void_p]),
         "oracle_world_get_presolve": (None, [c_void_p, c_void_p]),
@@ -287,6 +289,34 @@ def replay(bodies, state, rec: Recording, options, friction: float, precision: i
                       _p(csr_off, c_uint32), _p(csr_idx, c_uint32), _p(ioff, ctypes.c_uint64), ctypes.byref(o),
                       _p(pv, c_double), float(friction), sb, _p(pos, c_double), _p(rot, c_double),
                       _p(dpos, c_double), _p(drot, c_double))
+    return (pos, rot, dpos, drot) if dual else (pos, rot)
+
+
+def replay_frozen(bodies, state, rec: Recording, options, friction: float, precision: int = 64,
+                  seed_body: int = -1, params=None, sched_params=None, sched_precision: int = 32, freeze: int = 1):
+    """The frozen-penalty replay: the penalty schedule of the replay in
+    sched_precision at sched_params (None: the state's values and `friction`)
+    held fixed while the replay in `precision` runs at params (as replay()).
+    Its dual numbers (precision 65 / 33) are the frozen-penalty derivatives."""
+    lib = load()
+    n = rec.n
+    bodies = np.ascontiguousarray(bodies, R.GPU_BODY)
+    state = np.ascontiguousarray(state, R.BODY_STATE)
+    con, coff, csr_off, csr_idx, ioff = rec.packed()
+    pos = np.zeros((n, 3))
+    rot = np.zeros((n, 4))
+    dual = precision in (33, 65)
+    dpos = np.zeros((n, 3, 5)) if dual else None
+    drot = np.zeros((n, 4, 5)) if dual else None
+    pv = None if params is None else np.ascontiguousarray(params, np.float64)
+    sv = None if sched_params is None else np.ascontiguousarray(sched_params, np.float64)
+    o = options.to_c()
+    sb = n if seed_body is None or seed_body < 0 else seed_body
+    lib.oracle_replay_frozen(precision, sched_precision, freeze, _p(bodies), _p(state), n, rec.steps, _p(con),
+                             _p(coff, ctypes.c_uint64), _p(csr_off, c_uint32), _p(csr_idx, c_uint32),
+                             _p(ioff, ctypes.c_uint64), ctypes.byref(o), _p(sv, c_double), _p(pv, c_double),
+                             float(friction), sb, _p(pos, c_double), _p(rot, c_double), _p(dpos, c_double),
+                             _p(drot, c_double))
     return (pos, rot, dpos, drot) if dual else (pos, rot)
 
 

@@ -56,6 +56,8 @@ pub const DIO_NOALLOC_MAX_PAIRS: u32 = 65536;
 pub const DIO_NOALLOC_MAX_HULL_VERTICES: u32 = 16384;
 pub const DIO_NOALLOC_SCRATCH_BYTES: u32 = 47453184;
 pub const DIO_TUNE_SLOTS: usize = 16;
+pub const DIO_ADJOINT_EXACT: u32 = 0;
+pub const DIO_ADJOINT_FROZEN_MULTIPLIERS: u32 = 1;
 
 /// GpuShape, 80 bytes (compute.rs:28-34)
 #[repr(C)]
@@ -376,6 +378,12 @@ extern "C" {
     pub fn dio_adjoint_begin(ctx: *mut dio_ctx, max_steps: u32, checkpoint_every: u32, max_recorded_contacts: u64) -> dio_status;
     pub fn dio_adjoint_end(ctx: *mut dio_ctx) -> dio_status;
     pub fn dio_adjoint_steps(ctx: *mut dio_ctx, steps: *mut u32) -> dio_status;
+    /// What dio_adjoint_backward differentiates (a setting of the context,
+    /// kept across recordings; default DIO_ADJOINT_EXACT): the replay itself,
+    /// or (DIO_ADJOINT_FROZEN_MULTIPLIERS) the replay with the multipliers and
+    /// penalties each dual update writes held as constants.  Other values:
+    /// DIO_INVALID.
+    pub fn dio_adjoint_set_mode(ctx: *mut dio_ctx, mode: u32) -> dio_status;
     /// Replay and backward need at least one recorded step (DIO_NOT_READY
     /// otherwise) and a recording that did not overflow (DIO_CAPACITY).
     /// Replay every recorded step from the first checkpoint. state0 (n records

@@ -10,6 +10,10 @@ reverse mode, through the C ABI.
   replay (dual numbers with float values and float64 tangents: the same
   branches as the GPU run), and against the float64 dual numbers and central
   differences of the float64 replay where that replay is smooth.
+- The frozen-multiplier mode (the multipliers and penalties each dual update
+  writes held constant) against the oracle's frozen replay: its float-valued
+  dual numbers, and float64 central differences at h = 1e-3 on the 64 x 32
+  wall; over the full 120-step throw it stays finite.
 """
 
 from __future__ import annotations
@@ -267,6 +271,98 @@ def test_batched_gradient_equals_per_scene():
     ctx.close()
 
 
+# ------------------------------------------------------ frozen multipliers
+def _oracle_frozen_grad(oracle, sc, proj, rec, p, target, precision):
+    """The oracle's frozen replay at p: the penalties and multipliers of the
+    float run at p held fixed (what DIO_ADJOINT_FROZEN_MULTIPLIERS
+    differentiates)."""
+    nb = sc.n - 1
+    pos, _, dpos, _ = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, p[4], precision, proj, p, p, 32, 3)
+    return np.einsum("ij,ijk->k", 2.0 * (pos[:nb] - target) / nb, dpos[:nb]), pos
+
+
+@pytest.mark.parametrize("shape,seed,steps", [((6, 8), 1, 24), (None, 0, 40)])
+def test_frozen_gradient_matches_float_valued_dual_numbers(oracle, shape, seed, steps):
+    """DIO_ADJOINT_FROZEN_MULTIPLIERS: the reverse mode against the oracle's
+    frozen replay with float values and float64 tangents (the same float
+    trajectory and branches), within 1e-3 of the largest component, on the
+    6 x 8 wall and on the 64 x 32 wall over 40 steps (where the exact
+    derivative has already grown past 1e4).  Switching back to the exact mode
+    on the same recording gives the exact derivative again."""
+    sc = scenes.wall_throw(seed, columns=shape[0], rows=shape[1]) if shape else scenes.wall_throw(seed)
+    proj = sc.n - 1
+    target = scenes.wall_throw_target(sc, seed)
+    rec = oracle.record_world(sc, steps, precision=32)
+    ctx = _record(sc, steps, every=8)
+    ctx.adjoint_set_mode("frozen_multipliers")
+    p0 = _params(sc, proj)
+    g_gpu, pos, _ = _gpu_grad(ctx, sc, proj, p0, target)
+    g_ref, opos = _oracle_frozen_grad(oracle, sc, proj, rec, p0, target, 33)
+    assert_bits(pos, opos, "replay positions")
+    err = np.abs(g_gpu - g_ref)
+    _print(f"frozen, {sc.n} bodies, {steps} steps: gpu {g_gpu} oracle {g_ref} |err| / max {err / np.max(np.abs(g_ref))}")
+    assert np.all(np.isfinite(g_gpu)) and np.all(err <= 1e-3 * np.max(np.abs(g_ref))), (g_gpu, g_ref)
+    if shape:
+        ctx.adjoint_set_mode("exact")
+        g_exact, _, _ = _gpu_grad(ctx, sc, proj, p0, target)
+        g_ref_exact, _ = _oracle_grad(oracle, sc, proj, rec, p0, target, 33)
+        assert np.all(np.abs(g_exact - g_ref_exact) <= 1e-3 * np.max(np.abs(g_ref_exact)))
+        assert np.max(np.abs(g_exact - g_gpu)) > 1e-2 * np.max(np.abs(g_gpu))  # the modes differ here
+    ctx.close()
+
+
+def test_frozen_gradient_matches_float64_central_differences(oracle):
+    """The 64 x 32 wall (2049 bodies), seed 0, over 20 steps: the GPU frozen
+    gradient with respect to the projectile's v0 and mass against central
+    differences at h = 1e-3 of the float64 frozen replay, each within 1e-3
+    relative.  (The friction, and v0 past about 30 steps, are not smooth at
+    h = 1e-3 in the float64 map itself: the CPU tests show the differences
+    converging to the dual numbers only as h -> 0.)"""
+    sc = scenes.wall_throw(0)
+    proj = sc.n - 1
+    nb = sc.n - 1
+    steps = 20
+    target = scenes.wall_throw_target(sc, 0)
+    rec = oracle.record_world(sc, steps, precision=32)
+    ctx = _record(sc, steps, every=8)
+    ctx.adjoint_set_mode("frozen_multipliers")
+    p0 = _params(sc, proj)
+    g_gpu, _, _ = _gpu_grad(ctx, sc, proj, p0, target)
+    lf = lambda x: float(np.mean(np.sum((x[:nb] - target) ** 2, axis=1)))
+    fd = np.zeros(4)
+    for k in range(4):
+        e = np.eye(5)[k] * 1e-3
+        lp = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, 0.0, 64, proj, p0 + e, p0, 32, 3)[0]
+        lm = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, 0.0, 64, proj, p0 - e, p0, 32, 3)[0]
+        fd[k] = (lf(lp) - lf(lm)) / 2e-3
+    rel = np.abs(g_gpu[:4] - fd) / np.abs(fd)
+    _print("frozen gpu", g_gpu[:4], "float64 differences (h = 1e-3)", fd, "rel", rel)
+    assert np.all(rel <= 1e-3), rel
+    ctx.close()
+
+
+def test_frozen_gradient_over_120_steps_is_finite(oracle):
+    """The configured throw length: 120 steps of the 64 x 32 wall.  The exact
+    mode's float reverse mode overflows here; the frozen one stays finite and
+    agrees with the oracle's frozen float-valued dual numbers."""
+    sc = scenes.wall_throw(0)
+    proj = sc.n - 1
+    steps = 120
+    target = scenes.wall_throw_target(sc, 0)
+    rec = oracle.record_world(sc, steps, precision=32)
+    ctx = _record(sc, steps, every=8)
+    ctx.adjoint_set_mode("frozen_multipliers")
+    p0 = _params(sc, proj)
+    g_gpu, pos, _ = _gpu_grad(ctx, sc, proj, p0, target)
+    g_ref, opos = _oracle_frozen_grad(oracle, sc, proj, rec, p0, target, 33)
+    assert_bits(pos, opos, "replay positions")
+    err = np.abs(g_gpu - g_ref)
+    _print(f"frozen, 120 steps: gpu {g_gpu} oracle {g_ref} |err| / max {err / np.max(np.abs(g_ref))}")
+    assert np.all(np.isfinite(g_gpu))
+    assert np.all(err <= 1e-2 * np.max(np.abs(g_ref))), (g_gpu, g_ref)
+    ctx.close()
+
+
 # ------------------------------------------------------------ torch binding
 def test_torch_autograd_binding(oracle):
     import torch
@@ -326,6 +422,10 @@ def test_adjoint_errors():
         ctx.step(sc.options, 1)
         g = ctx.adjoint_backward(sc.options)  # a zero final adjoint: all zero
         assert all(np.all(g[k] == 0.0) for k in ("positions", "velocities", "inv_mass")) and g["friction"] == 0.0
+        with pytest.raises(ValueError):
+            ctx.adjoint_set_mode("smoothed")
+        with pytest.raises(DioError):
+            ctx._check(ctx._lib.dio_adjoint_set_mode(ctx._h, 2))
         ctx.upload(sc)  # a new world frees the recording
         with pytest.raises(DioError):
             ctx.adjoint_steps()

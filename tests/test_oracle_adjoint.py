@@ -90,3 +90,110 @@ def test_float_valued_dual_numbers_follow_the_float_replay(oracle):
     scale = np.max(np.abs(dpos65[:, :, :4]))
     assert scale > 0
     assert np.max(np.abs(dpos33[:, :, :4] - dpos65[:, :, :4])) <= 1e-3 * scale
+
+
+# ------------------------------------------------------- frozen multipliers
+def _p0(sc, proj):
+    return np.array([*sc.state["vel"][proj, :3].astype(np.float64), 1.0 / float(sc.state["inv_mass"][proj]),
+                     float(np.float32(sc.options.friction))])
+
+
+def test_frozen_replay_at_its_own_parameters_is_the_plain_replay(oracle):
+    """Held fixed at the values its own run writes, the schedule of penalties
+    and multipliers changes nothing: for every freeze (penalties, multipliers,
+    both) and precision, the frozen replay at the schedule's parameters equals
+    the plain replay bit for bit.  An off-by-one iteration or a swapped block
+    in the schedule breaks this."""
+    sc, proj = _throw()
+    rec = oracle.record_world(sc, 30, precision=32)
+    p0 = _p0(sc, proj)
+    for prec in (32, 64):
+        ref = oracle.replay(sc.bodies, sc.state, rec, sc.options, p0[4], prec, proj, p0)
+        for freeze in (1, 2, 3):
+            got = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, p0[4], prec, proj, p0, p0, prec, freeze)
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (prec, freeze)
+    # the float-valued dual numbers run on the float schedule too
+    got33 = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, p0[4], 33, proj, p0, p0, 32, 3)
+    ref32 = oracle.replay(sc.bodies, sc.state, rec, sc.options, p0[4], 32, proj, p0)
+    assert np.array_equal(got33[0], ref32[0])
+    # away from the schedule's parameters the frozen replay is another map
+    p1 = p0 + np.array([0.0, 0.0, 0.0, 0.0, 0.05])
+    a = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, p0[4], 64, proj, p1, p0, 64, 3)[0]
+    b = oracle.replay(sc.bodies, sc.state, rec, sc.options, p0[4], 64, proj, p1)[0]
+    assert not np.array_equal(a, b)
+
+
+def test_frozen_dual_numbers_are_the_derivative_of_the_frozen_replay(oracle):
+    """Forward-mode dual numbers of the frozen replay (penalties and
+    multipliers of the float run held fixed) against central differences of
+    the float64 frozen replay at h = 1e-6 relative, over 40 steps of a 6 x 8
+    wall: v0 and mass within 1e-6 of the largest component, the friction
+    within 1e-3.  The exact-mode dual numbers do not match these differences:
+    the check tells the two maps apart."""
+    sc = scenes.wall_throw(1, columns=6, rows=8)
+    proj = sc.n - 1
+    nb = sc.n - 1
+    steps = 40
+    rec = oracle.record_world(sc, steps, precision=32)
+    target = scenes.wall_throw_target(sc, 1)
+    p0 = _p0(sc, proj)
+    pos, _, dpos, _ = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, p0[4], 65, proj, p0, p0, 32, 3)
+    g = np.einsum("ij,ijk->k", 2.0 * (pos[:nb] - target) / nb, dpos[:nb])
+    fd = np.zeros(5)
+    for k in range(5):
+        h = 1e-6 * max(abs(p0[k]), 1.0)
+        e = np.eye(5)[k] * h
+        lp = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, 0.0, 64, proj, p0 + e, p0, 32, 3)[0]
+        lm = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, 0.0, 64, proj, p0 - e, p0, 32, 3)[0]
+        fd[k] = (_loss(lp, target, nb) - _loss(lm, target, nb)) / (2 * h)
+    print("frozen dual numbers", g, "central differences", fd)
+    scale = np.max(np.abs(g[:4]))
+    assert scale > 0 and np.max(np.abs(g[:4] - fd[:4])) <= 1e-6 * scale, (g, fd)
+    assert abs(g[4] - fd[4]) <= 1e-3 * abs(fd[4]), (g, fd)
+    pe, _, dpe, _ = oracle.replay(sc.bodies, sc.state, rec, sc.options, p0[4], 65, proj, p0)
+    g_exact = np.einsum("ij,ijk->k", 2.0 * (pe[:nb] - target) / nb, dpe[:nb])
+    assert np.max(np.abs(g_exact[:4] - fd[:4])) > 1e-2 * scale, (g_exact, fd)
+
+
+def test_frozen_tangents_stay_bounded_where_the_exact_ones_do_not(oracle):
+    """Over 120 steps of a 16 x 8 wall the exact tangent |d x / d v0| of the
+    float64 replay passes 1e12 (the dual update's feedback through the stiff
+    wall); with the multipliers and penalties frozen it stays below 1e4."""
+    sc = scenes.wall_throw(0, columns=16, rows=8)
+    proj = sc.n - 1
+    nb = sc.n - 1
+    rec = oracle.record_world(sc, 120, precision=32)
+    p0 = _p0(sc, proj)
+    _, _, d_exact, _ = oracle.replay(sc.bodies, sc.state, rec, sc.options, p0[4], 65, proj, p0)
+    _, _, d_frozen, _ = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, p0[4], 65, proj, p0, p0, 32, 3)
+    t_exact = np.max(np.abs(d_exact[:nb, :, :3]))
+    t_frozen = np.max(np.abs(d_frozen[:nb, :, :3]))
+    print(f"120 steps: max |dx/dv0| exact {t_exact:.3g}, frozen {t_frozen:.3g}")
+    assert t_exact > 1e12 and np.isfinite(t_frozen) and t_frozen < 1e4
+
+
+def test_frozen_gradient_of_the_full_wall_at_the_configured_difference_step(oracle):
+    """The 64 x 32 wall (2049 bodies), seed 0, over 20 steps: the frozen
+    float64 dual numbers against central differences at h = 1e-3, v0 and mass
+    each within 1e-3 relative.  (Past about 30 steps, or for the friction
+    already at 10, the frozen replay is not smooth at h = 1e-3: contacts
+    switch between the friction cone's and the normal clamp's pieces within
+    the step, and the differences change with h while the dual numbers are
+    the h -> 0 limit, as the 6 x 8 test shows.)"""
+    sc = scenes.wall_throw(0)
+    proj = sc.n - 1
+    nb = sc.n - 1
+    rec = oracle.record_world(sc, 20, precision=32)
+    target = scenes.wall_throw_target(sc, 0)
+    p0 = _p0(sc, proj)
+    pos, _, dpos, _ = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, p0[4], 65, proj, p0, p0, 32, 3)
+    g = np.einsum("ij,ijk->k", 2.0 * (pos[:nb] - target) / nb, dpos[:nb])
+    fd = np.zeros(4)
+    for k in range(4):
+        e = np.eye(5)[k] * 1e-3
+        lp = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, 0.0, 64, proj, p0 + e, p0, 32, 3)[0]
+        lm = oracle.replay_frozen(sc.bodies, sc.state, rec, sc.options, 0.0, 64, proj, p0 - e, p0, 32, 3)[0]
+        fd[k] = (_loss(lp, target, nb) - _loss(lm, target, nb)) / 2e-3
+    rel = np.abs(g[:4] - fd) / np.abs(fd)
+    print("full wall, 20 steps: frozen dual numbers", g[:4], "differences (h = 1e-3)", fd, "rel", rel)
+    assert np.all(np.abs(fd) > 0) and np.all(rel <= 1e-3), rel
