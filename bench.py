@@ -101,13 +101,15 @@ def batched_leg(args, tuning: dict, compare_defaults: bool = True) -> dict:
     n_dyn = batch.n_dynamic
     tune = not args.no_tune
 
-    def run(autotune: bool):
+    def run(autotune: bool, scene_solve: int = 0, shapes: dict | None = None):
         with PhysicsContext(batch.n, max_pairs=8 * batch.n, max_contacts=16 * batch.n,
                             max_hull_vertices=len(batch.hull)) as ctx:
             ctx.set_tuning(**tuning)
             ctx.batch_upload(batch)
             ctx.step(opts, max(args.warmup, 1))
             chosen = ctx.autotune(opts, reps=10) if autotune else None
+            if scene_solve:
+                ctx.set_tuning(**{**(shapes or {}), "scene_solve": scene_solve})
             ctx.step(opts, 0)
             t0 = time.perf_counter()
             ctx.step(opts, k)
@@ -147,6 +149,11 @@ def batched_leg(args, tuning: dict, compare_defaults: bool = True) -> dict:
                                    "ms_per_step": 1000.0 * dwall / k, "ms_per_step_avbd": dst.ms_avbd / k}
     if rest:
         out["to_step_300"] = rest
+    if chosen and compare_defaults:
+        # the whole solve in one launch (bundles of >= 128 bodies per workgroup) at the same shapes
+        swall, sst, _, _, _ = run(False, scene_solve=128, shapes=used)
+        out["scene_solve_128"] = {"ms_per_step": 1000.0 * swall / k, "ms_per_step_avbd": sst.ms_avbd / k,
+                                  "vs_launches_avbd": (sst.ms_avbd / k) / (st.ms_avbd / k)}
     if args.plain_sample <= 0:
         return out
     # plain: a sample of the same scenes, each in its own context, stepped in turn

@@ -15,6 +15,8 @@
 // through a per-body CSR instead of the reference's scan of every constraint
 // per body (avbd/src/lib.rs:422-539), which visits the same joints in the
 // same order.
+#include <algorithm>
+
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -336,8 +338,10 @@ __global__ void __launch_bounds__(kBlock) k_avbd_init(SolveView v, AvbdParams p)
     init_body(t, p, i, v.cur_pos[i], v.cur_rot[i], xyz(v.vel[i]), xyz(v.ang[i]), xyz(v.prev_vel[i]), v.inv_mm[i].w);
 }
 
+// The primal of body i (one lane per body): its 6x6 block over its contacts
+// and joints, solved, and the pose written to the other parity buffer.
 template <bool FUSED>
-__global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams p, uint32_t iter) {
+__device__ __forceinline__ void primal_one(const SolveView& v, const AvbdParams& p, uint32_t iter, uint32_t i) {
     const uint32_t parity = iter & 1u;
     const bool fuse = FUSED && iter > 0;
     const float4* __restrict__ lam_in = lam_buf(v, fuse ? iter - 1 : 0);
@@ -345,8 +349,6 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
     float4* lam_out = lam_buf(v, iter);
     float4* pen_out = pen_buf(v, iter);
     const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
-    const uint32_t i = body_at(v, p, xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
-    if (i >= p.n) return;
     const float4 imm = v.inv_mm[i];
     if (imm.w == 0.0f) return;
     const float4* __restrict__ rp = v.pos[parity];
@@ -530,6 +532,13 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
     const q4 new_rot = quat_step(snap_rot, f3{x[3], x[4], x[5]});
     v.pos[1 - parity][i] = f4(new_pos, 0.0f);
     v.rot[1 - parity][i] = f4q(new_rot);
+}
+
+template <bool FUSED>
+__global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams p, uint32_t iter) {
+    const uint32_t i = body_at(v, p, xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
+    if (i >= p.n) return;
+    primal_one<FUSED>(v, p, iter, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1039,6 +1048,75 @@ __device__ __forceinline__ void recover_one(const SolveView& v, const AvbdParams
     if (v.next.init_lin) init_body(v.next, p, i, pos4, rot4, nv, xyz(nang), xyz(vel), inv_mass);
 }
 
+// The whole solve of a scene table in one launch (no counterpart in the
+// reference).  Every contact of a batched world joins two bodies of one scene
+// (or a body and its scene's floor), so a scene's primal, dual and recover
+// read and write only that scene's bodies and contacts, and the launch-wide
+// order of the Jacobi schedule (every primal before any dual of the same
+// iteration) is needed only within each scene: a workgroup barrier stands in
+// for the kernel boundary.  A workgroup takes a bundle (a body range made of
+// whole consecutive scenes) from a queue, most bodies first so the long ones
+// start early, runs its whole solve, and takes the next until the queue is
+// empty; the last workgroup to finish resets the queue for the next launch.
+// Per body and per contact the arithmetic is that of the separate launches,
+// so the result is bit-identical.  A contact's dual update is run by the
+// thread that holds its body a (each contact is in a's CSR list once, with
+// the a side flagged).  The step's init runs before (the previous step's
+// final launch, or its own launch).  On the 4096-scene table this launch is
+// slower than the separate ones (0.63-0.68 ms against 0.44 ms of solve per
+// step at I = 4): a workgroup waits at every barrier for its slowest body
+// while the separate launches overlap every workgroup's latency, the largest
+// scenes' chain of phases bounds the launch, and the dual walks the CSR per
+// body instead of the contiguous contact arrays.
+template <bool FUSED, int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4)))
+k_avbd_scenes(SolveView v, AvbdParams p, SceneQueue q, uint32_t iters) {
+    __shared__ uint32_t s_slot;
+    const uint32_t parity = (iters - 1) & 1u;
+    if (v.stat_acc && blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t m = v.m_dev ? *v.m_dev : v.m_host;
+        v.stat_acc[0] += *v.stat_pairs;
+        v.stat_acc[1] += m;
+        v.stat_acc[2] += v.offsets[p.n];
+    }
+    for (;;) {
+        __syncthreads();  // every thread has read the previous slot
+        if (threadIdx.x == 0) s_slot = atomicAdd(&q.next[0], 1u);
+        __syncthreads();
+        const uint32_t slot = s_slot;
+        if (slot >= q.count) break;  // uniform over the workgroup
+        const uint32_t sc = q.order[slot];
+        const uint32_t b0 = q.body_off[sc], b1 = q.body_off[sc + 1];
+        for (uint32_t k = 0; k < iters; ++k) {
+            for (uint32_t i = b0 + threadIdx.x; i < b1; i += BS) primal_one<FUSED>(v, p, k, i);
+            __syncthreads();
+            if (!FUSED && k + 1 < iters) {
+                for (uint32_t i = b0 + threadIdx.x; i < b1; i += BS)
+                    for (uint32_t j = v.offsets[i]; j < v.offsets[i + 1]; ++j) {
+                        const uint32_t pc = v.peers[j];
+                        if (pc == kContainer || (pc & kPeerIsB) == 0u) dual_one(v, p, v.indices[j], k & 1u, 0u, 0u, 0u);
+                    }
+                __syncthreads();
+            }
+        }
+        // the last dual update and recover (as k_avbd_dual_recover: the dual
+        // reads only the final Jacobi poses and the initial poses)
+        for (uint32_t i = b0 + threadIdx.x; i < b1; i += BS) {
+            for (uint32_t j = v.offsets[i]; j < v.offsets[i + 1]; ++j) {
+                const uint32_t pc = v.peers[j];
+                if (pc == kContainer || (pc & kPeerIsB) == 0u)
+                    dual_one(v, p, v.indices[j], parity, FUSED ? parity : 0u, 0u, FUSED ? iters : 0u);
+            }
+            recover_one(v, p, i, parity);
+        }
+    }
+    if (threadIdx.x == 0 && atomicAdd(&q.next[1], 1u) + 1u == gridDim.x) {
+        // every other workgroup has taken its last slot
+        q.next[0] = 0u;
+        q.next[1] = 0u;
+    }
+}
+
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
@@ -1120,6 +1198,16 @@ void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parit
     if (p.n == 0) return;
     hipLaunchKernelGGL(k_avbd_recover, dim3(blocks_for(p.n)), dim3(kBlock), 0, s, v, p, parity);
 }
+void launch_avbd_scenes(const SolveView& v, const AvbdParams& p, uint32_t iters, bool fused, const SceneQueue& q,
+                        hipStream_t s, const LaunchEvents& ev) {
+    if (p.n == 0 || q.count == 0 || iters == 0) return;
+    // (one-wave workgroups for small bundles measured slower still: the
+    // largest scenes then take 8 bodies per lane through every phase)
+    const dim3 grid(std::min<uint32_t>(q.count, kSceneWorkgroups));
+    if (fused) launch_k(k_avbd_scenes<true, kBlock>, grid, dim3(kBlock), s, ev, v, p, q, iters);
+    else launch_k(k_avbd_scenes<false, kBlock>, grid, dim3(kBlock), s, ev, v, p, q, iters);
+}
+
 void launch_avbd_dual_recover(const SolveView& v, const AvbdParams& p, uint32_t iters, bool fused, uint32_t m_grid,
                               hipStream_t s, const LaunchEvents& ev) {
     launch_k(k_avbd_dual_recover, dim3(contact_blocks(m_grid > p.n ? m_grid : p.n)), dim3(kBlock), s, ev, v, p, iters,

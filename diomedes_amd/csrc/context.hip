@@ -273,6 +273,15 @@ struct dio_ctx {
     uint32_t* w_scene_off = nullptr;
     ContainerParams* w_scene_ct = nullptr;
     uint32_t w_num_scenes = 1, w_scene_bits = 0;
+    // the queue of the per-scene solve: runs of whole consecutive scenes
+    // ("bundles", each at least bundle_bodies bodies, or one scene), their
+    // body offsets, the order they are taken in (most bodies first) and the
+    // queue's two counters (zero between launches)
+    uint32_t* w_scene_order = nullptr;
+    uint32_t* w_bundle_off = nullptr;
+    uint32_t* w_scene_q = nullptr;
+    uint32_t w_bundles = 0, w_bundle_bodies = 0;
+    std::vector<uint32_t> w_scene_off_host;
     BroadBuffers wb{};
     struct Pending {
         uint2* pairs;
@@ -967,6 +976,9 @@ void ensure_world(dio_ctx* c) {
     c->w_scene_of = A.alloc<uint32_t>(n);
     c->w_scene_off = A.alloc<uint32_t>((size_t)n + 1);
     c->w_scene_ct = A.alloc<ContainerParams>(n);
+    c->w_scene_order = A.alloc<uint32_t>(n);
+    c->w_bundle_off = A.alloc<uint32_t>((size_t)n + 1);
+    c->w_scene_q = A.alloc<uint32_t>(2);
     c->w_j.abk = A.alloc<uint4>(c->c_cap);
     c->w_j.anc_a = A.alloc<float4>(c->c_cap);
     c->w_j.anc_b = A.alloc<float4>(c->c_cap);
@@ -1130,7 +1142,17 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         v.stat_pairs = c->pend[bn_pend].p_dev;
         v.stat_acc = c->w_acc;
         const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n, o.gamma, o.penalty_min);
-        schedule_avbd(v, p, o.iterations, c->m_cap, st, kt, c->tuning, false);
+        if (c->tuning.scene_solve && c->w_scene_bits && c->w_j.c == 0) {
+            if (c->w_bundle_bodies != c->tuning.scene_solve)
+                throw StatusError{DIO_HIP, "internal: scene bundles not built for this tuning", 0};
+            const SceneQueue q{c->w_scene_order, c->w_bundle_off, c->w_bundles, c->w_scene_q};
+            const bool fused = c->tuning.fuse_dual != 0u;
+            auto launch = [&](const LaunchEvents& ev) { launch_avbd_scenes(v, p, o.iterations, fused, q, st, ev); };
+            if (kt) kt->time(1, launch);
+            else launch(LaunchEvents{});
+        } else {
+            schedule_avbd(v, p, o.iterations, c->m_cap, st, kt, c->tuning, false);
+        }
     }
 }
 
@@ -1189,6 +1211,30 @@ void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const di
     v.stat_acc = c->w_acc;
     const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n, o.gamma, o.penalty_min);
     schedule_avbd_coloured(v, p, o.iterations, c->m_cap, st, kt, c->tuning, c->wc.vals[lb], start);
+}
+
+// The per-scene solve's bundles for tuning.scene_solve (a batched world
+// only): consecutive scenes are joined until a bundle holds at least that
+// many bodies (a larger scene is a bundle of its own), so the workgroups of
+// small scenes have work for their lanes; bundles are taken most bodies
+// first.  Synchronous (outside graph capture).
+void build_bundles(dio_ctx* c) {
+    const uint32_t want = c->tuning.scene_solve;
+    if (!c->w_scene_bits || want == 0 || c->w_bundle_bodies == want) return;
+    const std::vector<uint32_t>& so = c->w_scene_off_host;
+    std::vector<uint32_t> off{so[0]};
+    for (size_t k = 1; k < so.size(); ++k)
+        if (so[k] - off.back() >= want || (k + 1 == so.size() && so[k] > off.back())) off.push_back(so[k]);
+    const uint32_t count = (uint32_t)off.size() - 1;
+    std::vector<uint32_t> order(count);
+    for (uint32_t k = 0; k < count; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return off[x + 1] - off[x] > off[y + 1] - off[y]; });
+    DIO_HIP_CHECK(hipStreamSynchronize(c->stream));
+    DIO_HIP_CHECK(hipMemcpy(c->w_bundle_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    if (count > 0) DIO_HIP_CHECK(hipMemcpy(c->w_scene_order, order.data(), (size_t)count * 4, hipMemcpyHostToDevice));
+    c->w_bundles = count;
+    c->w_bundle_bodies = want;
 }
 
 // Whether buffer set s holds this step's init, prepared for these options.
@@ -1387,9 +1433,13 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
                                 hipMemcpyHostToDevice));
         uint32_t bits = 1;
         while ((1u << bits) < num_scenes) ++bits;
+        ctx->w_scene_off_host.assign(scene_offsets, scene_offsets + num_scenes + 1);
+        ctx->w_bundle_bodies = 0;  // built for the tuning below
+        launch_fill_u32(ctx->w_scene_q, 0u, 2, s);
         ctx->w_num_scenes = num_scenes;
         ctx->w_scene_bits = bits;
         ctx->w_ct = ContainerParams{};
+        build_bundles(ctx);
     } else {
         ctx->w_ct = container_params(scene_offsets ? &containers[0] : containers);
     }
@@ -1617,9 +1667,11 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
             invalid("adjoint_lanes must be 1, 2, 4, 8 or 16");
         if (v.overlap_broad > 1) invalid("overlap_broad must be 0 or 1");
         if (v.primal_order > 1) invalid("primal_order must be 0 (index) or 1 (Morton)");
+        if (v.scene_solve > 65536) invalid("scene_solve must be 0 (off) or the bodies per bundle, 1 .. 65536");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
+        build_bundles(ctx);
         return DIO_OK;
     });
 }

@@ -136,4 +136,21 @@ void launch_avbd_dual_recover(const SolveView& v, const AvbdParams& p, uint32_t 
 // recover alone, reading the final Jacobi buffer pos[1 - parity]
 void launch_avbd_recover(const SolveView& v, const AvbdParams& p, uint32_t parity, hipStream_t stream);
 
+// The scene queue of a batched world: scene order[t] (t < count, largest
+// first) holds bodies body_off[s] .. body_off[s + 1]; next[0] is the next
+// queue slot and next[1] the finished workgroups, both zero between launches.
+struct SceneQueue {
+    const uint32_t* order;
+    const uint32_t* body_off;
+    uint32_t count;
+    uint32_t* next;
+};
+// Workgroups of the scene launch (each takes scenes until the queue is empty).
+constexpr uint32_t kSceneWorkgroups = 2048;
+// Every primal, dual and the final dual + recover of a batched world's solve
+// in one launch, a workgroup per scene (the world has no joints; the step's
+// init ran before).  Bit-identical to the separate launches.
+void launch_avbd_scenes(const SolveView& v, const AvbdParams& p, uint32_t iters, bool fused, const SceneQueue& q,
+                        hipStream_t stream, const LaunchEvents& ev = {});
+
 }  // namespace dio

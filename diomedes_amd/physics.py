@@ -628,7 +628,7 @@ class PhysicsContext:
 
     def set_tuning(self, primal_lanes: int = 0, broad_lanes: int = 0, resort_interval: int = 0,
                    fuse_dual: int | None = None, adjoint_lanes: int = 0, overlap_broad: int | None = None,
-                   primal_order: int | None = None) -> None:
+                   primal_order: int | None = None, scene_solve: int | None = None) -> None:
         """Launch shapes and schedule (0 / None keeps the current value)."""
         cur = self.tuning()
         t = R.Tuning()
@@ -639,6 +639,7 @@ class PhysicsContext:
         t.adjoint_lanes = adjoint_lanes or cur["adjoint_lanes"]
         t.overlap_broad = cur["overlap_broad"] if overlap_broad is None else int(overlap_broad)
         t.primal_order = cur["primal_order"] if primal_order is None else int(primal_order)
+        t.scene_solve = cur["scene_solve"] if scene_solve is None else int(scene_solve)
         self._check(self._lib.dio_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -646,7 +647,7 @@ class PhysicsContext:
         self._check(self._lib.dio_get_tuning(self._h, ctypes.byref(t)))
         return {"primal_lanes": t.primal_lanes, "broad_lanes": t.broad_lanes, "resort_interval": t.resort_interval,
                 "fuse_dual": t.fuse_dual, "adjoint_lanes": t.adjoint_lanes, "overlap_broad": t.overlap_broad,
-                "primal_order": t.primal_order}
+                "primal_order": t.primal_order, "scene_solve": t.scene_solve}
 
     def autotune(self, options: AvbdOptions, reps: int = 20) -> dict:
         """Time every primal and broad-phase launch shape and both dual

@@ -262,7 +262,13 @@ typedef struct dio_tuning {
                                   result is the same either way; on the settled 8192-box field the
                                   primal's L2 misses halve).  The coloured mode takes each colour's
                                   list */
-    uint32_t reserved;
+    uint32_t scene_solve;      /* a batched world (scene table) without joints: N > 0 runs every primal,
+                                  dual and the final dual + recover of a step's solve in one launch, each
+                                  workgroup taking bundles of whole consecutive scenes (at least N
+                                  bodies each, or one larger scene) from a queue, most bodies first (the
+                                  result is the same either way); 0 (default): one launch per primal
+                                  and dual over the whole world.  At most 65536.  Ignored for other
+                                  worlds */
 } dio_tuning;
 
 typedef struct dio_ctx dio_ctx;

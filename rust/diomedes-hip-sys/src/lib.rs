@@ -241,7 +241,7 @@ pub struct dio_tuning {
     pub adjoint_lanes: u32,
     pub overlap_broad: u32,
     pub primal_order: u32,
-    pub reserved: u32,
+    pub scene_solve: u32,
 }
 
 /// The engine's names of the records it already defines (render/compute.rs, render/gpu_physics.rs).

@@ -152,6 +152,29 @@ def test_batched_4096_scenes_match_oracle_per_scene(oracle):
             _print("batched 4096 step", 60 * (k + 1), "contacts", len(gc), f"{time.perf_counter() - t0:.1f} s")
 
 
+def test_batched_4096_scene_solve_bitexact():
+    """The full 4096-scene table over 120 steps with the per-scene solve (one
+    launch, bundles of at least 256 bodies) against the per-launch schedule, both
+    tuned as the autotuner tunes this table (one lane per body, unfused):
+    every body bit-identical every 40 steps."""
+    batch = scenes.batched_scenes(0)
+    opts = batch.options
+    res = {}
+    for solve in (0, 256):
+        with PhysicsContext(batch.n, max_pairs=8 * batch.n, max_contacts=16 * batch.n,
+                            max_hull_vertices=len(batch.hull)) as ctx:
+            ctx.set_tuning(primal_lanes=1, fuse_dual=0, scene_solve=solve)
+            ctx.batch_upload(batch)
+            res[solve] = []
+            for _ in range(3):
+                ctx.step(opts, 40)
+                res[solve].append(ctx.download())
+    for k in range(3):
+        for key in KEYS:
+            assert_bits(res[256][k][key], res[0][k][key], f"scene solve step {40 * (k + 1)} {key}")
+    _print("batched 4096, scene solve: bit-identical over 120 steps")
+
+
 def test_wall_throw_64_seeds_120_steps_bitexact(oracle):
     """64 walls of 64 x 32 bricks, each hit by its own projectile, batched in
     one world for the 120 steps of the differentiable workload; each wall

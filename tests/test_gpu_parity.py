@@ -852,6 +852,58 @@ def test_batched_scenes_match_oracle_per_scene(oracle):
             _print("batched step", 20 * (k + 1), "contacts", len(gc))
 
 
+def _step_batch(batch, steps, **tuning):
+    with PhysicsContext(batch.n, max_pairs=8 * batch.n, max_contacts=16 * batch.n,
+                        max_hull_vertices=max(len(batch.hull), 1)) as ctx:
+        ctx.set_tuning(**tuning)
+        ctx.batch_upload(batch)
+        out = []
+        for chunk in steps:
+            ctx.step(batch.options, chunk)
+            out.append((ctx.download(), ctx.contacts()))
+        return out
+
+
+@pytest.mark.parametrize("bundle", [1, 300])
+@pytest.mark.parametrize("fuse", [0, 1])
+@pytest.mark.parametrize("table", ["mixed", "many_small", "with_empty"])
+def test_scene_solve_bitexact(table, fuse, bundle):
+    """scene_solve N (the whole solve in one launch, each workgroup taking
+    bundles of consecutive scenes of at least N bodies from a queue) against
+    the per-launch schedule: every body and contact bit-identical, over
+    several steps (the queue is reset by each launch's last workgroup), for a
+    table wider than the launch (3000 one-scene bundles against 2048
+    workgroups) and one holding empty and one-body scenes."""
+    from diomedes_amd.physics import Scene
+
+    if table == "mixed":
+        batch = scenes.batched_scenes(5, num_scenes=24, n_min=8, n_max=512)
+    elif table == "many_small":
+        batch = scenes.batched_scenes(9, num_scenes=3000, n_min=8, n_max=24)
+    else:
+        base = scenes.batched_scenes(7, num_scenes=3, n_min=8, n_max=30).scenes
+        one = base[0]
+        single = Scene(one.bodies[:1].copy(), one.state[:1].copy(), container=one.container, hull=one.hull,
+                       options=one.options)
+        none = Scene(one.bodies[:0].copy(), one.state[:0].copy(), container=one.container, hull=one.hull,
+                     options=one.options)
+        batch = scenes.BatchedScenes([none, base[0], none, single, base[1], none, base[2], none], one.hull,
+                                     one.options)
+    steps = (1, 14, 30)
+    ref = _step_batch(batch, steps, primal_lanes=1, fuse_dual=fuse, scene_solve=0)
+    got = _step_batch(batch, steps, primal_lanes=1, fuse_dual=fuse, scene_solve=bundle)
+    done = 0
+    for (g, gc), (r, rc), k in zip(got, ref, steps):
+        done += k
+        for key in ("positions", "orientations", "velocities", "angular_velocities", "prev_velocities"):
+            assert_bits(g[key], r[key], f"{table} fuse {fuse} step {done} {key}")
+        assert len(gc) == len(rc)
+        for key in ("lambda_", "penalty"):
+            assert_bits(gc[key], rc[key], f"{table} fuse {fuse} step {done} contact {key}")
+    _print(f"scene solve {table} ({len(batch.scenes)} scenes, {batch.n} bodies) fuse {fuse} bundle {bundle}: "
+           "bit-identical")
+
+
 def test_batched_scenes_match_plain_contexts():
     """A 24-scene batch (up to 512 bodies per scene) against each scene stepped
     in a plain context of its own, GPU against GPU."""
