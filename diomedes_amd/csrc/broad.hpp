@@ -17,6 +17,7 @@ struct BodyView {
     const uint32_t* tag;
     const float4* corners;  // 4 per body (GpuShape corners)
     const float4* hull;     // hull vertex pool (may be null)
+    uint32_t hull_count = 0;  // vertices in the pool; 0 when not known (the narrow phase then reads it in place)
 };
 
 // Device buffers of one broad-phase pipeline (capacity n_cap bodies).

@@ -765,7 +765,8 @@ static void narrow_dispatch(dio_ctx* ctx, const uint2* pairs, uint32_t count) {
     o.pairs = pairs;
     o.p_dev = ctx->g_np_dev;
     DIO_HIP_CHECK(hipMemcpyAsync(ctx->g_np_dev, &ctx->g_last_pairs, 4, hipMemcpyHostToDevice, ctx->stream));
-    BodyView bv{ctx->g_pos, ctx->g_rot, ctx->g_tag, ctx->g_corners, ctx->g_hull};
+    BodyView bv{ctx->g_pos, ctx->g_rot, ctx->g_tag, ctx->g_corners, ctx->g_hull,
+                ctx->g_hull == ctx->w_hull ? ctx->w_hull_count : 0u};
     run_narrow(bv, o, ctx->g_simplex, ctx->stream);
     launch_pack_contact_out(o, count, ctx->g_out, ctx->stream);
     DIO_HIP_CHECK(hipGetLastError());
@@ -1082,7 +1083,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         scenes.off = c->w_scene_off;
         scenes.bits = c->w_scene_bits;
     }
-    BodyView bv{c->w_pos, c->w_rot, c->w_tag, c->w_corners, c->w_hull};
+    BodyView bv{c->w_pos, c->w_rot, c->w_tag, c->w_corners, c->w_hull, c->w_hull_count};
     BuildOptions bo{o.dt, o.alpha, o.margin, o.penalty_min, o.penalty_max, o.gamma, o.friction};
     const int lag = so.contact_lag ? 1 : 0;
     if (seg == 0) {

@@ -32,6 +32,19 @@ constexpr int kMaxV = 256;
 constexpr int kMaxF = 512;
 constexpr int kSmallV = 64;
 constexpr int kSmallF = 128;
+// A hull vertex pool of at most this many vertices (16 KB) is staged in LDS
+// by every workgroup of the GJK and EPA launches: the support scans then read
+// LDS instead of the L2 (the same values, so the same results).
+constexpr uint32_t kLdsPoolMax = 1024;
+extern __shared__ float4 pool_lds[];
+
+// The pool the support functions read: staged in LDS (LDS_POOL) or in place.
+template <bool LDS_POOL> __device__ __forceinline__ HullPool stage_pool(const BodyView& bv) {
+    if (!LDS_POOL) return HullPool{bv.hull};
+    for (uint32_t i = threadIdx.x; i < bv.hull_count; i += blockDim.x) pool_lds[i] = bv.hull[i];
+    __syncthreads();
+    return HullPool{pool_lds};
+}
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
@@ -173,45 +186,86 @@ __device__ __forceinline__ void write_slot(const NarrowOut& o, uint32_t slot, f3
 }
 
 // One lane per pair: SAT, sphere, or GJK (intersecting pairs go to EPA).
-__global__ void __launch_bounds__(kBlock) k_narrow(BodyView bv, NarrowOut o, float4* simplex_store) {
+// The workgroup takes the pair list 256 pairs at a time and first sorts its
+// chunk by class in LDS (box / box and sphere / sphere pairs first, then the
+// GJK pairs, each in list order), so that a wave runs one branch instead of
+// both; every pair's result goes to its own slots, as before.  Four waves
+// per SIMD (at most 128 VGPRs, a few spilled): faster than three without
+// spills or five with more (684, 764 and 728 us on the 4096-scene table).
+template <bool LDS_POOL>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_narrow(BodyView bv, NarrowOut o, float4* simplex_store) {
+    constexpr int kWaves = kBlock / kWave;
+    __shared__ uint32_t s_pair[kBlock];
+    __shared__ uint32_t s_wave[2][kWaves];
     const uint32_t np = *o.p_dev;
-    const HullPool hp{bv.hull};
-    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < np; p += gridDim.x * kBlock) {
-        const uint2 pr = o.pairs[p];
-        const ShapeRef a = load_shape(bv, pr.x);
-        const ShapeRef b = load_shape(bv, pr.y);
-        uint32_t count = 0;
-        const bool boxa = a.tag == kCube || a.tag == kBox, boxb = b.tag == kCube || b.tag == kBox;
-        if (boxa && boxb) {
-            Manifold m{};
-            if (box_manifold(a, b, m)) {
+    const HullPool hp = stage_pool<LDS_POOL>(bv);
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (kWave - lane));
+    for (uint32_t base = blockIdx.x * kBlock; base < np; base += gridDim.x * kBlock) {
+        // the chunk's pairs by class: 0 SAT or sphere, 1 GJK, 2 past the list's end
+        const uint32_t q = base + threadIdx.x;
+        uint32_t cls = 2;
+        if (q < np) {
+            const uint2 pr = o.pairs[q];
+            const uint32_t ta = bv.tag[pr.x], tb = bv.tag[pr.y];
+            const bool bx = (ta == kCube || ta == kBox) && (tb == kCube || tb == kBox);
+            cls = (bx || (ta == kSphere && tb == kSphere)) ? 0u : 1u;
+        }
+        const uint64_t b0 = __ballot(cls == 0), b1 = __ballot(cls == 1);
+        if (lane == 0) {
+            s_wave[0][wave] = (uint32_t)__popcll(b0);
+            s_wave[1][wave] = (uint32_t)__popcll(b1);
+        }
+        __syncthreads();
+        uint32_t n0 = 0, before0 = 0, before1 = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            n0 += s_wave[0][w];
+            if (w < (int)wave) { before0 += s_wave[0][w]; before1 += s_wave[1][w]; }
+        }
+        if (cls == 0) s_pair[before0 + (uint32_t)__popcll(b0 & lt)] = q;
+        else if (cls == 1) s_pair[n0 + before1 + (uint32_t)__popcll(b1 & lt)] = q;
+        __syncthreads();
+        const uint32_t take = np - base < (uint32_t)kBlock ? np - base : (uint32_t)kBlock;
+        if (threadIdx.x < take) {
+            const uint32_t p = s_pair[threadIdx.x];
+            const uint2 pr = o.pairs[p];
+            const ShapeRef a = load_shape(bv, pr.x);
+            const ShapeRef b = load_shape(bv, pr.y);
+            uint32_t count = 0;
+            const bool boxa = a.tag == kCube || a.tag == kBox, boxb = b.tag == kCube || b.tag == kBox;
+            if (boxa && boxb) {
+                Manifold m{};
+                if (box_manifold(a, b, m)) {
 #pragma unroll
-                for (uint32_t k = 0; k < 4; ++k)
-                    if (k < m.count) write_slot(o, p * 4 + k, m.normal, m.depth, m.pa[k], m.pb[k]);
-                count = m.count;
-            }
-        } else if (a.tag == kSphere && b.tag == kSphere) {
-            const f3 delta = sub(b.pos, a.pos);
-            const float distance = len3(delta);
-            const float radius = a.c[0].x + b.c[0].x;
-            if (!(distance >= radius || distance <= 1.1920929e-7f)) {
-                const f3 n = divs(delta, distance);
-                write_slot(o, p * 4, n, radius - distance, add(a.pos, scale(n, a.c[0].x)),
-                           sub(b.pos, scale(n, b.c[0].x)));
-                count = 1;
-            }
-        } else {
-            Simplex s;
-            if (gjk(a, b, hp, s)) {
-                const uint32_t q = atomicAdd(o.gjk_count, 1u);
-                o.gjk_list[q] = p;
-                for (int k = 0; k < 4; ++k) {
-                    simplex_store[8 * (size_t)p + k] = f4(s.p[k], 0.0f);
-                    simplex_store[8 * (size_t)p + 4 + k] = f4(s.w[k], 0.0f);
+                    for (uint32_t k = 0; k < 4; ++k)
+                        if (k < m.count) write_slot(o, p * 4 + k, m.normal, m.depth, m.pa[k], m.pb[k]);
+                    count = m.count;
+                }
+            } else if (a.tag == kSphere && b.tag == kSphere) {
+                const f3 delta = sub(b.pos, a.pos);
+                const float distance = len3(delta);
+                const float radius = a.c[0].x + b.c[0].x;
+                if (!(distance >= radius || distance <= 1.1920929e-7f)) {
+                    const f3 n = divs(delta, distance);
+                    write_slot(o, p * 4, n, radius - distance, add(a.pos, scale(n, a.c[0].x)),
+                               sub(b.pos, scale(n, b.c[0].x)));
+                    count = 1;
+                }
+            } else {
+                Simplex s;
+                if (gjk(a, b, hp, s)) {
+                    const uint32_t at = atomicAdd(o.gjk_count, 1u);
+                    o.gjk_list[at] = p;
+                    for (int k = 0; k < 4; ++k) {
+                        simplex_store[8 * (size_t)p + k] = f4(s.p[k], 0.0f);
+                        simplex_store[8 * (size_t)p + 4 + k] = f4(s.w[k], 0.0f);
+                    }
                 }
             }
+            o.cnt[p] = count;
         }
-        o.cnt[p] = count;
+        __syncthreads();  // s_pair and s_wave are rewritten by the next chunk
     }
 }
 
@@ -227,29 +281,64 @@ template <int MV, int MF> struct EpaLds {
     uint32_t count;  // broadcast word
 };
 
+// Wave-wide reductions (all 64 lanes active).  Within each row of 16 lanes
+// four DPP steps (quad_perm xor 1, quad_perm xor 2, half-row mirror, row
+// mirror) leave every lane holding its row's result; the four rows' results
+// are then read from lanes 0, 16, 32 and 48.  The combining op returns one of
+// its operands (no NaNs reach it), so the result is one of the inputs.
+template <class Op> __device__ __forceinline__ uint32_t wave_reduce_bits(uint32_t x, Op op) {
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)x, 0);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)x, 16);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)x, 32);
+    const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+__device__ __forceinline__ float wave_max(float x) {
+    return __uint_as_float(wave_reduce_bits(__float_as_uint(x), [](uint32_t a, uint32_t b) {
+        return __float_as_uint(fmaxf(__uint_as_float(a), __uint_as_float(b)));
+    }));
+}
+__device__ __forceinline__ float wave_min(float x) {
+    return __uint_as_float(wave_reduce_bits(__float_as_uint(x), [](uint32_t a, uint32_t b) {
+        return __float_as_uint(fminf(__uint_as_float(a), __uint_as_float(b)));
+    }));
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+    return wave_reduce_bits(x, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
+}
+constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
+
 // support_local of a hull with the wavefront's lanes splitting the vertices:
 // the same dot products, and the vertex the sequential scan keeps (the first
-// maximal one, skipping NaNs; a NaN at vertex 0 is never replaced).
+// maximal one, skipping NaNs; a NaN at vertex 0 is never replaced).  Each
+// lane keeps its first maximal vertex; the wave's maximum value is reduced
+// alone, and the first vertex holding it is the lowest lane's (up to 64
+// vertices, where vertex i is lane i's) or the least index.
 __device__ __forceinline__ f3 support_local_wave(const ShapeRef& s, f3 d, const HullPool& hp, int lane) {
     if (s.tag != kHull) return support_local(s, d, hp);
     const uint32_t first = (uint32_t)s.c[0].x, count = (uint32_t)s.c[0].y;
-    float bd = 0.0f;
-    uint32_t bi = 0;
-    bool have = false;
+    float bd = -__builtin_inff();
+    uint32_t bi = kNoIndex;
     for (uint32_t i = (uint32_t)lane; i < count; i += kWave) {
         const float dd = dot3(xyz(hp.v[first + i]), d);
-        if (dd == dd && (!have || dd > bd)) { bd = dd; bi = i; have = true; }
+        if (dd == dd && (bi == kNoIndex || dd > bd)) { bd = dd; bi = i; }
     }
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const float od = __shfl_xor(bd, off, kWave);
-        const uint32_t oi = __shfl_xor(bi, off, kWave);
-        const bool oh = __shfl_xor((int)have, off, kWave) != 0;
-        if (oh && (!have || od > bd || (od == bd && oi < bi))) { bd = od; bi = oi; have = true; }
+    const float m = wave_max(bd);
+    const bool at_max = bi != kNoIndex && bd == m;
+    uint32_t win;
+    if (count <= (uint32_t)kWave) {
+        const uint64_t bal = __ballot(at_max);
+        win = bal ? (uint32_t)(__ffsll((long long)bal) - 1) : kNoIndex;
+    } else {
+        win = wave_min_u32(at_max ? bi : kNoIndex);
     }
     const float d0 = dot3(xyz(hp.v[first]), d);
-    if (!have || !(d0 == d0)) bi = 0;
-    return xyz(hp.v[first + bi]);
+    if (win == kNoIndex || !(d0 == d0)) win = 0;
+    return xyz(hp.v[first + win]);
 }
 
 // support_witness (gjk.rs:67-73) with wave-parallel hull supports
@@ -339,13 +428,13 @@ __device__ void face_contact(const L_t& L, uint32_t fi, f3& normal, float& depth
 // pair that would pass MV or MF (where the caps of epa.rs, 256 and 512, are
 // not yet reached) goes to the retry list instead of giving a result.
 // Otherwise (MV, MF = the caps of epa.rs): the pairs of the retry list.
-template <int MV, int MF, bool FIRST>
+template <int MV, int MF, bool FIRST, bool LDS_POOL>
 __global__ void __launch_bounds__(kWave) k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
     constexpr int kFacesPerLane = MF / kWave;
     __shared__ EpaLds<MV, MF> L;
     const int lane = threadIdx.x;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
-    const HullPool hp{bv.hull};
+    const HullPool hp = stage_pool<LDS_POOL>(bv);
     const uint32_t jobs = FIRST ? *o.gjk_count : *o.epa_retry_count;
     for (uint32_t job = blockIdx.x; job < jobs; job += gridDim.x) {
         const uint32_t p = FIRST ? o.gjk_list[job] : o.epa_retry_list[job];
@@ -377,30 +466,25 @@ __global__ void __launch_bounds__(kWave) k_epa(BodyView bv, NarrowOut o, const f
         int result_face = -1;
         bool retry = false;
         for (int it = 0; it < 64; ++it) {
-            // closest face: first minimal distance in index order (strict <)
-            float md = 0.0f;
-            uint32_t mi = 0;
-            bool have = false;
+            // closest face: first minimal distance in index order (strict <).
+            // The minimum value is reduced alone; the first face holding it is
+            // found by ballots, face group by face group (face k * 64 + lane)
+            float md = __builtin_inff();
+            float fdk[kFacesPerLane];
             for (int k = 0; k < kFacesPerLane; ++k) {
                 const uint32_t f = (uint32_t)(k * kWave + lane);
-                if (f < face_count) {
-                    const float d = L.fd[f];
-                    if (!have) {
-                        if (d == d) { md = d; mi = f; have = true; }
-                    } else if (d < md) {
-                        md = d; mi = f;
-                    }
-                }
+                fdk[k] = f < face_count ? L.fd[f] : __builtin_nanf("");
+                if (fdk[k] < md) md = fdk[k];
             }
-#pragma unroll
-            for (int off = 1; off < kWave; off <<= 1) {
-                const float od = __shfl_xor(md, off, kWave);
-                const uint32_t oi = __shfl_xor(mi, off, kWave);
-                const bool oh = __shfl_xor((int)have, off, kWave) != 0;
-                if (oh && (!have || od < md || (od == md && oi < mi))) { md = od; mi = oi; have = true; }
+            const float mval = wave_min(md);
+            uint32_t mi = kNoIndex;
+            for (int k = 0; k < kFacesPerLane && mi == kNoIndex; ++k) {
+                const uint64_t bal = __ballot(fdk[k] == mval);
+                if (bal) mi = (uint32_t)(k * kWave + __ffsll((long long)bal) - 1);
             }
-            // the sequential scan starts from face 0: a NaN there is never replaced
-            if (face_count == 0 || !(L.fd[0] == L.fd[0])) mi = 0;
+            // the sequential scan starts from face 0: a NaN there is never
+            // replaced (nor is face 0 when no distance is a number)
+            if (mi == kNoIndex || face_count == 0 || !(L.fd[0] == L.fd[0])) mi = 0;
             const f3 mn{L.fnx[mi], L.fny[mi], L.fnz[mi]};
             const float mdist = L.fd[mi];
             f3 sp, sw;
@@ -544,14 +628,22 @@ void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, h
     if (!counters_cleared) launch_fill2_u32(o.gjk_count, o.epa_retry_count, 0u, 1, s);
     uint32_t blocks = blocks_for(o.p_max > 0 ? o.p_max : 1);
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_narrow, dim3(blocks), dim3(kBlock), 0, s, bv, o, simplex_store);
+    const bool lds = bv.hull != nullptr && bv.hull_count > 0 && bv.hull_count <= kLdsPoolMax;
+    const size_t pool_bytes = lds ? (size_t)bv.hull_count * sizeof(float4) : 0;
+    if (lds) hipLaunchKernelGGL(k_narrow<true>, dim3(blocks), dim3(kBlock), pool_bytes, s, bv, o, simplex_store);
+    else hipLaunchKernelGGL(k_narrow<false>, dim3(blocks), dim3(kBlock), 0, s, bv, o, simplex_store);
     if (gjk_possible) {
         uint32_t eblocks = o.p_max > 0 ? o.p_max : 1;
         if (eblocks > 8192) eblocks = 8192;
-        hipLaunchKernelGGL((k_epa<kSmallV, kSmallF, true>), dim3(eblocks), dim3(kWave), 0, s, bv, o,
-                           (const float4*)simplex_store);
-        hipLaunchKernelGGL((k_epa<kMaxV, kMaxF, false>), dim3(eblocks < 512 ? eblocks : 512), dim3(kWave), 0, s, bv,
-                           o, (const float4*)simplex_store);
+        const uint32_t rblocks = eblocks < 512 ? eblocks : 512;
+        const float4* ss = simplex_store;
+        if (lds) {
+            hipLaunchKernelGGL((k_epa<kSmallV, kSmallF, true, true>), dim3(eblocks), dim3(kWave), pool_bytes, s, bv, o, ss);
+            hipLaunchKernelGGL((k_epa<kMaxV, kMaxF, false, true>), dim3(rblocks), dim3(kWave), pool_bytes, s, bv, o, ss);
+        } else {
+            hipLaunchKernelGGL((k_epa<kSmallV, kSmallF, true, false>), dim3(eblocks), dim3(kWave), 0, s, bv, o, ss);
+            hipLaunchKernelGGL((k_epa<kMaxV, kMaxF, false, false>), dim3(rblocks), dim3(kWave), 0, s, bv, o, ss);
+        }
     }
     launch_scan_exclusive(o.cnt, o.off, o.p_max, o.p_dev, o.scan_partials, s);
 }

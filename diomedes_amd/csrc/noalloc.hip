@@ -173,7 +173,7 @@ extern "C" dio_status dio_narrow_pairs_noalloc(int device, const dio_gpu_body* b
     o.p_dev = reinterpret_cast<const uint32_t*>(at(L.np));
     o.p_max = pair_count;
     if (scan_partials_size(pair_count + 1) > kPartialWords) return DIO_CAPACITY;
-    const BodyView bv{pos, rot, tag, corners, reinterpret_cast<const float4*>(at(L.hull))};
+    const BodyView bv{pos, rot, tag, corners, reinterpret_cast<const float4*>(at(L.hull)), hull_count};
     // counters_cleared = false: the EPA counters left by the last call are
     // zeroed by a fill kernel before the dispatch
     run_narrow(bv, o, reinterpret_cast<float4*>(at(L.simplex)), s, false, true);

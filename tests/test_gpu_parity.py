@@ -527,16 +527,20 @@ def test_narrow_convex_gjk_epa_bitexact(oracle, seed):
     assert hits > 50
 
 
-def test_narrow_epa_large_polytopes_bitexact(oracle):
-    """Deeply overlapping 642-vertex icosphere hulls: EPA polytopes outgrow
-    the first pass (64 vertices, 128 faces) and many reach the 256-vertex cap
-    of epa.rs; the retried pairs must give the full-size result."""
+@pytest.mark.parametrize("level", [3, 2])
+def test_narrow_epa_large_polytopes_bitexact(oracle, level):
+    """Deeply overlapping icosphere hulls of 642 (level 3) or 162 (level 2)
+    vertices: EPA polytopes outgrow the first pass (64 vertices, 128 faces)
+    and many reach the 256-vertex cap of epa.rs; the retried pairs must give
+    the full-size result.  The level-2 pool (324 vertices) is staged in LDS,
+    the level-3 pool (1284) is read in place; both have hulls of more than 64
+    vertices, whose wave-wide support takes the least index."""
     from diomedes_amd.physics import AvbdBody, Shape, gpu_bodies
     from diomedes_amd import records as RR
     from diomedes_amd.physics import Scene
 
     rng = np.random.default_rng(7)
-    ico = scenes.icosphere(3).astype(np.float32)
+    ico = scenes.icosphere(level).astype(np.float32)
     nv = len(ico)
     hull = np.concatenate([ico * np.float32(0.5), ico * np.float32(0.7)])
     bodies = []
