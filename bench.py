@@ -10,8 +10,11 @@ One "step" = broad phase + narrow phase + contact build + CSR + AVBD
 
 value = N_dyn * I * K / t_AVBD, where t_AVBD is the summed GPU time of the
 AVBD segment (HIP events around it in every step), over the K steps after W
-warm-up steps of the drop.  ms_per_step is the wall time of the whole K-step
-timed region / K.
+warm-up steps of the drop, with the library's default launch shapes.
+ms_per_step is the wall time of the whole K-step timed region / K.  The same
+window with dio_autotune run on the live world after the warm-up is reported
+beside it (autotuned_at_warmup): the shapes that win on the falling field can
+lose once the pile has settled.
 
 cube_field_300 runs the workload as specified, 300 steps from the drop, on
 the GPU and, over the same steps, the CPU baseline: the FP32 oracle (the C++
@@ -753,29 +756,45 @@ def main() -> int:
             tctx.autotune(opts, reps=20)
             tuning = tctx.tuning()
 
-    chosen = None
-    with PhysicsContext(n) as ctx:
-        if tuning:
-            ctx.set_tuning(**tuning)
-        tuning = ctx.tuning()
-        default_tuning = dict(tuning)
-        ctx.upload(scene)
-        if args.warmup:
-            ctx.step(opts, args.warmup)
-        _progress(f"warm-up done ({args.warmup} steps)")
-        if not args.no_tune and args.warmup and not args.tune_steps:
-            # the launch shapes for the timed window, chosen on the live world
-            chosen = ctx.autotune(opts, reps=20)
-            tuning = ctx.tuning()
-        ctx.step(opts, 0)  # synchronise (a zero-step call syncs the stream)
-        t0 = time.perf_counter()
-        ctx.step(opts, args.steps)  # returns after the stream is synchronised
-        _progress(f"timed region done ({args.steps} steps)")
-        wall = time.perf_counter() - t0
-        st = ctx.stats()
-        ms_avbd = st.ms_avbd
-        sum_m, sum_k, sum_p = st.sum_contacts, st.sum_csr_nnz, st.sum_pairs
-        last = {"pairs": st.pairs, "contacts": st.contacts, "csr_nnz": st.csr_nnz}
+    def timed_window(autotune: bool):
+        """Warm-up, then the timed window; with autotune, dio_autotune picks
+        the launch shapes on the live world between the two."""
+        chosen = None
+        with PhysicsContext(n) as ctx:
+            if tuning:
+                ctx.set_tuning(**tuning)
+            used = ctx.tuning()
+            ctx.upload(scene)
+            if args.warmup:
+                ctx.step(opts, args.warmup)
+            if autotune:
+                chosen = ctx.autotune(opts, reps=20)
+                used = ctx.tuning()
+            ctx.step(opts, 0)  # synchronise (a zero-step call syncs the stream)
+            t0 = time.perf_counter()
+            ctx.step(opts, args.steps)  # returns after the stream is synchronised
+            wall = time.perf_counter() - t0
+            return wall, ctx.stats(), used, chosen
+
+    # the headline runs the library's launch shapes (or those of --tune-steps):
+    # dio_autotune on the live world after the warm-up sees the falling field,
+    # whose best shapes (fewer lanes per body) lose once the pile has settled,
+    # so its window is timed separately and reported beside the headline
+    wall, st, tuning, _ = timed_window(False)
+    default_tuning = dict(tuning)
+    _progress(f"timed region done ({args.steps} steps)")
+    ms_avbd = st.ms_avbd
+    sum_m, sum_k, sum_p = st.sum_contacts, st.sum_csr_nnz, st.sum_pairs
+    last = {"pairs": st.pairs, "contacts": st.contacts, "csr_nnz": st.csr_nnz}
+    autotuned = None
+    if not args.no_tune and args.warmup and not args.tune_steps:
+        awall, ast_, aused, chosen = timed_window(True)
+        autotuned = {"value": n_dyn * iters * args.steps / (ast_.ms_avbd / 1000.0),
+                     "ms_per_step": 1000.0 * awall / args.steps, "ms_per_step_avbd": ast_.ms_avbd / args.steps,
+                     "tuning": aused,
+                     "autotune_us": {k2: chosen[k2] for k2 in ("us_per_launch", "broad_us_per_pass",
+                                                              "dual_schedule_us", "primal_order_us")}}
+        _progress(f"autotuned window done ({args.steps} steps)")
 
     k = args.steps
     value = n_dyn * iters * k / (ms_avbd / 1000.0)
@@ -808,10 +827,8 @@ def main() -> int:
             "window": f"steps {args.warmup}..{args.warmup + k} of the drop",
         },
         "tuning": tuning,
-        "tuning_source": ("dio_autotune on the live world after the warm-up" if chosen else
-                          "library defaults" if not args.tune_steps else f"dio_autotune after {args.tune_steps} steps"),
-        "autotune_us": ({k2: chosen[k2] for k2 in ("us_per_launch", "broad_us_per_pass", "dual_schedule_us",
-                                                  "primal_order_us")} if chosen else None),
+        "tuning_source": ("library defaults" if not args.tune_steps else f"dio_autotune after {args.tune_steps} steps"),
+        "autotuned_at_warmup": autotuned,
         "ms_per_step_avbd": ms_avbd / k,
         "body_iters_per_s_full_step": n_dyn * iters * k / wall,
         "avg_contacts": avg_m,
