@@ -214,14 +214,31 @@ def coloured_leg(args, tuning: dict) -> dict:
         ctx.step(opts, k, mode=1)
         wall = time.perf_counter() - t0
         st = ctx.stats()
+        # the settled pile: the coloured primal's lanes per body picked by
+        # dio_autotune there, then 60 more steps at library defaults and at the choice
+        settled = {}
+        if not args.no_tune:
+            chosen = ctx.autotune(opts, reps=10)
+            for name, lanes in (("library_defaults", 0), ("autotuned", chosen["colour_lanes"])):
+                ctx.set_tuning(colour_lanes=lanes)
+                ctx.step(opts, 1, mode=1)  # the first step after a tuning change captures the segment
+                t1 = time.perf_counter()
+                ctx.step(opts, 60, mode=1)
+                w1 = time.perf_counter() - t1
+                s1 = ctx.stats()
+                settled[name] = {"colour_lanes": lanes or ctx.tuning()["primal_lanes"],
+                                 "ms_per_step": 1000.0 * w1 / 60, "ms_per_step_avbd": s1.ms_avbd / 60}
+            settled["colour_us_per_iteration"] = chosen["colour_us_per_iteration"]
     _progress(f"coloured leg: {k} steps in {wall:.2f} s")
     return {
-        "workload": "the same cube field, coloured Gauss-Seidel primal (mode 1)",
+        "workload": "the same cube field, coloured Gauss-Seidel primal (mode 1), the coloured segment graph-captured "
+                    "after its first step",
         "value": scene.n_dynamic * opts.iterations * k / (st.ms_avbd / 1000.0),
         "unit": "body-iters/s",
         "ms_per_step": 1000.0 * wall / k,
         "ms_per_step_avbd": st.ms_avbd / k,
         "last_step": {"colours": st.colours, "colour_rounds": st.colour_rounds, "contacts": st.contacts},
+        "settled_colour_lanes": settled or None,
     }
 
 

@@ -271,7 +271,23 @@ __device__ __forceinline__ float4* lam_buf(const SolveView& v, uint32_t j) { ret
 // past its end: no body)
 __device__ __forceinline__ uint32_t body_at(const SolveView& v, const AvbdParams& p, uint32_t k) {
     if (!v.body_list) return k;
+    if (v.list_range) {
+        const uint32_t b0 = v.list_range[0], b1 = v.list_range[1];
+        return k < b1 - b0 ? v.body_list[b0 + k] : p.n;
+    }
     return k < v.list_count ? v.body_list[k] : p.n;
+}
+// The chunk of work items workgroup b takes: the XCD mapping below, or, for a
+// colour whose length only the device knows (a grid sized for every body),
+// chunk b itself, so the workgroups with work are spread over all the XCDs.
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb);
+__device__ __forceinline__ uint32_t work_chunk(const SolveView& v) {
+    return v.list_range ? blockIdx.x : xcd_chunk(blockIdx.x, gridDim.x);
+}
+// Whether a workgroup's first item lies past the end of a device-sized colour
+// (the whole workgroup then has nothing to do).
+__device__ __forceinline__ bool past_colour(const SolveView& v, uint32_t first_item) {
+    return v.list_range && first_item >= v.list_range[1] - v.list_range[0];
 }
 __device__ __forceinline__ float4* pen_buf(const SolveView& v, uint32_t j) { return (j & 1u) ? v.c_pen_alt : v.c_pen; }
 // The chunk of work items workgroup b takes.  Workgroups are handed to the 8
@@ -536,7 +552,7 @@ __device__ __forceinline__ void primal_one(const SolveView& v, const AvbdParams&
 
 template <bool FUSED>
 __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams p, uint32_t iter) {
-    const uint32_t i = body_at(v, p, xcd_chunk(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
+    const uint32_t i = body_at(v, p, work_chunk(v) * kBlock + threadIdx.x);
     if (i >= p.n) return;
     primal_one<FUSED>(v, p, iter, i);
 }
@@ -577,7 +593,8 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     __shared__ float4 body_f[kBodies][2];    // snapshot pos (.w: active), rot: for the solving thread
     const int slot = threadIdx.x / G;
     const int lane = threadIdx.x % G;
-    const uint32_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const uint32_t chunk = work_chunk(v);
+    if (past_colour(v, chunk * kBodies)) return;  // uniform over the workgroup, before any barrier
     const uint32_t i = body_at(v, p, chunk * kBodies + slot);
     const float4* __restrict__ rp = v.pos[parity];
     const float4* __restrict__ rr = v.rot[parity];
@@ -1117,6 +1134,20 @@ k_avbd_scenes(SolveView v, AvbdParams p, SceneQueue q, uint32_t iters) {
     }
 }
 
+// Colours first .. ncol - 1 (ncol = *ncol_at) one after another in one
+// workgroup, with a barrier between colours: the one-lane primal of every
+// body of the colour (bit-identical to every grouped shape).
+__global__ void __launch_bounds__(kBlock) k_avbd_colour_tail(SolveView v, AvbdParams p, uint32_t iter,
+                                                           const uint32_t* starts, uint32_t first,
+                                                           const uint32_t* ncol_at) {
+    const uint32_t ncol = *ncol_at;
+    for (uint32_t c = first; c < ncol; ++c) {
+        const uint32_t b0 = starts[c], b1 = starts[c + 1];
+        for (uint32_t t = b0 + threadIdx.x; t < b1; t += kBlock) primal_one<false>(v, p, iter, v.body_list[t]);
+        __syncthreads();  // the next colour reads these poses
+    }
+}
+
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
@@ -1185,6 +1216,16 @@ void launch_avbd_primal_colour(const SolveView& v, const AvbdParams& p, uint32_t
                                int lanes, const LaunchEvents& ev) {
     if (count == 0) return;
     launch_primal<false>(v, p, iter, s, lanes, ev, count);
+}
+void launch_avbd_primal_colour_range(const SolveView& v, const AvbdParams& p, uint32_t iter, hipStream_t s,
+                                     int lanes) {
+    if (p.n == 0) return;
+    launch_primal<false>(v, p, iter, s, lanes, LaunchEvents{}, p.n);
+}
+void launch_avbd_colour_tail(const SolveView& v, const AvbdParams& p, uint32_t iter, const uint32_t* starts,
+                             uint32_t first, uint32_t ncol_at, hipStream_t s) {
+    if (p.n == 0) return;
+    hipLaunchKernelGGL(k_avbd_colour_tail, dim3(1), dim3(kBlock), 0, s, v, p, iter, starts, first, starts + ncol_at);
 }
 static uint32_t contact_blocks(uint32_t work) {
     uint32_t blocks = blocks_for(work == 0 ? 1 : work);

@@ -64,10 +64,10 @@ __device__ __forceinline__ bool colour_visit(const ColourGraph& g, const ColourB
     return !colour_beats(g, o, i);
 }
 
-__global__ void __launch_bounds__(kBlock) k_colour_round(ColourGraph g, ColourBuffers b, uint32_t r) {
-    if (r > 0 && b.pending[r - 1] == 0u) return;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= g.n || b.round_of[i] != kColourUncoloured) return;
+// Body i's decision in round r.
+__device__ __forceinline__ void colour_round_body(const ColourGraph& g, const ColourBuffers& b, uint32_t i,
+                                                  uint32_t r) {
+    if (b.round_of[i] != kColourUncoloured) return;
     uint64_t m[4] = {0ull, 0ull, 0ull, 1ull << 63};  // colour 255 is never free
     bool best = true;
     for (uint32_t e = g.off[i], e1 = g.off[i + 1]; e < e1 && best; ++e) {
@@ -99,6 +99,59 @@ __global__ void __launch_bounds__(kBlock) k_colour_round(ColourGraph g, ColourBu
     b.round_of[i] = r;
 }
 
+__global__ void __launch_bounds__(kBlock) k_colour_round(ColourGraph g, ColourBuffers b, uint32_t r) {
+    if (r > 0 && b.pending[r - 1] == 0u) return;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < g.n) colour_round_body(g, b, i, r);
+}
+
+// Rounds first, first + 1, ... for as long as the round before left a body
+// uncoloured, in one workgroup with a barrier between rounds: the decisions
+// of the separate round launches (each reads only the state before its
+// round).  It finishes a colouring that a fixed batch of round launches (a
+// captured step) did not; when the batch sufficed it exits at once.  Past the
+// last round slot it sets the overflow flag.
+constexpr uint32_t kFinishBlock = 1024;
+__global__ void __launch_bounds__(kFinishBlock) k_colour_finish(ColourGraph g, ColourBuffers b, uint32_t first) {
+    __shared__ uint32_t more;
+    for (uint32_t r = first; r < kColourRoundSlots; ++r) {
+        __syncthreads();  // the previous round's flags and colours are written
+        if (threadIdx.x == 0) more = b.pending[r - 1];
+        __syncthreads();
+        if (more == 0u) return;  // uniform
+        if (r == kColourRoundSlots - 1) {
+            if (threadIdx.x == 0) b.pending[kColourRoundSlots - 1] = 1u;  // out of round slots
+            return;
+        }
+        for (uint32_t i = threadIdx.x; i < g.n; i += kFinishBlock) colour_round_body(g, b, i, r);
+    }
+}
+
+// Colour ranges of the sorted list: starts[c] .. starts[c + 1] holds colour c
+// (c < kMaxColours), and starts[kMaxColours + 1] the number of colours (the
+// largest colour in use + 1).  One workgroup, one count per thread.
+static_assert(kMaxColours + 1 == kBlock, "one colour count per thread");
+__global__ void __launch_bounds__(kBlock) k_colour_starts(ColourBuffers b, uint32_t* starts) {
+    __shared__ uint32_t sums[kBlock];
+    __shared__ uint32_t ncol;
+    const uint32_t t = threadIdx.x;
+    const uint32_t cnt = b.counts[t];
+    sums[t] = cnt;
+    if (t == 0) ncol = 0u;
+    __syncthreads();
+    for (uint32_t off = 1; off < (uint32_t)kBlock; off <<= 1) {
+        const uint32_t x = t >= off ? sums[t - off] : 0u;
+        __syncthreads();
+        sums[t] += x;
+        __syncthreads();
+    }
+    starts[t + 1] = sums[t];
+    if (t == 0) starts[0] = 0u;
+    if (t < kMaxColours && cnt != 0u) atomicMax(&ncol, t + 1);
+    __syncthreads();
+    if (t == 0) starts[kMaxColours + 1] = ncol;
+}
+
 // Sort keys (colour, or kMaxColours for a body not in the graph) and the
 // count per key: a block histogram in LDS, then one global add per (block,
 // key) -- a handful of colours would otherwise put every thread's atomic on
@@ -110,7 +163,10 @@ __global__ void __launch_bounds__(kBlock) k_colour_keys(ColourGraph g, ColourBuf
     __syncthreads();
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < g.n) {
-        const uint32_t key = b.round_of[i] == kColourNotInGraph ? kMaxColours : b.colour[i];
+        // a body left uncoloured (only past the last round slot, with the
+        // overflow flag set) still gets a key within the histogram
+        const uint32_t c = b.colour[i];
+        const uint32_t key = b.round_of[i] == kColourNotInGraph ? kMaxColours : (c < kMaxColours ? c : kMaxColours - 1);
         b.keys[0][i] = key;
         b.vals[0][i] = i;
         atomicAdd(&hist[key], 1u);
@@ -151,6 +207,15 @@ void launch_colour_rounds(const ColourGraph& g, const ColourBuffers& b, uint32_t
     if (g.n == 0) return;
     for (uint32_t r = first; r < first + count && r < kColourRoundSlots - 1; ++r)
         hipLaunchKernelGGL(k_colour_round, dim3(blocks_for(g.n)), dim3(kBlock), 0, s, g, b, r);
+}
+
+void launch_colour_finish(const ColourGraph& g, const ColourBuffers& b, uint32_t first, hipStream_t s) {
+    if (g.n == 0 || first == 0) return;
+    hipLaunchKernelGGL(k_colour_finish, dim3(1), dim3(kFinishBlock), 0, s, g, b, first);
+}
+
+void launch_colour_starts(const ColourBuffers& b, uint32_t* starts, hipStream_t s) {
+    hipLaunchKernelGGL(k_colour_starts, dim3(1), dim3(kBlock), 0, s, b, starts);
 }
 
 int launch_colour_lists(const ColourGraph& g, const ColourBuffers& b, hipStream_t s) {

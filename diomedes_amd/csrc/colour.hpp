@@ -57,8 +57,17 @@ void launch_colour_init(const ColourGraph& g, const ColourBuffers& b, hipStream_
 // round left nothing uncoloured).
 void launch_colour_rounds(const ColourGraph& g, const ColourBuffers& b, uint32_t first, uint32_t count,
                           hipStream_t s);
+// Rounds first, first + 1, ... in one workgroup while a body stays
+// uncoloured (first >= 1; exits at once when round first - 1 finished the
+// colouring).  Lets a fixed batch of round launches be captured.
+void launch_colour_finish(const ColourGraph& g, const ColourBuffers& b, uint32_t first, hipStream_t s);
 // Sort the bodies by colour and count each colour; returns the buffer index
 // (0 or 1) of keys / vals holding the sorted (colour, body) pairs.
 int launch_colour_lists(const ColourGraph& g, const ColourBuffers& b, hipStream_t s);
+// From the counts of the last launch_colour_lists: starts[c] .. starts[c + 1]
+// is colour c's range of the sorted list, starts[kMaxColours + 1] the number
+// of colours (kMaxColours + 2 words).
+constexpr uint32_t kColourStartsWords = kMaxColours + 2;
+void launch_colour_starts(const ColourBuffers& b, uint32_t* starts, hipStream_t s);
 
 }  // namespace dio

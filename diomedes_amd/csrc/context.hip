@@ -239,7 +239,7 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{32, 0, 16, 1, 8, 0, 1, 0};
+    dio_tuning tuning{32, 0, 16, 1, 8, 0, 1, 0, 0};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -315,6 +315,15 @@ struct dio_ctx {
     std::vector<hipEvent_t> events;
 
     void fail(const std::string& m) { err = m; }
+    // the coloured mode's captured AVBD segment (per parity), built with caps
+    // on the colouring rounds and on the colours launched one by one, and the
+    // device-side colour ranges it reads
+    hipGraphExec_t col_graphs[2] = {nullptr, nullptr};
+    bool col_valid = false;
+    uint32_t col_rcap = 0, col_ccap = 0;
+    dio_avbd_options col_opts{};
+    dio_step_opts col_sopts{};
+    uint32_t* col_starts = nullptr;
     void drop_graphs() {
         for (auto& seg : graphs)
             for (auto& g : seg)
@@ -323,6 +332,12 @@ struct dio_ctx {
                     g = nullptr;
                 }
         graphs_valid = false;
+        for (auto& g : col_graphs)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        col_valid = false;
     }
     // adjoint: the recording of dio_step calls and the replay / backward scratch
     AdjRecord adj_rec{};
@@ -581,10 +596,13 @@ void schedule_avbd(const SolveView& v, const AvbdParams& p, uint32_t iters, uint
 // per iteration one primal launch per colour over that colour's body list,
 // poses updated in place (v.pos[0] == v.pos[1]), then the dual update; the
 // last dual update with recover.  start[c] .. start[c + 1] index the list.
+// Lanes per body of the coloured primal.
+int colour_lanes_of(const dio_tuning& t) { return (int)(t.colour_lanes ? t.colour_lanes : t.primal_lanes); }
+
 void schedule_avbd_coloured(SolveView v, const AvbdParams& p, uint32_t iters, uint32_t m_grid, hipStream_t s,
                             KernelTimer* kt, const dio_tuning& t, const uint32_t* list,
                             const std::vector<uint32_t>& start) {
-    const int lanes = (int)t.primal_lanes;
+    const int lanes = colour_lanes_of(t);
     auto run = [&](int kind, auto&& launch) {
         if (kt) kt->time(kind, launch);
         else launch(LaunchEvents{});
@@ -1036,6 +1054,7 @@ void ensure_world(dio_ctx* c) {
     c->w_stage_c = A.alloc<dio_avbd_constraint>(c->c_cap);
     colour_buffers_bind(c->wc, A.alloc<uint32_t>(colour_buffer_words(n)), n);
     c->wc_host = c->pin.alloc<uint32_t>(kColourRoundSlots + kMaxColours + 2);
+    c->col_starts = A.alloc<uint32_t>(kColourStartsWords);
     c->w_ready = true;
 }
 
@@ -1160,11 +1179,10 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
 // The AVBD segment of a coloured step: colour the step's body graph, read
 // back the colour counts (the one host synchronisation of the mode), then
 // launch the solve colour by colour.
-void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const dio_step_opts& so, KernelTimer* kt) {
-    hipStream_t st = c->stream;
-    const uint32_t n = c->w_n;
+// The step's body graph for the colouring.
+ColourGraph colour_graph_of(const dio_ctx* c) {
     ColourGraph g;
-    g.n = n;
+    g.n = c->w_n;
     g.inv_mm = c->w_inv_mm;
     g.off = c->csr.off;
     g.peer = c->csr.peer;
@@ -1177,6 +1195,13 @@ void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const di
         g.scene_of = c->w_scene_of;
         g.scene_off = c->w_scene_off;
     }
+    return g;
+}
+
+void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const dio_step_opts& so, KernelTimer* kt) {
+    hipStream_t st = c->stream;
+    const uint32_t n = c->w_n;
+    const ColourGraph g = colour_graph_of(c);
     launch_colour_init(g, c->wc, st);
     // the first batch covers the previous coloured step's rounds and two more
     // (the graph changes little from step to step); later batches 16 each
@@ -1212,6 +1237,97 @@ void run_coloured_segment(dio_ctx* c, int s, const dio_avbd_options& o, const di
     v.stat_acc = c->w_acc;
     const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n, o.gamma, o.penalty_min);
     schedule_avbd_coloured(v, p, o.iterations, c->m_cap, st, kt, c->tuning, c->wc.vals[lb], start);
+}
+
+// The coloured AVBD segment with no host synchronisation, for capture: the
+// first col_rcap colouring rounds as separate launches (each exits at once
+// when the colouring is done) and one workgroup finishing any later rounds;
+// the per-colour ranges computed on the device; then per iteration one
+// launch for each of colours 0 .. col_ccap - 1 (a grid for every body,
+// workgroups past the colour's end exiting at once) and one workgroup
+// running any colours past those, one by one.  The caps come from an
+// uncaptured step, so the finishing launches do nothing unless the
+// colouring needs more rounds or colours than then; the results are the
+// uncaptured step's either way.
+void enqueue_coloured_captured(dio_ctx* c, int s, const dio_avbd_options& o, const dio_step_opts& so) {
+    hipStream_t st = c->stream;
+    const uint32_t n = c->w_n;
+    const ColourGraph g = colour_graph_of(c);
+    launch_colour_init(g, c->wc, st);
+    launch_colour_rounds(g, c->wc, 0, c->col_rcap, st);
+    launch_colour_finish(g, c->wc, c->col_rcap, st);
+    const int lb = launch_colour_lists(g, c->wc, st);
+    launch_colour_starts(c->wc, c->col_starts, st);
+    SolveView v = world_solve_view(c, s);
+    const int bn_pend = so.contact_lag ? 1 - s : s;
+    v.stat_pairs = c->pend[bn_pend].p_dev;
+    v.stat_acc = c->w_acc;
+    const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n, o.gamma, o.penalty_min);
+    const uint32_t* list = c->wc.vals[lb];
+    const int lanes = colour_lanes_of(c->tuning);
+    v.pos[1] = v.pos[0];
+    v.rot[1] = v.rot[0];
+    launch_avbd_init(v, p, st);
+    for (uint32_t k = 0; k < o.iterations; ++k) {
+        for (uint32_t col = 0; col < c->col_ccap; ++col) {
+            SolveView vc = v;
+            vc.body_list = list;
+            vc.list_range = c->col_starts + col;
+            launch_avbd_primal_colour_range(vc, p, k, st, lanes);
+        }
+        SolveView vt = v;
+        vt.body_list = list;
+        launch_avbd_colour_tail(vt, p, k, c->col_starts, c->col_ccap, kMaxColours + 1, st);
+        if (k + 1 < o.iterations) launch_avbd_dual(v, p, k & 1u, 0u, 0u, c->m_cap, st);
+    }
+    launch_avbd_dual_recover(v, p, o.iterations, false, c->m_cap, st);
+}
+
+// Capture the coloured segment of both parities with caps from the last
+// uncaptured coloured step: its rounds plus 4 (at least 8) and its colours
+// plus 2.
+void build_colour_graphs(dio_ctx* c, const dio_avbd_options& o, const dio_step_opts& so) {
+    for (auto& g : c->col_graphs)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    c->col_valid = false;
+    c->col_rcap = std::max<uint32_t>(8u, c->last_rounds + 4u);
+    c->col_ccap = c->last_colours + 2u;
+    DIO_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < 2; ++s) {
+        hipGraph_t g = nullptr;
+        DIO_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_coloured_captured(c, s, o, so);
+        DIO_HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+        hipGraphExec_t ge = nullptr;
+        hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        DIO_HIP_CHECK(e);
+        c->col_graphs[s] = ge;
+    }
+    c->col_opts = o;
+    c->col_sopts = so;
+    c->col_valid = true;
+}
+
+// After captured coloured steps: the last step's colour and round counts
+// (read back once the stream is idle); caps that were passed invalidate the
+// graphs, so the next call captures with larger ones.
+void read_colour_summary(dio_ctx* c) {
+    uint32_t* h = c->wc_host;
+    DIO_HIP_CHECK(hipStreamSynchronize(c->stream));
+    DIO_HIP_CHECK(hipMemcpy(h, c->wc.pending, (size_t)kColourRoundSlots * 4, hipMemcpyDeviceToHost));
+    DIO_HIP_CHECK(hipMemcpy(h + kColourRoundSlots, c->col_starts, (size_t)kColourStartsWords * 4,
+                            hipMemcpyDeviceToHost));
+    if (h[kColourRoundSlots - 1])
+        capacity("the colouring needs more rounds than it can track, or a body more than 254 colours", kMaxColours);
+    uint32_t r = 0;
+    while (r < kColourRoundSlots - 1 && h[r] != 0u) ++r;
+    c->last_rounds = c->w_n > 0 ? r + 1 : 0;
+    c->last_colours = h[kColourRoundSlots + kMaxColours + 1];
+    if (c->last_rounds > c->col_rcap || c->last_colours > c->col_ccap) c->col_valid = false;
 }
 
 // The per-scene solve's bundles for tuning.scene_solve (a batched world
@@ -1537,6 +1653,8 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
         const bool host_timing = std::getenv("DIO_HOST_TIMING") != nullptr;
         const auto h0 = std::chrono::steady_clock::now();
         double h_launch = 0.0;
+        bool colour_graph_last = false;  // the last step ran the captured coloured segment
+        uint32_t colour_graph_run = 0;   // captured coloured steps since the last look at the caps
         for (uint32_t k = 0; k < nsteps && ctx->w_n > 0; ++k) {
             const int par = (int)(ctx->w_step % 2);
             const bool resort = ctx->w_sort_pending || ctx->w_step - ctx->w_sorted_at >= ctx->tuning.resort_interval;
@@ -1553,8 +1671,27 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
                 DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[resort ? 2 : 0][par], s));
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[2 + 2 * k], s));
                 if (need_init) run_world_init(ctx, par, *options, nullptr);
-                if (coloured) run_coloured_segment(ctx, par, *options, so, nullptr);
-                else DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[1][par], s));
+                if (coloured) {
+                    // the captured coloured segment once an uncaptured step has
+                    // given its caps; an uncaptured step otherwise, which then
+                    // captures it for the following steps
+                    colour_graph_last = ctx->col_valid && same_opts(ctx->col_opts, *options, ctx->col_sopts, so);
+                    if (colour_graph_last) {
+                        DIO_HIP_CHECK(hipGraphLaunch(ctx->col_graphs[par], s));
+                        // every 16 captured steps, one look at the colouring: when it
+                        // has outgrown the caps (the extra rounds and colours then run
+                        // in one workgroup), the next step captures with larger ones
+                        if (++colour_graph_run == 16 && k + 1 < nsteps) {
+                            read_colour_summary(ctx);
+                            colour_graph_run = 0;
+                        }
+                    } else {
+                        run_coloured_segment(ctx, par, *options, so, nullptr);
+                        build_colour_graphs(ctx, *options, so);
+                    }
+                } else {
+                    DIO_HIP_CHECK(hipGraphLaunch(ctx->graphs[1][par], s));
+                }
                 DIO_HIP_CHECK(hipEventRecord(ctx->events[3 + 2 * k], s));
                 h_launch += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
             } else {
@@ -1588,6 +1725,7 @@ dio_status dio_step(dio_ctx* ctx, const dio_avbd_options* options, const dio_ste
                          "wait %.3f ms\n", nsteps, std::chrono::duration<double, std::milli>(h1 - h0).count(),
                          h_launch, std::chrono::duration<double, std::milli>(h2 - h1).count());
         }
+        if (coloured && colour_graph_last) read_colour_summary(ctx);
         // stats of the last step
         dio_step_stats& st = ctx->stats;
         std::memset(&st, 0, sizeof(st));
@@ -1669,6 +1807,9 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         if (v.overlap_broad > 1) invalid("overlap_broad must be 0 or 1");
         if (v.primal_order > 1) invalid("primal_order must be 0 (index) or 1 (Morton)");
         if (v.scene_solve > 65536) invalid("scene_solve must be 0 (off) or the bodies per bundle, 1 .. 65536");
+        if (v.colour_lanes != 0 && v.colour_lanes != 1 && v.colour_lanes != 4 && v.colour_lanes != 8 &&
+            v.colour_lanes != 16 && v.colour_lanes != 32 && v.colour_lanes != 64)
+            invalid("colour_lanes must be 0 (primal_lanes), 1, 4, 8, 16, 32 or 64");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
@@ -1795,6 +1936,62 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
             us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 3] = order_us[1];
         }
         ctx->tuning.primal_order = order_us[1] < order_us[0] ? 1u : 0u;
+        // the coloured primal: a colouring of the current contact graph (in the
+        // coloured mode's buffers), then one iteration's launches, colour by
+        // colour, at each shape; poses are updated in place in the Jacobi
+        // buffer the primal timings above write
+        if (ctx->w_n > 0) {
+            const ColourGraph g = colour_graph_of(ctx);
+            uint32_t* h = ctx->wc_host;
+            launch_colour_init(g, ctx->wc, s);
+            for (uint32_t r = 0;; r += 16) {
+                if (r + 16 > kColourRoundSlots - 1) capacity("the colouring needs more rounds than it can track", r);
+                launch_colour_rounds(g, ctx->wc, r, 16, s);
+                DIO_HIP_CHECK(hipMemcpyAsync(h, ctx->wc.pending + r, 16 * 4, hipMemcpyDeviceToHost, s));
+                DIO_HIP_CHECK(hipStreamSynchronize(s));
+                if (h[15] == 0u) break;
+            }
+            const int lb = launch_colour_lists(g, ctx->wc, s);
+            DIO_HIP_CHECK(hipMemcpyAsync(h, ctx->wc.counts, (kMaxColours + 1) * 4, hipMemcpyDeviceToHost, s));
+            DIO_HIP_CHECK(hipStreamSynchronize(s));
+            std::vector<uint32_t> start{0u};
+            uint32_t ncol = 0;
+            for (uint32_t k = 0; k < kMaxColours; ++k)
+                if (h[k]) ncol = k + 1;
+            for (uint32_t k = 0; k < ncol; ++k) start.push_back(start.back() + h[k]);
+            SolveView vc0 = v;
+            vc0.pos[1] = vc0.pos[0];
+            vc0.rot[1] = vc0.rot[0];
+            float cbest = 3.4e38f;
+            int cbest_lanes = 0;
+            for (int k = 0; k < DIO_TUNE_COLOUR_SHAPES; ++k) {
+                auto once = [&]() {
+                    for (uint32_t col = 0; col < ncol; ++col) {
+                        const uint32_t count = start[col + 1] - start[col];
+                        if (count == 0) continue;
+                        SolveView vc = vc0;
+                        vc.body_list = ctx->wc.vals[lb] + start[col];
+                        vc.list_count = count;
+                        launch_avbd_primal_colour(vc, p, 1, count, s, shapes[k]);
+                    }
+                };
+                once();
+                DIO_HIP_CHECK(hipEventRecord(e0, s));
+                for (uint32_t r = 0; r < reps; ++r) once();
+                DIO_HIP_CHECK(hipEventRecord(e1, s));
+                DIO_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                DIO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                const float us = 1000.0f * ms / (float)reps;
+                if (us_per_launch)
+                    us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4 + k] = us;
+                if (us < cbest) {
+                    cbest = us;
+                    cbest_lanes = shapes[k];
+                }
+            }
+            ctx->tuning.colour_lanes = (uint32_t)cbest_lanes;
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         ctx->tuning.primal_lanes = (uint32_t)best_lanes;

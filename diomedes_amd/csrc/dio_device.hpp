@@ -96,9 +96,12 @@ struct SolveView {
     float4* out_pos;
     float4* out_rot;
     // coloured Gauss-Seidel: the primal runs over body_list[0 .. list_count)
-    // (one colour) instead of every body; pos[0] / pos[1] then alias
+    // (one colour) instead of every body; pos[0] / pos[1] then alias.  When
+    // list_range is set (a captured step), the colour is body_list[list_range[0]
+    // .. list_range[1]), bounds read on the device, and list_count is unused.
     const uint32_t* body_list;
     uint32_t list_count;
+    const uint32_t* list_range;
     // when next.init_lin is set, the final launch (dual + recover) also runs
     // the next step's init from the recovered state into these buffers
     InitTarget next;
@@ -129,6 +132,17 @@ void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, 
 // poses read and written in place (v.pos[0] == v.pos[1]).
 void launch_avbd_primal_colour(const SolveView& v, const AvbdParams& p, uint32_t iter, uint32_t count,
                                hipStream_t s, int lanes = 32, const LaunchEvents& ev = LaunchEvents{});
+// The same over the colour v.list_range names on the device; the grid covers
+// every body (the most one colour can hold) and workgroups past the colour's
+// end exit at once.
+void launch_avbd_primal_colour_range(const SolveView& v, const AvbdParams& p, uint32_t iter, hipStream_t s,
+                                     int lanes = 32);
+// Colours first .. starts[ncol_at] - 1 of the sorted list body_list (ranges
+// starts[c] .. starts[c + 1]), one after another in one workgroup with the
+// one-lane primal: the colours a captured step has no launch for (none
+// unless the colouring needs more colours than when it was captured).
+void launch_avbd_colour_tail(const SolveView& v, const AvbdParams& p, uint32_t iter, const uint32_t* starts,
+                             uint32_t first, uint32_t ncol_at, hipStream_t s);
 void launch_avbd_dual(const SolveView& v, const AvbdParams& p, uint32_t parity, uint32_t src, uint32_t dst,
                       uint32_t m_grid, hipStream_t stream, const LaunchEvents& ev = {});
 void launch_avbd_dual_recover(const SolveView& v, const AvbdParams& p, uint32_t iters, bool fused, uint32_t m_grid,

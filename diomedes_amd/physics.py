@@ -628,7 +628,8 @@ class PhysicsContext:
 
     def set_tuning(self, primal_lanes: int = 0, broad_lanes: int = 0, resort_interval: int = 0,
                    fuse_dual: int | None = None, adjoint_lanes: int = 0, overlap_broad: int | None = None,
-                   primal_order: int | None = None, scene_solve: int | None = None) -> None:
+                   primal_order: int | None = None, scene_solve: int | None = None,
+                   colour_lanes: int | None = None) -> None:
         """Launch shapes and schedule (0 / None keeps the current value)."""
         cur = self.tuning()
         t = R.Tuning()
@@ -640,6 +641,7 @@ class PhysicsContext:
         t.overlap_broad = cur["overlap_broad"] if overlap_broad is None else int(overlap_broad)
         t.primal_order = cur["primal_order"] if primal_order is None else int(primal_order)
         t.scene_solve = cur["scene_solve"] if scene_solve is None else int(scene_solve)
+        t.colour_lanes = cur["colour_lanes"] if colour_lanes is None else int(colour_lanes)
         self._check(self._lib.dio_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -647,14 +649,14 @@ class PhysicsContext:
         self._check(self._lib.dio_get_tuning(self._h, ctypes.byref(t)))
         return {"primal_lanes": t.primal_lanes, "broad_lanes": t.broad_lanes, "resort_interval": t.resort_interval,
                 "fuse_dual": t.fuse_dual, "adjoint_lanes": t.adjoint_lanes, "overlap_broad": t.overlap_broad,
-                "primal_order": t.primal_order, "scene_solve": t.scene_solve}
+                "primal_order": t.primal_order, "scene_solve": t.scene_solve, "colour_lanes": t.colour_lanes}
 
     def autotune(self, options: AvbdOptions, reps: int = 20) -> dict:
         """Time every primal and broad-phase launch shape and both dual
         schedules on the current world, keep the fastest; returns the choice
         and the per-shape microseconds."""
         t = R.Tuning()
-        us = (ctypes.c_float * 16)()
+        us = (ctypes.c_float * 22)()
         o = options.to_c()
         self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
         u = list(us)
@@ -667,6 +669,8 @@ class PhysicsContext:
             "dual_schedule_us": {"unfused": u[12], "fused": u[13]},
             "primal_order": t.primal_order,
             "primal_order_us": {"index": u[14], "morton": u[15]},
+            "colour_lanes": t.colour_lanes,
+            "colour_us_per_iteration": dict(zip((1, 4, 8, 16, 32, 64), u[16:22])),
         }
 
     def stats(self) -> R.StepStats:

@@ -238,7 +238,8 @@ typedef struct dio_step_stats {
 /* Kernel launch shapes (the autotuner's search space). */
 #define DIO_TUNE_PRIMAL_SHAPES 6
 #define DIO_TUNE_BROAD_SHAPES 6
-#define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4)
+#define DIO_TUNE_COLOUR_SHAPES 6
+#define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4 + DIO_TUNE_COLOUR_SHAPES)
 typedef struct dio_tuning {
     uint32_t primal_lanes;     /* lanes per body in the primal: 1, 4, 8, 16, 32 or 64 (0 = default 32) */
     uint32_t broad_lanes;      /* lanes per query in the BVH traversal: 1, 2, 4, 8, 16 or 32; with 16
@@ -269,6 +270,8 @@ typedef struct dio_tuning {
                                   result is the same either way); 0 (default): one launch per primal
                                   and dual over the whole world.  At most 65536.  Ignored for other
                                   worlds */
+    uint32_t colour_lanes;     /* lanes per body in the coloured mode's primal (one launch per colour):
+                                  1, 4, 8, 16, 32 or 64; 0 (default): primal_lanes */
 } dio_tuning;
 
 typedef struct dio_ctx dio_ctx;
@@ -385,10 +388,15 @@ dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out);
    pass); then, at the chosen primal shape, an unfused primal plus a dual
    launch and a fused primal (fuse_dual is set to the faster); then that
    primal in index order and in Morton order (primal_order is set to the
-   faster; the shapes above are timed in the order current at the call).  The
-   primal with a fixed parity is idempotent and the broad timing runs in the
-   granular broad buffers, so the world state is unchanged; a granular broad
-   result submitted before the call must be submitted again. */
+   faster; the shapes above are timed in the order current at the call); then
+   the coloured mode's primal, one launch per colour of a colouring of the
+   current contact graph, at lanes 1, 4, 8, 16, 32, 64 (microseconds per
+   iteration over all colours; colour_lanes is set to the fastest).  The
+   primal with a fixed parity is idempotent, the broad timing runs in the
+   granular broad buffers and the colouring in the coloured mode's buffers,
+   so the world state is unchanged (dio_world_colours then reports that
+   colouring); a granular broad result submitted before the call must be
+   submitted again. */
 dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t reps, dio_tuning* chosen,
                         float* us_per_launch);
 /* ---- adjoint of the AVBD step (no counterpart in the reference) -------

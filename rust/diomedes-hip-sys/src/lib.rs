@@ -55,7 +55,8 @@ pub const DIO_NOALLOC_MAX_BODIES: u32 = 16384;
 pub const DIO_NOALLOC_MAX_PAIRS: u32 = 65536;
 pub const DIO_NOALLOC_MAX_HULL_VERTICES: u32 = 16384;
 pub const DIO_NOALLOC_SCRATCH_BYTES: u32 = 47453184;
-pub const DIO_TUNE_SLOTS: usize = 16;
+pub const DIO_TUNE_COLOUR_SHAPES: u32 = 6;
+pub const DIO_TUNE_SLOTS: usize = 22;
 pub const DIO_ADJOINT_EXACT: u32 = 0;
 pub const DIO_ADJOINT_FROZEN_MULTIPLIERS: u32 = 1;
 
@@ -242,6 +243,7 @@ pub struct dio_tuning {
     pub overlap_broad: u32,
     pub primal_order: u32,
     pub scene_solve: u32,
+    pub colour_lanes: u32,
 }
 
 /// The engine's names of the records it already defines (render/compute.rs, render/gpu_physics.rs).
@@ -267,7 +269,7 @@ const _: () = assert!(size_of::<dio_container>() == 36);
 const _: () = assert!(size_of::<dio_limits>() == 20);
 const _: () = assert!(size_of::<dio_step_opts>() == 12);
 const _: () = assert!(size_of::<dio_step_stats>() == 96);
-const _: () = assert!(size_of::<dio_tuning>() == 32);
+const _: () = assert!(size_of::<dio_tuning>() == 36);
 
 #[link(name = "diomedes_hip")]
 extern "C" {

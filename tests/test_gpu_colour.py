@@ -64,6 +64,55 @@ def test_coloured_step_bitexact(oracle, name, lanes):
                    st.contacts)
 
 
+@pytest.mark.parametrize("lanes", [32, 1])
+def test_captured_coloured_step_equals_uncaptured(lanes, monkeypatch):
+    """The coloured segment is captured after its first step, with caps on the
+    colouring rounds and on the colours launched one by one taken from that
+    step; later steps that need more (the field starts falling apart, with one
+    colour, and piles up) finish the colouring in one workgroup and run the
+    extra colours one by one.  Every step equals the uncaptured path (no
+    graphs) bit for bit, and the colour and round counts agree."""
+    sc = _scene("field")
+    res = {}
+    for graphs in (True, False):
+        if graphs:
+            monkeypatch.delenv("DIO_NO_GRAPH", raising=False)
+        else:
+            monkeypatch.setenv("DIO_NO_GRAPH", "1")
+        with PhysicsContext(sc.n) as ctx:
+            ctx.set_tuning(primal_lanes=lanes)
+            ctx.upload(sc)
+            out = []
+            for chunk in (1, 10, 25, 25):
+                ctx.step(sc.options, chunk, mode=1)
+                st = ctx.stats()
+                out.append((ctx.download(), ctx.colours(), st.colours, st.colour_rounds))
+            res[graphs] = out
+    for k, ((g, gc, gn, gr), (u, uc, un, ur)) in enumerate(zip(res[True], res[False])):
+        for key in ("positions", "orientations", "velocities", "angular_velocities"):
+            assert_bits(g[key], u[key], f"call {k} {key}")
+        assert np.array_equal(gc[0], uc[0]) and gc[1] == uc[1]
+        assert (gn, gr) == (un, ur), (k, gn, gr, un, ur)
+        _print(f"lanes {lanes} call {k}: colours {gn}, rounds {gr}: captured = uncaptured")
+    assert res[True][0][2] == 1 and res[True][-1][2] > 2  # the colour count grew past the first capture's cap
+
+
+def test_colour_lanes_apart_from_primal_lanes(oracle):
+    """colour_lanes sets the coloured primal's shape on its own (here 4 lanes
+    per body against the Jacobi primal's 32): still the oracle's step."""
+    sc = _scene("field")
+    w = oracle.OracleWorld(sc, 32)
+    with PhysicsContext(sc.n) as ctx:
+        ctx.set_tuning(primal_lanes=32, colour_lanes=4)
+        ctx.upload(sc)
+        for mode in (0, 1, 1):
+            w.set_mode(mode)
+            ctx.step(sc.options, 12, mode=mode)
+            w.step(nsteps=12)
+        _compare(ctx, w, "colour lanes 4")
+        assert ctx.tuning()["colour_lanes"] == 4
+
+
 def test_modes_alternate(oracle):
     """Jacobi and coloured steps can follow each other on one world."""
     sc = _scene("field")

@@ -397,10 +397,22 @@ def test_autotune_keeps_state_and_parity(oracle):
         assert res["broad_lanes"] in (1, 2, 4, 8, 16, 32)
         assert res["fuse_dual"] in (0, 1)
         assert res["primal_order"] in (0, 1) and ctx.tuning()["primal_order"] == res["primal_order"]
+        assert res["colour_lanes"] in (1, 4, 8, 16, 32, 64) and ctx.tuning()["colour_lanes"] == res["colour_lanes"]
+        assert all(v > 0 for v in res["colour_us_per_iteration"].values())
         _compare_world(ctx, w, 40)
+        # coloured steps at the chosen colour lanes keep parity too
+        w.set_mode(1)
+        ctx.step(scene.options, 12, mode=1)
+        w.step(nsteps=12)
+        _compare_world(ctx, w, 52)
+        from diomedes_amd._lib import DioError
+
+        with pytest.raises(DioError):
+            ctx.set_tuning(colour_lanes=3)
+        w.set_mode(0)
         ctx.step(scene.options, 20)
         w.step(nsteps=20)
-        _compare_world(ctx, w, 60)
+        _compare_world(ctx, w, 72)
 
 
 def test_cube_field_8192_first_steps_bitexact(oracle):
