@@ -44,15 +44,28 @@ DIO_HD f3 support_local(const ShapeRef& s, f3 d, const HullPool& hp) {
             return best;
         }
         case kHull: {
+            // the first maximal vertex (strict >, in vertex order; a NaN at
+            // vertex 0 is never replaced).  Four vertices per round: their
+            // loads and dot products are independent, and the comparisons
+            // keep the scan's order.
             const uint32_t first = (uint32_t)s.c[0].x, count = (uint32_t)s.c[0].y;
-            f3 best = xyz(hp.v[first]);
-            float bd = dot3(best, d);
-            for (uint32_t i = 1; i < count; ++i) {
-                const f3 cc = xyz(hp.v[first + i]);
-                const float dd = dot3(cc, d);
-                if (dd > bd) { bd = dd; best = cc; }
+            const float4* v = hp.v + first;
+            uint32_t bi = 0;
+            float bd = dot3(xyz(v[0]), d);
+            uint32_t i = 1;
+            for (; i + 4 <= count; i += 4) {
+                const float d0 = dot3(xyz(v[i]), d), d1 = dot3(xyz(v[i + 1]), d);
+                const float d2 = dot3(xyz(v[i + 2]), d), d3 = dot3(xyz(v[i + 3]), d);
+                if (d0 > bd) { bd = d0; bi = i; }
+                if (d1 > bd) { bd = d1; bi = i + 1; }
+                if (d2 > bd) { bd = d2; bi = i + 2; }
+                if (d3 > bd) { bd = d3; bi = i + 3; }
             }
-            return best;
+            for (; i < count; ++i) {
+                const float dd = dot3(xyz(v[i]), d);
+                if (dd > bd) { bd = dd; bi = i; }
+            }
+            return xyz(v[bi]);
         }
         default:
             return f3{0.0f, 0.0f, 0.0f};
@@ -92,9 +105,49 @@ DIO_HD f3 world_support(const ShapeRef& s, f3 d, const HullPool& hp) {
 }
 
 // broadphase/src/lib.rs:123-151
+// A hull's six axis supports in one pass over its vertices: per direction
+// the first maximal vertex, as six separate scans would find it.
+DIO_HD void hull_axis_supports(const ShapeRef& s, const HullPool& hp, f3 out[6]) {
+    const f3 axes[6] = {f3{1.0f, 0.0f, 0.0f}, neg(f3{1.0f, 0.0f, 0.0f}), f3{0.0f, 1.0f, 0.0f},
+                        neg(f3{0.0f, 1.0f, 0.0f}), f3{0.0f, 0.0f, 1.0f}, neg(f3{0.0f, 0.0f, 1.0f})};
+    const uint32_t first = (uint32_t)s.c[0].x, count = (uint32_t)s.c[0].y;
+    const float4* v = hp.v + first;
+    f3 dir[6];
+    float bd[6];
+    uint32_t bi[6];
+    const f3 v0 = xyz(v[0]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        dir[k] = qrot_inv(s.rot, axes[k]);
+        bd[k] = dot3(v0, dir[k]);
+        bi[k] = 0;
+    }
+    for (uint32_t i = 1; i < count; ++i) {
+        const f3 vi = xyz(v[i]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float dd = dot3(vi, dir[k]);
+            if (dd > bd[k]) { bd[k] = dd; bi[k] = i; }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = add(qrot(s.rot, xyz(v[bi[k]])), s.pos);
+}
+
 DIO_HD void shape_aabb(const ShapeRef& s, const HullPool& hp, f3& mn, f3& mx) {
     mn = s.pos;
     mx = s.pos;
+    if (s.tag == kHull) {
+        f3 sp[6];
+        hull_axis_supports(s, hp, sp);
+        if (sp[0].x > mx.x) mx.x = sp[0].x;
+        if (sp[1].x < mn.x) mn.x = sp[1].x;
+        if (sp[2].y > mx.y) mx.y = sp[2].y;
+        if (sp[3].y < mn.y) mn.y = sp[3].y;
+        if (sp[4].z > mx.z) mx.z = sp[4].z;
+        if (sp[5].z < mn.z) mn.z = sp[5].z;
+        return;
+    }
     const f3 sx = world_support(s, f3{1.0f, 0.0f, 0.0f}, hp);
     const f3 snx = world_support(s, neg(f3{1.0f, 0.0f, 0.0f}), hp);
     const f3 sy = world_support(s, f3{0.0f, 1.0f, 0.0f}, hp);
