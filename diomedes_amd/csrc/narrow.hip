@@ -36,6 +36,12 @@ constexpr int kSmallF = 128;
 // by every workgroup of the GJK and EPA launches: the support scans then read
 // LDS instead of the L2 (the same values, so the same results).
 constexpr uint32_t kLdsPoolMax = 1024;
+// Workgroups of the EPA first pass (one wavefront each, taking pairs i,
+// i + grid, ...): a pair's cost varies with its iteration count, and a grid
+// far wider than the resident wavefronts lets the dispatcher even that out
+// (4096-scene table: 3.18, 3.12, 3.06 and 3.03 ms per step at 8192, 16384,
+// 32768 and 65536 workgroups; no better beyond).
+constexpr uint32_t kEpaBlocks = 65536;
 extern __shared__ float4 pool_lds[];
 
 // The pool the support functions read: staged in LDS (LDS_POOL) or in place.
@@ -434,8 +440,9 @@ __global__ void __launch_bounds__(kWave) k_epa(BodyView bv, NarrowOut o, const f
     __shared__ EpaLds<MV, MF> L;
     const int lane = threadIdx.x;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
-    const HullPool hp = stage_pool<LDS_POOL>(bv);
     const uint32_t jobs = FIRST ? *o.gjk_count : *o.epa_retry_count;
+    if (blockIdx.x >= jobs) return;  // uniform: a workgroup with no pair stages nothing
+    const HullPool hp = stage_pool<LDS_POOL>(bv);
     for (uint32_t job = blockIdx.x; job < jobs; job += gridDim.x) {
         const uint32_t p = FIRST ? o.gjk_list[job] : o.epa_retry_list[job];
         const uint2 pr = o.pairs[p];
@@ -634,7 +641,7 @@ void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, h
     else hipLaunchKernelGGL(k_narrow<false>, dim3(blocks), dim3(kBlock), 0, s, bv, o, simplex_store);
     if (gjk_possible) {
         uint32_t eblocks = o.p_max > 0 ? o.p_max : 1;
-        if (eblocks > 8192) eblocks = 8192;
+        if (eblocks > kEpaBlocks) eblocks = kEpaBlocks;
         const uint32_t rblocks = eblocks < 512 ? eblocks : 512;
         const float4* ss = simplex_store;
         if (lds) {
