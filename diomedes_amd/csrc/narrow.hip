@@ -435,7 +435,8 @@ __device__ void face_contact(const L_t& L, uint32_t fi, f3& normal, float& depth
 // not yet reached) goes to the retry list instead of giving a result.
 // Otherwise (MV, MF = the caps of epa.rs): the pairs of the retry list.
 template <int MV, int MF, bool FIRST, bool LDS_POOL>
-__global__ void __launch_bounds__(kWave) k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FIRST ? 5 : 1)))
+k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
     constexpr int kFacesPerLane = MF / kWave;
     __shared__ EpaLds<MV, MF> L;
     const int lane = threadIdx.x;
