@@ -561,19 +561,43 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal(SolveView v, AvbdParams 
 // Grouped primal: G lanes per body.  A body's contacts and joints ("items")
 // are processed in chunks of G; lane l of the group builds the up-to-3
 // constraint rows of item chunk*G + l (each row: j_lin, j_ang, force,
-// stiffness) and stages them in LDS.  Each lane owns a fixed subset of the 27
-// block entries and adds, for every staged row in item order, that row's
-// term (a[p] * b[q]) * k (or j[p] * f for the gradient) -- the same sequence
-// of roundings per entry as the one-lane-per-body loop, so the result is
-// bit-identical.  The 27 entries are then gathered through LDS and every lane
-// of the group runs the 6x6 LDL^T; lane 0 stores the pose.
+// stiffness, 1) and stages them in LDS.  Each lane owns a fixed subset of the
+// 27 block entries and adds, for every staged row in item order, that row's
+// term (a[p] * b[q]) * w, w the stiffness for the Hessian entries and 1 for
+// the gradient entries (multiplying by 1 is exact) -- the same sequence of
+// roundings per entry as the one-lane-per-body loop, so the result is
+// bit-identical.  A row an item does not have is staged as zeros with force
+// and stiffness -0, so each of its terms is exactly -0 and adding it leaves
+// every sum unchanged (x + (-0) == x for every x, +0 and -0 included): the
+// loop needs no per-row test.  The 27 entries are then gathered through LDS
+// and one thread per body runs the 6x6 LDL^T.
+//
+// Work table: in a step's solve (v.work_tab set) iteration 0 stores each work
+// item's body and CSR ranges; iterations >= 1 read them in one load, so the
+// body's snapshot and its first CSR entries are fetched together, one
+// dependent global load fewer per launch than the order lookup followed by
+// the CSR offsets.  Each lane also fetches the CSR entry of its item in the
+// next chunk while the current chunk is accumulated.
 namespace {
-// entry e: term = R[kP[e]] * R[kQ[e]] (* R[7] = k for the Hessian entries);
-// R = (j_lin.xyz, j_ang.xyz, force, stiffness)
+// entry e: term = (R[kP[e]] * R[kQ[e]]) * R[kW[e]];
+// R = (j_lin.xyz, j_ang.xyz, force, stiffness, 1)
 __constant__ int8_t kP[27] = {0, 1, 2, 1, 2, 2, 3, 4, 5, 4, 5, 5, 3, 3, 3, 4, 4, 4, 5, 5, 5, 0, 1, 2, 3, 4, 5};
 __constant__ int8_t kQ[27] = {0, 0, 0, 1, 1, 2, 3, 3, 3, 4, 4, 5, 0, 1, 2, 0, 1, 2, 0, 1, 2, 6, 6, 6, 6, 6, 6};
-constexpr int kRowFloats = 8;
-constexpr int kItemFloats = 3 * kRowFloats + 1;  // 3 rows + row count
+constexpr int kRowFloats = 9;
+constexpr int kItemFloats = 28;  // 3 rows of 9, padded to 16 bytes
+
+// stage row r of an item: (j_lin, j_ang, force, stiffness, 1)
+__device__ __forceinline__ void stage_row(float* st, int r, f3 jl, f3 ja, float f, float k) {
+    float* R = st + kRowFloats * r;
+    R[0] = jl.x; R[1] = jl.y; R[2] = jl.z;
+    R[3] = ja.x; R[4] = ja.y; R[5] = ja.z;
+    R[6] = f; R[7] = k; R[8] = 1.0f;
+}
+// a row the item does not have: every term it contributes is -0
+__device__ __forceinline__ void stage_empty_row(float* st, int r) {
+    const f3 z{0.0f, 0.0f, 0.0f};
+    stage_row(st, r, z, z, -0.0f, -0.0f);
+}
 }  // namespace
 
 // HARD: the world has hard joints (their rows and fused dual updates are
@@ -588,25 +612,55 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     float4* pen_out = pen_buf(v, iter);
     constexpr int kBodies = kBlock / G;
     constexpr int kOwned = (27 + G - 1) / G;
-    __shared__ float stage[kBodies][G][kItemFloats];
+    __shared__ __align__(16) float stage[kBodies][G][kItemFloats];
     __shared__ float gathered[kBodies][28];  // the accumulated block
     __shared__ float4 body_f[kBodies][2];    // snapshot pos (.w: active), rot: for the solving thread
+    __shared__ uint32_t body_i[kBodies];     // the body of each group
     const int slot = threadIdx.x / G;
     const int lane = threadIdx.x % G;
     const uint32_t chunk = work_chunk(v);
     if (past_colour(v, chunk * kBodies)) return;  // uniform over the workgroup, before any barrier
-    const uint32_t i = body_at(v, p, chunk * kBodies + slot);
+    const uint32_t wk = chunk * kBodies + slot;   // this group's work item
     const float4* __restrict__ rp = v.pos[parity];
     const float4* __restrict__ rr = v.rot[parity];
     const float dt = p.dt;
 
-    // ---- every lane of the group loads its body's snapshot and item ranges
-    // and computes the initial block entries it owns (no set-up barrier)
+    // ---- the body and its item ranges: from the work table (iterations >= 1
+    // of a step's solve), or from the work order and the CSR offsets
+    uint32_t i, jb = 0, ncontacts = 0, kb = 0, njoints = 0;
+    if (v.work_tab && iter > 0) {
+        const uint4 w4 = wk < p.n ? v.work_tab[wk] : make_uint4(p.n, 0u, 0u, 0u);
+        i = w4.x;
+        jb = w4.y;
+        ncontacts = w4.z;
+        kb = w4.w;
+        if (v.j_off && i < p.n) njoints = v.j_off[i + 1] - kb;
+    } else {
+        i = body_at(v, p, wk);
+        if (i < p.n) {
+            jb = v.offsets[i];
+            ncontacts = v.offsets[i + 1] - jb;
+            if (v.j_off) {
+                kb = v.j_off[i];
+                njoints = v.j_off[i + 1] - kb;
+            }
+        }
+        if (v.work_tab && lane == 0 && wk < p.n) v.work_tab[wk] = make_uint4(i, jb, ncontacts, kb);
+    }
+    // the CSR entry of the lane's item in the first chunk (fetched beside the
+    // snapshot; a static body's entries are fetched and not used)
+    uint32_t ci_c = 0u, pc_c = 0u;
+    if ((uint32_t)lane < ncontacts) {
+        ci_c = v.indices[jb + lane];
+        pc_c = v.peers[jb + lane];
+    }
+
+    // ---- every lane of the group loads its body's snapshot and computes the
+    // initial block entries it owns (no set-up barrier)
     bool active = i < p.n;
     float4 imm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float4 sp4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sr4 = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     float4 il4 = sp4, ia4 = sr4, nl4 = sp4, na4 = sr4;
-    uint32_t jb = 0, ncontacts = 0, kb = 0, njoints = 0;
     if (active) {
         imm = v.inv_mm[i];
         sp4 = rp[i];
@@ -615,12 +669,6 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
         ia4 = v.init_ang[i];
         nl4 = v.iner_lin[i];
         na4 = v.iner_ang[i];
-        jb = v.offsets[i];
-        ncontacts = v.offsets[i + 1] - jb;
-        if (v.j_off) {
-            kb = v.j_off[i];
-            njoints = v.j_off[i + 1] - kb;
-        }
         active = imm.w != 0.0f;
     }
     if (!active) { ncontacts = 0; njoints = 0; }
@@ -631,6 +679,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     if (lane == 0) {
         body_f[slot][0] = f4(snap_pos, active ? 1.0f : 0.0f);
         body_f[slot][1] = sr4;
+        body_i[slot] = i;
     }
     float acc[kOwned];
     {
@@ -669,25 +718,34 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     const uint32_t nitems = ncontacts + njoints;
     const float kk = 1.0f - p.alpha;
     const f3 zero{0.0f, 0.0f, 0.0f};
-    // row offsets of each owned entry's two factors, loaded once (a table
-    // lookup inside the row loop would be a memory load per row)
-    int pofs[kOwned], qofs[kOwned];
+    // offsets of each owned entry's three factors within a row, loaded once
+    // (a table lookup inside the row loop would be a memory load per row);
+    // lanes past entry 26 read row offset 0 and keep nothing
+    int pofs[kOwned], qofs[kOwned], wofs[kOwned];
 #pragma unroll
     for (int t = 0; t < kOwned; ++t) {
         const int e = lane + G * t;
         pofs[t] = e < 27 ? (int)kP[e] : 0;
         qofs[t] = e < 27 ? (int)kQ[e] : 0;
+        wofs[t] = e < 21 ? 7 : 8;
     }
 
     for (uint32_t base = 0;; base += G) {
+        const uint32_t item = base + (uint32_t)lane;
+        // the CSR entry of the lane's item in the next chunk, fetched while
+        // this chunk is built and accumulated
+        uint32_t ci_n = 0u, pc_n = 0u;
+        if (item + G < ncontacts) {
+            ci_n = v.indices[jb + item + G];
+            pc_n = v.peers[jb + item + G];
+        }
         // ---- build the rows of item base + lane
         float* st = stage[slot][lane];
         int nrows = 0;
-        const uint32_t item = base + (uint32_t)lane;
         if (active && item < nitems) {
             if (item < ncontacts) {
-                const uint32_t ci = v.indices[jb + item];
-                const uint32_t pc = v.peers[jb + item];
+                const uint32_t ci = ci_c;
+                const uint32_t pc = pc_c;
                 const bool is_a = pc == kContainer || (pc & kPeerIsB) == 0u;
                 const uint32_t other = pc == kContainer ? kContainer : (pc & ~kPeerIsB);
                 const float4 n4 = v.c_n[ci];
@@ -729,16 +787,9 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
                 c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
                 c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
                 const Force3 F = contact_force(pen, lam, n4.w, c0, c1, c2);
-                const f3 jl[3] = {r0, r1, r2};
-                const f3 ja[3] = {j0, j1, j2};
-                const float fr[3] = {F.f0, F.f1, F.f2};
-                const float kr[3] = {pen.x, pen.y, pen.z};
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    st[8 * r + 0] = jl[r].x; st[8 * r + 1] = jl[r].y; st[8 * r + 2] = jl[r].z;
-                    st[8 * r + 3] = ja[r].x; st[8 * r + 4] = ja[r].y; st[8 * r + 5] = ja[r].z;
-                    st[8 * r + 6] = fr[r]; st[8 * r + 7] = kr[r];
-                }
+                stage_row(st, 0, r0, j0, F.f0, pen.x);
+                stage_row(st, 1, r1, j1, F.f1, pen.y);
+                stage_row(st, 2, r2, j2, F.f2, pen.z);
                 nrows = 3;
             } else {
                 const uint32_t cidx = v.j_idx[kb + (item - ncontacts)];
@@ -782,11 +833,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
                     const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
                                       scale(f3{0.0f, 0.0f, 1.0f}, sign)};
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        st[8 * r + 0] = 0.0f; st[8 * r + 1] = 0.0f; st[8 * r + 2] = 0.0f;
-                        st[8 * r + 3] = js[r].x; st[8 * r + 4] = js[r].y; st[8 * r + 5] = js[r].z;
-                        st[8 * r + 6] = fs[r]; st[8 * r + 7] = ks[r];
-                    }
+                    for (int r = 0; r < 3; ++r) stage_row(st, r, zero, js[r], fs[r], ks[r]);
                     nrows = 3;
                 } else if (HARD && is_hard(abk.z)) {
                     const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, abn);
@@ -810,21 +857,13 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
                                              hard_force(abk.z, pen.z, hr.chat.z, lam.z)};
                         const float ks[3] = {pen.x, pen.y, pen.z};
 #pragma unroll
-                        for (int r = 0; r < 3; ++r) {
-                            const f3 ja = cross3(r_self, js[r]);
-                            st[8 * r + 0] = js[r].x; st[8 * r + 1] = js[r].y; st[8 * r + 2] = js[r].z;
-                            st[8 * r + 3] = ja.x; st[8 * r + 4] = ja.y; st[8 * r + 5] = ja.z;
-                            st[8 * r + 6] = fs[r]; st[8 * r + 7] = ks[r];
-                        }
+                        for (int r = 0; r < 3; ++r) stage_row(st, r, js[r], cross3(r_self, js[r]), fs[r], ks[r]);
                         nrows = 3;
                     } else if (hr.ok) {
                         const float f = hard_force(abk.z, pen.x, hr.chat.x, lam.x);
                         if (abk.z != kHardRope || f > 0.0f) {
                             const f3 jl = scale(hr.dir, sign);
-                            const f3 ja = cross3(r_self, jl);
-                            st[0] = jl.x; st[1] = jl.y; st[2] = jl.z;
-                            st[3] = ja.x; st[4] = ja.y; st[5] = ja.z;
-                            st[6] = f; st[7] = pen.x;
+                            stage_row(st, 0, jl, cross3(r_self, jl), f, pen.x);
                             nrows = 1;
                         }
                     }
@@ -833,54 +872,39 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
                                       scale(f3{0.0f, 0.0f, 1.0f}, sign)};
                     const float fs[3] = {delta.x * stiff, delta.y * stiff, delta.z * stiff};
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        const f3 ja = cross3(r_self, js[r]);
-                        st[8 * r + 0] = js[r].x; st[8 * r + 1] = js[r].y; st[8 * r + 2] = js[r].z;
-                        st[8 * r + 3] = ja.x; st[8 * r + 4] = ja.y; st[8 * r + 5] = ja.z;
-                        st[8 * r + 6] = fs[r]; st[8 * r + 7] = stiff;
-                    }
+                    for (int r = 0; r < 3; ++r) stage_row(st, r, js[r], cross3(r_self, js[r]), fs[r], stiff);
                     nrows = 3;
                 } else {
                     const float len = len3(delta);
                     if (len > 1.0e-6f && (abk.z != kRope || len > aa4.w)) {
                         const f3 jl = scale(divs(delta, len), sign);
-                        const f3 ja = cross3(r_self, jl);
-                        st[0] = jl.x; st[1] = jl.y; st[2] = jl.z;
-                        st[3] = ja.x; st[4] = ja.y; st[5] = ja.z;
-                        st[6] = stiff * (len - aa4.w); st[7] = stiff;
+                        stage_row(st, 0, jl, cross3(r_self, jl), stiff * (len - aa4.w), stiff);
                         nrows = 1;
                     }
                 }
             }
         }
-        st[24] = __int_as_float(nrows);
+        if (nrows < 1) stage_empty_row(st, 0);
+        if (nrows < 2) stage_empty_row(st, 1);
+        if (nrows < 3) stage_empty_row(st, 2);
         __syncthreads();
         // ---- accumulate the owned entries over the staged rows, in item order
         if (active) {
             const int nin = (int)min((uint32_t)G, nitems > base ? nitems - base : 0u);
             for (int c = 0; c < nin; ++c) {
                 const float* it = stage[slot][c];
-                const int nr = __float_as_int(it[24]);
-                // all three rows are read unconditionally (one LDS round trip
-                // per item); rows past nr hold stale values and are not added
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
-                    const float* R = it + 8 * r;
-                    const float w = R[7];
+                    const float* R = it + kRowFloats * r;
 #pragma unroll
-                    for (int t = 0; t < kOwned; ++t) {
-                        // lanes past entry 26 read row offset 0 and keep nothing
-                        const int e = lane + G * t;
-                        float term = R[pofs[t]] * R[qofs[t]];
-                        if (e < 21) term = term * w;
-                        const float sum = acc[t] + term;
-                        acc[t] = (r < nr && e < 27) ? sum : acc[t];
-                    }
+                    for (int t = 0; t < kOwned; ++t) acc[t] = acc[t] + (R[pofs[t]] * R[qofs[t]]) * R[wofs[t]];
                 }
             }
         }
         const bool more = active && base + G < nitems;
         if (!__syncthreads_or(more)) break;
+        ci_c = ci_n;
+        pc_c = pc_n;
     }
     // ---- gather the 27 entries; one thread per body solves (wave 0)
     if (active) {
@@ -895,7 +919,7 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     const int t = threadIdx.x;
     const float4 f0 = body_f[t][0];
     if (f0.w == 0.0f) return;
-    const uint32_t bi = body_at(v, p, chunk * kBodies + t);
+    const uint32_t bi = body_i[t];
     const float* gs = gathered[t];
     Block6 B;
 #pragma unroll

@@ -303,6 +303,7 @@ struct dio_ctx {
     uint32_t* w_overflow = nullptr;  // [0] candidate pairs past max_pairs, [1] contacts past max_contacts
     unsigned long long* w_acc = nullptr;  // sums of pairs, contacts, CSR nnz over a dio_step call
     float4 *w_lam_alt = nullptr, *w_pen_alt = nullptr;  // scratch multipliers of a solve
+    uint4* w_work = nullptr;  // the Jacobi primal's work table (body and CSR ranges per work item)
     std::vector<hipEvent_t> prof_events;
     uint64_t w_step = 0;
     dio_step_stats stats{};
@@ -1049,6 +1050,7 @@ void ensure_world(dio_ctx* c) {
     c->w_acc = A.alloc<unsigned long long>(4);
     c->w_lam_alt = A.alloc<float4>(m);
     c->w_pen_alt = A.alloc<float4>(m);
+    c->w_work = A.alloc<uint4>(n);
     c->w_stage_b = A.alloc<dio_gpu_body>(n);
     c->w_stage_s = A.alloc<dio_avbd_body_state>(n);
     c->w_stage_c = A.alloc<dio_avbd_constraint>(c->c_cap);
@@ -1161,6 +1163,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         const int bn_pend = so.contact_lag ? 1 - s : s;
         v.stat_pairs = c->pend[bn_pend].p_dev;
         v.stat_acc = c->w_acc;
+        v.work_tab = c->w_work;  // iteration 0 of the solve below fills it
         const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n, o.gamma, o.penalty_min);
         if (c->tuning.scene_solve && c->w_scene_bits && c->w_j.c == 0) {
             if (c->w_bundle_bodies != c->tuning.scene_solve)

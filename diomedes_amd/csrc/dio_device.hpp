@@ -102,6 +102,11 @@ struct SolveView {
     const uint32_t* body_list;
     uint32_t list_count;
     const uint32_t* list_range;
+    // Jacobi solve of a step: the grouped primal's iteration 0 stores work
+    // item k's (body, CSR start, CSR count, joint-CSR start) here and
+    // iterations >= 1 read it (null: every iteration reads the order and the
+    // offsets).  Holds at least n entries.
+    uint4* work_tab;
     // when next.init_lin is set, the final launch (dual + recover) also runs
     // the next step's init from the recovered state into these buffers
     InitTarget next;
