@@ -598,6 +598,206 @@ __device__ __forceinline__ void stage_empty_row(float* st, int r) {
     const f3 z{0.0f, 0.0f, 0.0f};
     stage_row(st, r, z, z, -0.0f, -0.0f);
 }
+
+// The body an item's rows are built for: its pose at the iteration's start
+// and its displacement from the step's initial pose.
+struct RowBody {
+    f3 pos;
+    q4 rot;
+    f3 dsl, dsa;
+    uint32_t i;
+};
+
+// A contact item's inputs (the CSR entry's contact and the other body),
+// loaded apart from the arithmetic so a kernel can fetch them early.
+struct ContactIn {
+    uint32_t ci, other;
+    bool is_a, wall;
+    float4 n4, ra4, pen, lam;
+    f3 rb;
+    q4 orr, oia;  // the other body's rotation and initial rotation
+    f3 orp, oil;  // its position and initial position
+    float oinv;   // its inverse mass, read when the fused update needs it
+};
+__device__ __forceinline__ ContactIn load_contact(const SolveView& v, const float4* __restrict__ lam_in,
+                                                  const float4* __restrict__ pen_in, const float4* __restrict__ rp,
+                                                  const float4* __restrict__ rr, bool fuse, uint32_t ci,
+                                                  uint32_t pc) {
+    ContactIn c;
+    c.ci = ci;
+    c.wall = pc == kContainer;
+    c.is_a = c.wall || (pc & kPeerIsB) == 0u;
+    c.other = c.wall ? kContainer : (pc & ~kPeerIsB);
+    c.n4 = v.c_n[ci];
+    c.ra4 = v.c_ra[ci];
+    c.pen = pen_in[ci];
+    c.lam = lam_in[ci];
+    c.rb = f3{0.0f, 0.0f, 0.0f};  // a wall contact's r_b is zero
+    c.orr = q4{0.0f, 0.0f, 0.0f, 1.0f};
+    c.oia = c.orr;
+    c.orp = c.rb;
+    c.oil = c.rb;
+    c.oinv = 1.0f;
+    if (!c.wall) {
+        c.rb = xyz(v.c_rb[ci]);
+        c.orr = as_q(rr[c.other]);
+        c.orp = xyz(rp[c.other]);
+        c.oil = xyz(v.init_lin[c.other]);
+        c.oia = as_q(v.init_ang[c.other]);
+        if (fuse && !c.is_a) c.oinv = v.inv_mm[c.other].w;
+    }
+    return c;
+}
+
+// The three rows of a contact item for body b (avbd/src/lib.rs:290-418), with
+// the fused schedule's dual update k - 1 first (stored by body a, or by b when
+// a is static).
+__device__ __forceinline__ void stage_contact(const SolveView& v, const AvbdParams& p, float kk, bool fuse,
+                                              float4* lam_out, float4* pen_out, const RowBody& b,
+                                              const ContactIn& c, float* st) {
+    const f3 zero{0.0f, 0.0f, 0.0f};
+    const f3 n0 = xyz(c.n4);
+    const f3 ra = xyz(c.ra4);
+    const ContactFrame cf = contact_frame(v, c.ci, n0, c.ra4, c.wall);
+    const f3 t1 = cf.t1, t2 = cf.t2;
+    const f3 r0 = c.is_a ? n0 : neg(n0);
+    const f3 r1 = c.is_a ? t1 : neg(t1);
+    const f3 r2 = c.is_a ? t2 : neg(t2);
+    const f3 r_world = qrot(b.rot, c.is_a ? ra : c.rb);
+    f3 dol = zero, doa = zero, r_other = zero;
+    if (!c.wall) {
+        dol = sub(c.orp, c.oil);
+        doa = ang_diff(c.orr, c.oia);
+        r_other = qrot(c.orr, c.is_a ? c.rb : ra);
+    }
+    const float4 c04 = cf.c0;
+    float4 pen = c.pen;
+    float4 lam = c.lam;
+    if (fuse) {
+        // dual update k - 1 of the contact, in its own (a, b) order
+        const DualState d = dual_update(p, kk, n0, t1, t2, c.n4.w, c.is_a ? r_world : r_other,
+                                        c.is_a ? r_other : r_world, c.is_a ? b.dsl : dol, c.is_a ? b.dsa : doa,
+                                        c.is_a ? dol : b.dsl, c.is_a ? doa : b.dsa, c04, pen, lam);
+        pen = d.pen;
+        lam = d.lam;
+        if (c.is_a || c.oinv == 0.0f) {
+            lam_out[c.ci] = lam;
+            pen_out[c.ci] = pen;
+        }
+    }
+    const f3 j0 = cross3(r_world, r0), j1 = cross3(r_world, r1), j2 = cross3(r_world, r2);
+    float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
+    c0 = c0 + (dot3(r0, b.dsl) + dot3(j0, b.dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
+    c1 = c1 + (dot3(r1, b.dsl) + dot3(j1, b.dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
+    c2 = c2 + (dot3(r2, b.dsl) + dot3(j2, b.dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
+    const Force3 F = contact_force(pen, lam, c.n4.w, c0, c1, c2);
+    stage_row(st, 0, r0, j0, F.f0, pen.x);
+    stage_row(st, 1, r1, j1, F.f1, pen.y);
+    stage_row(st, 2, r2, j2, F.f2, pen.z);
+}
+
+// The rows of joint cidx for body b (avbd/src/lib.rs:422-539; hard and
+// bending kinds, with their fused dual update, when HARD); returns how many
+// rows it staged (0, 1 or 3).
+template <bool HARD>
+__device__ __forceinline__ int stage_joint(const SolveView& v, const AvbdParams& p, uint32_t iter, bool fuse,
+                                           const float4* __restrict__ rp, const float4* __restrict__ rr,
+                                           const RowBody& b, uint32_t cidx, float* st) {
+    const f3 zero{0.0f, 0.0f, 0.0f};
+    const uint4 abk = v.j_abk[cidx];
+    const float4 aa4 = v.j_anc_a[cidx];
+    const float4 ab4 = v.j_anc_b[cidx];
+    const bool is_a = abk.x == b.i;
+    const f3 aa = xyz(aa4), abn = xyz(ab4);
+    const f3 pa = is_a ? b.pos : xyz(rp[abk.x]);
+    const f3 pb = !is_a ? b.pos : xyz(rp[abk.y]);
+    const q4 qa = is_a ? b.rot : as_q(rr[abk.x]);
+    const q4 qb = !is_a ? b.rot : as_q(rr[abk.y]);
+    const f3 delta = sub(add(pa, qrot(qa, aa)), add(pb, qrot(qb, abn)));
+    const f3 r_self = qrot(b.rot, is_a ? aa : abn);
+    const float sign = is_a ? 1.0f : -1.0f;
+    const float stiff = ab4.w;
+    if (HARD && is_bend(abk.z)) {
+        // three angular rows: j_lin = 0, j_ang = +-e_r
+        const f3 err = bend_error(qa, as_q(aa4), qb);
+        float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pen = make_float4(stiff, stiff, stiff, 0.0f);
+        float fs[3] = {err.x * stiff, err.y * stiff, err.z * stiff};
+        if (abk.z == kHardBend) {
+            const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, abn);
+            const HardRows hr = hard_rows(abk.z, err, d0, aa4.w, p.alpha);
+            const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
+            lam = v.j_lam[jin][cidx];
+            pen = v.j_pen[jin][cidx];
+            if (fuse) {
+                hard_update(p, abk.z, hr, lam, pen);
+                if (is_a || v.inv_mm[abk.x].w == 0.0f) {
+                    v.j_lam[iter & 1u][cidx] = lam;
+                    v.j_pen[iter & 1u][cidx] = pen;
+                }
+            }
+            fs[0] = hard_force(abk.z, pen.x, hr.chat.x, lam.x);
+            fs[1] = hard_force(abk.z, pen.y, hr.chat.y, lam.y);
+            fs[2] = hard_force(abk.z, pen.z, hr.chat.z, lam.z);
+        }
+        const float ks[3] = {pen.x, pen.y, pen.z};
+        // e_r * sign, as the one-lane loop forms it (zeros keep the sign's sign)
+        const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
+                          scale(f3{0.0f, 0.0f, 1.0f}, sign)};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) stage_row(st, r, zero, js[r], fs[r], ks[r]);
+        return 3;
+    }
+    if (HARD && is_hard(abk.z)) {
+        const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, abn);
+        const HardRows hr = hard_rows(abk.z, delta, d0, aa4.w, p.alpha);
+        const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
+        float4 lam = v.j_lam[jin][cidx];
+        float4 pen = v.j_pen[jin][cidx];
+        if (fuse) {
+            // dual update k - 1 of the joint (the same on both bodies)
+            hard_update(p, abk.z, hr, lam, pen);
+            if (is_a || v.inv_mm[abk.x].w == 0.0f) {
+                v.j_lam[iter & 1u][cidx] = lam;
+                v.j_pen[iter & 1u][cidx] = pen;
+            }
+        }
+        if (abk.z == kHardSocket) {
+            const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
+                              scale(f3{0.0f, 0.0f, 1.0f}, sign)};
+            const float fs[3] = {hard_force(abk.z, pen.x, hr.chat.x, lam.x),
+                                 hard_force(abk.z, pen.y, hr.chat.y, lam.y),
+                                 hard_force(abk.z, pen.z, hr.chat.z, lam.z)};
+            const float ks[3] = {pen.x, pen.y, pen.z};
+#pragma unroll
+            for (int r = 0; r < 3; ++r) stage_row(st, r, js[r], cross3(r_self, js[r]), fs[r], ks[r]);
+            return 3;
+        }
+        if (hr.ok) {
+            const float f = hard_force(abk.z, pen.x, hr.chat.x, lam.x);
+            if (abk.z != kHardRope || f > 0.0f) {
+                const f3 jl = scale(hr.dir, sign);
+                stage_row(st, 0, jl, cross3(r_self, jl), f, pen.x);
+                return 1;
+            }
+        }
+        return 0;
+    }
+    if (abk.z == kBallSocket) {
+        const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
+                          scale(f3{0.0f, 0.0f, 1.0f}, sign)};
+        const float fs[3] = {delta.x * stiff, delta.y * stiff, delta.z * stiff};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) stage_row(st, r, js[r], cross3(r_self, js[r]), fs[r], stiff);
+        return 3;
+    }
+    const float len = len3(delta);
+    if (len > 1.0e-6f && (abk.z != kRope || len > aa4.w)) {
+        const f3 jl = scale(divs(delta, len), sign);
+        stage_row(st, 0, jl, cross3(r_self, jl), stiff * (len - aa4.w), stiff);
+        return 1;
+    }
+    return 0;
+}
 }  // namespace
 
 // HARD: the world has hard joints (their rows and fused dual updates are
@@ -743,145 +943,13 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
         float* st = stage[slot][lane];
         int nrows = 0;
         if (active && item < nitems) {
+            const RowBody rb{snap_pos, snap_rot, dsl, dsa, i};
             if (item < ncontacts) {
-                const uint32_t ci = ci_c;
-                const uint32_t pc = pc_c;
-                const bool is_a = pc == kContainer || (pc & kPeerIsB) == 0u;
-                const uint32_t other = pc == kContainer ? kContainer : (pc & ~kPeerIsB);
-                const float4 n4 = v.c_n[ci];
-                const f3 n0 = xyz(n4);
-                const float4 ra4 = v.c_ra[ci];
-                const f3 ra = xyz(ra4);
-                const f3 rb = other != kContainer ? xyz(v.c_rb[ci]) : zero;  // a wall contact's r_b is zero
-                const ContactFrame cf = contact_frame(v, ci, n0, ra4, other == kContainer);
-                const f3 t1 = cf.t1, t2 = cf.t2;
-                const f3 r0 = is_a ? n0 : neg(n0);
-                const f3 r1 = is_a ? t1 : neg(t1);
-                const f3 r2 = is_a ? t2 : neg(t2);
-                const f3 r_world = qrot(snap_rot, is_a ? ra : rb);
-                f3 dol = zero, doa = zero, r_other = zero;
-                if (other != kContainer) {
-                    const q4 orr = as_q(rr[other]);
-                    dol = sub(xyz(rp[other]), xyz(v.init_lin[other]));
-                    doa = ang_diff(orr, as_q(v.init_ang[other]));
-                    r_other = qrot(orr, is_a ? rb : ra);
-                }
-                const float4 c04 = cf.c0;
-                float4 pen = pen_in[ci];
-                float4 lam = lam_in[ci];
-                if (fuse) {
-                    // dual update k - 1 of the contact, in its own (a, b) order
-                    const DualState d = dual_update(p, kk, n0, t1, t2, n4.w, is_a ? r_world : r_other,
-                                                    is_a ? r_other : r_world, is_a ? dsl : dol, is_a ? dsa : doa,
-                                                    is_a ? dol : dsl, is_a ? doa : dsa, c04, pen, lam);
-                    pen = d.pen;
-                    lam = d.lam;
-                    if (is_a || v.inv_mm[other].w == 0.0f) {
-                        lam_out[ci] = lam;
-                        pen_out[ci] = pen;
-                    }
-                }
-                const f3 j0 = cross3(r_world, r0), j1 = cross3(r_world, r1), j2 = cross3(r_world, r2);
-                float c0 = c04.x * kk, c1 = c04.y * kk, c2 = c04.z * kk;
-                c0 = c0 + (dot3(r0, dsl) + dot3(j0, dsa) - dot3(r0, dol) - dot3(cross3(r_other, r0), doa));
-                c1 = c1 + (dot3(r1, dsl) + dot3(j1, dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
-                c2 = c2 + (dot3(r2, dsl) + dot3(j2, dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
-                const Force3 F = contact_force(pen, lam, n4.w, c0, c1, c2);
-                stage_row(st, 0, r0, j0, F.f0, pen.x);
-                stage_row(st, 1, r1, j1, F.f1, pen.y);
-                stage_row(st, 2, r2, j2, F.f2, pen.z);
+                const ContactIn cin = load_contact(v, lam_in, pen_in, rp, rr, fuse, ci_c, pc_c);
+                stage_contact(v, p, kk, fuse, lam_out, pen_out, rb, cin, st);
                 nrows = 3;
             } else {
-                const uint32_t cidx = v.j_idx[kb + (item - ncontacts)];
-                const uint4 abk = v.j_abk[cidx];
-                const float4 aa4 = v.j_anc_a[cidx];
-                const float4 ab4 = v.j_anc_b[cidx];
-                const bool is_a = abk.x == i;
-                const f3 aa = xyz(aa4), abn = xyz(ab4);
-                const f3 pa = is_a ? snap_pos : xyz(rp[abk.x]);
-                const f3 pb = !is_a ? snap_pos : xyz(rp[abk.y]);
-                const q4 qa = is_a ? snap_rot : as_q(rr[abk.x]);
-                const q4 qb = !is_a ? snap_rot : as_q(rr[abk.y]);
-                const f3 delta = sub(add(pa, qrot(qa, aa)), add(pb, qrot(qb, abn)));
-                const f3 r_self = qrot(snap_rot, is_a ? aa : abn);
-                const float sign = is_a ? 1.0f : -1.0f;
-                const float stiff = ab4.w;
-                if (HARD && is_bend(abk.z)) {
-                    // three angular rows: j_lin = 0, j_ang = +-e_r
-                    const f3 err = bend_error(qa, as_q(aa4), qb);
-                    float4 lam = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pen = make_float4(stiff, stiff, stiff, 0.0f);
-                    float fs[3] = {err.x * stiff, err.y * stiff, err.z * stiff};
-                    if (abk.z == kHardBend) {
-                        const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, abn);
-                        const HardRows hr = hard_rows(abk.z, err, d0, aa4.w, p.alpha);
-                        const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
-                        lam = v.j_lam[jin][cidx];
-                        pen = v.j_pen[jin][cidx];
-                        if (fuse) {
-                            hard_update(p, abk.z, hr, lam, pen);
-                            if (is_a || v.inv_mm[abk.x].w == 0.0f) {
-                                v.j_lam[iter & 1u][cidx] = lam;
-                                v.j_pen[iter & 1u][cidx] = pen;
-                            }
-                        }
-                        fs[0] = hard_force(abk.z, pen.x, hr.chat.x, lam.x);
-                        fs[1] = hard_force(abk.z, pen.y, hr.chat.y, lam.y);
-                        fs[2] = hard_force(abk.z, pen.z, hr.chat.z, lam.z);
-                    }
-                    const float ks[3] = {pen.x, pen.y, pen.z};
-                    // e_r * sign, as the one-lane loop forms it (zeros keep the sign's sign)
-                    const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
-                                      scale(f3{0.0f, 0.0f, 1.0f}, sign)};
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) stage_row(st, r, zero, js[r], fs[r], ks[r]);
-                    nrows = 3;
-                } else if (HARD && is_hard(abk.z)) {
-                    const f3 d0 = joint_delta(v.init_lin, v.init_ang, abk, aa4, abn);
-                    const HardRows hr = hard_rows(abk.z, delta, d0, aa4.w, p.alpha);
-                    const uint32_t jin = fuse ? (iter - 1) & 1u : 0u;
-                    float4 lam = v.j_lam[jin][cidx];
-                    float4 pen = v.j_pen[jin][cidx];
-                    if (fuse) {
-                        // dual update k - 1 of the joint (the same on both bodies)
-                        hard_update(p, abk.z, hr, lam, pen);
-                        if (is_a || v.inv_mm[abk.x].w == 0.0f) {
-                            v.j_lam[iter & 1u][cidx] = lam;
-                            v.j_pen[iter & 1u][cidx] = pen;
-                        }
-                    }
-                    if (abk.z == kHardSocket) {
-                        const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
-                                          scale(f3{0.0f, 0.0f, 1.0f}, sign)};
-                        const float fs[3] = {hard_force(abk.z, pen.x, hr.chat.x, lam.x),
-                                             hard_force(abk.z, pen.y, hr.chat.y, lam.y),
-                                             hard_force(abk.z, pen.z, hr.chat.z, lam.z)};
-                        const float ks[3] = {pen.x, pen.y, pen.z};
-#pragma unroll
-                        for (int r = 0; r < 3; ++r) stage_row(st, r, js[r], cross3(r_self, js[r]), fs[r], ks[r]);
-                        nrows = 3;
-                    } else if (hr.ok) {
-                        const float f = hard_force(abk.z, pen.x, hr.chat.x, lam.x);
-                        if (abk.z != kHardRope || f > 0.0f) {
-                            const f3 jl = scale(hr.dir, sign);
-                            stage_row(st, 0, jl, cross3(r_self, jl), f, pen.x);
-                            nrows = 1;
-                        }
-                    }
-                } else if (abk.z == kBallSocket) {
-                    const f3 js[3] = {scale(f3{1.0f, 0.0f, 0.0f}, sign), scale(f3{0.0f, 1.0f, 0.0f}, sign),
-                                      scale(f3{0.0f, 0.0f, 1.0f}, sign)};
-                    const float fs[3] = {delta.x * stiff, delta.y * stiff, delta.z * stiff};
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) stage_row(st, r, js[r], cross3(r_self, js[r]), fs[r], stiff);
-                    nrows = 3;
-                } else {
-                    const float len = len3(delta);
-                    if (len > 1.0e-6f && (abk.z != kRope || len > aa4.w)) {
-                        const f3 jl = scale(divs(delta, len), sign);
-                        stage_row(st, 0, jl, cross3(r_self, jl), stiff * (len - aa4.w), stiff);
-                        nrows = 1;
-                    }
-                }
+                nrows = stage_joint<HARD>(v, p, iter, fuse, rp, rr, rb, v.j_idx[kb + (item - ncontacts)], st);
             }
         }
         if (nrows < 1) stage_empty_row(st, 0);
@@ -891,6 +959,10 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
         // ---- accumulate the owned entries over the staged rows, in item order
         if (active) {
             const int nin = (int)min((uint32_t)G, nitems > base ? nitems - base : 0u);
+            // items taken four at a time with one lane per entry: all their
+            // LDS reads are issued before the first add
+            constexpr int kUnroll = kOwned == 1 ? 4 : 2;
+#pragma unroll kUnroll
             for (int c = 0; c < nin; ++c) {
                 const float* it = stage[slot][c];
 #pragma unroll
