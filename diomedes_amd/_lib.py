@@ -64,6 +64,10 @@ SIGNATURES = {
     "dio_world_pairs": (c_int, [_VP, _VP, c_uint32, _U32P]),
     "dio_world_colours": (c_int, [_VP, _U32P, c_uint32, _U32P]),
     "dio_world_joints": (c_int, [_VP, _FP, _FP, c_uint32, _U32P]),
+    "dio_world_set_instances": (c_int, [_VP, _FP, _U32P, c_uint32]),
+    "dio_world_instances": (c_int, [_VP, _FP, _VP, c_uint64]),
+    "dio_import_external_memory": (c_int, [_VP, c_int, c_uint64, c_uint64, c_uint64, POINTER(_VP), POINTER(_VP)]),
+    "dio_release_external_memory": (c_int, [_VP, _VP]),
 }
 
 _lib = None

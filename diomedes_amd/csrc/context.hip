@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 
@@ -25,6 +26,10 @@
 #include <string>
 #include <vector>
 
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include "../../include/diomedes_hip.h"
 #include "abi_checks.hpp"
 #include "adjoint.hpp"
@@ -33,6 +38,7 @@
 #include "colour.hpp"
 #include "contacts.hpp"
 #include "dio_device.hpp"
+#include "instances.hpp"
 #include "narrow.hpp"
 #include "prim.hpp"
 
@@ -188,6 +194,30 @@ struct JointDev {
 
 }  // namespace
 
+// Device memory imported from another API: the HIP import of a duplicate of
+// the caller's file descriptor (the duplicate's identity kept to close it
+// once the import is gone) and the mapped range.
+struct dio_external_memory {
+    hipExternalMemory_t ext = nullptr;
+    void* ptr = nullptr;
+    int fd = -1;
+    dev_t st_dev = 0;
+    ino_t st_ino = 0;
+};
+
+namespace {
+void release_external(dio_external_memory* m) {
+    if (m->ext) (void)hipDestroyExternalMemory(m->ext);
+    // the HIP runtime may or may not have closed the descriptor it was given:
+    // close it only if it still names the imported object
+    struct stat st;
+    if (m->fd >= 0 && fcntl(m->fd, F_GETFD) != -1 && fstat(m->fd, &st) == 0 && st.st_dev == m->st_dev &&
+        st.st_ino == m->st_ino)
+        (void)close(m->fd);
+    delete m;
+}
+}  // namespace
+
 struct dio_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -304,6 +334,14 @@ struct dio_ctx {
     unsigned long long* w_acc = nullptr;  // sums of pairs, contacts, CSR nnz over a dio_step call
     float4 *w_lam_alt = nullptr, *w_pen_alt = nullptr;  // scratch multipliers of a solve
     uint4* w_work = nullptr;  // the Jacobi primal's work table (body and CSR ranges per work item)
+    // instanced-renderer hand-off: per-body scale (xyz) and the slot order
+    float4* inst_scale = nullptr;
+    uint32_t* inst_order = nullptr;  // grown by dio_world_set_instances, freed with the context
+    uint32_t inst_order_cap = 0;
+    uint32_t inst_count = 0;
+    bool inst_ordered = false;
+    // device memory imported from other APIs (dio_import_external_memory)
+    std::vector<dio_external_memory*> ext_mems;
     std::vector<hipEvent_t> prof_events;
     uint64_t w_step = 0;
     dio_step_stats stats{};
@@ -363,6 +401,9 @@ struct dio_ctx {
     ~dio_ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (stream2) (void)hipStreamSynchronize(stream2);
+        for (dio_external_memory* m : ext_mems) release_external(m);
+        ext_mems.clear();
+        if (inst_order) (void)hipFree(inst_order);
         free_adjoint();
         drop_graphs();
         for (auto e : events) (void)hipEventDestroy(e);
@@ -1051,6 +1092,7 @@ void ensure_world(dio_ctx* c) {
     c->w_lam_alt = A.alloc<float4>(m);
     c->w_pen_alt = A.alloc<float4>(m);
     c->w_work = A.alloc<uint4>(n);
+    c->inst_scale = A.alloc<float4>(n);
     c->w_stage_b = A.alloc<dio_gpu_body>(n);
     c->w_stage_s = A.alloc<dio_avbd_body_state>(n);
     c->w_stage_c = A.alloc<dio_avbd_constraint>(c->c_cap);
@@ -1499,6 +1541,7 @@ dio_status world_upload(dio_ctx* ctx, const dio_gpu_body* bodies, const dio_avbd
     ctx->free_adjoint();  // a recording belongs to the world it was made in
     ctx->init_set = -1;
     ctx->w_n = n;
+    ctx->inst_count = 0;  // instance slots name the previous world's bodies
     ctx->w_hull_need = hull_need;
     ctx->w_step = 0;
     ctx->w_sort_pending = true;
@@ -2144,6 +2187,114 @@ dio_status dio_world_pairs(dio_ctx* ctx, dio_gpu_pair* out, uint32_t cap, uint32
     });
 }
 
+
+// ------------------------------------------------ instanced-renderer hand-off
+dio_status dio_world_set_instances(dio_ctx* ctx, const float* scales, const uint32_t* order, uint32_t count) {
+    return guarded(ctx, [&]() {
+        if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
+        const uint32_t n = ctx->w_n;
+        if (count > 0 && !scales) invalid("scales is null");
+        if (!order && count > n) invalid("without an order, count must not exceed the world's bodies");
+        if (order)
+            for (uint32_t k = 0; k < count; ++k)
+                if (order[k] >= n) invalid("order names a body past the world's bodies");
+        for (uint32_t i = 0; count > 0 && i < 3 * n; ++i)
+            if (!std::isfinite(scales[i])) invalid("a scale is not finite");
+        hipStream_t s = ctx->stream;
+        DIO_HIP_CHECK(hipStreamSynchronize(s));
+        ctx->inst_count = 0;
+        if (count == 0) return DIO_OK;
+        std::vector<float> s4(4 * (size_t)n, 0.0f);
+        for (uint32_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) s4[4 * (size_t)i + a] = scales[3 * (size_t)i + a];
+        DIO_HIP_CHECK(hipMemcpy(ctx->inst_scale, s4.data(), s4.size() * sizeof(float), hipMemcpyHostToDevice));
+        if (order) {
+            if (count > ctx->inst_order_cap) {
+                if (ctx->inst_order) DIO_HIP_CHECK(hipFree(ctx->inst_order));
+                ctx->inst_order = nullptr;
+                ctx->inst_order_cap = 0;
+                DIO_HIP_CHECK(hipMalloc(&ctx->inst_order, (size_t)count * 4));
+                ctx->inst_order_cap = count;
+            }
+            DIO_HIP_CHECK(hipMemcpy(ctx->inst_order, order, (size_t)count * 4, hipMemcpyHostToDevice));
+        }
+        ctx->inst_ordered = order != nullptr;
+        ctx->inst_count = count;
+        return DIO_OK;
+    });
+}
+
+dio_status dio_world_instances(dio_ctx* ctx, const float* view_proj, void* dst, uint64_t dst_bytes) {
+    return guarded(ctx, [&]() {
+        if (!ctx->w_ready) throw StatusError{DIO_NOT_READY, "no world uploaded", 0};
+        if (!view_proj) invalid("view_proj is null");
+        const uint32_t count = ctx->inst_count;
+        if (count == 0) return DIO_OK;
+        if (!dst) invalid("dst is null");
+        if ((reinterpret_cast<uintptr_t>(dst) & 15u) != 0u) invalid("dst must be 16-byte aligned");
+        const uint64_t need = 128ull * count;
+        if (dst_bytes < need) capacity("dst holds fewer bytes than 128 per instance slot", need);
+        Mat4Cols vp;
+        for (int j = 0; j < 4; ++j)
+            vp.c[j] = make_float4(view_proj[4 * j], view_proj[4 * j + 1], view_proj[4 * j + 2], view_proj[4 * j + 3]);
+        launch_instances(ctx->w_pos, ctx->w_rot, ctx->inst_scale, ctx->inst_ordered ? ctx->inst_order : nullptr,
+                         count, vp, static_cast<float4*>(dst), ctx->stream);
+        DIO_HIP_CHECK(hipGetLastError());
+        return DIO_OK;
+    });
+}
+
+dio_status dio_import_external_memory(dio_ctx* ctx, int fd, uint64_t object_size, uint64_t offset, uint64_t size,
+                                      void** dev_ptr, dio_external_memory** out) {
+    return guarded(ctx, [&]() {
+        if (!dev_ptr || !out) invalid("dev_ptr or out is null");
+        if (fd < 0) invalid("fd is negative");
+        if (size == 0 || object_size == 0) invalid("size is zero");
+        if (offset > object_size || size > object_size - offset) invalid("offset + size exceeds the object");
+        auto* m = new dio_external_memory();
+        m->fd = fcntl(fd, F_DUPFD_CLOEXEC, 0);
+        struct stat st;
+        if (m->fd < 0 || fstat(m->fd, &st) != 0) {
+            if (m->fd >= 0) (void)close(m->fd);
+            delete m;
+            invalid("fd is not an open file descriptor");
+        }
+        m->st_dev = st.st_dev;
+        m->st_ino = st.st_ino;
+        hipExternalMemoryHandleDesc hd;
+        std::memset(&hd, 0, sizeof(hd));
+        hd.type = hipExternalMemoryHandleTypeOpaqueFd;
+        hd.handle.fd = m->fd;
+        hd.size = object_size;
+        hipError_t e = hipImportExternalMemory(&m->ext, &hd);
+        if (e == hipSuccess) {
+            hipExternalMemoryBufferDesc bd;
+            std::memset(&bd, 0, sizeof(bd));
+            bd.offset = offset;
+            bd.size = size;
+            e = hipExternalMemoryGetMappedBuffer(&m->ptr, m->ext, &bd);
+        }
+        if (e != hipSuccess) {
+            release_external(m);
+            throw HipError{e, "importing the external memory"};
+        }
+        ctx->ext_mems.push_back(m);
+        *dev_ptr = m->ptr;
+        *out = m;
+        return DIO_OK;
+    });
+}
+
+dio_status dio_release_external_memory(dio_ctx* ctx, dio_external_memory* mem) {
+    return guarded(ctx, [&]() {
+        auto it = std::find(ctx->ext_mems.begin(), ctx->ext_mems.end(), mem);
+        if (!mem || it == ctx->ext_mems.end()) invalid("not an external memory of this context");
+        DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // writes into it may still be queued
+        ctx->ext_mems.erase(it);
+        release_external(mem);
+        return DIO_OK;
+    });
+}
 
 // ------------------------------------------------------------------ adjoint
 }  // extern "C"

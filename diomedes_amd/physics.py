@@ -436,6 +436,10 @@ class PhysicsContext:
     def _check(self, status: int) -> None:
         _lib.check(self._h, status)
 
+    def synchronize(self) -> None:
+        """Wait for everything enqueued on the context's stream."""
+        self._check(self._lib.dio_synchronize(self._h))
+
     # ---- Renderer::broad_phase_submit / broad_phase_count
     def broad_phase_submit(self, bodies: np.ndarray) -> None:
         bodies = np.ascontiguousarray(bodies, R.GPU_BODY)
@@ -722,6 +726,48 @@ class PhysicsContext:
         self._check(self._lib.dio_world_joints(self._h, _lib.ptr(lam, ctypes.c_float), _lib.ptr(pen, ctypes.c_float),
                                                c.value, ctypes.byref(c)))
         return lam, pen
+
+    def set_instances(self, scales, order=None) -> None:
+        """The instanced renderer's data (engine/src/render/renderer.rs:222-255):
+        per-body render scales (n x 3) and the instance order (order[s] is
+        the body drawn in slot s; None: slot s draws body s, one slot per
+        body).  Kept until the next call or upload."""
+        sc = np.ascontiguousarray(scales, np.float32).reshape(-1, 3)
+        if sc.shape[0] != self._n:
+            raise ValueError(f"scales must hold 3 floats per body ({self._n} bodies)")
+        if order is None:
+            count, optr = self._n, None
+        else:
+            order = np.ascontiguousarray(order, np.uint32).reshape(-1)
+            count, optr = order.size, _lib.ptr(order, ctypes.c_uint32)
+        self._check(self._lib.dio_world_set_instances(self._h, _lib.ptr(sc, ctypes.c_float), optr, count))
+
+    def instances(self, view_proj, dst: int, dst_bytes: int) -> None:
+        """Enqueue, for every instance slot s, (view_proj @ model, model) into
+        the device memory at address dst + 128 s: two 4 x 4 float matrices
+        stored column by column (engine/shaders/mesh.vert's InstanceTransform),
+        model built from the body's current pose and scale.  view_proj is the
+        4 x 4 matrix projection @ view (rows by columns); dst is 16-byte
+        aligned device memory of at least 128 bytes per slot."""
+        vp = np.ascontiguousarray(np.asarray(view_proj, np.float32).reshape(4, 4).T).reshape(16)
+        self._check(self._lib.dio_world_instances(self._h, _lib.ptr(vp, ctypes.c_float), ctypes.c_void_p(dst),
+                                                  dst_bytes))
+
+    def import_external(self, fd: int, object_size: int, offset: int = 0, size: int | None = None) -> tuple:
+        """Map device memory another API exported as a file descriptor (a
+        Vulkan buffer's opaque-fd memory, a DMA-BUF) into the context's device:
+        returns (device address of bytes offset .. offset + size, handle).
+        The library imports a duplicate; the caller keeps fd."""
+        if size is None:
+            size = object_size - offset
+        ptr = ctypes.c_void_p()
+        handle = ctypes.c_void_p()
+        self._check(self._lib.dio_import_external_memory(self._h, fd, object_size, offset, size, ctypes.byref(ptr),
+                                                         ctypes.byref(handle)))
+        return ptr.value, handle
+
+    def release_external(self, handle) -> None:
+        self._check(self._lib.dio_release_external_memory(self._h, handle))
 
     def pairs(self) -> np.ndarray:
         n = ctypes.c_uint32()

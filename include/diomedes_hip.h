@@ -477,6 +477,48 @@ dio_status dio_world_joints(dio_ctx* ctx, float* lambda, float* penalty, uint32_
 /* Candidate pairs of the last step's broad phase (sorted, a < b). */
 dio_status dio_world_pairs(dio_ctx* ctx, dio_gpu_pair* out, uint32_t cap, uint32_t* n);
 
+/* ---- instanced-renderer hand-off --------------------------------------
+   The reference's frame copies each body's pose into its scene instance
+   (demos/avbd_box/src/main.rs:274-280), builds one model matrix per
+   instance, Mat4::from_scale_rotation_translation(scale, orientation,
+   position), in the renderer's instance order (instances grouped by shape
+   and style, a stable sort: engine/src/render/renderer.rs:222-255), and
+   uploads (projection * view * model, model) per instance into a uniform
+   buffer of at most 256 instances (engine/src/render/context.rs:171-199,
+   :18-21; engine/shaders/mesh.vert).  Here the pairs are written on the
+   device, from the world's device-resident poses, into device memory the
+   caller names (an imported renderer buffer, below), with no instance limit.
+
+   dio_world_set_instances: scales (3 floats per body, the world's n bodies)
+   and order (count entries: order[s] is the body drawn in instance slot s,
+   each < n, any count; null: slot s draws body s, count <= n).  Copied to
+   the device; kept until the next call or upload.  count 0 clears them.
+   dio_world_instances: for every slot s < count, writes the two column-major
+   4 x 4 float matrices (view_proj * model, model) -- mesh.vert's
+   InstanceTransform, 128 bytes -- at (char*)dst + 128 * s, where view_proj
+   (16 floats) is the caller's projection * view (column-major, as glam stores a Mat4)
+   and model is built from the body's current pose and scale.  dst_bytes <
+   128 * count is DIO_CAPACITY (the required size from
+   dio_required_capacity).  Enqueued on the context's stream after its last
+   step; dio_synchronize waits for it. */
+dio_status dio_world_set_instances(dio_ctx* ctx, const float* scales, const uint32_t* order, uint32_t count);
+dio_status dio_world_instances(dio_ctx* ctx, const float* view_proj, void* dst, uint64_t dst_bytes);
+/* Device memory another API exported as a file descriptor -- a Vulkan
+   buffer's memory exported with VK_EXTERNAL_MEMORY_HANDLE_TYPE_OPAQUE_FD_BIT
+   (a DMA-BUF on AMD's drivers), or a DMA-BUF of another allocation -- mapped
+   into the context's device: *dev_ptr addresses bytes [offset, offset + size)
+   of the object whose total size is object_size.  The descriptor names a
+   whole memory object: for a Vulkan buffer, the VkDeviceMemory it is bound
+   to (vkGetMemoryFdKHR), with offset the buffer's bind offset in it.  The
+   library imports a
+   duplicate of fd, so the caller keeps fd and may close it at once.  The
+   mapping lives until dio_release_external_memory (or dio_destroy).  fd < 0,
+   size 0 or offset + size > object_size: DIO_INVALID. */
+typedef struct dio_external_memory dio_external_memory;
+dio_status dio_import_external_memory(dio_ctx* ctx, int fd, uint64_t object_size, uint64_t offset,
+                                      uint64_t size, void** dev_ptr, dio_external_memory** out);
+dio_status dio_release_external_memory(dio_ctx* ctx, dio_external_memory* mem);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,7 +41,9 @@
 //                                           spinning-neighbour sign, spring / socket / slack rope
 //   hard joints, bending, colouring pinned: closed forms and invariants (no reference counterpart)
 //   quaternion product / rotation   pinned: constraints.rs:158 test + rotation identities
-// tests/test_oracle_mutations.py plants 41 plausible mistakes in copies of
+//   instance transforms (model,     pinned: FP64 products, rotation part = qrot of the basis,
+//     view_proj * model)                    orthonormality, closed-form poses
+// tests/test_oracle_mutations.py plants 45 plausible mistakes in copies of
 // this file; each fails at least one of the designed pins above.
 #pragma once
 
@@ -1893,6 +1895,55 @@ void replay(std::vector<Body<T>>& bodies, const std::vector<ReplayStep>& steps, 
         solve_bodies(bodies, cs, st.offsets, st.indices, o, (freeze & 1) && in ? in : nullptr, out,
                      (freeze & 2) && in ? in + blk : nullptr, out ? out + blk : nullptr);
         at += 2 * blk;
+    }
+}
+
+// ------------------------------------------------------- instance transforms
+// The renderer's per-instance matrices (engine/src/render/renderer.rs:250-255
+// and engine/src/render/context.rs:188-193): model =
+// Mat4::from_scale_rotation_translation(scale, rotation, translation), then
+// (projection * view) * model.  Both are glam 0.33.3 (the reference's
+// third-party math crate, absent from the reference tree), restated in its
+// operation order: the rotation's axes from the doubled components, each
+// axis scaled, the translation column with w = 1; a product's column j is
+// ((a0 * b_j.x + a1 * b_j.y) + a2 * b_j.z) + a3 * b_j.w component by
+// component.  Matrices are column-major, 16 scalars.
+template <class T> inline void model_matrix(const T pos[3], const T q[4], const T s[3], T m[16]) {
+    const T x = q[0], y = q[1], z = q[2], w = q[3];
+    const T x2 = x + x, y2 = y + y, z2 = z + z;
+    const T xx = x * x2, xy = x * y2, xz = x * z2;
+    const T yy = y * y2, yz = y * z2, zz = z * z2;
+    const T wx = w * x2, wy = w * y2, wz = w * z2;
+    const T ax[4] = {T(1) - (yy + zz), xy + wz, xz - wy, T(0)};
+    const T ay[4] = {xy - wz, T(1) - (xx + zz), yz + wx, T(0)};
+    const T az[4] = {xz + wy, yz - wx, T(1) - (xx + yy), T(0)};
+    for (int r = 0; r < 4; ++r) {
+        m[r] = ax[r] * s[0];
+        m[4 + r] = ay[r] * s[1];
+        m[8 + r] = az[r] * s[2];
+    }
+    m[12] = pos[0];
+    m[13] = pos[1];
+    m[14] = pos[2];
+    m[15] = T(1);
+}
+template <class T> inline void mat4_mul(const T a[16], const T b[16], T out[16]) {
+    for (int j = 0; j < 4; ++j)
+        for (int r = 0; r < 4; ++r)
+            out[4 * j + r] = ((a[r] * b[4 * j] + a[4 + r] * b[4 * j + 1]) + a[8 + r] * b[4 * j + 2]) +
+                             a[12 + r] * b[4 * j + 3];
+}
+// Instance slot k < count draws body order[k] (order null: body k); out holds
+// (view_proj * model, model) per slot, 32 scalars.
+template <class T>
+void instance_transforms(const T* pos, const T* rot, const T* scale, const uint32_t* order, uint32_t count,
+                         const T view_proj[16], T* out) {
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t b = order ? order[k] : k;
+        T m[16];
+        model_matrix(pos + 3 * (size_t)b, rot + 4 * (size_t)b, scale + 3 * (size_t)b, m);
+        mat4_mul(view_proj, m, out + 32 * (size_t)k);
+        std::memcpy(out + 32 * (size_t)k + 16, m, sizeof(m));
     }
 }
 

@@ -537,6 +537,22 @@ OAPI void oracle_avbd_init(int precision, const dio_avbd_body_state* st, const f
     else avbd_init_t<float>(st, pos, rot, n, ro, warm_pos, warm_rot, iner_pos, iner_rot);
 }
 
+// The renderer's instance matrices (view_proj * model, model) per slot, in
+// float (precision 32; inputs rounded to float, outputs exact) or double.
+OAPI void oracle_instance_transforms(int precision, const double* pos, const double* rot, const double* scale,
+                                     uint32_t n, const uint32_t* order, uint32_t count, const double* view_proj,
+                                     double* out) {
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        std::vector<T> p(pos, pos + 3 * (size_t)n), r(rot, rot + 4 * (size_t)n), s(scale, scale + 3 * (size_t)n);
+        std::vector<T> vp(view_proj, view_proj + 16), o(32 * (size_t)count);
+        instance_transforms<T>(p.data(), r.data(), s.data(), order, count, vp.data(), o.data());
+        for (size_t k = 0; k < o.size(); ++k) out[k] = (double)o[k];
+    };
+    if (precision == 64) run(0.0);
+    else run(0.0f);
+}
+
 // glam primitives for the known-answer tests: Quat * Vec3 and Quat * Quat
 OAPI void oracle_qrot(int precision, const double* q, const double* v, double* out) {
     if (precision == 64) put3(out, qrot(Q<double>{q[0], q[1], q[2], q[3]}, V3<double>{v[0], v[1], v[2]}));

@@ -73,6 +73,7 @@ def load() -> ctypes.CDLL:
         "oracle_avbd_init": (None, [c_int, c_void_p, _F, _F, c_uint32, POINTER(R.RunOptions), _D, _D, _D, _D]),
         "oracle_qrot": (None, [c_int, _D, _D, _D]),
         "oracle_qmul": (None, [c_int, _D, _D, _D]),
+        "oracle_instance_transforms": (None, [c_int, _D, _D, _D, c_uint32, _U, c_uint32, _D, _D]),
         "oracle_support": (None, [c_int, c_int, c_void_p, _D, _F, c_uint32, _D]),
         "oracle_replay": (None, [c_int, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, _U64, _U, _U, _U64,
                                  POINTER(R.Options), _D, c_double, c_uint32, _D, _D, _D, _D]),
@@ -445,6 +446,27 @@ def qrot(q, v, precision: int = 32) -> np.ndarray:
     out = np.zeros(3)
     load().oracle_qrot(precision, _p(q, c_double), _p(v, c_double), _p(out, c_double))
     return out
+
+
+def instance_transforms(pos, rot, scale, view_proj, order=None, precision: int = 32) -> np.ndarray:
+    """(view_proj @ model, model) per instance slot, shape (count, 2, 4, 4) with
+    [k, i] the column-major 16 scalars of the matrix as rows of columns
+    ([k, i, col, row]); order[k] is slot k's body (None: slot k draws body k).
+    view_proj is given column-major (16 values, glam's Mat4 storage)."""
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    rot = np.ascontiguousarray(rot, np.float64).reshape(-1, 4)
+    scale = np.ascontiguousarray(scale, np.float64).reshape(-1, 3)
+    vp = np.ascontiguousarray(view_proj, np.float64).reshape(16)
+    n = pos.shape[0]
+    if order is None:
+        count, optr = n, None
+    else:
+        order = np.ascontiguousarray(order, np.uint32)
+        count, optr = order.size, _p(order, c_uint32)
+    out = np.zeros(32 * count)
+    load().oracle_instance_transforms(precision, _p(pos, c_double), _p(rot, c_double), _p(scale, c_double), n,
+                                      optr, count, _p(vp, c_double), _p(out, c_double))
+    return out.reshape(count, 2, 4, 4)
 
 
 def qmul(a, b, precision: int = 32) -> np.ndarray:

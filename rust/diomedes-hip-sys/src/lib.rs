@@ -12,11 +12,14 @@
 #![allow(non_camel_case_types)]
 #![no_std]
 
-use core::ffi::c_char;
+use core::ffi::{c_char, c_void};
 use core::mem::size_of;
 
 /// The library context (opaque; created by `dio_create`, freed by `dio_destroy`).
 pub enum dio_ctx {}
+
+/// Device memory imported from another API (opaque; `dio_import_external_memory`).
+pub enum dio_external_memory {}
 
 /// The status every call returns.
 pub type dio_status = i32;
@@ -420,4 +423,20 @@ extern "C" {
     pub fn dio_world_joints(ctx: *mut dio_ctx, lambda: *mut f32, penalty: *mut f32, cap: u32, c: *mut u32) -> dio_status;
     /// Candidate pairs of the last step's broad phase (sorted, a < b).
     pub fn dio_world_pairs(ctx: *mut dio_ctx, out: *mut dio_gpu_pair, cap: u32, n: *mut u32) -> dio_status;
+    /// The instanced renderer's data: per-body render scales (3 floats per
+    /// body) and the instance order (order[s] = the body drawn in slot s; null:
+    /// slot s draws body s), as the renderer groups instances by shape and style
+    /// (engine/src/render/renderer.rs:222-255).
+    pub fn dio_world_set_instances(ctx: *mut dio_ctx, scales: *const f32, order: *const u32, count: u32) -> dio_status;
+    /// For every instance slot s: (view_proj * model, model), two column-major
+    /// 4 x 4 float matrices (mesh.vert's InstanceTransform, 128 bytes) at
+    /// dst + 128 s, model = Mat4::from_scale_rotation_translation of the body's
+    /// pose and scale; view_proj = projection * view (16 floats, column-major).
+    /// dst: 16-byte aligned device memory, e.g. an imported renderer buffer.
+    pub fn dio_world_instances(ctx: *mut dio_ctx, view_proj: *const f32, dst: *mut c_void, dst_bytes: u64) -> dio_status;
+    /// Map device memory exported as a file descriptor (a Vulkan buffer's
+    /// opaque-fd memory, a DMA-BUF) into the context's device; the library
+    /// imports a duplicate, so the caller keeps fd.
+    pub fn dio_import_external_memory(ctx: *mut dio_ctx, fd: i32, object_size: u64, offset: u64, size: u64, dev_ptr: *mut *mut c_void, out: *mut *mut dio_external_memory) -> dio_status;
+    pub fn dio_release_external_memory(ctx: *mut dio_ctx, mem: *mut dio_external_memory) -> dio_status;
 }

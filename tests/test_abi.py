@@ -93,4 +93,8 @@ def test_invalid_arguments_return_status_without_a_device():
     assert lib.dio_create(0, None, ctypes.byref(h)) == 2
     assert lib.dio_synchronize(None) == 2
     assert lib.dio_step(None, None, None, 1) == 2
+    assert lib.dio_world_set_instances(None, None, None, 0) == 2
+    assert lib.dio_world_instances(None, None, None, 0) == 2
+    assert lib.dio_import_external_memory(None, -1, 0, 0, 0, None, None) == 2
+    assert lib.dio_release_external_memory(None, None) == 2
     assert lib.dio_last_error(None) == b"null context"

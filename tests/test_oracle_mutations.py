@@ -31,9 +31,9 @@ HPP, CAPI = "avbd_oracle.hpp", "oracle_capi.cpp"
 CXXFLAGS = ["-O1", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-march=x86-64-v2",
             "-shared"]
 
-P, K, G, J, C = ("tests/test_oracle_pins.py", "tests/test_oracle_reference_kats.py",
-                 "tests/test_oracle_geometry_pins.py", "tests/test_oracle_joints.py",
-                 "tests/test_oracle_colouring.py")
+P, K, G, J, C, I = ("tests/test_oracle_pins.py", "tests/test_oracle_reference_kats.py",
+                    "tests/test_oracle_geometry_pins.py", "tests/test_oracle_joints.py",
+                    "tests/test_oracle_colouring.py", "tests/test_oracle_instances.py")
 
 # (name, file, snippet, replacement, tests that must catch it)
 MUTATIONS = [
@@ -183,6 +183,17 @@ MUTATIONS = [
     ("colouring skips the smallest free colour", HPP,
      "while (mask[w] & (1u << b)) ++b;", "while (mask[w] & (1u << b)) ++b;\n                    b += b < 31;",
      [C]),
+    # ---- instance transforms (the renderer hand-off)
+    ("model matrix x axis with the sign of w z flipped", HPP,
+     "const T ax[4] = {T(1) - (yy + zz), xy + wz, xz - wy, T(0)};",
+     "const T ax[4] = {T(1) - (yy + zz), xy - wz, xz - wy, T(0)};",
+     [I + "::test_rotation_columns_are_the_rotated_basis", I + "::test_rotation_part_is_orthonormal"]),
+    ("model matrix without the translation", HPP, "m[12] = pos[0];", "m[12] = T(0);",
+     [I + "::test_model_maps_local_vertices_to_world", I + "::test_identity_and_z_rotation_closed_forms"]),
+    ("view_proj * model taken as model * view_proj", HPP, "mat4_mul(view_proj, m, out + 32 * (size_t)k);",
+     "mat4_mul(m, view_proj, out + 32 * (size_t)k);", [I + "::test_mvp_is_the_matrix_product"]),
+    ("instance slots ignore the order", HPP, "const uint32_t b = order ? order[k] : k;", "const uint32_t b = k;",
+     [I + "::test_slots_follow_the_order"]),
 ]
 
 

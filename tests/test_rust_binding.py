@@ -38,6 +38,8 @@ def _c_type(decl: str) -> str:
     name = base[0]
     if name == "char" and stars == 1:
         return "*const c_char"
+    if name == "void" and stars > 0:
+        name = "c_void"
     rust = SCALARS.get(name, name)
     if stars == 0:
         return rust
