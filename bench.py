@@ -85,6 +85,40 @@ def measured_traffic(kernel: str) -> dict | None:
     return None
 
 
+INSTANCE_BYTES = 4 + 16 + 16 + 16 + 128  # order entry, position, orientation, scale read; the pair written
+
+
+def instances_of(ctx, n: int, calls: int = 200) -> dict:
+    """The renderer hand-off on the stepped world: every body's
+    (view_proj * model, model) pair written by dio_world_instances into device
+    memory, `calls` times back to back; time per call from the wall clock
+    around them (one synchronisation at the end), against 8 TB/s for the
+    algorithmic bytes (INSTANCE_BYTES per slot)."""
+    import numpy as np
+
+    from diomedes_amd.devmem import DeviceBuffer
+
+    rng = np.random.default_rng(0)
+    vp = np.eye(4, dtype=np.float32)
+    out = {"slots": n, "bytes_per_slot": INSTANCE_BYTES,
+           "timing": f"wall clock over {calls} calls, one synchronisation"}
+    with DeviceBuffer(128 * n) as buf:
+        for key, order in (("body_order", None), ("random_order", rng.permutation(n).astype(np.uint32))):
+            ctx.set_instances(np.ones((n, 3), np.float32), order)
+            ctx.instances(vp, buf.ptr, buf.nbytes)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                ctx.instances(vp, buf.ptr, buf.nbytes)
+            ctx.synchronize()
+            us = 1e6 * (time.perf_counter() - t0) / calls
+            gbs = INSTANCE_BYTES * n / (us * 1e-6) / 1e9
+            out[key] = {"us_per_call": us, "achieved_gbs": gbs, "frac_of_8tbs": gbs / 8000.0}
+    out["note"] = ("body_order: slot s draws body s (a world whose bodies are grouped by shape); random_order: "
+                   "each slot gathers a random body's pose, so every 16-byte read fetches a cache line of its own")
+    return out
+
+
 def batched_leg(args, tuning: dict, compare_defaults: bool = True) -> dict:
     """The batched-scenes workload: every scene of the table stepped by the
     same launches (one world with a scene table, the step captured in HIP
@@ -128,6 +162,8 @@ def batched_leg(args, tuning: dict, compare_defaults: bool = True) -> dict:
                 rest = {"steps": f"{FULL_STEPS - left}..{FULL_STEPS}", "value": n_dyn * iters * left / (rs.ms_avbd / 1000.0),
                         "ms_per_step": 1000.0 * rw / left, "ms_per_step_avbd": rs.ms_avbd / left,
                         "last_step": {"pairs": rs.pairs, "contacts": rs.contacts, "csr_nnz": rs.csr_nnz}}
+            if autotune:
+                rest = {"to_step_300": rest, "instances": instances_of(ctx, batch.n)}
             return wall, st, ctx.tuning(), chosen, rest
 
     wall, st, used, chosen, rest = run(tune)
@@ -151,7 +187,9 @@ def batched_leg(args, tuning: dict, compare_defaults: bool = True) -> dict:
         out["library_defaults"] = {"value": n_dyn * iters * k / (dst.ms_avbd / 1000.0),
                                    "ms_per_step": 1000.0 * dwall / k, "ms_per_step_avbd": dst.ms_avbd / k}
     if rest:
-        out["to_step_300"] = rest
+        if rest["to_step_300"]:
+            out["to_step_300"] = rest["to_step_300"]
+        out["instances"] = rest["instances"]
     if chosen and compare_defaults:
         # the whole solve in one launch (bundles of >= 128 bodies per workgroup) at the same shapes
         swall, sst, _, _, _ = run(False, scene_solve=128, shapes=used)
@@ -546,8 +584,13 @@ def roofline_of(g: dict, tuning: dict) -> dict:
     counter runs of the same window and command, when they exist."""
     pst = g["profile"]
     lanes = tuning["primal_lanes"]
-    kname = (f"k_avbd_primal_grouped<{lanes}, {'*' if tuning['fuse_dual'] else 'false'}>" if lanes > 1
-             else f"k_avbd_primal<{'*' if tuning['fuse_dual'] else 'false'}>")
+    fz = '*' if tuning['fuse_dual'] else 'false'
+    if tuning.get("primal_packed"):
+        kname = f"k_avbd_primal_packed<{fz}, false>"
+    elif lanes > 1:
+        kname = f"k_avbd_primal_grouped<{lanes}, {fz}>"
+    else:
+        kname = f"k_avbd_primal<{fz}>"
     p_launches = max(pst.n_primal, 1)
     primal_ms = pst.ms_primal / p_launches
     pm = pst.sum_contacts / max(pst.steps, 1)

@@ -1009,6 +1009,270 @@ __global__ void __launch_bounds__(kBlock) k_avbd_primal_grouped(SolveView v, Avb
     v.rot[1 - parity][bi] = f4q(new_rot);
 }
 
+// Packed primal (the Jacobi solve of a step; no counterpart in the
+// reference's thread-per-body kernel).  A workgroup of kPackThreads threads
+// takes kPackBodies consecutive work items.  Their items -- each body's
+// contacts, then its joints, body after body -- form one list, and in round r
+// thread t builds the rows of list entry r * kPackThreads + t, so nearly every
+// lane of a round builds an item however unevenly the items fall on the
+// bodies (with G lanes per body, the lanes past a body's item count sit idle
+// through the costliest phase).  Each body's kPackLanes lanes then add its
+// entries over its own range of the list in item order, as in the grouped
+// kernel (bit-identical to the one-lane loop), and one thread per body
+// solves.
+//
+// Iteration 0 of a solve stores, per work item, the body, its range in the
+// list and its CSR ranges (v.work_tab, two entries per work item), and per
+// workgroup the identity of its first kPackCap list entries (v.item_tab:
+// contact and peer code, or constraint).  Iterations >= 1 read both in their
+// first load.
+namespace {
+constexpr int kPackBodies = 8;
+constexpr int kPackLanes = 32;
+constexpr int kPackThreads = kPackBodies * kPackLanes;
+constexpr int kPackCap = kPackThreads;  // list entries per workgroup in the item table (round 0)
+}  // namespace
+
+template <bool FUSED, bool HARD>
+__global__ void __launch_bounds__(kPackThreads) k_avbd_primal_packed(SolveView v, AvbdParams p, uint32_t iter) {
+    constexpr int G = kPackLanes;
+    constexpr int kOwned = (27 + G - 1) / G;
+    const uint32_t parity = iter & 1u;
+    const bool fuse = FUSED && iter > 0;
+    const float4* __restrict__ lam_in = lam_buf(v, fuse ? iter - 1 : 0);
+    const float4* __restrict__ pen_in = pen_buf(v, fuse ? iter - 1 : 0);
+    float4* lam_out = lam_buf(v, iter);
+    float4* pen_out = pen_buf(v, iter);
+    __shared__ __align__(16) float stage[kPackThreads][kItemFloats];
+    __shared__ float gathered[kPackBodies][28];
+    __shared__ float4 body_f[kPackBodies][4];  // pos (.w: active), rot, dsl, dsa of each slot's body
+    __shared__ uint4 slot_csr[kPackBodies];    // body, CSR start, contacts, joint-CSR start
+    __shared__ uint32_t slot_n[kPackBodies];   // list entries of each slot (0: inactive)
+    const int slot = threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    const uint32_t chunk = work_chunk(v);
+    const uint32_t wk = chunk * kPackBodies + slot;
+    const float4* __restrict__ rp = v.pos[parity];
+    const float4* __restrict__ rr = v.rot[parity];
+    const bool from_tab = iter > 0;
+    uint4* __restrict__ wt = v.work_tab;
+    uint2* __restrict__ itab = v.item_tab + (size_t)chunk * kPackCap;
+
+    // ---- the slot's body and ranges, and (iterations >= 1) this thread's
+    // list entries of rounds 0 and 1, in one round of loads
+    uint32_t i = p.n, jb = 0, nc = 0, kb = 0, nj = 0, pg = 0, ng = 0, total = 0;
+    uint2 pt0 = make_uint2(0u, 0u);
+    if (from_tab) {
+        const uint4 a = wt[2 * wk], b = wt[2 * wk + 1];
+        pt0 = itab[threadIdx.x];
+        i = a.x; pg = a.y; ng = a.z; total = a.w;
+        jb = b.x; nc = b.y; kb = b.z; nj = b.w;
+    } else {
+        i = body_at(v, p, wk);
+        if (i < p.n) {
+            jb = v.offsets[i];
+            nc = v.offsets[i + 1] - jb;
+            if (v.j_off) {
+                kb = v.j_off[i];
+                nj = v.j_off[i + 1] - kb;
+            }
+        }
+    }
+    // ---- the body's snapshot and the initial block entries this lane owns
+    bool active = i < p.n;
+    float4 imm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 sp4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sr4 = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    float4 il4 = sp4, ia4 = sr4, nl4 = sp4, na4 = sr4;
+    if (active) {
+        imm = v.inv_mm[i];
+        sp4 = rp[i];
+        sr4 = rr[i];
+        il4 = v.init_lin[i];
+        ia4 = v.init_ang[i];
+        nl4 = v.iner_lin[i];
+        na4 = v.iner_ang[i];
+        active = imm.w != 0.0f;
+    }
+    if (!active) { nc = 0; nj = 0; }
+    const f3 snap_pos = xyz(sp4);
+    const q4 snap_rot = as_q(sr4);
+    const f3 dsl = active ? sub(snap_pos, xyz(il4)) : f3{0.0f, 0.0f, 0.0f};
+    const f3 dsa = active ? ang_diff(snap_rot, as_q(ia4)) : f3{0.0f, 0.0f, 0.0f};
+    if (lane == 0) {
+        body_f[slot][0] = f4(snap_pos, active ? 1.0f : 0.0f);
+        body_f[slot][1] = sr4;
+        body_f[slot][2] = f4(dsl, 0.0f);
+        body_f[slot][3] = f4(dsa, 0.0f);
+        slot_csr[slot] = make_uint4(i, jb, nc, kb);
+        slot_n[slot] = nc + nj;
+    }
+    float acc[kOwned];
+    {
+        // initial block: diag(M / dt^2, I / dt^2) and the inertial gradient
+        // (entry order: lin 0-5, ang 6-11, cross 12-20, rhs lin 21-23, rhs ang 24-26)
+        float m_dt2 = 0.0f;
+        f3 i_dt2{0.0f, 0.0f, 0.0f}, gl{0.0f, 0.0f, 0.0f}, ga{0.0f, 0.0f, 0.0f};
+        if (active) {
+            const float mass = 1.0f / imm.w;
+            const f3 moment{1.0f / imm.x, 1.0f / imm.y, 1.0f / imm.z};
+            const float dt2 = p.dt * p.dt;
+            m_dt2 = mass / dt2;
+            i_dt2 = divs(moment, dt2);
+            const f3 dl = sub(snap_pos, xyz(nl4));
+            const f3 ad = ang_diff(snap_rot, as_q(na4));
+            gl = f3{m_dt2 * dl.x, m_dt2 * dl.y, m_dt2 * dl.z};
+            ga = f3{i_dt2.x * ad.x, i_dt2.y * ad.y, i_dt2.z * ad.z};
+        }
+#pragma unroll
+        for (int t = 0; t < kOwned; ++t) {
+            const int e = lane + G * t;
+            float x = 0.0f;
+            x = (e == 0 || e == 3 || e == 5) ? m_dt2 : x;
+            x = e == 6 ? i_dt2.x : x;
+            x = e == 9 ? i_dt2.y : x;
+            x = e == 11 ? i_dt2.z : x;
+            x = e == 21 ? gl.x : x;
+            x = e == 22 ? gl.y : x;
+            x = e == 23 ? gl.z : x;
+            x = e == 24 ? ga.x : x;
+            x = e == 25 ? ga.y : x;
+            x = e == 26 ? ga.z : x;
+            acc[t] = x;
+        }
+    }
+    int pofs[kOwned], qofs[kOwned], wofs[kOwned];
+#pragma unroll
+    for (int t = 0; t < kOwned; ++t) {
+        const int e = lane + G * t;
+        pofs[t] = e < 27 ? (int)kP[e] : 0;
+        qofs[t] = e < 27 ? (int)kQ[e] : 0;
+        wofs[t] = e < 21 ? 7 : 8;
+    }
+    __syncthreads();
+    // ---- list starts of the slots (a prefix over the slots' entry counts)
+    uint32_t pstart[kPackBodies];
+    {
+        uint32_t run = 0;
+#pragma unroll
+        for (int s = 0; s < kPackBodies; ++s) {
+            pstart[s] = run;
+            run += slot_n[s];
+        }
+        if (!from_tab) {
+            total = run;
+            pg = pstart[slot];
+            ng = slot_n[slot];
+            if (lane == 0) {
+                wt[2 * wk] = make_uint4(i, pg, ng, total);
+                wt[2 * wk + 1] = make_uint4(jb, nc, kb, nj);
+            }
+        }
+    }
+    const float kk = 1.0f - p.alpha;
+    const uint32_t rounds = (total + kPackThreads - 1) / kPackThreads;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t f = r * kPackThreads + threadIdx.x;
+        float* st = stage[threadIdx.x];
+        int nrows = 0;
+        if (f < total) {
+            // ---- which item: its slot and place in the slot's range from the
+            // list starts; its contact and peer code (or constraint) from the
+            // item table, or from the CSR
+            uint32_t s = 0;
+#pragma unroll
+            for (int q = 1; q < kPackBodies; ++q) s += pstart[q] <= f ? 1u : 0u;
+            uint32_t ps = 0;
+#pragma unroll
+            for (int q = 0; q < kPackBodies; ++q) ps = (uint32_t)q == s ? pstart[q] : ps;
+            const uint32_t c = f - ps;
+            const uint4 sc = slot_csr[s];
+            const bool joint = c >= sc.z;
+            uint32_t x, y;
+            if (from_tab && f < (uint32_t)kPackCap) {
+                x = pt0.x;
+                y = pt0.y;
+            } else {
+                if (!joint) {
+                    x = v.indices[sc.y + c];
+                    y = v.peers[sc.y + c];
+                } else {
+                    x = v.j_idx[sc.w + (c - sc.z)];
+                    y = 0u;
+                }
+                if (!from_tab && f < (uint32_t)kPackCap) itab[f] = make_uint2(x, y);
+            }
+            RowBody rb;
+            {
+                const float4 b0 = body_f[s][0];
+                rb.pos = xyz(b0);
+                rb.rot = as_q(body_f[s][1]);
+                rb.dsl = xyz(body_f[s][2]);
+                rb.dsa = xyz(body_f[s][3]);
+                rb.i = slot_csr[s].x;
+            }
+            if (!joint) {
+                const ContactIn cin = load_contact(v, lam_in, pen_in, rp, rr, fuse, x, y);
+                stage_contact(v, p, kk, fuse, lam_out, pen_out, rb, cin, st);
+                nrows = 3;
+            } else {
+                nrows = stage_joint<HARD>(v, p, iter, fuse, rp, rr, rb, x, st);
+            }
+        }
+        if (nrows < 1) stage_empty_row(st, 0);
+        if (nrows < 2) stage_empty_row(st, 1);
+        if (nrows < 3) stage_empty_row(st, 2);
+        __syncthreads();
+        // ---- accumulate the owned entries over this round's part of the
+        // slot's range, in item order
+        if (active) {
+            const uint32_t r0 = r * kPackThreads;
+            const uint32_t lo = pg > r0 ? pg : r0;
+            const uint32_t end = pg + ng, r1 = r0 + kPackThreads;
+            const uint32_t hi = end < r1 ? end : r1;
+#pragma unroll 4
+            for (uint32_t q = lo; q < hi; ++q) {
+                const float* it = stage[q - r0];
+#pragma unroll
+                for (int rw = 0; rw < 3; ++rw) {
+                    const float* R = it + kRowFloats * rw;
+#pragma unroll
+                    for (int t = 0; t < kOwned; ++t) acc[t] = acc[t] + (R[pofs[t]] * R[qofs[t]]) * R[wofs[t]];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---- gather the 27 entries; one thread per body solves
+    if (active) {
+#pragma unroll
+        for (int t = 0; t < kOwned; ++t) {
+            const int e = lane + G * t;
+            if (e < 27) gathered[slot][e] = acc[t];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x >= (unsigned)kPackBodies) return;
+    const int t = threadIdx.x;
+    const float4 f0 = body_f[t][0];
+    if (f0.w == 0.0f) return;
+    const uint32_t bi = slot_csr[t].x;
+    const float* gs = gathered[t];
+    Block6 B;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { B.l[k] = gs[k]; B.a[k] = gs[6 + k]; }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) B.c[k] = gs[12 + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { B.rl[k] = gs[21 + k]; B.ra[k] = gs[24 + k]; }
+    float x[6];
+    solve6(B, x);
+    const q4 sr = as_q(body_f[t][1]);
+    const f3 new_pos = add(xyz(f0), f3{x[0], x[1], x[2]});
+    const q4 new_rot = quat_step(sr, f3{x[3], x[4], x[5]});
+    v.pos[1 - parity][bi] = f4(new_pos, 0.0f);
+    v.rot[1 - parity][bi] = f4q(new_rot);
+}
+
 // One dual update of contact ci from the Jacobi poses written by a primal of
 // parity `parity` (pos[1 - parity]): multipliers read from buffer src, stored
 // to buffer dst.  static_updates > 0 (fused schedule, final launch): a
@@ -1261,6 +1525,9 @@ void launch_k(F kernel, dim3 grid, dim3 block, hipStream_t s, const LaunchEvents
 }
 }  // namespace
 
+size_t primal_work_entries(uint32_t n) { return 2 * (((size_t)n + kPackBodies - 1) / kPackBodies) * kPackBodies; }
+size_t primal_item_entries(uint32_t n) { return (((size_t)n + kPackBodies - 1) / kPackBodies) * kPackCap; }
+
 void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t s, const LaunchEvents& ev) {
     if (p.n == 0) return;
     launch_k(k_avbd_init, dim3(blocks_for(p.n)), dim3(kBlock), s, ev, v, p);
@@ -1305,6 +1572,18 @@ void launch_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, hipSt
 void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, bool fused, hipStream_t s, int lanes,
                         const LaunchEvents& ev) {
     if (p.n == 0) return;
+    if (v.work_tab && v.item_tab) {
+        const dim3 grid((p.n + kPackBodies - 1) / kPackBodies);
+        const bool f = fused && iter > 0;
+        if (v.j_hard) {
+            if (f) launch_k(k_avbd_primal_packed<true, true>, grid, dim3(kPackThreads), s, ev, v, p, iter);
+            else launch_k(k_avbd_primal_packed<false, true>, grid, dim3(kPackThreads), s, ev, v, p, iter);
+        } else {
+            if (f) launch_k(k_avbd_primal_packed<true, false>, grid, dim3(kPackThreads), s, ev, v, p, iter);
+            else launch_k(k_avbd_primal_packed<false, false>, grid, dim3(kPackThreads), s, ev, v, p, iter);
+        }
+        return;
+    }
     if (fused && iter > 0) launch_primal<true>(v, p, iter, s, lanes, ev, p.n);
     else launch_primal<false>(v, p, iter, s, lanes, ev, p.n);
 }

@@ -269,7 +269,7 @@ struct dio_ctx {
     uint32_t *a_off = nullptr, *a_idx = nullptr, *a_peer = nullptr;
     JointDev a_j;
     dio_avbd_constraint* a_cons_stage = nullptr;
-    dio_tuning tuning{32, 0, 16, 1, 8, 0, 1, 0, 0};
+    dio_tuning tuning{32, 0, 16, 1, 8, 0, 1, 0, 0, 1};
     uint32_t g_lanes = 4;     // traversal lanes of the last granular broad phase
     uint64_t w_sorted_at = 0; // step of the last Morton re-sort of the world
     bool w_sort_pending = true;
@@ -334,6 +334,7 @@ struct dio_ctx {
     unsigned long long* w_acc = nullptr;  // sums of pairs, contacts, CSR nnz over a dio_step call
     float4 *w_lam_alt = nullptr, *w_pen_alt = nullptr;  // scratch multipliers of a solve
     uint4* w_work = nullptr;  // the Jacobi primal's work table (body and CSR ranges per work item)
+    uint2* w_items = nullptr;  // the packed primal's item table
     // instanced-renderer hand-off: per-body scale (xyz) and the slot order
     float4* inst_scale = nullptr;
     uint32_t* inst_order = nullptr;  // grown by dio_world_set_instances, freed with the context
@@ -1091,7 +1092,8 @@ void ensure_world(dio_ctx* c) {
     c->w_acc = A.alloc<unsigned long long>(4);
     c->w_lam_alt = A.alloc<float4>(m);
     c->w_pen_alt = A.alloc<float4>(m);
-    c->w_work = A.alloc<uint4>(n);
+    c->w_work = A.alloc<uint4>(primal_work_entries(n));
+    c->w_items = A.alloc<uint2>(primal_item_entries(n));
     c->inst_scale = A.alloc<float4>(n);
     c->w_stage_b = A.alloc<dio_gpu_body>(n);
     c->w_stage_s = A.alloc<dio_avbd_body_state>(n);
@@ -1206,6 +1208,7 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         v.stat_pairs = c->pend[bn_pend].p_dev;
         v.stat_acc = c->w_acc;
         v.work_tab = c->w_work;  // iteration 0 of the solve below fills it
+        if (c->tuning.primal_packed) v.item_tab = c->w_items;
         const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n, o.gamma, o.penalty_min);
         if (c->tuning.scene_solve && c->w_scene_bits && c->w_j.c == 0) {
             if (c->w_bundle_bodies != c->tuning.scene_solve)
@@ -1856,6 +1859,7 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         if (v.colour_lanes != 0 && v.colour_lanes != 1 && v.colour_lanes != 4 && v.colour_lanes != 8 &&
             v.colour_lanes != 16 && v.colour_lanes != 32 && v.colour_lanes != 64)
             invalid("colour_lanes must be 0 (primal_lanes), 1, 4, 8, 16, 32 or 64");
+        if (v.primal_packed > 1) invalid("primal_packed must be 0 or 1");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
@@ -2037,6 +2041,29 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
                 }
             }
             ctx->tuning.colour_lanes = (uint32_t)cbest_lanes;
+        }
+        // the packed primal at the chosen dual schedule and body order:
+        // iteration 0 fills its work and item tables (and writes pose buffer
+        // 1 of the same set), then iteration 1 is timed as the shapes above
+        {
+            SolveView vp = v;
+            vp.body_list = ctx->tuning.primal_order == 1 && ctx->w_n > 0 ? ctx->wb.perm : nullptr;
+            vp.list_count = ctx->tuning.primal_order == 1 ? ctx->w_n : 0;
+            vp.work_tab = ctx->w_work;
+            vp.item_tab = ctx->w_items;
+            const bool fz = ctx->tuning.fuse_dual == 1;
+            launch_avbd_primal(vp, p, 0, false, s, 32);
+            launch_avbd_primal(vp, p, 1, fz, s, 32);
+            DIO_HIP_CHECK(hipEventRecord(e0, s));
+            for (uint32_t r = 0; r < reps; ++r) launch_avbd_primal(vp, p, 1, fz, s, 32);
+            DIO_HIP_CHECK(hipEventRecord(e1, s));
+            DIO_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            DIO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            const float packed_us = 1000.0f * ms / (float)reps;
+            if (us_per_launch)
+                us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4 + DIO_TUNE_COLOUR_SHAPES] = packed_us;
+            ctx->tuning.primal_packed = packed_us < order_us[ctx->tuning.primal_order] ? 1u : 0u;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

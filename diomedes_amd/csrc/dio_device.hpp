@@ -107,6 +107,9 @@ struct SolveView {
     // iterations >= 1 read it (null: every iteration reads the order and the
     // offsets).  Holds at least n entries.
     uint4* work_tab;
+    // with work_tab: the packed primal's item table (kPackCap entries per
+    // workgroup), written by iteration 0 (null: the grouped primal)
+    uint2* item_tab;
     // when next.init_lin is set, the final launch (dual + recover) also runs
     // the next step's init from the recovered state into these buffers
     InitTarget next;
@@ -130,6 +133,10 @@ struct LaunchEvents {
 // update, then recover).  Multipliers after j updates of a fused solve live
 // in buffer j % 2 (0: the contact set's lam / pen, 1: the scratch pair).
 void launch_avbd_init(const SolveView& v, const AvbdParams& p, hipStream_t stream, const LaunchEvents& ev = {});
+// Entries of the Jacobi primal's work table (SolveView::work_tab) and of the
+// packed primal's item table (SolveView::item_tab) for n bodies.
+size_t primal_work_entries(uint32_t n);
+size_t primal_item_entries(uint32_t n);
 // lanes: lanes per body (1 = one lane per body; 4, 8, 16, 32 = grouped kernel)
 void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, bool fused, hipStream_t stream,
                         int lanes = 32, const LaunchEvents& ev = {});

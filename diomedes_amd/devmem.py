@@ -1,8 +1,10 @@
-"""Raw device memory for GPU tests, through the HIP runtime the library links.
+"""Raw device memory through the HIP runtime the library links.
 
-Used to hand the library device addresses (instance-transform outputs) and to
-export an allocation as a DMA-BUF file descriptor, the form a Vulkan renderer
-on AMD's drivers exports its buffer memory in, for the import test.
+For callers of the Python binding that need a device address to hand the
+library -- the destination of dio_world_instances -- without another GPU
+framework, and to export an allocation as a DMA-BUF file descriptor (the form
+a Vulkan renderer on AMD's drivers exports its buffer memory in), which
+dio_import_external_memory maps back.
 """
 
 from __future__ import annotations
@@ -21,7 +23,7 @@ def hip() -> ctypes.CDLL:
     """libamdhip64 by its soname: the instance the library already loaded."""
     global _hip
     if _hip is None:
-        from diomedes_amd import _lib
+        from . import _lib
 
         _lib.load()
         h = ctypes.CDLL("libamdhip64.so.7")

@@ -633,9 +633,13 @@ class PhysicsContext:
     def set_tuning(self, primal_lanes: int = 0, broad_lanes: int = 0, resort_interval: int = 0,
                    fuse_dual: int | None = None, adjoint_lanes: int = 0, overlap_broad: int | None = None,
                    primal_order: int | None = None, scene_solve: int | None = None,
-                   colour_lanes: int | None = None) -> None:
-        """Launch shapes and schedule (0 / None keeps the current value)."""
+                   colour_lanes: int | None = None, primal_packed: int | None = None) -> None:
+        """Launch shapes and schedule (0 / None keeps the current value).  A
+        primal_lanes given without primal_packed selects that many lanes per
+        body for the step's Jacobi primal (primal_packed 0)."""
         cur = self.tuning()
+        if primal_lanes and primal_packed is None:
+            primal_packed = 0
         t = R.Tuning()
         t.primal_lanes = primal_lanes or cur["primal_lanes"]
         t.broad_lanes = broad_lanes or cur["broad_lanes"]
@@ -646,6 +650,7 @@ class PhysicsContext:
         t.primal_order = cur["primal_order"] if primal_order is None else int(primal_order)
         t.scene_solve = cur["scene_solve"] if scene_solve is None else int(scene_solve)
         t.colour_lanes = cur["colour_lanes"] if colour_lanes is None else int(colour_lanes)
+        t.primal_packed = cur["primal_packed"] if primal_packed is None else int(primal_packed)
         self._check(self._lib.dio_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -653,14 +658,15 @@ class PhysicsContext:
         self._check(self._lib.dio_get_tuning(self._h, ctypes.byref(t)))
         return {"primal_lanes": t.primal_lanes, "broad_lanes": t.broad_lanes, "resort_interval": t.resort_interval,
                 "fuse_dual": t.fuse_dual, "adjoint_lanes": t.adjoint_lanes, "overlap_broad": t.overlap_broad,
-                "primal_order": t.primal_order, "scene_solve": t.scene_solve, "colour_lanes": t.colour_lanes}
+                "primal_order": t.primal_order, "scene_solve": t.scene_solve, "colour_lanes": t.colour_lanes,
+                "primal_packed": t.primal_packed}
 
     def autotune(self, options: AvbdOptions, reps: int = 20) -> dict:
         """Time every primal and broad-phase launch shape and both dual
         schedules on the current world, keep the fastest; returns the choice
         and the per-shape microseconds."""
         t = R.Tuning()
-        us = (ctypes.c_float * 22)()
+        us = (ctypes.c_float * 23)()
         o = options.to_c()
         self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
         u = list(us)
@@ -675,6 +681,8 @@ class PhysicsContext:
             "primal_order_us": {"index": u[14], "morton": u[15]},
             "colour_lanes": t.colour_lanes,
             "colour_us_per_iteration": dict(zip((1, 4, 8, 16, 32, 64), u[16:22])),
+            "primal_packed": t.primal_packed,
+            "packed_us": u[22],
         }
 
     def stats(self) -> R.StepStats:

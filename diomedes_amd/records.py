@@ -183,7 +183,7 @@ class Tuning(ctypes.Structure):
                 ("resort_interval", ctypes.c_uint32), ("fuse_dual", ctypes.c_uint32),
                 ("adjoint_lanes", ctypes.c_uint32), ("overlap_broad", ctypes.c_uint32),
                 ("primal_order", ctypes.c_uint32), ("scene_solve", ctypes.c_uint32),
-                ("colour_lanes", ctypes.c_uint32)]
+                ("colour_lanes", ctypes.c_uint32), ("primal_packed", ctypes.c_uint32)]
 
 
 def f32(x) -> float:

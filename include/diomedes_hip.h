@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define DIO_ABI_VERSION 1u
+#define DIO_ABI_VERSION 2u
 
 /* `b` of a contact against the container box (physics/gpu.rs:15, avbd/src/lib.rs:75) */
 #define DIO_CONTAINER 0xFFFFFFFFu
@@ -239,7 +239,7 @@ typedef struct dio_step_stats {
 #define DIO_TUNE_PRIMAL_SHAPES 6
 #define DIO_TUNE_BROAD_SHAPES 6
 #define DIO_TUNE_COLOUR_SHAPES 6
-#define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4 + DIO_TUNE_COLOUR_SHAPES)
+#define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4 + DIO_TUNE_COLOUR_SHAPES + 1)
 typedef struct dio_tuning {
     uint32_t primal_lanes;     /* lanes per body in the primal: 1, 4, 8, 16, 32 or 64 (0 = default 32) */
     uint32_t broad_lanes;      /* lanes per query in the BVH traversal: 1, 2, 4, 8, 16 or 32; with 16
@@ -272,6 +272,13 @@ typedef struct dio_tuning {
                                   worlds */
     uint32_t colour_lanes;     /* lanes per body in the coloured mode's primal (one launch per colour):
                                   1, 4, 8, 16, 32 or 64; 0 (default): primal_lanes */
+    uint32_t primal_packed;    /* the Jacobi primal of dio_step: 1 (default), packed -- a workgroup of 256
+                                  threads takes 8 bodies, every thread builds the rows of one item of their
+                                  joined item list (so a body with many contacts takes no second round
+                                  of loads), and 32 lanes per body add its block; primal_lanes is then
+                                  unused by the step.  0: primal_lanes lanes per body.  The result is the
+                                  same either way.  The granular solve, the coloured mode and the
+                                  adjoint's replay take primal_lanes */
 } dio_tuning;
 
 typedef struct dio_ctx dio_ctx;
@@ -391,7 +398,9 @@ dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out);
    faster; the shapes above are timed in the order current at the call); then
    the coloured mode's primal, one launch per colour of a colouring of the
    current contact graph, at lanes 1, 4, 8, 16, 32, 64 (microseconds per
-   iteration over all colours; colour_lanes is set to the fastest).  The
+   iteration over all colours; colour_lanes is set to the fastest); then the
+   packed primal at the chosen dual schedule and order (primal_packed is set
+   if it beats the chosen primal_lanes).  The
    primal with a fixed parity is idempotent, the broad timing runs in the
    granular broad buffers and the colouring in the coloured mode's buffers,
    so the world state is unchanged (dio_world_colours then reports that

@@ -50,7 +50,7 @@ pub const DIO_CONSTRAINT_HARD_BEND: u32 = 8;
 
 /// ABI version, the container index of a wall contact's `b`, the autotuner's slot counts, the
 /// allocation-free narrow phase's limits.
-pub const DIO_ABI_VERSION: u32 = 1;
+pub const DIO_ABI_VERSION: u32 = 2;
 pub const DIO_CONTAINER: u32 = 0xFFFFFFFF;
 pub const DIO_TUNE_PRIMAL_SHAPES: u32 = 6;
 pub const DIO_TUNE_BROAD_SHAPES: u32 = 6;
@@ -59,7 +59,7 @@ pub const DIO_NOALLOC_MAX_PAIRS: u32 = 65536;
 pub const DIO_NOALLOC_MAX_HULL_VERTICES: u32 = 16384;
 pub const DIO_NOALLOC_SCRATCH_BYTES: u32 = 47453184;
 pub const DIO_TUNE_COLOUR_SHAPES: u32 = 6;
-pub const DIO_TUNE_SLOTS: usize = 22;
+pub const DIO_TUNE_SLOTS: usize = 23;
 pub const DIO_ADJOINT_EXACT: u32 = 0;
 pub const DIO_ADJOINT_FROZEN_MULTIPLIERS: u32 = 1;
 
@@ -247,6 +247,7 @@ pub struct dio_tuning {
     pub primal_order: u32,
     pub scene_solve: u32,
     pub colour_lanes: u32,
+    pub primal_packed: u32,
 }
 
 /// The engine's names of the records it already defines (render/compute.rs, render/gpu_physics.rs).
@@ -272,7 +273,7 @@ const _: () = assert!(size_of::<dio_container>() == 36);
 const _: () = assert!(size_of::<dio_limits>() == 20);
 const _: () = assert!(size_of::<dio_step_opts>() == 12);
 const _: () = assert!(size_of::<dio_step_stats>() == 96);
-const _: () = assert!(size_of::<dio_tuning>() == 36);
+const _: () = assert!(size_of::<dio_tuning>() == 40);
 
 #[link(name = "diomedes_hip")]
 extern "C" {

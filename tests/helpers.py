@@ -54,3 +54,9 @@ def rel_err(a, b) -> float:
     b = np.asarray(b, np.float64)
     scale = max(1.0, float(np.max(np.abs(b))) if b.size else 1.0)
     return float(np.max(np.abs(a - b)) / scale) if a.size else 0.0
+
+
+def primal_tuning(lanes) -> dict:
+    """set_tuning arguments for a primal shape of the world step: "packed"
+    (the packed primal) or that many lanes per body."""
+    return {"primal_packed": 1} if lanes == "packed" else {"primal_lanes": lanes, "primal_packed": 0}

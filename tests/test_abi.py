@@ -82,7 +82,7 @@ def test_default_options_match_the_reference():
     assert (o.alpha, o.beta_lin, o.margin) == (np.float32(0.99), 10000.0, np.float32(0.01))
     assert (o.penalty_min, o.penalty_max, o.gamma, o.friction) == (1.0, np.float32(1e10), np.float32(0.999),
                                                                     np.float32(0.8))
-    assert lib.dio_abi_version() == 1
+    assert lib.dio_abi_version() == 2
 
 
 def test_invalid_arguments_return_status_without_a_device():

@@ -24,7 +24,7 @@ from diomedes_amd import PhysicsContext, scenes
 from diomedes_amd._lib import DioError
 
 from .helpers import assert_bits
-from .hipmem import DeviceBuffer
+from diomedes_amd.devmem import DeviceBuffer
 
 pytestmark = pytest.mark.gpu
 

@@ -13,7 +13,7 @@ from diomedes_amd import PhysicsContext, records as R, scenes
 from diomedes_amd._lib import DioError
 from diomedes_amd.physics import AvbdBody, AvbdOptions, HardBallSocket, HardRod, HardRope, Scene
 
-from .helpers import assert_bits
+from .helpers import assert_bits, primal_tuning
 
 pytestmark = pytest.mark.gpu
 
@@ -75,14 +75,14 @@ def test_hard_chains_coloured_bitexact(oracle, lanes):
         assert np.all(pen.max(axis=1) > 1.0)
 
 
-@pytest.mark.parametrize("lanes,fuse", [(32, 1), (8, 1), (1, 1), (32, 0)])
+@pytest.mark.parametrize("lanes,fuse", [(32, 1), (8, 1), (1, 1), (32, 0), ("packed", 1), ("packed", 0)])
 def test_pendulums_jacobi_bitexact(oracle, lanes, fuse):
     """Jacobi mode: the fused schedule (each joint's dual update inside the
     next primal, written by the dynamic body) and the unfused one."""
     sc = _pendulums()
     w = oracle.OracleWorld(sc, 32)
     with PhysicsContext(sc.n, max_constraints=len(sc.constraints)) as ctx:
-        ctx.set_tuning(primal_lanes=lanes, fuse_dual=fuse)
+        ctx.set_tuning(**primal_tuning(lanes), fuse_dual=fuse)
         ctx.upload(sc)
         for k in range(3):
             ctx.step(sc.options, 20, mode=0)
@@ -197,7 +197,7 @@ def test_bending_chains_bitexact(oracle, bend, lanes):
         _print(bend, "bend chains lanes", lanes, "contacts", ctx.stats().contacts, "colours", ctx.stats().colours)
 
 
-@pytest.mark.parametrize("lanes,fuse", [(32, 1), (1, 1), (32, 0)])
+@pytest.mark.parametrize("lanes,fuse", [(32, 1), (1, 1), (32, 0), ("packed", 1), ("packed", 0)])
 def test_cantilevers_jacobi_bitexact(oracle, lanes, fuse):
     """Single links held from static anchors by a hard socket and a hard bend
     (no joint between dynamic bodies: the Jacobi mode's fused and unfused
@@ -205,7 +205,7 @@ def test_cantilevers_jacobi_bitexact(oracle, lanes, fuse):
     sc = scenes.hard_chains(4, chains=12, links=1, bend="hard")
     w = oracle.OracleWorld(sc, 32)
     with PhysicsContext(sc.n, max_constraints=len(sc.constraints)) as ctx:
-        ctx.set_tuning(primal_lanes=lanes, fuse_dual=fuse)
+        ctx.set_tuning(**primal_tuning(lanes), fuse_dual=fuse)
         ctx.upload(sc)
         for k in range(3):
             ctx.step(sc.options, 20, mode=0)

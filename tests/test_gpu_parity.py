@@ -15,7 +15,7 @@ from diomedes_amd import PhysicsContext, scenes
 from diomedes_amd import records as R
 from diomedes_amd.physics import AvbdOptions
 
-from .helpers import assert_bits, contacts_to_records
+from .helpers import assert_bits, contacts_to_records, primal_tuning
 
 pytestmark = pytest.mark.gpu
 
@@ -135,7 +135,39 @@ def test_avbd_granular_solve_bitexact(oracle, lanes, fuse):
         assert_bits(got[k], ref[k], k)
 
 
-@pytest.mark.parametrize("lanes", [1, 8, 64])
+@pytest.mark.parametrize("spokes", [300, 700])
+@pytest.mark.parametrize("lanes", ["packed", 32])
+def test_hub_with_hundreds_of_items_bitexact(oracle, spokes, lanes):
+    """A hub body held by springs to `spokes` satellites: its workgroup's
+    joined item list passes 256 entries, so the packed primal builds it in
+    several rounds and, in iterations >= 1, takes the entries past its item
+    table from the CSR; the grouped primal takes the hub in many chunks."""
+    from diomedes_amd.physics import AvbdBody, Scene, Spring
+
+    rng = np.random.default_rng(spokes)
+    bodies = [AvbdBody.cube((0.0, 4.0, 0.0), 0.5, 20.0)]
+    cons = []
+    for k in range(spokes):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        bodies.append(AvbdBody.cube(tuple(np.array([0.0, 4.0, 0.0]) + 2.5 * d), 0.08, 0.05))
+        cons.append(Spring(0, k + 1, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), 40.0, 2.4))
+    opts = AvbdOptions(gravity=(0.0, -9.81, 0.0), iterations=6)
+    scene = Scene.from_bodies(bodies, cons, scenes.floor_container(20.0), opts)
+    w = oracle.OracleWorld(scene, 32)
+    with PhysicsContext(scene.n, max_constraints=len(cons)) as ctx:
+        ctx.set_tuning(**primal_tuning(lanes))
+        ctx.upload(scene)
+        for k in range(3):
+            w.step(opts, nsteps=15)
+            ctx.step(opts, 15)
+            got, ref = ctx.download(), w.state()
+            for key in ("positions", "orientations", "velocities", "angular_velocities"):
+                assert_bits(got[key], ref[key], f"{lanes} {spokes} spokes, step {15 * (k + 1)} {key}")
+        _print("hub", spokes, lanes, "contacts", ctx.stats().contacts)
+
+
+@pytest.mark.parametrize("lanes", [1, 8, 64, "packed"])
 def test_avbd_granular_joints_bitexact(oracle, lanes):
     """Springs, rods, a ball socket and a rope (the constraints demo's set,
     demos/constraints/src/main.rs:110-146) through one solve."""
@@ -158,7 +190,7 @@ def test_avbd_granular_joints_bitexact(oracle, lanes):
     scene = Scene.from_bodies(bodies, cons, None, opts)
     w = oracle.OracleWorld(scene, 32)
     with PhysicsContext(scene.n) as ctx:
-        ctx.set_tuning(primal_lanes=lanes)
+        ctx.set_tuning(**primal_tuning(lanes))
         ctx.upload(scene)
         for k in range(4):
             w.step(opts, nsteps=30)
@@ -259,7 +291,8 @@ def test_box_stack_300_steps_bitexact(oracle):
 @pytest.mark.parametrize(
     "lag,lanes,blanes,resort,fuse",
     [(1, 8, 4, 4, 0), (0, 8, 4, 4, 1), (1, 4, 2, 1, 0), (1, 16, 8, 1000, 1), (1, 1, 1, 4, 1), (0, 4, 8, 1000, 0),
-     (1, 32, 1, 4, 1), (1, 32, 8, 4, 0), (1, 64, 8, 4, 1), (1, 32, 16, 16, 1), (0, 32, 32, 4, 1)],
+     (1, 32, 1, 4, 1), (1, 32, 8, 4, 0), (1, 64, 8, 4, 1), (1, 32, 16, 16, 1), (0, 32, 32, 4, 1),
+     (1, "packed", 8, 4, 1), (1, "packed", 16, 16, 0), (0, "packed", 4, 1000, 1), (0, "packed", 1, 4, 0)],
 )
 def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort, fuse):
     """resort: Morton re-sort interval of the fused step; 1000 refits the first
@@ -268,7 +301,7 @@ def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort, fuse):
     scene = scenes.cube_field(7, nx=8, ny=2, nz=8)
     w = oracle.OracleWorld(scene, 32)
     with PhysicsContext(scene.n) as ctx:
-        ctx.set_tuning(primal_lanes=lanes, broad_lanes=blanes, resort_interval=resort, fuse_dual=fuse)
+        ctx.set_tuning(**primal_tuning(lanes), broad_lanes=blanes, resort_interval=resort, fuse_dual=fuse)
         ctx.upload(scene)
         for k in range(6):
             ctx.step(scene.options, 20, lag=lag)
@@ -323,7 +356,8 @@ def test_refit_carried_by_the_contact_build_at_depth_six(oracle):
         assert np.array_equal(q0, q1)
 
 
-@pytest.mark.parametrize("order,lanes,fuse", [(1, 32, 1), (1, 1, 0), (1, 8, 1), (0, 32, 1), (0, 1, 0)])
+@pytest.mark.parametrize("order,lanes,fuse", [(1, 32, 1), (1, 1, 0), (1, 8, 1), (0, 32, 1), (0, 1, 0),
+                                              (1, "packed", 1), (0, "packed", 1), (1, "packed", 0)])
 def test_primal_body_order_bitexact(oracle, order, lanes, fuse):
     """primal_order 1 (the default): the Jacobi primal takes the bodies in the
     broad phase's Morton order, one contiguous range per XCD; 0: in index
@@ -332,7 +366,7 @@ def test_primal_body_order_bitexact(oracle, order, lanes, fuse):
     w = oracle.OracleWorld(scene, 32)
     with PhysicsContext(scene.n) as ctx:
         assert ctx.tuning()["primal_order"] == 1
-        ctx.set_tuning(primal_lanes=lanes, fuse_dual=fuse, primal_order=order)
+        ctx.set_tuning(**primal_tuning(lanes), fuse_dual=fuse, primal_order=order)
         assert ctx.tuning()["primal_order"] == order
         ctx.upload(scene)
         for k in range(4):
