@@ -108,27 +108,20 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_partials(const uint32_t* in
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-// pass 2: one block scans the tile sums in place (exclusive)
-__global__ void __launch_bounds__(1024) k_scan_tiles(uint32_t* partials, uint32_t tiles) {
-    __shared__ uint32_t lds[1024 / kWave + 1];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < tiles; base += 1024) {
-        const uint32_t i = base + threadIdx.x;
-        uint32_t x = i < tiles ? partials[i] : 0;
-        uint32_t total;
-        uint32_t ex = block_exclusive<1024>(x, lds, &total);
-        if (i < tiles) partials[i] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0) partials[tiles] = carry;
-}
-
-// pass 3: scan each tile and add its offset
+// pass 2: scan each tile and add its offset, the sum of the tile sums before
+// it, which each workgroup adds up itself (integer sums: any order gives the
+// same result); workgroup 0 also writes the total.  Two launches in all.
 __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* in, uint32_t* out, uint32_t n_cap,
                                                            const uint32_t* n_dev, const uint32_t* partials,
                                                            uint32_t tiles) {
     __shared__ uint32_t lds[kScanBlock / kWave + 1];
     const uint32_t n = read_n(n_cap, n_dev);
+    const uint32_t used = (n + kScanTile - 1) / kScanTile;  // tiles holding elements (<= tiles)
+    const uint32_t upto = blockIdx.x == 0 ? used : (blockIdx.x < used ? blockIdx.x : used);
+    uint32_t part = 0;
+    for (uint32_t t = threadIdx.x; t < upto; t += kScanBlock) part += partials[t];
+    uint32_t before;
+    block_exclusive<kScanBlock>(part, lds, &before);  // the workgroup's sum of tile sums
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     const bool al = aligned16(in) && aligned16(out);
     uint32_t v[kScanItems];
@@ -137,9 +130,9 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* in, u
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) s += v[k];
     uint32_t total;
-    const uint32_t ex = block_exclusive<kScanBlock>(s, lds, &total) + partials[blockIdx.x];
+    const uint32_t ex = block_exclusive<kScanBlock>(s, lds, &total) + (blockIdx.x == 0 ? 0u : before);
     store8_exclusive(out, base, n, al, ex, v);
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partials[tiles];
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = before;  // workgroup 0 added every used tile
 }
 
 // One workgroup scans up to kSingleMax elements in tiles of kSingleTile with a
@@ -275,7 +268,6 @@ void launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n_cap, co
     }
     const uint32_t tiles = ceil_div(n_cap > 0 ? n_cap : 1, kScanTile);
     hipLaunchKernelGGL(k_scan_partials, dim3(tiles), dim3(kScanBlock), 0, s, in, n_cap, n_dev, partials);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, partials, tiles);
     hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanBlock), 0, s, in, out, n_cap, n_dev, partials, tiles);
 }
 
