@@ -181,7 +181,8 @@ def batched_leg(args, tuning: dict, compare_defaults: bool = True) -> dict:
         "tuning": used,
     }
     if chosen:
-        out["autotune_us"] = {k2: chosen[k2] for k2 in ("us_per_launch", "dual_schedule_us", "primal_order_us")}
+        out["autotune_us"] = {k2: chosen[k2] for k2 in ("us_per_launch", "dual_schedule_us", "primal_order_us",
+                                                        "packed_us")}
     if chosen and compare_defaults:
         dwall, dst, _, _, _ = run(False)
         out["library_defaults"] = {"value": n_dyn * iters * k / (dst.ms_avbd / 1000.0),
@@ -853,7 +854,7 @@ def main() -> int:
                      "ms_per_step": 1000.0 * awall / args.steps, "ms_per_step_avbd": ast_.ms_avbd / args.steps,
                      "tuning": aused,
                      "autotune_us": {k2: chosen[k2] for k2 in ("us_per_launch", "broad_us_per_pass",
-                                                              "dual_schedule_us", "primal_order_us")}}
+                                                              "dual_schedule_us", "primal_order_us", "packed_us")}}
         _progress(f"autotuned window done ({args.steps} steps)")
 
     k = args.steps
