@@ -7,13 +7,14 @@
 //  - sphere / sphere: the analytic contact of lib.rs:125-144;
 //  - everything else: GJK (gjk.rs) runs on one lane per pair; intersecting
 //    pairs are appended (with their final simplex) to a list that the EPA
-//    kernel consumes with one 64-lane wavefront per pair.  The EPA polytope
-//    (256 vertices, 512 faces; ~26 KB, too large for one lane's registers)
-//    lives in LDS, and the face loops of epa.rs — closest face, visibility,
-//    horizon search, compaction, stitching — are split across the lanes with
-//    ballot / shuffle prefix sums that keep the reference's sequential
-//    orders (first minimal face, horizon edges in face-then-edge order,
-//    kept faces in index order).
+//    kernel consumes: a first pass with 16 lanes per pair (four pairs per
+//    wavefront) and a polytope of 24 vertices and 32 faces, then the pairs
+//    that outgrow it again with one 64-lane wavefront per pair at the caps of
+//    epa.rs (256 vertices, 512 faces; ~26 KB).  The polytopes live in LDS,
+//    and the face loops of epa.rs — closest face, visibility, horizon search,
+//    compaction, stitching — are split across a pair's lanes with ballot
+//    prefix sums that keep the reference's sequential orders (first minimal
+//    face, horizon edges in face-then-edge order, kept faces in index order).
 #include <hip/hip_runtime.h>
 
 #include "narrow.hpp"
@@ -30,18 +31,20 @@ constexpr int kWave = 64;
 // every pair's result is the full-size algorithm's
 constexpr int kMaxV = 256;
 constexpr int kMaxF = 512;
-constexpr int kSmallV = 64;
-constexpr int kSmallF = 128;
+constexpr int kSmallV = 24;
+constexpr int kSmallF = 32;
+constexpr int kEpaLanes = 16;  // lanes per pair in the first pass (four pairs per wave)
 // A hull vertex pool of at most this many vertices (16 KB) is staged in LDS
 // by every workgroup of the GJK and EPA launches: the support scans then read
 // LDS instead of the L2 (the same values, so the same results).
 constexpr uint32_t kLdsPoolMax = 1024;
-// Workgroups of the EPA first pass (one wavefront each, taking pairs i,
-// i + grid, ...): a pair's cost varies with its iteration count, and a grid
-// far wider than the resident wavefronts lets the dispatcher even that out
-// (4096-scene table: 3.18, 3.12, 3.06 and 3.03 ms per step at 8192, 16384,
-// 32768 and 65536 workgroups; no better beyond).
+// Workgroups of the EPA first pass (one wavefront each, taking pairs 4 i ..
+// 4 i + 3, then 4 (i + grid) ..): a pair's cost varies with its iteration
+// count, and a grid far wider than the resident wavefronts lets the
+// dispatcher even that out.  The retry pass takes up to kRetryBlocks pairs at
+// once (one wavefront and 26 KB of LDS each).
 constexpr uint32_t kEpaBlocks = 65536;
+constexpr uint32_t kRetryBlocks = 2048;
 extern __shared__ float4 pool_lds[];
 
 // The pool the support functions read: staged in LDS (LDS_POOL) or in place.
@@ -287,71 +290,102 @@ template <int MV, int MF> struct EpaLds {
     uint32_t count;  // broadcast word
 };
 
-// Wave-wide reductions (all 64 lanes active).  Within each row of 16 lanes
-// four DPP steps (quad_perm xor 1, quad_perm xor 2, half-row mirror, row
-// mirror) leave every lane holding its row's result; the four rows' results
-// are then read from lanes 0, 16, 32 and 48.  The combining op returns one of
-// its operands (no NaNs reach it), so the result is one of the inputs.
-template <class Op> __device__ __forceinline__ uint32_t wave_reduce_bits(uint32_t x, Op op) {
+// A group of GL lanes (16 or 64) of a 64-lane wave works on one pair.
+template <int GL> struct LaneGroup {
+    int lane;         // lane within the group
+    int base;         // the group's first lane in the wave
+    uint64_t mask;    // the group's lanes
+    uint64_t below;   // the group's lanes below this one
+};
+template <int GL> __device__ __forceinline__ LaneGroup<GL> lane_group() {
+    static_assert(GL == 16 || GL == 64, "16 or 64 lanes per group");
+    LaneGroup<GL> g;
+    const int wl = (int)(threadIdx.x % (unsigned)kWave);
+    g.lane = wl % GL;
+    g.base = wl - g.lane;
+    g.mask = GL == 64 ? ~0ull : (0xFFFFull << g.base);
+    g.below = g.lane == 0 ? 0ull : ((~0ull >> (64 - g.lane)) << g.base);
+    return g;
+}
+template <int GL> __device__ __forceinline__ uint64_t group_ballot(const LaneGroup<GL>& g, bool x) {
+    return __ballot(x) & g.mask;
+}
+template <int GL> __device__ __forceinline__ uint32_t group_first(const LaneGroup<GL>& g, uint64_t bal) {
+    return (uint32_t)(__ffsll((long long)bal) - 1 - g.base);
+}
+
+// Reductions over a group, every lane of the group active.  Within each row
+// of 16 lanes four DPP steps (quad_perm xor 1, quad_perm xor 2, half-row
+// mirror, row mirror) leave every lane holding its row's result; for a
+// 64-lane group the four rows' results are then read from lanes 0, 16, 32 and
+// 48.  The combining op returns one of its operands (no NaNs reach it), so
+// the result is one of the inputs.
+template <int GL, class Op> __device__ __forceinline__ uint32_t group_reduce_bits(uint32_t x, Op op) {
     x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));
     x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));
     x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));
     x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));
+    if (GL == 16) return x;
     const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)x, 0);
     const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)x, 16);
     const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)x, 32);
     const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
     return op(op(r0, r1), op(r2, r3));
 }
-__device__ __forceinline__ float wave_max(float x) {
-    return __uint_as_float(wave_reduce_bits(__float_as_uint(x), [](uint32_t a, uint32_t b) {
+template <int GL> __device__ __forceinline__ float group_max(float x) {
+    return __uint_as_float(group_reduce_bits<GL>(__float_as_uint(x), [](uint32_t a, uint32_t b) {
         return __float_as_uint(fmaxf(__uint_as_float(a), __uint_as_float(b)));
     }));
 }
-__device__ __forceinline__ float wave_min(float x) {
-    return __uint_as_float(wave_reduce_bits(__float_as_uint(x), [](uint32_t a, uint32_t b) {
+template <int GL> __device__ __forceinline__ float group_min(float x) {
+    return __uint_as_float(group_reduce_bits<GL>(__float_as_uint(x), [](uint32_t a, uint32_t b) {
         return __float_as_uint(fminf(__uint_as_float(a), __uint_as_float(b)));
     }));
 }
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
-    return wave_reduce_bits(x, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
+template <int GL> __device__ __forceinline__ uint32_t group_min_u32(uint32_t x) {
+    return group_reduce_bits<GL>(x, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
 }
 constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
 
-// support_local of a hull with the wavefront's lanes splitting the vertices:
-// the same dot products, and the vertex the sequential scan keeps (the first
+// support_local of a hull with the group's lanes splitting the vertices: the
+// same dot products, and the vertex the sequential scan keeps (the first
 // maximal one, skipping NaNs; a NaN at vertex 0 is never replaced).  Each
-// lane keeps its first maximal vertex; the wave's maximum value is reduced
-// alone, and the first vertex holding it is the lowest lane's (up to 64
-// vertices, where vertex i is lane i's) or the least index.
-__device__ __forceinline__ f3 support_local_wave(const ShapeRef& s, f3 d, const HullPool& hp, int lane) {
+// lane keeps its first maximal vertex; the group's maximum value is reduced
+// alone, and the first vertex holding it is the lowest lane's (up to GL
+// vertices, where vertex i is lane i's) or the least index.  The shape is the
+// same on every lane of a group, so each group takes one side of the branch.
+template <int GL>
+__device__ __forceinline__ f3 support_local_group(const ShapeRef& s, f3 d, const HullPool& hp,
+                                                  const LaneGroup<GL>& g) {
     if (s.tag != kHull) return support_local(s, d, hp);
     const uint32_t first = (uint32_t)s.c[0].x, count = (uint32_t)s.c[0].y;
     float bd = -__builtin_inff();
     uint32_t bi = kNoIndex;
-    for (uint32_t i = (uint32_t)lane; i < count; i += kWave) {
+    for (uint32_t i = (uint32_t)g.lane; i < count; i += GL) {
         const float dd = dot3(xyz(hp.v[first + i]), d);
         if (dd == dd && (bi == kNoIndex || dd > bd)) { bd = dd; bi = i; }
     }
-    const float m = wave_max(bd);
+    const float m = group_max<GL>(bd);
     const bool at_max = bi != kNoIndex && bd == m;
     uint32_t win;
-    if (count <= (uint32_t)kWave) {
-        const uint64_t bal = __ballot(at_max);
-        win = bal ? (uint32_t)(__ffsll((long long)bal) - 1) : kNoIndex;
+    if (count <= (uint32_t)GL) {
+        const uint64_t bal = group_ballot(g, at_max);
+        win = bal ? group_first(g, bal) : kNoIndex;
     } else {
-        win = wave_min_u32(at_max ? bi : kNoIndex);
+        win = group_min_u32<GL>(at_max ? bi : kNoIndex);
     }
     const float d0 = dot3(xyz(hp.v[first]), d);
     if (win == kNoIndex || !(d0 == d0)) win = 0;
     return xyz(hp.v[first + win]);
 }
 
-// support_witness (gjk.rs:67-73) with wave-parallel hull supports
-__device__ __forceinline__ void support_witness_wave(const ShapeRef& a, const ShapeRef& b, f3 d, const HullPool& hp,
-                                                     int lane, f3& point, f3& witness) {
-    const f3 ap = add(qrot(a.rot, support_local_wave(a, qrot_inv(a.rot, d), hp, lane)), a.pos);
-    const f3 bp = add(qrot(b.rot, support_local_wave(b, qrot_inv(b.rot, neg(d)), hp, lane)), b.pos);
+// support_witness (gjk.rs:67-73) with group-parallel hull supports
+template <int GL>
+__device__ __forceinline__ void support_witness_group(const ShapeRef& a, const ShapeRef& b, f3 d,
+                                                      const HullPool& hp, const LaneGroup<GL>& g, f3& point,
+                                                      f3& witness) {
+    const f3 ap = add(qrot(a.rot, support_local_group<GL>(a, qrot_inv(a.rot, d), hp, g)), a.pos);
+    const f3 bp = add(qrot(b.rot, support_local_group<GL>(b, qrot_inv(b.rot, neg(d)), hp, g)), b.pos);
     point = sub(ap, bp);
     witness = ap;
 }
@@ -429,34 +463,43 @@ __device__ void face_contact(const L_t& L, uint32_t fi, f3& normal, float& depth
     pb = sub(pw, mk);
 }
 
-// One 64-lane workgroup per GJK-intersecting pair (epa.rs:89-189), with a
-// polytope of MV vertices and MF faces.  FIRST: the pairs of the GJK list; a
-// pair that would pass MV or MF (where the caps of epa.rs, 256 and 512, are
-// not yet reached) goes to the retry list instead of giving a result.
-// Otherwise (MV, MF = the caps of epa.rs): the pairs of the retry list.
-template <int MV, int MF, bool FIRST, bool LDS_POOL>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FIRST ? 5 : 1)))
+// EPA (epa.rs:89-189) with a group of GL lanes per GJK-intersecting pair and
+// 64 / GL pairs per one-wave workgroup, each with a polytope of MV vertices
+// and MF faces in LDS.  FIRST: the pairs of the GJK list; a pair that would
+// pass MV or MF (where the caps of epa.rs, 256 and 512, are not yet reached)
+// goes to the retry list instead of giving a result.  Otherwise (MV, MF = the
+// caps of epa.rs): the pairs of the retry list.  The groups of a wave run the
+// loop together; a group whose pair is done (or that has none) stays in it
+// with every update masked off until the wave's last pair is done.
+template <int GL, int MV, int MF, bool FIRST, bool LDS_POOL>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FIRST ? 4 : 1)))
 k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
-    constexpr int kFacesPerLane = MF / kWave;
-    __shared__ EpaLds<MV, MF> L;
-    const int lane = threadIdx.x;
-    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
+    constexpr int kGroups = kWave / GL;
+    constexpr int kFacesPerLane = MF / GL;
+    static_assert(MF % GL == 0, "faces split evenly over the lanes");
+    __shared__ EpaLds<MV, MF> polys[kGroups];
+    const LaneGroup<GL> g = lane_group<GL>();
+    const int lane = g.lane;
+    EpaLds<MV, MF>& L = polys[threadIdx.x / GL];
     const uint32_t jobs = FIRST ? *o.gjk_count : *o.epa_retry_count;
-    if (blockIdx.x >= jobs) return;  // uniform: a workgroup with no pair stages nothing
+    if (blockIdx.x * kGroups >= jobs) return;  // uniform: a workgroup with no pair stages nothing
     const HullPool hp = stage_pool<LDS_POOL>(bv);
-    for (uint32_t job = blockIdx.x; job < jobs; job += gridDim.x) {
-        const uint32_t p = FIRST ? o.gjk_list[job] : o.epa_retry_list[job];
+    for (uint32_t j0 = blockIdx.x * kGroups; j0 < jobs; j0 += gridDim.x * kGroups) {
+        const uint32_t job = j0 + threadIdx.x / GL;
+        const bool has = job < jobs;  // the same on every lane of a group
+        // a group with no pair computes on the round's first pair, and stores nothing
+        const uint32_t p = FIRST ? o.gjk_list[has ? job : j0] : o.epa_retry_list[has ? job : j0];
         const uint2 pr = o.pairs[p];
         const ShapeRef a = load_shape(bv, pr.x);
         const ShapeRef b = load_shape(bv, pr.y);
-        if (lane < 4) {
+        if (has && lane < 4) {
             const float4 sp = simplex_store[8 * (size_t)p + lane];
             const float4 sw = simplex_store[8 * (size_t)p + 4 + lane];
             L.vx[lane] = sp.x; L.vy[lane] = sp.y; L.vz[lane] = sp.z;
             L.wx[lane] = sw.x; L.wy[lane] = sw.y; L.wz[lane] = sw.z;
         }
         __syncthreads();
-        if (lane == 0) {
+        if (has && lane == 0) {
             // Polytope::new zeroes the faces; then the tetrahedron's four faces
             L.fa[0] = 0; L.fb[0] = 0; L.fc[0] = 0;
             L.fnx[0] = 0.0f; L.fny[0] = 0.0f; L.fnz[0] = 0.0f; L.fd[0] = 0.0f;
@@ -469,26 +512,28 @@ k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
             L.count = fc;
         }
         __syncthreads();
-        uint32_t face_count = L.count;
+        uint32_t face_count = has ? L.count : 0u;
         uint32_t vertex_count = 4;
         int result_face = -1;
         bool retry = false;
+        bool live = has;
         for (int it = 0; it < 64; ++it) {
+            if (__ballot(live) == 0ull) break;  // every lane is active here: uniform
             // closest face: first minimal distance in index order (strict <).
             // The minimum value is reduced alone; the first face holding it is
-            // found by ballots, face group by face group (face k * 64 + lane)
+            // found by ballots, face group by face group (face k * GL + lane)
             float md = __builtin_inff();
             float fdk[kFacesPerLane];
             for (int k = 0; k < kFacesPerLane; ++k) {
-                const uint32_t f = (uint32_t)(k * kWave + lane);
+                const uint32_t f = (uint32_t)(k * GL + lane);
                 fdk[k] = f < face_count ? L.fd[f] : __builtin_nanf("");
                 if (fdk[k] < md) md = fdk[k];
             }
-            const float mval = wave_min(md);
+            const float mval = group_min<GL>(md);
             uint32_t mi = kNoIndex;
             for (int k = 0; k < kFacesPerLane && mi == kNoIndex; ++k) {
-                const uint64_t bal = __ballot(fdk[k] == mval);
-                if (bal) mi = (uint32_t)(k * kWave + __ffsll((long long)bal) - 1);
+                const uint64_t bal = group_ballot(g, fdk[k] == mval);
+                if (bal) mi = (uint32_t)(k * GL) + group_first(g, bal);
             }
             // the sequential scan starts from face 0: a NaN there is never
             // replaced (nor is face 0 when no distance is a number)
@@ -496,41 +541,45 @@ k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
             const f3 mn{L.fnx[mi], L.fny[mi], L.fnz[mi]};
             const float mdist = L.fd[mi];
             f3 sp, sw;
-            support_witness_wave(a, b, mn, hp, lane, sp, sw);
+            support_witness_group<GL>(a, b, mn, hp, g, sp, sw);
             const float sd = dot3(mn, sp);
-            if (sd - mdist < 1e-4f) { result_face = (int)mi; break; }
             const uint32_t ni = vertex_count;
-            if (ni >= (uint32_t)MV) {
-                if (MV < kMaxV) retry = true;
-                else result_face = (int)mi;
-                break;
+            if (live) {
+                if (sd - mdist < 1e-4f) {
+                    result_face = (int)mi;
+                    live = false;
+                } else if (ni >= (uint32_t)MV) {
+                    if (MV < kMaxV) retry = true;
+                    else result_face = (int)mi;
+                    live = false;
+                }
             }
-            if (lane == 0) {
+            if (live && lane == 0) {
                 L.vx[ni] = sp.x; L.vy[ni] = sp.y; L.vz[ni] = sp.z;
                 L.wx[ni] = sw.x; L.wy[ni] = sw.y; L.wz[ni] = sw.z;
             }
-            vertex_count += 1;
+            if (live) vertex_count += 1;
             // visibility of each face from the new vertex, and the list of the
             // visible faces in index order
             uint32_t nvis = 0;
             for (int k = 0; k < kFacesPerLane; ++k) {
-                const uint32_t f = (uint32_t)(k * kWave + lane);
+                const uint32_t f = (uint32_t)(k * GL + lane);
                 bool v = false;
-                if (f < face_count) {
+                if (live && f < face_count) {
                     const f3 fn{L.fnx[f], L.fny[f], L.fnz[f]};
                     v = dot3(fn, sub(sp, vtx(L, L.fa[f]))) > 0.0f;
                     L.vis[f] = v ? 1u : 0u;
                 }
-                const uint64_t bal = __ballot(v);
-                if (v) L.vlist[nvis + (uint32_t)__popcll(bal & lt)] = (uint16_t)f;
+                const uint64_t bal = group_ballot(g, v);
+                if (v) L.vlist[nvis + (uint32_t)__popcll(bal & g.below)] = (uint16_t)f;
                 nvis += (uint32_t)__popcll(bal);
             }
             __syncthreads();
             // horizon: edges of visible faces that no other visible face has,
-            // in (face, edge) order, at most kMaxF of them.  Candidate c is
-            // edge c % 3 of visible face c / 3; only visible faces can share it.
+            // in (face, edge) order, at most MF of them.  Candidate c is edge
+            // c % 3 of visible face c / 3; only visible faces can share it.
             uint32_t hbase = 0;
-            for (uint32_t cb = 0; cb < 3 * nvis; cb += kWave) {
+            for (uint32_t cb = 0; cb < 3 * nvis; cb += GL) {
                 const uint32_t c = cb + (uint32_t)lane;
                 bool hz = false;
                 uint32_t ea = 0, eb = 0;
@@ -541,27 +590,27 @@ k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
                     eb = q == 0 ? fb : (q == 1 ? fcc : fa);
                     bool shared = false;
                     for (uint32_t t = 0; t < nvis && !shared; ++t) {
-                        const uint32_t g = L.vlist[t];
-                        if (g != f && edge_in(ea, eb, L.fa[g], L.fb[g], L.fc[g])) shared = true;
+                        const uint32_t h = L.vlist[t];
+                        if (h != f && edge_in(ea, eb, L.fa[h], L.fb[h], L.fc[h])) shared = true;
                     }
                     hz = !shared;
                 }
-                const uint64_t bal = __ballot(hz);
-                const uint32_t pos = hbase + (uint32_t)__popcll(bal & lt);
+                const uint64_t bal = group_ballot(g, hz);
+                const uint32_t pos = hbase + (uint32_t)__popcll(bal & g.below);
                 if (hz && pos < (uint32_t)MF) { L.ha[pos] = ea; L.hb[pos] = eb; }
                 hbase += (uint32_t)__popcll(bal);
             }
-            if (MF < kMaxF && hbase > (uint32_t)MF) {
+            if (live && MF < kMaxF && hbase > (uint32_t)MF) {
                 retry = true;
-                break;
+                live = false;
             }
             const uint32_t hcount = hbase < (uint32_t)MF ? hbase : (uint32_t)MF;
             // compact the kept (non-visible) faces in index order
             FaceR keep[kFacesPerLane];
             uint32_t kflag = 0;
             for (int k = 0; k < kFacesPerLane; ++k) {
-                const uint32_t f = (uint32_t)(k * kWave + lane);
-                if (f < face_count && !L.vis[f]) {
+                const uint32_t f = (uint32_t)(k * GL + lane);
+                if (live && f < face_count && !L.vis[f]) {
                     kflag |= 1u << k;
                     keep[k].a = L.fa[f]; keep[k].b = L.fb[f]; keep[k].c = L.fc[f];
                     keep[k].n = f3{L.fnx[f], L.fny[f], L.fnz[f]};
@@ -572,27 +621,28 @@ k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
             uint32_t wbase = 0;
             for (int k = 0; k < kFacesPerLane; ++k) {
                 const bool kf = (kflag >> k) & 1u;
-                const uint64_t bal = __ballot(kf);
-                if (kf) store_face(L, wbase + (uint32_t)__popcll(bal & lt), keep[k]);
+                const uint64_t bal = group_ballot(g, kf);
+                if (kf) store_face(L, wbase + (uint32_t)__popcll(bal & g.below), keep[k]);
                 wbase += (uint32_t)__popcll(bal);
             }
-            face_count = wbase;
+            if (live) face_count = wbase;
             __syncthreads();
             // stitch the horizon to the new vertex (epa.rs:182-185)
-            for (uint32_t hb0 = 0; hb0 < hcount; hb0 += kWave) {
+            const uint32_t hn = live ? hcount : 0u;
+            for (uint32_t hb0 = 0; hb0 < hn; hb0 += GL) {
                 const uint32_t h = hb0 + lane;
                 FaceR f;
                 f.ok = false;
-                if (h < hcount) f = make_face(L, ni, L.ha[h], L.hb[h]);
-                const uint64_t bal = __ballot(f.ok);
-                const uint32_t slot = face_count + (uint32_t)__popcll(bal & lt);
+                if (h < hn) f = make_face(L, ni, L.ha[h], L.hb[h]);
+                const uint64_t bal = group_ballot(g, f.ok);
+                const uint32_t slot = face_count + (uint32_t)__popcll(bal & g.below);
                 if (f.ok && slot < (uint32_t)MF) store_face(L, slot, f);
                 const uint32_t added = face_count + (uint32_t)__popcll(bal);
                 if (MF < kMaxF && added > (uint32_t)MF) retry = true;
                 face_count = added < (uint32_t)MF ? added : (uint32_t)MF;
             }
+            if (retry) live = false;
             __syncthreads();
-            if (retry) break;
         }
         if (retry && lane == 0) o.epa_retry_list[atomicAdd(o.epa_retry_count, 1u)] = p;
         if (result_face >= 0 && lane == 0) {
@@ -641,16 +691,21 @@ void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, h
     if (lds) hipLaunchKernelGGL(k_narrow<true>, dim3(blocks), dim3(kBlock), pool_bytes, s, bv, o, simplex_store);
     else hipLaunchKernelGGL(k_narrow<false>, dim3(blocks), dim3(kBlock), 0, s, bv, o, simplex_store);
     if (gjk_possible) {
-        uint32_t eblocks = o.p_max > 0 ? o.p_max : 1;
+        const uint32_t np = o.p_max > 0 ? o.p_max : 1;
+        constexpr uint32_t kPairsPerWave = kWave / kEpaLanes;
+        uint32_t eblocks = (np + kPairsPerWave - 1) / kPairsPerWave;
         if (eblocks > kEpaBlocks) eblocks = kEpaBlocks;
-        const uint32_t rblocks = eblocks < 512 ? eblocks : 512;
+        const uint32_t rblocks = np < kRetryBlocks ? np : kRetryBlocks;
         const float4* ss = simplex_store;
         if (lds) {
-            hipLaunchKernelGGL((k_epa<kSmallV, kSmallF, true, true>), dim3(eblocks), dim3(kWave), pool_bytes, s, bv, o, ss);
-            hipLaunchKernelGGL((k_epa<kMaxV, kMaxF, false, true>), dim3(rblocks), dim3(kWave), pool_bytes, s, bv, o, ss);
+            hipLaunchKernelGGL((k_epa<kEpaLanes, kSmallV, kSmallF, true, true>), dim3(eblocks), dim3(kWave), pool_bytes, s,
+                               bv, o, ss);
+            hipLaunchKernelGGL((k_epa<kWave, kMaxV, kMaxF, false, true>), dim3(rblocks), dim3(kWave), pool_bytes, s, bv,
+                               o, ss);
         } else {
-            hipLaunchKernelGGL((k_epa<kSmallV, kSmallF, true, false>), dim3(eblocks), dim3(kWave), 0, s, bv, o, ss);
-            hipLaunchKernelGGL((k_epa<kMaxV, kMaxF, false, false>), dim3(rblocks), dim3(kWave), 0, s, bv, o, ss);
+            hipLaunchKernelGGL((k_epa<kEpaLanes, kSmallV, kSmallF, true, false>), dim3(eblocks), dim3(kWave), 0, s, bv, o,
+                               ss);
+            hipLaunchKernelGGL((k_epa<kWave, kMaxV, kMaxF, false, false>), dim3(rblocks), dim3(kWave), 0, s, bv, o, ss);
         }
     }
     launch_scan_exclusive(o.cnt, o.off, o.p_max, o.p_dev, o.scan_partials, s);
