@@ -182,7 +182,7 @@ def batched_leg(args, tuning: dict, compare_defaults: bool = True) -> dict:
     }
     if chosen:
         out["autotune_us"] = {k2: chosen[k2] for k2 in ("us_per_launch", "dual_schedule_us", "primal_order_us",
-                                                        "packed_us")}
+                                                        "packed_us", "span_us")}
     if chosen and compare_defaults:
         dwall, dst, _, _, _ = run(False)
         out["library_defaults"] = {"value": n_dyn * iters * k / (dst.ms_avbd / 1000.0),
@@ -586,7 +586,9 @@ def roofline_of(g: dict, tuning: dict) -> dict:
     pst = g["profile"]
     lanes = tuning["primal_lanes"]
     fz = '*' if tuning['fuse_dual'] else 'false'
-    if tuning.get("primal_packed"):
+    if tuning.get("primal_packed") == 2:
+        kname = f"k_avbd_primal_span<{fz}>"
+    elif tuning.get("primal_packed"):
         kname = f"k_avbd_primal_packed<{fz}, false>"
     elif lanes > 1:
         kname = f"k_avbd_primal_grouped<{lanes}, {fz}>"
@@ -854,7 +856,8 @@ def main() -> int:
                      "ms_per_step": 1000.0 * awall / args.steps, "ms_per_step_avbd": ast_.ms_avbd / args.steps,
                      "tuning": aused,
                      "autotune_us": {k2: chosen[k2] for k2 in ("us_per_launch", "broad_us_per_pass",
-                                                              "dual_schedule_us", "primal_order_us", "packed_us")}}
+                                                              "dual_schedule_us", "primal_order_us", "packed_us",
+                                                              "span_us")}}
         _progress(f"autotuned window done ({args.steps} steps)")
 
     k = args.steps

@@ -1273,6 +1273,155 @@ __global__ void __launch_bounds__(kPackThreads) k_avbd_primal_packed(SolveView v
     v.rot[1 - parity][bi] = f4q(new_rot);
 }
 
+// Span primal (the Jacobi solve of a step for bodies with few contacts; no
+// counterpart in the reference's thread-per-body kernel).  A workgroup takes
+// kBlock consecutive bodies in index order, one thread each, and with them one
+// contiguous range of the CSR, [offsets[i0], offsets[i0 + kBlock]).  In round
+// r thread t builds the rows of range entry r * kBlock + t and stages them in
+// LDS: consecutive threads read consecutive CSR entries, and a body's
+// contacts with the container or as body a are consecutive records, so the
+// gathers that one thread per body would make entry by entry become
+// neighbouring loads.  Each thread then adds the staged rows of its own
+// body's entries, in CSR order, into its block (Block6::row: per entry the
+// operations of the one-lane loop, in its order) and, after the last round,
+// solves.  Worlds without joints.
+constexpr int kSpanBodies = 256;  // bodies (threads) per workgroup of the span primal
+
+template <bool FUSED>
+__global__ void __launch_bounds__(kSpanBodies) k_avbd_primal_span(SolveView v, AvbdParams p, uint32_t iter) {
+    constexpr uint32_t SB = kSpanBodies;
+    constexpr int kSteps = __builtin_ctz(SB);  // binary-search steps over the slot starts
+    const uint32_t parity = iter & 1u;
+    const bool fuse = FUSED && iter > 0;
+    const float4* __restrict__ lam_in = lam_buf(v, fuse ? iter - 1 : 0);
+    const float4* __restrict__ pen_in = pen_buf(v, fuse ? iter - 1 : 0);
+    float4* lam_out = lam_buf(v, iter);
+    float4* pen_out = pen_buf(v, iter);
+    __shared__ __align__(16) float stage[SB][kItemFloats];
+    __shared__ float4 body_f[SB][4];  // pos (.w: active), rot, dsl, dsa of each thread's body
+    __shared__ uint32_t offs[SB + 1];  // the bodies' CSR starts, and the range's end
+    const float4* __restrict__ rp = v.pos[parity];
+    const float4* __restrict__ rr = v.rot[parity];
+    const uint32_t t = threadIdx.x;
+    const uint32_t i0 = work_chunk(v) * SB;
+    const uint32_t i = i0 + t;
+    offs[t] = v.offsets[i < p.n ? i : p.n];
+    if (t == 0) offs[SB] = v.offsets[i0 + SB < p.n ? i0 + SB : p.n];
+    // ---- the body's snapshot and initial block (as the one-lane loop)
+    bool active = i < p.n;
+    float4 imm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 sp4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sr4 = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    float4 il4 = sp4, ia4 = sr4, nl4 = sp4, na4 = sr4;
+    if (active) {
+        imm = v.inv_mm[i];
+        sp4 = rp[i];
+        sr4 = rr[i];
+        il4 = v.init_lin[i];
+        ia4 = v.init_ang[i];
+        nl4 = v.iner_lin[i];
+        na4 = v.iner_ang[i];
+    }
+    active = active && imm.w != 0.0f;
+    const f3 snap_pos = xyz(sp4);
+    const q4 snap_rot = as_q(sr4);
+    Block6 B;
+    {
+        const float mass = 1.0f / imm.w;
+        const f3 moment{1.0f / imm.x, 1.0f / imm.y, 1.0f / imm.z};
+        const float dt2 = p.dt * p.dt;
+        const float m_dt2 = mass / dt2;
+        const f3 i_dt2 = divs(moment, dt2);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) { B.l[k] = 0.0f; B.a[k] = 0.0f; }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) B.c[k] = 0.0f;
+        B.l[0] = m_dt2; B.l[3] = m_dt2; B.l[5] = m_dt2;
+        B.a[0] = i_dt2.x; B.a[3] = i_dt2.y; B.a[5] = i_dt2.z;
+        const f3 rl = scale(sub(snap_pos, xyz(nl4)), 1.0f);
+        B.rl[0] = m_dt2 * rl.x; B.rl[1] = m_dt2 * rl.y; B.rl[2] = m_dt2 * rl.z;
+        const f3 ad = ang_diff(snap_rot, as_q(na4));
+        B.ra[0] = i_dt2.x * ad.x; B.ra[1] = i_dt2.y * ad.y; B.ra[2] = i_dt2.z * ad.z;
+    }
+    const f3 dsl = active ? sub(snap_pos, xyz(il4)) : f3{0.0f, 0.0f, 0.0f};
+    const f3 dsa = active ? ang_diff(snap_rot, as_q(ia4)) : f3{0.0f, 0.0f, 0.0f};
+    body_f[t][0] = f4(snap_pos, active ? 1.0f : 0.0f);
+    body_f[t][1] = sr4;
+    body_f[t][2] = f4(dsl, 0.0f);
+    body_f[t][3] = f4(dsa, 0.0f);
+    __syncthreads();
+    const uint32_t o0 = offs[0], o1 = offs[SB];
+    const uint32_t my0 = offs[t], my1 = offs[t + 1];
+    const float kk = 1.0f - p.alpha;
+    // the CSR entry of this thread's first round, fetched ahead
+    uint32_t nci = 0, npc = 0;
+    if (o0 + t < o1) {
+        nci = v.indices[o0 + t];
+        npc = v.peers[o0 + t];
+    }
+    for (uint32_t r0 = o0; r0 < o1; r0 += SB) {
+        const uint32_t f = r0 + t;
+        const uint32_t ci = nci, pc = npc;
+        // the next round's entry, in flight while this round is built and added
+        if (f + SB < o1) {
+            nci = v.indices[f + SB];
+            npc = v.peers[f + SB];
+        }
+        float* st = stage[t];
+        bool built = false;
+        if (f < o1) {
+            // ---- the entry's body: the last slot whose start is at or below f
+            // (offs[0] <= f < offs[SB]; an empty slot never ends the search)
+            uint32_t s = 0, hi = SB;
+#pragma unroll
+            for (int step = 0; step < kSteps; ++step) {
+                const uint32_t mid = (s + hi) >> 1;
+                if (offs[mid] <= f) s = mid;
+                else hi = mid;
+            }
+            const float4 b0 = body_f[s][0];
+            if (b0.w != 0.0f) {
+                RowBody rb;
+                rb.pos = xyz(b0);
+                rb.rot = as_q(body_f[s][1]);
+                rb.dsl = xyz(body_f[s][2]);
+                rb.dsa = xyz(body_f[s][3]);
+                rb.i = i0 + s;
+                const ContactIn cin = load_contact(v, lam_in, pen_in, rp, rr, fuse, ci, pc);
+                stage_contact(v, p, kk, fuse, lam_out, pen_out, rb, cin, st);
+                built = true;
+            }
+        }
+        if (!built) {
+            stage_empty_row(st, 0);
+            stage_empty_row(st, 1);
+            stage_empty_row(st, 2);
+        }
+        __syncthreads();
+        // ---- this body's entries of the round, in CSR order
+        if (active) {
+            const uint32_t lo = my0 > r0 ? my0 : r0;
+            const uint32_t end = r0 + SB;
+            const uint32_t hi = my1 < end ? my1 : end;
+            for (uint32_t q = lo; q < hi; ++q) {
+                const float* it = stage[q - r0];
+#pragma unroll
+                for (int rw = 0; rw < 3; ++rw) {
+                    const float* R = it + kRowFloats * rw;
+                    B.row(f3{R[0], R[1], R[2]}, f3{R[3], R[4], R[5]}, R[6], R[7]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    float x[6];
+    solve6(B, x);
+    const f3 new_pos = add(snap_pos, f3{x[0], x[1], x[2]});
+    const q4 new_rot = quat_step(snap_rot, f3{x[3], x[4], x[5]});
+    v.pos[1 - parity][i] = f4(new_pos, 0.0f);
+    v.rot[1 - parity][i] = f4q(new_rot);
+}
+
 // One dual update of contact ci from the Jacobi poses written by a primal of
 // parity `parity` (pos[1 - parity]): multipliers read from buffer src, stored
 // to buffer dst.  static_updates > 0 (fused schedule, final launch): a
@@ -1572,6 +1721,12 @@ void launch_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, hipSt
 void launch_avbd_primal(const SolveView& v, const AvbdParams& p, uint32_t iter, bool fused, hipStream_t s, int lanes,
                         const LaunchEvents& ev) {
     if (p.n == 0) return;
+    if (v.span_primal && v.j_count == 0) {
+        const dim3 grid((p.n + kSpanBodies - 1) / kSpanBodies);
+        if (fused && iter > 0) launch_k(k_avbd_primal_span<true>, grid, dim3(kSpanBodies), s, ev, v, p, iter);
+        else launch_k(k_avbd_primal_span<false>, grid, dim3(kSpanBodies), s, ev, v, p, iter);
+        return;
+    }
     if (v.work_tab && v.item_tab) {
         const dim3 grid((p.n + kPackBodies - 1) / kPackBodies);
         const bool f = fused && iter > 0;

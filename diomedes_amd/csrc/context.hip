@@ -1208,7 +1208,8 @@ void enqueue_segment(dio_ctx* c, int seg, int s, const dio_avbd_options& o, cons
         v.stat_pairs = c->pend[bn_pend].p_dev;
         v.stat_acc = c->w_acc;
         v.work_tab = c->w_work;  // iteration 0 of the solve below fills it
-        if (c->tuning.primal_packed) v.item_tab = c->w_items;
+        if (c->tuning.primal_packed == 2 && c->w_j.c == 0) v.span_primal = true;  // worlds without joints
+        else if (c->tuning.primal_packed) v.item_tab = c->w_items;
         const AvbdParams p = params_of(o.dt, o.gravity, o.alpha, o.beta_lin, o.penalty_max, n, o.gamma, o.penalty_min);
         if (c->tuning.scene_solve && c->w_scene_bits && c->w_j.c == 0) {
             if (c->w_bundle_bodies != c->tuning.scene_solve)
@@ -1859,7 +1860,7 @@ dio_status dio_set_tuning(dio_ctx* ctx, const dio_tuning* t) {
         if (v.colour_lanes != 0 && v.colour_lanes != 1 && v.colour_lanes != 4 && v.colour_lanes != 8 &&
             v.colour_lanes != 16 && v.colour_lanes != 32 && v.colour_lanes != 64)
             invalid("colour_lanes must be 0 (primal_lanes), 1, 4, 8, 16, 32 or 64");
-        if (v.primal_packed > 1) invalid("primal_packed must be 0 or 1");
+        if (v.primal_packed > 2) invalid("primal_packed must be 0, 1 or 2");
         DIO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = v;
         ctx->drop_graphs();
@@ -2044,7 +2045,9 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
         }
         // the packed primal at the chosen dual schedule and body order:
         // iteration 0 fills its work and item tables (and writes pose buffer
-        // 1 of the same set), then iteration 1 is timed as the shapes above
+        // 1 of the same set), then iteration 1 is timed as the shapes above;
+        // then the span primal; the fastest of these and the chosen grouped
+        // shape is kept
         {
             SolveView vp = v;
             vp.body_list = ctx->tuning.primal_order == 1 && ctx->w_n > 0 ? ctx->wb.perm : nullptr;
@@ -2063,7 +2066,48 @@ dio_status dio_autotune(dio_ctx* ctx, const dio_avbd_options* options, uint32_t 
             const float packed_us = 1000.0f * ms / (float)reps;
             if (us_per_launch)
                 us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4 + DIO_TUNE_COLOUR_SHAPES] = packed_us;
-            ctx->tuning.primal_packed = packed_us < order_us[ctx->tuning.primal_order] ? 1u : 0u;
+            // per-iteration cost: the primal, plus a dual launch when unfused
+            // (the difference of the unfused schedule and the unfused primal
+            // at the chosen shape)
+            const float dual_us = sched_us[0] > best ? sched_us[0] - best : 0.0f;
+            float best_it = order_us[ctx->tuning.primal_order] + (fz ? 0.0f : dual_us);
+            ctx->tuning.primal_packed = 0;
+            if (packed_us + (fz ? 0.0f : dual_us) < best_it) {
+                best_it = packed_us + (fz ? 0.0f : dual_us);
+                ctx->tuning.primal_packed = 1;
+            }
+            // the span primal (worlds without joints) in both schedules; it
+            // takes the schedule it is fastest in if it wins
+            float span_it = 0.0f;
+            if (ctx->w_j.c == 0) {
+                SolveView vs = v;
+                vs.body_list = nullptr;
+                vs.list_count = 0;
+                vs.span_primal = true;
+                span_it = 3.4e38f;
+                uint32_t span_fused = 0;
+                for (uint32_t f = 0; f < 2; ++f) {
+                    launch_avbd_primal(vs, p, 0, false, s, 1);
+                    launch_avbd_primal(vs, p, 1, f == 1, s, 1);
+                    DIO_HIP_CHECK(hipEventRecord(e0, s));
+                    for (uint32_t r = 0; r < reps; ++r) launch_avbd_primal(vs, p, 1, f == 1, s, 1);
+                    DIO_HIP_CHECK(hipEventRecord(e1, s));
+                    DIO_HIP_CHECK(hipEventSynchronize(e1));
+                    float sms = 0.0f;
+                    DIO_HIP_CHECK(hipEventElapsedTime(&sms, e0, e1));
+                    const float it_us = 1000.0f * sms / (float)reps + (f == 1 ? 0.0f : dual_us);
+                    if (it_us < span_it) {
+                        span_it = it_us;
+                        span_fused = f;
+                    }
+                }
+                if (span_it < best_it) {
+                    ctx->tuning.primal_packed = 2;
+                    ctx->tuning.fuse_dual = span_fused;
+                }
+            }
+            if (us_per_launch)
+                us_per_launch[DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4 + DIO_TUNE_COLOUR_SHAPES + 1] = span_it;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

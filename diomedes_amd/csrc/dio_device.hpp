@@ -110,6 +110,10 @@ struct SolveView {
     // with work_tab: the packed primal's item table (kPackCap entries per
     // workgroup), written by iteration 0 (null: the grouped primal)
     uint2* item_tab;
+    // the span primal instead (k_avbd_primal_span: one thread per body, a
+    // workgroup's contiguous CSR range built workgroup-wide); worlds without
+    // joints
+    bool span_primal;
     // when next.init_lin is set, the final launch (dual + recover) also runs
     // the next step's init from the recovered state into these buffers
     InitTarget next;

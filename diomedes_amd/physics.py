@@ -666,7 +666,7 @@ class PhysicsContext:
         schedules on the current world, keep the fastest; returns the choice
         and the per-shape microseconds."""
         t = R.Tuning()
-        us = (ctypes.c_float * 23)()
+        us = (ctypes.c_float * 24)()
         o = options.to_c()
         self._check(self._lib.dio_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(t), us))
         u = list(us)
@@ -683,6 +683,7 @@ class PhysicsContext:
             "colour_us_per_iteration": dict(zip((1, 4, 8, 16, 32, 64), u[16:22])),
             "primal_packed": t.primal_packed,
             "packed_us": u[22],
+            "span_us": u[23],
         }
 
     def stats(self) -> R.StepStats:

@@ -239,7 +239,7 @@ typedef struct dio_step_stats {
 #define DIO_TUNE_PRIMAL_SHAPES 6
 #define DIO_TUNE_BROAD_SHAPES 6
 #define DIO_TUNE_COLOUR_SHAPES 6
-#define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4 + DIO_TUNE_COLOUR_SHAPES + 1)
+#define DIO_TUNE_SLOTS (DIO_TUNE_PRIMAL_SHAPES + DIO_TUNE_BROAD_SHAPES + 4 + DIO_TUNE_COLOUR_SHAPES + 2)
 typedef struct dio_tuning {
     uint32_t primal_lanes;     /* lanes per body in the primal: 1, 4, 8, 16, 32 or 64 (0 = default 32) */
     uint32_t broad_lanes;      /* lanes per query in the BVH traversal: 1, 2, 4, 8, 16 or 32; with 16
@@ -276,9 +276,13 @@ typedef struct dio_tuning {
                                   threads takes 8 bodies, every thread builds the rows of one item of their
                                   joined item list (so a body with many contacts takes no second round
                                   of loads), and 32 lanes per body add its block; primal_lanes is then
-                                  unused by the step.  0: primal_lanes lanes per body.  The result is the
-                                  same either way.  The granular solve, the coloured mode and the
-                                  adjoint's replay take primal_lanes */
+                                  unused by the step.  2: span -- a workgroup takes 256 consecutive
+                                  bodies, one thread each, builds the rows of their contiguous range of
+                                  the contact CSR across all its threads, and each thread adds its own
+                                  body's rows (bodies with few contacts; a world with constraints takes
+                                  1 instead).  0: primal_lanes lanes per body.  The result is the same
+                                  every way.  The granular solve, the coloured mode and the adjoint's
+                                  replay take primal_lanes */
 } dio_tuning;
 
 typedef struct dio_ctx dio_ctx;
@@ -399,8 +403,12 @@ dio_status dio_get_tuning(dio_ctx* ctx, dio_tuning* out);
    the coloured mode's primal, one launch per colour of a colouring of the
    current contact graph, at lanes 1, 4, 8, 16, 32, 64 (microseconds per
    iteration over all colours; colour_lanes is set to the fastest); then the
-   packed primal at the chosen dual schedule and order (primal_packed is set
-   if it beats the chosen primal_lanes).  The
+   packed primal at the chosen dual schedule and order; then (a world without
+   constraints; else 0 is reported) the span primal in both schedules,
+   reported as microseconds per iteration at its faster one, a dual launch
+   included when unfused.  primal_packed is set to 1 or 2 for the packed or
+   the span primal if its cost per iteration beats the chosen primal_lanes,
+   else 0; the span primal brings its schedule into fuse_dual.  The
    primal with a fixed parity is idempotent, the broad timing runs in the
    granular broad buffers and the colouring in the coloured mode's buffers,
    so the world state is unchanged (dio_world_colours then reports that

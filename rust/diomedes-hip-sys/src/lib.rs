@@ -59,7 +59,7 @@ pub const DIO_NOALLOC_MAX_PAIRS: u32 = 65536;
 pub const DIO_NOALLOC_MAX_HULL_VERTICES: u32 = 16384;
 pub const DIO_NOALLOC_SCRATCH_BYTES: u32 = 47453184;
 pub const DIO_TUNE_COLOUR_SHAPES: u32 = 6;
-pub const DIO_TUNE_SLOTS: usize = 23;
+pub const DIO_TUNE_SLOTS: usize = 24;
 pub const DIO_ADJOINT_EXACT: u32 = 0;
 pub const DIO_ADJOINT_FROZEN_MULTIPLIERS: u32 = 1;
 
@@ -247,6 +247,8 @@ pub struct dio_tuning {
     pub primal_order: u32,
     pub scene_solve: u32,
     pub colour_lanes: u32,
+    /// the Jacobi primal of dio_step: 1 (default) packed, 2 span (worlds without
+    /// constraints), 0 primal_lanes lanes per body; the result is the same every way
     pub primal_packed: u32,
 }
 
@@ -364,10 +366,15 @@ extern "C" {
     /// pass); then, at the chosen primal shape, an unfused primal plus a dual
     /// launch and a fused primal (fuse_dual is set to the faster); then that
     /// primal in index order and in Morton order (primal_order is set to the
-    /// faster; the shapes above are timed in the order current at the call). The
-    /// primal with a fixed parity is idempotent and the broad timing runs in the
-    /// granular broad buffers, so the world state is unchanged; a granular broad
-    /// result submitted before the call must be submitted again.
+    /// faster; the shapes above are timed in the order current at the call); then
+    /// the coloured mode's primal per colour shape (colour_lanes); then the packed
+    /// primal and, in a world without constraints, the span primal in both
+    /// schedules, compared per iteration with a dual launch counted when unfused
+    /// (primal_packed is set to 1 or 2 for the winner, else 0; the span primal
+    /// brings its schedule into fuse_dual). The primal with a fixed parity is
+    /// idempotent and the broad timing runs in the granular broad buffers, so the
+    /// world state is unchanged; a granular broad result submitted before the call
+    /// must be submitted again.
     pub fn dio_autotune(ctx: *mut dio_ctx, options: *const dio_avbd_options, reps: u32, chosen: *mut dio_tuning, us_per_launch: *mut f32) -> dio_status;
     // ---- adjoint of the AVBD step (no counterpart in the reference)
     /// The differentiated function is the frozen-contact replay: each recorded

@@ -58,5 +58,9 @@ def rel_err(a, b) -> float:
 
 def primal_tuning(lanes) -> dict:
     """set_tuning arguments for a primal shape of the world step: "packed"
-    (the packed primal) or that many lanes per body."""
-    return {"primal_packed": 1} if lanes == "packed" else {"primal_lanes": lanes, "primal_packed": 0}
+    (the packed primal), "span" (the span primal) or that many lanes per body."""
+    if lanes == "packed":
+        return {"primal_packed": 1}
+    if lanes == "span":
+        return {"primal_packed": 2}
+    return {"primal_lanes": lanes, "primal_packed": 0}

@@ -154,25 +154,29 @@ def test_batched_4096_scenes_match_oracle_per_scene(oracle):
 
 def test_batched_4096_scene_solve_bitexact():
     """The full 4096-scene table over 120 steps with the per-scene solve (one
-    launch, bundles of at least 256 bodies) against the per-launch schedule, both
-    tuned as the autotuner tunes this table (one lane per body, unfused):
-    every body bit-identical every 40 steps."""
+    launch, bundles of at least 256 bodies) and with the span primal against
+    the per-launch schedule with one lane per body, all unfused as the
+    autotuner tunes this table: every body bit-identical every 40 steps."""
     batch = scenes.batched_scenes(0)
     opts = batch.options
     res = {}
-    for solve in (0, 256):
+    runs = {"lanes": dict(primal_lanes=1, primal_packed=0, fuse_dual=0, scene_solve=0),
+            "scene": dict(primal_lanes=1, primal_packed=0, fuse_dual=0, scene_solve=256),
+            "span": dict(primal_packed=2, fuse_dual=0, scene_solve=0)}
+    for name, tuning in runs.items():
         with PhysicsContext(batch.n, max_pairs=8 * batch.n, max_contacts=16 * batch.n,
                             max_hull_vertices=len(batch.hull)) as ctx:
-            ctx.set_tuning(primal_lanes=1, fuse_dual=0, scene_solve=solve)
+            ctx.set_tuning(**tuning)
             ctx.batch_upload(batch)
-            res[solve] = []
+            res[name] = []
             for _ in range(3):
                 ctx.step(opts, 40)
-                res[solve].append(ctx.download())
-    for k in range(3):
-        for key in KEYS:
-            assert_bits(res[256][k][key], res[0][k][key], f"scene solve step {40 * (k + 1)} {key}")
-    _print("batched 4096, scene solve: bit-identical over 120 steps")
+                res[name].append(ctx.download())
+    for name in ("scene", "span"):
+        for k in range(3):
+            for key in KEYS:
+                assert_bits(res[name][k][key], res["lanes"][k][key], f"{name} step {40 * (k + 1)} {key}")
+    _print("batched 4096, scene solve and span primal: bit-identical over 120 steps")
 
 
 def test_wall_throw_64_seeds_120_steps_bitexact(oracle):

@@ -167,7 +167,7 @@ def test_hub_with_hundreds_of_items_bitexact(oracle, spokes, lanes):
         _print("hub", spokes, lanes, "contacts", ctx.stats().contacts)
 
 
-@pytest.mark.parametrize("lanes", [1, 8, 64, "packed"])
+@pytest.mark.parametrize("lanes", [1, 8, 64, "packed", "span"])
 def test_avbd_granular_joints_bitexact(oracle, lanes):
     """Springs, rods, a ball socket and a rope (the constraints demo's set,
     demos/constraints/src/main.rs:110-146) through one solve."""
@@ -292,7 +292,8 @@ def test_box_stack_300_steps_bitexact(oracle):
     "lag,lanes,blanes,resort,fuse",
     [(1, 8, 4, 4, 0), (0, 8, 4, 4, 1), (1, 4, 2, 1, 0), (1, 16, 8, 1000, 1), (1, 1, 1, 4, 1), (0, 4, 8, 1000, 0),
      (1, 32, 1, 4, 1), (1, 32, 8, 4, 0), (1, 64, 8, 4, 1), (1, 32, 16, 16, 1), (0, 32, 32, 4, 1),
-     (1, "packed", 8, 4, 1), (1, "packed", 16, 16, 0), (0, "packed", 4, 1000, 1), (0, "packed", 1, 4, 0)],
+     (1, "packed", 8, 4, 1), (1, "packed", 16, 16, 0), (0, "packed", 4, 1000, 1), (0, "packed", 1, 4, 0),
+     (1, "span", 8, 4, 1), (0, "span", 4, 1000, 0), (1, "span", 16, 16, 0)],
 )
 def test_cube_field_small_bitexact(oracle, lag, lanes, blanes, resort, fuse):
     """resort: Morton re-sort interval of the fused step; 1000 refits the first
@@ -357,7 +358,8 @@ def test_refit_carried_by_the_contact_build_at_depth_six(oracle):
 
 
 @pytest.mark.parametrize("order,lanes,fuse", [(1, 32, 1), (1, 1, 0), (1, 8, 1), (0, 32, 1), (0, 1, 0),
-                                              (1, "packed", 1), (0, "packed", 1), (1, "packed", 0)])
+                                              (1, "packed", 1), (0, "packed", 1), (1, "packed", 0),
+                                              (1, "span", 1), (0, "span", 0)])
 def test_primal_body_order_bitexact(oracle, order, lanes, fuse):
     """primal_order 1 (the default): the Jacobi primal takes the bodies in the
     broad phase's Morton order, one contiguous range per XCD; 0: in index
@@ -876,13 +878,17 @@ def _scene_contacts(gc, lo, hi):
     return c
 
 
-def test_batched_scenes_match_oracle_per_scene(oracle):
+@pytest.mark.parametrize("tuning", [{}, {"primal_packed": 2, "fuse_dual": 1}, {"primal_packed": 2, "fuse_dual": 0}])
+def test_batched_scenes_match_oracle_per_scene(oracle, tuning):
     """Every scene of a batched world steps bit-identically to the FP32 oracle
-    stepping that scene alone (boxes and icosphere hulls, own floors)."""
+    stepping that scene alone (boxes and icosphere hulls, own floors); with the
+    library's default primal and with the span primal."""
     batch = scenes.batched_scenes(3, num_scenes=6, n_min=8, n_max=48)
     opts = batch.options
     worlds = [oracle.OracleWorld(sc, 32) for sc in batch.scenes]
     with PhysicsContext(batch.n, max_hull_vertices=len(batch.hull)) as ctx:
+        if tuning:
+            ctx.set_tuning(**tuning)
         ctx.batch_upload(batch)
         for k in range(3):
             ctx.step(opts, 20)

@@ -182,7 +182,7 @@ def test_constants_and_sizes():
         if k in r:
             assert r[k] == v, (k, r[k], v)
     assert r["DIO_TUNE_SLOTS"] == (r["DIO_TUNE_PRIMAL_SHAPES"] + r["DIO_TUNE_BROAD_SHAPES"] + 4
-                                   + r["DIO_TUNE_COLOUR_SHAPES"] + 1)
+                                   + r["DIO_TUNE_COLOUR_SHAPES"] + 2)
     # the crate asserts each record's size at compile time, with the header's sizes
     from .test_abi import _c_sizeof
 
