@@ -586,17 +586,21 @@ __constant__ int8_t kQ[27] = {0, 0, 0, 1, 1, 2, 3, 3, 3, 4, 4, 5, 0, 1, 2, 0, 1,
 constexpr int kRowFloats = 9;
 constexpr int kItemFloats = 28;  // 3 rows of 9, padded to 16 bytes
 
-// stage row r of an item: (j_lin, j_ang, force, stiffness, 1)
+// stage row r of an item: (j_lin, j_ang, force, stiffness, 1); RF = 8 leaves
+// out the trailing 1 (a reader that applies the row as a whole needs none)
+template <int RF = kRowFloats>
 __device__ __forceinline__ void stage_row(float* st, int r, f3 jl, f3 ja, float f, float k) {
-    float* R = st + kRowFloats * r;
+    float* R = st + RF * r;
     R[0] = jl.x; R[1] = jl.y; R[2] = jl.z;
     R[3] = ja.x; R[4] = ja.y; R[5] = ja.z;
-    R[6] = f; R[7] = k; R[8] = 1.0f;
+    R[6] = f; R[7] = k;
+    if (RF > 8) R[8] = 1.0f;
 }
 // a row the item does not have: every term it contributes is -0
+template <int RF = kRowFloats>
 __device__ __forceinline__ void stage_empty_row(float* st, int r) {
     const f3 z{0.0f, 0.0f, 0.0f};
-    stage_row(st, r, z, z, -0.0f, -0.0f);
+    stage_row<RF>(st, r, z, z, -0.0f, -0.0f);
 }
 
 // The body an item's rows are built for: its pose at the iteration's start
@@ -652,6 +656,7 @@ __device__ __forceinline__ ContactIn load_contact(const SolveView& v, const floa
 // The three rows of a contact item for body b (avbd/src/lib.rs:290-418), with
 // the fused schedule's dual update k - 1 first (stored by body a, or by b when
 // a is static).
+template <int RF = kRowFloats>
 __device__ __forceinline__ void stage_contact(const SolveView& v, const AvbdParams& p, float kk, bool fuse,
                                               float4* lam_out, float4* pen_out, const RowBody& b,
                                               const ContactIn& c, float* st) {
@@ -691,9 +696,9 @@ __device__ __forceinline__ void stage_contact(const SolveView& v, const AvbdPara
     c1 = c1 + (dot3(r1, b.dsl) + dot3(j1, b.dsa) - dot3(r1, dol) - dot3(cross3(r_other, r1), doa));
     c2 = c2 + (dot3(r2, b.dsl) + dot3(j2, b.dsa) - dot3(r2, dol) - dot3(cross3(r_other, r2), doa));
     const Force3 F = contact_force(pen, lam, c.n4.w, c0, c1, c2);
-    stage_row(st, 0, r0, j0, F.f0, pen.x);
-    stage_row(st, 1, r1, j1, F.f1, pen.y);
-    stage_row(st, 2, r2, j2, F.f2, pen.z);
+    stage_row<RF>(st, 0, r0, j0, F.f0, pen.x);
+    stage_row<RF>(st, 1, r1, j1, F.f1, pen.y);
+    stage_row<RF>(st, 2, r2, j2, F.f2, pen.z);
 }
 
 // The rows of joint cidx for body b (avbd/src/lib.rs:422-539; hard and
@@ -1297,8 +1302,9 @@ __global__ void __launch_bounds__(kSpanBodies) k_avbd_primal_span(SolveView v, A
     const float4* __restrict__ pen_in = pen_buf(v, fuse ? iter - 1 : 0);
     float4* lam_out = lam_buf(v, iter);
     float4* pen_out = pen_buf(v, iter);
-    __shared__ __align__(16) float stage[SB][kItemFloats];
-    __shared__ float4 body_f[SB][4];  // pos (.w: active), rot, dsl, dsa of each thread's body
+    constexpr int kSpanRow = 8;                  // rows staged without their trailing 1
+    __shared__ __align__(16) float stage[SB][3 * kSpanRow];
+    __shared__ float4 body_f[SB][3];   // rot; dsl, dsa.x; dsa.y, dsa.z, active of each thread's body
     __shared__ uint32_t offs[SB + 1];  // the bodies' CSR starts, and the range's end
     const float4* __restrict__ rp = v.pos[parity];
     const float4* __restrict__ rr = v.rot[parity];
@@ -1344,10 +1350,9 @@ __global__ void __launch_bounds__(kSpanBodies) k_avbd_primal_span(SolveView v, A
     }
     const f3 dsl = active ? sub(snap_pos, xyz(il4)) : f3{0.0f, 0.0f, 0.0f};
     const f3 dsa = active ? ang_diff(snap_rot, as_q(ia4)) : f3{0.0f, 0.0f, 0.0f};
-    body_f[t][0] = f4(snap_pos, active ? 1.0f : 0.0f);
-    body_f[t][1] = sr4;
-    body_f[t][2] = f4(dsl, 0.0f);
-    body_f[t][3] = f4(dsa, 0.0f);
+    body_f[t][0] = sr4;
+    body_f[t][1] = f4(dsl, dsa.x);
+    body_f[t][2] = make_float4(dsa.y, dsa.z, active ? 1.0f : 0.0f, 0.0f);
     __syncthreads();
     const uint32_t o0 = offs[0], o1 = offs[SB];
     const uint32_t my0 = offs[t], my1 = offs[t + 1];
@@ -1378,23 +1383,24 @@ __global__ void __launch_bounds__(kSpanBodies) k_avbd_primal_span(SolveView v, A
                 if (offs[mid] <= f) s = mid;
                 else hi = mid;
             }
-            const float4 b0 = body_f[s][0];
-            if (b0.w != 0.0f) {
-                RowBody rb;
-                rb.pos = xyz(b0);
-                rb.rot = as_q(body_f[s][1]);
-                rb.dsl = xyz(body_f[s][2]);
-                rb.dsa = xyz(body_f[s][3]);
+            const float4 b2 = body_f[s][2];
+            if (b2.z != 0.0f) {
+                const float4 b1 = body_f[s][1];
+                RowBody rb;  // (a contact's rows use no position)
+                rb.pos = f3{0.0f, 0.0f, 0.0f};
+                rb.rot = as_q(body_f[s][0]);
+                rb.dsl = xyz(b1);
+                rb.dsa = f3{b1.w, b2.x, b2.y};
                 rb.i = i0 + s;
                 const ContactIn cin = load_contact(v, lam_in, pen_in, rp, rr, fuse, ci, pc);
-                stage_contact(v, p, kk, fuse, lam_out, pen_out, rb, cin, st);
+                stage_contact<kSpanRow>(v, p, kk, fuse, lam_out, pen_out, rb, cin, st);
                 built = true;
             }
         }
         if (!built) {
-            stage_empty_row(st, 0);
-            stage_empty_row(st, 1);
-            stage_empty_row(st, 2);
+            stage_empty_row<kSpanRow>(st, 0);
+            stage_empty_row<kSpanRow>(st, 1);
+            stage_empty_row<kSpanRow>(st, 2);
         }
         __syncthreads();
         // ---- this body's entries of the round, in CSR order
@@ -1406,7 +1412,7 @@ __global__ void __launch_bounds__(kSpanBodies) k_avbd_primal_span(SolveView v, A
                 const float* it = stage[q - r0];
 #pragma unroll
                 for (int rw = 0; rw < 3; ++rw) {
-                    const float* R = it + kRowFloats * rw;
+                    const float* R = it + kSpanRow * rw;
                     B.row(f3{R[0], R[1], R[2]}, f3{R[3], R[4], R[5]}, R[6], R[7]);
                 }
             }
