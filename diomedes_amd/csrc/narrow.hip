@@ -34,15 +34,22 @@ constexpr int kMaxF = 512;
 constexpr int kSmallV = 24;
 constexpr int kSmallF = 32;
 constexpr int kEpaLanes = 16;  // lanes per pair in the first pass (four pairs per wave)
+// With at most kEpaFewPairs GJK-intersecting pairs the first pass runs them a
+// wavefront each at 64 vertices and 128 faces instead: then the pairs are in
+// flight together either way, and a pair's own latency (shorter with 64 lanes)
+// and the retry launch (which the larger polytope leaves with next to nothing)
+// set the time.
+constexpr int kFewV = 64;
+constexpr int kFewF = 128;
+constexpr uint32_t kEpaFewPairs = 32768;
 // A hull vertex pool of at most this many vertices (16 KB) is staged in LDS
 // by every workgroup of the GJK and EPA launches: the support scans then read
 // LDS instead of the L2 (the same values, so the same results).
 constexpr uint32_t kLdsPoolMax = 1024;
-// Workgroups of the EPA first pass (one wavefront each, taking pairs 4 i ..
-// 4 i + 3, then 4 (i + grid) ..): a pair's cost varies with its iteration
-// count, and a grid far wider than the resident wavefronts lets the
-// dispatcher even that out.  The retry pass takes up to kRetryBlocks pairs at
-// once (one wavefront and 26 KB of LDS each).
+// Workgroups of the EPA first pass (one wavefront each, taking its pairs a
+// grid apart): a grid far wider than the resident wavefronts evens out the
+// pairs' uneven iteration counts.  The retry pass takes up to kRetryBlocks
+// pairs at once (one wavefront and 26 KB of LDS each).
 constexpr uint32_t kEpaBlocks = 65536;
 constexpr uint32_t kRetryBlocks = 2048;
 extern __shared__ float4 pool_lds[];
@@ -471,20 +478,17 @@ __device__ void face_contact(const L_t& L, uint32_t fi, f3& normal, float& depth
 // caps of epa.rs): the pairs of the retry list.  The groups of a wave run the
 // loop together; a group whose pair is done (or that has none) stays in it
 // with every update masked off until the wave's last pair is done.
-template <int GL, int MV, int MF, bool FIRST, bool LDS_POOL>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FIRST ? 4 : 1)))
-k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
+template <int GL, int MV, int MF, bool FIRST, class Next>
+__device__ __forceinline__ void epa_jobs(const BodyView& bv, const NarrowOut& o, const float4* simplex_store,
+                                         const HullPool& hp, EpaLds<MV, MF>* polys, uint32_t jobs, uint32_t j0,
+                                         Next next) {
     constexpr int kGroups = kWave / GL;
     constexpr int kFacesPerLane = MF / GL;
     static_assert(MF % GL == 0, "faces split evenly over the lanes");
-    __shared__ EpaLds<MV, MF> polys[kGroups];
     const LaneGroup<GL> g = lane_group<GL>();
     const int lane = g.lane;
     EpaLds<MV, MF>& L = polys[threadIdx.x / GL];
-    const uint32_t jobs = FIRST ? *o.gjk_count : *o.epa_retry_count;
-    if (blockIdx.x * kGroups >= jobs) return;  // uniform: a workgroup with no pair stages nothing
-    const HullPool hp = stage_pool<LDS_POOL>(bv);
-    for (uint32_t j0 = blockIdx.x * kGroups; j0 < jobs; j0 += gridDim.x * kGroups) {
+    for (; j0 < jobs; j0 = next()) {
         const uint32_t job = j0 + threadIdx.x / GL;
         const bool has = job < jobs;  // the same on every lane of a group
         // a group with no pair computes on the round's first pair, and stores nothing
@@ -656,6 +660,47 @@ k_epa(BodyView bv, NarrowOut o, const float4* simplex_store) {
     }
 }
 
+// The first pass over the GJK list.  With at most kEpaFewPairs pairs, a
+// wavefront per pair and a polytope of kFewV vertices and kFewF faces;
+// otherwise four pairs per wavefront, 16 lanes and a polytope of kSmallV
+// vertices and kSmallF faces each.  Workgroup b takes job b (or jobs 4 b ..
+// 4 b + 3), then the ones a grid further on, and so on.  (A queue claimed by
+// atomics instead was slower: the claims on one word from every XCD
+// serialise.)
+template <bool LDS_POOL>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
+k_epa_first(BodyView bv, NarrowOut o, const float4* simplex_store) {
+    constexpr size_t kFewBytes = sizeof(EpaLds<kFewV, kFewF>);
+    constexpr size_t kManyBytes = (kWave / kEpaLanes) * sizeof(EpaLds<kSmallV, kSmallF>);
+    __shared__ __align__(16) unsigned char polys[kFewBytes > kManyBytes ? kFewBytes : kManyBytes];
+    const uint32_t jobs = *o.gjk_count;
+    const bool few = jobs <= kEpaFewPairs;
+    const uint32_t per = few ? 1u : (uint32_t)(kWave / kEpaLanes);  // jobs per claim
+    if (blockIdx.x * per >= jobs) return;  // uniform: a workgroup with no pair stages nothing
+    const HullPool hp = stage_pool<LDS_POOL>(bv);
+    uint32_t j = blockIdx.x * per;
+    auto next = [&]() { return j += gridDim.x * per; };
+    if (few)
+        epa_jobs<kWave, kFewV, kFewF, true>(bv, o, simplex_store, hp, reinterpret_cast<EpaLds<kFewV, kFewF>*>(polys),
+                                            jobs, j, next);
+    else
+        epa_jobs<kEpaLanes, kSmallV, kSmallF, true>(bv, o, simplex_store, hp,
+                                                    reinterpret_cast<EpaLds<kSmallV, kSmallF>*>(polys), jobs, j, next);
+}
+
+// The retry pass: a wavefront per pair at the caps of epa.rs, workgroup b
+// taking pairs b, b + grid, ...
+template <bool LDS_POOL>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1)))
+k_epa_retry(BodyView bv, NarrowOut o, const float4* simplex_store) {
+    __shared__ EpaLds<kMaxV, kMaxF> poly;
+    const uint32_t jobs = *o.epa_retry_count;
+    if (blockIdx.x >= jobs) return;
+    const HullPool hp = stage_pool<LDS_POOL>(bv);
+    uint32_t j = blockIdx.x;
+    epa_jobs<kWave, kMaxV, kMaxF, false>(bv, o, simplex_store, hp, &poly, jobs, j, [&]() { return j += gridDim.x; });
+}
+
 __global__ void __launch_bounds__(kBlock) k_pack_contact_out(NarrowOut o, uint32_t np, dio_gpu_contact_out* rec) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= np * 4) return;
@@ -692,20 +737,15 @@ void run_narrow(const BodyView& bv, const NarrowOut& o, float4* simplex_store, h
     else hipLaunchKernelGGL(k_narrow<false>, dim3(blocks), dim3(kBlock), 0, s, bv, o, simplex_store);
     if (gjk_possible) {
         const uint32_t np = o.p_max > 0 ? o.p_max : 1;
-        constexpr uint32_t kPairsPerWave = kWave / kEpaLanes;
-        uint32_t eblocks = (np + kPairsPerWave - 1) / kPairsPerWave;
-        if (eblocks > kEpaBlocks) eblocks = kEpaBlocks;
+        const uint32_t eblocks = np < kEpaBlocks ? np : kEpaBlocks;
         const uint32_t rblocks = np < kRetryBlocks ? np : kRetryBlocks;
         const float4* ss = simplex_store;
         if (lds) {
-            hipLaunchKernelGGL((k_epa<kEpaLanes, kSmallV, kSmallF, true, true>), dim3(eblocks), dim3(kWave), pool_bytes, s,
-                               bv, o, ss);
-            hipLaunchKernelGGL((k_epa<kWave, kMaxV, kMaxF, false, true>), dim3(rblocks), dim3(kWave), pool_bytes, s, bv,
-                               o, ss);
+            hipLaunchKernelGGL(k_epa_first<true>, dim3(eblocks), dim3(kWave), pool_bytes, s, bv, o, ss);
+            hipLaunchKernelGGL(k_epa_retry<true>, dim3(rblocks), dim3(kWave), pool_bytes, s, bv, o, ss);
         } else {
-            hipLaunchKernelGGL((k_epa<kEpaLanes, kSmallV, kSmallF, true, false>), dim3(eblocks), dim3(kWave), 0, s, bv, o,
-                               ss);
-            hipLaunchKernelGGL((k_epa<kWave, kMaxV, kMaxF, false, false>), dim3(rblocks), dim3(kWave), 0, s, bv, o, ss);
+            hipLaunchKernelGGL(k_epa_first<false>, dim3(eblocks), dim3(kWave), 0, s, bv, o, ss);
+            hipLaunchKernelGGL(k_epa_retry<false>, dim3(rblocks), dim3(kWave), 0, s, bv, o, ss);
         }
     }
     launch_scan_exclusive(o.cnt, o.off, o.p_max, o.p_dev, o.scan_partials, s);
