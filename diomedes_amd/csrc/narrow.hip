@@ -558,6 +558,7 @@ __device__ __forceinline__ void epa_jobs(const BodyView& bv, const NarrowOut& o,
                     live = false;
                 }
             }
+            if (__ballot(live) == 0ull) break;  // the wave's last pair is done (every lane is active here)
             if (live && lane == 0) {
                 L.vx[ni] = sp.x; L.vy[ni] = sp.y; L.vz[ni] = sp.z;
                 L.wx[ni] = sw.x; L.wy[ni] = sw.y; L.wz[ni] = sw.z;
@@ -608,6 +609,7 @@ __device__ __forceinline__ void epa_jobs(const BodyView& bv, const NarrowOut& o,
                 retry = true;
                 live = false;
             }
+            if (__ballot(live) == 0ull) break;
             const uint32_t hcount = hbase < (uint32_t)MF ? hbase : (uint32_t)MF;
             // compact the kept (non-visible) faces in index order
             FaceR keep[kFacesPerLane];
@@ -670,9 +672,11 @@ __device__ __forceinline__ void epa_jobs(const BodyView& bv, const NarrowOut& o,
 template <bool LDS_POOL>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
 k_epa_first(BodyView bv, NarrowOut o, const float4* simplex_store) {
-    constexpr size_t kFewBytes = sizeof(EpaLds<kFewV, kFewF>);
-    constexpr size_t kManyBytes = (kWave / kEpaLanes) * sizeof(EpaLds<kSmallV, kSmallF>);
-    __shared__ __align__(16) unsigned char polys[kFewBytes > kManyBytes ? kFewBytes : kManyBytes];
+    union Polys {
+        EpaLds<kFewV, kFewF> few;
+        EpaLds<kSmallV, kSmallF> many[kWave / kEpaLanes];
+    };
+    __shared__ Polys polys;
     const uint32_t jobs = *o.gjk_count;
     const bool few = jobs <= kEpaFewPairs;
     const uint32_t per = few ? 1u : (uint32_t)(kWave / kEpaLanes);  // jobs per claim
@@ -681,11 +685,9 @@ k_epa_first(BodyView bv, NarrowOut o, const float4* simplex_store) {
     uint32_t j = blockIdx.x * per;
     auto next = [&]() { return j += gridDim.x * per; };
     if (few)
-        epa_jobs<kWave, kFewV, kFewF, true>(bv, o, simplex_store, hp, reinterpret_cast<EpaLds<kFewV, kFewF>*>(polys),
-                                            jobs, j, next);
+        epa_jobs<kWave, kFewV, kFewF, true>(bv, o, simplex_store, hp, &polys.few, jobs, j, next);
     else
-        epa_jobs<kEpaLanes, kSmallV, kSmallF, true>(bv, o, simplex_store, hp,
-                                                    reinterpret_cast<EpaLds<kSmallV, kSmallF>*>(polys), jobs, j, next);
+        epa_jobs<kEpaLanes, kSmallV, kSmallF, true>(bv, o, simplex_store, hp, polys.many, jobs, j, next);
 }
 
 // The retry pass: a wavefront per pair at the caps of epa.rs, workgroup b
