@@ -1280,10 +1280,12 @@ __global__ void __launch_bounds__(kPackThreads) k_avbd_primal_packed(SolveView v
 
 // Span primal (the Jacobi solve of a step for bodies with few contacts; no
 // counterpart in the reference's thread-per-body kernel).  A workgroup takes
-// kBlock consecutive bodies in index order, one thread each, and with them one
-// contiguous range of the CSR, [offsets[i0], offsets[i0 + kBlock]).  In round
-// r thread t builds the rows of range entry r * kBlock + t and stages them in
-// LDS: consecutive threads read consecutive CSR entries, and a body's
+// kSpanBodies consecutive bodies in index order, one thread each, and with
+// them one contiguous range of the CSR, [offsets[i0], offsets[i0 +
+// kSpanBodies]).  In round r thread t builds the rows of range entry
+// r * kSpanBodies + t (the next round's CSR entry fetched ahead) and stages
+// them in LDS, without each row's trailing 1: consecutive threads read
+// consecutive CSR entries, and a body's
 // contacts with the container or as body a are consecutive records, so the
 // gathers that one thread per body would make entry by entry become
 // neighbouring loads.  Each thread then adds the staged rows of its own
