@@ -1234,7 +1234,7 @@ __global__ void __launch_bounds__(kPackThreads) k_avbd_primal_packed(SolveView v
             const uint32_t lo = pg > r0 ? pg : r0;
             const uint32_t end = pg + ng, r1 = r0 + kPackThreads;
             const uint32_t hi = end < r1 ? end : r1;
-#pragma unroll 4
+#pragma unroll 2
             for (uint32_t q = lo; q < hi; ++q) {
                 const float* it = stage[q - r0];
 #pragma unroll
